@@ -1,0 +1,127 @@
+/*
+ * tflows_gpu: MI355X (gfx950) Krylov solvers for T-Flows' CSR Matrix_Type.
+ *
+ * C ABI of the drop-in replacement for the native solver call
+ *   Nat % Solve_Native(solver, prec, A, x, b, miter, niter, tol, fin_res, norm)
+ * (Sources/Process/Cpu/Native_Mod/Solve_Native.f90:2-46), reached from
+ * Solver_Mod % Run (Sources/Process/Cpu/Solver_Mod/Run.f90:23-33).
+ *
+ * Plain pointers and sizes only.  Index arrays are the reference's 1-based
+ * Matrix_Type arrays: row(nt+1), col(nnz), dia(nt)
+ * (Sources/Process/Cpu/Matrix_Mod.f90:57-76); the diagonal must be the first
+ * entry of every row, as Create_Matrix.f90:112-119 guarantees.
+ *
+ * Every call returns TFG_OK (0) or an error code; tfg_last_error() gives the
+ * message of the last failure on the calling thread.
+ */
+#ifndef TFLOWS_GPU_H
+#define TFLOWS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TFG_OK 0
+#define TFG_EINVAL 1      /* invalid argument: unknown solver, bad sizes, ... */
+#define TFG_EHIP 2        /* HIP runtime failure                              */
+#define TFG_ENOMEM 3      /* device or host allocation failed                 */
+#define TFG_EBREAKDOWN 4  /* non-finite residual or zero curvature (p.Ap = 0) */
+#define TFG_ENOTSUP 5     /* combination not supported (e.g. MPI buffer cells)*/
+
+/* ---- options of the extended solve ------------------------------------- */
+#define TFG_FP64 0        /* all arithmetic in FP64 (the reference's)          */
+#define TFG_FP32_IR 1     /* FP32 inner CG inside FP64 iterative refinement   */
+
+#define TFG_CRIT_NRMS 0   /* Normalized_Root_Mean_Square < tol (reference)     */
+#define TFG_CRIT_RELRES 1 /* ||b - A x||_2 / ||b||_2 < tol                     */
+
+#define TFG_PATH_AUTO 0        /* pick by size                                 */
+#define TFG_PATH_KERNELS 1     /* one launch per phase, device-side stop flag  */
+#define TFG_PATH_GRAPH 2       /* the same phases captured in a hipGraph       */
+#define TFG_PATH_PERSISTENT 3  /* one cooperative launch for the whole solve   */
+
+typedef struct {
+  int precision;      /* TFG_FP64 | TFG_FP32_IR                                */
+  int criterion;      /* TFG_CRIT_NRMS | TFG_CRIT_RELRES                       */
+  int path;           /* TFG_PATH_*                                            */
+  int inner_max;      /* FP32_IR: inner iterations per refinement step (0: 500)*/
+  double inner_tol;   /* FP32_IR: inner residual reduction per step (0: 1e-4)  */
+  int timing;         /* 1: time sampled SpMV launches with hipEvents          */
+  int true_residual;  /* 1: recompute ||b - A x||_2/||b||_2 in FP64 at exit    */
+} tfg_options;
+
+typedef struct {
+  int status;           /* same as the return code                            */
+  int iterations;       /* Krylov iterations (FP32_IR: inner iterations total)*/
+  int refinements;      /* FP32_IR: outer refinement steps                    */
+  double true_relres;   /* when requested, else -1                            */
+  double solve_ms;      /* device time of the solve (hipEvents)               */
+  double spmv_ms;       /* mean duration of the sampled SpMV launches         */
+  int spmv_samples;     /* number of SpMV launches timed                      */
+} tfg_report;
+
+typedef struct tfg_matrix tfg_matrix;
+
+/* Upload the CSR structure once per grid.  Replaces the device side of
+ * Native_Type % Create_Native / Matrix_Type % Create_Matrix
+ * (Native_Mod/Create_Native.f90:14-21, Matrix_Mod/Create_Matrix.f90:40-195).
+ * ni: rows solved (n_cells - n_buff_cells); nt: rows stored (n_cells);
+ * nnz = row[nt] - 1.  Only ni == nt (one domain) is supported. */
+int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col,
+                      const int* dia, tfg_matrix** handle);
+int tfg_matrix_destroy(tfg_matrix* handle);
+
+/* Drop-in for Solve_Native (Native_Mod/Solve_Native.f90:2-46), host arrays.
+ * solver: "cg" | "bicg" (Fortran blank padding allowed; len < 0: strlen);
+ * prec: "jacobi" | "icc" | anything else = none (Prec_Form.f90:47-89).
+ * val(nnz), x(nt) = x0 on entry / solution on exit, b(nt).  val and b are
+ * left unchanged (the reference scales and de-scales them in place,
+ * Cg.f90:77-84, 245-250).  norm: NULL when the optional argument is absent.
+ * niter, fin_res: as the reference returns them (Cg.f90:226-253). */
+int tfg_solve_native(tfg_matrix* h, const char* solver, int solver_len,
+                     const char* prec, int prec_len, const double* val,
+                     double* x, double* b, int miter, int* niter, double tol,
+                     double* fin_res, const double* norm);
+
+/* Same call with device-resident arrays (device pointers) and options.
+ * device_ptrs = 1: val, x, b are device pointers on the handle's device;
+ * device_ptrs = 0: host pointers as in tfg_solve_native.  opt/rep may be
+ * NULL (reference semantics, no report). */
+int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len,
+                        const char* prec, int prec_len, const double* val,
+                        double* x, double* b, int miter, int* niter, double tol,
+                        double* fin_res, const double* norm, int device_ptrs,
+                        const tfg_options* opt, tfg_report* rep);
+
+/* Building blocks on device pointers (the product loop Cg.f90:177-185 and
+ * Residual_Vector.f90:34-42, without scaling).  Stream-ordered on the
+ * handle's stream; call tfg_synchronize() before reading results. */
+int tfg_spmv(tfg_matrix* h, const double* d_val, const double* d_p, double* d_q);
+int tfg_residual(tfg_matrix* h, const double* d_val, const double* d_b,
+                 const double* d_x, double* d_r);
+
+/* Time `reps` back-to-back SpMV launches with hipEvents on the handle's
+ * stream; *ms_avg = mean duration of one launch. */
+int tfg_spmv_bench(tfg_matrix* h, const double* d_val, const double* d_p,
+                   double* d_q, int reps, double* ms_avg);
+
+/* Device memory helpers (bytes; copies are synchronous on the handle-free
+ * default path). */
+int tfg_device_alloc(void** dptr, long long bytes);
+int tfg_device_free(void* dptr);
+int tfg_memcpy_h2d(void* dst, const void* src, long long bytes);
+int tfg_memcpy_d2h(void* dst, const void* src, long long bytes);
+int tfg_synchronize(tfg_matrix* h);
+
+/* Structure facts chosen at upload (for tests and the autotuner). */
+int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_blocks,
+                    int* spmv_cap, int* col_mode);
+
+const char* tfg_last_error(void);
+int tfg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,390 @@
+/*
+ * CPU restatement of T-Flows' Native_Mod (Sources/Process/Cpu/Native_Mod/).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the GPU library and the timed
+ * CPU baseline.  Nothing in the product links or calls it.
+ *
+ * Each routine follows the reference's operation order so that, compiled
+ * without floating-point contraction (-ffp-contract=off) and without OpenMP,
+ * it performs the same IEEE FP64 operations in the same order as the
+ * reference built with its default options (OMP=no, REAL=double,
+ * Sources/Process/Cpu/makefile:10-17).  The `omp` pragmas sit where the
+ * reference has `!$omp` directives; they only take effect in the separately
+ * built OpenMP variant used to time the CPU baseline on all cores.
+ *
+ * Conventions: C index i is Fortran index i+1.  Index arrays carry 1-based
+ * values, so col[j]-1 / row[i]-1 / dia[i]-1 are C positions.
+ */
+#include "native_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Sources/Shared/Const_Mod.f90:84-93 */
+#define OR_HUGE 1.0e+15
+#define OR_TINY 1.0e-15
+
+/* Summation order of the global reductions.  0 (default) is the reference's
+ * serial left-to-right order.  1 sums blocks of 64 terms left to right and
+ * combines the block sums pairwise: an equally valid parallel order, used
+ * only to measure how far rounding alone moves a solve (the envelope the
+ * GPU's own order is judged against).  Not part of the reference. */
+static int g_sum_order = 0;
+
+void or_set_sum_order(int order) { g_sum_order = order; }
+
+static double pairwise(const double* s, int n) {
+  if (n == 1) return s[0];
+  int h = n / 2;
+  return pairwise(s, h) + pairwise(s + h, n - h);
+}
+
+/* sum_i f(i) in the blocked order; kind: 0 a*b, 1 (a*a)/(b*b), 2 a */
+static double blocked(const double* a, const double* b, int n, int kind) {
+  if (n <= 0) return 0.0;
+  int nb = (n + 63) / 64;
+  double* part = (double*)malloc(sizeof(double) * (size_t)nb);
+  for (int k = 0; k < nb; k++) {
+    double s = 0.0;
+    for (int i = 64 * k; i < n && i < 64 * k + 64; i++) {
+      double t = kind == 0 ? a[i] * b[i] : (kind == 1 ? (a[i] * a[i]) / (b[i] * b[i]) : a[i]);
+      s = s + t;
+    }
+    part[k] = s;
+  }
+  double s = pairwise(part, nb);
+  free(part);
+  return s;
+}
+
+/* Normalized_Root_Mean_Square.f90:2-65 */
+double or_nrms(const or_matrix* A, int ni, const double* r, const double* x,
+               const double* norm) {
+  double rms = 0.0;
+  if (g_sum_order) {
+    double* dg = (double*)malloc(sizeof(double) * (size_t)(ni > 0 ? ni : 1));
+    for (int i = 0; i < ni; i++) dg[i] = A->val[A->dia[i] - 1];
+    rms = blocked(r, dg, ni, 1);
+    free(dg);
+  } else {
+    for (int i = 0; i < ni; i++) {             /* :34-36, serial in the reference */
+      double a = A->val[A->dia[i] - 1];
+      rms = rms + (r[i] * r[i]) / (a * a);
+    }
+  }
+  rms = sqrt(rms);                              /* :39 */
+  double x_min, x_max;
+  if (!norm) {                                  /* :42-48 */
+    x_min = +OR_HUGE;
+    x_max = -OR_HUGE;
+    for (int i = 0; i < ni; i++) {
+      x_min = fmin(x_min, x[i]);
+      x_max = fmax(x_max, x[i]);
+    }
+  } else {                                      /* :49-52 */
+    x_min = 0.0;
+    x_max = *norm;
+  }
+  double x_max_min = x_max - x_min;             /* :59-61 */
+  x_max_min = fmax(x_max_min, OR_TINY);
+  return rms / x_max_min;
+}
+
+/* Residual_Vector.f90:34-42: r_i = b_i - sum_j a_ij x_j, subtracting in CSR
+ * order.  (The reference's OpenMP clause leaves k shared, a race; here k is
+ * private, which is what the serial reference computes.) */
+void or_residual(const or_matrix* A, int ni, double* r, const double* b,
+                 const double* x) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ni; i++) {
+    double ri = b[i];
+    for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) {
+      int k = A->col[j] - 1;
+      ri = ri - A->val[j] * x[k];
+    }
+    r[i] = ri;
+  }
+}
+
+/* Cg.f90:177-185: q_i = 0 + sum_j a_ij p_j in CSR order. */
+void or_spmv(const or_matrix* A, int ni, const double* p, double* q) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ni; i++) {
+    double qi = 0.0;
+    for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) {
+      int k = A->col[j] - 1;
+      qi = qi + A->val[j] * p[k];
+    }
+    q[i] = qi;
+  }
+}
+
+/* Prec_Form.f90:47-89 */
+void or_prec_form(const or_matrix* A, int ni, double* d, double* d_inv, int prec) {
+  if (prec == OR_PREC_JACOBI) {                 /* :47-53 */
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ni; i++) {
+      d[i] = A->val[A->dia[i] - 1];
+      d_inv[i] = 1.0 / d[i];
+    }
+  } else if (prec == OR_PREC_ICC) {             /* :68-78, serial recurrence */
+    for (int i = 0; i < ni; i++) {
+      double sum = A->val[A->dia[i] - 1];
+      for (int j = A->row[i]; j < A->row[i + 1] - 1; j++) { /* a_row(i)+1 .. */
+        int k = A->col[j] - 1;
+        if (k > i) break;
+        sum = sum - d[k] * A->val[j] * A->val[j];
+      }
+      d_inv[i] = sum;
+      d[i] = 1.0 / sum;
+    }
+  } else {                                      /* :83-89 */
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ni; i++) d[i] = 1.0;
+  }
+}
+
+/* Prec_Solve.f90:44-103 */
+void or_prec_solve(const or_matrix* A, int ni, const double* d,
+                   const double* d_inv, double* x, const double* b, int prec) {
+  if (prec == OR_PREC_JACOBI) {                 /* :44-49 */
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ni; i++) x[i] = b[i] * d_inv[i];
+  } else if (prec == OR_PREC_ICC) {             /* :64-92 */
+    for (int i = 0; i < ni; i++) {              /* forward substitution */
+      double sum = b[i];
+      for (int j = A->row[i]; j < A->row[i + 1] - 1; j++) {
+        int k = A->col[j] - 1;
+        if (k > i) break;
+        sum = sum - A->val[j] * x[k];
+      }
+      x[i] = sum * d[i];
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ni; i++) x[i] = x[i] * d_inv[i];
+    for (int i = ni - 1; i >= 0; i--) {         /* backward substitution */
+      double sum = x[i];
+      for (int j = A->row[i + 1] - 2; j >= A->row[i]; j--) { /* down to diag+1 */
+        int k = A->col[j] - 1;
+        if (k < i) break;
+        if (k < ni) sum = sum - A->val[j] * x[k];
+      }
+      x[i] = sum * d[i];
+    }
+  } else {                                      /* :94-101 */
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ni; i++) x[i] = b[i];
+  }
+}
+
+/* Cg.f90:62-84 (Bicg.f90:63-85): fn = 1/(sum_i a_ii / ni); val *= fn and
+ * b *= fn over rows 1..nt. */
+static double normalise(or_matrix* A, double* b) {
+  double sum_a = 0.0;
+  int sum_n = 0;
+  if (g_sum_order) {
+    double* dg = (double*)malloc(sizeof(double) * (size_t)A->ni);
+    for (int i = 0; i < A->ni; i++) dg[i] = A->val[A->dia[i] - 1];
+    sum_a = blocked(dg, NULL, A->ni, 2);
+    sum_n = A->ni;
+    free(dg);
+  } else {
+#pragma omp parallel for schedule(static) reduction(+ : sum_a) reduction(+ : sum_n)
+    for (int i = 0; i < A->ni; i++) {
+      sum_a = sum_a + A->val[A->dia[i] - 1];
+      sum_n = sum_n + 1;
+    }
+  }
+  sum_a = sum_a / (double)sum_n;
+  double fn = 1.0 / sum_a;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < A->nt; i++) {
+    for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) A->val[j] = A->val[j] * fn;
+    b[i] = b[i] * fn;
+  }
+  return fn;
+}
+
+/* Cg.f90:245-250: de-normalise (serial in the reference's Cg). */
+static void denormalise(or_matrix* A, double* b, double fn) {
+  for (int i = 0; i < A->nt; i++) {
+    for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) A->val[j] = A->val[j] / fn;
+    b[i] = b[i] / fn;
+  }
+}
+
+static double sumsq(const double* v, int n) {
+  if (g_sum_order) return blocked(v, v, n, 0);
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s = s + v[i] * v[i];
+  return s;
+}
+
+/* Stopping measure: the reference's NRMS, or ||r||_2/||b||_2 (bnorm given). */
+static double measure(const or_matrix* A, int ni, const double* r,
+                      const double* x, const double* norm, int crit,
+                      double bnorm) {
+  if (crit == OR_CRIT_RELRES) return sqrt(sumsq(r, ni)) / bnorm;
+  return or_nrms(A, ni, r, x, norm);
+}
+
+/* Cg.f90:2-257, with the stopping measure selectable (OR_CRIT_NRMS is the
+ * reference's). */
+int or_cg_ex(or_matrix* A, double* x, double* b, int prec, int miter,
+             int* niter, double tol, double* fin_res, const double* norm,
+             int crit) {
+  const int nt = A->nt, ni = A->ni;
+  /* Work % Connect_Real_Cell zero-fills the scratch arrays (Cg.f90:44) */
+  double* p1 = (double*)calloc((size_t)nt, sizeof(double));
+  double* q1 = (double*)calloc((size_t)nt, sizeof(double));
+  double* r1 = (double*)calloc((size_t)nt, sizeof(double));
+  double* d = (double*)calloc((size_t)nt, sizeof(double));
+  double* d_inv = (double*)calloc((size_t)nt, sizeof(double));
+  if (!p1 || !q1 || !r1 || !d || !d_inv) {
+    free(p1); free(q1); free(r1); free(d); free(d_inv);
+    return -2;
+  }
+  double res = 0.0, rho = 0.0, rho_old = 0.0, alfa, beta, bnrm2;
+  int iter;
+
+  double fn = normalise(A, b);                  /* :62-84 */
+  or_prec_form(A, ni, d, d_inv, prec);          /* :89 */
+
+  double bnorm = sqrt(sumsq(b, ni));
+  if (crit == OR_CRIT_RELRES)
+    bnrm2 = bnorm > 0.0 ? 1.0 : 0.0;            /* b = 0 solves trivially */
+  else
+    bnrm2 = or_nrms(A, ni, b, x, norm);         /* :95-99 */
+  if (bnrm2 < tol) { iter = 0; goto done; }     /* :101-104 */
+
+  or_residual(A, ni, r1, b, x);                 /* :109 */
+  res = measure(A, ni, r1, x, norm, crit, bnorm);   /* :114-118 */
+  if (res < tol) { iter = 0; goto done; }       /* :120-123 */
+
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ni; i++) p1[i] = r1[i];   /* :128-132 */
+
+  for (iter = 1; iter <= miter; iter++) {       /* :139 */
+    or_prec_solve(A, ni, d, d_inv, q1, r1, prec);   /* :145 */
+    rho = 0.0;                                  /* :150-156 */
+    if (g_sum_order) {
+      rho = blocked(q1, r1, ni, 0);
+    } else {
+#pragma omp parallel for schedule(static) reduction(+ : rho)
+      for (int i = 0; i < ni; i++) rho = rho + q1[i] * r1[i];
+    }
+    if (iter == 1) {                            /* :158-163 */
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < ni; i++) p1[i] = q1[i];
+    } else {                                    /* :164-171 */
+      beta = rho / rho_old;
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < ni; i++) p1[i] = q1[i] + beta * p1[i];
+    }
+    or_spmv(A, ni, p1, q1);                     /* :176-185 */
+    alfa = 0.0;                                 /* :190-198 */
+    if (g_sum_order) {
+      alfa = blocked(p1, q1, ni, 0);
+    } else {
+#pragma omp parallel for schedule(static) reduction(+ : alfa)
+      for (int i = 0; i < ni; i++) alfa = alfa + p1[i] * q1[i];
+    }
+    alfa = rho / alfa;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < ni; i++) {              /* :204-209 */
+      x[i] = x[i] + alfa * p1[i];
+      r1[i] = r1[i] - alfa * q1[i];
+    }
+    res = measure(A, ni, r1, x, norm, crit, bnorm);  /* :214-218 */
+    if (res < tol) goto done;                   /* :220 */
+    rho_old = rho;                              /* :222 */
+  }
+  iter = iter - 1;                              /* :228 */
+
+done:
+  denormalise(A, b, fn);                        /* :245-250 */
+  *fin_res = res;                               /* :252-253 */
+  *niter = iter;
+  free(p1); free(q1); free(r1); free(d); free(d_inv);
+  return 0;
+}
+
+int or_cg(or_matrix* A, double* x, double* b, int prec, int miter, int* niter,
+          double tol, double* fin_res, const double* norm) {
+  return or_cg_ex(A, x, b, prec, miter, niter, tol, fin_res, norm, OR_CRIT_NRMS);
+}
+
+/* Bicg.f90:2-269, restated literally with its shadow vectors. */
+int or_bicg(or_matrix* A, double* x, double* b, int prec, int miter, int* niter,
+            double tol, double* fin_res, const double* norm) {
+  const int nt = A->nt, ni = A->ni;
+  double* p1 = (double*)calloc((size_t)nt, sizeof(double));
+  double* q1 = (double*)calloc((size_t)nt, sizeof(double));
+  double* r1 = (double*)calloc((size_t)nt, sizeof(double));
+  double* p2 = (double*)calloc((size_t)nt, sizeof(double));
+  double* q2 = (double*)calloc((size_t)nt, sizeof(double));
+  double* r2 = (double*)calloc((size_t)nt, sizeof(double));
+  double* d = (double*)calloc((size_t)nt, sizeof(double));
+  double* d_inv = (double*)calloc((size_t)nt, sizeof(double));
+  if (!p1 || !q1 || !r1 || !p2 || !q2 || !r2 || !d || !d_inv) {
+    free(p1); free(q1); free(r1); free(p2); free(q2); free(r2); free(d); free(d_inv);
+    return -2;
+  }
+  double res = 0.0, rho = 0.0, rho_old = 0.0, alfa, beta, bnrm2;
+  int iter;
+
+  double fn = normalise(A, b);                  /* :63-85 */
+  or_prec_form(A, ni, d, d_inv, prec);          /* :90 */
+  bnrm2 = or_nrms(A, ni, b, x, norm);           /* :96-100 */
+  if (bnrm2 < tol) { iter = 0; goto done; }
+  or_residual(A, ni, r1, b, x);                 /* :110 */
+  res = or_nrms(A, ni, r1, x, norm);            /* :115-119 */
+  if (res < tol) { iter = 0; goto done; }
+  for (int i = 0; i < ni; i++) r2[i] = r1[i];   /* :129-133 */
+
+  for (iter = 1; iter <= miter; iter++) {       /* :140 */
+    or_prec_solve(A, ni, d, d_inv, q1, r1, prec);   /* :147 */
+    or_prec_solve(A, ni, d, d_inv, q2, r2, prec);   /* :148 */
+    rho = 0.0;                                  /* :153-159 */
+    for (int i = 0; i < ni; i++) rho = rho + q1[i] * r2[i];
+    if (iter == 1) {                            /* :161-167 */
+      for (int i = 0; i < ni; i++) { p1[i] = q1[i]; p2[i] = q2[i]; }
+    } else {                                    /* :168-176 */
+      beta = rho / rho_old;
+      for (int i = 0; i < ni; i++) {
+        p1[i] = q1[i] + beta * p1[i];
+        p2[i] = q2[i] + beta * p2[i];
+      }
+    }
+    for (int i = 0; i < ni; i++) {              /* :184-195, both with A */
+      double s1 = 0.0, s2 = 0.0;
+      for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) {
+        int k = A->col[j] - 1;
+        s1 = s1 + A->val[j] * p1[k];
+        s2 = s2 + A->val[j] * p2[k];
+      }
+      q1[i] = s1;
+      q2[i] = s2;
+    }
+    alfa = 0.0;                                 /* :200-207 */
+    for (int i = 0; i < ni; i++) alfa = alfa + p2[i] * q1[i];
+    alfa = rho / alfa;
+    for (int i = 0; i < ni; i++) {              /* :213-219 */
+      x[i] = x[i] + alfa * p1[i];
+      r1[i] = r1[i] - alfa * q1[i];
+      r2[i] = r2[i] - alfa * q2[i];
+    }
+    res = or_nrms(A, ni, r1, x, norm);          /* :224-228 */
+    if (res < tol) goto done;                   /* :230 */
+    rho_old = rho;                              /* :232 */
+  }
+  iter = iter - 1;                              /* :238 */
+
+done:
+  denormalise(A, b, fn);                        /* :255-262 */
+  *fin_res = res;
+  *niter = iter;
+  free(p1); free(q1); free(r1); free(p2); free(q2); free(r2); free(d); free(d_inv);
+  return 0;
+}

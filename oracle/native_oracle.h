@@ -1,0 +1,62 @@
+/*
+ * CPU restatement of T-Flows' native Krylov solvers (Native_Mod).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU path is
+ * compared against and the CPU baseline that is timed beside it.  Only the
+ * tests, the smoke check and the benchmark's CPU-baseline leg may load it; the
+ * GPU library never calls it and has no CPU fallback.
+ *
+ * Arrays use C indexing: element i here is Fortran element i+1 of the
+ * reference.  Index arrays (row, col, dia) keep the reference's 1-based
+ * VALUES, exactly as Matrix_Type stores them.
+ */
+#ifndef TFLOWS_NATIVE_ORACLE_H
+#define TFLOWS_NATIVE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_PREC_NONE = 0, OR_PREC_JACOBI = 1, OR_PREC_ICC = 2 };
+enum { OR_CRIT_NRMS = 0, OR_CRIT_RELRES = 1 };
+
+typedef struct {
+  int ni;          /* rows solved (cells minus buffer cells)          */
+  int nt;          /* rows stored (all cells)                          */
+  const int* row;  /* nt+1, 1-based positions                          */
+  const int* col;  /* nnz,  1-based cell indices                       */
+  const int* dia;  /* nt,   1-based position of the diagonal           */
+  double* val;     /* nnz,  scaled and de-scaled in place by the solve */
+} or_matrix;
+
+double or_nrms(const or_matrix* A, int ni, const double* r, const double* x,
+               const double* norm);
+void or_residual(const or_matrix* A, int ni, double* r, const double* b,
+                 const double* x);
+void or_spmv(const or_matrix* A, int ni, const double* p, double* q);
+void or_prec_form(const or_matrix* A, int ni, double* d, double* d_inv, int prec);
+void or_prec_solve(const or_matrix* A, int ni, const double* d,
+                   const double* d_inv, double* x, const double* b, int prec);
+
+/* Cg.f90 / Bicg.f90.  x: nt entries (x0 in, solution out); b: nt entries.
+ * norm: NULL when the reference's optional argument is absent. */
+int or_cg(or_matrix* A, double* x, double* b, int prec, int miter, int* niter,
+          double tol, double* fin_res, const double* norm);
+int or_bicg(or_matrix* A, double* x, double* b, int prec, int miter, int* niter,
+            double tol, double* fin_res, const double* norm);
+
+/* Same CG recurrence with a selectable stopping test: OR_CRIT_NRMS is the
+ * reference's; OR_CRIT_RELRES stops on ||r||_2 / ||b||_2 < tol (recursive
+ * residual, in the scaled system, which has the same ratio). */
+int or_cg_ex(or_matrix* A, double* x, double* b, int prec, int miter,
+             int* niter, double tol, double* fin_res, const double* norm,
+             int criterion);
+
+/* 0: the reference's serial summation (default); 1: blocked-pairwise
+ * summation of every global reduction, to measure rounding sensitivity. */
+void or_set_sum_order(int order);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,167 @@
+"""ctypes front end of the CPU restatement of T-Flows' Native_Mod.
+
+TEST INFRASTRUCTURE ONLY: imported by the tests, the smoke check and the
+benchmark's CPU-baseline leg, as the checker.  The GPU library never uses it.
+
+Each wrapper mirrors one reference routine (Sources/Process/Cpu/Native_Mod/):
+  cg / bicg        Cg.f90:2-257 / Bicg.f90:2-269
+  nrms             Normalized_Root_Mean_Square.f90:2-65
+  residual         Residual_Vector.f90:2-44
+  spmv             the product loop of Cg.f90:177-185
+  prec_form/solve  Prec_Form.f90:2-93 / Prec_Solve.f90:2-107
+Index arrays keep the reference's 1-based values; vectors are plain arrays of
+length nt (Fortran element i is Python element i-1).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PREC = {"none": 0, "jacobi": 1, "icc": 2}
+CRIT = {"nrms": 0, "relres": 1}
+
+
+class _Mat(ctypes.Structure):
+    _fields_ = [("ni", ctypes.c_int), ("nt", ctypes.c_int),
+                ("row", ctypes.c_void_p), ("col", ctypes.c_void_p),
+                ("dia", ctypes.c_void_p), ("val", ctypes.c_void_p)]
+
+
+_libs = {}
+
+
+def lib(omp=False):
+    name = "liboracle_omp.so" if omp else "liboracle.so"
+    if name not in _libs:
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        I, D = ctypes.c_int, ctypes.c_double
+        L.or_nrms.restype = D
+        L.or_nrms.argtypes = [P, I, P, P, P]
+        L.or_residual.argtypes = [P, I, P, P, P]
+        L.or_spmv.argtypes = [P, I, P, P]
+        L.or_prec_form.argtypes = [P, I, P, P, I]
+        L.or_prec_solve.argtypes = [P, I, P, P, P, P, I]
+        for f in (L.or_cg, L.or_bicg):
+            f.restype = I
+            f.argtypes = [P, P, P, I, I, P, D, P, P]
+        L.or_cg_ex.restype = I
+        L.or_cg_ex.argtypes = [P, P, P, I, I, P, D, P, P, I]
+        L.or_set_sum_order.argtypes = [I]
+        _libs[name] = L
+    return _libs[name]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Matrix:
+    """Matrix_Type restated: row/col/dia (1-based int32) and val (FP64).
+
+    The solve scales `val` in place and scales it back afterwards, exactly as
+    the reference does (Cg.f90:77-84, 245-250)."""
+
+    def __init__(self, row, col, dia, val, ni=None):
+        self.row = np.ascontiguousarray(row, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.dia = np.ascontiguousarray(dia, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.nt = len(self.row) - 1
+        self.ni = self.nt if ni is None else int(ni)
+        self._s = _Mat(self.ni, self.nt, self.row.ctypes.data, self.col.ctypes.data,
+                       self.dia.ctypes.data, self.val.ctypes.data)
+
+    @property
+    def ref(self):
+        return ctypes.byref(self._s)
+
+    def copy(self):
+        return Matrix(self.row.copy(), self.col.copy(), self.dia.copy(),
+                      self.val.copy(), self.ni)
+
+
+def _f64(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    assert a.shape == (n,)
+    return a
+
+
+def nrms(A, r, x, norm=None):
+    r, x = _f64(r, A.nt), _f64(x, A.nt)
+    nv = ctypes.c_double(norm) if norm is not None else None
+    return lib().or_nrms(A.ref, A.ni, _p(r), _p(x), ctypes.byref(nv) if nv is not None else None)
+
+
+def residual(A, b, x):
+    b, x = _f64(b, A.nt), _f64(x, A.nt)
+    r = np.zeros(A.nt)
+    lib().or_residual(A.ref, A.ni, _p(r), _p(b), _p(x))
+    return r
+
+
+def spmv(A, p, omp=False):
+    p = _f64(p, A.nt)
+    q = np.zeros(A.nt)
+    lib(omp).or_spmv(A.ref, A.ni, _p(p), _p(q))
+    return q
+
+
+def prec_form(A, prec):
+    d = np.zeros(A.nt)
+    dinv = np.zeros(A.nt)
+    lib().or_prec_form(A.ref, A.ni, _p(d), _p(dinv), PREC[prec])
+    return d, dinv
+
+
+def prec_solve(A, d, dinv, b, prec):
+    d, dinv, b = _f64(d, A.nt), _f64(dinv, A.nt), _f64(b, A.nt)
+    x = np.zeros(A.nt)
+    lib().or_prec_solve(A.ref, A.ni, _p(d), _p(dinv), _p(x), _p(b), PREC[prec])
+    return x
+
+
+def solve(solver, prec, A, x, b, miter, tol, norm=None, criterion="nrms", omp=False,
+          sum_order=0):
+    """Solve_Native.f90:2-46 restated: dispatches 'cg' / 'bicg'.
+
+    x and b are updated in place like the reference's intent(out)/(inout)
+    arguments.  Returns (niter, fin_res).  sum_order=1 switches every global
+    reduction to a blocked-pairwise order (not the reference's): it measures
+    how far rounding alone moves the answer."""
+    assert x.dtype == np.float64 and x.flags.c_contiguous and x.shape == (A.nt,)
+    assert b.dtype == np.float64 and b.flags.c_contiguous and b.shape == (A.nt,)
+    niter = ctypes.c_int(0)
+    fin = ctypes.c_double(0.0)
+    nv = ctypes.c_double(norm) if norm is not None else None
+    nref = ctypes.byref(nv) if nv is not None else None
+    L = lib(omp)
+    L.or_set_sum_order(int(sum_order))
+    try:
+        return _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin)
+    finally:
+        L.or_set_sum_order(0)
+
+
+def _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin):
+    if criterion != "nrms":
+        if solver not in ("cg", "bicg"):
+            raise ValueError(f"Unknown native solver: {solver}")
+        rc = L.or_cg_ex(A.ref, _p(x), _p(b), PREC.get(prec, 0), int(miter),
+                        ctypes.byref(niter), float(tol), ctypes.byref(fin), nref,
+                        CRIT[criterion])
+    elif solver == "cg":
+        rc = L.or_cg(A.ref, _p(x), _p(b), PREC.get(prec, 0), int(miter),
+                     ctypes.byref(niter), float(tol), ctypes.byref(fin), nref)
+    elif solver == "bicg":
+        rc = L.or_bicg(A.ref, _p(x), _p(b), PREC.get(prec, 0), int(miter),
+                       ctypes.byref(niter), float(tol), ctypes.byref(fin), nref)
+    else:
+        raise ValueError(f"Unknown native solver: {solver}")
+    if rc != 0:
+        raise MemoryError("oracle scratch allocation failed")
+    return niter.value, fin.value

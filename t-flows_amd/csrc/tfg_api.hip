@@ -1,0 +1,498 @@
+// C-ABI of tflows_gpu: structure upload, the Solve_Native drop-in and the
+// device building blocks.  Host orchestration only; kernels live in
+// tfg_kernels.hip (FP64) and tfg_ir.hip (FP32 inner / FP64 refinement).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tfg_internal.h"
+
+using tfg::State;
+
+struct tfg_matrix {
+  int device = 0;
+  int ni = 0, nt = 0, nnz = 0;
+  int cu_count = 0;
+  hipStream_t stream = nullptr;
+  // structure (0-based)
+  int* row = nullptr;
+  int* col = nullptr;
+  int* blk = nullptr;
+  int nblk = 0;
+  int cap = 2048;
+  // FP64 work
+  double* val = nullptr;  // host-path copy of val
+  double* b = nullptr;    // host-path copy of b
+  double* x = nullptr;
+  double* r = nullptr;
+  double* p = nullptr;
+  double* q = nullptr;
+  double* ad = nullptr;
+  double* part = nullptr;     // partial sums of every phase (see Partials)
+  int pstride = 0;
+  int spmv_per_cu = 1;        // resident SpMV workgroups per CU
+  State* st = nullptr;       // device scalars of the solve
+  State* st_in = nullptr;    // device scalars of an inner (FP32) solve
+  State* hst = nullptr;      // pinned: [0] set-up, [1],[2] polling slots
+  hipEvent_t ev_poll[2] = {nullptr, nullptr};
+  hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
+  std::vector<hipEvent_t> ev_sample;
+  // FP32 inner-solve work (allocated on first use)
+  float* val32 = nullptr;
+  float* e32 = nullptr;
+  float* r32 = nullptr;
+  float* p32 = nullptr;
+  float* q32 = nullptr;
+  float* ad32 = nullptr;
+};
+
+namespace {
+
+// Rows of the partials buffer: diag 1, prologue 7, spmv 1, update 5, aux 2.
+constexpr int kPartRows = 16;
+
+tfg::Partials partials_of(const tfg_matrix* h, const tfg::Launch& L) {
+  tfg::Partials pt;
+  pt.pstride = h->pstride;
+  pt.diag = h->part;
+  pt.pro = pt.diag + 1 * (size_t)h->pstride;
+  pt.spmv = pt.pro + 7 * (size_t)h->pstride;
+  pt.upd = pt.spmv + 1 * (size_t)h->pstride;
+  pt.aux = pt.upd + 5 * (size_t)h->pstride;
+  pt.g_vec = L.grid_vec;
+  pt.g_spmv = L.grid_spmv;
+  return pt;
+}
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(e_ == hipErrorOutOfMemory ? TFG_ENOMEM : TFG_EHIP, "%s: %s (%s:%d)", \
+                  #expr, hipGetErrorString(e_), __FILE__, __LINE__);                  \
+  } while (0)
+
+// Fortran CHARACTER comparison ignores trailing blanks.
+std::string fstring(const char* s, int len) {
+  if (!s) return std::string();
+  size_t n = len < 0 ? strlen(s) : (size_t)len;
+  while (n > 0 && (s[n - 1] == ' ' || s[n - 1] == '\0')) --n;
+  return std::string(s, n);
+}
+
+// Greedy row blocks: at most 256 rows and at most cap nonzeros each; a single
+// longer row forms a block of its own.
+void make_blocks(const int* row1, int n, int cap, std::vector<int>& blk) {
+  blk.clear();
+  blk.push_back(0);
+  int r = 0;
+  while (r < n) {
+    int start = r;
+    long long s = row1[r];
+    while (r < n && r - start < tfg::kRowsPerBlock && (long long)row1[r + 1] - s <= cap) ++r;
+    if (r == start) ++r;  // long row
+    blk.push_back(r);
+  }
+}
+
+int pick_cap(int n, int nnz) {
+  double avg = (double)nnz / (double)n;
+  return avg * tfg::kRowsPerBlock <= 2048.0 * 0.97 ? 2048 : 4096;
+}
+
+tfg::Launch launch_of(const tfg_matrix* h, const void* val) {
+  tfg::Launch L;
+  L.stream = h->stream;
+  L.cap = h->cap;
+  int gs = std::min(h->nblk, std::max(1, h->cu_count) * h->spmv_per_cu);
+  if (gs >= tfg::kXcds) gs -= gs % tfg::kXcds;
+  L.grid_spmv = std::max(1, gs);
+  int pairs = (h->nt + 1) / 2;
+  int gv = (pairs + tfg::kBlock - 1) / tfg::kBlock;
+  L.grid_vec = std::max(1, std::min(gv, std::max(1, h->cu_count) * 4));
+  L.vec_val = ((uintptr_t)val % 16) == 0;
+  return L;
+}
+
+int alloc_work(tfg_matrix* h) {
+  const size_t n = (size_t)h->nt;
+  HIP_TRY(hipMalloc(&h->x, n * sizeof(double)));
+  HIP_TRY(hipMalloc(&h->r, n * sizeof(double)));
+  HIP_TRY(hipMalloc(&h->p, n * sizeof(double)));
+  HIP_TRY(hipMalloc(&h->q, n * sizeof(double)));
+  HIP_TRY(hipMalloc(&h->ad, n * sizeof(double)));
+  h->spmv_per_cu = tfg::spmv_blocks_per_cu(h->cap);
+  tfg::Launch L = launch_of(h, nullptr);
+  h->pstride = std::max(L.grid_spmv, L.grid_vec);
+  HIP_TRY(hipMalloc(&h->part, sizeof(double) * kPartRows * (size_t)h->pstride));
+  HIP_TRY(hipMemset(h->part, 0, sizeof(double) * kPartRows * (size_t)h->pstride));
+  HIP_TRY(hipMalloc(&h->st, sizeof(State)));
+  HIP_TRY(hipMalloc(&h->st_in, sizeof(State)));
+  HIP_TRY(hipMemset(h->st, 0, sizeof(State)));
+  HIP_TRY(hipMemset(h->st_in, 0, sizeof(State)));
+  HIP_TRY(hipHostMalloc(&h->hst, 4 * sizeof(State), hipHostMallocDefault));
+  memset(h->hst, 0, 4 * sizeof(State));
+  HIP_TRY(hipEventCreate(&h->ev_poll[0]));
+  HIP_TRY(hipEventCreate(&h->ev_poll[1]));
+  HIP_TRY(hipEventCreate(&h->ev_t0));
+  HIP_TRY(hipEventCreate(&h->ev_t1));
+  return TFG_OK;
+}
+
+void free_all(tfg_matrix* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  void* dev[] = {h->row, h->col, h->blk, h->val, h->b, h->x, h->r, h->p, h->q, h->ad,
+                 h->part, h->st, h->st_in, h->val32, h->e32, h->r32,
+                 h->p32, h->q32, h->ad32};
+  for (void* d : dev)
+    if (d) (void)hipFree(d);
+  if (h->hst) (void)hipHostFree(h->hst);
+  for (hipEvent_t e : {h->ev_poll[0], h->ev_poll[1], h->ev_t0, h->ev_t1})
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->ev_sample)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+}  // namespace
+
+namespace tfg {
+// Defined in tfg_ir.hip.
+int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int miter,
+             int* niter, double tol, double* fin_res, const double* norm,
+             const tfg_options& o, tfg_report* rep);
+}
+
+// ===========================================================================
+extern "C" {
+
+const char* tfg_last_error(void) { return g_err.c_str(); }
+
+int tfg_version(void) { return 1; }
+
+int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, const int* dia,
+                      tfg_matrix** handle) {
+  if (!handle || !row || !col || !dia) return fail(TFG_EINVAL, "null argument");
+  *handle = nullptr;
+  if (nt < 1 || ni < 1 || ni > nt) return fail(TFG_EINVAL, "bad sizes ni=%d nt=%d", ni, nt);
+  if (ni != nt)
+    return fail(TFG_ENOTSUP, "buffer cells (ni=%d < nt=%d) need halo exchange hooks", ni, nt);
+  if (row[0] != 1) return fail(TFG_EINVAL, "row(1) must be 1");
+  if ((long long)row[nt] - 1 != nnz || nnz < nt)
+    return fail(TFG_EINVAL, "nnz=%d does not match row(nt+1)-1=%d", nnz, row[nt] - 1);
+  for (int i = 0; i < nt; ++i) {
+    if (row[i + 1] <= row[i]) return fail(TFG_EINVAL, "row %d is empty or row not increasing", i + 1);
+    if (dia[i] != row[i] || col[row[i] - 1] != i + 1)
+      return fail(TFG_EINVAL, "diagonal must be the first entry of row %d", i + 1);
+  }
+  for (long long j = 0; j < nnz; ++j)
+    if (col[j] < 1 || col[j] > nt) return fail(TFG_EINVAL, "col(%lld)=%d out of range", j + 1, col[j]);
+
+  tfg_matrix* h = new tfg_matrix();
+  h->ni = ni;
+  h->nt = nt;
+  h->nnz = nnz;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return fail(TFG_EHIP, "no HIP device");
+  }
+  int rc = TFG_OK;
+  do {
+    hipError_t e = hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount,
+                                         h->device);
+    if (e != hipSuccess) { rc = fail(TFG_EHIP, "device attribute: %s", hipGetErrorString(e)); break; }
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { rc = fail(TFG_EHIP, "stream: %s", hipGetErrorString(e)); break; }
+
+    std::vector<int> tmp((size_t)std::max(nnz, nt + 1));
+    h->cap = pick_cap(nt, nnz);
+    std::vector<int> blk;
+    make_blocks(row, nt, h->cap, blk);
+    // make_blocks works on 1-based starts; differences are what matter.
+    h->nblk = (int)blk.size() - 1;
+
+    e = hipMalloc(&h->row, sizeof(int) * (size_t)(nt + 1));
+    if (e == hipSuccess) e = hipMalloc(&h->col, sizeof(int) * (size_t)nnz);
+    if (e == hipSuccess) e = hipMalloc(&h->blk, sizeof(int) * blk.size());
+    if (e != hipSuccess) { rc = fail(TFG_ENOMEM, "structure allocation: %s", hipGetErrorString(e)); break; }
+    for (int i = 0; i <= nt; ++i) tmp[i] = row[i] - 1;
+    e = hipMemcpy(h->row, tmp.data(), sizeof(int) * (size_t)(nt + 1), hipMemcpyHostToDevice);
+    for (long long j = 0; j < nnz; ++j) tmp[j] = col[j] - 1;
+    if (e == hipSuccess) e = hipMemcpy(h->col, tmp.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { rc = fail(TFG_EHIP, "structure upload: %s", hipGetErrorString(e)); break; }
+    rc = alloc_work(h);
+  } while (0);
+  if (rc != TFG_OK) {
+    free_all(h);
+    return rc;
+  }
+  *handle = h;
+  return TFG_OK;
+}
+
+int tfg_matrix_destroy(tfg_matrix* h) {
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  free_all(h);
+  return TFG_OK;
+}
+
+int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_blocks, int* spmv_cap,
+                    int* col_mode) {
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  if (n) *n = h->nt;
+  if (nnz) *nnz = h->nnz;
+  if (spmv_blocks) *spmv_blocks = h->nblk;
+  if (spmv_cap) *spmv_cap = h->cap;
+  if (col_mode) *col_mode = 32;
+  return TFG_OK;
+}
+
+int tfg_solve_native(tfg_matrix* h, const char* solver, int solver_len, const char* prec,
+                     int prec_len, const double* val, double* x, double* b, int miter,
+                     int* niter, double tol, double* fin_res, const double* norm) {
+  return tfg_solve_native_ex(h, solver, solver_len, prec, prec_len, val, x, b, miter, niter,
+                             tol, fin_res, norm, 0, nullptr, nullptr);
+}
+
+int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const char* prec,
+                        int prec_len, const double* val, double* x, double* b, int miter,
+                        int* niter, double tol, double* fin_res, const double* norm,
+                        int device_ptrs, const tfg_options* opt, tfg_report* rep) {
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  if (!val || !x || !b || !niter || !fin_res) return fail(TFG_EINVAL, "null argument");
+  // Solve_Native.f90:33-44: 'bicg' and 'cg' only.  The reference's "bicg"
+  // applies A and M (not their transposes) to the shadow vectors and starts
+  // them equal to the primal ones (Bicg.f90:129-133,143-195), so its iterates
+  // are bit-identical to Cg's; it runs the CG recurrence here.
+  const std::string sv = fstring(solver, solver_len);
+  if (sv != "cg" && sv != "bicg")
+    return fail(TFG_EINVAL, "Unknown native solver: %s", sv.c_str());
+  // Prec_Form.f90:47-89: 'jacobi', 'icc', anything else = none.
+  const std::string pv = fstring(prec, prec_len);
+  const int pr = pv == "jacobi" ? tfg::PREC_JACOBI : (pv == "icc" ? tfg::PREC_ICC : tfg::PREC_NONE);
+  if (pr == tfg::PREC_ICC)
+    return fail(TFG_ENOTSUP, "preconditioner 'icc' is not available on the GPU path yet");
+  tfg_options o{};
+  if (opt) o = *opt;
+  if (o.criterion != TFG_CRIT_NRMS && o.criterion != TFG_CRIT_RELRES)
+    return fail(TFG_EINVAL, "unknown criterion %d", o.criterion);
+  if (o.precision != TFG_FP64 && o.precision != TFG_FP32_IR)
+    return fail(TFG_EINVAL, "unknown precision %d", o.precision);
+  if (rep) {
+    memset(rep, 0, sizeof(*rep));
+    rep->true_relres = -1.0;
+  }
+  HIP_TRY(hipSetDevice(h->device));
+  const size_t n = (size_t)h->nt;
+  const double* dval = val;
+  const double* db = b;
+  if (!device_ptrs) {
+    if (!h->val) HIP_TRY(hipMalloc(&h->val, sizeof(double) * (size_t)h->nnz));
+    if (!h->b) HIP_TRY(hipMalloc(&h->b, sizeof(double) * n));
+    HIP_TRY(hipMemcpyAsync(h->val, val, sizeof(double) * (size_t)h->nnz, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->b, b, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->x, x, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    dval = h->val;
+    db = h->b;
+  } else {
+    HIP_TRY(hipMemcpyAsync(h->x, x, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+  }
+
+  int rc;
+  if (o.precision == TFG_FP32_IR) {
+    rc = tfg::solve_ir(h, pr, dval, db, miter, niter, tol, fin_res, norm, o, rep);
+    if (rc != TFG_OK && rc != TFG_EBREAKDOWN) return rc;
+  } else {
+    tfg::Launch L = launch_of(h, dval);
+    tfg::CgArgs a{};
+    a.n = h->ni;
+    a.row = h->row;
+    a.col = h->col;
+    a.blk = h->blk;
+    a.nblk = h->nblk;
+    a.val = dval;
+    a.b = db;
+    a.x = h->x;
+    a.r = h->r;
+    a.p = h->p;
+    a.q = h->q;
+    a.ad = h->ad;
+    a.pt = partials_of(h, L);
+    a.st = h->st;
+
+    State* s0 = &h->hst[0];
+    memset(s0, 0, sizeof(State));
+    s0->tol = tol;
+    s0->criterion = o.criterion;
+    s0->has_norm = norm ? 1 : 0;
+    s0->norm = norm ? *norm : 0.0;
+    HIP_TRY(hipMemcpyAsync(h->st, s0, sizeof(State), hipMemcpyHostToDevice, h->stream));
+
+    const bool timing = o.timing != 0;
+    const int max_samples = 256;
+    if (timing && h->ev_sample.empty()) {
+      h->ev_sample.resize(2 * max_samples, nullptr);
+      for (auto& e : h->ev_sample) HIP_TRY(hipEventCreate(&e));
+    }
+    std::vector<int> sample_iter;
+
+    HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
+    HIP_TRY(tfg::launch_diag_sum(L, a));
+    HIP_TRY(tfg::launch_prologue(L, a, pr));
+    HIP_TRY(tfg::launch_start(L, a));
+
+    const int chunk = h->nnz > (1 << 22) ? 8 : 32;
+    int k = 1, slot = 0, prev = -1;
+    bool stop = false;
+    while (k <= miter && !stop) {
+      const int kend = std::min(miter, k + chunk - 1);
+      for (; k <= kend; ++k) {
+        HIP_TRY(tfg::launch_pupdate(L, a, pr, k));
+        const bool smp = timing && (k % 4 == 1) && (int)sample_iter.size() < max_samples;
+        if (smp) HIP_TRY(hipEventRecord(h->ev_sample[2 * sample_iter.size()], h->stream));
+        HIP_TRY(tfg::launch_spmv_pq(L, a, k));
+        if (smp) {
+          HIP_TRY(hipEventRecord(h->ev_sample[2 * sample_iter.size() + 1], h->stream));
+          sample_iter.push_back(k);
+        }
+        HIP_TRY(tfg::launch_update(L, a, pr, k));
+      }
+      HIP_TRY(tfg::launch_finish(L, a, kend));
+      HIP_TRY(hipMemcpyAsync(&h->hst[1 + slot], h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
+      HIP_TRY(hipEventRecord(h->ev_poll[slot], h->stream));
+      if (prev >= 0) {
+        HIP_TRY(hipEventSynchronize(h->ev_poll[prev]));
+        if (h->hst[1 + prev].done) stop = true;
+      }
+      prev = slot;
+      slot ^= 1;
+    }
+    HIP_TRY(hipEventRecord(h->ev_t1, h->stream));
+    State* sf = &h->hst[3];
+    HIP_TRY(hipMemcpyAsync(sf, h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *niter = sf->done ? sf->niter : std::max(miter, 0);
+    *fin_res = sf->res;
+    rc = sf->status;
+    if (rep) {
+      rep->status = rc;
+      rep->iterations = *niter;
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, h->ev_t0, h->ev_t1));
+      rep->solve_ms = ms;
+      double tot = 0.0;
+      int cnt = 0;
+      for (size_t s = 0; s < sample_iter.size(); ++s) {
+        if (sample_iter[s] > *niter) break;  // launches after the stop return at once
+        float e = 0.f;
+        HIP_TRY(hipEventElapsedTime(&e, h->ev_sample[2 * s], h->ev_sample[2 * s + 1]));
+        tot += e;
+        ++cnt;
+      }
+      rep->spmv_ms = cnt ? tot / cnt : 0.0;
+      rep->spmv_samples = cnt;
+    }
+    if (o.true_residual) {
+      tfg::CgArgs t = a;
+      t.val = dval;
+      t.b = db;
+      HIP_TRY(tfg::launch_true_residual(L, t));
+      HIP_TRY(hipMemcpyAsync(sf, h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      if (rep) rep->true_relres = sf->aux0;
+    }
+  }
+
+  if (!device_ptrs)
+    HIP_TRY(hipMemcpyAsync(x, h->x, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+  else
+    HIP_TRY(hipMemcpyAsync(x, h->x, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (rc == TFG_EBREAKDOWN) return fail(TFG_EBREAKDOWN, "Krylov breakdown at iteration %d", *niter);
+  return TFG_OK;
+}
+
+int tfg_spmv(tfg_matrix* h, const double* d_val, const double* d_p, double* d_q) {
+  if (!h || !d_val || !d_p || !d_q) return fail(TFG_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  tfg::Launch L = launch_of(h, d_val);
+  HIP_TRY(tfg::launch_spmv_plain(L, h->nt, h->row, h->col, h->blk, h->nblk, d_val, d_p, d_q));
+  return TFG_OK;
+}
+
+int tfg_residual(tfg_matrix* h, const double* d_val, const double* d_b, const double* d_x,
+                 double* d_r) {
+  if (!h || !d_val || !d_b || !d_x || !d_r) return fail(TFG_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  tfg::Launch L = launch_of(h, d_val);
+  HIP_TRY(tfg::launch_residual_plain(L, h->nt, h->row, h->col, h->blk, h->nblk, d_val, d_b,
+                                     d_x, d_r));
+  return TFG_OK;
+}
+
+int tfg_spmv_bench(tfg_matrix* h, const double* d_val, const double* d_p, double* d_q, int reps,
+                   double* ms_avg) {
+  if (!h || !ms_avg || reps < 1) return fail(TFG_EINVAL, "bad argument");
+  HIP_TRY(hipSetDevice(h->device));
+  tfg::Launch L = launch_of(h, d_val);
+  HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
+  for (int i = 0; i < reps; ++i)
+    HIP_TRY(tfg::launch_spmv_plain(L, h->nt, h->row, h->col, h->blk, h->nblk, d_val, d_p, d_q));
+  HIP_TRY(hipEventRecord(h->ev_t1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev_t1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev_t0, h->ev_t1));
+  *ms_avg = ms / reps;
+  return TFG_OK;
+}
+
+int tfg_device_alloc(void** dptr, long long bytes) {
+  if (!dptr || bytes <= 0) return fail(TFG_EINVAL, "bad allocation request");
+  HIP_TRY(hipMalloc(dptr, (size_t)bytes));
+  return TFG_OK;
+}
+
+int tfg_device_free(void* dptr) {
+  HIP_TRY(hipFree(dptr));
+  return TFG_OK;
+}
+
+int tfg_memcpy_h2d(void* dst, const void* src, long long bytes) {
+  HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  return TFG_OK;
+}
+
+int tfg_memcpy_d2h(void* dst, const void* src, long long bytes) {
+  HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return TFG_OK;
+}
+
+int tfg_synchronize(tfg_matrix* h) {
+  if (!h) HIP_TRY(hipDeviceSynchronize());
+  else HIP_TRY(hipStreamSynchronize(h->stream));
+  return TFG_OK;
+}
+
+}  // extern "C"

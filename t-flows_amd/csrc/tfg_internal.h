@@ -1,0 +1,99 @@
+// Internal declarations shared by the tflows_gpu kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tflows_gpu.h"
+
+namespace tfg {
+
+constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
+constexpr int kRowsPerBlock = 256; // SpMV rows per staged block (one per thread)
+constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dealt round robin
+constexpr double kHuge = 1.0e+15;  // Const_Mod.f90:84-93 (HUGE, TINY)
+constexpr double kTiny = 1.0e-15;
+
+enum Prec { PREC_NONE = 0, PREC_JACOBI = 1, PREC_ICC = 2 };
+
+// Scalars of one solve, resident in device memory.  Each field is written by
+// one thread (workgroup 0 of a kernel, or a one-workgroup kernel) and read by
+// later kernels; fields read inside the kernel that writes them are never
+// the ones it writes (rho is double-buffered by iteration parity).
+struct State {
+  double fn;        // 1 / mean(diag)                     (Cg.f90:62-76)
+  double rho[2];    // rho_k = (z, r) in rho[k & 1]       (Cg.f90:150-156)
+  double res;       // last stopping measure
+  double bnorm;     // ||b_scaled||_2
+  double norm;      // optional normalisation (valid when has_norm)
+  double tol;
+  double aux0;      // scratch results (true residual, refinement norms)
+  double aux1;
+  double aux2;
+  int has_norm;
+  int criterion;
+  int done;         // 1: stop, every later phase returns at once
+  int niter;        // iteration at which done was raised
+  int status;       // TFG_OK or TFG_EBREAKDOWN
+  int iter;         // last iteration whose stopping test was evaluated
+};
+
+// Partial sums written by one kernel and swept by the next ones.
+struct Partials {
+  double* diag;     // [1][pstride]  sum of the diagonal
+  double* pro;      // [7][pstride]  prologue sums
+  double* spmv;     // [1][pstride]  (p, q)
+  double* upd;      // [5][pstride]  NRMS sum, x min, x max, (z,r), (r,r)
+  double* aux;      // [2][pstride]  true-residual sums
+  int pstride;
+  int g_vec;        // grid of the element-wise kernels (producers of diag/upd)
+  int g_spmv;       // grid of the SpMV kernels (producers of pro/spmv/aux)
+};
+
+// Everything a CG phase kernel needs.  Device pointers.
+struct CgArgs {
+  int n;               // rows solved (ni)
+  const int* row;      // 0-based row starts, n+1
+  const int* col;      // 0-based columns, nnz
+  const int* blk;      // SpMV block row boundaries, nblk+1
+  int nblk;
+  const double* val;   // unscaled values (caller's)
+  const double* b;     // unscaled right-hand side (caller's)
+  double* x;
+  double* r;
+  double* p;
+  double* q;
+  double* ad;          // scaled diagonal a_ii * fn
+  Partials pt;
+  State* st;
+};
+
+// Launch shapes chosen per matrix.
+struct Launch {
+  hipStream_t stream;
+  int grid_vec;    // workgroups of the element-wise kernels
+  int grid_spmv;   // workgroups of the SpMV kernels (resident, multiple of kXcds)
+  int cap;         // staged nonzeros per SpMV block: 2048 | 4096
+  bool vec_val;    // val is 16-byte aligned (wide nontemporal loads)
+};
+
+// Resident SpMV workgroups per CU for a stage capacity (occupancy query).
+int spmv_blocks_per_cu(int cap);
+
+hipError_t launch_diag_sum(const Launch& L, const CgArgs& a);
+hipError_t launch_prologue(const Launch& L, const CgArgs& a, int prec);
+hipError_t launch_start(const Launch& L, const CgArgs& a);
+hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k);
+hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k);
+hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k);
+hipError_t launch_finish(const Launch& L, const CgArgs& a, int k);
+hipError_t launch_true_residual(const Launch& L, const CgArgs& a);
+
+hipError_t launch_spmv_plain(const Launch& L, int n, const int* row, const int* col,
+                             const int* blk, int nblk, const double* val,
+                             const double* p, double* q);
+hipError_t launch_residual_plain(const Launch& L, int n, const int* row, const int* col,
+                                 const int* blk, int nblk, const double* val,
+                                 const double* b, const double* x, double* r);
+
+}  // namespace tfg

@@ -1,0 +1,638 @@
+// FP64 kernels of the Jacobi / unpreconditioned CG (and the reference's
+// "bicg", which runs the same recurrence) for gfx950.
+//
+// The reference loop (Sources/Process/Cpu/Native_Mod/Cg.f90:139-224) is
+// re-cut into three bandwidth-bound phases per iteration:
+//   pupdate : p = z + beta p, z = r * d_inv              (Cg.f90:145,158-171)
+//   spmv_pq : q = A p, (p,q) -> alpha = rho / (p,q)       (Cg.f90:176-198)
+//   update  : x += alpha p, r -= alpha q, then NRMS sums, x range and the
+//             next (z,r) -> res, stop flag, beta          (Cg.f90:204-222 and
+//             the rho of the following iteration, Cg.f90:145-156)
+// Scalars live in device memory (State).  Every reduction is finished by the
+// kernel that consumes it (all of its workgroups sweep the partials in the
+// same fixed order), so the host never waits inside the loop and no kernel
+// needs atomics or device-scope fences.
+//
+// Parity: the matrix scaling A*fn of Cg.f90:77-84 is folded into the product
+// (each a_ij*fn is rounded exactly as the reference's scaled value), every
+// row sum runs in CSR order, and this file is compiled without FP
+// contraction, so each SpMV row and each element-wise update is bit-identical
+// to the reference's serial arithmetic; only the order of the global dot
+// products differs (deterministic trees here).
+#include "tfg_reduce.h"
+
+namespace tfg {
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef int int4v __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Workgroup -> row-block mapping.  Workgroup g runs on XCD g % 8; the row
+// blocks are cut into 8 contiguous ranges and XCD x sweeps range x, so each
+// XCD's L2 sees one contiguous window of the x-gather.
+// ---------------------------------------------------------------------------
+struct BlockWalk {
+  int first, end, step;
+  __device__ __forceinline__ BlockWalk(int nblk) {
+    const int G = gridDim.x, g = blockIdx.x;
+    if (G >= kXcds && (G % kXcds) == 0) {
+      const int x = g % kXcds, l = g / kXcds, per = G / kXcds;
+      first = (int)(((long long)nblk * x) / kXcds) + l;
+      end = (int)(((long long)nblk * (x + 1)) / kXcds);
+      step = per;
+    } else {
+      first = g;
+      end = nblk;
+      step = G;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Staging of one row block (rows r0..r0+nr-1, nr <= 256) into LDS with
+// coalesced nontemporal loads: the CSR streams are read once and never reused,
+// so they should not displace the x window from L2.
+// ---------------------------------------------------------------------------
+template <int CAP>
+struct Stage {
+  double val[CAP + 2];
+  int col[CAP + 4];
+  int row[kRowsPerBlock + 1];
+  int s0v, s0c;  // LDS origin of val / col (aligned down)
+};
+
+template <int CAP, bool VEC, typename T>
+__device__ __forceinline__ void stage_rows(const int* __restrict__ row, const int* __restrict__ col,
+                                           const T* __restrict__ val, int r0, int nr,
+                                           Stage<CAP>& sm) {
+  const int tid = threadIdx.x;
+  for (int t = tid; t <= nr; t += kBlock) sm.row[t] = row[r0 + t];
+  __syncthreads();
+  const int s = sm.row[0], e = sm.row[nr];
+  if (e - s > CAP) return;  // long row: handled from global memory
+  if (VEC) {
+    const int s0v = s & ~1, s0c = s & ~3;
+    const int nv = (e - s0v + 1) >> 1;
+    for (int t = tid; t < nv; t += kBlock) {
+      const int j = s0v + 2 * t;
+      if (j + 1 < e) {
+        dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(val + j));
+        sm.val[2 * t] = v.x;
+        sm.val[2 * t + 1] = v.y;
+      } else {
+        sm.val[2 * t] = val[j];
+      }
+    }
+    const int nc = (e - s0c + 3) >> 2;
+    for (int t = tid; t < nc; t += kBlock) {
+      const int j = s0c + 4 * t;
+      if (j + 3 < e) {
+        int4v c = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(col + j));
+        sm.col[4 * t] = c.x;
+        sm.col[4 * t + 1] = c.y;
+        sm.col[4 * t + 2] = c.z;
+        sm.col[4 * t + 3] = c.w;
+      } else {
+        for (int u = 0; u < 4 && j + u < e; ++u) sm.col[4 * t + u] = col[j + u];
+      }
+    }
+    if (tid == 0) { sm.s0v = s0v; sm.s0c = s0c; }
+  } else {
+    for (int t = tid; t < e - s; t += kBlock) {
+      sm.val[t] = val[s + t];
+      sm.col[t] = col[s + t];
+    }
+    if (tid == 0) { sm.s0v = s; sm.s0c = s; }
+  }
+  __syncthreads();
+}
+
+// Row sum from LDS in CSR order: acc = init (+|-) sum_j (a_j*fn) * x_{c_j}.
+// Gathers are issued eight at a time ahead of the ordered accumulation.
+// Returns the diagonal-slot value x_i (first entry of the row) in *xi and the
+// scaled diagonal in *di.
+template <int CAP, bool SUB>
+__device__ __forceinline__ double row_sum(const Stage<CAP>& sm, int t, double fn, double init,
+                                          const double* __restrict__ x, double* xi, double* di) {
+  const int js = sm.row[t], je = sm.row[t + 1];
+  const int ov = sm.s0v, oc = sm.s0c;
+  double acc = init;
+  for (int j0 = js; j0 < je; j0 += 8) {
+    double av[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = j0 + u;
+      if (j < je) {
+        av[u] = sm.val[j - ov] * fn;
+        xv[u] = x[sm.col[j - oc]];
+      }
+    }
+    if (j0 == js) { *xi = xv[0]; *di = av[0]; }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (j0 + u < je) acc = SUB ? acc - av[u] * xv[u] : acc + av[u] * xv[u];
+    }
+  }
+  return acc;
+}
+
+// A row longer than the stage: the workgroup sums it from global memory
+// (strided partial sums, fixed-order combination).  Only rows with more than
+// CAP nonzeros come here; none of the grids T-Flows builds have them.
+template <bool SUB>
+__device__ double long_row(const int* __restrict__ col, const double* __restrict__ val, int s,
+                           int e, double fn, double init, const double* __restrict__ x,
+                           double* s_tmp, double* s_out) {
+  double part[1] = {0.0};
+  for (int j = s + threadIdx.x; j < e; j += kBlock) part[0] = part[0] + (val[j] * fn) * x[col[j]];
+  const int ops[1] = {OP_SUM};
+  block_reduce<1>(part, ops, s_tmp, s_out);
+  return SUB ? init - s_out[0] : init + s_out[0];
+}
+
+// ---------------------------------------------------------------------------
+// Phase kernels
+// ---------------------------------------------------------------------------
+
+// Stopping measure: Normalized_Root_Mean_Square.f90:33-61 (sqrt of the
+// diagonal-normalised sum over the x range or over `norm`), or the relative
+// residual ||r||_2 / ||b||_2.
+__device__ __forceinline__ double measure(const State* st, double nrms, double xmin, double xmax,
+                                          double rr) {
+  if (st->criterion == TFG_CRIT_RELRES) return sqrt(rr) / st->bnorm;
+  const double range = st->has_norm ? fmax(st->norm - 0.0, kTiny) : fmax(xmax - xmin, kTiny);
+  return sqrt(nrms) / range;
+}
+
+__device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// sum_i a_ii (partials).                                        Cg.f90:62-73
+__global__ __launch_bounds__(kBlock) void k_diag_sum(CgArgs a) {
+  __shared__ double s_tmp[4], s_out[1];
+  double v[1] = {0.0};
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) v[0] = v[0] + a.val[a.row[i]];
+  const int ops[1] = {OP_SUM};
+  publish<1>(v, ops, a.pt.diag, a.pt.pstride, s_tmp, s_out);
+}
+
+// fn = 1/(sum_a/ni) (Cg.f90:75-76), then the initial residual
+// r = b*fn - (A*fn) x (Residual_Vector.f90:34-42 on the scaled system), the
+// scaled diagonal, and the sums for NRMS(b), NRMS(r), the x range, ||b||,
+// ||r|| and rho_1 = (z, r).
+template <int CAP, bool VEC, int PREC>
+__global__ __launch_bounds__(kBlock) void k_prologue(CgArgs a) {
+  __shared__ Stage<CAP> sm;
+  __shared__ double s_tmp[7 * 4], s_out[7];
+  {
+    const int ops1[1] = {OP_SUM};
+    gather<1>(a.pt.diag, a.pt.pstride, a.pt.g_vec, ops1, s_tmp, s_out);
+  }
+  const double sum_a = s_out[0] / (double)a.n;
+  const double fn = 1.0 / sum_a;
+  if (lead()) a.st->fn = fn;
+  double v[7] = {0.0, 0.0, kHuge, -kHuge, 0.0, 0.0, 0.0};
+  auto acc = [&](double bi, double ri, double di, double xi) {
+    const double z = PREC == PREC_JACOBI ? ri * (1.0 / di) : ri;
+    v[0] = v[0] + (bi * bi) / (di * di);
+    v[1] = v[1] + (ri * ri) / (di * di);
+    v[2] = fmin(v[2], xi);
+    v[3] = fmax(v[3], xi);
+    v[4] = v[4] + bi * bi;
+    v[5] = v[5] + ri * ri;
+    v[6] = v[6] + z * ri;
+  };
+  __syncthreads();
+  const BlockWalk w(a.nblk);
+  for (int lb = w.first; lb < w.end; lb += w.step) {
+    const int r0 = a.blk[lb], nr = a.blk[lb + 1] - r0;
+    stage_rows<CAP, VEC>(a.row, a.col, a.val, r0, nr, sm);
+    const int s = sm.row[0], e = sm.row[nr];
+    if (e - s > CAP) {
+      const int i = r0;
+      const double bi = a.b[i] * fn;
+      const double ri = long_row<true>(a.col, a.val, s, e, fn, bi, a.x, s_tmp, s_out);
+      if (threadIdx.x == 0) {
+        const double di = a.val[s] * fn;
+        a.r[i] = ri;
+        a.ad[i] = di;
+        acc(bi, ri, di, a.x[i]);
+      }
+    } else if ((int)threadIdx.x < nr) {
+      const int t = threadIdx.x, i = r0 + t;
+      const double bi = a.b[i] * fn;
+      double xi = 0.0, di = 1.0;
+      const double ri = row_sum<CAP, true>(sm, t, fn, bi, a.x, &xi, &di);
+      a.r[i] = ri;
+      a.ad[i] = di;
+      acc(bi, ri, di, xi);
+    }
+    __syncthreads();
+  }
+  const int ops[7] = {OP_SUM, OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM, OP_SUM};
+  publish<7>(v, ops, a.pt.pro, a.pt.pstride, s_tmp, s_out);
+}
+
+// One workgroup: the two early exits of Cg.f90:95-123 and rho_1.
+__global__ __launch_bounds__(kBlock) void k_start(CgArgs a) {
+  __shared__ double s_tmp[7 * 4], s_out[7];
+  const int ops[7] = {OP_SUM, OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM, OP_SUM};
+  gather<7>(a.pt.pro, a.pt.pstride, a.pt.g_spmv, ops, s_tmp, s_out);
+  if (threadIdx.x != 0) return;
+  State* st = a.st;
+  const double range = st->has_norm ? fmax(st->norm - 0.0, kTiny) : fmax(s_out[3] - s_out[2], kTiny);
+  const double bnorm = sqrt(s_out[4]);
+  double bn, res0;
+  if (st->criterion == TFG_CRIT_RELRES) {
+    bn = bnorm > 0.0 ? 1.0 : 0.0;
+    res0 = sqrt(s_out[5]) / bnorm;
+  } else {
+    bn = sqrt(s_out[0]) / range;
+    res0 = sqrt(s_out[1]) / range;
+  }
+  st->bnorm = bnorm;
+  st->iter = 0;
+  st->status = TFG_OK;
+  st->niter = 0;
+  if (bn < st->tol) {                 // Cg.f90:101-104: res stays 0
+    st->res = 0.0;
+    st->done = 1;
+  } else if (res0 < st->tol) {        // Cg.f90:120-123
+    st->res = res0;
+    st->done = 1;
+  } else {
+    st->res = res0;
+    st->rho[1] = s_out[6];
+    st->done = 0;
+  }
+}
+
+// Iteration k, first phase.  For k > 1 it finishes the stopping test of
+// iteration k-1 (Cg.f90:214-222) and beta = rho_k / rho_{k-1}
+// (Cg.f90:165); then p = z + beta p, or p = z when k = 1 (Cg.f90:158-171),
+// with z = r * d_inv (Prec_Solve.f90:44-49) recomputed on the fly.
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
+  __shared__ double s_tmp[5 * 4], s_out[5];
+  State* st = a.st;
+  if (st->done) return;
+  double beta = 0.0;
+  if (k > 1) {
+    const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
+    gather<5>(a.pt.upd, a.pt.pstride, a.pt.g_vec, ops, s_tmp, s_out);
+    const double res = measure(st, s_out[0], s_out[1], s_out[2], s_out[4]);
+    const bool bad = !isfinite(res);
+    if (bad || res < st->tol) {
+      if (lead()) {
+        st->res = res;
+        st->iter = k - 1;
+        st->niter = k - 1;
+        st->status = bad ? TFG_EBREAKDOWN : TFG_OK;
+        st->done = 1;
+      }
+      return;
+    }
+    const double rho = s_out[3];
+    beta = rho / st->rho[(k - 1) & 1];
+    if (lead()) {
+      st->rho[k & 1] = rho;
+      st->res = res;
+      st->iter = k - 1;
+    }
+  }
+  const int stride = gridDim.x * kBlock;
+  const int npair = a.n >> 1;
+  dbl2* p2 = reinterpret_cast<dbl2*>(a.p);
+  const dbl2* r2 = reinterpret_cast<const dbl2*>(a.r);
+  const dbl2* d2 = reinterpret_cast<const dbl2*>(a.ad);
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < npair; t += stride) {
+    const dbl2 rv = r2[t];
+    dbl2 z;
+    if (PREC == PREC_JACOBI) {
+      const dbl2 dv = d2[t];
+      z.x = rv.x * (1.0 / dv.x);
+      z.y = rv.y * (1.0 / dv.y);
+    } else {
+      z = rv;
+    }
+    if (k == 1) {
+      p2[t] = z;
+    } else {
+      const dbl2 pv = p2[t];
+      dbl2 o;
+      o.x = z.x + beta * pv.x;
+      o.y = z.y + beta * pv.y;
+      p2[t] = o;
+    }
+  }
+  if ((a.n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const int i = a.n - 1;
+    const double z = PREC == PREC_JACOBI ? a.r[i] * (1.0 / a.ad[i]) : a.r[i];
+    a.p[i] = (k == 1) ? z : z + beta * a.p[i];
+  }
+}
+
+// q = (A*fn) p and the partials of (p, q)                 Cg.f90:176-195
+template <int CAP, bool VEC>
+__global__ __launch_bounds__(kBlock) void k_spmv_pq(CgArgs a, int k) {
+  __shared__ Stage<CAP> sm;
+  __shared__ double s_tmp[4], s_out[1];
+  (void)k;
+  if (a.st->done) return;
+  const double fn = a.st->fn;
+  double v[1] = {0.0};
+  const BlockWalk w(a.nblk);
+  for (int lb = w.first; lb < w.end; lb += w.step) {
+    const int r0 = a.blk[lb], nr = a.blk[lb + 1] - r0;
+    stage_rows<CAP, VEC>(a.row, a.col, a.val, r0, nr, sm);
+    const int s = sm.row[0], e = sm.row[nr];
+    if (e - s > CAP) {
+      const double qi = long_row<false>(a.col, a.val, s, e, fn, 0.0, a.p, s_tmp, s_out);
+      if (threadIdx.x == 0) {
+        a.q[r0] = qi;
+        v[0] = v[0] + a.p[r0] * qi;
+      }
+    } else if ((int)threadIdx.x < nr) {
+      const int t = threadIdx.x;
+      double pi = 0.0, di = 0.0;
+      const double qi = row_sum<CAP, false>(sm, t, fn, 0.0, a.p, &pi, &di);
+      a.q[r0 + t] = qi;
+      v[0] = v[0] + pi * qi;
+    }
+    __syncthreads();
+  }
+  const int ops[1] = {OP_SUM};
+  publish<1>(v, ops, a.pt.spmv, a.pt.pstride, s_tmp, s_out);
+}
+
+// alpha = rho / (p,q) (Cg.f90:190-198); x += alpha p, r -= alpha q
+// (Cg.f90:204-209); partials of the NRMS sum, the x range, the next (z, r)
+// and ||r||^2 on the new r and x.
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
+  __shared__ double s_tmp[5 * 4], s_out[5];
+  State* st = a.st;
+  if (st->done) return;
+  {
+    const int ops1[1] = {OP_SUM};
+    gather<1>(a.pt.spmv, a.pt.pstride, a.pt.g_spmv, ops1, s_tmp, s_out);
+  }
+  const double pq = s_out[0];
+  if (pq == 0.0 || !isfinite(pq)) {
+    if (lead()) {
+      st->status = TFG_EBREAKDOWN;
+      st->niter = k;
+      st->done = 1;
+    }
+    return;
+  }
+  const double alpha = st->rho[k & 1] / pq;
+  double v[5] = {0.0, kHuge, -kHuge, 0.0, 0.0};
+  auto one = [&](double& xv, double& rv, double pv, double qv, double dv) {
+    xv = xv + alpha * pv;
+    rv = rv - alpha * qv;
+    const double z = PREC == PREC_JACOBI ? rv * (1.0 / dv) : rv;
+    v[0] = v[0] + (rv * rv) / (dv * dv);
+    v[1] = fmin(v[1], xv);
+    v[2] = fmax(v[2], xv);
+    v[3] = v[3] + z * rv;
+    v[4] = v[4] + rv * rv;
+  };
+  const int stride = gridDim.x * kBlock;
+  const int npair = a.n >> 1;
+  dbl2* x2 = reinterpret_cast<dbl2*>(a.x);
+  dbl2* r2 = reinterpret_cast<dbl2*>(a.r);
+  const dbl2* p2 = reinterpret_cast<const dbl2*>(a.p);
+  const dbl2* q2 = reinterpret_cast<const dbl2*>(a.q);
+  const dbl2* d2 = reinterpret_cast<const dbl2*>(a.ad);
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < npair; t += stride) {
+    const dbl2 xv = x2[t], rv = r2[t];
+    const dbl2 pv = p2[t], qv = q2[t], dv = d2[t];
+    double x0 = xv.x, x1 = xv.y, r0 = rv.x, r1 = rv.y;
+    one(x0, r0, pv.x, qv.x, dv.x);
+    one(x1, r1, pv.y, qv.y, dv.y);
+    dbl2 xo, ro;
+    xo.x = x0;
+    xo.y = x1;
+    ro.x = r0;
+    ro.y = r1;
+    x2[t] = xo;
+    r2[t] = ro;
+  }
+  if ((a.n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const int i = a.n - 1;
+    double xv = a.x[i], rv = a.r[i];
+    one(xv, rv, a.p[i], a.q[i], a.ad[i]);
+    a.x[i] = xv;
+    a.r[i] = rv;
+  }
+  const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
+  publish<5>(v, ops, a.pt.upd, a.pt.pstride, s_tmp, s_out);
+}
+
+// One workgroup: stopping test of iteration k (Cg.f90:214-220) so that the
+// host-visible state is exact at chunk boundaries and at the end.
+__global__ __launch_bounds__(kBlock) void k_finish(CgArgs a, int k) {
+  __shared__ double s_tmp[5 * 4], s_out[5];
+  State* st = a.st;
+  if (st->done) return;
+  const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
+  gather<5>(a.pt.upd, a.pt.pstride, a.pt.g_vec, ops, s_tmp, s_out);
+  if (threadIdx.x != 0) return;
+  const double res = measure(st, s_out[0], s_out[1], s_out[2], s_out[4]);
+  st->res = res;
+  st->iter = k;
+  if (!isfinite(res)) {
+    st->status = TFG_EBREAKDOWN;
+    st->niter = k;
+    st->done = 1;
+  } else if (res < st->tol) {
+    st->niter = k;
+    st->done = 1;
+  }
+}
+
+// ||b - A x||^2 and ||b||^2 on the unscaled system (FP64), partials.
+template <int CAP, bool VEC>
+__global__ __launch_bounds__(kBlock) void k_true_residual(CgArgs a) {
+  __shared__ Stage<CAP> sm;
+  __shared__ double s_tmp[2 * 4], s_out[2];
+  double v[2] = {0.0, 0.0};
+  const BlockWalk w(a.nblk);
+  for (int lb = w.first; lb < w.end; lb += w.step) {
+    const int r0 = a.blk[lb], nr = a.blk[lb + 1] - r0;
+    stage_rows<CAP, VEC>(a.row, a.col, a.val, r0, nr, sm);
+    const int s = sm.row[0], e = sm.row[nr];
+    if (e - s > CAP) {
+      const double bi = a.b[r0];
+      const double ri = long_row<true>(a.col, a.val, s, e, 1.0, bi, a.x, s_tmp, s_out);
+      if (threadIdx.x == 0) {
+        v[0] = v[0] + ri * ri;
+        v[1] = v[1] + bi * bi;
+      }
+    } else if ((int)threadIdx.x < nr) {
+      const int t = threadIdx.x;
+      const double bi = a.b[r0 + t];
+      double xi, di;
+      const double ri = row_sum<CAP, true>(sm, t, 1.0, bi, a.x, &xi, &di);
+      v[0] = v[0] + ri * ri;
+      v[1] = v[1] + bi * bi;
+    }
+    __syncthreads();
+  }
+  const int ops[2] = {OP_SUM, OP_SUM};
+  publish<2>(v, ops, a.pt.aux, a.pt.pstride, s_tmp, s_out);
+}
+
+// One workgroup: aux0 = ||b - A x|| / ||b||.
+__global__ __launch_bounds__(kBlock) void k_true_finish(CgArgs a) {
+  __shared__ double s_tmp[2 * 4], s_out[2];
+  const int ops[2] = {OP_SUM, OP_SUM};
+  gather<2>(a.pt.aux, a.pt.pstride, a.pt.g_spmv, ops, s_tmp, s_out);
+  if (threadIdx.x == 0)
+    a.st->aux0 = s_out[1] > 0.0 ? sqrt(s_out[0]) / sqrt(s_out[1]) : sqrt(s_out[0]);
+}
+
+// Plain q = A p / r = b - A x without scaling or reductions.
+template <int CAP, bool VEC, bool RESID>
+__global__ __launch_bounds__(kBlock) void k_spmv_plain(int nblk, const int* __restrict__ row,
+                                                       const int* __restrict__ col,
+                                                       const int* __restrict__ blk,
+                                                       const double* __restrict__ val,
+                                                       const double* __restrict__ b,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ y) {
+  __shared__ Stage<CAP> sm;
+  __shared__ double s_tmp[4], s_out[1];
+  const BlockWalk w(nblk);
+  for (int lb = w.first; lb < w.end; lb += w.step) {
+    const int r0 = blk[lb], nr = blk[lb + 1] - r0;
+    stage_rows<CAP, VEC>(row, col, val, r0, nr, sm);
+    const int s = sm.row[0], e = sm.row[nr];
+    if (e - s > CAP) {
+      const double yi = long_row<RESID>(col, val, s, e, 1.0, RESID ? b[r0] : 0.0, x, s_tmp, s_out);
+      if (threadIdx.x == 0) y[r0] = yi;
+    } else if ((int)threadIdx.x < nr) {
+      const int t = threadIdx.x;
+      double xi, di;
+      y[r0 + t] = row_sum<CAP, RESID>(sm, t, 1.0, RESID ? b[r0 + t] : 0.0, x, &xi, &di);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+#define TFG_SPMV_LAUNCH(L, KERNEL, ...)                                                   \
+  do {                                                                                    \
+    if ((L).cap == 2048) {                                                                \
+      if ((L).vec_val)                                                                    \
+        KERNEL<2048, true><<<(L).grid_spmv, kBlock, 0, (L).stream>>>(__VA_ARGS__);        \
+      else                                                                                \
+        KERNEL<2048, false><<<(L).grid_spmv, kBlock, 0, (L).stream>>>(__VA_ARGS__);       \
+    } else {                                                                              \
+      if ((L).vec_val)                                                                    \
+        KERNEL<4096, true><<<(L).grid_spmv, kBlock, 0, (L).stream>>>(__VA_ARGS__);        \
+      else                                                                                \
+        KERNEL<4096, false><<<(L).grid_spmv, kBlock, 0, (L).stream>>>(__VA_ARGS__);       \
+    }                                                                                     \
+  } while (0)
+
+int spmv_blocks_per_cu(int cap) {
+  int nb = 0;
+  hipError_t e = cap == 2048
+                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<2048, true>, kBlock, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<4096, true>, kBlock, 0);
+  if (e != hipSuccess || nb < 1) return 1;
+  return nb;
+}
+
+hipError_t launch_diag_sum(const Launch& L, const CgArgs& a) {
+  k_diag_sum<<<L.grid_vec, kBlock, 0, L.stream>>>(a);
+  return hipGetLastError();
+}
+
+template <int CAP, bool VEC>
+static void prologue_p(const Launch& L, const CgArgs& a, int prec) {
+  if (prec == PREC_JACOBI) k_prologue<CAP, VEC, PREC_JACOBI><<<L.grid_spmv, kBlock, 0, L.stream>>>(a);
+  else k_prologue<CAP, VEC, PREC_NONE><<<L.grid_spmv, kBlock, 0, L.stream>>>(a);
+}
+
+hipError_t launch_prologue(const Launch& L, const CgArgs& a, int prec) {
+  if (L.cap == 2048) {
+    if (L.vec_val) prologue_p<2048, true>(L, a, prec); else prologue_p<2048, false>(L, a, prec);
+  } else {
+    if (L.vec_val) prologue_p<4096, true>(L, a, prec); else prologue_p<4096, false>(L, a, prec);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_start(const Launch& L, const CgArgs& a) {
+  k_start<<<1, kBlock, 0, L.stream>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k) {
+  if (prec == PREC_JACOBI) k_pupdate<PREC_JACOBI><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+  else k_pupdate<PREC_NONE><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k) {
+  TFG_SPMV_LAUNCH(L, k_spmv_pq, a, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k) {
+  if (prec == PREC_JACOBI) k_update<PREC_JACOBI><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+  else k_update<PREC_NONE><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_finish(const Launch& L, const CgArgs& a, int k) {
+  k_finish<<<1, kBlock, 0, L.stream>>>(a, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_true_residual(const Launch& L, const CgArgs& a) {
+  TFG_SPMV_LAUNCH(L, k_true_residual, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_true_finish<<<1, kBlock, 0, L.stream>>>(a);
+  return hipGetLastError();
+}
+
+template <bool RESID>
+static void plain(const Launch& L, int nblk, const int* row, const int* col, const int* blk,
+                  const double* val, const double* b, const double* x, double* y) {
+  if (L.cap == 2048) {
+    if (L.vec_val)
+      k_spmv_plain<2048, true, RESID><<<L.grid_spmv, kBlock, 0, L.stream>>>(nblk, row, col, blk, val, b, x, y);
+    else
+      k_spmv_plain<2048, false, RESID><<<L.grid_spmv, kBlock, 0, L.stream>>>(nblk, row, col, blk, val, b, x, y);
+  } else {
+    if (L.vec_val)
+      k_spmv_plain<4096, true, RESID><<<L.grid_spmv, kBlock, 0, L.stream>>>(nblk, row, col, blk, val, b, x, y);
+    else
+      k_spmv_plain<4096, false, RESID><<<L.grid_spmv, kBlock, 0, L.stream>>>(nblk, row, col, blk, val, b, x, y);
+  }
+}
+
+hipError_t launch_spmv_plain(const Launch& L, int n, const int* row, const int* col,
+                             const int* blk, int nblk, const double* val, const double* p,
+                             double* q) {
+  (void)n;
+  plain<false>(L, nblk, row, col, blk, val, nullptr, p, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual_plain(const Launch& L, int n, const int* row, const int* col,
+                                 const int* blk, int nblk, const double* val, const double* b,
+                                 const double* x, double* r) {
+  (void)n;
+  plain<true>(L, nblk, row, col, blk, val, b, x, r);
+  return hipGetLastError();
+}
+
+}  // namespace tfg

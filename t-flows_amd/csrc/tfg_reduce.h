@@ -1,0 +1,79 @@
+// Deterministic reductions for gfx950 (wave64) without atomics or fences.
+//
+// A producer kernel reduces inside each workgroup (fixed butterfly inside the
+// wavefront, fixed-order combination of the four wavefronts) and stores one
+// partial per workgroup.  The consumer -- the next kernel in stream order,
+// which sees those stores because a kernel boundary separates them -- sweeps
+// the partials in a fixed order in EVERY workgroup that needs the total.  All
+// workgroups therefore compute bit-identical scalars, and the result depends
+// only on the grid shape, never on timing.  (A last-workgroup ticket would
+// need a device-scope release per workgroup, which on this multi-XCD part is
+// an L2 write-back.)
+#pragma once
+
+#include "tfg_internal.h"
+
+namespace tfg {
+
+enum RedOp { OP_SUM = 0, OP_MIN = 1, OP_MAX = 2 };
+
+__device__ __forceinline__ double red_apply(int op, double a, double b) {
+  return op == OP_SUM ? a + b : (op == OP_MIN ? fmin(a, b) : fmax(a, b));
+}
+
+__device__ __forceinline__ double red_identity(int op) {
+  return op == OP_SUM ? 0.0 : (op == OP_MIN ? INFINITY : -INFINITY);
+}
+
+__device__ __forceinline__ double wave_reduce(double v, int op) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = red_apply(op, v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Reduce NV values over the workgroup; on return s_out[k] holds the totals
+// (visible to all threads).  s_tmp needs NV * (kBlock/64) doubles.
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], const int (&ops)[NV],
+                                             double* s_tmp, double* s_out) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double w = wave_reduce(v[k], ops[k]);
+    if (lane == 0) s_tmp[k * (kBlock / 64) + wid] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int k = threadIdx.x;
+    double t = s_tmp[k * (kBlock / 64)];
+    for (int w = 1; w < kBlock / 64; ++w) t = red_apply(ops[k], t, s_tmp[k * (kBlock / 64) + w]);
+    s_out[k] = t;
+  }
+  __syncthreads();
+}
+
+// Producer side: reduce over the workgroup and store the totals at
+// part[k*pstride + blockIdx.x].
+template <int NV>
+__device__ __forceinline__ void publish(double (&v)[NV], const int (&ops)[NV], double* part,
+                                        int pstride, double* s_tmp, double* s_out) {
+  block_reduce<NV>(v, ops, s_tmp, s_out);
+  if (threadIdx.x < NV) part[threadIdx.x * pstride + blockIdx.x] = s_out[threadIdx.x];
+}
+
+// Consumer side: combine the G partials of a previous kernel in a fixed order.
+// Result in s_out[k] for all threads.
+template <int NV>
+__device__ __forceinline__ void gather(const double* __restrict__ part, int pstride, int G,
+                                       const int (&ops)[NV], double* s_tmp, double* s_out) {
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = red_identity(ops[k]);
+  for (int g = threadIdx.x; g < G; g += kBlock) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = red_apply(ops[k], acc[k], part[k * pstride + g]);
+  }
+  block_reduce<NV>(acc, ops, s_tmp, s_out);
+}
+
+}  // namespace tfg

@@ -1,0 +1,219 @@
+"""Host-side mirror of T-Flows' native solver interface over the tflows_gpu C ABI.
+
+The reference call being replaced is
+    Nat % Solve_Native(solver, prec, A, x, b, miter, niter, tol, fin_res, norm)
+(Sources/Process/Cpu/Native_Mod/Solve_Native.f90:2-46).  `Native.solve_native`
+keeps that argument list: it updates `x` in place and returns
+(niter, fin_res).  An unknown solver name raises, as the reference stops with
+Message % Error (Solve_Native.f90:38-44).
+
+There is no CPU fallback: if libtflows_gpu.so is missing or no GPU is
+present, every call raises.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libtflows_gpu.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "tflows_gpu.h")
+
+TFG_OK, TFG_EINVAL, TFG_EHIP, TFG_ENOMEM, TFG_EBREAKDOWN, TFG_ENOTSUP = 0, 1, 2, 3, 4, 5
+FP64, FP32_IR = 0, 1
+CRIT_NRMS, CRIT_RELRES = 0, 1
+PATH_AUTO, PATH_KERNELS, PATH_GRAPH, PATH_PERSISTENT = 0, 1, 2, 3
+
+
+class TFlowsGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tflows_gpu error {code}: {msg}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("criterion", ctypes.c_int),
+                ("path", ctypes.c_int), ("inner_max", ctypes.c_int),
+                ("inner_tol", ctypes.c_double), ("timing", ctypes.c_int),
+                ("true_residual", ctypes.c_int)]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("refinements", ctypes.c_int), ("true_relres", ctypes.c_double),
+                ("solve_ms", ctypes.c_double), ("spmv_ms", ctypes.c_double),
+                ("spmv_samples", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libtflows_gpu.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P, I, D, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
+        L.tfg_matrix_create.argtypes = [I, I, I, P, P, P, ctypes.POINTER(P)]
+        L.tfg_matrix_destroy.argtypes = [P]
+        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P]
+        L.tfg_solve_native.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I,
+                                       P, D, P, P]
+        L.tfg_solve_native_ex.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P,
+                                          I, P, D, P, P, I, P, P]
+        L.tfg_spmv.argtypes = [P, P, P, P]
+        L.tfg_residual.argtypes = [P, P, P, P, P]
+        L.tfg_spmv_bench.argtypes = [P, P, P, P, I, ctypes.POINTER(D)]
+        L.tfg_device_alloc.argtypes = [ctypes.POINTER(P), LL]
+        L.tfg_device_free.argtypes = [P]
+        L.tfg_memcpy_h2d.argtypes = [P, P, LL]
+        L.tfg_memcpy_d2h.argtypes = [P, P, LL]
+        L.tfg_synchronize.argtypes = [P]
+        L.tfg_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def declared_symbols(header=HEADER):
+    """Names of the functions include/tflows_gpu.h declares."""
+    text = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(tfg_\w+)\s*\(", text, re.M)))
+
+
+def _check(rc):
+    if rc != TFG_OK:
+        raise TFlowsGpuError(rc, lib().tfg_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class DeviceArray:
+    """A device buffer owned by the library's allocator."""
+
+    def __init__(self, nbytes_or_array, dtype=np.float64):
+        if isinstance(nbytes_or_array, np.ndarray):
+            a = np.ascontiguousarray(nbytes_or_array)
+            self.dtype, self.shape = a.dtype, a.shape
+            nbytes = a.nbytes
+        else:
+            self.dtype, self.shape = np.dtype(dtype), (int(nbytes_or_array) // np.dtype(dtype).itemsize,)
+            nbytes = int(nbytes_or_array)
+            a = None
+        self.nbytes = max(nbytes, 8)
+        p = ctypes.c_void_p()
+        _check(lib().tfg_device_alloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p
+        if a is not None:
+            self.upload(a)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes <= self.nbytes
+        _check(lib().tfg_memcpy_h2d(self.ptr, _ptr(a), a.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(lib().tfg_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().tfg_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Native:
+    """Native_Type restated for the GPU: the uploaded CSR structure of a grid
+    (Native_Mod.f90:30-56, Create_Native.f90:14-21).  row/col/dia are the
+    reference's 1-based Matrix_Type arrays."""
+
+    def __init__(self, row, col, dia, ni=None):
+        self.row = np.ascontiguousarray(row, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.dia = np.ascontiguousarray(dia, dtype=np.int32)
+        self.nt = len(self.row) - 1
+        self.ni = self.nt if ni is None else int(ni)
+        self.nnz = len(self.col)
+        h = ctypes.c_void_p()
+        _check(lib().tfg_matrix_create(self.ni, self.nt, self.nnz, _ptr(self.row), _ptr(self.col),
+                                       _ptr(self.dia), ctypes.byref(h)))
+        self.h = h
+
+    def info(self):
+        vals = [ctypes.c_int() for _ in range(5)]
+        _check(lib().tfg_matrix_info(self.h, *[ctypes.byref(v) for v in vals]))
+        return dict(zip(["n", "nnz", "spmv_blocks", "spmv_cap", "col_mode"], [v.value for v in vals]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tfg_matrix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- Solve_Native ------------------------------------------------------
+    def solve_native(self, solver, prec, val, x, b, miter, tol, norm=None, options=None,
+                     report=False):
+        """Solve A x = b like Solve_Native.  Host numpy arrays: val (nnz),
+        x (nt; x0 in, solution out), b (nt).  Returns (niter, fin_res) or,
+        with report=True, (niter, fin_res, report_dict)."""
+        for name, a, n in (("val", val, self.nnz), ("x", x, self.nt), ("b", b, self.nt)):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+                    and a.shape == (n,)):
+                raise TypeError(f"{name} must be a contiguous float64 array of length {n}")
+        return self._solve(solver, prec, _ptr(val), _ptr(x), _ptr(b), miter, tol, norm, options,
+                           report, device=False)
+
+    def solve_native_device(self, solver, prec, dval, dx, db, miter, tol, norm=None,
+                            options=None, report=False):
+        """Same with DeviceArray operands (device-resident solve)."""
+        return self._solve(solver, prec, dval.ptr, dx.ptr, db.ptr, miter, tol, norm, options,
+                           report, device=True)
+
+    def _solve(self, solver, prec, pval, px, pb, miter, tol, norm, options, report, device):
+        niter = ctypes.c_int(0)
+        fin = ctypes.c_double(0.0)
+        nv = ctypes.c_double(norm) if norm is not None else None
+        opt = options if options is not None else Options()
+        rep = Report()
+        s, p = solver.encode(), prec.encode()
+        rc = lib().tfg_solve_native_ex(self.h, s, len(s), p, len(p), pval, px, pb, int(miter),
+                                       ctypes.byref(niter), float(tol), ctypes.byref(fin),
+                                       ctypes.byref(nv) if nv is not None else None,
+                                       1 if device else 0, ctypes.byref(opt), ctypes.byref(rep))
+        _check(rc)
+        if report:
+            return niter.value, fin.value, rep.as_dict()
+        return niter.value, fin.value
+
+    # -- building blocks ----------------------------------------------------
+    def spmv(self, dval, dp, dq):
+        _check(lib().tfg_spmv(self.h, dval.ptr, dp.ptr, dq.ptr))
+        _check(lib().tfg_synchronize(self.h))
+
+    def residual(self, dval, db, dx, dr):
+        _check(lib().tfg_residual(self.h, dval.ptr, db.ptr, dx.ptr, dr.ptr))
+        _check(lib().tfg_synchronize(self.h))
+
+    def spmv_bench(self, dval, dp, dq, reps):
+        ms = ctypes.c_double()
+        _check(lib().tfg_spmv_bench(self.h, dval.ptr, dp.ptr, dq.ptr, int(reps), ctypes.byref(ms)))
+        return ms.value
