@@ -67,8 +67,10 @@ class System:
         return len(self.col)
 
     def scipy(self):
+        """scipy CSR copy (scipy may sort the indices of its matrix in place, so
+        it must not share `val` with this system)."""
         import scipy.sparse as sp
-        return sp.csr_matrix((self.val, self.col - 1, self.row - 1), shape=(self.n, self.n))
+        return sp.csr_matrix((self.val.copy(), self.col - 1, self.row - 1), shape=(self.n, self.n))
 
 
 def _take(s, name):
