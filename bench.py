@@ -148,7 +148,8 @@ def main():
                         "relres 1e-8, device-resident",
             "n": s.n, "nnz": s.nnz, "index": "int32", "solver": "cg", "prec": "jacobi",
             "tol": tol, "iterations": its[-1], "true_relres": reps[-1][2]["true_relres"],
-            "spmv_cap": info["spmv_cap"], "spmv_blocks": info["spmv_blocks"],
+            "spmv_engine": G.Native.SPMV_KINDS[nat.info()["spmv_kind"]],
+            "spmv_grid": nat.info()["spmv_grid"],
             "solve_GBps_algorithmic": solve_gbs, "parallelism": "single GPU",
         },
         "roofline": {

@@ -114,9 +114,20 @@ int tfg_memcpy_h2d(void* dst, const void* src, long long bytes);
 int tfg_memcpy_d2h(void* dst, const void* src, long long bytes);
 int tfg_synchronize(tfg_matrix* h);
 
-/* Structure facts chosen at upload (for tests and the autotuner). */
-int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_blocks,
-                    int* spmv_cap, int* col_mode);
+/* SpMV engines: 0 LDS-staged rows (2048 nonzeros per block), 1 the same
+ * with 4096, 2/3/4 sub-wavefront rows with 4/8/16 lanes per row, 5 the
+ * 2048 stage with the next block's loads in flight.  All give bit-identical
+ * products.  The handle picks the fastest by timing them on the first product
+ * or solve (autotuner); tfg_spmv_tune re-runs it on d_val (us[6] receives
+ * each engine's mean launch time), tfg_spmv_select forces an engine (-1: tune
+ * again on next use). */
+int tfg_spmv_tune(tfg_matrix* h, const double* d_val, int* kind, float* us);
+int tfg_spmv_select(tfg_matrix* h, int kind);
+
+/* Facts about the handle: sizes, chosen SpMV engine (-1: not tuned yet), its
+ * grid, and the number of 2048-nonzero row blocks. */
+int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind,
+                    int* spmv_grid, int* stage_blocks);
 
 const char* tfg_last_error(void);
 int tfg_version(void);

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -23,9 +24,11 @@ struct tfg_matrix {
   // structure (0-based)
   int* row = nullptr;
   int* col = nullptr;
-  int* blk = nullptr;
-  int nblk = 0;
-  int cap = 2048;
+  int* blk[2] = {nullptr, nullptr};  // stage row blocks for 2048 / 4096 nonzeros
+  int nblk[2] = {0, 0};
+  int grid[tfg::kKinds] = {};         // resident SpMV workgroups per engine
+  int kind = -1;                      // chosen SpMV engine (-1: tune on first use)
+  float tune_us[tfg::kKinds] = {};
   // FP64 work
   double* val = nullptr;  // host-path copy of val
   double* b = nullptr;    // host-path copy of b
@@ -36,7 +39,6 @@ struct tfg_matrix {
   double* ad = nullptr;
   double* part = nullptr;     // partial sums of every phase (see Partials)
   int pstride = 0;
-  int spmv_per_cu = 1;        // resident SpMV workgroups per CU
   State* st = nullptr;       // device scalars of the solve
   State* st_in = nullptr;    // device scalars of an inner (FP32) solve
   State* hst = nullptr;      // pinned: [0] set-up, [1],[2] polling slots
@@ -113,23 +115,54 @@ void make_blocks(const int* row1, int n, int cap, std::vector<int>& blk) {
   }
 }
 
-int pick_cap(int n, int nnz) {
-  double avg = (double)nnz / (double)n;
-  return avg * tfg::kRowsPerBlock <= 2048.0 * 0.97 ? 2048 : 4096;
+// Rows a workgroup of each engine covers per pass (stage: one block of up to
+// 256 rows; wave: 256 / L rows).
+int rows_per_pass(int kind) {
+  switch (kind) {
+    case tfg::K_WAVE4: return tfg::kBlock / 4;
+    case tfg::K_WAVE8: return tfg::kBlock / 8;
+    case tfg::K_WAVE16: return tfg::kBlock / 16;
+    default: return tfg::kRowsPerBlock;
+  }
 }
 
-tfg::Launch launch_of(const tfg_matrix* h, const void* val) {
+void size_grids(tfg_matrix* h) {
+  for (int k = 0; k < tfg::kKinds; ++k) {
+    const int per_cu = tfg::spmv_blocks_per_cu(k);
+    const int work = (k == tfg::K_STAGE2048 || k == tfg::K_PIPE2048) ? h->nblk[0]
+                     : k == tfg::K_STAGE4096 ? h->nblk[1]
+                                             : (h->nt + rows_per_pass(k) - 1) / rows_per_pass(k);
+    int gs = std::min(work, std::max(1, h->cu_count) * per_cu);
+    if (gs >= tfg::kXcds) gs -= gs % tfg::kXcds;
+    h->grid[k] = std::max(1, gs);
+  }
+}
+
+int grid_vec_of(const tfg_matrix* h) {
+  const int pairs = (h->nt + 1) / 2;
+  const int gv = (pairs + tfg::kBlock - 1) / tfg::kBlock;
+  return std::max(1, std::min(gv, std::max(1, h->cu_count) * 4));
+}
+
+tfg::Launch launch_of(const tfg_matrix* h, const void* val, int kind) {
   tfg::Launch L;
   L.stream = h->stream;
-  L.cap = h->cap;
-  int gs = std::min(h->nblk, std::max(1, h->cu_count) * h->spmv_per_cu);
-  if (gs >= tfg::kXcds) gs -= gs % tfg::kXcds;
-  L.grid_spmv = std::max(1, gs);
-  int pairs = (h->nt + 1) / 2;
-  int gv = (pairs + tfg::kBlock - 1) / tfg::kBlock;
-  L.grid_vec = std::max(1, std::min(gv, std::max(1, h->cu_count) * 4));
-  L.vec_val = ((uintptr_t)val % 16) == 0;
+  L.kind = kind;
+  L.grid_spmv = h->grid[kind];
+  L.grid_vec = grid_vec_of(h);
+  L.vec_val = ((uintptr_t)val % 16) == 0 && h->nnz >= 4;
   return L;
+}
+
+tfg::SpmvShape shape_of(const tfg_matrix* h, int kind) {
+  tfg::SpmvShape m;
+  m.n = h->nt;
+  m.row = h->row;
+  m.col = h->col;
+  const int b = kind == tfg::K_STAGE4096 ? 1 : 0;
+  m.blk = h->blk[b];
+  m.nblk = h->nblk[b];
+  return m;
 }
 
 int alloc_work(tfg_matrix* h) {
@@ -139,9 +172,9 @@ int alloc_work(tfg_matrix* h) {
   HIP_TRY(hipMalloc(&h->p, n * sizeof(double)));
   HIP_TRY(hipMalloc(&h->q, n * sizeof(double)));
   HIP_TRY(hipMalloc(&h->ad, n * sizeof(double)));
-  h->spmv_per_cu = tfg::spmv_blocks_per_cu(h->cap);
-  tfg::Launch L = launch_of(h, nullptr);
-  h->pstride = std::max(L.grid_spmv, L.grid_vec);
+  size_grids(h);
+  h->pstride = grid_vec_of(h);
+  for (int k = 0; k < tfg::kKinds; ++k) h->pstride = std::max(h->pstride, h->grid[k]);
   HIP_TRY(hipMalloc(&h->part, sizeof(double) * kPartRows * (size_t)h->pstride));
   HIP_TRY(hipMemset(h->part, 0, sizeof(double) * kPartRows * (size_t)h->pstride));
   HIP_TRY(hipMalloc(&h->st, sizeof(State)));
@@ -161,7 +194,7 @@ void free_all(tfg_matrix* h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* dev[] = {h->row, h->col, h->blk, h->val, h->b, h->x, h->r, h->p, h->q, h->ad,
+  void* dev[] = {h->row, h->col, h->blk[0], h->blk[1], h->val, h->b, h->x, h->r, h->p, h->q, h->ad,
                  h->part, h->st, h->st_in, h->val32, h->e32, h->r32,
                  h->p32, h->q32, h->ad32};
   for (void* d : dev)
@@ -173,6 +206,35 @@ void free_all(tfg_matrix* h) {
     if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
+}
+
+// Autotuner: times every SpMV engine on this matrix (plain product with the
+// caller's values, 2 warm-up + 5 timed launches each) and keeps the fastest.
+// The choice depends on the sparsity structure only and is made once per
+// handle; TFG_SPMV_KIND=<0..5> in the environment forces an engine.
+int ensure_tuned(tfg_matrix* h, const double* val) {
+  if (h->kind >= 0) return TFG_OK;
+  HIP_TRY(hipMemsetAsync(h->p, 0, sizeof(double) * (size_t)h->nt, h->stream));
+  float best = 1e30f;
+  int best_k = tfg::K_STAGE2048;
+  for (int k = 0; k < tfg::kKinds; ++k) {
+    tfg::Launch L = launch_of(h, val, k);
+    const tfg::SpmvShape m = shape_of(h, k);
+    for (int w = 0; w < 2; ++w) HIP_TRY(tfg::launch_spmv_plain(L, m, val, h->p, h->q));
+    HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
+    for (int w = 0; w < 5; ++w) HIP_TRY(tfg::launch_spmv_plain(L, m, val, h->p, h->q));
+    HIP_TRY(hipEventRecord(h->ev_t1, h->stream));
+    HIP_TRY(hipEventSynchronize(h->ev_t1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev_t0, h->ev_t1));
+    h->tune_us[k] = ms * 1000.f / 5.f;
+    if (h->tune_us[k] < best) {
+      best = h->tune_us[k];
+      best_k = k;
+    }
+  }
+  h->kind = best_k;
+  return TFG_OK;
 }
 
 }  // namespace
@@ -226,23 +288,31 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
     if (e != hipSuccess) { rc = fail(TFG_EHIP, "stream: %s", hipGetErrorString(e)); break; }
 
     std::vector<int> tmp((size_t)std::max(nnz, nt + 1));
-    h->cap = pick_cap(nt, nnz);
-    std::vector<int> blk;
-    make_blocks(row, nt, h->cap, blk);
-    // make_blocks works on 1-based starts; differences are what matter.
-    h->nblk = (int)blk.size() - 1;
-
+    // Row blocks of the two stage engines (make_blocks reads 1-based starts;
+    // only differences matter).
+    std::vector<int> blk[2];
+    make_blocks(row, nt, 2048, blk[0]);
+    make_blocks(row, nt, 4096, blk[1]);
     e = hipMalloc(&h->row, sizeof(int) * (size_t)(nt + 1));
     if (e == hipSuccess) e = hipMalloc(&h->col, sizeof(int) * (size_t)nnz);
-    if (e == hipSuccess) e = hipMalloc(&h->blk, sizeof(int) * blk.size());
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+      h->nblk[b] = (int)blk[b].size() - 1;
+      e = hipMalloc(&h->blk[b], sizeof(int) * blk[b].size());
+    }
     if (e != hipSuccess) { rc = fail(TFG_ENOMEM, "structure allocation: %s", hipGetErrorString(e)); break; }
     for (int i = 0; i <= nt; ++i) tmp[i] = row[i] - 1;
     e = hipMemcpy(h->row, tmp.data(), sizeof(int) * (size_t)(nt + 1), hipMemcpyHostToDevice);
     for (long long j = 0; j < nnz; ++j) tmp[j] = col[j] - 1;
     if (e == hipSuccess) e = hipMemcpy(h->col, tmp.data(), sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(h->blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice);
+    for (int b = 0; b < 2 && e == hipSuccess; ++b)
+      e = hipMemcpy(h->blk[b], blk[b].data(), sizeof(int) * blk[b].size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) { rc = fail(TFG_EHIP, "structure upload: %s", hipGetErrorString(e)); break; }
     rc = alloc_work(h);
+    const char* force = getenv("TFG_SPMV_KIND");
+    if (rc == TFG_OK && force && *force) {
+      const int k = atoi(force);
+      if (k >= 0 && k < tfg::kKinds) h->kind = k;
+    }
   } while (0);
   if (rc != TFG_OK) {
     free_all(h);
@@ -258,14 +328,33 @@ int tfg_matrix_destroy(tfg_matrix* h) {
   return TFG_OK;
 }
 
-int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_blocks, int* spmv_cap,
-                    int* col_mode) {
+int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind, int* spmv_grid,
+                    int* stage_blocks) {
   if (!h) return fail(TFG_EINVAL, "null handle");
   if (n) *n = h->nt;
   if (nnz) *nnz = h->nnz;
-  if (spmv_blocks) *spmv_blocks = h->nblk;
-  if (spmv_cap) *spmv_cap = h->cap;
-  if (col_mode) *col_mode = 32;
+  if (spmv_kind) *spmv_kind = h->kind;
+  if (spmv_grid) *spmv_grid = h->kind >= 0 ? h->grid[h->kind] : 0;
+  if (stage_blocks) *stage_blocks = h->nblk[0];
+  return TFG_OK;
+}
+
+int tfg_spmv_select(tfg_matrix* h, int kind) {
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  if (kind < -1 || kind >= tfg::kKinds) return fail(TFG_EINVAL, "unknown SpMV engine %d", kind);
+  h->kind = kind;
+  return TFG_OK;
+}
+
+int tfg_spmv_tune(tfg_matrix* h, const double* d_val, int* kind, float* us) {
+  if (!h || !d_val) return fail(TFG_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  h->kind = -1;
+  int rc = ensure_tuned(h, d_val);
+  if (rc != TFG_OK) return rc;
+  if (kind) *kind = h->kind;
+  if (us)
+    for (int k = 0; k < tfg::kKinds; ++k) us[k] = h->tune_us[k];
   return TFG_OK;
 }
 
@@ -325,13 +414,15 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     rc = tfg::solve_ir(h, pr, dval, db, miter, niter, tol, fin_res, norm, o, rep);
     if (rc != TFG_OK && rc != TFG_EBREAKDOWN) return rc;
   } else {
-    tfg::Launch L = launch_of(h, dval);
+    { const int rt = ensure_tuned(h, dval); if (rt != TFG_OK) return rt; }
+    tfg::Launch L = launch_of(h, dval, h->kind);
+    const tfg::SpmvShape m = shape_of(h, h->kind);
     tfg::CgArgs a{};
     a.n = h->ni;
     a.row = h->row;
     a.col = h->col;
-    a.blk = h->blk;
-    a.nblk = h->nblk;
+    a.blk = m.blk;
+    a.nblk = m.nblk;
     a.val = dval;
     a.b = db;
     a.x = h->x;
@@ -437,8 +528,9 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
 int tfg_spmv(tfg_matrix* h, const double* d_val, const double* d_p, double* d_q) {
   if (!h || !d_val || !d_p || !d_q) return fail(TFG_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
-  tfg::Launch L = launch_of(h, d_val);
-  HIP_TRY(tfg::launch_spmv_plain(L, h->nt, h->row, h->col, h->blk, h->nblk, d_val, d_p, d_q));
+  { const int rt = ensure_tuned(h, d_val); if (rt != TFG_OK) return rt; }
+  tfg::Launch L = launch_of(h, d_val, h->kind);
+  HIP_TRY(tfg::launch_spmv_plain(L, shape_of(h, h->kind), d_val, d_p, d_q));
   return TFG_OK;
 }
 
@@ -446,9 +538,9 @@ int tfg_residual(tfg_matrix* h, const double* d_val, const double* d_b, const do
                  double* d_r) {
   if (!h || !d_val || !d_b || !d_x || !d_r) return fail(TFG_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
-  tfg::Launch L = launch_of(h, d_val);
-  HIP_TRY(tfg::launch_residual_plain(L, h->nt, h->row, h->col, h->blk, h->nblk, d_val, d_b,
-                                     d_x, d_r));
+  { const int rt = ensure_tuned(h, d_val); if (rt != TFG_OK) return rt; }
+  tfg::Launch L = launch_of(h, d_val, h->kind);
+  HIP_TRY(tfg::launch_residual_plain(L, shape_of(h, h->kind), d_val, d_b, d_x, d_r));
   return TFG_OK;
 }
 
@@ -456,10 +548,11 @@ int tfg_spmv_bench(tfg_matrix* h, const double* d_val, const double* d_p, double
                    double* ms_avg) {
   if (!h || !ms_avg || reps < 1) return fail(TFG_EINVAL, "bad argument");
   HIP_TRY(hipSetDevice(h->device));
-  tfg::Launch L = launch_of(h, d_val);
+  { const int rt = ensure_tuned(h, d_val); if (rt != TFG_OK) return rt; }
+  tfg::Launch L = launch_of(h, d_val, h->kind);
+  const tfg::SpmvShape m = shape_of(h, h->kind);
   HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
-  for (int i = 0; i < reps; ++i)
-    HIP_TRY(tfg::launch_spmv_plain(L, h->nt, h->row, h->col, h->blk, h->nblk, d_val, d_p, d_q));
+  for (int i = 0; i < reps; ++i) HIP_TRY(tfg::launch_spmv_plain(L, m, d_val, d_p, d_q));
   HIP_TRY(hipEventRecord(h->ev_t1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev_t1));
   float ms = 0.f;

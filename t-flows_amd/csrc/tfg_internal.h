@@ -68,17 +68,30 @@ struct CgArgs {
   State* st;
 };
 
+// CSR structure as the row engines see it.
+struct SpmvShape {
+  int n;
+  const int* __restrict__ row;   // 0-based, n+1
+  const int* __restrict__ col;   // 0-based
+  const int* __restrict__ blk;   // row-block boundaries (stage engines)
+  int nblk;
+};
+
+// SpMV row engines (see tfg_spmv.h).
+enum SpmvKind { K_STAGE2048 = 0, K_STAGE4096 = 1, K_WAVE4 = 2, K_WAVE8 = 3, K_WAVE16 = 4,
+                K_PIPE2048 = 5, kKinds = 6 };
+
 // Launch shapes chosen per matrix.
 struct Launch {
   hipStream_t stream;
   int grid_vec;    // workgroups of the element-wise kernels
   int grid_spmv;   // workgroups of the SpMV kernels (resident, multiple of kXcds)
-  int cap;         // staged nonzeros per SpMV block: 2048 | 4096
+  int kind;        // SpmvKind
   bool vec_val;    // val is 16-byte aligned (wide nontemporal loads)
 };
 
-// Resident SpMV workgroups per CU for a stage capacity (occupancy query).
-int spmv_blocks_per_cu(int cap);
+// Resident SpMV workgroups per CU for an engine (occupancy query).
+int spmv_blocks_per_cu(int kind);
 
 hipError_t launch_diag_sum(const Launch& L, const CgArgs& a);
 hipError_t launch_prologue(const Launch& L, const CgArgs& a, int prec);
@@ -89,11 +102,9 @@ hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k);
 hipError_t launch_finish(const Launch& L, const CgArgs& a, int k);
 hipError_t launch_true_residual(const Launch& L, const CgArgs& a);
 
-hipError_t launch_spmv_plain(const Launch& L, int n, const int* row, const int* col,
-                             const int* blk, int nblk, const double* val,
+hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* val,
                              const double* p, double* q);
-hipError_t launch_residual_plain(const Launch& L, int n, const int* row, const int* col,
-                                 const int* blk, int nblk, const double* val,
+hipError_t launch_residual_plain(const Launch& L, const SpmvShape& m, const double* val,
                                  const double* b, const double* x, double* r);
 
 }  // namespace tfg

@@ -19,139 +19,12 @@
 // contraction, so each SpMV row and each element-wise update is bit-identical
 // to the reference's serial arithmetic; only the order of the global dot
 // products differs (deterministic trees here).
-#include "tfg_reduce.h"
+#include "tfg_spmv.h"
 
 namespace tfg {
 
-typedef double dbl2 __attribute__((ext_vector_type(2)));
-typedef int int4v __attribute__((ext_vector_type(4)));
-
 // ---------------------------------------------------------------------------
-// Workgroup -> row-block mapping.  Workgroup g runs on XCD g % 8; the row
-// blocks are cut into 8 contiguous ranges and XCD x sweeps range x, so each
-// XCD's L2 sees one contiguous window of the x-gather.
-// ---------------------------------------------------------------------------
-struct BlockWalk {
-  int first, end, step;
-  __device__ __forceinline__ BlockWalk(int nblk) {
-    const int G = gridDim.x, g = blockIdx.x;
-    if (G >= kXcds && (G % kXcds) == 0) {
-      const int x = g % kXcds, l = g / kXcds, per = G / kXcds;
-      first = (int)(((long long)nblk * x) / kXcds) + l;
-      end = (int)(((long long)nblk * (x + 1)) / kXcds);
-      step = per;
-    } else {
-      first = g;
-      end = nblk;
-      step = G;
-    }
-  }
-};
-
-// ---------------------------------------------------------------------------
-// Staging of one row block (rows r0..r0+nr-1, nr <= 256) into LDS with
-// coalesced nontemporal loads: the CSR streams are read once and never reused,
-// so they should not displace the x window from L2.
-// ---------------------------------------------------------------------------
-template <int CAP>
-struct Stage {
-  double val[CAP + 2];
-  int col[CAP + 4];
-  int row[kRowsPerBlock + 1];
-  int s0v, s0c;  // LDS origin of val / col (aligned down)
-};
-
-template <int CAP, bool VEC, typename T>
-__device__ __forceinline__ void stage_rows(const int* __restrict__ row, const int* __restrict__ col,
-                                           const T* __restrict__ val, int r0, int nr,
-                                           Stage<CAP>& sm) {
-  const int tid = threadIdx.x;
-  for (int t = tid; t <= nr; t += kBlock) sm.row[t] = row[r0 + t];
-  __syncthreads();
-  const int s = sm.row[0], e = sm.row[nr];
-  if (e - s > CAP) return;  // long row: handled from global memory
-  if (VEC) {
-    const int s0v = s & ~1, s0c = s & ~3;
-    const int nv = (e - s0v + 1) >> 1;
-    for (int t = tid; t < nv; t += kBlock) {
-      const int j = s0v + 2 * t;
-      if (j + 1 < e) {
-        dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(val + j));
-        sm.val[2 * t] = v.x;
-        sm.val[2 * t + 1] = v.y;
-      } else {
-        sm.val[2 * t] = val[j];
-      }
-    }
-    const int nc = (e - s0c + 3) >> 2;
-    for (int t = tid; t < nc; t += kBlock) {
-      const int j = s0c + 4 * t;
-      if (j + 3 < e) {
-        int4v c = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(col + j));
-        sm.col[4 * t] = c.x;
-        sm.col[4 * t + 1] = c.y;
-        sm.col[4 * t + 2] = c.z;
-        sm.col[4 * t + 3] = c.w;
-      } else {
-        for (int u = 0; u < 4 && j + u < e; ++u) sm.col[4 * t + u] = col[j + u];
-      }
-    }
-    if (tid == 0) { sm.s0v = s0v; sm.s0c = s0c; }
-  } else {
-    for (int t = tid; t < e - s; t += kBlock) {
-      sm.val[t] = val[s + t];
-      sm.col[t] = col[s + t];
-    }
-    if (tid == 0) { sm.s0v = s; sm.s0c = s; }
-  }
-  __syncthreads();
-}
-
-// Row sum from LDS in CSR order: acc = init (+|-) sum_j (a_j*fn) * x_{c_j}.
-// Gathers are issued eight at a time ahead of the ordered accumulation.
-// Returns the diagonal-slot value x_i (first entry of the row) in *xi and the
-// scaled diagonal in *di.
-template <int CAP, bool SUB>
-__device__ __forceinline__ double row_sum(const Stage<CAP>& sm, int t, double fn, double init,
-                                          const double* __restrict__ x, double* xi, double* di) {
-  const int js = sm.row[t], je = sm.row[t + 1];
-  const int ov = sm.s0v, oc = sm.s0c;
-  double acc = init;
-  for (int j0 = js; j0 < je; j0 += 8) {
-    double av[8], xv[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int j = j0 + u;
-      if (j < je) {
-        av[u] = sm.val[j - ov] * fn;
-        xv[u] = x[sm.col[j - oc]];
-      }
-    }
-    if (j0 == js) { *xi = xv[0]; *di = av[0]; }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (j0 + u < je) acc = SUB ? acc - av[u] * xv[u] : acc + av[u] * xv[u];
-    }
-  }
-  return acc;
-}
-
-// A row longer than the stage: the workgroup sums it from global memory
-// (strided partial sums, fixed-order combination).  Only rows with more than
-// CAP nonzeros come here; none of the grids T-Flows builds have them.
-template <bool SUB>
-__device__ double long_row(const int* __restrict__ col, const double* __restrict__ val, int s,
-                           int e, double fn, double init, const double* __restrict__ x,
-                           double* s_tmp, double* s_out) {
-  double part[1] = {0.0};
-  for (int j = s + threadIdx.x; j < e; j += kBlock) part[0] = part[0] + (val[j] * fn) * x[col[j]];
-  const int ops[1] = {OP_SUM};
-  block_reduce<1>(part, ops, s_tmp, s_out);
-  return SUB ? init - s_out[0] : init + s_out[0];
-}
-
-// ---------------------------------------------------------------------------
-// Phase kernels
+// Small helpers
 // ---------------------------------------------------------------------------
 
 // Stopping measure: Normalized_Root_Mean_Square.f90:33-61 (sqrt of the
@@ -165,6 +38,20 @@ __device__ __forceinline__ double measure(const State* st, double nrms, double x
 }
 
 __device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+__device__ __forceinline__ SpmvShape shape_of(const CgArgs& a) {
+  SpmvShape m;
+  m.n = a.n;
+  m.row = a.row;
+  m.col = a.col;
+  m.blk = a.blk;
+  m.nblk = a.nblk;
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Phase kernels
+// ---------------------------------------------------------------------------
 
 // sum_i a_ii (partials).                                        Cg.f90:62-73
 __global__ __launch_bounds__(kBlock) void k_diag_sum(CgArgs a) {
@@ -180,9 +67,9 @@ __global__ __launch_bounds__(kBlock) void k_diag_sum(CgArgs a) {
 // r = b*fn - (A*fn) x (Residual_Vector.f90:34-42 on the scaled system), the
 // scaled diagonal, and the sums for NRMS(b), NRMS(r), the x range, ||b||,
 // ||r|| and rho_1 = (z, r).
-template <int CAP, bool VEC, int PREC>
+template <class E, int PREC>
 __global__ __launch_bounds__(kBlock) void k_prologue(CgArgs a) {
-  __shared__ Stage<CAP> sm;
+  __shared__ typename E::Smem sm;
   __shared__ double s_tmp[7 * 4], s_out[7];
   {
     const int ops1[1] = {OP_SUM};
@@ -191,44 +78,23 @@ __global__ __launch_bounds__(kBlock) void k_prologue(CgArgs a) {
   const double sum_a = s_out[0] / (double)a.n;
   const double fn = 1.0 / sum_a;
   if (lead()) a.st->fn = fn;
-  double v[7] = {0.0, 0.0, kHuge, -kHuge, 0.0, 0.0, 0.0};
-  auto acc = [&](double bi, double ri, double di, double xi) {
-    const double z = PREC == PREC_JACOBI ? ri * (1.0 / di) : ri;
-    v[0] = v[0] + (bi * bi) / (di * di);
-    v[1] = v[1] + (ri * ri) / (di * di);
-    v[2] = fmin(v[2], xi);
-    v[3] = fmax(v[3], xi);
-    v[4] = v[4] + bi * bi;
-    v[5] = v[5] + ri * ri;
-    v[6] = v[6] + z * ri;
-  };
   __syncthreads();
-  const BlockWalk w(a.nblk);
-  for (int lb = w.first; lb < w.end; lb += w.step) {
-    const int r0 = a.blk[lb], nr = a.blk[lb + 1] - r0;
-    stage_rows<CAP, VEC>(a.row, a.col, a.val, r0, nr, sm);
-    const int s = sm.row[0], e = sm.row[nr];
-    if (e - s > CAP) {
-      const int i = r0;
-      const double bi = a.b[i] * fn;
-      const double ri = long_row<true>(a.col, a.val, s, e, fn, bi, a.x, s_tmp, s_out);
-      if (threadIdx.x == 0) {
-        const double di = a.val[s] * fn;
+  double v[7] = {0.0, 0.0, kHuge, -kHuge, 0.0, 0.0, 0.0};
+  E::template run<true>(
+      sm, shape_of(a), a.val, fn, a.x, s_tmp, s_out, [&](int i) { return a.b[i] * fn; },
+      [&](int i, double ri, double xi, double di) {
+        const double bi = a.b[i] * fn;
+        const double z = PREC == PREC_JACOBI ? ri * (1.0 / di) : ri;
         a.r[i] = ri;
         a.ad[i] = di;
-        acc(bi, ri, di, a.x[i]);
-      }
-    } else if ((int)threadIdx.x < nr) {
-      const int t = threadIdx.x, i = r0 + t;
-      const double bi = a.b[i] * fn;
-      double xi = 0.0, di = 1.0;
-      const double ri = row_sum<CAP, true>(sm, t, fn, bi, a.x, &xi, &di);
-      a.r[i] = ri;
-      a.ad[i] = di;
-      acc(bi, ri, di, xi);
-    }
-    __syncthreads();
-  }
+        v[0] = v[0] + (bi * bi) / (di * di);
+        v[1] = v[1] + (ri * ri) / (di * di);
+        v[2] = fmin(v[2], xi);
+        v[3] = fmax(v[3], xi);
+        v[4] = v[4] + bi * bi;
+        v[5] = v[5] + ri * ri;
+        v[6] = v[6] + z * ri;
+      });
   const int ops[7] = {OP_SUM, OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM, OP_SUM};
   publish<7>(v, ops, a.pt.pro, a.pt.pstride, s_tmp, s_out);
 }
@@ -333,34 +199,20 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
 }
 
 // q = (A*fn) p and the partials of (p, q)                 Cg.f90:176-195
-template <int CAP, bool VEC>
+template <class E>
 __global__ __launch_bounds__(kBlock) void k_spmv_pq(CgArgs a, int k) {
-  __shared__ Stage<CAP> sm;
+  __shared__ typename E::Smem sm;
   __shared__ double s_tmp[4], s_out[1];
   (void)k;
   if (a.st->done) return;
   const double fn = a.st->fn;
   double v[1] = {0.0};
-  const BlockWalk w(a.nblk);
-  for (int lb = w.first; lb < w.end; lb += w.step) {
-    const int r0 = a.blk[lb], nr = a.blk[lb + 1] - r0;
-    stage_rows<CAP, VEC>(a.row, a.col, a.val, r0, nr, sm);
-    const int s = sm.row[0], e = sm.row[nr];
-    if (e - s > CAP) {
-      const double qi = long_row<false>(a.col, a.val, s, e, fn, 0.0, a.p, s_tmp, s_out);
-      if (threadIdx.x == 0) {
-        a.q[r0] = qi;
-        v[0] = v[0] + a.p[r0] * qi;
-      }
-    } else if ((int)threadIdx.x < nr) {
-      const int t = threadIdx.x;
-      double pi = 0.0, di = 0.0;
-      const double qi = row_sum<CAP, false>(sm, t, fn, 0.0, a.p, &pi, &di);
-      a.q[r0 + t] = qi;
-      v[0] = v[0] + pi * qi;
-    }
-    __syncthreads();
-  }
+  E::template run<false>(
+      sm, shape_of(a), a.val, fn, a.p, s_tmp, s_out, [](int) { return 0.0; },
+      [&](int i, double qi, double pi, double) {
+        a.q[i] = qi;
+        v[0] = v[0] + pi * qi;
+      });
   const int ops[1] = {OP_SUM};
   publish<1>(v, ops, a.pt.spmv, a.pt.pstride, s_tmp, s_out);
 }
@@ -453,33 +305,18 @@ __global__ __launch_bounds__(kBlock) void k_finish(CgArgs a, int k) {
 }
 
 // ||b - A x||^2 and ||b||^2 on the unscaled system (FP64), partials.
-template <int CAP, bool VEC>
+template <class E>
 __global__ __launch_bounds__(kBlock) void k_true_residual(CgArgs a) {
-  __shared__ Stage<CAP> sm;
+  __shared__ typename E::Smem sm;
   __shared__ double s_tmp[2 * 4], s_out[2];
   double v[2] = {0.0, 0.0};
-  const BlockWalk w(a.nblk);
-  for (int lb = w.first; lb < w.end; lb += w.step) {
-    const int r0 = a.blk[lb], nr = a.blk[lb + 1] - r0;
-    stage_rows<CAP, VEC>(a.row, a.col, a.val, r0, nr, sm);
-    const int s = sm.row[0], e = sm.row[nr];
-    if (e - s > CAP) {
-      const double bi = a.b[r0];
-      const double ri = long_row<true>(a.col, a.val, s, e, 1.0, bi, a.x, s_tmp, s_out);
-      if (threadIdx.x == 0) {
+  E::template run<true>(
+      sm, shape_of(a), a.val, 1.0, a.x, s_tmp, s_out, [&](int i) { return a.b[i]; },
+      [&](int i, double ri, double, double) {
+        const double bi = a.b[i];
         v[0] = v[0] + ri * ri;
         v[1] = v[1] + bi * bi;
-      }
-    } else if ((int)threadIdx.x < nr) {
-      const int t = threadIdx.x;
-      const double bi = a.b[r0 + t];
-      double xi, di;
-      const double ri = row_sum<CAP, true>(sm, t, 1.0, bi, a.x, &xi, &di);
-      v[0] = v[0] + ri * ri;
-      v[1] = v[1] + bi * bi;
-    }
-    __syncthreads();
-  }
+      });
   const int ops[2] = {OP_SUM, OP_SUM};
   publish<2>(v, ops, a.pt.aux, a.pt.pstride, s_tmp, s_out);
 }
@@ -493,57 +330,57 @@ __global__ __launch_bounds__(kBlock) void k_true_finish(CgArgs a) {
     a.st->aux0 = s_out[1] > 0.0 ? sqrt(s_out[0]) / sqrt(s_out[1]) : sqrt(s_out[0]);
 }
 
-// Plain q = A p / r = b - A x without scaling or reductions.
-template <int CAP, bool VEC, bool RESID>
-__global__ __launch_bounds__(kBlock) void k_spmv_plain(int nblk, const int* __restrict__ row,
-                                                       const int* __restrict__ col,
-                                                       const int* __restrict__ blk,
-                                                       const double* __restrict__ val,
+// Plain y = A x (RESID: y = b - A x) without scaling or reductions.
+template <class E, bool RESID>
+__global__ __launch_bounds__(kBlock) void k_spmv_plain(SpmvShape m, const double* __restrict__ val,
                                                        const double* __restrict__ b,
                                                        const double* __restrict__ x,
                                                        double* __restrict__ y) {
-  __shared__ Stage<CAP> sm;
+  __shared__ typename E::Smem sm;
   __shared__ double s_tmp[4], s_out[1];
-  const BlockWalk w(nblk);
-  for (int lb = w.first; lb < w.end; lb += w.step) {
-    const int r0 = blk[lb], nr = blk[lb + 1] - r0;
-    stage_rows<CAP, VEC>(row, col, val, r0, nr, sm);
-    const int s = sm.row[0], e = sm.row[nr];
-    if (e - s > CAP) {
-      const double yi = long_row<RESID>(col, val, s, e, 1.0, RESID ? b[r0] : 0.0, x, s_tmp, s_out);
-      if (threadIdx.x == 0) y[r0] = yi;
-    } else if ((int)threadIdx.x < nr) {
-      const int t = threadIdx.x;
-      double xi, di;
-      y[r0 + t] = row_sum<CAP, RESID>(sm, t, 1.0, RESID ? b[r0 + t] : 0.0, x, &xi, &di);
-    }
-    __syncthreads();
-  }
+  E::template run<RESID>(
+      sm, m, val, 1.0, x, s_tmp, s_out, [&](int i) { return RESID ? b[i] : 0.0; },
+      [&](int i, double yi, double, double) { y[i] = yi; });
 }
 
 // ---------------------------------------------------------------------------
-// Launchers
+// Engine dispatch
 // ---------------------------------------------------------------------------
-#define TFG_SPMV_LAUNCH(L, KERNEL, ...)                                                   \
-  do {                                                                                    \
-    if ((L).cap == 2048) {                                                                \
-      if ((L).vec_val)                                                                    \
-        KERNEL<2048, true><<<(L).grid_spmv, kBlock, 0, (L).stream>>>(__VA_ARGS__);        \
-      else                                                                                \
-        KERNEL<2048, false><<<(L).grid_spmv, kBlock, 0, (L).stream>>>(__VA_ARGS__);       \
-    } else {                                                                              \
-      if ((L).vec_val)                                                                    \
-        KERNEL<4096, true><<<(L).grid_spmv, kBlock, 0, (L).stream>>>(__VA_ARGS__);        \
-      else                                                                                \
-        KERNEL<4096, false><<<(L).grid_spmv, kBlock, 0, (L).stream>>>(__VA_ARGS__);       \
-    }                                                                                     \
+// kind: 0 stage/2048, 1 stage/4096, 2 wave/4, 3 wave/8, 4 wave/16,
+// 5 pipelined stage/2048.  The stage engines fall back to scalar staging
+// (ScalarStage) when val is not 16-byte aligned or nnz < 4.
+#define TFG_ENGINE_SWITCH(L, CALL)                                                 \
+  do {                                                                             \
+    switch ((L).kind) {                                                            \
+      case 0:                                                                      \
+        if ((L).vec_val) { using E = StreamEngine<2048, false>; CALL; }            \
+        else { using E = ScalarStage<2048>; CALL; }                                \
+        break;                                                                     \
+      case 1:                                                                      \
+        if ((L).vec_val) { using E = StreamEngine<4096, false>; CALL; }            \
+        else { using E = ScalarStage<4096>; CALL; }                                \
+        break;                                                                     \
+      case 5:                                                                      \
+        if ((L).vec_val) { using E = StreamEngine<2048, true>; CALL; }             \
+        else { using E = ScalarStage<2048>; CALL; }                                \
+        break;                                                                     \
+      case 2: { using E = WaveEngine<4>; CALL; } break;                            \
+      case 3: { using E = WaveEngine<8>; CALL; } break;                            \
+      default: { using E = WaveEngine<16>; CALL; } break;                          \
+    }                                                                              \
   } while (0)
 
-int spmv_blocks_per_cu(int cap) {
+int spmv_blocks_per_cu(int kind) {
   int nb = 0;
-  hipError_t e = cap == 2048
-                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<2048, true>, kBlock, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<4096, true>, kBlock, 0);
+  hipError_t e = hipSuccess;
+  switch (kind) {
+    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, false>>, kBlock, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<4096, false>>, kBlock, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<4>>, kBlock, 0); break;
+    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<8>>, kBlock, 0); break;
+    case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, true>>, kBlock, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<16>>, kBlock, 0); break;
+  }
   if (e != hipSuccess || nb < 1) return 1;
   return nb;
 }
@@ -553,18 +390,11 @@ hipError_t launch_diag_sum(const Launch& L, const CgArgs& a) {
   return hipGetLastError();
 }
 
-template <int CAP, bool VEC>
-static void prologue_p(const Launch& L, const CgArgs& a, int prec) {
-  if (prec == PREC_JACOBI) k_prologue<CAP, VEC, PREC_JACOBI><<<L.grid_spmv, kBlock, 0, L.stream>>>(a);
-  else k_prologue<CAP, VEC, PREC_NONE><<<L.grid_spmv, kBlock, 0, L.stream>>>(a);
-}
-
 hipError_t launch_prologue(const Launch& L, const CgArgs& a, int prec) {
-  if (L.cap == 2048) {
-    if (L.vec_val) prologue_p<2048, true>(L, a, prec); else prologue_p<2048, false>(L, a, prec);
-  } else {
-    if (L.vec_val) prologue_p<4096, true>(L, a, prec); else prologue_p<4096, false>(L, a, prec);
-  }
+  if (prec == PREC_JACOBI)
+    TFG_ENGINE_SWITCH(L, (k_prologue<E, PREC_JACOBI><<<L.grid_spmv, kBlock, 0, L.stream>>>(a)));
+  else
+    TFG_ENGINE_SWITCH(L, (k_prologue<E, PREC_NONE><<<L.grid_spmv, kBlock, 0, L.stream>>>(a)));
   return hipGetLastError();
 }
 
@@ -580,7 +410,7 @@ hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k) {
 }
 
 hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k) {
-  TFG_SPMV_LAUNCH(L, k_spmv_pq, a, k);
+  TFG_ENGINE_SWITCH(L, (k_spmv_pq<E><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k)));
   return hipGetLastError();
 }
 
@@ -596,42 +426,22 @@ hipError_t launch_finish(const Launch& L, const CgArgs& a, int k) {
 }
 
 hipError_t launch_true_residual(const Launch& L, const CgArgs& a) {
-  TFG_SPMV_LAUNCH(L, k_true_residual, a);
+  TFG_ENGINE_SWITCH(L, (k_true_residual<E><<<L.grid_spmv, kBlock, 0, L.stream>>>(a)));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_true_finish<<<1, kBlock, 0, L.stream>>>(a);
   return hipGetLastError();
 }
 
-template <bool RESID>
-static void plain(const Launch& L, int nblk, const int* row, const int* col, const int* blk,
-                  const double* val, const double* b, const double* x, double* y) {
-  if (L.cap == 2048) {
-    if (L.vec_val)
-      k_spmv_plain<2048, true, RESID><<<L.grid_spmv, kBlock, 0, L.stream>>>(nblk, row, col, blk, val, b, x, y);
-    else
-      k_spmv_plain<2048, false, RESID><<<L.grid_spmv, kBlock, 0, L.stream>>>(nblk, row, col, blk, val, b, x, y);
-  } else {
-    if (L.vec_val)
-      k_spmv_plain<4096, true, RESID><<<L.grid_spmv, kBlock, 0, L.stream>>>(nblk, row, col, blk, val, b, x, y);
-    else
-      k_spmv_plain<4096, false, RESID><<<L.grid_spmv, kBlock, 0, L.stream>>>(nblk, row, col, blk, val, b, x, y);
-  }
-}
-
-hipError_t launch_spmv_plain(const Launch& L, int n, const int* row, const int* col,
-                             const int* blk, int nblk, const double* val, const double* p,
-                             double* q) {
-  (void)n;
-  plain<false>(L, nblk, row, col, blk, val, nullptr, p, q);
+hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* val,
+                             const double* p, double* q) {
+  TFG_ENGINE_SWITCH(L, (k_spmv_plain<E, false><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, val, nullptr, p, q)));
   return hipGetLastError();
 }
 
-hipError_t launch_residual_plain(const Launch& L, int n, const int* row, const int* col,
-                                 const int* blk, int nblk, const double* val, const double* b,
-                                 const double* x, double* r) {
-  (void)n;
-  plain<true>(L, nblk, row, col, blk, val, b, x, r);
+hipError_t launch_residual_plain(const Launch& L, const SpmvShape& m, const double* val,
+                                 const double* b, const double* x, double* r) {
+  TFG_ENGINE_SWITCH(L, (k_spmv_plain<E, true><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, val, b, x, r)));
   return hipGetLastError();
 }
 

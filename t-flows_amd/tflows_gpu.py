@@ -70,6 +70,8 @@ def lib():
         L.tfg_spmv.argtypes = [P, P, P, P]
         L.tfg_residual.argtypes = [P, P, P, P, P]
         L.tfg_spmv_bench.argtypes = [P, P, P, P, I, ctypes.POINTER(D)]
+        L.tfg_spmv_tune.argtypes = [P, P, ctypes.POINTER(I), P]
+        L.tfg_spmv_select.argtypes = [P, I]
         L.tfg_device_alloc.argtypes = [ctypes.POINTER(P), LL]
         L.tfg_device_free.argtypes = [P]
         L.tfg_memcpy_h2d.argtypes = [P, P, LL]
@@ -156,7 +158,8 @@ class Native:
     def info(self):
         vals = [ctypes.c_int() for _ in range(5)]
         _check(lib().tfg_matrix_info(self.h, *[ctypes.byref(v) for v in vals]))
-        return dict(zip(["n", "nnz", "spmv_blocks", "spmv_cap", "col_mode"], [v.value for v in vals]))
+        return dict(zip(["n", "nnz", "spmv_kind", "spmv_grid", "stage_blocks"],
+                        [v.value for v in vals]))
 
     def close(self):
         if getattr(self, "h", None):
@@ -212,6 +215,18 @@ class Native:
     def residual(self, dval, db, dx, dr):
         _check(lib().tfg_residual(self.h, dval.ptr, db.ptr, dx.ptr, dr.ptr))
         _check(lib().tfg_synchronize(self.h))
+
+    SPMV_KINDS = ("stage2048", "stage4096", "wave4", "wave8", "wave16", "pipe2048")
+
+    def spmv_tune(self, dval):
+        """Time every SpMV engine on dval; returns (chosen kind, {name: us})."""
+        k = ctypes.c_int()
+        us = (ctypes.c_float * len(self.SPMV_KINDS))()
+        _check(lib().tfg_spmv_tune(self.h, dval.ptr, ctypes.byref(k), us))
+        return k.value, dict(zip(self.SPMV_KINDS, list(us)))
+
+    def spmv_select(self, kind):
+        _check(lib().tfg_spmv_select(self.h, int(kind)))
 
     def spmv_bench(self, dval, dp, dq, reps):
         ms = ctypes.c_double()

@@ -102,6 +102,21 @@ def test_bicg_equals_cg_bitwise(cases, gpu):
     assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2])
 
 
+def test_every_engine_same_solve(cases, oracle, gpu):
+    """Every SpMV engine solves to the same answer.  The products are
+    bit-identical; the engines' grids differ, so the (p,q) partial sums are
+    combined in a different (fixed) order and the runs agree to rounding."""
+    s = cases.knn(10, 10, 10)
+    it_o, _, x_o = _oracle_solve(oracle, s, "cg", "jacobi", 500, 1e-9)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    for kind in range(6):
+        nat.spmv_select(kind)
+        r1 = _gpu_solve(gpu, s, "cg", "jacobi", 500, 1e-9, nat=nat)
+        r2 = _gpu_solve(gpu, s, "cg", "jacobi", 500, 1e-9, nat=nat)
+        assert r1[0] == r2[0] and np.array_equal(r1[2], r2[2]), kind   # deterministic
+        assert abs(r1[0] - it_o) <= 1 and _rel(r1[2], x_o) <= X_TOL, kind
+
+
 def test_deterministic(cases, gpu):
     s = cases.hex_pressure(24, rho_out=1000.0)
     nat = gpu.Native(s.row, s.col, s.dia)

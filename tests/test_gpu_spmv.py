@@ -1,10 +1,11 @@
 """GPU SpMV / residual kernels against the oracle's product loops.
 
 The product loop of Cg.f90:177-185 and Residual_Vector.f90:34-42 sum each row
-in CSR order; the GPU kernels keep that order and round every product and sum
-separately, so rows staged through LDS must agree BIT FOR BIT with the
+in CSR order; every GPU row engine keeps that order and rounds every product
+and sum separately, so all five engines must agree BIT FOR BIT with the
 oracle.  Rows longer than the LDS stage (none in T-Flows grids) are summed
-by a workgroup tree and are checked to 1e-14 relative instead.
+by a workgroup tree in the stage engines and are checked to 1e-14 relative
+there instead.
 """
 import ctypes
 
@@ -27,23 +28,32 @@ def _systems(cases):
             cases.upwind(20), cases.knn(14, 12, 10), cases.hex_pressure(1, 1, 7)]
 
 
+KINDS = range(6)  # stage2048, stage4096, wave4, wave8, wave16, pipe2048
+
+
 @pytest.mark.parametrize("idx", range(5))
-def test_spmv_bitexact(cases, oracle, gpu, idx):
+def test_spmv_bitexact_every_engine(cases, oracle, gpu, idx):
     s = _systems(cases)[idx]
     print(f"spmv case {s.name} n={s.n} nnz={s.nnz}", flush=True)
     rng = np.random.default_rng(idx)
     p = rng.standard_normal(s.n)
     A = oracle.Matrix(s.row, s.col, s.dia, s.val)
     q_ref = oracle.spmv(A, p)
+    r_ref = oracle.residual(A, s.b, p)
     nat = gpu.Native(s.row, s.col, s.dia)
     dv, dp, dq = gpu.DeviceArray(s.val), gpu.DeviceArray(p), gpu.DeviceArray(np.zeros(s.n))
-    nat.spmv(dv, dp, dq)
-    q = dq.download()
-    assert np.array_equal(q, q_ref), np.abs(q - q_ref).max()
-    # residual r = b - A x
     db, dr = gpu.DeviceArray(s.b), gpu.DeviceArray(np.zeros(s.n))
-    nat.residual(dv, db, dp, dr)
-    assert np.array_equal(dr.download(), oracle.residual(A, s.b, p))
+    for kind in KINDS:
+        nat.spmv_select(kind)
+        nat.spmv(dv, dp, dq)
+        q = dq.download()
+        assert np.array_equal(q, q_ref), (kind, np.abs(q - q_ref).max())
+        nat.residual(dv, db, dp, dr)
+        assert np.array_equal(dr.download(), r_ref), kind
+    kind, us = nat.spmv_tune(dv)
+    print(f"  autotuner picked {nat.SPMV_KINDS[kind]}: " +
+          ", ".join(f"{k} {v:.1f} us" for k, v in us.items()), flush=True)
+    assert 0 <= kind < 6 and us[nat.SPMV_KINDS[kind]] == min(us.values())
 
 
 def test_spmv_misaligned_values(cases, oracle, gpu):
@@ -55,8 +65,10 @@ def test_spmv_misaligned_values(cases, oracle, gpu):
     buf = gpu.DeviceArray(np.concatenate([[0.0], s.val]))
     dv = _View(buf, 8)
     dp, dq = gpu.DeviceArray(p), gpu.DeviceArray(np.zeros(s.n))
-    nat.spmv(dv, dp, dq)
-    assert np.array_equal(dq.download(), oracle.spmv(A, p))
+    for kind in KINDS:
+        nat.spmv_select(kind)
+        nat.spmv(dv, dp, dq)
+        assert np.array_equal(dq.download(), oracle.spmv(A, p)), kind
 
 
 def test_spmv_long_rows(oracle, gpu):
@@ -80,7 +92,12 @@ def test_spmv_long_rows(oracle, gpu):
     q_ref = oracle.spmv(A, p)
     nat = gpu.Native(row, col, dia)
     dv, dp, dq = gpu.DeviceArray(val), gpu.DeviceArray(p), gpu.DeviceArray(np.zeros(n))
-    nat.spmv(dv, dp, dq)
-    q = dq.download()
-    assert np.array_equal(q[1:], q_ref[1:])
-    assert abs(q[0] - q_ref[0]) <= 1e-14 * np.abs(val[:n] * p[col[:n] - 1]).sum()
+    for kind in KINDS:
+        nat.spmv_select(kind)
+        nat.spmv(dv, dp, dq)
+        q = dq.download()
+        assert np.array_equal(q[1:], q_ref[1:]), kind
+        if kind in (2, 3, 4):   # the wave engines keep CSR order on long rows too
+            assert q[0] == q_ref[0]
+        else:           # the stage engines sum a row longer than the stage as a tree
+            assert abs(q[0] - q_ref[0]) <= 1e-14 * np.abs(val[:n] * p[col[:n] - 1]).sum()
