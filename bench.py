@@ -33,23 +33,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(s, gpu_iters, seconds_budget=15.0):
-    """Oracle CG (1 thread, FP64) on the same system for a bounded number of
-    iterations; returns the projected time to the GPU's iteration count."""
+def _time_oracle(s, iters, omp):
     from oracle import oracle
 
     A = oracle.Matrix(s.row, s.col, s.dia, s.val)
-    # one short probe to size the sample to the budget
     x = np.zeros(s.n)
     t0 = time.perf_counter()
-    oracle.solve("cg", "jacobi", A, x, s.b.copy(), 2, 1e-30, criterion="relres")
-    probe = (time.perf_counter() - t0) / 2.0
+    oracle.solve("cg", "jacobi", A, x, s.b.copy(), iters, 1e-30, criterion="relres", omp=omp)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(s, gpu_iters, seconds_budget=15.0):
+    """Oracle CG (FP64) on the same system for a bounded number of iterations,
+    1 thread (the reference's default OMP=no build) and all host cores (its
+    OMP=yes analogue); returns the projected time to the GPU's iteration
+    count."""
+    probe = _time_oracle(s, 2, False) / 2.0
     iters = int(max(3, min(200, seconds_budget / max(probe, 1e-6))))
-    x = np.zeros(s.n)
-    t0 = time.perf_counter()
-    oracle.solve("cg", "jacobi", A, x, s.b.copy(), iters, 1e-30, criterion="relres")
-    dt = time.perf_counter() - t0
+    dt = _time_oracle(s, iters, False)
     per_iter = dt / iters
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    probe_omp = _time_oracle(s, 2, True) / 2.0
+    iters_omp = int(max(3, min(400, seconds_budget / 2 / max(probe_omp, 1e-6))))
+    dt_omp = _time_oracle(s, iters_omp, True)
     return {
         "value": per_iter * gpu_iters,
         "unit": "s",
@@ -60,7 +66,39 @@ def cpu_baseline(s, gpu_iters, seconds_budget=15.0):
                    f"and initial residual), {per_iter * 1e3:.1f} ms/iteration scaled to the GPU's "
                    f"{gpu_iters} iterations"),
         "ms_per_iteration": per_iter * 1e3,
+        "all_cores": {
+            "value": dt_omp / iters_omp * gpu_iters, "cores": cores,
+            "sample": f"same oracle built with OpenMP, {iters_omp} iterations in {dt_omp:.1f} s",
+            "ms_per_iteration": dt_omp / iters_omp * 1e3,
+        },
     }
+
+
+TRAFFIC = os.path.join(ROOT, "profiles", "spmv_traffic_twophase200.json")
+
+
+def traffic_per_launch(kernel_prefix):
+    """HBM bytes per launch of the kernel, from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel calibration)."""
+    if not os.path.exists(TRAFFIC):
+        return None, None
+    with open(TRAFFIC) as f:
+        t = json.load(f)
+    k = t["kernels"].get(kernel_prefix)
+    if not k:
+        return None, None
+    return k["fetch_kib"] * 1024 * t["fetch_scale"] + k["write_kib"] * 1024 * t["write_scale"], TRAFFIC
+
+
+def calibrate():
+    """Known-traffic copy (1 GiB read + 1 GiB write) for counter calibration."""
+    import tflows_gpu as G
+
+    nbytes = 1 << 30
+    src, dst = G.DeviceArray(nbytes, np.uint8), G.DeviceArray(nbytes, np.uint8)
+    ms = G.stream_copy(dst, src, nbytes, reps=5)
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                      "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
 def main():
@@ -71,7 +109,12 @@ def main():
     ap.add_argument("--n", type=int, default=200, help="grid side (default 200 -> 8M cells)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run the known-traffic copy kernel (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate:
+        calibrate()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
               "(the solver is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
@@ -125,6 +168,7 @@ def main():
     iter_bytes = spmv_bytes + 88 * s.n
     solve_gbs = iter_bytes * its[-1] / (times[-1]) / 1e9
 
+    traffic, traffic_src = traffic_per_launch("k_spmv_pq")
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline leg (oracle, 1 thread)...")
@@ -158,7 +202,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": os.path.relpath(traffic_src, ROOT) if traffic_src else None,
             "kernel": "k_spmv_pq (FP64 CSR SpMV with fused p.q)",
             "bytes_per_launch": spmv_bytes,
             "launch_us": spmv_ms * 1e3,

@@ -114,6 +114,12 @@ int tfg_memcpy_h2d(void* dst, const void* src, long long bytes);
 int tfg_memcpy_d2h(void* dst, const void* src, long long bytes);
 int tfg_synchronize(tfg_matrix* h);
 
+/* Measurement helper with no counterpart in the reference: a plain 16-byte
+ * streaming copy of `bytes` (multiple of 16, 16-byte aligned device
+ * pointers), launched reps+1 times on the default stream; *ms_avg = mean of
+ * the last reps.  Its known traffic calibrates the HBM byte counters. */
+int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, double* ms_avg);
+
 /* SpMV engines: 0 LDS-staged rows (2048 nonzeros per block), 1 the same
  * with 4096, 2/3/4 sub-wavefront rows with 4/8/16 lanes per row, 5 the
  * 2048 stage with the next block's loads in flight.  All give bit-identical

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
+#include <utility>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -41,7 +43,7 @@ struct tfg_matrix {
   int pstride = 0;
   State* st = nullptr;       // device scalars of the solve
   State* st_in = nullptr;    // device scalars of an inner (FP32) solve
-  State* hst = nullptr;      // pinned: [0] set-up, [1],[2] polling slots
+  State* hst = nullptr;      // pinned: [0] set-up, [1],[2] polling, [3] final, [4] inner
   hipEvent_t ev_poll[2] = {nullptr, nullptr};
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
   std::vector<hipEvent_t> ev_sample;
@@ -181,8 +183,8 @@ int alloc_work(tfg_matrix* h) {
   HIP_TRY(hipMalloc(&h->st_in, sizeof(State)));
   HIP_TRY(hipMemset(h->st, 0, sizeof(State)));
   HIP_TRY(hipMemset(h->st_in, 0, sizeof(State)));
-  HIP_TRY(hipHostMalloc(&h->hst, 4 * sizeof(State), hipHostMallocDefault));
-  memset(h->hst, 0, 4 * sizeof(State));
+  HIP_TRY(hipHostMalloc(&h->hst, 6 * sizeof(State), hipHostMallocDefault));
+  memset(h->hst, 0, 6 * sizeof(State));
   HIP_TRY(hipEventCreate(&h->ev_poll[0]));
   HIP_TRY(hipEventCreate(&h->ev_poll[1]));
   HIP_TRY(hipEventCreate(&h->ev_t0));
@@ -240,11 +242,193 @@ int ensure_tuned(tfg_matrix* h, const double* val) {
 }  // namespace
 
 namespace tfg {
-// Defined in tfg_ir.hip.
+
+// FP32 inner CG inside FP64 iterative refinement (kernels in tfg_ir.hip).
+// niter = total inner iterations (capped by miter); fin_res = the criterion
+// on the last FP64 true residual.
 int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int miter,
              int* niter, double tol, double* fin_res, const double* norm,
-             const tfg_options& o, tfg_report* rep);
+             const tfg_options& o, tfg_report* rep) {
+  const size_t n = (size_t)h->nt;
+  if (!h->val32) HIP_TRY(hipMalloc(&h->val32, sizeof(float) * (size_t)h->nnz));
+  if (!h->ad32) HIP_TRY(hipMalloc(&h->ad32, sizeof(float) * n));
+  if (!h->r32) HIP_TRY(hipMalloc(&h->r32, sizeof(float) * n));
+  if (!h->e32) HIP_TRY(hipMalloc(&h->e32, sizeof(float) * n));
+  if (!h->p32) HIP_TRY(hipMalloc(&h->p32, sizeof(float) * n));
+  if (!h->q32) HIP_TRY(hipMalloc(&h->q32, sizeof(float) * n));
+  { const int rt = ensure_tuned(h, val); if (rt != TFG_OK) return rt; }
+  const tfg::Launch L = launch_of(h, val, h->kind);
+  const tfg::Launch L32 = launch_of(h, h->val32, h->kind);
+  const tfg::SpmvShape m = shape_of(h, h->kind);
+  tfg::CgArgs a{};
+  a.n = h->ni;
+  a.row = h->row;
+  a.col = h->col;
+  a.blk = m.blk;
+  a.nblk = m.nblk;
+  a.val = val;
+  a.b = b;
+  a.x = h->x;
+  a.r = h->r;
+  a.p = h->p;
+  a.q = h->q;
+  a.ad = h->ad;
+  a.pt = partials_of(h, L);
+  a.st = h->st;
+  tfg::IrArgs f{};
+  f.n = h->ni;
+  f.val32 = h->val32;
+  f.dinv32 = h->ad32;
+  f.r32 = h->r32;
+  f.e32 = h->e32;
+  f.p32 = h->p32;
+  f.q32 = h->q32;
+  f.st_in = h->st_in;
+  const int ipr = prec == tfg::PREC_JACOBI ? tfg::PREC_JACOBI : tfg::PREC_NONE;
+  // Refinement (and restart of the FP32 CG) when its recursive residual has
+  // dropped by inner_tol (default 1e-4) since the last refinement, and every
+  // inner_max iterations when inner_max > 0.  (Keeping the FP32 direction
+  // across refinements -- residual replacement -- stagnates on the nearly
+  // singular pinned-pressure systems, so every refinement restarts.)
+  const double delta = o.inner_tol > 0.0 ? o.inner_tol : 1e-4;
+  const int restart = o.inner_max > 0 ? o.inner_max : 0;
+
+  State* s0 = &h->hst[0];
+  memset(s0, 0, sizeof(State));
+  s0->tol = tol;
+  s0->criterion = o.criterion;
+  s0->has_norm = norm ? 1 : 0;
+  s0->norm = norm ? *norm : 0.0;
+  HIP_TRY(hipMemcpyAsync(h->st, s0, sizeof(State), hipMemcpyHostToDevice, h->stream));
+
+  const bool timing = o.timing != 0;
+  const int max_samples = 256;
+  if (timing && h->ev_sample.empty()) {
+    h->ev_sample.resize(2 * max_samples, nullptr);
+    for (auto& e : h->ev_sample) HIP_TRY(hipEventCreate(&e));
+  }
+  std::vector<int> sample_k;
+  int status = TFG_OK, refinements = 0;
+  State* sf = &h->hst[3];
+
+  HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
+  HIP_TRY(tfg::launch_diag_sum(L, a));
+  HIP_TRY(tfg::launch_prologue(L, a, ipr));
+  HIP_TRY(tfg::launch_ir_convert(L, a, f, ipr, h->nnz));
+  HIP_TRY(tfg::launch_ir_check(L, a, 0));
+  HIP_TRY(tfg::launch_ir_zero(L, f));
+  HIP_TRY(tfg::launch_ir_replace(L, a, f, ipr));
+
+  const int chunk = h->nnz > (1 << 22) ? 8 : 32;
+  int k = 1, first = 1, slot = 0, prev = -1, last_refine = 0;
+  bool stop = false, ignore_next = false;
+  // refinement: fold e32 into x, FP64 true residual, stopping test (st->iter
+  // records the iteration count K), residual replacement
+  auto refine = [&](int K, bool restart_dir) -> int {
+    HIP_TRY(tfg::launch_ir_correct(L, a, f));
+    HIP_TRY(tfg::launch_prologue(L, a, ipr));
+    HIP_TRY(tfg::launch_ir_check(L, a, K));
+    HIP_TRY(tfg::launch_ir_replace(L, a, f, ipr));
+    ++refinements;
+    last_refine = K;
+    if (restart_dir) first = 1;
+    return TFG_OK;
+  };
+  while (k <= miter && !stop) {
+    int kend = std::min(miter, k + chunk - 1);
+    if (restart) kend = std::min(kend, last_refine + restart);
+    for (; k <= kend; ++k) {
+      HIP_TRY(tfg::launch_ir_pupdate(L, a, f, ipr, k, first));
+      first = 0;
+      const bool take = timing && (k % 4 == 1) && (int)sample_k.size() < max_samples;
+      if (take) HIP_TRY(hipEventRecord(h->ev_sample[2 * sample_k.size()], h->stream));
+      HIP_TRY(tfg::launch_ir_spmv_pq(L32, a, f, k));
+      if (take) {
+        HIP_TRY(hipEventRecord(h->ev_sample[2 * sample_k.size() + 1], h->stream));
+        sample_k.push_back(k);
+      }
+      HIP_TRY(tfg::launch_ir_update(L, a, f, ipr, k));
+    }
+    HIP_TRY(tfg::launch_ir_finish(L, a, f, kend, delta));
+    if (restart && kend == last_refine + restart && kend < miter) {
+      const int rr = refine(kend, true);
+      if (rr != TFG_OK) return rr;
+      ignore_next = true;
+    }
+    HIP_TRY(hipMemcpyAsync(&h->hst[1 + slot], h->st_in, sizeof(State), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipEventRecord(h->ev_poll[slot], h->stream));
+    if (prev >= 0) {
+      HIP_TRY(hipEventSynchronize(h->ev_poll[prev]));
+      const State& sp = h->hst[1 + prev];
+      if (sp.done) {
+        stop = true;
+      } else if (ignore_next) {
+        ignore_next = false;  // this flag predates the refinement just queued
+      } else if (sp.aux0 != 0.0 && k <= miter) {
+        const int rr = refine(k - 1, true);
+        if (rr != TFG_OK) return rr;
+        ignore_next = true;
+      }
+    }
+    prev = slot;
+    slot ^= 1;
+  }
+  // final fold of the FP32 correction and the stopping measure on the
+  // FP64 true residual (a no-op fold when the last refinement converged)
+  HIP_TRY(hipMemcpyAsync(sf, h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  int iters;
+  if (sf->done) {
+    iters = sf->iter;
+  } else {
+    iters = std::max(0, std::min(k - 1, miter));
+    HIP_TRY(tfg::launch_ir_correct(L, a, f));
+    HIP_TRY(tfg::launch_prologue(L, a, ipr));
+    HIP_TRY(tfg::launch_ir_check(L, a, iters));
+    ++refinements;
+  }
+  HIP_TRY(hipMemcpyAsync(&h->hst[4], h->st_in, sizeof(State), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->hst[4].status == TFG_EBREAKDOWN) status = TFG_EBREAKDOWN;
+  double sample_ms = 0.0;
+  int samples = 0;
+  if (timing) {
+    for (size_t q = 0; q < sample_k.size(); ++q) {
+      if (sample_k[q] > iters) break;
+      float e = 0.f;
+      HIP_TRY(hipEventElapsedTime(&e, h->ev_sample[2 * q], h->ev_sample[2 * q + 1]));
+      sample_ms += e;
+      ++samples;
+    }
+  }
+  const int outer = refinements;
+  const long long total = iters;
+  HIP_TRY(hipEventRecord(h->ev_t1, h->stream));
+  HIP_TRY(hipMemcpyAsync(sf, h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (sf->status == TFG_EBREAKDOWN) status = TFG_EBREAKDOWN;
+  *niter = (int)std::min<long long>(total, INT32_MAX);
+  *fin_res = sf->res;
+  if (rep) {
+    rep->status = status;
+    rep->iterations = *niter;
+    rep->refinements = outer;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev_t0, h->ev_t1));
+    rep->solve_ms = ms;
+    rep->spmv_ms = samples ? sample_ms / samples : 0.0;
+    rep->spmv_samples = samples;
+  }
+  if (o.true_residual) {
+    HIP_TRY(tfg::launch_true_residual(L, a));
+    HIP_TRY(hipMemcpyAsync(sf, h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (rep) rep->true_relres = sf->aux0;
+  }
+  return status;
 }
+
+}  // namespace tfg
 
 // ===========================================================================
 extern "C" {
@@ -579,6 +763,30 @@ int tfg_memcpy_h2d(void* dst, const void* src, long long bytes) {
 
 int tfg_memcpy_d2h(void* dst, const void* src, long long bytes) {
   HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  return TFG_OK;
+}
+
+int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, double* ms_avg) {
+  if (!d_dst || !d_src || bytes < 16 || (bytes % 16) || reps < 1 || !ms_avg)
+    return fail(TFG_EINVAL, "bad copy request");
+  if (((uintptr_t)d_dst % 16) || ((uintptr_t)d_src % 16)) return fail(TFG_EINVAL, "copy needs 16-byte alignment");
+  int dev = 0, cus = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const int grid = std::max(1, cus * 4);
+  HIP_TRY(tfg::launch_copy16(nullptr, grid, d_src, d_dst, bytes / 16));
+  HIP_TRY(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < reps; ++r) HIP_TRY(tfg::launch_copy16(nullptr, grid, d_src, d_dst, bytes / 16));
+  HIP_TRY(hipEventRecord(e1, nullptr));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  *ms_avg = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return TFG_OK;
 }
 

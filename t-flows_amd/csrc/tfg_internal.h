@@ -81,6 +81,19 @@ struct SpmvShape {
 enum SpmvKind { K_STAGE2048 = 0, K_STAGE4096 = 1, K_WAVE4 = 2, K_WAVE8 = 3, K_WAVE16 = 4,
                 K_PIPE2048 = 5, kKinds = 6 };
 
+// FP32 sweeps of iterative refinement (FP32 vectors, FP64 scalars and
+// reductions).  Reuses the FP64 solve's partial buffers (spmv, upd).
+struct IrArgs {
+  int n;
+  const float* val32;   // float(a_ij * fn)
+  const float* dinv32;  // float(1 / (a_ii * fn)) (Jacobi) or 1
+  float* r32;
+  float* e32;
+  float* p32;
+  float* q32;
+  State* st_in;         // inner scalars
+};
+
 // Launch shapes chosen per matrix.
 struct Launch {
   hipStream_t stream;
@@ -104,6 +117,23 @@ hipError_t launch_true_residual(const Launch& L, const CgArgs& a);
 
 hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* val,
                              const double* p, double* q);
+hipError_t launch_copy16(hipStream_t stream, int grid, const void* src, void* dst, long long n16);
+
+// Iterative refinement (tfg_ir.hip).
+hipError_t launch_ir_convert(const Launch& L, const CgArgs& a, const IrArgs& f, int prec,
+                             int nnz);
+hipError_t launch_ir_check(const Launch& L, const CgArgs& a, int outer);
+hipError_t launch_ir_replace(const Launch& L, const CgArgs& a, const IrArgs& f, int prec);
+hipError_t launch_ir_zero(const Launch& L, const IrArgs& f);
+hipError_t launch_ir_pupdate(const Launch& L, const CgArgs& a, const IrArgs& f, int prec, int k,
+                             int first);
+hipError_t launch_ir_spmv_pq(const Launch& L, const CgArgs& a, const IrArgs& f, int k);
+hipError_t launch_ir_update(const Launch& L, const CgArgs& a, const IrArgs& f, int prec, int k);
+hipError_t launch_ir_finish(const Launch& L, const CgArgs& a, const IrArgs& f, int k,
+                            double delta);
+hipError_t launch_ir_correct(const Launch& L, const CgArgs& a, const IrArgs& f);
+hipError_t launch_spmv_plain32(const Launch& L, const SpmvShape& m, const float* val,
+                               const float* p, float* q);
 hipError_t launch_residual_plain(const Launch& L, const SpmvShape& m, const double* val,
                                  const double* b, const double* x, double* r);
 

@@ -346,40 +346,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv_plain(SpmvShape m, const double
 // ---------------------------------------------------------------------------
 // Engine dispatch
 // ---------------------------------------------------------------------------
-// kind: 0 stage/2048, 1 stage/4096, 2 wave/4, 3 wave/8, 4 wave/16,
-// 5 pipelined stage/2048.  The stage engines fall back to scalar staging
-// (ScalarStage) when val is not 16-byte aligned or nnz < 4.
-#define TFG_ENGINE_SWITCH(L, CALL)                                                 \
-  do {                                                                             \
-    switch ((L).kind) {                                                            \
-      case 0:                                                                      \
-        if ((L).vec_val) { using E = StreamEngine<2048, false>; CALL; }            \
-        else { using E = ScalarStage<2048>; CALL; }                                \
-        break;                                                                     \
-      case 1:                                                                      \
-        if ((L).vec_val) { using E = StreamEngine<4096, false>; CALL; }            \
-        else { using E = ScalarStage<4096>; CALL; }                                \
-        break;                                                                     \
-      case 5:                                                                      \
-        if ((L).vec_val) { using E = StreamEngine<2048, true>; CALL; }             \
-        else { using E = ScalarStage<2048>; CALL; }                                \
-        break;                                                                     \
-      case 2: { using E = WaveEngine<4>; CALL; } break;                            \
-      case 3: { using E = WaveEngine<8>; CALL; } break;                            \
-      default: { using E = WaveEngine<16>; CALL; } break;                          \
-    }                                                                              \
-  } while (0)
+#define TFG_ENGINE_SWITCH(L, CALL) TFG_ENGINE_SWITCH_T(L, double, CALL)
 
 int spmv_blocks_per_cu(int kind) {
   int nb = 0;
   hipError_t e = hipSuccess;
   switch (kind) {
-    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, false>>, kBlock, 0); break;
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<4096, false>>, kBlock, 0); break;
-    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<4>>, kBlock, 0); break;
-    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<8>>, kBlock, 0); break;
-    case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, true>>, kBlock, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<16>>, kBlock, 0); break;
+    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, false, double>>, kBlock, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<4096, false, double>>, kBlock, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<4, double>>, kBlock, 0); break;
+    case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<8, double>>, kBlock, 0); break;
+    case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, true, double>>, kBlock, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<16, double>>, kBlock, 0); break;
   }
   if (e != hipSuccess || nb < 1) return 1;
   return nb;
@@ -430,6 +408,19 @@ hipError_t launch_true_residual(const Launch& L, const CgArgs& a) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_true_finish<<<1, kBlock, 0, L.stream>>>(a);
+  return hipGetLastError();
+}
+
+// Plain 16-byte streaming copy: a kernel of known traffic (reads n16*16
+// bytes, writes n16*16 bytes) used to calibrate the HBM byte counters.
+__global__ __launch_bounds__(kBlock) void k_copy16(const dbl2* __restrict__ src,
+                                                   dbl2* __restrict__ dst, long long n16) {
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy16(hipStream_t stream, int grid, const void* src, void* dst, long long n16) {
+  k_copy16<<<grid, kBlock, 0, stream>>>(reinterpret_cast<const dbl2*>(src), reinterpret_cast<dbl2*>(dst), n16);
   return hipGetLastError();
 }
 

@@ -17,6 +17,9 @@ namespace tfg {
 
 enum RedOp { OP_SUM = 0, OP_MIN = 1, OP_MAX = 2 };
 
+// Thread 0 of workgroup 0: the one thread that writes a kernel's scalars.
+__device__ __forceinline__ bool lead_thread() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
 __device__ __forceinline__ double red_apply(int op, double a, double b) {
   return op == OP_SUM ? a + b : (op == OP_MIN ? fmin(a, b) : fmax(a, b));
 }

@@ -6,23 +6,26 @@
 // contraction this reproduces the reference's row sums bit for bit.  The
 // engine then calls epi(i, acc_i, x_i, d_i) on exactly one thread, where x_i
 // is the x value of the row's first (diagonal) entry and d_i = a_ii * fn.
+// T is the value type of the matrix and of x (double, or float for the
+// inner solves of iterative refinement).
 //
-//  StreamEngine<CAP, PIPE> one row per thread; the workgroup first streams
-//                         the CSR values/columns of up to 256 consecutive rows
-//                         (<= CAP nonzeros) into LDS with coalesced 16-byte
-//                         nontemporal loads, then each thread sums its row
-//                         from LDS.  With PIPE, while a workgroup sums block b
-//                         the CSR streams of its next block are already in
-//                         flight into registers (issued after block b's x
-//                         gathers, so waiting for those does not wait for them).
-//  ScalarStage<CAP>       the same with element-wise staging (unaligned val).
-//  WaveEngine<L>          L lanes per row (L = 4, 8, 16), 64/L rows per
+//  StreamEngine<CAP, PIPE, T>  one row per thread; the workgroup first
+//                         streams the CSR values/columns of up to 256
+//                         consecutive rows (<= CAP nonzeros) into LDS with
+//                         coalesced 16-byte nontemporal loads, then each thread
+//                         sums its row from LDS.  With PIPE, while a workgroup
+//                         sums block b the CSR streams of its next block are
+//                         already in flight into registers (issued after block
+//                         b's x gathers, so waiting for those does not wait
+//                         for them: vector loads complete in issue order).
+//  ScalarStage<CAP, T>    the same with element-wise staging (unaligned val).
+//  WaveEngine<L, T>       L lanes per row (L = 4, 8, 16), 64/L rows per
 //                         wavefront, no LDS and no barriers: lane k of a
 //                         group holds term k of a chunk of L terms, and an
 //                         ordered scan over the group (DPP row_shr:1, one lane
 //                         per step) adds them left to right onto the carried
 //                         sum, so the association stays sequential.
-// Both deal row blocks to XCDs in contiguous ranges (workgroup g runs on XCD
+// All deal row blocks to XCDs in contiguous ranges (workgroup g runs on XCD
 // g % 8), so each XCD's L2 holds one window of the x gather.
 #pragma once
 
@@ -53,37 +56,96 @@ struct XcdWalk {
 };
 
 // ---------------------------------------------------------------------------
-// Stage engines.  The CSR streams of a row block are loaded into registers
-// with 16-byte nontemporal loads whose indices are clamped inside the block
-// and inside the arrays, so every load is issued unconditionally and back to
-// back (a guarded load would make the compiler wait for each one in turn).
-// Chunks past the block's end re-read its last chunk (an L1 hit).  The only
-// chunks that cannot be read 16 bytes wide are the last 1-3 elements of the
-// whole array; those come from uniform scalar loads.  Needs nnz >= 4 and a
-// 16-byte aligned val (otherwise ScalarStage is used).
-template <int CAP, bool PIPE>
+// A 16-byte-chunk stream of one CSR array (val or col) for one row block
+// [s, e): chunk t of the workgroup covers elements o + EPC*t .. + EPC-1 with
+// o = s rounded down to a chunk.  Loads are issued unconditionally with the
+// chunk index clamped inside the block and inside the array, so the compiler
+// can keep them all in flight (a guarded load makes it wait for each in
+// turn); chunks past the block re-read its last chunk (an L1 hit).  The one
+// chunk that holds the array's last 1..EPC-1 elements cannot be read 16 bytes
+// wide; those elements come from uniform scalar loads (`tail`) and are
+// patched in when the chunk is written to LDS.  Needs nnz >= EPC.
+template <typename E, int NCH>
+struct ChunkStream {
+  static constexpr int EPC = 16 / (int)sizeof(E);
+  typedef E V __attribute__((ext_vector_type(16 / sizeof(E))));
+  V v[NCH];
+  V tail;
+
+  __device__ __forceinline__ void fetch(const E* __restrict__ arr, int nnz, int s, int e) {
+    typedef __attribute__((address_space(4))) const E cE;
+    const int tid = threadIdx.x;
+    const int o = s & ~(EPC - 1);
+    const int jmax = (nnz - EPC) & ~(EPC - 1);
+    const int lim = min((e - 1) & ~(EPC - 1), jmax);
+#pragma unroll
+    for (int k = 0; k < EPC; ++k) tail[k] = ((cE*)arr)[min(jmax + EPC + k, nnz - 1)];
+#pragma unroll
+    for (int u = 0; u < NCH; ++u) {
+      const int j = o + EPC * (tid + u * kBlock);
+      v[u] = __builtin_nontemporal_load(reinterpret_cast<const V*>(arr + min(j, lim)));
+    }
+  }
+
+  // Registers -> LDS.  Writes indices below e - o + EPC - 1 <= CAP + 2*EPC - 2.
+  __device__ __forceinline__ void put(E* lds, int nnz, int s, int e) const {
+    const int tid = threadIdx.x;
+    const int o = s & ~(EPC - 1);
+    const int n = (e - o + EPC - 1) / EPC;
+    const int jmax = (nnz - EPC) & ~(EPC - 1);
+#pragma unroll
+    for (int u = 0; u < NCH; ++u) {
+      const int t = tid + u * kBlock;
+      if (t < n) {
+        const V c = o + EPC * t > jmax ? tail : v[u];
+#pragma unroll
+        for (int k = 0; k < EPC; ++k) lds[EPC * t + k] = c[k];
+      }
+    }
+  }
+};
+
+// Sum of a row longer than the stage: strided partial sums combined in a
+// fixed order (rows this long do not occur in T-Flows grids).
+template <bool SUB, typename T, typename Init, typename Epi>
+__device__ __forceinline__ void long_row(const SpmvShape& m, const T* __restrict__ val, T fn,
+                                         const T* __restrict__ x, int r0, int s, int e,
+                                         double* s_tmp, double* s_out, Init& init, Epi& epi) {
+  double part[1] = {0.0};
+  for (int j = s + (int)threadIdx.x; j < e; j += kBlock)
+    part[0] = part[0] + (double)((val[j] * fn) * x[m.col[j]]);
+  const int ops[1] = {OP_SUM};
+  block_reduce<1>(part, ops, s_tmp, s_out);
+  if (threadIdx.x == 0) {
+    const T a0 = init(r0);
+    const T sum = (T)s_out[0];
+    epi(r0, SUB ? a0 - sum : a0 + sum, x[r0], val[s] * fn);
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+template <int CAP, bool PIPE, typename T>
 struct StreamEngine {
-  static constexpr int NV = (CAP / 2 + 2 + kBlock - 1) / kBlock;  // val chunks per thread
-  static constexpr int NC = (CAP / 4 + 2 + kBlock - 1) / kBlock;  // col chunks per thread
+  static constexpr int EPV = 16 / (int)sizeof(T);
+  static constexpr int NV = (CAP / EPV + 2 + kBlock - 1) / kBlock;  // val chunks per thread
+  static constexpr int NC = (CAP / 4 + 2 + kBlock - 1) / kBlock;    // col chunks per thread
   struct Smem {
-    double val[CAP + 4];
+    T val[CAP + 2 * EPV];
     int col[CAP + 8];
   };
   struct Regs {
-    dbl2 v[NV];     // raw 16-byte chunks (the array tail is patched in put)
-    int4v c[NC];
-    double tv;      // val[nnz-1]
-    int4v tc;       // col[jmax_c+4 ..]
+    ChunkStream<T, NV> v;
+    ChunkStream<int, NC> c;
     int r0, nr, s, e, rs, re;
   };
 
-  __device__ __forceinline__ static void fetch(const SpmvShape& m, const double* __restrict__ val,
+  __device__ __forceinline__ static void fetch(const SpmvShape& m, const T* __restrict__ val,
                                                int nnz, int lb, Regs& g) {
     // Block-uniform facts are read through the constant address space: the
     // arrays are never written during a launch, so these become scalar
     // loads (counted apart from the vector loads in flight).
     typedef __attribute__((address_space(4))) const int cint;
-    typedef __attribute__((address_space(4))) const double cdouble;
     const int tid = threadIdx.x;
     const int r0 = ((cint*)m.blk)[lb], r1 = ((cint*)m.blk)[lb + 1];
     const int s = ((cint*)m.row)[r0], e = ((cint*)m.row)[r1];
@@ -94,70 +156,23 @@ struct StreamEngine {
     g.rs = m.row[r0 + min(tid, r1 - r0)];
     g.re = m.row[r0 + min(tid + 1, r1 - r0)];
     // A block that is one long row (e - s > CAP) is summed from global memory
-    // and never staged; its loads below still stay inside the arrays.
-    const int ov = s & ~1, oc = s & ~3;
-    const int jmax_v = (nnz - 2) & ~1, jmax_c = (nnz - 4) & ~3;
-    const int lim_v = min((e - 1) & ~1, jmax_v), lim_c = min((e - 1) & ~3, jmax_c);
-    // tail elements of the arrays (uniform, scalar loads)
-    g.tv = ((cdouble*)val)[nnz - 1];
-    g.tc.x = ((cint*)m.col)[min(jmax_c + 4, nnz - 1)];
-    g.tc.y = ((cint*)m.col)[min(jmax_c + 5, nnz - 1)];
-    g.tc.z = ((cint*)m.col)[min(jmax_c + 6, nnz - 1)];
-    g.tc.w = 0;
-    // No use of the loaded values here: the loads stay in flight until put().
-#pragma unroll
-    for (int u = 0; u < NV; ++u) {
-      const int j = ov + 2 * (tid + u * kBlock);
-      g.v[u] = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(val + min(j, lim_v)));
-    }
-#pragma unroll
-    for (int u = 0; u < NC; ++u) {
-      const int j = oc + 4 * (tid + u * kBlock);
-      g.c[u] = __builtin_nontemporal_load(reinterpret_cast<const int4v*>(m.col + min(j, lim_c)));
-    }
-  }
-
-  // Registers -> LDS, patching the chunk that holds the array's last
-  // elements.  Every index written is below e - ov + 2 <= CAP + 3 (val) and
-  // e - oc + 4 <= CAP + 7 (col).
-  __device__ __forceinline__ static void put(Smem& sm, const Regs& g, int nnz) {
-    const int tid = threadIdx.x;
-    const int ov = g.s & ~1, oc = g.s & ~3;
-    const int nv = (g.e - ov + 1) >> 1, nc = (g.e - oc + 3) >> 2;
-    const int jmax_v = (nnz - 2) & ~1, jmax_c = (nnz - 4) & ~3;
-#pragma unroll
-    for (int u = 0; u < NV; ++u) {
-      const int t = tid + u * kBlock;
-      if (t < nv) {
-        sm.val[2 * t] = ov + 2 * t > jmax_v ? g.tv : g.v[u].x;
-        sm.val[2 * t + 1] = g.v[u].y;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < NC; ++u) {
-      const int t = tid + u * kBlock;
-      if (t < nc) {
-        const int4v c = oc + 4 * t > jmax_c ? g.tc : g.c[u];
-        sm.col[4 * t] = c.x;
-        sm.col[4 * t + 1] = c.y;
-        sm.col[4 * t + 2] = c.z;
-        sm.col[4 * t + 3] = c.w;
-      }
-    }
+    // and never staged; its loads here still stay inside the arrays.  No use
+    // of the loaded values here: they stay in flight until put().
+    g.v.fetch(val, nnz, s, e);
+    g.c.fetch(m.col, nnz, s, e);
   }
 
   template <bool SUB, typename Init, typename Epi>
   __device__ __forceinline__ static void run(Smem& sm, const SpmvShape& m,
-                                             const double* __restrict__ val, double fn,
-                                             const double* x, double* s_tmp,
-                                             double* s_out, Init init, Epi epi) {
+                                             const T* __restrict__ val, T fn, const T* x,
+                                             double* s_tmp, double* s_out, Init init, Epi epi) {
     const int tid = threadIdx.x;
     const int nnz = m.row[m.n];
     // The gather pointer is laundered through an empty asm (keeping the
     // global address space) so its loads are not hoisted or sunk across the
     // prefetch as freely as loads through a __restrict__ pointer.
-    typedef __attribute__((address_space(1))) const double gdouble;
-    gdouble* xg = (gdouble*)x;
+    typedef __attribute__((address_space(1))) const T gT;
+    gT* xg = (gT*)x;
     asm volatile("" : "+s"(xg));
     const XcdWalk w(m.nblk);
     Regs g;
@@ -168,27 +183,18 @@ struct StreamEngine {
       if (!PIPE) fetch(m, val, nnz, lb, g);
       const int r0 = g.r0, nr = g.nr, s = g.s, e = g.e, rs = g.rs, re = g.re;
       if (e - s > CAP) {
-        // a single row longer than the stage: strided partial sums combined
-        // in a fixed order (rows this long do not occur in T-Flows grids)
-        double part[1] = {0.0};
-        for (int j = s + tid; j < e; j += kBlock) part[0] = part[0] + (val[j] * fn) * x[m.col[j]];
-        const int ops[1] = {OP_SUM};
-        block_reduce<1>(part, ops, s_tmp, s_out);
-        if (tid == 0) {
-          const double a0 = init(r0);
-          epi(r0, SUB ? a0 - s_out[0] : a0 + s_out[0], x[r0], val[s] * fn);
-        }
-        __syncthreads();
+        long_row<SUB>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
         if (PIPE) fetch(m, val, nnz, min(nxt, w.end - 1), g);
         lb = nxt;
         continue;
       }
-      put(sm, g, nnz);
+      g.v.put(sm.val, nnz, s, e);
+      g.c.put(sm.col, nnz, s, e);
       __syncthreads();
-      const int ov = s & ~1, oc = s & ~3;
+      const int ov = s & ~(EPV - 1), oc = s & ~3;
       // first eight terms of the row: gathers issued unconditionally with the
       // index clamped into the row (rows are never empty)
-      double av[8], xv[8];
+      T av[8], xv[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int j = min(rs + u, re - 1);
@@ -198,8 +204,7 @@ struct StreamEngine {
       // next block's loads, in flight while this block is summed (past the
       // last block the current one is fetched again: no branch, so the
       // compiler can keep them outstanding across the sums).  The scheduling
-      // barriers keep them AFTER the gathers: loads complete in issue order,
-      // so waiting for a gather must not mean waiting for the prefetch.
+      // barriers keep them AFTER the gathers.
       if (PIPE) {
         __builtin_amdgcn_sched_barrier(0);
         fetch(m, val, nnz, min(nxt, w.end - 1), g);
@@ -208,13 +213,13 @@ struct StreamEngine {
       // every thread sums (threads past the block's rows on clamped data), so
       // the gathers above have an unconditional use and stay where they are
       const int i = r0 + min(tid, nr - 1);
-      double acc = init(i);
+      T acc = init(i);
 #pragma unroll
       for (int u = 0; u < 8; ++u)
         if (rs + u < re) acc = SUB ? acc - av[u] * xv[u] : acc + av[u] * xv[u];
       if (tid < nr) {
         for (int j = rs + 8; j < re; ++j) {
-          const double a = sm.val[j - ov] * fn;
+          const T a = sm.val[j - ov] * fn;
           acc = SUB ? acc - a * x[sm.col[j - oc]] : acc + a * x[sm.col[j - oc]];
         }
         epi(i, acc, xv[0], av[0]);
@@ -227,17 +232,17 @@ struct StreamEngine {
 
 // Stage engine with element-wise staging loads, for a val array that is not
 // 16-byte aligned or a matrix with fewer than 4 nonzeros.
-template <int CAP>
+template <int CAP, typename T>
 struct ScalarStage {
   struct Smem {
-    double val[CAP];
+    T val[CAP];
     int col[CAP];
   };
 
   template <bool SUB, typename Init, typename Epi>
   __device__ __forceinline__ static void run(Smem& sm, const SpmvShape& m,
-                                             const double* __restrict__ val, double fn,
-                                             const double* __restrict__ x, double* s_tmp,
+                                             const T* __restrict__ val, T fn,
+                                             const T* __restrict__ x, double* s_tmp,
                                              double* s_out, Init init, Epi epi) {
     const int tid = threadIdx.x;
     const XcdWalk w(m.nblk);
@@ -245,15 +250,7 @@ struct ScalarStage {
       const int r0 = m.blk[lb], nr = m.blk[lb + 1] - r0;
       const int s = m.row[r0], e = m.row[r0 + nr];
       if (e - s > CAP) {
-        double part[1] = {0.0};
-        for (int j = s + tid; j < e; j += kBlock) part[0] = part[0] + (val[j] * fn) * x[m.col[j]];
-        const int ops[1] = {OP_SUM};
-        block_reduce<1>(part, ops, s_tmp, s_out);
-        if (tid == 0) {
-          const double a0 = init(r0);
-          epi(r0, SUB ? a0 - s_out[0] : a0 + s_out[0], x[r0], val[s] * fn);
-        }
-        __syncthreads();
+        long_row<SUB>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
         continue;
       }
       for (int t = tid; t < e - s; t += kBlock) {
@@ -264,9 +261,9 @@ struct ScalarStage {
       if (tid < nr) {
         const int i = r0 + tid;
         const int rs = m.row[i], re = m.row[i + 1];
-        double acc = init(i);
+        T acc = init(i);
         for (int j = rs; j < re; ++j) {
-          const double a = sm.val[j - s] * fn;
+          const T a = sm.val[j - s] * fn;
           acc = SUB ? acc - a * x[sm.col[j - s]] : acc + a * x[sm.col[j - s]];
         }
         epi(i, acc, x[sm.col[rs - s]], sm.val[rs - s] * fn);
@@ -284,7 +281,11 @@ __device__ __forceinline__ double dpp_shr1(double v) {
   return __hiloint2double(hi, lo);
 }
 
-template <int L>
+__device__ __forceinline__ float dpp_shr1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+}
+
+template <int L, typename T>
 struct WaveEngine {
   static_assert(L == 4 || L == 8 || L == 16, "lanes per row must divide a DPP row");
   struct Smem {
@@ -293,8 +294,8 @@ struct WaveEngine {
 
   template <bool SUB, typename Init, typename Epi>
   __device__ __forceinline__ static void run(Smem&, const SpmvShape& m,
-                                             const double* __restrict__ val, double fn,
-                                             const double* __restrict__ x, double*, double*,
+                                             const T* __restrict__ val, T fn,
+                                             const T* __restrict__ x, double*, double*,
                                              Init init, Epi epi) {
     const int jl = threadIdx.x & (L - 1);
     const int gpb = kBlock / L;  // row groups per workgroup
@@ -303,20 +304,20 @@ struct WaveEngine {
       const int i = i0 + (int)threadIdx.x / L;
       if (i >= w.end) continue;  // whole groups drop out together
       const int rs = m.row[i], re = m.row[i + 1];
-      double acc = jl == 0 ? init(i) : 0.0, xi = 0.0, di = 0.0;
+      T acc = jl == 0 ? init(i) : T(0), xi = T(0), di = T(0);
       for (int j0 = rs; j0 < re; j0 += L) {
         const int j = j0 + jl;
         const int jc = min(j, re - 1);  // loads never branch; terms past the end are 0
-        const double a = val[jc] * fn, xv = x[m.col[jc]];
-        const double t = j < re ? a * xv : 0.0;
+        const T a = val[jc] * fn, xv = x[m.col[jc]];
+        const T t = j < re ? a * xv : T(0);
         if (j == rs) {
           xi = xv;
           di = a;
         }
-        double sacc = SUB ? acc - t : acc + t;  // meaningful on lane 0
+        T sacc = SUB ? acc - t : acc + t;  // meaningful on lane 0
 #pragma unroll
         for (int k = 1; k < L; ++k) {
-          const double prev = dpp_shr1(sacc);
+          const T prev = dpp_shr1(sacc);
           if (jl == k) sacc = SUB ? prev - t : prev + t;
         }
         // carry = the group's last lane (terms past the row end added +-0)
@@ -326,5 +327,29 @@ struct WaveEngine {
     }
   }
 };
+
+// kind: 0 stage/2048, 1 stage/4096, 2 wave/4, 3 wave/8, 4 wave/16,
+// 5 pipelined stage/2048.  The stage engines fall back to scalar staging
+// (ScalarStage) when val is not 16-byte aligned or nnz < 4.
+#define TFG_ENGINE_SWITCH_T(L, T, CALL)                                            \
+  do {                                                                             \
+    switch ((L).kind) {                                                            \
+      case 0:                                                                      \
+        if ((L).vec_val) { using E = StreamEngine<2048, false, T>; CALL; }         \
+        else { using E = ScalarStage<2048, T>; CALL; }                             \
+        break;                                                                     \
+      case 1:                                                                      \
+        if ((L).vec_val) { using E = StreamEngine<4096, false, T>; CALL; }         \
+        else { using E = ScalarStage<4096, T>; CALL; }                             \
+        break;                                                                     \
+      case 5:                                                                      \
+        if ((L).vec_val) { using E = StreamEngine<2048, true, T>; CALL; }          \
+        else { using E = ScalarStage<2048, T>; CALL; }                             \
+        break;                                                                     \
+      case 2: { using E = WaveEngine<4, T>; CALL; } break;                         \
+      case 3: { using E = WaveEngine<8, T>; CALL; } break;                         \
+      default: { using E = WaveEngine<16, T>; CALL; } break;                       \
+    }                                                                              \
+  } while (0)
 
 }  // namespace tfg

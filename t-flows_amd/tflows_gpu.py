@@ -77,6 +77,7 @@ def lib():
         L.tfg_memcpy_h2d.argtypes = [P, P, LL]
         L.tfg_memcpy_d2h.argtypes = [P, P, LL]
         L.tfg_synchronize.argtypes = [P]
+        L.tfg_stream_copy.argtypes = [P, P, LL, I, ctypes.POINTER(D)]
         L.tfg_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -136,6 +137,13 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def stream_copy(dst, src, nbytes, reps=10):
+    """Mean time (ms) of a plain device copy of nbytes (calibration kernel)."""
+    ms = ctypes.c_double()
+    _check(lib().tfg_stream_copy(dst.ptr, src.ptr, int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 class Native:
