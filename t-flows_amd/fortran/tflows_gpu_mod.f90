@@ -1,0 +1,139 @@
+!==============================================================================!
+  module Tflows_Gpu_Mod
+!------------------------------------------------------------------------------!
+!>  Fortran binding of the tflows_gpu C ABI (include/tflows_gpu.h) for the
+!>  T-Flows Process program.  Tflows_Gpu_Solve keeps the argument list of
+!>  Nat % Solve_Native(solver, prec, A, x, b, miter, niter, tol, fin_res, norm)
+!>  (Sources/Process/Cpu/Native_Mod/Solve_Native.f90:2-29), with the matrix
+!>  passed as the handle of its uploaded structure plus its values A % val.
+!>  Any failure stops the program, as Message % Error does for the
+!>  reference's fatal conditions (Message_Mod/Error.f90:73-74).
+!------------------------------------------------------------------------------!
+  use iso_c_binding
+  implicit none
+!==============================================================================!
+
+  interface
+
+    integer(c_int) function tfg_matrix_create(ni, nt, nnz, row, col, dia,  &
+                                              handle)                      &
+                                              bind(C, name='tfg_matrix_create')
+      import :: c_int, c_ptr
+      integer(c_int), value       :: ni, nt, nnz
+      integer(c_int), intent(in)  :: row(*), col(*), dia(*)
+      type(c_ptr),    intent(out) :: handle
+    end function
+
+    integer(c_int) function tfg_matrix_destroy(handle)                     &
+                                              bind(C, name='tfg_matrix_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+    end function
+
+    integer(c_int) function tfg_solve_native(handle, solver, solver_len,   &
+                                             prec, prec_len, val, x, b,    &
+                                             miter, niter, tol, fin_res,   &
+                                             norm)                         &
+                                             bind(C, name='tfg_solve_native')
+      import :: c_int, c_ptr, c_char, c_double
+      type(c_ptr),            value          :: handle
+      character(kind=c_char), intent(in)     :: solver(*), prec(*)
+      integer(c_int),         value          :: solver_len, prec_len
+      real(c_double),         intent(in)     :: val(*)
+      real(c_double),         intent(inout)  :: x(*), b(*)
+      integer(c_int),         value          :: miter
+      integer(c_int),         intent(out)    :: niter
+      real(c_double),         value          :: tol
+      real(c_double),         intent(out)    :: fin_res
+      real(c_double), optional, intent(in)   :: norm
+    end function
+
+    type(c_ptr) function tfg_last_error() bind(C, name='tfg_last_error')
+      import :: c_ptr
+    end function
+
+  end interface
+
+  contains
+
+!==============================================================================!
+  function Tflows_Gpu_Error_Text() result(text)
+!------------------------------------------------------------------------------!
+!>  Message of the last failed call.
+!------------------------------------------------------------------------------!
+  character(len=256)               :: text
+  character(kind=c_char), pointer  :: chars(:)
+  integer                          :: i
+!==============================================================================!
+  text = ' '
+  call c_f_pointer(tfg_last_error(), chars, [256])
+  do i = 1, 256
+    if(chars(i) == c_null_char) exit
+    text(i:i) = chars(i)
+  end do
+  end function
+
+!==============================================================================!
+  subroutine Tflows_Gpu_Check(rc, where)
+!------------------------------------------------------------------------------!
+  integer(c_int), intent(in) :: rc
+  character(*),   intent(in) :: where
+!==============================================================================!
+  if(rc /= 0) then
+    print '(a,a,a,i0,a,a)', ' # ', where, ' failed (', rc, '): ',           &
+                            trim(Tflows_Gpu_Error_Text())
+    stop 1
+  end if
+  end subroutine
+
+!==============================================================================!
+  subroutine Tflows_Gpu_Create(handle, ni, nt, a_row, a_col, a_dia)
+!------------------------------------------------------------------------------!
+!>  Uploads the CSR structure of Matrix_Type (Matrix_Mod.f90:57-76) once per
+!>  grid: a_row(nt+1), a_col(nnz), a_dia(nt), all 1-based as built by
+!>  Create_Matrix.f90:87-119.
+!------------------------------------------------------------------------------!
+  type(c_ptr),    intent(out) :: handle
+  integer,        intent(in)  :: ni, nt
+  integer(c_int), intent(in)  :: a_row(:), a_col(:), a_dia(:)
+!==============================================================================!
+  call Tflows_Gpu_Check(tfg_matrix_create(ni, nt, a_row(nt+1) - 1, a_row,    &
+                                          a_col, a_dia, handle),             &
+                        'tfg_matrix_create')
+  end subroutine
+
+!==============================================================================!
+  subroutine Tflows_Gpu_Solve(handle, solver, prec, a_val, x, b,            &
+                              miter, niter, tol, fin_res, norm)
+!------------------------------------------------------------------------------!
+!>  Solve_Native's call (Solve_Native.f90:2-46) on the GPU.  x(1:nt) holds
+!>  x0 on entry and the solution on exit; a_val and b are left unchanged.
+!------------------------------------------------------------------------------!
+  type(c_ptr),              intent(in)    :: handle
+  character(*),             intent(in)    :: solver, prec
+  real(c_double),           intent(in)    :: a_val(:)
+  real(c_double),           intent(inout) :: x(:), b(:)
+  integer,                  intent(in)    :: miter
+  integer,                  intent(out)   :: niter
+  real(c_double),           intent(in)    :: tol
+  real(c_double),           intent(out)   :: fin_res
+  real(c_double), optional, intent(in)    :: norm
+!-----------------------------------[Locals]-----------------------------------!
+  integer(c_int) :: n_out
+!==============================================================================!
+  call Tflows_Gpu_Check(tfg_solve_native(handle, solver, len(solver),        &
+                                         prec, len(prec), a_val, x, b,       &
+                                         miter, n_out, tol, fin_res, norm),  &
+                        'tfg_solve_native')
+  niter = n_out
+  end subroutine
+
+!==============================================================================!
+  subroutine Tflows_Gpu_Destroy(handle)
+!------------------------------------------------------------------------------!
+  type(c_ptr), intent(in) :: handle
+!==============================================================================!
+  call Tflows_Gpu_Check(tfg_matrix_destroy(handle), 'tfg_matrix_destroy')
+  end subroutine
+
+  end module

@@ -1,0 +1,64 @@
+"""The Fortran host binding (t-flows_amd/fortran: Tflows_Gpu_Mod, an
+iso_c_binding module keeping Solve_Native's argument list) driving the GPU
+solver, against the oracle and against the same solve through Python."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                      "t-flows_amd", "fortran", "solve_driver")
+
+
+def _write(d, s, x0):
+    with open(os.path.join(d, "dims.txt"), "w") as f:
+        f.write(f"{s.n} {s.n} {s.nnz}\n")
+    for name, a in (("row", s.row), ("col", s.col), ("dia", s.dia)):
+        a.astype(np.int32).tofile(os.path.join(d, f"{name}.bin"))
+    for name, a in (("val", s.val), ("b", s.b), ("x0", x0)):
+        a.astype(np.float64).tofile(os.path.join(d, f"{name}.bin"))
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no amdflang)")
+@pytest.mark.parametrize("solver,prec,norm", [("cg", "jacobi", None), ("bicg", "jacobi", 1.0),
+                                              ("cg", "none", None)])
+def test_fortran_driver(cases, oracle, gpu, tmp_path, solver, prec, norm):
+    s = cases.hex_pressure(20, 16, 12, rho_out=10.0)
+    x0 = np.zeros(s.n)
+    _write(str(tmp_path), s, x0)
+    args = [DRIVER, str(tmp_path), solver, prec, "2000", "1e-8"] + ([str(norm)] if norm else [])
+    print("running", " ".join(args), flush=True)
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    niter, fin_res = out.stdout.split()
+    niter, fin_res = int(niter), float(fin_res)
+    x = np.fromfile(os.path.join(str(tmp_path), "x.bin"), dtype=np.float64)
+    # the same call through the Python mirror gives the same bits
+    nat = gpu.Native(s.row, s.col, s.dia)
+    xp = x0.copy()
+    it_p, res_p = nat.solve_native(solver, prec, s.val.copy(), xp, s.b.copy(), 2000, 1e-8, norm=norm)
+    assert (niter, fin_res) == (it_p, res_p) and np.array_equal(x, xp)
+    # and agrees with the oracle: +-2 iterations, x within 1e-10 relative or
+    # twice the spread that reordering the oracle's sums causes by itself
+    A = oracle.Matrix(s.row, s.col, s.dia, s.val)
+    xo, xe = x0.copy(), x0.copy()
+    it_o, res_o = oracle.solve(solver, prec, A, xo, s.b.copy(), 2000, 1e-8, norm=norm)
+    oracle.solve(solver, prec, A, xe, s.b.copy(), 2000, 1e-8, norm=norm, sum_order=1)
+    env = np.linalg.norm(xe - xo) / np.linalg.norm(xo)
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print(f"  niter {niter} (oracle {it_o}), x rel err {err:.2e}, envelope {env:.2e}", flush=True)
+    assert abs(niter - it_o) <= 2
+    assert err <= max(1e-10, 2.0 * env)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no amdflang)")
+def test_fortran_unknown_solver_stops(cases, tmp_path):
+    """An unknown solver name is fatal, as Message % Error is in the reference."""
+    s = cases.hex_pressure(4)
+    _write(str(tmp_path), s, np.zeros(s.n))
+    out = subprocess.run([DRIVER, str(tmp_path), "gmres", "jacobi", "10", "1e-8"],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode != 0 and "Unknown native solver" in out.stdout
