@@ -90,6 +90,283 @@ def traffic_per_launch(kernel_prefix):
     return k["fetch_kib"] * 1024 * t["fetch_scale"] + k["write_kib"] * 1024 * t["write_scale"], TRAFFIC
 
 
+def _kernel_key(name):
+    """Short kernel name: 'void tfg::k_spmv_pq<...>(...)' -> 'k_spmv_pq'."""
+    s = name.split("tfg::", 1)[-1] if "tfg::" in name else name
+    for c in "<(":
+        s = s.split(c, 1)[0]
+    return s.strip()
+
+
+def _counter_means(path, counter):
+    """Mean value per dispatch of one rocprofv3 counter, per kernel, over the
+    dispatches that did work (launches after convergence return at once)."""
+    import csv
+
+    rows = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            rows.setdefault(_kernel_key(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+    out = {}
+    for k, v in rows.items():
+        v = np.asarray(v)
+        busy = v[v >= 0.5 * v.max()] if v.max() > 0 else v
+        out[k] = {"mean_kib": float(busy.mean()), "dispatches": int(busy.size),
+                  "idle_dispatches": int(v.size - busy.size)}
+    return out
+
+
+def summarize_counters(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, out_path, commands):
+    """Write the HBM-traffic summary bench.py reads: FETCH_SIZE / WRITE_SIZE
+    (KiB per dispatch) of every kernel of a solve, and the factors that map
+    each counter onto true bytes, from the copy kernel of known size (1 GiB
+    read + 1 GiB written per launch)."""
+    fetch, write = _counter_means(fetch_csv, "FETCH_SIZE"), _counter_means(write_csv, "WRITE_SIZE")
+    cf = _counter_means(cal_fetch_csv, "FETCH_SIZE")["k_copy16"]["mean_kib"]
+    cw = _counter_means(cal_write_csv, "WRITE_SIZE")["k_copy16"]["mean_kib"]
+    gib_kib = (1 << 30) / 1024.0
+    doc = {
+        "what": "HBM traffic per dispatch from rocprofv3 counters, 200^3 two-phase Jacobi-CG solve",
+        "commands": commands,
+        "fetch_scale": gib_kib / cf, "write_scale": gib_kib / cw,
+        "calibration": {"kernel": "k_copy16", "bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                        "fetch_kib": cf, "write_kib": cw},
+        "kernels": {k: {"fetch_kib": fetch[k]["mean_kib"],
+                        "write_kib": write.get(k, {}).get("mean_kib", 0.0),
+                        "dispatches": fetch[k]["dispatches"]} for k in sorted(fetch)},
+    }
+    with open(out_path, "w") as f:
+        json.dump(doc, f, indent=1)
+        f.write("\n")
+    print(json.dumps({k: round((v["fetch_kib"] * doc["fetch_scale"] + v["write_kib"] * doc["write_scale"])
+                               * 1024 / 1e6, 1) for k, v in doc["kernels"].items()}), flush=True)
+
+
+def summarize_stats(stats_csv, out_path, command):
+    """Markdown table of a rocprofv3 --stats kernel summary."""
+    import csv
+
+    with open(stats_csv) as f:
+        rows = list(csv.DictReader(f))
+    lines = [f"# Kernel statistics: {command['workload']}", "", "Command:", "", "```",
+             command["cmd"], "```", "",
+             "| kernel | calls | total ms | avg us | min us | max us | % |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        lines.append(f"| `{r['Name'][:110]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.1f} | "
+                     f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                     f"{float(r['MaxNs']) / 1e3:.1f} | {r['Percentage']} |")
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def _oracle_ms_per_iter(s, solver, prec, iters, criterion="relres"):
+    """Oracle time per iteration (1 thread) of a solver on system s."""
+    from oracle import oracle
+
+    A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
+    x = np.zeros(s.n)
+    t0 = time.perf_counter()
+    try:
+        oracle.solve(solver, prec, A, x, s.b.copy(), iters, 1e-300, criterion=criterion)
+    except ArithmeticError:
+        pass
+    return (time.perf_counter() - t0) / iters * 1e3
+
+
+def bench_upwind(args):
+    """Non-symmetric workload: 120^3 first-order upwind convection-diffusion
+    (1.73M cells), Jacobi, ||r||/||b|| < 1e-8, with the reference's 'bicg'
+    (the CG recurrence), textbook BiCG, CGS and BiCGSTAB."""
+    import tflows_cases
+    import tflows_gpu as G
+
+    N = 120 if args.n == 200 else args.n
+    s = tflows_cases.upwind(N)
+    nat = G.Native(s.row, s.col, s.dia)
+    dv, db = G.DeviceArray(s.val), G.DeviceArray(s.b)
+    dx = G.DeviceArray(np.zeros(s.n))
+    runs = {}
+    spmv_bytes = 12 * s.nnz + 20 * s.n + 4
+    for name, solver, textbook in (("reference_bicg", "bicg", 0), ("bicg", "bicg", 1),
+                                   ("cgs", "cgs", 0), ("bicgstab", "bcgs", 0)):
+        opt = G.Options()
+        opt.criterion = G.CRIT_RELRES
+        opt.timing = 1
+        opt.true_residual = 1
+        opt.textbook_bicg = textbook
+        best = None
+        for _ in range(args.steps + args.warmup):
+            dx.upload(np.zeros(s.n))
+            t = time.perf_counter()
+            try:
+                it, res, rep = nat.solve_native_device(solver, "jacobi", dv, dx, db, 2000, 1e-8,
+                                                       options=opt, report=True)
+                status = "ok"
+            except G.TFlowsGpuError as e:
+                it, res, rep, status = -1, float("nan"), {"true_relres": float("nan"),
+                                                          "spmv_ms": 0.0}, str(e)
+            dt = time.perf_counter() - t
+            if best is None or dt < best[0]:
+                best = (dt, it, res, rep, status)
+        dt, it, res, rep, status = best
+        runs[name] = {"seconds": dt, "iterations": it, "relres": res,
+                      "true_relres": rep["true_relres"], "status": status,
+                      "converged": bool(it >= 0 and res < 1e-8),
+                      "spmv_GBps": spmv_bytes / (rep["spmv_ms"] * 1e-3) / 1e9 if rep["spmv_ms"] else None}
+        log(f"bench upwind{N}: {name}: {dt:.3f} s, {it} iterations, relres {res:.2e}, "
+            f"true {rep['true_relres']:.2e} ({status})")
+    cpu = None
+    if not args.no_cpu:
+        cpu = {k: _oracle_ms_per_iter(s, sv, "jacobi", 20) for k, sv in
+               (("reference_bicg", "bicg"), ("bicg", "bicg_textbook"), ("cgs", "cgs"),
+                ("bicgstab", "bcgs"))}
+    best_name = min((k for k in runs if runs[k]["converged"]), key=lambda k: runs[k]["seconds"])
+    out = {
+        "metric": f"time to 1e-8 rel. residual, {N}^3 upwind convection-diffusion (non-symmetric)",
+        "value": runs[best_name]["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"upwind convection-diffusion {N}^3, Jacobi, relres 1e-8",
+                   "n": s.n, "nnz": s.nnz, "fastest": best_name},
+        "runs": runs,
+        "cpu_baseline": None if cpu is None else {
+            "unit": "ms per iteration", "cores": 1, "kind": "port",
+            "sample": "oracle (C restatement), 20 iterations of each method, 1 thread",
+            "ms_per_iteration": cpu},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_knn_ir(args):
+    """Unstructured workload: 4M-point kNN Laplacian (jittered 200x200x100
+    lattice, 10..18 neighbours, random permutation), Jacobi CG to relres
+    1e-10 in FP64, and FP32 sweeps (restart every 50) inside FP64
+    refinement; error of each against an FP64 CG solution at relres 1e-13."""
+    import tflows_cases
+    import tflows_gpu as G
+
+    t0 = time.perf_counter()
+    s = tflows_cases.knn(200, 200, 100)
+    log(f"bench knn: n={s.n} nnz={s.nnz} generated in {time.perf_counter() - t0:.1f} s")
+    nat = G.Native(s.row, s.col, s.dia)
+    dv, db = G.DeviceArray(s.val), G.DeviceArray(s.b)
+    dx = G.DeviceArray(np.zeros(s.n))
+
+    def run(precision, tol, inner_max=0):
+        opt = G.Options()
+        opt.precision = precision
+        opt.criterion = G.CRIT_RELRES
+        opt.true_residual = 1
+        opt.inner_max = inner_max
+        best = None
+        for _ in range(args.steps + args.warmup):
+            dx.upload(np.zeros(s.n))
+            t = time.perf_counter()
+            it, res, rep = nat.solve_native_device("cg", "jacobi", dv, dx, db, 100000, tol,
+                                                   options=opt, report=True)
+            dt = time.perf_counter() - t
+            if best is None or dt < best[0]:
+                best = (dt, it, res, rep, dx.download())
+        return best
+
+    ref = run(G.FP64, 1e-13)
+    out_runs = {}
+    for name, prec, tol, im in (("fp64_cg", G.FP64, 1e-10, 0), ("fp32_ir", G.FP32_IR, 1e-10, 50),
+                                ("fp32_ir_tighter", G.FP32_IR, 1e-12, 50)):
+        dt, it, res, rep, x = run(prec, tol, im)
+        err = float(np.linalg.norm(x - ref[4]) / np.linalg.norm(ref[4]))
+        out_runs[name] = {"seconds": dt, "iterations": it, "refinements": rep["refinements"],
+                          "relres": res, "true_relres": rep["true_relres"],
+                          "error_vs_fp64_solution": err}
+        log(f"bench knn: {name} tol {tol}: {dt:.3f} s, {it} iterations, true {rep['true_relres']:.2e}, "
+            f"error {err:.2e}")
+    out = {
+        "metric": "time to 1e-10 rel. residual, 4M-point kNN Laplacian, FP32 sweeps in FP64 refinement vs FP64 CG",
+        "value": out_runs["fp32_ir"]["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "kNN 200x200x100 jittered lattice, k 10..18, permuted; Jacobi; relres 1e-10",
+                   "n": s.n, "nnz": s.nnz},
+        "fp32_ir_speedup_vs_fp64": out_runs["fp64_cg"]["seconds"] / out_runs["fp32_ir"]["seconds"],
+        "reference_solution": {"tol": 1e-13, "iterations": ref[1], "seconds": ref[0]},
+        "runs": out_runs,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """Batched workload: 256 two-phase pressure systems, sides U{20..80},
+    density ratios logU(1, 1000), own seeded RHS each; FP64 Jacobi CG to
+    relres 1e-8 per system, solved as one batch (masks, hipGraph chunks) and
+    one after another through the single-system call."""
+    import tflows_cases
+    import tflows_gpu as G
+
+    count = 256
+    sides, ratios = tflows_cases.batch_params(count, 20, 80, 1000.0)
+    t0 = time.perf_counter()
+    systems = [tflows_cases.hex_pressure(int(sd), rho_out=float(rt), seed=7000 + k)
+               for k, (sd, rt) in enumerate(zip(sides, ratios))]
+    total = sum(s.n for s in systems)
+    log(f"bench batch: {count} systems, {total} cells, {sum(s.nnz for s in systems)} nonzeros "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
+    bt = G.Batch([(s.row, s.col, s.dia) for s in systems])
+    val = np.concatenate([s.val for s in systems])
+    b = np.concatenate([s.b for s in systems])
+    dv, db = G.DeviceArray(val), G.DeviceArray(b)
+    dx = G.DeviceArray(np.zeros(total))
+    opt = G.Options()
+    opt.criterion = G.CRIT_RELRES
+    best = None
+    for _ in range(args.steps + args.warmup):
+        dx.upload(np.zeros(total))
+        t = time.perf_counter()
+        niter, fin, rep = bt.solve("cg", "jacobi", dv, dx, db, 20000, 1e-8, options=opt,
+                                   device=True)
+        dt = time.perf_counter() - t
+        if best is None or dt < best[0]:
+            best = (dt, niter.copy(), fin.copy(), rep, bt.times())
+    dt, niter, fin, rep, done_ms = best
+    log(f"bench batch: {dt:.3f} s wall, device {rep['solve_ms']:.1f} ms, max niter {niter.max()}")
+    # one after another through the single-system drop-in (device-resident)
+    seq_ms, seq_it = [], []
+    for k, s in enumerate(systems):
+        nat = G.Native(s.row, s.col, s.dia)
+        sv, sb, sx = G.DeviceArray(s.val), G.DeviceArray(s.b), G.DeviceArray(np.zeros(s.n))
+        o = G.Options()
+        o.criterion = G.CRIT_RELRES
+        it, res, r = nat.solve_native_device("cg", "jacobi", sv, sx, sb, 20000, 1e-8, options=o,
+                                             report=True)
+        seq_ms.append(r["solve_ms"])
+        seq_it.append(it)
+        nat.close()
+    seq_ms = np.array(seq_ms)
+    seq_done = np.cumsum(seq_ms)
+    out = {
+        "metric": "batched solves/s: 256 two-phase pressure systems (sides 20..80), FP64 Jacobi CG to relres 1e-8",
+        "value": count / (rep["solve_ms"] * 1e-3), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "256 hex pressure systems, sides U{20..80}, rho ratio logU(1,1000)",
+                   "cells": int(total), "nnz": int(sum(s.nnz for s in systems)),
+                   "blocks": bt.info()["blocks"], "max_iterations": int(niter.max()),
+                   "all_converged": bool(np.all(fin < 1e-8))},
+        "batch": {"device_ms": rep["solve_ms"], "wall_s": dt,
+                  "p50_system_ms": float(np.percentile(done_ms, 50)),
+                  "p99_system_ms": float(np.percentile(done_ms, 99))},
+        "sequential": {"device_ms": float(seq_ms.sum()),
+                       "solves_per_s": count / (seq_ms.sum() * 1e-3),
+                       "p50_system_ms": float(np.percentile(seq_ms, 50)),
+                       "p99_system_ms": float(np.percentile(seq_ms, 99)),
+                       "p50_completion_ms": float(np.percentile(seq_done, 50)),
+                       "p99_completion_ms": float(np.percentile(seq_done, 99)),
+                       "same_iterations": bool(np.all(np.abs(np.array(seq_it) - niter) <= 2))},
+        "speedup_vs_sequential": float(seq_ms.sum() / rep["solve_ms"]),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def calibrate():
     """Known-traffic copy (1 GiB read + 1 GiB write) for counter calibration."""
     import tflows_gpu as G
@@ -108,17 +385,37 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=200, help="grid side (default 200 -> 8M cells)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the known-traffic copy kernel (counter calibration)")
+    ap.add_argument("--workload", default="twophase200",
+                    choices=["twophase200", "upwind120", "knn4m", "batch256"],
+                    help="the headline solve (default) or one of the other measured workloads")
+    ap.add_argument("--summarize-counters", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
+                    help="write the traffic summary from four rocprofv3 counter_collection.csv files")
+    ap.add_argument("--summarize-stats", metavar="KERNEL_STATS_CSV",
+                    help="write a markdown table of a rocprofv3 kernel_stats.csv")
+    ap.add_argument("--out", help="output path of --summarize-*")
+    ap.add_argument("--command", default="", help="command line recorded in the summary")
     args = ap.parse_args()
     if args.calibrate:
         calibrate()
+        return
+    if args.summarize_counters:
+        summarize_counters(*args.summarize_counters, args.out or TRAFFIC, args.command.split(" ;; "))
+        return
+    if args.summarize_stats:
+        summarize_stats(args.summarize_stats, args.out,
+                        {"workload": "200^3 two-phase Jacobi-CG FP64 solve (bench.py --steps 1 --warmup 0)",
+                         "cmd": args.command})
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
               "(the solver is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "twophase200":
+        {"upwind120": bench_upwind, "knn4m": bench_knn_ir, "batch256": bench_batch}[args.workload](args)
+        return
 
     import tflows_cases
     import tflows_gpu as G

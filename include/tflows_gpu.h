@@ -49,6 +49,8 @@ typedef struct {
   double inner_tol;   /* FP32_IR: inner residual reduction per step (0: 1e-4)  */
   int timing;         /* 1: time sampled SpMV launches with hipEvents          */
   int true_residual;  /* 1: recompute ||b - A x||_2/||b||_2 in FP64 at exit    */
+  int textbook_bicg;  /* 1: "bicg" is textbook BiCG (with A^T); 0: the        */
+                      /* reference's Bicg.f90, which equals CG                 */
 } tfg_options;
 
 typedef struct {
@@ -74,6 +76,10 @@ int tfg_matrix_destroy(tfg_matrix* handle);
 
 /* Drop-in for Solve_Native (Native_Mod/Solve_Native.f90:2-46), host arrays.
  * solver: "cg" | "bicg" (Fortran blank padding allowed; len < 0: strlen);
+ * beyond the reference, the textbook methods of its PETSc path
+ * (Petsc_Interface.cpp:416-436): "cgs", "bcgs" (alias "bicgstab"), and
+ * textbook "bicg" via tfg_options.textbook_bicg -- FP64, jacobi or none, with
+ * the same normalisation, stopping test and niter/fin_res conventions;
  * prec: "jacobi" | "icc" | anything else = none (Prec_Form.f90:47-89).
  * val(nnz), x(nt) = x0 on entry / solution on exit, b(nt).  val and b are
  * left unchanged (the reference scales and de-scales them in place,
@@ -123,10 +129,13 @@ int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, d
 /* SpMV engines: 0 LDS-staged rows (2048 nonzeros per block), 1 the same
  * with 4096, 2/3/4 sub-wavefront rows with 4/8/16 lanes per row, 5 the
  * 2048 stage with the next block's loads in flight.  All give bit-identical
- * products.  The handle picks the fastest by timing them on the first product
- * or solve (autotuner); tfg_spmv_tune re-runs it on d_val (us[6] receives
- * each engine's mean launch time), tfg_spmv_select forces an engine (-1: tune
- * again on next use). */
+ * products; the engine sets the SpMV launch grid and so the tree of the dot
+ * products fused into it.  Default: engine 5, so that a solve is bitwise
+ * reproducible from run to run.  The autotuner is opt-in: tfg_spmv_tune
+ * times every engine on d_val (us[6] receives each mean launch time) and
+ * keeps the fastest; TFG_SPMV_AUTOTUNE=1 in the environment makes the first
+ * product or solve of a handle do the same; tfg_spmv_select forces an engine
+ * (-1: default choice again on next use). */
 int tfg_spmv_tune(tfg_matrix* h, const double* d_val, int* kind, float* us);
 int tfg_spmv_select(tfg_matrix* h, int kind);
 
@@ -134,6 +143,37 @@ int tfg_spmv_select(tfg_matrix* h, int kind);
  * grid, and the number of 2048-nonzero row blocks. */
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind,
                     int* spmv_grid, int* stage_blocks);
+
+/* ---- batched solves ------------------------------------------------------
+ * No counterpart in the reference, which solves one system per Solve_Native
+ * call: nsys independent systems solved together, each with exactly the
+ * semantics of tfg_solve_native_ex (its own normalisation, stopping test,
+ * niter and fin_res).  Systems that stop are masked and dropped from the
+ * launches; a chunk of iterations is replayed as a hipGraph (opt->path =
+ * TFG_PATH_KERNELS launches kernels one by one instead).  FP64, "cg"/"bicg",
+ * jacobi or none. */
+typedef struct tfg_batch tfg_batch;
+
+/* n[s], nnz[s]: rows and nonzeros of system s; row, col, dia: the systems'
+ * 1-based Matrix_Type arrays concatenated system after system (row holds
+ * n[s]+1 entries per system). */
+int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row,
+                     const int* col, const int* dia, tfg_batch** handle);
+int tfg_batch_destroy(tfg_batch* handle);
+
+/* val, x, b: concatenated like the structure (host or device pointers);
+ * niter[nsys], fin_res[nsys] out; norm: NULL or nsys values.  rep (may be
+ * NULL): iterations = the largest niter, solve_ms = device time of the batch. */
+int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len,
+                    const char* prec, int prec_len, const double* val, double* x,
+                    double* b, int miter, int* niter, double tol, double* fin_res,
+                    const double* norm, int device_ptrs, const tfg_options* opt,
+                    tfg_report* rep);
+
+/* ms[nsys]: time from the start of the last batched solve to the stop of
+ * each system (device clock). */
+int tfg_batch_times(tfg_batch* h, double* ms);
+int tfg_batch_info(tfg_batch* h, int* nsys, int* n, int* nnz, int* blocks);
 
 const char* tfg_last_error(void);
 int tfg_version(void);

@@ -26,10 +26,10 @@
 #define OR_TINY 1.0e-15
 
 /* Summation order of the global reductions.  0 (default) is the reference's
- * serial left-to-right order.  1 sums blocks of 64 terms left to right and
- * combines the block sums pairwise: an equally valid parallel order, used
- * only to measure how far rounding alone moves a solve (the envelope the
- * GPU's own order is judged against).  Not part of the reference. */
+ * serial left-to-right order.  1 (2) sums blocks of 64 (1000) terms left to
+ * right and combines the block sums pairwise: equally valid parallel orders,
+ * used only to measure how far rounding alone moves a solve (the envelope
+ * the GPU's own order is judged against).  Not part of the reference. */
 static int g_sum_order = 0;
 
 void or_set_sum_order(int order) { g_sum_order = order; }
@@ -43,11 +43,12 @@ static double pairwise(const double* s, int n) {
 /* sum_i f(i) in the blocked order; kind: 0 a*b, 1 (a*a)/(b*b), 2 a */
 static double blocked(const double* a, const double* b, int n, int kind) {
   if (n <= 0) return 0.0;
-  int nb = (n + 63) / 64;
+  const int bs = g_sum_order == 2 ? 1000 : 64;
+  int nb = (n + bs - 1) / bs;
   double* part = (double*)malloc(sizeof(double) * (size_t)nb);
   for (int k = 0; k < nb; k++) {
     double s = 0.0;
-    for (int i = 64 * k; i < n && i < 64 * k + 64; i++) {
+    for (int i = bs * k; i < n && i < bs * k + bs; i++) {
       double t = kind == 0 ? a[i] * b[i] : (kind == 1 ? (a[i] * a[i]) / (b[i] * b[i]) : a[i]);
       s = s + t;
     }
@@ -387,4 +388,195 @@ done:
   *niter = iter;
   free(p1); free(q1); free(r1); free(p2); free(q2); free(r2); free(d); free(d_inv);
   return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * Textbook Krylov methods for non-symmetric systems (not in Native_Mod).
+ *
+ * The reference's "bicg" is the CG recurrence (or_bicg above).  These are
+ * the methods its PETSc path offers by name (KSPBICG, KSPCGS, KSPBCGS,
+ * Sources/Process/Cpu/Petsc_Interface.cpp:416-436), restated from Barrett et
+ * al., "Templates for the Solution of Linear Systems", SIAM 1994: BiCG
+ * (Fig. 2.8), CGS (Fig. 2.9) and BiCGSTAB (Fig. 2.10), each with the
+ * preconditioner M applied on the left as Templates writes it.  They share
+ * Cg's prologue, stopping test and iteration conventions: normalisation by
+ * 1/mean(diag) (Cg.f90:62-84), M from Prec_Form (jacobi or none; the Jacobi
+ * M is symmetric, so M^-T = M^-1), the two early exits (Cg.f90:95-123), the
+ * stopping measure after every iteration on the recursive residual with the
+ * updated x, niter = miter when the loop runs out.  The shadow residual is
+ * r~0 = r0.  Breakdown (a zero rho, a zero denominator, omega = 0 or a
+ * non-finite measure) returns 1 with niter = the failing iteration.
+ *
+ * A^T p is formed row by row on the symmetric sparsity pattern of the face
+ * matrices: (A^T p)_i = 0 + sum over row i's entries j (CSR order) of
+ * a(col_j, i) * p(col_j), the transposed value found in row col_j.
+ * ------------------------------------------------------------------------ */
+
+/* tpos[j] = C position of the transposed entry of entry j; -1 if the
+ * pattern is not symmetric. */
+int or_transpose_positions(const or_matrix* A, int* tpos) {
+  for (int i = 0; i < A->nt; i++) {
+    for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) {
+      int k = A->col[j] - 1, found = -1;
+      for (int m = A->row[k] - 1; m < A->row[k + 1] - 1; m++)
+        if (A->col[m] - 1 == i) { found = m; break; }
+      if (found < 0) return -1;
+      tpos[j] = found;
+    }
+  }
+  return 0;
+}
+
+static void spmv_t(const or_matrix* A, int ni, const int* tpos, const double* p, double* q) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ni; i++) {
+    double qi = 0.0;
+    for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) {
+      int k = A->col[j] - 1;
+      qi = qi + A->val[tpos[j]] * p[k];
+    }
+    q[i] = qi;
+  }
+}
+
+static double dot(const double* a, const double* b, int n) {
+  if (g_sum_order) return blocked(a, b, n, 0);
+  double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+  for (int i = 0; i < n; i++) s = s + a[i] * b[i];
+  return s;
+}
+
+int or_krylov(or_matrix* A, double* x, double* b, int method, int prec, int miter,
+              int* niter, double tol, double* fin_res, const double* norm, int crit,
+              double* omegas) {
+  const int nt = A->nt, ni = A->ni;
+  const size_t nv = 11;
+  double* w = (double*)calloc(nv * (size_t)nt, sizeof(double));
+  int* tpos = method == OR_BICG_TRUE ? (int*)malloc(sizeof(int) * (size_t)(A->row[nt] - 1)) : NULL;
+  if (!w || (method == OR_BICG_TRUE && !tpos)) { free(w); free(tpos); return -2; }
+  if (tpos && or_transpose_positions(A, tpos) != 0) { free(w); free(tpos); return -3; }
+  double *r = w, *rt = w + nt, *p = w + 2 * (size_t)nt, *q = w + 3 * (size_t)nt,
+         *z = w + 4 * (size_t)nt, *u = w + 5 * (size_t)nt, *v = w + 6 * (size_t)nt,
+         *t = w + 7 * (size_t)nt, *y = w + 8 * (size_t)nt, *d = w + 9 * (size_t)nt,
+         *d_inv = w + 10 * (size_t)nt;
+  double res = 0.0, rho = 0.0, rho_old = 1.0, alfa = 0.0, omega = 1.0, beta, bnrm2;
+  int iter, status = 0;
+  if (prec == OR_PREC_ICC) prec = -1; /* ICC: not offered for these methods */
+  if (prec < 0) { free(w); free(tpos); return -4; }
+
+  double fn = normalise(A, b);
+  or_prec_form(A, ni, d, d_inv, prec);
+  double bnorm = sqrt(sumsq(b, ni));
+  if (crit == OR_CRIT_RELRES)
+    bnrm2 = bnorm > 0.0 ? 1.0 : 0.0;
+  else
+    bnrm2 = or_nrms(A, ni, b, x, norm);
+  if (bnrm2 < tol) { iter = 0; goto done; }
+  or_residual(A, ni, r, b, x);
+  res = measure(A, ni, r, x, norm, crit, bnorm);
+  if (res < tol) { iter = 0; goto done; }
+  for (int i = 0; i < ni; i++) rt[i] = r[i];     /* r~0 = r0 */
+
+  for (iter = 1; iter <= miter; iter++) {
+    if (method == OR_BICG_TRUE) {                 /* Templates Fig. 2.8 */
+      /* z = M^-1 r (in z), z~ = M^-T r~ (in y) */
+      or_prec_solve(A, ni, d, d_inv, z, r, prec);
+      or_prec_solve(A, ni, d, d_inv, y, rt, prec);
+      rho = dot(z, rt, ni);
+      if (rho == 0.0) { status = 1; goto done; }
+      if (iter == 1) {
+        for (int i = 0; i < ni; i++) { p[i] = z[i]; u[i] = y[i]; }   /* u: p~ */
+      } else {
+        beta = rho / rho_old;
+        for (int i = 0; i < ni; i++) {
+          p[i] = z[i] + beta * p[i];
+          u[i] = y[i] + beta * u[i];
+        }
+      }
+      or_spmv(A, ni, p, q);                       /* q = A p     */
+      spmv_t(A, ni, tpos, u, v);                  /* v = A^T p~  */
+      double den = dot(u, q, ni);
+      if (den == 0.0) { status = 1; goto done; }
+      alfa = rho / den;
+      for (int i = 0; i < ni; i++) {
+        x[i] = x[i] + alfa * p[i];
+        r[i] = r[i] - alfa * q[i];
+        rt[i] = rt[i] - alfa * v[i];
+      }
+    } else if (method == OR_CGS) {                /* Templates Fig. 2.9 */
+      rho = dot(rt, r, ni);
+      if (rho == 0.0) { status = 1; goto done; }
+      if (iter == 1) {
+        for (int i = 0; i < ni; i++) { u[i] = r[i]; p[i] = u[i]; }
+      } else {
+        beta = rho / rho_old;
+        for (int i = 0; i < ni; i++) {
+          u[i] = r[i] + beta * q[i];
+          p[i] = u[i] + beta * (q[i] + beta * p[i]);
+        }
+      }
+      or_prec_solve(A, ni, d, d_inv, z, p, prec); /* p^ (in z) */
+      or_spmv(A, ni, z, v);                       /* v^ = A p^ */
+      double den = dot(rt, v, ni);
+      if (den == 0.0) { status = 1; goto done; }
+      alfa = rho / den;
+      for (int i = 0; i < ni; i++) {
+        q[i] = u[i] - alfa * v[i];
+        y[i] = u[i] + q[i];
+      }
+      or_prec_solve(A, ni, d, d_inv, t, y, prec); /* u^ = M^-1 (u + q) (in t) */
+      for (int i = 0; i < ni; i++) x[i] = x[i] + alfa * t[i];
+      or_spmv(A, ni, t, y);                       /* q^ = A u^ (in y) */
+      for (int i = 0; i < ni; i++) r[i] = r[i] - alfa * y[i];
+    } else {                                      /* BiCGSTAB, Fig. 2.10 */
+      rho = dot(rt, r, ni);
+      if (rho == 0.0) { status = 1; goto done; }
+      if (iter == 1) {
+        for (int i = 0; i < ni; i++) p[i] = r[i];
+      } else {
+        beta = (rho / rho_old) * (alfa / omega);
+        for (int i = 0; i < ni; i++) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+      }
+      or_prec_solve(A, ni, d, d_inv, z, p, prec); /* p^ (in z) */
+      or_spmv(A, ni, z, v);                       /* v = A p^ */
+      double den = dot(rt, v, ni);
+      if (den == 0.0) { status = 1; goto done; }
+      alfa = rho / den;
+      for (int i = 0; i < ni; i++) {
+        u[i] = r[i] - alfa * v[i];                /* s (in u) */
+        q[i] = x[i] + alfa * z[i];                /* x + alpha p^ (in q) */
+      }
+      res = measure(A, ni, u, q, norm, crit, bnorm);
+      if (!isfinite(res)) { status = 1; goto done; }
+      if (res < tol) {                            /* early exit on s */
+        for (int i = 0; i < ni; i++) { x[i] = q[i]; r[i] = u[i]; }
+        goto done;
+      }
+      or_prec_solve(A, ni, d, d_inv, y, u, prec); /* s^ (in y) */
+      or_spmv(A, ni, y, t);                       /* t = A s^ */
+      double tt = dot(t, t, ni);
+      if (tt == 0.0) { status = 1; goto done; }
+      omega = dot(t, u, ni) / tt;
+      if (omegas) omegas[iter - 1] = omega;
+      for (int i = 0; i < ni; i++) {
+        x[i] = q[i] + omega * y[i];
+        r[i] = u[i] - omega * t[i];
+      }
+    }
+    res = measure(A, ni, r, x, norm, crit, bnorm);
+    if (!isfinite(res)) { status = 1; goto done; }
+    if (res < tol) goto done;
+    if (method == OR_BCGS && omega == 0.0) { status = 1; goto done; }
+    rho_old = rho;
+  }
+  iter = iter - 1;
+
+done:
+  denormalise(A, b, fn);
+  *fin_res = res;
+  *niter = iter;
+  free(w);
+  free(tpos);
+  return status;
 }

@@ -52,8 +52,20 @@ int or_cg_ex(or_matrix* A, double* x, double* b, int prec, int miter,
              int* niter, double tol, double* fin_res, const double* norm,
              int criterion);
 
-/* 0: the reference's serial summation (default); 1: blocked-pairwise
- * summation of every global reduction, to measure rounding sensitivity. */
+/* Textbook BiCG / CGS / BiCGSTAB (Templates, Barrett et al. 1994) with Cg's
+ * prologue, stopping test and conventions; prec: jacobi or none.  Returns 0,
+ * 1 on breakdown (niter = failing iteration), -2 out of memory, -3 pattern
+ * not symmetric (BiCG needs A^T on the same pattern), -4 unsupported prec.
+ * omegas (may be NULL) receives BiCGSTAB's omega of every iteration. */
+enum { OR_BICG_TRUE = 1, OR_CGS = 2, OR_BCGS = 3 };
+int or_krylov(or_matrix* A, double* x, double* b, int method, int prec, int miter,
+              int* niter, double tol, double* fin_res, const double* norm, int crit,
+              double* omegas);
+int or_transpose_positions(const or_matrix* A, int* tpos);
+
+/* 0: the reference's serial summation (default); 1, 2: blocked-pairwise
+ * summation (blocks of 64, 1000) of every global reduction, to measure
+ * rounding sensitivity. */
 void or_set_sum_order(int order);
 
 #ifdef __cplusplus

@@ -9,6 +9,9 @@ Each wrapper mirrors one reference routine (Sources/Process/Cpu/Native_Mod/):
   residual         Residual_Vector.f90:2-44
   spmv             the product loop of Cg.f90:177-185
   prec_form/solve  Prec_Form.f90:2-93 / Prec_Solve.f90:2-107
+  krylov           textbook BiCG / CGS / BiCGSTAB (Templates, Barrett et al.
+                   1994) with Cg's prologue and conventions -- the methods the
+                   reference's PETSc path offers (Petsc_Interface.cpp:416-436)
 Index arrays keep the reference's 1-based values; vectors are plain arrays of
 length nt (Fortran element i is Python element i-1).
 """
@@ -20,6 +23,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PREC = {"none": 0, "jacobi": 1, "icc": 2}
 CRIT = {"nrms": 0, "relres": 1}
+KRYLOV = {"bicg_textbook": 1, "cgs": 2, "bcgs": 3, "bicgstab": 3}
 
 
 class _Mat(ctypes.Structure):
@@ -52,6 +56,10 @@ def lib(omp=False):
         L.or_cg_ex.restype = I
         L.or_cg_ex.argtypes = [P, P, P, I, I, P, D, P, P, I]
         L.or_set_sum_order.argtypes = [I]
+        L.or_krylov.restype = I
+        L.or_krylov.argtypes = [P, P, P, I, I, I, P, D, P, P, I, P]
+        L.or_transpose_positions.restype = I
+        L.or_transpose_positions.argtypes = [P, P]
         _libs[name] = L
     return _libs[name]
 
@@ -125,9 +133,18 @@ def prec_solve(A, d, dinv, b, prec):
     return x
 
 
+class BreakdownError(ArithmeticError):
+    """A textbook method broke down (zero rho or denominator, omega = 0)."""
+
+    def __init__(self, niter, fin_res):
+        super().__init__(f"Krylov breakdown at iteration {niter}")
+        self.niter, self.fin_res = niter, fin_res
+
+
 def solve(solver, prec, A, x, b, miter, tol, norm=None, criterion="nrms", omp=False,
-          sum_order=0):
-    """Solve_Native.f90:2-46 restated: dispatches 'cg' / 'bicg'.
+          sum_order=0, omegas=None):
+    """Solve_Native.f90:2-46 restated: dispatches 'cg' / 'bicg', plus the
+    textbook methods 'bicg_textbook', 'cgs', 'bcgs' ('bicgstab').
 
     x and b are updated in place like the reference's intent(out)/(inout)
     arguments.  Returns (niter, fin_res).  sum_order=1 switches every global
@@ -142,12 +159,26 @@ def solve(solver, prec, A, x, b, miter, tol, norm=None, criterion="nrms", omp=Fa
     L = lib(omp)
     L.or_set_sum_order(int(sum_order))
     try:
-        return _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin)
+        return _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin, omegas)
     finally:
         L.or_set_sum_order(0)
 
 
-def _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin):
+def _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin, omegas=None):
+    if solver in KRYLOV:
+        if prec not in ("jacobi", "none"):
+            raise ValueError(f"{solver}: preconditioner {prec} not offered")
+        om = _p(omegas) if omegas is not None else None
+        rc = L.or_krylov(A.ref, _p(x), _p(b), KRYLOV[solver], PREC[prec], int(miter),
+                         ctypes.byref(niter), float(tol), ctypes.byref(fin), nref,
+                         CRIT[criterion], om)
+        if rc == 1:
+            raise BreakdownError(niter.value, fin.value)
+        if rc == -3:
+            raise ValueError("textbook BiCG needs a symmetric sparsity pattern")
+        if rc != 0:
+            raise MemoryError("oracle scratch allocation failed")
+        return niter.value, fin.value
     if criterion != "nrms":
         if solver not in ("cg", "bicg"):
             raise ValueError(f"Unknown native solver: {solver}")
@@ -165,3 +196,10 @@ def _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin):
     if rc != 0:
         raise MemoryError("oracle scratch allocation failed")
     return niter.value, fin.value
+
+
+def transpose_positions(A):
+    """0-based position of the transposed entry of every entry (symmetric
+    sparsity pattern), or None when the pattern is not symmetric."""
+    t = np.zeros(len(A.col), dtype=np.int32)
+    return t if lib().or_transpose_positions(A.ref, _p(t)) == 0 else None

@@ -17,7 +17,8 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["csrc/tfg_api.hip", "csrc/tfg_kernels.hip", "csrc/tfg_ir.hip"]
+HIP_SOURCES = ["csrc/tfg_api.hip", "csrc/tfg_kernels.hip", "csrc/tfg_ir.hip",
+               "csrc/tfg_nonsym.hip", "csrc/tfg_batch.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # every product/update is rounded on its own, as in the reference's

@@ -14,9 +14,62 @@
 #include <string>
 #include <vector>
 
-#include "tfg_internal.h"
+#include "tfg_host.h"
 
 using tfg::State;
+using tfg::fail;
+using tfg::fstring;
+using tfg::make_blocks;
+
+namespace tfg {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+std::string fstring(const char* s, int len) {
+  if (!s) return std::string();
+  size_t n = len < 0 ? strlen(s) : (size_t)len;
+  while (n > 0 && (s[n - 1] == ' ' || s[n - 1] == '\0')) --n;
+  return std::string(s, n);
+}
+
+void make_blocks(const int* row1, int n, int cap, std::vector<int>& blk) {
+  blk.clear();
+  blk.push_back(0);
+  int r = 0;
+  while (r < n) {
+    int start = r;
+    long long s = row1[r];
+    while (r < n && r - start < kRowsPerBlock && (long long)row1[r + 1] - s <= cap) ++r;
+    if (r == start) ++r;  // long row
+    blk.push_back(r);
+  }
+}
+
+int check_csr(int nt, int nnz, const int* row, const int* col, const int* dia) {
+  if (row[0] != 1) return fail(TFG_EINVAL, "row(1) must be 1");
+  if ((long long)row[nt] - 1 != nnz || nnz < nt)
+    return fail(TFG_EINVAL, "nnz=%d does not match row(nt+1)-1=%d", nnz, row[nt] - 1);
+  for (int i = 0; i < nt; ++i) {
+    if (row[i + 1] <= row[i]) return fail(TFG_EINVAL, "row %d is empty or row not increasing", i + 1);
+    if (dia[i] != row[i] || col[row[i] - 1] != i + 1)
+      return fail(TFG_EINVAL, "diagonal must be the first entry of row %d", i + 1);
+  }
+  for (long long j = 0; j < nnz; ++j)
+    if (col[j] < 1 || col[j] > nt) return fail(TFG_EINVAL, "col(%lld)=%d out of range", j + 1, col[j]);
+  return TFG_OK;
+}
+
+}  // namespace tfg
 
 struct tfg_matrix {
   int device = 0;
@@ -54,12 +107,19 @@ struct tfg_matrix {
   float* p32 = nullptr;
   float* q32 = nullptr;
   float* ad32 = nullptr;
+  // textbook BiCG / CGS / BiCGSTAB work (allocated on first use)
+  double* rt = nullptr;
+  double* nv[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* valt = nullptr;
+  int* tpos = nullptr;
+  int tpos_state = 0;  // 0 not built, 1 built, -1 the pattern is not symmetric
 };
 
 namespace {
 
-// Rows of the partials buffer: diag 1, prologue 7, spmv 1, update 5, aux 2.
-constexpr int kPartRows = 16;
+// Rows of the partials buffer: diag 1, prologue 7, spmv 1, update 5, aux 2,
+// ext 6.
+constexpr int kPartRows = 22;
 
 tfg::Partials partials_of(const tfg_matrix* h, const tfg::Launch& L) {
   tfg::Partials pt;
@@ -69,53 +129,13 @@ tfg::Partials partials_of(const tfg_matrix* h, const tfg::Launch& L) {
   pt.spmv = pt.pro + 7 * (size_t)h->pstride;
   pt.upd = pt.spmv + 1 * (size_t)h->pstride;
   pt.aux = pt.upd + 5 * (size_t)h->pstride;
+  pt.ext = pt.aux + 2 * (size_t)h->pstride;
   pt.g_vec = L.grid_vec;
   pt.g_spmv = L.grid_spmv;
+  pt.g_upd = L.grid_vec;
   return pt;
 }
 
-thread_local std::string g_err;
-
-int fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                                 \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess)                                                             \
-      return fail(e_ == hipErrorOutOfMemory ? TFG_ENOMEM : TFG_EHIP, "%s: %s (%s:%d)", \
-                  #expr, hipGetErrorString(e_), __FILE__, __LINE__);                  \
-  } while (0)
-
-// Fortran CHARACTER comparison ignores trailing blanks.
-std::string fstring(const char* s, int len) {
-  if (!s) return std::string();
-  size_t n = len < 0 ? strlen(s) : (size_t)len;
-  while (n > 0 && (s[n - 1] == ' ' || s[n - 1] == '\0')) --n;
-  return std::string(s, n);
-}
-
-// Greedy row blocks: at most 256 rows and at most cap nonzeros each; a single
-// longer row forms a block of its own.
-void make_blocks(const int* row1, int n, int cap, std::vector<int>& blk) {
-  blk.clear();
-  blk.push_back(0);
-  int r = 0;
-  while (r < n) {
-    int start = r;
-    long long s = row1[r];
-    while (r < n && r - start < tfg::kRowsPerBlock && (long long)row1[r + 1] - s <= cap) ++r;
-    if (r == start) ++r;  // long row
-    blk.push_back(r);
-  }
-}
 
 // Rows a workgroup of each engine covers per pass (stage: one block of up to
 // 256 rows; wave: 256 / L rows).
@@ -198,7 +218,8 @@ void free_all(tfg_matrix* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* dev[] = {h->row, h->col, h->blk[0], h->blk[1], h->val, h->b, h->x, h->r, h->p, h->q, h->ad,
                  h->part, h->st, h->st_in, h->val32, h->e32, h->r32,
-                 h->p32, h->q32, h->ad32};
+                 h->p32, h->q32, h->ad32, h->rt, h->nv[0], h->nv[1], h->nv[2],
+                 h->nv[3], h->valt, h->tpos};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   if (h->hst) (void)hipHostFree(h->hst);
@@ -210,12 +231,26 @@ void free_all(tfg_matrix* h) {
   delete h;
 }
 
-// Autotuner: times every SpMV engine on this matrix (plain product with the
-// caller's values, 2 warm-up + 5 timed launches each) and keeps the fastest.
-// The choice depends on the sparsity structure only and is made once per
-// handle; TFG_SPMV_KIND=<0..5> in the environment forces an engine.
+// Engine choice.  By default the pipelined 2048-nonzero stream engine (the
+// fastest on every T-Flows-like structure measured).  The choice fixes the
+// SpMV launch grid and with it the tree of the fused dot products, so a
+// timed choice could differ between two runs on the same matrix and move
+// the last bits of the result; timing is therefore opt-in: tfg_spmv_tune(),
+// or TFG_SPMV_AUTOTUNE=1 in the environment (times every engine, 2 warm-up +
+// 5 timed launches each, keeps the fastest).  TFG_SPMV_KIND=<0..5> forces one.
+int tune_all(tfg_matrix* h, const double* val);
+
 int ensure_tuned(tfg_matrix* h, const double* val) {
   if (h->kind >= 0) return TFG_OK;
+  const char* at = getenv("TFG_SPMV_AUTOTUNE");
+  if (!(at && *at && *at != '0')) {
+    h->kind = tfg::K_PIPE2048;
+    return TFG_OK;
+  }
+  return tune_all(h, val);
+}
+
+int tune_all(tfg_matrix* h, const double* val) {
   HIP_TRY(hipMemsetAsync(h->p, 0, sizeof(double) * (size_t)h->nt, h->stream));
   float best = 1e30f;
   int best_k = tfg::K_STAGE2048;
@@ -236,6 +271,34 @@ int ensure_tuned(tfg_matrix* h, const double* val) {
     }
   }
   h->kind = best_k;
+  return TFG_OK;
+}
+
+// Work vectors of the textbook methods; for BiCG also the transposed-entry
+// map (built once per handle on the device) and the transposed values.
+int ensure_nonsym(tfg_matrix* h, int method, const double* val, const tfg::Launch& L) {
+  const size_t n = (size_t)h->nt;
+  if (!h->rt) HIP_TRY(hipMalloc(&h->rt, n * sizeof(double)));
+  for (double*& v : h->nv)
+    if (!v) HIP_TRY(hipMalloc(&v, n * sizeof(double)));
+  if (method != tfg::NS_BICG) return TFG_OK;
+  if (h->tpos_state == 0) {
+    HIP_TRY(hipMalloc(&h->tpos, sizeof(int) * (size_t)h->nnz));
+    int* bad = nullptr;
+    HIP_TRY(hipMalloc(&bad, sizeof(int)));
+    int hb = 0;
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(int), h->stream);
+    if (e == hipSuccess) e = tfg::launch_transpose_pos(h->stream, h->nt, h->row, h->col, h->tpos, bad);
+    if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(bad);
+    HIP_TRY(e);
+    h->tpos_state = hb ? -1 : 1;
+  }
+  if (h->tpos_state < 0)
+    return fail(TFG_ENOTSUP, "textbook BiCG needs a structurally symmetric matrix");
+  if (!h->valt) HIP_TRY(hipMalloc(&h->valt, sizeof(double) * (size_t)h->nnz));
+  HIP_TRY(tfg::launch_gather_valt(L, h->nnz, h->tpos, val, h->valt));
   return TFG_OK;
 }
 
@@ -433,7 +496,7 @@ int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int mi
 // ===========================================================================
 extern "C" {
 
-const char* tfg_last_error(void) { return g_err.c_str(); }
+const char* tfg_last_error(void) { return tfg::g_err.c_str(); }
 
 int tfg_version(void) { return 1; }
 
@@ -444,16 +507,7 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
   if (nt < 1 || ni < 1 || ni > nt) return fail(TFG_EINVAL, "bad sizes ni=%d nt=%d", ni, nt);
   if (ni != nt)
     return fail(TFG_ENOTSUP, "buffer cells (ni=%d < nt=%d) need halo exchange hooks", ni, nt);
-  if (row[0] != 1) return fail(TFG_EINVAL, "row(1) must be 1");
-  if ((long long)row[nt] - 1 != nnz || nnz < nt)
-    return fail(TFG_EINVAL, "nnz=%d does not match row(nt+1)-1=%d", nnz, row[nt] - 1);
-  for (int i = 0; i < nt; ++i) {
-    if (row[i + 1] <= row[i]) return fail(TFG_EINVAL, "row %d is empty or row not increasing", i + 1);
-    if (dia[i] != row[i] || col[row[i] - 1] != i + 1)
-      return fail(TFG_EINVAL, "diagonal must be the first entry of row %d", i + 1);
-  }
-  for (long long j = 0; j < nnz; ++j)
-    if (col[j] < 1 || col[j] > nt) return fail(TFG_EINVAL, "col(%lld)=%d out of range", j + 1, col[j]);
+  { const int rc = tfg::check_csr(nt, nnz, row, col, dia); if (rc != TFG_OK) return rc; }
 
   tfg_matrix* h = new tfg_matrix();
   h->ni = ni;
@@ -533,8 +587,7 @@ int tfg_spmv_select(tfg_matrix* h, int kind) {
 int tfg_spmv_tune(tfg_matrix* h, const double* d_val, int* kind, float* us) {
   if (!h || !d_val) return fail(TFG_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
-  h->kind = -1;
-  int rc = ensure_tuned(h, d_val);
+  int rc = tune_all(h, d_val);
   if (rc != TFG_OK) return rc;
   if (kind) *kind = h->kind;
   if (us)
@@ -559,16 +612,25 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
   // applies A and M (not their transposes) to the shadow vectors and starts
   // them equal to the primal ones (Bicg.f90:129-133,143-195), so its iterates
   // are bit-identical to Cg's; it runs the CG recurrence here.
+  // Beyond the reference: the textbook methods under their PETSc names
+  // ('cgs', 'bcgs' / 'bicgstab'), and textbook BiCG for 'bicg' when
+  // opt->textbook_bicg is set.
+  tfg_options o{};
+  if (opt) o = *opt;
   const std::string sv = fstring(solver, solver_len);
-  if (sv != "cg" && sv != "bicg")
+  int method = tfg::NS_NONE;
+  if (sv == "cgs") method = tfg::NS_CGS;
+  else if (sv == "bcgs" || sv == "bicgstab") method = tfg::NS_BCGS;
+  else if (sv == "bicg" && o.textbook_bicg) method = tfg::NS_BICG;
+  else if (sv != "cg" && sv != "bicg")
     return fail(TFG_EINVAL, "Unknown native solver: %s", sv.c_str());
   // Prec_Form.f90:47-89: 'jacobi', 'icc', anything else = none.
   const std::string pv = fstring(prec, prec_len);
   const int pr = pv == "jacobi" ? tfg::PREC_JACOBI : (pv == "icc" ? tfg::PREC_ICC : tfg::PREC_NONE);
   if (pr == tfg::PREC_ICC)
     return fail(TFG_ENOTSUP, "preconditioner 'icc' is not available on the GPU path yet");
-  tfg_options o{};
-  if (opt) o = *opt;
+  if (method != tfg::NS_NONE && o.precision != TFG_FP64)
+    return fail(TFG_ENOTSUP, "%s runs in FP64 only", sv.c_str());
   if (o.criterion != TFG_CRIT_NRMS && o.criterion != TFG_CRIT_RELRES)
     return fail(TFG_EINVAL, "unknown criterion %d", o.criterion);
   if (o.precision != TFG_FP64 && o.precision != TFG_FP32_IR)
@@ -616,6 +678,18 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     a.ad = h->ad;
     a.pt = partials_of(h, L);
     a.st = h->st;
+    if (method == tfg::NS_CGS) a.pt.g_upd = L.grid_spmv;
+    tfg::NsArgs ns{};
+    if (method != tfg::NS_NONE) {
+      const int rn = ensure_nonsym(h, method, dval, L);
+      if (rn != TFG_OK) return rn;
+      ns.valt = h->valt;
+      ns.rt = h->rt;
+      ns.v1 = h->nv[0];
+      ns.v2 = h->nv[1];
+      ns.v3 = h->nv[2];
+      ns.v4 = h->nv[3];
+    }
 
     State* s0 = &h->hst[0];
     memset(s0, 0, sizeof(State));
@@ -636,7 +710,9 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
     HIP_TRY(tfg::launch_diag_sum(L, a));
     HIP_TRY(tfg::launch_prologue(L, a, pr));
-    HIP_TRY(tfg::launch_start(L, a));
+    // rho_1: (z, r0) for CG and BiCG, (r~0, r0) = (r0, r0) for CGS / BiCGSTAB
+    HIP_TRY(tfg::launch_start(L, a, (method == tfg::NS_CGS || method == tfg::NS_BCGS) ? 5 : 6));
+    if (method != tfg::NS_NONE) HIP_TRY(tfg::launch_ns_init(L, a, ns));
 
     const int chunk = h->nnz > (1 << 22) ? 8 : 32;
     int k = 1, slot = 0, prev = -1;
@@ -644,17 +720,21 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     while (k <= miter && !stop) {
       const int kend = std::min(miter, k + chunk - 1);
       for (; k <= kend; ++k) {
-        HIP_TRY(tfg::launch_pupdate(L, a, pr, k));
         const bool smp = timing && (k % 4 == 1) && (int)sample_iter.size() < max_samples;
-        if (smp) HIP_TRY(hipEventRecord(h->ev_sample[2 * sample_iter.size()], h->stream));
-        HIP_TRY(tfg::launch_spmv_pq(L, a, k));
-        if (smp) {
-          HIP_TRY(hipEventRecord(h->ev_sample[2 * sample_iter.size() + 1], h->stream));
-          sample_iter.push_back(k);
+        hipEvent_t e0 = smp ? h->ev_sample[2 * sample_iter.size()] : nullptr;
+        hipEvent_t e1 = smp ? h->ev_sample[2 * sample_iter.size() + 1] : nullptr;
+        if (method != tfg::NS_NONE) {
+          HIP_TRY(tfg::launch_ns_iteration(L, a, ns, method, pr, k, e0, e1));
+        } else {
+          HIP_TRY(tfg::launch_pupdate(L, a, pr, k));
+          if (smp) HIP_TRY(hipEventRecord(e0, h->stream));
+          HIP_TRY(tfg::launch_spmv_pq(L, a, k));
+          if (smp) HIP_TRY(hipEventRecord(e1, h->stream));
+          HIP_TRY(tfg::launch_update(L, a, pr, k));
         }
-        HIP_TRY(tfg::launch_update(L, a, pr, k));
+        if (smp) sample_iter.push_back(k);
       }
-      HIP_TRY(tfg::launch_finish(L, a, kend));
+      HIP_TRY(tfg::launch_finish(L, a, kend, method == tfg::NS_BCGS ? 1 : 0));
       HIP_TRY(hipMemcpyAsync(&h->hst[1 + slot], h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
       HIP_TRY(hipEventRecord(h->ev_poll[slot], h->stream));
       if (prev >= 0) {

@@ -1,0 +1,666 @@
+// Batched FP64 CG: many independent Solve_Native calls in one pass per
+// phase.  No counterpart in the reference (which solves one system per
+// call); each system gets exactly the single-system semantics (its own
+// 1/mean(diag) normalisation, stopping test and niter / fin_res conventions,
+// Cg.f90:62-253), so a batch equals that many Solve_Native calls.
+//
+// Layout: the systems' CSR arrays are concatenated into one block-diagonal
+// matrix (global 0-based rows and columns).  Row blocks (<= 256 rows,
+// <= 2048 nonzeros) never straddle two systems.  Every block-level kernel
+// runs one workgroup per block of the ACTIVE list (blocks of systems not yet
+// stopped) and publishes one partial per block; a one-workgroup-per-system
+// kernel then sweeps that system's partials in a fixed order and writes its
+// scalars.  A system that stops is masked at once (its workgroups return)
+// and removed from the active list at the next poll, so late iterations
+// only launch the systems still running.  Per iteration:
+//   pupdate (blocks) | spmv + (p,q) (blocks) | alpha (systems)
+//   | update + sums (blocks) | check (systems: stop test, beta)
+// The kernels read their iteration number from the system's state, so a
+// chunk of iterations is captured once as a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "tfg_host.h"
+#include "tfg_spmv.h"
+
+namespace tfg {
+
+namespace {
+
+// Partial rows of the batch (each nblk long).
+enum { BP_DIAG = 0, BP_PRO = 1, BP_SPMV = 8, BP_UPD = 9, BP_ROWS = 14 };
+
+struct BatchArgs {
+  int nsys, nblk, nact;
+  SpmvShape m;              // concatenated CSR; m.blk = all row blocks
+  const int* bsys;          // system of each block
+  const int* sblk;          // nsys+1: first block of each system
+  const int* srow;          // nsys+1: first row of each system
+  const int* act;           // active blocks
+  const double* val;
+  const double* b;
+  double* x;
+  double* r;
+  double* p;
+  double* q;
+  double* ad;
+  double* part;             // [BP_ROWS][nblk]
+  State* st;                // one per system
+  unsigned long long* clk;  // [0] start; [1 + s] stop of system s (device clock)
+};
+
+__device__ __forceinline__ double bmeasure(const State* st, double nrms, double xmin,
+                                           double xmax, double rr) {
+  if (st->criterion == TFG_CRIT_RELRES) return sqrt(rr) / st->bnorm;
+  const double range = st->has_norm ? fmax(st->norm - 0.0, kTiny) : fmax(xmax - xmin, kTiny);
+  return sqrt(nrms) / range;
+}
+
+// Block of this workgroup in the active list, dealt to the XCDs in
+// contiguous ranges (workgroup g runs on XCD g % 8); -1 past the list.
+__device__ __forceinline__ int active_block(const BatchArgs& a) {
+  const int g = blockIdx.x, x = g % kXcds, l = g / kXcds;
+  const int first = (int)(((long long)a.nact * x) / kXcds);
+  const int end = (int)(((long long)a.nact * (x + 1)) / kXcds);
+  return first + l < end ? a.act[first + l] : -1;
+}
+
+// Sum (min, max) of one system's block partials, fixed order; s_out[k].
+template <int NV>
+__device__ __forceinline__ void sys_gather(const BatchArgs& a, int row0, int s, const int (&ops)[NV],
+                                           double* s_tmp, double* s_out) {
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = red_identity(ops[k]);
+  for (int bl = a.sblk[s] + (int)threadIdx.x; bl < a.sblk[s + 1]; bl += kBlock) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      acc[k] = red_apply(ops[k], acc[k], a.part[(size_t)(row0 + k) * a.nblk + bl]);
+  }
+  block_reduce<NV>(acc, ops, s_tmp, s_out);
+}
+
+// One partial per block: totals of the workgroup at part[(row0+k)*nblk + bl].
+template <int NV>
+__device__ __forceinline__ void block_publish(const BatchArgs& a, int row0, int bl, double (&v)[NV],
+                                              const int (&ops)[NV], double* s_tmp, double* s_out) {
+  block_reduce<NV>(v, ops, s_tmp, s_out);
+  if (threadIdx.x < NV) a.part[(size_t)(row0 + threadIdx.x) * a.nblk + bl] = s_out[threadIdx.x];
+}
+
+__device__ __forceinline__ void stamp(const BatchArgs& a, int s) {
+  a.clk[1 + s] = wall_clock64();
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_bt_clock(BatchArgs a) {
+  if (threadIdx.x == 0) a.clk[0] = wall_clock64();
+}
+
+// sum of the diagonal per block                              Cg.f90:62-73
+__global__ __launch_bounds__(kBlock) void k_bt_diag(BatchArgs a) {
+  __shared__ double s_tmp[4], s_out[1];
+  const int bl = blockIdx.x;
+  const int r0 = a.m.blk[bl], r1 = a.m.blk[bl + 1];
+  double v[1] = {0.0};
+  if (r0 + (int)threadIdx.x < r1) v[0] = a.val[a.m.row[r0 + threadIdx.x]];
+  const int ops[1] = {OP_SUM};
+  block_publish<1>(a, BP_DIAG, bl, v, ops, s_tmp, s_out);
+}
+
+// fn = 1 / (sum_a / n) per system (Cg.f90:75-76), in its fixed block order.
+// (A block's diagonal sum above runs as a tree; the system total is the same
+// fixed tree for every run.)
+__global__ __launch_bounds__(kBlock) void k_bt_fn(BatchArgs a) {
+  __shared__ double s_tmp[4], s_out[1];
+  const int s = blockIdx.x;
+  const int ops[1] = {OP_SUM};
+  sys_gather<1>(a, BP_DIAG, s, ops, s_tmp, s_out);
+  if (threadIdx.x == 0) {
+    const double n = (double)(a.srow[s + 1] - a.srow[s]);
+    a.st[s].fn = 1.0 / (s_out[0] / n);
+  }
+}
+
+// r = b fn - (A fn) x, the scaled diagonal and the 7 start sums per block
+template <class E, int PREC>
+__global__ __launch_bounds__(kBlock) void k_bt_prologue(BatchArgs a) {
+  __shared__ typename E::Smem sm;
+  __shared__ double s_tmp[7 * 4], s_out[7];
+  const int bl = blockIdx.x;
+  const double fn = a.st[a.bsys[bl]].fn;
+  double v[7] = {0.0, 0.0, kHuge, -kHuge, 0.0, 0.0, 0.0};
+  E::template one<true>(
+      sm, a.m, a.val, fn, a.x, s_tmp, s_out, bl, [&](int i) { return a.b[i] * fn; },
+      [&](int i, double ri, double xi, double di) {
+        const double bi = a.b[i] * fn;
+        const double z = PREC == PREC_JACOBI ? ri * (1.0 / di) : ri;
+        a.r[i] = ri;
+        a.ad[i] = di;
+        v[0] = v[0] + (bi * bi) / (di * di);
+        v[1] = v[1] + (ri * ri) / (di * di);
+        v[2] = fmin(v[2], xi);
+        v[3] = fmax(v[3], xi);
+        v[4] = v[4] + bi * bi;
+        v[5] = v[5] + ri * ri;
+        v[6] = v[6] + z * ri;
+      });
+  const int ops[7] = {OP_SUM, OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM, OP_SUM};
+  block_publish<7>(a, BP_PRO, bl, v, ops, s_tmp, s_out);
+}
+
+// early exits of Cg.f90:95-123 and rho_1, per system
+__global__ __launch_bounds__(kBlock) void k_bt_start(BatchArgs a) {
+  __shared__ double s_tmp[7 * 4], s_out[7];
+  const int s = blockIdx.x;
+  const int ops[7] = {OP_SUM, OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM, OP_SUM};
+  sys_gather<7>(a, BP_PRO, s, ops, s_tmp, s_out);
+  if (threadIdx.x != 0) return;
+  State* st = &a.st[s];
+  const double range = st->has_norm ? fmax(st->norm - 0.0, kTiny) : fmax(s_out[3] - s_out[2], kTiny);
+  const double bnorm = sqrt(s_out[4]);
+  double bn, res0;
+  if (st->criterion == TFG_CRIT_RELRES) {
+    bn = bnorm > 0.0 ? 1.0 : 0.0;
+    res0 = sqrt(s_out[5]) / bnorm;
+  } else {
+    bn = sqrt(s_out[0]) / range;
+    res0 = sqrt(s_out[1]) / range;
+  }
+  st->bnorm = bnorm;
+  st->iter = 0;
+  st->niter = 0;
+  st->status = TFG_OK;
+  if (bn < st->tol) {
+    st->res = 0.0;
+    st->done = 1;
+    stamp(a, s);
+  } else if (res0 < st->tol) {
+    st->res = res0;
+    st->done = 1;
+    stamp(a, s);
+  } else {
+    st->res = res0;
+    st->rho[1] = s_out[6];
+    st->done = 0;
+  }
+}
+
+// p = z + beta p (p = z in a system's first iteration)      Cg.f90:145-171
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_bt_pupdate(BatchArgs a) {
+  const int bl = active_block(a);
+  if (bl < 0) return;
+  const State* st = &a.st[a.bsys[bl]];
+  if (st->done) return;
+  const int i = a.m.blk[bl] + threadIdx.x;
+  if (i >= a.m.blk[bl + 1]) return;
+  const double z = PREC == PREC_JACOBI ? a.r[i] * (1.0 / a.ad[i]) : a.r[i];
+  a.p[i] = st->iter == 0 ? z : z + st->aux1 * a.p[i];
+}
+
+// q = (A fn) p and the block's (p, q)                       Cg.f90:176-195
+template <class E>
+__global__ __launch_bounds__(kBlock) void k_bt_spmv(BatchArgs a) {
+  __shared__ typename E::Smem sm;
+  __shared__ double s_tmp[4], s_out[1];
+  const int bl = active_block(a);
+  if (bl < 0) return;
+  const State* st = &a.st[a.bsys[bl]];
+  if (st->done) return;
+  const double fn = st->fn;
+  double v[1] = {0.0};
+  E::template one<false>(
+      sm, a.m, a.val, fn, a.p, s_tmp, s_out, bl, [](int) { return 0.0; },
+      [&](int i, double qi, double pi, double) {
+        a.q[i] = qi;
+        v[0] = v[0] + pi * qi;
+      });
+  const int ops[1] = {OP_SUM};
+  block_publish<1>(a, BP_SPMV, bl, v, ops, s_tmp, s_out);
+}
+
+// alpha = rho / (p, q) per system                            Cg.f90:190-198
+__global__ __launch_bounds__(kBlock) void k_bt_alpha(BatchArgs a) {
+  __shared__ double s_tmp[4], s_out[1];
+  const int s = blockIdx.x;
+  State* st = &a.st[s];
+  if (st->done) return;
+  const int ops[1] = {OP_SUM};
+  sys_gather<1>(a, BP_SPMV, s, ops, s_tmp, s_out);
+  if (threadIdx.x != 0) return;
+  const int k = st->iter + 1;
+  const double pq = s_out[0];
+  if (pq == 0.0 || !isfinite(pq)) {
+    st->status = TFG_EBREAKDOWN;
+    st->niter = k;
+    st->done = 1;
+    stamp(a, s);
+    return;
+  }
+  st->alpha = st->rho[k & 1] / pq;
+}
+
+// x += alpha p, r -= alpha q and the block's stopping sums  Cg.f90:204-218
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_bt_update(BatchArgs a) {
+  __shared__ double s_tmp[5 * 4], s_out[5];
+  const int bl = active_block(a);
+  if (bl < 0) return;
+  const State* st = &a.st[a.bsys[bl]];
+  if (st->done) return;
+  const double alpha = st->alpha;
+  const int i = a.m.blk[bl] + threadIdx.x;
+  double v[5] = {0.0, kHuge, -kHuge, 0.0, 0.0};
+  if (i < a.m.blk[bl + 1]) {
+    const double d = a.ad[i];
+    const double xv = a.x[i] + alpha * a.p[i];
+    const double rv = a.r[i] - alpha * a.q[i];
+    a.x[i] = xv;
+    a.r[i] = rv;
+    const double z = PREC == PREC_JACOBI ? rv * (1.0 / d) : rv;
+    v[0] = (rv * rv) / (d * d);
+    v[1] = xv;
+    v[2] = xv;
+    v[3] = z * rv;
+    v[4] = rv * rv;
+  }
+  const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
+  block_publish<5>(a, BP_UPD, bl, v, ops, s_tmp, s_out);
+}
+
+// stopping test of iteration k (Cg.f90:214-222), next rho and beta
+__global__ __launch_bounds__(kBlock) void k_bt_check(BatchArgs a) {
+  __shared__ double s_tmp[5 * 4], s_out[5];
+  const int s = blockIdx.x;
+  State* st = &a.st[s];
+  if (st->done) return;
+  const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
+  sys_gather<5>(a, BP_UPD, s, ops, s_tmp, s_out);
+  if (threadIdx.x != 0) return;
+  const int k = st->iter + 1;
+  const double res = bmeasure(st, s_out[0], s_out[1], s_out[2], s_out[4]);
+  st->res = res;
+  st->iter = k;
+  if (!isfinite(res)) {
+    st->status = TFG_EBREAKDOWN;
+    st->niter = k;
+    st->done = 1;
+    stamp(a, s);
+  } else if (res < st->tol) {
+    st->niter = k;
+    st->done = 1;
+    stamp(a, s);
+  } else {
+    st->rho[(k + 1) & 1] = s_out[3];
+    st->aux1 = s_out[3] / st->rho[k & 1];
+  }
+}
+
+}  // namespace tfg
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+struct tfg_batch {
+  int device = 0, nsys = 0, nt = 0, nnz = 0, nblk = 0, cu_count = 0;
+  hipStream_t stream = nullptr;
+  std::vector<int> h_srow, h_sblk;  // host copies of the system offsets
+  int *row = nullptr, *col = nullptr, *blk = nullptr, *bsys = nullptr, *sblk = nullptr,
+      *srow = nullptr, *act = nullptr;
+  double *val = nullptr, *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr,
+         *ad = nullptr, *part = nullptr;
+  tfg::State* st = nullptr;
+  tfg::State* hst = nullptr;          // pinned, 2 x nsys (polling, double-buffered)
+  unsigned long long* clk = nullptr;  // device, nsys + 1
+  std::vector<double> done_ms;        // stop time of each system, last solve
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_poll[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+using tfg::fail;
+
+void batch_free(tfg_batch* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  void* dev[] = {h->row, h->col, h->blk, h->bsys, h->sblk, h->srow, h->act, h->val, h->b,
+                 h->x, h->r, h->p, h->q, h->ad, h->part, h->st, h->clk};
+  for (void* d : dev)
+    if (d) (void)hipFree(d);
+  if (h->hst) (void)hipHostFree(h->hst);
+  for (hipEvent_t e : {h->ev0, h->ev1, h->ev_poll[0], h->ev_poll[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, const int* col,
+                     const int* dia, tfg_batch** handle) {
+  if (!handle || !n || !nnz || !row || !col || !dia) return fail(TFG_EINVAL, "null argument");
+  *handle = nullptr;
+  if (nsys < 1) return fail(TFG_EINVAL, "nsys=%d", nsys);
+  long long NT = 0, NNZ = 0;
+  for (int s = 0; s < nsys; ++s) {
+    if (n[s] < 1 || nnz[s] < n[s]) return fail(TFG_EINVAL, "system %d: n=%d nnz=%d", s, n[s], nnz[s]);
+    NT += n[s];
+    NNZ += nnz[s];
+  }
+  if (NT > INT_MAX / 2 || NNZ > INT_MAX - 64) return fail(TFG_EINVAL, "batch too large for int32 indices");
+  // validate every system and build the global arrays
+  std::vector<int> grow((size_t)NT + 1), gcol((size_t)NNZ), blk, bsys, sblk, srow;
+  {
+    long long ro = 0, co = 0, rpos = 0, dpos = 0;
+    std::vector<int> b1;
+    blk.push_back(0);
+    sblk.push_back(0);
+    srow.push_back(0);
+    for (int s = 0; s < nsys; ++s) {
+      const int* rs = row + rpos;
+      const int rc = tfg::check_csr(n[s], nnz[s], rs, col + co, dia + dpos);
+      if (rc != TFG_OK) return fail(rc, "system %d: %s", s, tfg_last_error());
+      for (int i = 0; i < n[s]; ++i) grow[ro + i] = (int)(co + rs[i] - 1);
+      for (int j = 0; j < nnz[s]; ++j) gcol[co + j] = (int)(ro + col[co + j] - 1);
+      tfg::make_blocks(rs, n[s], 2048, b1);
+      for (size_t t = 1; t < b1.size(); ++t) {
+        blk.push_back((int)(ro + b1[t]));
+        bsys.push_back(s);
+      }
+      sblk.push_back((int)bsys.size());
+      ro += n[s];
+      co += nnz[s];
+      srow.push_back((int)ro);
+      rpos += n[s] + 1;
+      dpos += n[s];
+    }
+    grow[NT] = (int)NNZ;
+  }
+  tfg_batch* h = new tfg_batch();
+  h->nsys = nsys;
+  h->nt = (int)NT;
+  h->nnz = (int)NNZ;
+  h->nblk = (int)bsys.size();
+  h->h_srow = srow;
+  h->h_sblk = sblk;
+  h->done_ms.assign(nsys, 0.0);
+  int rc = TFG_OK;
+  do {
+    hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount, h->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    const size_t N = (size_t)NT;
+    auto dmal = [&](void** p, size_t bytes) { if (e == hipSuccess) e = hipMalloc(p, bytes); };
+    dmal((void**)&h->row, sizeof(int) * (N + 1));
+    dmal((void**)&h->col, sizeof(int) * (size_t)NNZ);
+    dmal((void**)&h->blk, sizeof(int) * blk.size());
+    dmal((void**)&h->bsys, sizeof(int) * bsys.size());
+    dmal((void**)&h->sblk, sizeof(int) * sblk.size());
+    dmal((void**)&h->srow, sizeof(int) * srow.size());
+    dmal((void**)&h->act, sizeof(int) * bsys.size());
+    for (double** v : {&h->x, &h->r, &h->p, &h->q, &h->ad}) dmal((void**)v, sizeof(double) * N);
+    dmal((void**)&h->part, sizeof(double) * (size_t)tfg::BP_ROWS * bsys.size());
+    dmal((void**)&h->st, sizeof(tfg::State) * (size_t)nsys);
+    dmal((void**)&h->clk, sizeof(unsigned long long) * (size_t)(nsys + 1));
+    if (e != hipSuccess) { rc = fail(TFG_ENOMEM, "batch allocation: %s", hipGetErrorString(e)); break; }
+    e = hipHostMalloc((void**)&h->hst, 2 * sizeof(tfg::State) * (size_t)nsys, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev_poll[0]);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev_poll[1]);
+    auto up = [&](void* d, const void* s, size_t bytes) {
+      if (e == hipSuccess) e = hipMemcpy(d, s, bytes, hipMemcpyHostToDevice);
+    };
+    up(h->row, grow.data(), sizeof(int) * (N + 1));
+    up(h->col, gcol.data(), sizeof(int) * (size_t)NNZ);
+    up(h->blk, blk.data(), sizeof(int) * blk.size());
+    up(h->bsys, bsys.data(), sizeof(int) * bsys.size());
+    up(h->sblk, sblk.data(), sizeof(int) * sblk.size());
+    up(h->srow, srow.data(), sizeof(int) * srow.size());
+    if (e != hipSuccess) { rc = fail(TFG_EHIP, "batch upload: %s", hipGetErrorString(e)); break; }
+  } while (0);
+  if (rc != TFG_OK) {
+    batch_free(h);
+    return rc;
+  }
+  *handle = h;
+  return TFG_OK;
+}
+
+int tfg_batch_destroy(tfg_batch* h) {
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  batch_free(h);
+  return TFG_OK;
+}
+
+int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len, const char* prec,
+                    int prec_len, const double* val, double* x, double* b, int miter, int* niter,
+                    double tol, double* fin_res, const double* norm, int device_ptrs,
+                    const tfg_options* opt, tfg_report* rep) {
+  using namespace tfg;
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  if (!val || !x || !b || !niter || !fin_res) return fail(TFG_EINVAL, "null argument");
+  const std::string sv = fstring(solver, solver_len);
+  if (sv != "cg" && sv != "bicg") return fail(TFG_EINVAL, "Unknown native solver: %s", sv.c_str());
+  const std::string pv = fstring(prec, prec_len);
+  const int pr = pv == "jacobi" ? PREC_JACOBI : (pv == "icc" ? PREC_ICC : PREC_NONE);
+  if (pr == PREC_ICC) return fail(TFG_ENOTSUP, "preconditioner 'icc' is not available on the GPU path yet");
+  tfg_options o{};
+  if (opt) o = *opt;
+  if (o.criterion != TFG_CRIT_NRMS && o.criterion != TFG_CRIT_RELRES)
+    return fail(TFG_EINVAL, "unknown criterion %d", o.criterion);
+  if (o.precision != TFG_FP64) return fail(TFG_ENOTSUP, "batched solves run in FP64");
+  if (rep) {
+    memset(rep, 0, sizeof(*rep));
+    rep->true_relres = -1.0;
+  }
+  HIP_TRY(hipSetDevice(h->device));
+  const size_t N = (size_t)h->nt;
+  const double* dval = val;
+  const double* db = b;
+  if (!device_ptrs) {
+    if (!h->val) HIP_TRY(hipMalloc(&h->val, sizeof(double) * (size_t)h->nnz));
+    if (!h->b) HIP_TRY(hipMalloc(&h->b, sizeof(double) * N));
+    HIP_TRY(hipMemcpyAsync(h->val, val, sizeof(double) * (size_t)h->nnz, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->b, b, sizeof(double) * N, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->x, x, sizeof(double) * N, hipMemcpyHostToDevice, h->stream));
+    dval = h->val;
+    db = h->b;
+  } else {
+    HIP_TRY(hipMemcpyAsync(h->x, x, sizeof(double) * N, hipMemcpyDeviceToDevice, h->stream));
+  }
+  // per-system state
+  for (int s = 0; s < h->nsys; ++s) {
+    State& q = h->hst[s];
+    memset(&q, 0, sizeof(State));
+    q.tol = tol;
+    q.criterion = o.criterion;
+    q.has_norm = norm ? 1 : 0;
+    q.norm = norm ? norm[s] : 0.0;
+  }
+  HIP_TRY(hipMemcpyAsync(h->st, h->hst, sizeof(State) * (size_t)h->nsys, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+
+  BatchArgs a{};
+  a.nsys = h->nsys;
+  a.nblk = h->nblk;
+  a.m.n = h->nt;
+  a.m.row = h->row;
+  a.m.col = h->col;
+  a.m.blk = h->blk;
+  a.m.nblk = h->nblk;
+  a.bsys = h->bsys;
+  a.sblk = h->sblk;
+  a.srow = h->srow;
+  a.act = h->act;
+  a.val = dval;
+  a.b = db;
+  a.x = h->x;
+  a.r = h->r;
+  a.p = h->p;
+  a.q = h->q;
+  a.ad = h->ad;
+  a.part = h->part;
+  a.st = h->st;
+  a.clk = h->clk;
+  const bool vec = ((uintptr_t)dval % 16) == 0 && h->nnz >= 4;
+  const hipStream_t S = h->stream;
+  const int B = kBlock;
+
+#define TFG_BT_ENGINE(CALL)                                                 \
+  do {                                                                      \
+    if (vec) { using E = OneBlock<2048, double>; CALL; }               \
+    else { using E = ScalarStage<2048, double>; CALL; }                     \
+  } while (0)
+
+  HIP_TRY(hipEventRecord(h->ev0, S));
+  k_bt_clock<<<1, B, 0, S>>>(a);
+  k_bt_diag<<<h->nblk, B, 0, S>>>(a);
+  k_bt_fn<<<h->nsys, B, 0, S>>>(a);
+  if (pr == PREC_JACOBI) TFG_BT_ENGINE((k_bt_prologue<E, PREC_JACOBI><<<h->nblk, B, 0, S>>>(a)));
+  else TFG_BT_ENGINE((k_bt_prologue<E, PREC_NONE><<<h->nblk, B, 0, S>>>(a)));
+  k_bt_start<<<h->nsys, B, 0, S>>>(a);
+  HIP_TRY(hipGetLastError());
+
+  // active list: every block; rebuilt from the polled state as systems stop
+  std::vector<int> act(h->nblk);
+  for (int t = 0; t < h->nblk; ++t) act[t] = t;
+  a.nact = h->nblk;
+  HIP_TRY(hipMemcpyAsync(h->act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, S));
+  HIP_TRY(hipStreamSynchronize(S));
+
+  const bool use_graph = o.path != TFG_PATH_KERNELS;
+  const int chunk = 16;
+  hipGraphExec_t exec = nullptr;
+  auto enqueue_chunk = [&](int count) -> hipError_t {
+    const int G = kXcds * ((a.nact + kXcds - 1) / kXcds);
+    for (int c = 0; c < count; ++c) {
+      if (pr == PREC_JACOBI) k_bt_pupdate<PREC_JACOBI><<<G, B, 0, S>>>(a);
+      else k_bt_pupdate<PREC_NONE><<<G, B, 0, S>>>(a);
+      TFG_BT_ENGINE((k_bt_spmv<E><<<G, B, 0, S>>>(a)));
+      k_bt_alpha<<<a.nsys, B, 0, S>>>(a);
+      if (pr == PREC_JACOBI) k_bt_update<PREC_JACOBI><<<G, B, 0, S>>>(a);
+      else k_bt_update<PREC_NONE><<<G, B, 0, S>>>(a);
+      k_bt_check<<<a.nsys, B, 0, S>>>(a);
+    }
+    return hipGetLastError();
+  };
+  auto build_graph = [&]() -> int {
+    if (exec) {
+      HIP_TRY(hipGraphExecDestroy(exec));
+      exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal));
+    const hipError_t e1 = enqueue_chunk(chunk);
+    hipError_t e2 = hipStreamEndCapture(S, &g);
+    if (e1 != hipSuccess) e2 = e1;
+    if (e2 == hipSuccess) e2 = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    HIP_TRY(e2);
+    return TFG_OK;
+  };
+  int rc = TFG_OK;
+  if (use_graph) rc = build_graph();
+  if (rc != TFG_OK) return rc;
+
+  // Chunks of iterations; the state of chunk c is read while chunk c+1
+  // runs (stopped systems are masked on the device, so the lag only costs
+  // the masked launches of one chunk).
+  int k = 0, slot = 0, prev = -1;
+  std::vector<int> nact;
+  while (k < miter) {
+    const int cnt = std::min(chunk, miter - k);
+    if (use_graph && cnt == chunk) HIP_TRY(hipGraphLaunch(exec, S));
+    else HIP_TRY(enqueue_chunk(cnt));
+    k += cnt;
+    State* poll = h->hst + (size_t)slot * h->nsys;
+    HIP_TRY(hipMemcpyAsync(poll, h->st, sizeof(State) * (size_t)h->nsys, hipMemcpyDeviceToHost, S));
+    HIP_TRY(hipEventRecord(h->ev_poll[slot], S));
+    if (prev >= 0) {
+      HIP_TRY(hipEventSynchronize(h->ev_poll[prev]));
+      const State* ps = h->hst + (size_t)prev * h->nsys;
+      int running = 0;
+      nact.clear();
+      for (int s = 0; s < h->nsys; ++s) {
+        if (!ps[s].done) {
+          ++running;
+          for (int t = h->h_sblk[s]; t < h->h_sblk[s + 1]; ++t) nact.push_back(t);
+        }
+      }
+      if (!running) break;
+      // compaction once an eighth of the active blocks have stopped (the
+      // copy is stream-ordered after the chunk already queued)
+      if ((int)nact.size() * 8 <= a.nact * 7) {
+        HIP_TRY(hipMemcpyAsync(h->act, nact.data(), sizeof(int) * nact.size(), hipMemcpyHostToDevice, S));
+        HIP_TRY(hipStreamSynchronize(S));
+        a.nact = (int)nact.size();
+        if (use_graph) {
+          rc = build_graph();
+          if (rc != TFG_OK) return rc;
+        }
+      }
+    }
+    prev = slot;
+    slot ^= 1;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  HIP_TRY(hipEventRecord(h->ev1, S));
+  HIP_TRY(hipMemcpyAsync(h->hst, h->st, sizeof(State) * (size_t)h->nsys, hipMemcpyDeviceToHost, S));
+  std::vector<unsigned long long> clk(h->nsys + 1);
+  HIP_TRY(hipMemcpyAsync(clk.data(), h->clk, sizeof(unsigned long long) * clk.size(), hipMemcpyDeviceToHost, S));
+  if (!device_ptrs)
+    HIP_TRY(hipMemcpyAsync(x, h->x, sizeof(double) * N, hipMemcpyDeviceToHost, S));
+  else
+    HIP_TRY(hipMemcpyAsync(x, h->x, sizeof(double) * N, hipMemcpyDeviceToDevice, S));
+  HIP_TRY(hipStreamSynchronize(S));
+  int rate_khz = 0;
+  HIP_TRY(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, h->device));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  int status = TFG_OK, maxit = 0;
+  for (int s = 0; s < h->nsys; ++s) {
+    const State& q = h->hst[s];
+    niter[s] = q.done ? q.niter : std::max(miter, 0);
+    fin_res[s] = q.res;
+    maxit = std::max(maxit, niter[s]);
+    if (q.status == TFG_EBREAKDOWN) status = TFG_EBREAKDOWN;
+    h->done_ms[s] = q.done && rate_khz > 0 ? (double)(clk[1 + s] - clk[0]) / (double)rate_khz : ms;
+  }
+  if (rep) {
+    rep->status = status;
+    rep->iterations = maxit;
+    rep->solve_ms = ms;
+  }
+  if (status == TFG_EBREAKDOWN) return fail(TFG_EBREAKDOWN, "Krylov breakdown in a batched system");
+  return TFG_OK;
+#undef TFG_BT_ENGINE
+}
+
+int tfg_batch_times(tfg_batch* h, double* ms) {
+  if (!h || !ms) return fail(TFG_EINVAL, "null argument");
+  for (int s = 0; s < h->nsys; ++s) ms[s] = h->done_ms[s];
+  return TFG_OK;
+}
+
+int tfg_batch_info(tfg_batch* h, int* nsys, int* n, int* nnz, int* blocks) {
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  if (nsys) *nsys = h->nsys;
+  if (n) *n = h->nt;
+  if (nnz) *nnz = h->nnz;
+  if (blocks) *blocks = h->nblk;
+  return TFG_OK;
+}
+
+}  // extern "C"

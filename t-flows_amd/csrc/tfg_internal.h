@@ -36,6 +36,8 @@ struct State {
   int niter;        // iteration at which done was raised
   int status;       // TFG_OK or TFG_EBREAKDOWN
   int iter;         // last iteration whose stopping test was evaluated
+  double alpha;     // CGS / BiCGSTAB: alpha of the current iteration
+  double omega;     // BiCGSTAB: omega of the last iteration
 };
 
 // Partial sums written by one kernel and swept by the next ones.
@@ -45,9 +47,11 @@ struct Partials {
   double* spmv;     // [1][pstride]  (p, q)
   double* upd;      // [5][pstride]  NRMS sum, x min, x max, (z,r), (r,r)
   double* aux;      // [2][pstride]  true-residual sums
+  double* ext;      // [6][pstride]  BiCGSTAB: sums of s, then (t,s), (t,t)
   int pstride;
-  int g_vec;        // grid of the element-wise kernels (producers of diag/upd)
+  int g_vec;        // grid of the element-wise kernels (producers of diag)
   int g_spmv;       // grid of the SpMV kernels (producers of pro/spmv/aux)
+  int g_upd;        // grid of the kernel that produces upd (method dependent)
 };
 
 // Everything a CG phase kernel needs.  Device pointers.
@@ -75,6 +79,20 @@ struct SpmvShape {
   const int* __restrict__ col;   // 0-based
   const int* __restrict__ blk;   // row-block boundaries (stage engines)
   int nblk;
+};
+
+// Textbook methods for non-symmetric systems (tfg_nonsym.hip).
+enum NsMethod { NS_NONE = 0, NS_BICG = 1, NS_CGS = 2, NS_BCGS = 3 };
+
+// Extra vectors of those methods; roles per method are listed in
+// tfg_nonsym.hip.
+struct NsArgs {
+  const double* valt;  // transposed values (BiCG), unscaled
+  double* rt;          // shadow residual
+  double* v1;
+  double* v2;
+  double* v3;
+  double* v4;
 };
 
 // SpMV row engines (see tfg_spmv.h).
@@ -108,11 +126,21 @@ int spmv_blocks_per_cu(int kind);
 
 hipError_t launch_diag_sum(const Launch& L, const CgArgs& a);
 hipError_t launch_prologue(const Launch& L, const CgArgs& a, int prec);
-hipError_t launch_start(const Launch& L, const CgArgs& a);
+hipError_t launch_start(const Launch& L, const CgArgs& a, int rho_slot = 6);
 hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k);
 hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k);
 hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k);
-hipError_t launch_finish(const Launch& L, const CgArgs& a, int k);
+hipError_t launch_finish(const Launch& L, const CgArgs& a, int k, int check_omega = 0);
+
+// Textbook methods (tfg_nonsym.hip).
+hipError_t launch_ns_init(const Launch& L, const CgArgs& a, const NsArgs& s);
+hipError_t launch_transpose_pos(hipStream_t stream, int n, const int* row, const int* col,
+                                int* tpos, int* bad);
+hipError_t launch_gather_valt(const Launch& L, long long nnz, const int* tpos, const double* val,
+                              double* valt);
+// One iteration k of method m; ev0/ev1 (may be null) bracket its first product.
+hipError_t launch_ns_iteration(const Launch& L, const CgArgs& a, const NsArgs& s, int method,
+                               int prec, int k, hipEvent_t ev0, hipEvent_t ev1);
 hipError_t launch_true_residual(const Launch& L, const CgArgs& a);
 
 hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* val,

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_prologue(CgArgs a) {
 }
 
 // One workgroup: the two early exits of Cg.f90:95-123 and rho_1.
-__global__ __launch_bounds__(kBlock) void k_start(CgArgs a) {
+__global__ __launch_bounds__(kBlock) void k_start(CgArgs a, int rho_slot) {
   __shared__ double s_tmp[7 * 4], s_out[7];
   const int ops[7] = {OP_SUM, OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM, OP_SUM};
   gather<7>(a.pt.pro, a.pt.pstride, a.pt.g_spmv, ops, s_tmp, s_out);
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_start(CgArgs a) {
     st->done = 1;
   } else {
     st->res = res0;
-    st->rho[1] = s_out[6];
+    st->rho[1] = s_out[rho_slot];   // (z, r); (r, r) = (r~0, r) for CGS / BiCGSTAB
     st->done = 0;
   }
 }
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
   double beta = 0.0;
   if (k > 1) {
     const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
-    gather<5>(a.pt.upd, a.pt.pstride, a.pt.g_vec, ops, s_tmp, s_out);
+    gather<5>(a.pt.upd, a.pt.pstride, a.pt.g_upd, ops, s_tmp, s_out);
     const double res = measure(st, s_out[0], s_out[1], s_out[2], s_out[4]);
     const bool bad = !isfinite(res);
     if (bad || res < st->tol) {
@@ -284,12 +284,13 @@ __global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
 
 // One workgroup: stopping test of iteration k (Cg.f90:214-220) so that the
 // host-visible state is exact at chunk boundaries and at the end.
-__global__ __launch_bounds__(kBlock) void k_finish(CgArgs a, int k) {
+// check_omega: BiCGSTAB also stops (breakdown) when omega = 0.
+__global__ __launch_bounds__(kBlock) void k_finish(CgArgs a, int k, int check_omega) {
   __shared__ double s_tmp[5 * 4], s_out[5];
   State* st = a.st;
   if (st->done) return;
   const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
-  gather<5>(a.pt.upd, a.pt.pstride, a.pt.g_vec, ops, s_tmp, s_out);
+  gather<5>(a.pt.upd, a.pt.pstride, a.pt.g_upd, ops, s_tmp, s_out);
   if (threadIdx.x != 0) return;
   const double res = measure(st, s_out[0], s_out[1], s_out[2], s_out[4]);
   st->res = res;
@@ -299,6 +300,10 @@ __global__ __launch_bounds__(kBlock) void k_finish(CgArgs a, int k) {
     st->niter = k;
     st->done = 1;
   } else if (res < st->tol) {
+    st->niter = k;
+    st->done = 1;
+  } else if (check_omega && st->omega == 0.0) {
+    st->status = TFG_EBREAKDOWN;
     st->niter = k;
     st->done = 1;
   }
@@ -376,8 +381,8 @@ hipError_t launch_prologue(const Launch& L, const CgArgs& a, int prec) {
   return hipGetLastError();
 }
 
-hipError_t launch_start(const Launch& L, const CgArgs& a) {
-  k_start<<<1, kBlock, 0, L.stream>>>(a);
+hipError_t launch_start(const Launch& L, const CgArgs& a, int rho_slot) {
+  k_start<<<1, kBlock, 0, L.stream>>>(a, rho_slot);
   return hipGetLastError();
 }
 
@@ -398,8 +403,8 @@ hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k) {
   return hipGetLastError();
 }
 
-hipError_t launch_finish(const Launch& L, const CgArgs& a, int k) {
-  k_finish<<<1, kBlock, 0, L.stream>>>(a, k);
+hipError_t launch_finish(const Launch& L, const CgArgs& a, int k, int check_omega) {
+  k_finish<<<1, kBlock, 0, L.stream>>>(a, k, check_omega);
   return hipGetLastError();
 }
 

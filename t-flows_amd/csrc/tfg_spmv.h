@@ -230,6 +230,44 @@ struct StreamEngine {
   }
 };
 
+// One given row block lb per call (the batched solver's one-block-per-
+// workgroup launches): StreamEngine staging without the pipelining.
+template <int CAP, typename T>
+struct OneBlock {
+  using SE = StreamEngine<CAP, false, T>;
+  using Smem = typename SE::Smem;
+  template <bool SUB, typename Init, typename Epi>
+  __device__ __forceinline__ static void one(Smem& sm, const SpmvShape& m,
+                                             const T* __restrict__ val, T fn, const T* x,
+                                             double* s_tmp, double* s_out, int lb, Init init,
+                                             Epi epi) {
+    constexpr int EPV = SE::EPV;
+    const int tid = threadIdx.x;
+    const int nnz = m.row[m.n];
+    typename SE::Regs g;
+    SE::fetch(m, val, nnz, lb, g);
+    const int r0 = g.r0, nr = g.nr, s = g.s, e = g.e, rs = g.rs, re = g.re;
+    if (e - s > CAP) {
+      long_row<SUB>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
+      return;
+    }
+    g.v.put(sm.val, nnz, s, e);
+    g.c.put(sm.col, nnz, s, e);
+    __syncthreads();
+    const int ov = s & ~(EPV - 1), oc = s & ~3;
+    if (tid < nr) {
+      const int i = r0 + tid;
+      T acc = init(i);
+      for (int j = rs; j < re; ++j) {
+        const T a = sm.val[j - ov] * fn;
+        acc = SUB ? acc - a * x[sm.col[j - oc]] : acc + a * x[sm.col[j - oc]];
+      }
+      epi(i, acc, x[sm.col[rs - oc]], sm.val[rs - ov] * fn);
+    }
+    __syncthreads();
+  }
+};
+
 // Stage engine with element-wise staging loads, for a val array that is not
 // 16-byte aligned or a matrix with fewer than 4 nonzeros.
 template <int CAP, typename T>
@@ -270,6 +308,37 @@ struct ScalarStage {
       }
       __syncthreads();
     }
+  }
+
+  // one given row block lb (see OneBlock)
+  template <bool SUB, typename Init, typename Epi>
+  __device__ __forceinline__ static void one(Smem& sm, const SpmvShape& m,
+                                             const T* __restrict__ val, T fn,
+                                             const T* __restrict__ x, double* s_tmp,
+                                             double* s_out, int lb, Init init, Epi epi) {
+    const int tid = threadIdx.x;
+    const int r0 = m.blk[lb], nr = m.blk[lb + 1] - r0;
+    const int s = m.row[r0], e = m.row[r0 + nr];
+    if (e - s > CAP) {
+      long_row<SUB>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
+      return;
+    }
+    for (int t = tid; t < e - s; t += kBlock) {
+      sm.val[t] = val[s + t];
+      sm.col[t] = m.col[s + t];
+    }
+    __syncthreads();
+    if (tid < nr) {
+      const int i = r0 + tid;
+      const int rs = m.row[i], re = m.row[i + 1];
+      T acc = init(i);
+      for (int j = rs; j < re; ++j) {
+        const T a = sm.val[j - s] * fn;
+        acc = SUB ? acc - a * x[sm.col[j - s]] : acc + a * x[sm.col[j - s]];
+      }
+      epi(i, acc, x[sm.col[rs - s]], sm.val[rs - s] * fn);
+    }
+    __syncthreads();
   }
 };
 

@@ -36,7 +36,7 @@ class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("criterion", ctypes.c_int),
                 ("path", ctypes.c_int), ("inner_max", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("timing", ctypes.c_int),
-                ("true_residual", ctypes.c_int)]
+                ("true_residual", ctypes.c_int), ("textbook_bicg", ctypes.c_int)]
 
 
 class Report(ctypes.Structure):
@@ -78,6 +78,12 @@ def lib():
         L.tfg_memcpy_d2h.argtypes = [P, P, LL]
         L.tfg_synchronize.argtypes = [P]
         L.tfg_stream_copy.argtypes = [P, P, LL, I, ctypes.POINTER(D)]
+        L.tfg_batch_create.argtypes = [I, P, P, P, P, P, ctypes.POINTER(P)]
+        L.tfg_batch_destroy.argtypes = [P]
+        L.tfg_batch_solve.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I, P,
+                                      D, P, P, I, P, P]
+        L.tfg_batch_times.argtypes = [P, P]
+        L.tfg_batch_info.argtypes = [P, P, P, P, P]
         L.tfg_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -240,3 +246,67 @@ class Native:
         ms = ctypes.c_double()
         _check(lib().tfg_spmv_bench(self.h, dval.ptr, dp.ptr, dq.ptr, int(reps), ctypes.byref(ms)))
         return ms.value
+
+
+class Batch:
+    """Many independent systems solved together (tfg_batch_*).  systems: a
+    list of (row, col, dia) 1-based Matrix_Type arrays."""
+
+    def __init__(self, systems):
+        self.n = np.array([len(r) - 1 for r, _, _ in systems], dtype=np.int32)
+        self.nnz = np.array([len(c) for _, c, _ in systems], dtype=np.int32)
+        self.nsys = len(systems)
+        self.row = np.ascontiguousarray(np.concatenate([r for r, _, _ in systems]), dtype=np.int32)
+        self.col = np.ascontiguousarray(np.concatenate([c for _, c, _ in systems]), dtype=np.int32)
+        self.dia = np.ascontiguousarray(np.concatenate([d for _, _, d in systems]), dtype=np.int32)
+        self.offsets = np.concatenate([[0], np.cumsum(self.n)]).astype(np.int64)
+        self.nnz_offsets = np.concatenate([[0], np.cumsum(self.nnz)]).astype(np.int64)
+        self.h = ctypes.c_void_p()
+        _check(lib().tfg_batch_create(self.nsys, _ptr(self.n), _ptr(self.nnz), _ptr(self.row),
+                                      _ptr(self.col), _ptr(self.dia), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().tfg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib().tfg_batch_info(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("nsys", "n", "nnz", "blocks"), (x.value for x in v)))
+
+    def solve(self, solver, prec, val, x, b, miter, tol, norm=None, options=None, device=False):
+        """val/x/b: concatenated host arrays (or DeviceArrays with device=True).
+        Returns (niter[nsys], fin_res[nsys], report dict)."""
+        niter = np.zeros(self.nsys, dtype=np.int32)
+        fin = np.zeros(self.nsys)
+        nv = None if norm is None else np.ascontiguousarray(norm, dtype=np.float64)
+        opt = options if options is not None else Options()
+        rep = Report()
+        s, p = solver.encode(), prec.encode()
+        if device:
+            pv, px, pb = val.ptr, x.ptr, b.ptr
+        else:
+            for name, a, n in (("val", val, len(self.col)), ("x", x, len(self.dia)),
+                               ("b", b, len(self.dia))):
+                if not (isinstance(a, np.ndarray) and a.dtype == np.float64
+                        and a.flags.c_contiguous and a.shape == (n,)):
+                    raise TypeError(f"{name} must be a contiguous float64 array of length {n}")
+            pv, px, pb = _ptr(val), _ptr(x), _ptr(b)
+        rc = lib().tfg_batch_solve(self.h, s, len(s), p, len(p), pv, px, pb, int(miter),
+                                   _ptr(niter), float(tol), _ptr(fin),
+                                   _ptr(nv) if nv is not None else None, 1 if device else 0,
+                                   ctypes.byref(opt), ctypes.byref(rep))
+        _check(rc)
+        return niter, fin, rep.as_dict()
+
+    def times(self):
+        ms = np.zeros(self.nsys)
+        _check(lib().tfg_batch_times(self.h, _ptr(ms)))
+        return ms

@@ -50,6 +50,24 @@ def test_structure_validation_on_host(gpu):
     assert gpu.lib().tfg_matrix_create(2, 2, 2, None, None, None, ctypes.byref(h)) == gpu.TFG_EINVAL
 
 
+def test_batch_validation_on_host(gpu):
+    """tfg_batch_create checks every system like tfg_matrix_create and names
+    the failing one, before touching a device."""
+    i32 = lambda a: np.array(a, dtype=np.int32)  # noqa: E731
+    n, nnz = i32([2, 2]), i32([3, 3])
+    row = i32([1, 3, 4, 1, 3, 4])
+    col = i32([1, 2, 2, 2, 1, 2])      # system 1: diagonal not first in row 1
+    dia = i32([1, 3, 1, 3])
+    h = ctypes.c_void_p()
+    rc = gpu.lib().tfg_batch_create(2, n.ctypes.data, nnz.ctypes.data, row.ctypes.data,
+                                    col.ctypes.data, dia.ctypes.data, ctypes.byref(h))
+    msg = gpu.lib().tfg_last_error().decode()
+    assert rc == gpu.TFG_EINVAL and "system 1" in msg and "diagonal" in msg
+    rc = gpu.lib().tfg_batch_create(0, n.ctypes.data, nnz.ctypes.data, row.ctypes.data,
+                                    col.ctypes.data, dia.ctypes.data, ctypes.byref(h))
+    assert rc == gpu.TFG_EINVAL
+
+
 def test_python_mirror_checks_operands(cases, gpu):
     s = cases.hex_pressure(3)
     nat = object.__new__(gpu.Native)     # no device handle needed for argument checks

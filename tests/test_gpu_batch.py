@@ -1,0 +1,151 @@
+"""Batched solves (tfg_batch_*) against the oracle, system by system.
+
+Each system of a batch must behave as its own Solve_Native call: iteration
+count within +-2 of the oracle's, x within 1e-10 relative of the oracle's or
+within twice the spread that reordering the oracle's global sums causes.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(cases, specs):
+    out = []
+    for kind, a, b in specs:
+        if kind == "hex":
+            out.append(cases.hex_pressure(a, rho_out=b))
+        elif kind == "knn":
+            out.append(cases.knn(a, a, a))
+        else:
+            out.append(cases.upwind(a))
+    return out
+
+
+def _concat(systems, attr):
+    return np.ascontiguousarray(np.concatenate([getattr(s, attr) for s in systems]))
+
+
+def _oracle(oracle, s, prec, miter, tol, norm=None, criterion="nrms", sum_order=0, x0=None):
+    A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
+    x = np.zeros(s.n) if x0 is None else x0.copy()
+    it, res = oracle.solve("cg", prec, A, x, s.b.copy(), miter, tol, norm=norm,
+                           criterion=criterion, sum_order=sum_order)
+    return it, res, x
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+SPECS = [("hex", 6, 1.0), ("hex", 12, 1000.0), ("hex", 9, 30.0), ("knn", 7, 0),
+         ("hex", 16, 300.0), ("upwind", 8, 0), ("hex", 4, 1.0), ("hex", 14, 5.0)]
+
+
+def _batch(gpu, systems):
+    return gpu.Batch([(s.row, s.col, s.dia) for s in systems])
+
+
+@pytest.mark.parametrize("prec,crit,norm", [("jacobi", 0, None), ("none", 0, None),
+                                            ("jacobi", 1, None), ("jacobi", 0, 1.0)])
+def test_batch_parity(cases, oracle, gpu, prec, crit, norm):
+    systems = _systems(cases, SPECS)
+    bt = _batch(gpu, systems)
+    print("batch", bt.info(), prec, crit, norm, flush=True)
+    x = np.zeros(int(bt.offsets[-1]))
+    opt = gpu.Options()
+    opt.criterion = crit
+    miter, tol = 3000, 1e-8
+    norms = None if norm is None else np.full(len(systems), norm)
+    niter, fin, rep = bt.solve("cg", prec, _concat(systems, "val"), x, _concat(systems, "b"),
+                               miter, tol, norm=norms, options=opt)
+    cname = "nrms" if crit == 0 else "relres"
+    for k, s in enumerate(systems):
+        xs = x[bt.offsets[k]:bt.offsets[k + 1]]
+        it_o, res_o, x_o = _oracle(oracle, s, prec, miter, tol, norm, cname)
+        _, _, x_e = _oracle(oracle, s, prec, miter, tol, norm, cname, sum_order=1)
+        _, _, x_f = _oracle(oracle, s, prec, miter, tol, norm, cname, sum_order=2)
+        env = max(_rel(x_e, x_o), _rel(x_f, x_o))
+        err = _rel(xs, x_o)
+        print(f"  system {k} {s.name}: niter {niter[k]} (oracle {it_o}) res {fin[k]:.3e} "
+              f"(oracle {res_o:.3e}) err {err:.2e} env {env:.2e}", flush=True)
+        assert abs(int(niter[k]) - it_o) <= 2
+        if it_o < miter:
+            assert fin[k] < tol
+        assert err <= max(1e-10, 2.0 * env)
+    assert rep["iterations"] == int(niter.max())
+
+
+def test_graph_and_kernels_paths_agree(cases, gpu):
+    systems = _systems(cases, SPECS[:5])
+    bt = _batch(gpu, systems)
+    val, b = _concat(systems, "val"), _concat(systems, "b")
+    out = []
+    for path in (gpu.PATH_GRAPH, gpu.PATH_KERNELS, gpu.PATH_GRAPH):
+        opt = gpu.Options()
+        opt.path = path
+        x = np.zeros(int(bt.offsets[-1]))
+        niter, fin, _ = bt.solve("cg", "jacobi", val, x, b, 2000, 1e-9, options=opt)
+        out.append((niter.copy(), fin.copy(), x))
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0]) and np.array_equal(o[1], out[0][1])
+        assert np.array_equal(o[2], out[0][2])
+
+
+def test_edge_systems(cases, oracle, gpu):
+    """A zero right-hand side, an exact initial guess and the iteration cap,
+    side by side with normal systems."""
+    systems = _systems(cases, [("hex", 8, 10.0), ("hex", 6, 1.0), ("hex", 7, 100.0),
+                               ("hex", 10, 1.0)])
+    systems[1].b[:] = 0.0
+    bt = _batch(gpu, systems)
+    x0 = np.zeros(int(bt.offsets[-1]))
+    # exact guess for system 2: a converged oracle solution
+    it, _, xs2 = _oracle(oracle, systems[2], "jacobi", 5000, 1e-14)
+    x0[bt.offsets[2]:bt.offsets[3]] = xs2
+    x = x0.copy()
+    niter, fin, _ = bt.solve("cg", "jacobi", _concat(systems, "val"), x, _concat(systems, "b"),
+                             5, 1e-8)
+    assert niter[1] == 0 and fin[1] == 0.0
+    assert niter[2] == 0
+    assert niter[0] == 5 and niter[3] == 5       # capped
+    for k in (0, 3):
+        it_o, res_o, x_o = _oracle(oracle, systems[k], "jacobi", 5, 1e-8)
+        assert it_o == 5 and abs(fin[k] - res_o) <= 1e-8 * res_o
+        assert _rel(x[bt.offsets[k]:bt.offsets[k + 1]], x_o) < 1e-12
+
+
+def test_device_pointers_and_determinism(cases, gpu):
+    systems = _systems(cases, SPECS[:4])
+    bt = _batch(gpu, systems)
+    val, b = _concat(systems, "val"), _concat(systems, "b")
+    n = int(bt.offsets[-1])
+    xh = np.zeros(n)
+    r1 = bt.solve("cg", "jacobi", val, xh, b, 1000, 1e-9)
+    dv, db, dx = gpu.DeviceArray(val), gpu.DeviceArray(b), gpu.DeviceArray(np.zeros(n))
+    r2 = bt.solve("cg", "jacobi", dv, dx, db, 1000, 1e-9, device=True)
+    xd = dx.download()
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
+    assert np.array_equal(xh, xd)
+    t = bt.times()
+    assert np.all(t > 0) and np.all(t <= r2[2]["solve_ms"] * 1.01 + 1e-3)
+
+
+def test_batch_many_medium(cases, gpu):
+    """64 two-phase systems of sides 10..30 to relres 1e-8, checked on each
+    system's FP64 true residual computed on the host."""
+    sides, ratios = cases.batch_params(64, 10, 30, 1000.0)
+    systems = [cases.hex_pressure(int(sd), rho_out=float(rt), seed=1000 + k)
+               for k, (sd, rt) in enumerate(zip(sides, ratios))]
+    bt = _batch(gpu, systems)
+    opt = gpu.Options()
+    opt.criterion = gpu.CRIT_RELRES
+    x = np.zeros(int(bt.offsets[-1]))
+    niter, fin, rep = bt.solve("cg", "jacobi", _concat(systems, "val"), x, _concat(systems, "b"),
+                               5000, 1e-8, options=opt)
+    print(f"64 systems: max niter {niter.max()}, {rep['solve_ms']:.1f} ms", flush=True)
+    for k, s in enumerate(systems):
+        xs = x[bt.offsets[k]:bt.offsets[k + 1]]
+        r = s.b - s.scipy() @ xs
+        assert niter[k] < 5000 and fin[k] < 1e-8
+        assert np.linalg.norm(r) / np.linalg.norm(s.b) < 1e-7
