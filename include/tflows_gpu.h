@@ -80,7 +80,8 @@ int tfg_matrix_destroy(tfg_matrix* handle);
  * (Petsc_Interface.cpp:416-436): "cgs", "bcgs" (alias "bicgstab"), and
  * textbook "bicg" via tfg_options.textbook_bicg -- FP64, jacobi or none, with
  * the same normalisation, stopping test and niter/fin_res conventions;
- * prec: "jacobi" | "icc" | anything else = none (Prec_Form.f90:47-89).
+ * prec: "jacobi" | "icc" | anything else = none (Prec_Form.f90:47-89);
+ * "icc" (the reference's default) with cg / bicg in FP64.
  * val(nnz), x(nt) = x0 on entry / solution on exit, b(nt).  val and b are
  * left unchanged (the reference scales and de-scales them in place,
  * Cg.f90:77-84, 245-250).  norm: NULL when the optional argument is absent.
@@ -107,6 +108,12 @@ int tfg_spmv(tfg_matrix* h, const double* d_val, const double* d_p, double* d_q)
 int tfg_residual(tfg_matrix* h, const double* d_val, const double* d_b,
                  const double* d_x, double* d_r);
 
+/* z = M^-1 r with the reference's preconditioner of the unscaled matrix
+ * (Prec_Form.f90 + Prec_Solve.f90): "icc" (DIC, level-scheduled, bit-exact
+ * to the serial sweeps), "jacobi", anything else = copy.  Stream-ordered. */
+int tfg_prec_apply(tfg_matrix* h, const char* prec, int prec_len,
+                   const double* d_val, const double* d_r, double* d_z);
+
 /* Time `reps` back-to-back SpMV launches with hipEvents on the handle's
  * stream; *ms_avg = mean duration of one launch. */
 int tfg_spmv_bench(tfg_matrix* h, const double* d_val, const double* d_p,
@@ -128,21 +135,27 @@ int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, d
 
 /* SpMV engines: 0 LDS-staged rows (2048 nonzeros per block), 1 the same
  * with 4096, 2/3/4 sub-wavefront rows with 4/8/16 lanes per row, 5 the
- * 2048 stage with the next block's loads in flight.  All give bit-identical
- * products; the engine sets the SpMV launch grid and so the tree of the dot
- * products fused into it.  Default: engine 5, so that a solve is bitwise
+ * 2048 stage with the next block's loads in flight, 6 the same reading a
+ * 1-byte code per nonzero instead of its int32 column (col = row + table[code];
+ * available when the matrix has at most 256 distinct col - row offsets, as
+ * every structured grid has).  All give bit-identical products; the engine
+ * sets the SpMV launch grid and so the tree of the dot products fused into
+ * it.  Default: engine 6 when available, else 5, so that a solve is bitwise
  * reproducible from run to run.  The autotuner is opt-in: tfg_spmv_tune
- * times every engine on d_val (us[6] receives each mean launch time) and
+ * times every engine on d_val (us[7] receives each mean launch time, -1
+ * for an engine the matrix does not offer) and
  * keeps the fastest; TFG_SPMV_AUTOTUNE=1 in the environment makes the first
  * product or solve of a handle do the same; tfg_spmv_select forces an engine
  * (-1: default choice again on next use). */
 int tfg_spmv_tune(tfg_matrix* h, const double* d_val, int* kind, float* us);
 int tfg_spmv_select(tfg_matrix* h, int kind);
 
-/* Facts about the handle: sizes, chosen SpMV engine (-1: not tuned yet), its
- * grid, and the number of 2048-nonzero row blocks. */
+/* Facts about the handle: sizes, chosen SpMV engine (-1: not chosen yet),
+ * its grid, the number of 2048-nonzero row blocks, and the number of
+ * distinct col - row offsets when engine 6 is available (else 0).  Any
+ * pointer may be NULL. */
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind,
-                    int* spmv_grid, int* stage_blocks);
+                    int* spmv_grid, int* stage_blocks, int* offsets);
 
 /* ---- batched solves ------------------------------------------------------
  * No counterpart in the reference, which solves one system per Solve_Native

@@ -113,6 +113,17 @@ struct tfg_matrix {
   double* valt = nullptr;
   int* tpos = nullptr;
   int tpos_state = 0;  // 0 not built, 1 built, -1 the pattern is not symmetric
+  // DIC levels (built at create) and work
+  int* icc_rows[2] = {nullptr, nullptr};   // forward, backward
+  std::vector<int> icc_off[2];
+  double* icc_d = nullptr;
+  double* icc_dinv = nullptr;
+  double* icc_y = nullptr;
+  double* icc_z = nullptr;
+  // offset-coded columns (structured grids): col = row + tab[code]
+  uint8_t* code = nullptr;
+  int* tab = nullptr;
+  int ntab = 0;        // 0: more than 256 distinct offsets, no coded engine
 };
 
 namespace {
@@ -151,7 +162,8 @@ int rows_per_pass(int kind) {
 void size_grids(tfg_matrix* h) {
   for (int k = 0; k < tfg::kKinds; ++k) {
     const int per_cu = tfg::spmv_blocks_per_cu(k);
-    const int work = (k == tfg::K_STAGE2048 || k == tfg::K_PIPE2048) ? h->nblk[0]
+    const int work = (k == tfg::K_STAGE2048 || k == tfg::K_PIPE2048 || k == tfg::K_PIPE2048_CODED)
+                         ? h->nblk[0]
                      : k == tfg::K_STAGE4096 ? h->nblk[1]
                                              : (h->nt + rows_per_pass(k) - 1) / rows_per_pass(k);
     int gs = std::min(work, std::max(1, h->cu_count) * per_cu);
@@ -184,7 +196,25 @@ tfg::SpmvShape shape_of(const tfg_matrix* h, int kind) {
   const int b = kind == tfg::K_STAGE4096 ? 1 : 0;
   m.blk = h->blk[b];
   m.nblk = h->nblk[b];
+  m.code = h->code;
+  m.tab = h->tab;
+  m.ntab = h->ntab;
   return m;
+}
+
+// CgArgs with the structure of the chosen engine filled in.
+tfg::CgArgs args_of(const tfg_matrix* h, int kind) {
+  const tfg::SpmvShape m = shape_of(h, kind);
+  tfg::CgArgs a{};
+  a.n = h->ni;
+  a.row = h->row;
+  a.col = h->col;
+  a.blk = m.blk;
+  a.nblk = m.nblk;
+  a.code = m.code;
+  a.tab = m.tab;
+  a.ntab = m.ntab;
+  return a;
 }
 
 int alloc_work(tfg_matrix* h) {
@@ -219,7 +249,8 @@ void free_all(tfg_matrix* h) {
   void* dev[] = {h->row, h->col, h->blk[0], h->blk[1], h->val, h->b, h->x, h->r, h->p, h->q, h->ad,
                  h->part, h->st, h->st_in, h->val32, h->e32, h->r32,
                  h->p32, h->q32, h->ad32, h->rt, h->nv[0], h->nv[1], h->nv[2],
-                 h->nv[3], h->valt, h->tpos};
+                 h->nv[3], h->valt, h->tpos, h->code, h->tab, h->icc_rows[0],
+                 h->icc_rows[1], h->icc_d, h->icc_dinv, h->icc_y, h->icc_z};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   if (h->hst) (void)hipHostFree(h->hst);
@@ -244,7 +275,7 @@ int ensure_tuned(tfg_matrix* h, const double* val) {
   if (h->kind >= 0) return TFG_OK;
   const char* at = getenv("TFG_SPMV_AUTOTUNE");
   if (!(at && *at && *at != '0')) {
-    h->kind = tfg::K_PIPE2048;
+    h->kind = h->ntab > 0 ? tfg::K_PIPE2048_CODED : tfg::K_PIPE2048;
     return TFG_OK;
   }
   return tune_all(h, val);
@@ -255,6 +286,10 @@ int tune_all(tfg_matrix* h, const double* val) {
   float best = 1e30f;
   int best_k = tfg::K_STAGE2048;
   for (int k = 0; k < tfg::kKinds; ++k) {
+    if (k == tfg::K_PIPE2048_CODED && h->ntab == 0) {
+      h->tune_us[k] = -1.f;  // not available on this matrix
+      continue;
+    }
     tfg::Launch L = launch_of(h, val, k);
     const tfg::SpmvShape m = shape_of(h, k);
     for (int w = 0; w < 2; ++w) HIP_TRY(tfg::launch_spmv_plain(L, m, val, h->p, h->q));
@@ -302,6 +337,65 @@ int ensure_nonsym(tfg_matrix* h, int method, const double* val, const tfg::Launc
   return TFG_OK;
 }
 
+// Dependency levels of the DIC sweeps, read exactly as the sweeps read rows
+// (Prec_Solve.f90:64-92): forward, row i needs the entries after its
+// diagonal up to the first column above i; backward, the entries from the
+// row's end down to the first column below i.  row0/col0: 0-based host.
+void icc_levels(const int* row0, const int* col0, int n, bool forward, std::vector<int>& rows,
+                std::vector<int>& off) {
+  std::vector<int> lev(n, 0);
+  int nlev = 0;
+  for (int t = 0; t < n; ++t) {
+    const int i = forward ? t : n - 1 - t;
+    int l = 0;
+    if (forward) {
+      for (int j = row0[i] + 1; j < row0[i + 1]; ++j) {
+        const int k = col0[j];
+        if (k > i) break;
+        l = std::max(l, lev[k] + 1);
+      }
+    } else {
+      for (int j = row0[i + 1] - 1; j > row0[i]; --j) {
+        const int k = col0[j];
+        if (k < i) break;
+        if (k < n) l = std::max(l, lev[k] + 1);
+      }
+    }
+    lev[i] = l;
+    nlev = std::max(nlev, l + 1);
+  }
+  off.assign(nlev + 1, 0);
+  for (int i = 0; i < n; ++i) off[lev[i] + 1]++;
+  for (int l = 0; l < nlev; ++l) off[l + 1] += off[l];
+  std::vector<int> pos(off.begin(), off.end() - 1);
+  rows.assign(n, 0);
+  for (int i = 0; i < n; ++i) rows[pos[lev[i]]++] = i;
+}
+
+int ensure_icc(tfg_matrix* h) {
+  const size_t n = (size_t)h->nt;
+  for (double** v : {&h->icc_d, &h->icc_dinv, &h->icc_y, &h->icc_z})
+    if (!*v) HIP_TRY(hipMalloc(v, n * sizeof(double)));
+  return TFG_OK;
+}
+
+tfg::IccArgs icc_args(const tfg_matrix* h) {
+  tfg::IccArgs c{};
+  c.n = h->ni;
+  c.row = h->row;
+  c.col = h->col;
+  c.rows_f = h->icc_rows[0];
+  c.rows_b = h->icc_rows[1];
+  c.d = h->icc_d;
+  c.dinv = h->icc_dinv;
+  c.y = h->icc_y;
+  c.nlev_f = (int)h->icc_off[0].size() - 1;
+  c.nlev_b = (int)h->icc_off[1].size() - 1;
+  c.off_f_host = h->icc_off[0].data();
+  c.off_b_host = h->icc_off[1].data();
+  return c;
+}
+
 }  // namespace
 
 namespace tfg {
@@ -322,13 +416,7 @@ int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int mi
   { const int rt = ensure_tuned(h, val); if (rt != TFG_OK) return rt; }
   const tfg::Launch L = launch_of(h, val, h->kind);
   const tfg::Launch L32 = launch_of(h, h->val32, h->kind);
-  const tfg::SpmvShape m = shape_of(h, h->kind);
-  tfg::CgArgs a{};
-  a.n = h->ni;
-  a.row = h->row;
-  a.col = h->col;
-  a.blk = m.blk;
-  a.nblk = m.nblk;
+  tfg::CgArgs a = args_of(h, h->kind);
   a.val = val;
   a.b = b;
   a.x = h->x;
@@ -545,6 +633,54 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
     for (int b = 0; b < 2 && e == hipSuccess; ++b)
       e = hipMemcpy(h->blk[b], blk[b].data(), sizeof(int) * blk[b].size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) { rc = fail(TFG_EHIP, "structure upload: %s", hipGetErrorString(e)); break; }
+    // DIC dependency levels
+    {
+      std::vector<int> r0(row, row + nt + 1), c0(col, col + nnz);
+      for (auto& v : r0) v -= 1;
+      for (auto& v : c0) v -= 1;
+      for (int f = 0; f < 2 && e == hipSuccess; ++f) {
+        std::vector<int> rows;
+        icc_levels(r0.data(), c0.data(), nt, f == 0, rows, h->icc_off[f]);
+        e = hipMalloc(&h->icc_rows[f], sizeof(int) * (size_t)nt);
+        if (e == hipSuccess)
+          e = hipMemcpy(h->icc_rows[f], rows.data(), sizeof(int) * (size_t)nt, hipMemcpyHostToDevice);
+      }
+      if (e != hipSuccess) { rc = fail(TFG_EHIP, "level upload: %s", hipGetErrorString(e)); break; }
+    }
+    // Offset codes when the matrix has at most 256 distinct col - row
+    // offsets (every structured grid): 1 byte per nonzero instead of 4.
+    {
+      std::vector<int> table;
+      std::vector<uint8_t> code((size_t)nnz + 64, 0);
+      bool ok = nnz >= 64;
+      int last_off = INT_MIN, last_code = 0;
+      for (int i = 0; i < nt && ok; ++i) {
+        for (int j = row[i] - 1; j < row[i + 1] - 1; ++j) {
+          const int off = col[j] - 1 - i;
+          if (off != last_off) {
+            int c = -1;
+            for (size_t t = 0; t < table.size(); ++t)
+              if (table[t] == off) { c = (int)t; break; }
+            if (c < 0) {
+              if (table.size() == 256) { ok = false; break; }
+              c = (int)table.size();
+              table.push_back(off);
+            }
+            last_off = off;
+            last_code = c;
+          }
+          code[j] = (uint8_t)last_code;
+        }
+      }
+      if (ok) {
+        e = hipMalloc(&h->code, code.size());
+        if (e == hipSuccess) e = hipMalloc(&h->tab, sizeof(int) * 256);
+        if (e == hipSuccess) e = hipMemcpy(h->code, code.data(), code.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->tab, table.data(), sizeof(int) * table.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { rc = fail(TFG_EHIP, "code upload: %s", hipGetErrorString(e)); break; }
+        h->ntab = (int)table.size();
+      }
+    }
     rc = alloc_work(h);
     const char* force = getenv("TFG_SPMV_KIND");
     if (rc == TFG_OK && force && *force) {
@@ -567,8 +703,9 @@ int tfg_matrix_destroy(tfg_matrix* h) {
 }
 
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind, int* spmv_grid,
-                    int* stage_blocks) {
+                    int* stage_blocks, int* offsets) {
   if (!h) return fail(TFG_EINVAL, "null handle");
+  if (offsets) *offsets = h->ntab;
   if (n) *n = h->nt;
   if (nnz) *nnz = h->nnz;
   if (spmv_kind) *spmv_kind = h->kind;
@@ -580,6 +717,8 @@ int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind, int* spmv_g
 int tfg_spmv_select(tfg_matrix* h, int kind) {
   if (!h) return fail(TFG_EINVAL, "null handle");
   if (kind < -1 || kind >= tfg::kKinds) return fail(TFG_EINVAL, "unknown SpMV engine %d", kind);
+  if (kind == tfg::K_PIPE2048_CODED && h->ntab == 0)
+    return fail(TFG_ENOTSUP, "engine 6 needs at most 256 distinct column offsets");
   h->kind = kind;
   return TFG_OK;
 }
@@ -627,8 +766,8 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
   // Prec_Form.f90:47-89: 'jacobi', 'icc', anything else = none.
   const std::string pv = fstring(prec, prec_len);
   const int pr = pv == "jacobi" ? tfg::PREC_JACOBI : (pv == "icc" ? tfg::PREC_ICC : tfg::PREC_NONE);
-  if (pr == tfg::PREC_ICC)
-    return fail(TFG_ENOTSUP, "preconditioner 'icc' is not available on the GPU path yet");
+  if (pr == tfg::PREC_ICC && (method != tfg::NS_NONE || o.precision != TFG_FP64))
+    return fail(TFG_ENOTSUP, "preconditioner 'icc' runs with FP64 cg / bicg only");
   if (method != tfg::NS_NONE && o.precision != TFG_FP64)
     return fail(TFG_ENOTSUP, "%s runs in FP64 only", sv.c_str());
   if (o.criterion != TFG_CRIT_NRMS && o.criterion != TFG_CRIT_RELRES)
@@ -662,13 +801,7 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
   } else {
     { const int rt = ensure_tuned(h, dval); if (rt != TFG_OK) return rt; }
     tfg::Launch L = launch_of(h, dval, h->kind);
-    const tfg::SpmvShape m = shape_of(h, h->kind);
-    tfg::CgArgs a{};
-    a.n = h->ni;
-    a.row = h->row;
-    a.col = h->col;
-    a.blk = m.blk;
-    a.nblk = m.nblk;
+    tfg::CgArgs a = args_of(h, h->kind);
     a.val = dval;
     a.b = db;
     a.x = h->x;
@@ -709,7 +842,18 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
 
     HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
     HIP_TRY(tfg::launch_diag_sum(L, a));
-    HIP_TRY(tfg::launch_prologue(L, a, pr));
+    const bool icc = pr == tfg::PREC_ICC;
+    tfg::IccArgs ic{};
+    if (icc) {
+      const int ri = ensure_icc(h);
+      if (ri != TFG_OK) return ri;
+      ic = icc_args(h);
+    }
+    // ICC: the products and updates run unpreconditioned; z comes from the
+    // sweeps (PREC_NONE kernels never read the Jacobi diagonal)
+    const int pk = icc ? tfg::PREC_NONE : pr;
+    HIP_TRY(tfg::launch_prologue(L, a, pk));
+    if (icc) HIP_TRY(tfg::launch_icc_form(h->stream, ic, dval, h->st));
     // rho_1: (z, r0) for CG and BiCG, (r~0, r0) = (r0, r0) for CGS / BiCGSTAB
     HIP_TRY(tfg::launch_start(L, a, (method == tfg::NS_CGS || method == tfg::NS_BCGS) ? 5 : 6));
     if (method != tfg::NS_NONE) HIP_TRY(tfg::launch_ns_init(L, a, ns));
@@ -725,6 +869,15 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
         hipEvent_t e1 = smp ? h->ev_sample[2 * sample_iter.size() + 1] : nullptr;
         if (method != tfg::NS_NONE) {
           HIP_TRY(tfg::launch_ns_iteration(L, a, ns, method, pr, k, e0, e1));
+        } else if (icc) {
+          if (k > 1) HIP_TRY(tfg::launch_icc_check(L, a, k));
+          HIP_TRY(tfg::launch_icc_apply(h->stream, ic, dval, h->st, h->r, h->icc_z));
+          HIP_TRY(tfg::launch_icc_rho(L, a, h->icc_z));
+          HIP_TRY(tfg::launch_icc_pupdate(L, a, h->icc_z, k));
+          if (smp) HIP_TRY(hipEventRecord(e0, h->stream));
+          HIP_TRY(tfg::launch_spmv_pq(L, a, k));
+          if (smp) HIP_TRY(hipEventRecord(e1, h->stream));
+          HIP_TRY(tfg::launch_update(L, a, tfg::PREC_NONE, k));
         } else {
           HIP_TRY(tfg::launch_pupdate(L, a, pr, k));
           if (smp) HIP_TRY(hipEventRecord(e0, h->stream));
@@ -786,6 +939,25 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     HIP_TRY(hipMemcpyAsync(x, h->x, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (rc == TFG_EBREAKDOWN) return fail(TFG_EBREAKDOWN, "Krylov breakdown at iteration %d", *niter);
+  return TFG_OK;
+}
+
+int tfg_prec_apply(tfg_matrix* h, const char* prec, int prec_len, const double* d_val,
+                   const double* d_r, double* d_z) {
+  if (!h || !d_val || !d_r || !d_z) return fail(TFG_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  const std::string pv = fstring(prec, prec_len);
+  if (pv == "icc") {
+    const int ri = ensure_icc(h);
+    if (ri != TFG_OK) return ri;
+    const tfg::IccArgs ic = icc_args(h);
+    HIP_TRY(tfg::launch_icc_form(h->stream, ic, d_val, nullptr));
+    HIP_TRY(tfg::launch_icc_apply(h->stream, ic, d_val, nullptr, d_r, d_z));
+  } else if (pv == "jacobi") {
+    HIP_TRY(tfg::launch_jacobi_apply(h->stream, h->ni, h->row, d_val, d_r, d_z));
+  } else {
+    HIP_TRY(hipMemcpyAsync(d_z, d_r, sizeof(double) * (size_t)h->ni, hipMemcpyDeviceToDevice, h->stream));
+  }
   return TFG_OK;
 }
 

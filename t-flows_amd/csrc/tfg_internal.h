@@ -61,6 +61,9 @@ struct CgArgs {
   const int* col;      // 0-based columns, nnz
   const int* blk;      // SpMV block row boundaries, nblk+1
   int nblk;
+  const uint8_t* code; // offset codes (coded engine) or null
+  const int* tab;
+  int ntab;
   const double* val;   // unscaled values (caller's)
   const double* b;     // unscaled right-hand side (caller's)
   double* x;
@@ -79,6 +82,9 @@ struct SpmvShape {
   const int* __restrict__ col;   // 0-based
   const int* __restrict__ blk;   // row-block boundaries (stage engines)
   int nblk;
+  const uint8_t* __restrict__ code;  // offset code of each nonzero (coded engine)
+  const int* __restrict__ tab;       // offset table: col = row + tab[code]
+  int ntab;
 };
 
 // Textbook methods for non-symmetric systems (tfg_nonsym.hip).
@@ -97,7 +103,7 @@ struct NsArgs {
 
 // SpMV row engines (see tfg_spmv.h).
 enum SpmvKind { K_STAGE2048 = 0, K_STAGE4096 = 1, K_WAVE4 = 2, K_WAVE8 = 3, K_WAVE16 = 4,
-                K_PIPE2048 = 5, kKinds = 6 };
+                K_PIPE2048 = 5, K_PIPE2048_CODED = 6, kKinds = 7 };
 
 // FP32 sweeps of iterative refinement (FP32 vectors, FP64 scalars and
 // reductions).  Reuses the FP64 solve's partial buffers (spmv, upd).
@@ -110,6 +116,22 @@ struct IrArgs {
   float* p32;
   float* q32;
   State* st_in;         // inner scalars
+};
+
+// Level-scheduled DIC (tfg_icc.hip).  Device arrays except the two level
+// offset tables, which the host reads to size each level's launch.
+struct IccArgs {
+  int n;
+  const int* row;       // 0-based
+  const int* col;
+  const int* rows_f;    // rows sorted by forward level
+  const int* rows_b;    // rows sorted by backward level
+  double* d;            // 1 / d_inv (the reference's d)
+  double* dinv;         // a_ii - sum d_k a_ik^2 (the reference's d_inv)
+  double* y;            // forward-substitution result
+  int nlev_f, nlev_b;
+  const int* off_f_host;  // host: nlev_f + 1 level starts in rows_f
+  const int* off_b_host;  // host: nlev_b + 1 level starts in rows_b
 };
 
 // Launch shapes chosen per matrix.
@@ -131,6 +153,17 @@ hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k);
 hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k);
 hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k);
 hipError_t launch_finish(const Launch& L, const CgArgs& a, int k, int check_omega = 0);
+
+// DIC preconditioner (tfg_icc.hip).  st = null: unscaled matrix (fn = 1)
+// and no stop flag.
+hipError_t launch_icc_form(hipStream_t s, const IccArgs& c, const double* val, const State* st);
+hipError_t launch_icc_apply(hipStream_t s, const IccArgs& c, const double* val, const State* st,
+                            const double* r, double* z);
+hipError_t launch_icc_check(const Launch& L, const CgArgs& a, int k);
+hipError_t launch_icc_rho(const Launch& L, const CgArgs& a, const double* z);
+hipError_t launch_icc_pupdate(const Launch& L, const CgArgs& a, const double* z, int k);
+hipError_t launch_jacobi_apply(hipStream_t s, int n, const int* row, const double* val,
+                               const double* r, double* z);
 
 // Textbook methods (tfg_nonsym.hip).
 hipError_t launch_ns_init(const Launch& L, const CgArgs& a, const NsArgs& s);

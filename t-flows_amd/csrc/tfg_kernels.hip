@@ -46,6 +46,9 @@ __device__ __forceinline__ SpmvShape shape_of(const CgArgs& a) {
   m.col = a.col;
   m.blk = a.blk;
   m.nblk = a.nblk;
+  m.code = a.code;
+  m.tab = a.tab;
+  m.ntab = a.ntab;
   return m;
 }
 
@@ -362,6 +365,7 @@ int spmv_blocks_per_cu(int kind) {
     case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<4, double>>, kBlock, 0); break;
     case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<8, double>>, kBlock, 0); break;
     case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, true, double>>, kBlock, 0); break;
+    case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, true, double, true>>, kBlock, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<16, double>>, kBlock, 0); break;
   }
   if (e != hipSuccess || nb < 1) return 1;

@@ -39,6 +39,9 @@ __device__ __forceinline__ SpmvShape shape1(const CgArgs& a) {
   m.col = a.col;
   m.blk = a.blk;
   m.nblk = a.nblk;
+  m.code = a.code;
+  m.tab = a.tab;
+  m.ntab = a.ntab;
   return m;
 }
 

@@ -29,6 +29,8 @@
 // g % 8), so each XCD's L2 holds one window of the x gather.
 #pragma once
 
+#include <type_traits>
+
 #include "tfg_reduce.h"
 
 namespace tfg {
@@ -98,8 +100,12 @@ struct ChunkStream {
       const int t = tid + u * kBlock;
       if (t < n) {
         const V c = o + EPC * t > jmax ? tail : v[u];
+        if constexpr (sizeof(E) == 1) {
+          reinterpret_cast<V*>(lds)[t] = c;  // one 16-byte LDS store
+        } else {
 #pragma unroll
-        for (int k = 0; k < EPC; ++k) lds[EPC * t + k] = c[k];
+          for (int k = 0; k < EPC; ++k) lds[EPC * t + k] = c[k];
+        }
       }
     }
   }
@@ -125,18 +131,24 @@ __device__ __forceinline__ void long_row(const SpmvShape& m, const T* __restrict
 }
 
 // ---------------------------------------------------------------------------
-template <int CAP, bool PIPE, typename T>
+// CODED: the column stream is the 1-byte code of each nonzero's offset
+// col - row in a per-matrix table of at most 256 distinct offsets (a
+// structured grid has 7), a quarter of the int32 column traffic.
+template <int CAP, bool PIPE, typename T, bool CODED = false>
 struct StreamEngine {
+  typedef typename std::conditional<CODED, uint8_t, int>::type C;
   static constexpr int EPV = 16 / (int)sizeof(T);
+  static constexpr int EPC = 16 / (int)sizeof(C);
   static constexpr int NV = (CAP / EPV + 2 + kBlock - 1) / kBlock;  // val chunks per thread
-  static constexpr int NC = (CAP / 4 + 2 + kBlock - 1) / kBlock;    // col chunks per thread
+  static constexpr int NC = (CAP / EPC + 2 + kBlock - 1) / kBlock;  // col chunks per thread
   struct Smem {
     T val[CAP + 2 * EPV];
-    int col[CAP + 8];
+    __attribute__((aligned(16))) C col[CAP + 2 * EPC];
+    int tab[CODED ? 256 : 1];
   };
   struct Regs {
     ChunkStream<T, NV> v;
-    ChunkStream<int, NC> c;
+    ChunkStream<C, NC> c;
     int r0, nr, s, e, rs, re;
   };
 
@@ -159,7 +171,18 @@ struct StreamEngine {
     // and never staged; its loads here still stay inside the arrays.  No use
     // of the loaded values here: they stay in flight until put().
     g.v.fetch(val, nnz, s, e);
-    g.c.fetch(m.col, nnz, s, e);
+    if constexpr (CODED)
+      g.c.fetch(m.code, nnz, s, e);
+    else
+      g.c.fetch(m.col, nnz, s, e);
+  }
+
+  // column of staged entry j of row i
+  __device__ __forceinline__ static int column(const Smem& sm, int i, int j, int oc) {
+    if constexpr (CODED)
+      return i + sm.tab[sm.col[j - oc]];
+    else
+      return sm.col[j - oc];
   }
 
   template <bool SUB, typename Init, typename Epi>
@@ -168,6 +191,10 @@ struct StreamEngine {
                                              double* s_tmp, double* s_out, Init init, Epi epi) {
     const int tid = threadIdx.x;
     const int nnz = m.row[m.n];
+    if constexpr (CODED) {
+      for (int t = tid; t < m.ntab; t += kBlock) sm.tab[t] = m.tab[t];
+      __syncthreads();
+    }
     // The gather pointer is laundered through an empty asm (keeping the
     // global address space) so its loads are not hoisted or sunk across the
     // prefetch as freely as loads through a __restrict__ pointer.
@@ -191,7 +218,9 @@ struct StreamEngine {
       g.v.put(sm.val, nnz, s, e);
       g.c.put(sm.col, nnz, s, e);
       __syncthreads();
-      const int ov = s & ~(EPV - 1), oc = s & ~3;
+      const int ov = s & ~(EPV - 1), oc = s & ~(EPC - 1);
+      // the thread's row (threads past the block's rows take its last row)
+      const int i = r0 + min(tid, nr - 1);
       // first eight terms of the row: gathers issued unconditionally with the
       // index clamped into the row (rows are never empty)
       T av[8], xv[8];
@@ -199,7 +228,7 @@ struct StreamEngine {
       for (int u = 0; u < 8; ++u) {
         const int j = min(rs + u, re - 1);
         av[u] = sm.val[j - ov] * fn;
-        xv[u] = xg[sm.col[j - oc]];
+        xv[u] = xg[column(sm, i, j, oc)];
       }
       // next block's loads, in flight while this block is summed (past the
       // last block the current one is fetched again: no branch, so the
@@ -212,7 +241,6 @@ struct StreamEngine {
       }
       // every thread sums (threads past the block's rows on clamped data), so
       // the gathers above have an unconditional use and stay where they are
-      const int i = r0 + min(tid, nr - 1);
       T acc = init(i);
 #pragma unroll
       for (int u = 0; u < 8; ++u)
@@ -220,7 +248,8 @@ struct StreamEngine {
       if (tid < nr) {
         for (int j = rs + 8; j < re; ++j) {
           const T a = sm.val[j - ov] * fn;
-          acc = SUB ? acc - a * x[sm.col[j - oc]] : acc + a * x[sm.col[j - oc]];
+          const T xj = x[column(sm, i, j, oc)];
+          acc = SUB ? acc - a * xj : acc + a * xj;
         }
         epi(i, acc, xv[0], av[0]);
       }
@@ -254,7 +283,7 @@ struct OneBlock {
     g.v.put(sm.val, nnz, s, e);
     g.c.put(sm.col, nnz, s, e);
     __syncthreads();
-    const int ov = s & ~(EPV - 1), oc = s & ~3;
+    const int ov = s & ~(EPV - 1), oc = s & ~(SE::EPC - 1);
     if (tid < nr) {
       const int i = r0 + tid;
       T acc = init(i);
@@ -413,6 +442,10 @@ struct WaveEngine {
         break;                                                                     \
       case 5:                                                                      \
         if ((L).vec_val) { using E = StreamEngine<2048, true, T>; CALL; }          \
+        else { using E = ScalarStage<2048, T>; CALL; }                             \
+        break;                                                                     \
+      case 6:                                                                      \
+        if ((L).vec_val) { using E = StreamEngine<2048, true, T, true>; CALL; }    \
         else { using E = ScalarStage<2048, T>; CALL; }                             \
         break;                                                                     \
       case 2: { using E = WaveEngine<4, T>; CALL; } break;                         \

@@ -62,12 +62,13 @@ def lib():
         P, I, D, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
         L.tfg_matrix_create.argtypes = [I, I, I, P, P, P, ctypes.POINTER(P)]
         L.tfg_matrix_destroy.argtypes = [P]
-        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P]
+        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P, P]
         L.tfg_solve_native.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I,
                                        P, D, P, P]
         L.tfg_solve_native_ex.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P,
                                           I, P, D, P, P, I, P, P]
         L.tfg_spmv.argtypes = [P, P, P, P]
+        L.tfg_prec_apply.argtypes = [P, ctypes.c_char_p, I, P, P, P]
         L.tfg_residual.argtypes = [P, P, P, P, P]
         L.tfg_spmv_bench.argtypes = [P, P, P, P, I, ctypes.POINTER(D)]
         L.tfg_spmv_tune.argtypes = [P, P, ctypes.POINTER(I), P]
@@ -170,10 +171,14 @@ class Native:
         self.h = h
 
     def info(self):
-        vals = [ctypes.c_int() for _ in range(5)]
+        vals = [ctypes.c_int() for _ in range(6)]
         _check(lib().tfg_matrix_info(self.h, *[ctypes.byref(v) for v in vals]))
-        return dict(zip(["n", "nnz", "spmv_kind", "spmv_grid", "stage_blocks"],
+        return dict(zip(["n", "nnz", "spmv_kind", "spmv_grid", "stage_blocks", "offsets"],
                         [v.value for v in vals]))
+
+    def kinds(self):
+        """The SpMV engines this matrix offers (engine 6 needs offset codes)."""
+        return list(range(len(self.SPMV_KINDS) if self.info()["offsets"] else 6))
 
     def close(self):
         if getattr(self, "h", None):
@@ -226,18 +231,25 @@ class Native:
         _check(lib().tfg_spmv(self.h, dval.ptr, dp.ptr, dq.ptr))
         _check(lib().tfg_synchronize(self.h))
 
+    def prec_apply(self, prec, dval, dr, dz):
+        """z = M^-1 r with the reference's preconditioner of the unscaled matrix."""
+        p = prec.encode()
+        _check(lib().tfg_prec_apply(self.h, p, len(p), dval.ptr, dr.ptr, dz.ptr))
+        _check(lib().tfg_synchronize(self.h))
+
     def residual(self, dval, db, dx, dr):
         _check(lib().tfg_residual(self.h, dval.ptr, db.ptr, dx.ptr, dr.ptr))
         _check(lib().tfg_synchronize(self.h))
 
-    SPMV_KINDS = ("stage2048", "stage4096", "wave4", "wave8", "wave16", "pipe2048")
+    SPMV_KINDS = ("stage2048", "stage4096", "wave4", "wave8", "wave16", "pipe2048",
+                  "pipe2048c")
 
     def spmv_tune(self, dval):
         """Time every SpMV engine on dval; returns (chosen kind, {name: us})."""
         k = ctypes.c_int()
         us = (ctypes.c_float * len(self.SPMV_KINDS))()
         _check(lib().tfg_spmv_tune(self.h, dval.ptr, ctypes.byref(k), us))
-        return k.value, dict(zip(self.SPMV_KINDS, list(us)))
+        return k.value, {n: t for n, t in zip(self.SPMV_KINDS, list(us)) if t >= 0}
 
     def spmv_select(self, kind):
         _check(lib().tfg_spmv_select(self.h, int(kind)))

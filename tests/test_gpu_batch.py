@@ -63,13 +63,16 @@ def test_batch_parity(cases, oracle, gpu, prec, crit, norm):
     for k, s in enumerate(systems):
         xs = x[bt.offsets[k]:bt.offsets[k + 1]]
         it_o, res_o, x_o = _oracle(oracle, s, prec, miter, tol, norm, cname)
-        _, _, x_e = _oracle(oracle, s, prec, miter, tol, norm, cname, sum_order=1)
-        _, _, x_f = _oracle(oracle, s, prec, miter, tol, norm, cname, sum_order=2)
+        it_e, _, x_e = _oracle(oracle, s, prec, miter, tol, norm, cname, sum_order=1)
+        it_f, _, x_f = _oracle(oracle, s, prec, miter, tol, norm, cname, sum_order=2)
         env = max(_rel(x_e, x_o), _rel(x_f, x_o))
+        # unpreconditioned CG on the 1:1000 system: reordering the sums alone
+        # moves the iteration count by more than 2; the bar widens likewise
+        it_env = max(abs(it_e - it_o), abs(it_f - it_o))
         err = _rel(xs, x_o)
-        print(f"  system {k} {s.name}: niter {niter[k]} (oracle {it_o}) res {fin[k]:.3e} "
-              f"(oracle {res_o:.3e}) err {err:.2e} env {env:.2e}", flush=True)
-        assert abs(int(niter[k]) - it_o) <= 2
+        print(f"  system {k} {s.name}: niter {niter[k]} (oracle {it_o}, reordered {it_e} {it_f}) "
+              f"res {fin[k]:.3e} (oracle {res_o:.3e}) err {err:.2e} env {env:.2e}", flush=True)
+        assert abs(int(niter[k]) - it_o) <= max(2, 2 * it_env)
         if it_o < miter:
             assert fin[k] < tol
         assert err <= max(1e-10, 2.0 * env)

@@ -72,7 +72,7 @@ def test_parity_fp64(cases, oracle, gpu, name, make, solver, prec, norm, miter, 
     print(f"  oracle niter={it_o} res={res_o:.4e}; gpu niter={it_g} res={res_g:.4e}; "
           f"x rel err={err:.2e} (reordered-oracle envelope {env:.2e}, niter {it_e}); "
           f"true_relres={rep['true_relres']:.3e}", flush=True)
-    assert abs(it_g - it_o) <= 2
+    assert abs(it_g - it_o) <= max(2, 2 * abs(it_e - it_o))
     if it_o < miter:
         assert res_g < tol
     else:
@@ -102,14 +102,17 @@ def test_bicg_equals_cg_bitwise(cases, gpu):
     assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2])
 
 
-def test_every_engine_same_solve(cases, oracle, gpu):
+@pytest.mark.parametrize("which", ["knn10", "twophase20"])
+def test_every_engine_same_solve(cases, oracle, gpu, which):
     """Every SpMV engine solves to the same answer.  The products are
     bit-identical; the engines' grids differ, so the (p,q) partial sums are
-    combined in a different (fixed) order and the runs agree to rounding."""
-    s = cases.knn(10, 10, 10)
+    combined in a different (fixed) order and the runs agree to rounding.
+    (The kNN matrix has too many distinct offsets for the coded engine; the
+    hex matrix offers it.)"""
+    s = cases.knn(10, 10, 10) if which == "knn10" else cases.hex_pressure(20, rho_out=10.0)
     it_o, _, x_o = _oracle_solve(oracle, s, "cg", "jacobi", 500, 1e-9)
     nat = gpu.Native(s.row, s.col, s.dia)
-    for kind in range(6):
+    for kind in nat.kinds():
         nat.spmv_select(kind)
         r1 = _gpu_solve(gpu, s, "cg", "jacobi", 500, 1e-9, nat=nat)
         r2 = _gpu_solve(gpu, s, "cg", "jacobi", 500, 1e-9, nat=nat)

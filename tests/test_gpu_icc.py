@@ -1,0 +1,94 @@
+"""The reference's default preconditioner, incomplete Cholesky (DIC), on the
+GPU: z = M^-1 r bit-identical to the oracle's serial sweeps, and ICC-CG
+solves at the FP64 parity bar."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("which", ["poisson12", "twophase10x8x6", "knn8", "upwind9", "line7"])
+def test_prec_apply_bitexact(cases, oracle, gpu, which):
+    s = {"poisson12": lambda: cases.hex_pressure(12),
+         "twophase10x8x6": lambda: cases.hex_pressure(10, 8, 6, rho_out=1000.0),
+         "knn8": lambda: cases.knn(8, 8, 8),
+         "upwind9": lambda: cases.upwind(9),
+         "line7": lambda: cases.hex_pressure(1, 1, 7)}[which]()
+    print(f"icc apply case {s.name} n={s.n} nnz={s.nnz}", flush=True)
+    r = np.random.default_rng(5).standard_normal(s.n)
+    A = oracle.Matrix(s.row, s.col, s.dia, s.val)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    dv, dr, dz = gpu.DeviceArray(s.val), gpu.DeviceArray(r), gpu.DeviceArray(np.zeros(s.n))
+    for prec in ("icc", "jacobi", "none"):
+        d, dinv = oracle.prec_form(A, prec)
+        z_ref = oracle.prec_solve(A, d, dinv, r, prec)
+        nat.prec_apply(prec, dv, dr, dz)
+        z = dz.download()
+        assert np.array_equal(z, z_ref), (prec, np.abs(z - z_ref).max())
+
+
+CASES = [
+    ("poisson24", lambda c: c.hex_pressure(24), None, 1000, 1e-8),
+    ("twophase20_norm1", lambda c: c.hex_pressure(20, rho_out=1000.0), 1.0, 2000, 1e-8),
+    ("knn10", lambda c: c.knn(10, 10, 10), None, 1000, 1e-8),
+]
+
+
+@pytest.mark.parametrize("name,make,norm,miter,tol", CASES, ids=[c[0] for c in CASES])
+def test_icc_cg_parity(cases, oracle, gpu, name, make, norm, miter, tol):
+    s = make(cases)
+    print(f"icc-cg case {name} n={s.n} nnz={s.nnz}", flush=True)
+    res = []
+    for so in (0, 1, 2):
+        A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
+        x = np.zeros(s.n)
+        it, fr = oracle.solve("cg", "icc", A, x, s.b.copy(), miter, tol, norm=norm, sum_order=so)
+        res.append((it, fr, x))
+    it_o, res_o, x_o = res[0]
+    env = max(_rel(res[1][2], x_o), _rel(res[2][2], x_o))
+    nat = gpu.Native(s.row, s.col, s.dia)
+    x = np.zeros(s.n)
+    it_g, res_g = nat.solve_native("cg", "icc", s.val.copy(), x, s.b.copy(), miter, tol, norm=norm)
+    err = _rel(x, x_o)
+    print(f"  oracle niter={it_o} res={res_o:.3e}; gpu niter={it_g} res={res_g:.3e}; "
+          f"err {err:.2e} envelope {env:.2e}", flush=True)
+    assert it_o < miter
+    it_env = max(abs(res[1][0] - it_o), abs(res[2][0] - it_o))
+    assert abs(it_g - it_o) <= max(2, 2 * it_env) and res_g < tol
+    assert err <= max(1e-10, 2.0 * env)
+
+
+def test_icc_bicg_equals_cg_and_caps(cases, oracle, gpu):
+    s = cases.hex_pressure(14, rho_out=50.0)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    xa, xb = np.zeros(s.n), np.zeros(s.n)
+    ra = nat.solve_native("cg", "icc", s.val.copy(), xa, s.b.copy(), 500, 1e-9)
+    rb = nat.solve_native("bicg", "icc", s.val.copy(), xb, s.b.copy(), 500, 1e-9)
+    assert ra == rb and np.array_equal(xa, xb)
+    for miter in (0, 1, 3):
+        A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
+        xo = np.zeros(s.n)
+        it_o, res_o = oracle.solve("cg", "icc", A, xo, s.b.copy(), miter, 1e-30)
+        x = np.zeros(s.n)
+        it_g, res_g = nat.solve_native("cg", "icc", s.val.copy(), x, s.b.copy(), miter, 1e-30)
+        assert it_g == it_o == miter and abs(res_g - res_o) <= 1e-12 * res_o
+        assert _rel(x, xo) < 1e-13 if miter else np.array_equal(x, xo)
+
+
+def test_icc_fewer_iterations_than_jacobi(cases, gpu):
+    s = cases.hex_pressure(32, rho_out=1000.0)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    o = gpu.Options()
+    o.criterion = gpu.CRIT_RELRES
+    out = {}
+    for prec in ("jacobi", "icc"):
+        x = np.zeros(s.n)
+        it, res, rep = nat.solve_native("cg", prec, s.val.copy(), x, s.b.copy(), 5000, 1e-8,
+                                        options=o, report=True)
+        out[prec] = (it, rep["solve_ms"])
+        print(f"  {prec}: {it} iterations, {rep['solve_ms']:.1f} ms", flush=True)
+    assert out["icc"][0] < out["jacobi"][0]

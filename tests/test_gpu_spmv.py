@@ -28,7 +28,7 @@ def _systems(cases):
             cases.upwind(20), cases.knn(14, 12, 10), cases.hex_pressure(1, 1, 7)]
 
 
-KINDS = range(6)  # stage2048, stage4096, wave4, wave8, wave16, pipe2048
+# stage2048, stage4096, wave4, wave8, wave16, pipe2048, pipe2048 with offset codes
 
 
 @pytest.mark.parametrize("idx", range(5))
@@ -43,7 +43,8 @@ def test_spmv_bitexact_every_engine(cases, oracle, gpu, idx):
     nat = gpu.Native(s.row, s.col, s.dia)
     dv, dp, dq = gpu.DeviceArray(s.val), gpu.DeviceArray(p), gpu.DeviceArray(np.zeros(s.n))
     db, dr = gpu.DeviceArray(s.b), gpu.DeviceArray(np.zeros(s.n))
-    for kind in KINDS:
+    print(f"  engines {nat.kinds()} (offsets {nat.info()['offsets']})", flush=True)
+    for kind in nat.kinds():
         nat.spmv_select(kind)
         nat.spmv(dv, dp, dq)
         q = dq.download()
@@ -53,7 +54,7 @@ def test_spmv_bitexact_every_engine(cases, oracle, gpu, idx):
     kind, us = nat.spmv_tune(dv)
     print(f"  autotuner picked {nat.SPMV_KINDS[kind]}: " +
           ", ".join(f"{k} {v:.1f} us" for k, v in us.items()), flush=True)
-    assert 0 <= kind < 6 and us[nat.SPMV_KINDS[kind]] == min(us.values())
+    assert kind in nat.kinds() and us[nat.SPMV_KINDS[kind]] == min(us.values())
 
 
 def test_spmv_misaligned_values(cases, oracle, gpu):
@@ -65,7 +66,7 @@ def test_spmv_misaligned_values(cases, oracle, gpu):
     buf = gpu.DeviceArray(np.concatenate([[0.0], s.val]))
     dv = _View(buf, 8)
     dp, dq = gpu.DeviceArray(p), gpu.DeviceArray(np.zeros(s.n))
-    for kind in KINDS:
+    for kind in nat.kinds():
         nat.spmv_select(kind)
         nat.spmv(dv, dp, dq)
         assert np.array_equal(dq.download(), oracle.spmv(A, p)), kind
@@ -92,7 +93,7 @@ def test_spmv_long_rows(oracle, gpu):
     q_ref = oracle.spmv(A, p)
     nat = gpu.Native(row, col, dia)
     dv, dp, dq = gpu.DeviceArray(val), gpu.DeviceArray(p), gpu.DeviceArray(np.zeros(n))
-    for kind in KINDS:
+    for kind in nat.kinds():
         nat.spmv_select(kind)
         nat.spmv(dv, dp, dq)
         q = dq.download()
