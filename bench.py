@@ -48,8 +48,11 @@ def cpu_baseline(s, gpu_iters, seconds_budget=15.0):
     1 thread (the reference's default OMP=no build) and all host cores (its
     OMP=yes analogue); returns the projected time to the GPU's iteration
     count."""
-    probe = _time_oracle(s, 2, False) / 2.0
-    iters = int(max(3, min(200, seconds_budget / max(probe, 1e-6))))
+    # marginal time per iteration (the solve's set-up excluded), then a sample
+    # of about seconds_budget
+    t2, t6 = _time_oracle(s, 2, False), _time_oracle(s, 6, False)
+    probe = max((t6 - t2) / 4.0, 1e-6)
+    iters = int(max(3, min(1000, (seconds_budget - t2) / probe)))
     dt = _time_oracle(s, iters, False)
     per_iter = dt / iters
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
@@ -254,7 +257,7 @@ def bench_knn_ir(args):
     dv, db = G.DeviceArray(s.val), G.DeviceArray(s.b)
     dx = G.DeviceArray(np.zeros(s.n))
 
-    def run(precision, tol, inner_max=0):
+    def run(precision, tol, inner_max=0, miter=100000):
         opt = G.Options()
         opt.precision = precision
         opt.criterion = G.CRIT_RELRES
@@ -264,8 +267,10 @@ def bench_knn_ir(args):
         for _ in range(args.steps + args.warmup):
             dx.upload(np.zeros(s.n))
             t = time.perf_counter()
-            it, res, rep = nat.solve_native_device("cg", "jacobi", dv, dx, db, 100000, tol,
+            it, res, rep = nat.solve_native_device("cg", "jacobi", dv, dx, db, miter, tol,
                                                    options=opt, report=True)
+            log(f"bench knn: precision {precision} tol {tol} inner_max {inner_max}: "
+                f"{time.perf_counter() - t:.3f} s, {it} iterations, relres {res:.2e}")
             dt = time.perf_counter() - t
             if best is None or dt < best[0]:
                 best = (dt, it, res, rep, dx.download())
@@ -273,9 +278,12 @@ def bench_knn_ir(args):
 
     ref = run(G.FP64, 1e-13)
     out_runs = {}
-    for name, prec, tol, im in (("fp64_cg", G.FP64, 1e-10, 0), ("fp32_ir", G.FP32_IR, 1e-10, 50),
-                                ("fp32_ir_tighter", G.FP32_IR, 1e-12, 50)):
-        dt, it, res, rep, x = run(prec, tol, im)
+    # FP32 sweeps: refinement whenever the inner residual fell by 1e-4, and
+    # the restart-every-50 form, both capped at 20000 inner iterations
+    for name, prec, tol, im in (("fp64_cg", G.FP64, 1e-10, 0), ("fp32_ir", G.FP32_IR, 1e-10, 0),
+                                ("fp32_ir_restart50", G.FP32_IR, 1e-10, 50),
+                                ("fp32_ir_tighter", G.FP32_IR, 1e-12, 0)):
+        dt, it, res, rep, x = run(prec, tol, im, 100000 if prec == G.FP64 else 20000)
         err = float(np.linalg.norm(x - ref[4]) / np.linalg.norm(ref[4]))
         out_runs[name] = {"seconds": dt, "iterations": it, "refinements": rep["refinements"],
                           "relres": res, "true_relres": rep["true_relres"],
@@ -328,7 +336,11 @@ def bench_batch(args):
         if best is None or dt < best[0]:
             best = (dt, niter.copy(), fin.copy(), rep, bt.times())
     dt, niter, fin, rep, done_ms = best
-    log(f"bench batch: {dt:.3f} s wall, device {rep['solve_ms']:.1f} ms, max niter {niter.max()}")
+    # algorithmic bytes of the whole batch: every system's iterations at
+    # 12 nnz + 108 n each (the single-system CG iteration)
+    batch_bytes = float(sum(int(it) * (12 * s.nnz + 108 * s.n) for it, s in zip(niter, systems)))
+    log(f"bench batch: {dt:.3f} s wall, device {rep['solve_ms']:.1f} ms, max niter {niter.max()}, "
+        f"{batch_bytes / (rep['solve_ms'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
     # one after another through the single-system drop-in (device-resident)
     seq_ms, seq_it = [], []
     for k, s in enumerate(systems):
@@ -353,9 +365,12 @@ def bench_batch(args):
                    "blocks": bt.info()["blocks"], "max_iterations": int(niter.max()),
                    "all_converged": bool(np.all(fin < 1e-8))},
         "batch": {"device_ms": rep["solve_ms"], "wall_s": dt,
+                  "GBps_algorithmic": batch_bytes / (rep["solve_ms"] * 1e-3) / 1e9,
+                  "iterations_total": int(niter.sum()), "iterations_median": float(np.median(niter)),
                   "p50_system_ms": float(np.percentile(done_ms, 50)),
                   "p99_system_ms": float(np.percentile(done_ms, 99))},
         "sequential": {"device_ms": float(seq_ms.sum()),
+                       "GBps_algorithmic": batch_bytes / (seq_ms.sum() * 1e-3) / 1e9,
                        "solves_per_s": count / (seq_ms.sum() * 1e-3),
                        "p50_system_ms": float(np.percentile(seq_ms, 50)),
                        "p99_system_ms": float(np.percentile(seq_ms, 99)),

@@ -69,7 +69,8 @@ typedef struct tfg_matrix tfg_matrix;
  * Native_Type % Create_Native / Matrix_Type % Create_Matrix
  * (Native_Mod/Create_Native.f90:14-21, Matrix_Mod/Create_Matrix.f90:40-195).
  * ni: rows solved (n_cells - n_buff_cells); nt: rows stored (n_cells);
- * nnz = row[nt] - 1.  Only ni == nt (one domain) is supported. */
+ * nnz = row[nt] - 1.  ni < nt (an MPI rank's matrix) needs the hooks of
+ * tfg_matrix_set_comm before a solve. */
 int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col,
                       const int* dia, tfg_matrix** handle);
 int tfg_matrix_destroy(tfg_matrix* handle);
@@ -133,9 +134,9 @@ int tfg_synchronize(tfg_matrix* h);
  * the last reps.  Its known traffic calibrates the HBM byte counters. */
 int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, double* ms_avg);
 
-/* SpMV engines: 0 LDS-staged rows (2048 nonzeros per block), 1 the same
- * with 4096, 2/3/4 sub-wavefront rows with 4/8/16 lanes per row, 5 the
- * 2048 stage with the next block's loads in flight, 6 the same reading a
+/* SpMV engines: 0 LDS-staged rows (2040 nonzeros per block), 1 the same
+ * with 4088, 2/3/4 sub-wavefront rows with 4/8/16 lanes per row, 5 the
+ * 2040 stage with the next block's loads in flight, 6 the same reading a
  * 1-byte code per nonzero instead of its int32 column (col = row + table[code];
  * available when the matrix has at most 256 distinct col - row offsets, as
  * every structured grid has).  All give bit-identical products; the engine
@@ -151,11 +152,35 @@ int tfg_spmv_tune(tfg_matrix* h, const double* d_val, int* kind, float* us);
 int tfg_spmv_select(tfg_matrix* h, int kind);
 
 /* Facts about the handle: sizes, chosen SpMV engine (-1: not chosen yet),
- * its grid, the number of 2048-nonzero row blocks, and the number of
+ * its grid, the number of 2040-nonzero row blocks, and the number of
  * distinct col - row offsets when engine 6 is available (else 0).  Any
  * pointer may be NULL. */
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind,
                     int* spmv_grid, int* stage_blocks, int* offsets);
+
+/* ---- several MPI ranks -----------------------------------------------------
+ * A matrix with buffer cells (ni < nt: rows ni+1..nt are copies of other
+ * ranks' cells) is solved with the caller's communication, which stands in
+ * for Comm_Mod where Cg.f90 calls it:
+ *   allreduce(ctx, values, ops, count): combine `count` host doubles in place
+ *     over all ranks, ops[i] = TFG_SUM | TFG_MIN | TFG_MAX
+ *     (Global % Sum_Real / Min_Real / Max_Real,
+ *     Comm_Mod/Parallel/Global/Sum_Real.f90 and its siblings);
+ *   exchange(ctx, d_x): refresh the buffer cells d_x[ni..nt-1] of a device
+ *     vector from their owners (Grid % Exchange_Cells_Real,
+ *     Grid_Mod/Parallel/Exchange_Cells_Real.f90:2-129); d_x[0..ni-1] is current.
+ * Both return 0 on success; anything else aborts the solve (TFG_EINVAL).
+ * Called from the thread that runs the solve, with the device idle.  Every
+ * rank makes the same calls in the same order.  FP64 cg / bicg with jacobi
+ * or none; x's buffer cells must be current on entry, as in the reference,
+ * and are refreshed at exit (Cg.f90:240). */
+#define TFG_SUM 0
+#define TFG_MIN 1
+#define TFG_MAX 2
+typedef int (*tfg_allreduce_fn)(void* ctx, double* values, const int* ops, int count);
+typedef int (*tfg_exchange_fn)(void* ctx, double* d_x);
+int tfg_matrix_set_comm(tfg_matrix* h, tfg_allreduce_fn allreduce,
+                        tfg_exchange_fn exchange, void* ctx);
 
 /* ---- batched solves ------------------------------------------------------
  * No counterpart in the reference, which solves one system per Solve_Native

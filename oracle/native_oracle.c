@@ -24,6 +24,7 @@
 /* Sources/Shared/Const_Mod.f90:84-93 */
 #define OR_HUGE 1.0e+15
 #define OR_TINY 1.0e-15
+#define OR_MICRO 1.0e-6
 
 /* Summation order of the global reductions.  0 (default) is the reference's
  * serial left-to-right order.  1 (2) sums blocks of 64 (1000) terms left to
@@ -579,4 +580,51 @@ done:
   free(w);
   free(tpos);
   return status;
+}
+
+/* ------------------------------------------------------------------------
+ * Pressure-system assembly from face data: A % val = 0 and b = 0
+ * (Compute_Pressure.f90:121-122), the matrix lines of the Rhie and Chow face
+ * loop over internal faces (Rhie_And_Chow.f90:184-205), the right-hand side
+ * from the volume fluxes and the diagonal terms of boundary faces with a
+ * pressure / ambient condition (Compute_Pressure.f90:147-169), b = 0 in
+ * pressure regions (:173-178), and Set_Singular's a_ii * (1 + MICRO)
+ * (Solver_Mod/Set_Singular.f90:28-31) when `singular`.  Serial, in face order.
+ * c1, c2: 1-based cells (c2 <= 0: boundary face); pos: 2 x n_faces 1-based
+ * CSR positions of (c1,c2) and (c2,c1); diag_bc / b_zero: per-face flags.
+ * ------------------------------------------------------------------------ */
+void or_pressure_assemble(int nc, int nnz, int n_faces, const int* c1, const int* c2,
+                          const int* pos, const int* dia, const int* diag_bc,
+                          const int* b_zero, const double* fc, const double* fw,
+                          const double* vm, const double* vflux, double* val, double* b,
+                          int singular) {
+  for (int j = 0; j < nnz; j++) val[j] = 0.0;
+  for (int i = 0; i < nc; i++) b[i] = 0.0;
+  for (int s = 0; s < n_faces; s++) {           /* Rhie_And_Chow.f90:184-205 */
+    if (c2[s] <= 0) continue;
+    const int a = c1[s] - 1, c = c2[s] - 1;
+    const double fs = fw[s];
+    const double a12 = fc[s] * (fs * vm[a] + (1.0 - fs) * vm[c]);
+    val[pos[2 * s] - 1] = -a12;
+    val[pos[2 * s + 1] - 1] = -a12;
+    val[dia[a] - 1] = val[dia[a] - 1] + a12;
+    val[dia[c] - 1] = val[dia[c] - 1] + a12;
+  }
+  for (int s = 0; s < n_faces; s++) {           /* Compute_Pressure.f90:147-169 */
+    const int a = c1[s] - 1;
+    if (c2[s] > 0) {
+      b[a] = b[a] - vflux[s];
+      b[c2[s] - 1] = b[c2[s] - 1] + vflux[s];
+    } else {
+      b[a] = b[a] - vflux[s];
+      if (diag_bc[s]) {
+        const double a12 = fc[s] * vm[a];
+        val[dia[a] - 1] = val[dia[a] - 1] + a12;
+      }
+    }
+  }
+  for (int s = 0; s < n_faces; s++)             /* :173-178, pressure regions */
+    if (b_zero[s]) b[c1[s] - 1] = 0.0;
+  if (singular)                                 /* Set_Singular.f90:28-31 */
+    for (int i = 0; i < nc; i++) val[dia[i] - 1] = val[dia[i] - 1] * (1.0 + OR_MICRO);
 }

@@ -63,6 +63,15 @@ int or_krylov(or_matrix* A, double* x, double* b, int method, int prec, int mite
               double* omegas);
 int or_transpose_positions(const or_matrix* A, int* tpos);
 
+/* Pressure-system assembly from face data (Rhie_And_Chow.f90:184-205,
+ * Compute_Pressure.f90:121-178, Set_Singular.f90:28-31), serial in face
+ * order; see native_oracle.c. */
+void or_pressure_assemble(int nc, int nnz, int n_faces, const int* c1, const int* c2,
+                          const int* pos, const int* dia, const int* diag_bc,
+                          const int* b_zero, const double* fc, const double* fw,
+                          const double* vm, const double* vflux, double* val, double* b,
+                          int singular);
+
 /* 0: the reference's serial summation (default); 1, 2: blocked-pairwise
  * summation (blocks of 64, 1000) of every global reduction, to measure
  * rounding sensitivity. */

@@ -79,7 +79,7 @@ struct tfg_matrix {
   // structure (0-based)
   int* row = nullptr;
   int* col = nullptr;
-  int* blk[2] = {nullptr, nullptr};  // stage row blocks for 2048 / 4096 nonzeros
+  int* blk[2] = {nullptr, nullptr};  // stage row blocks of kCap / kCap2 nonzeros
   int nblk[2] = {0, 0};
   int grid[tfg::kKinds] = {};         // resident SpMV workgroups per engine
   int kind = -1;                      // chosen SpMV engine (-1: tune on first use)
@@ -120,6 +120,11 @@ struct tfg_matrix {
   double* icc_dinv = nullptr;
   double* icc_y = nullptr;
   double* icc_z = nullptr;
+  // multi-rank hooks
+  tfg_allreduce_fn allreduce = nullptr;
+  tfg_exchange_fn exchange = nullptr;
+  void* comm_ctx = nullptr;
+  double* loc = nullptr;   // device: this rank's totals of a reduction
   // offset-coded columns (structured grids): col = row + tab[code]
   uint8_t* code = nullptr;
   int* tab = nullptr;
@@ -190,7 +195,7 @@ tfg::Launch launch_of(const tfg_matrix* h, const void* val, int kind) {
 
 tfg::SpmvShape shape_of(const tfg_matrix* h, int kind) {
   tfg::SpmvShape m;
-  m.n = h->nt;
+  m.n = h->ni;
   m.row = h->row;
   m.col = h->col;
   const int b = kind == tfg::K_STAGE4096 ? 1 : 0;
@@ -249,7 +254,7 @@ void free_all(tfg_matrix* h) {
   void* dev[] = {h->row, h->col, h->blk[0], h->blk[1], h->val, h->b, h->x, h->r, h->p, h->q, h->ad,
                  h->part, h->st, h->st_in, h->val32, h->e32, h->r32,
                  h->p32, h->q32, h->ad32, h->rt, h->nv[0], h->nv[1], h->nv[2],
-                 h->nv[3], h->valt, h->tpos, h->code, h->tab, h->icc_rows[0],
+                 h->nv[3], h->valt, h->tpos, h->code, h->tab, h->loc, h->icc_rows[0],
                  h->icc_rows[1], h->icc_d, h->icc_dinv, h->icc_y, h->icc_z};
   for (void* d : dev)
     if (d) (void)hipFree(d);
@@ -262,7 +267,7 @@ void free_all(tfg_matrix* h) {
   delete h;
 }
 
-// Engine choice.  By default the pipelined 2048-nonzero stream engine (the
+// Engine choice.  By default the pipelined ~2K-nonzero stream engine (the
 // fastest on every T-Flows-like structure measured).  The choice fixes the
 // SpMV launch grid and with it the tree of the fused dot products, so a
 // timed choice could differ between two runs on the same matrix and move
@@ -579,10 +584,146 @@ int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int mi
   return status;
 }
 
+// One rank of a multi-rank FP64 CG (Cg.f90 with Comm_Mod's calls): every
+// global scalar goes through the caller's allreduce, p's buffer cells through
+// its exchange before each product, x's at exit.
+int solve_comm(tfg_matrix* h, int prec, const double* val, const double* b, int miter,
+               int* niter, double tol, double* fin_res, const double* norm,
+               const tfg_options& o, tfg_report* rep) {
+  { const int rt = ensure_tuned(h, val); if (rt != TFG_OK) return rt; }
+  if (!h->loc) HIP_TRY(hipMalloc(&h->loc, 8 * sizeof(double)));
+  const tfg::Launch L = launch_of(h, val, h->kind);
+  tfg::CgArgs a = args_of(h, h->kind);
+  a.val = val;
+  a.b = b;
+  a.x = h->x;
+  a.r = h->r;
+  a.p = h->p;
+  a.q = h->q;
+  a.ad = h->ad;
+  a.pt = partials_of(h, L);
+  a.st = h->st;
+  const hipStream_t S = h->stream;
+  State* s0 = &h->hst[0];
+  memset(s0, 0, sizeof(State));
+  s0->tol = tol;
+  s0->criterion = o.criterion;
+  s0->has_norm = norm ? 1 : 0;
+  s0->norm = norm ? *norm : 0.0;
+  s0->comm = 1;
+  HIP_TRY(hipMemcpyAsync(h->st, s0, sizeof(State), hipMemcpyHostToDevice, S));
+  double tot[8];
+  // this rank's totals of nv partial rows, then the caller's allreduce
+  auto reduce = [&](const double* part, int G, int nv, const int* ops) -> int {
+    HIP_TRY(tfg::launch_cm_local(S, part, h->pstride, G, nv, ops, h->loc));
+    HIP_TRY(hipMemcpyAsync(tot, h->loc, sizeof(double) * nv, hipMemcpyDeviceToHost, S));
+    HIP_TRY(hipStreamSynchronize(S));
+    int rops[8];
+    for (int q = 0; q < nv; ++q) rops[q] = ops[q] == tfg::OP_SUM ? TFG_SUM : (ops[q] == tfg::OP_MIN ? TFG_MIN : TFG_MAX);
+    if (h->allreduce(h->comm_ctx, tot, rops, nv) != 0) return fail(TFG_EINVAL, "allreduce hook failed");
+    return TFG_OK;
+  };
+  auto set_field = [&](double* dfield, double v) -> int {
+    HIP_TRY(hipMemcpy(dfield, &v, sizeof(double), hipMemcpyHostToDevice));
+    return TFG_OK;
+  };
+  auto exchange = [&](double* d_x) -> int {
+    HIP_TRY(hipStreamSynchronize(S));
+    if (h->exchange(h->comm_ctx, d_x) != 0) return fail(TFG_EINVAL, "exchange hook failed");
+    return TFG_OK;
+  };
+#define TFG_CM(expr) do { const int rc_ = (expr); if (rc_ != TFG_OK) return rc_; } while (0)
+  HIP_TRY(hipEventRecord(h->ev_t0, S));
+  // fn = 1 / (sum_a / sum_n) over all ranks                     Cg.f90:62-76
+  HIP_TRY(tfg::launch_diag_sum(L, a));
+  const int sum1[1] = {tfg::OP_SUM};
+  TFG_CM(reduce(a.pt.diag, L.grid_vec, 1, sum1));
+  {
+    double vals[2] = {tot[0], (double)h->ni};
+    const int ops2[2] = {TFG_SUM, TFG_SUM};
+    if (h->allreduce(h->comm_ctx, vals + 1, ops2 + 1, 1) != 0) return fail(TFG_EINVAL, "allreduce hook failed");
+    TFG_CM(set_field(&h->st->fn, 1.0 / (vals[0] / vals[1])));
+  }
+  // r0 = b - A x and the start sums                            Cg.f90:95-123
+  HIP_TRY(tfg::launch_prologue(L, a, prec));
+  const int pro_ops[7] = {tfg::OP_SUM, tfg::OP_SUM, tfg::OP_MIN, tfg::OP_MAX,
+                          tfg::OP_SUM, tfg::OP_SUM, tfg::OP_SUM};
+  TFG_CM(reduce(a.pt.pro, L.grid_spmv, 7, pro_ops));
+  const double range0 = norm ? std::max(*norm - 0.0, tfg::kTiny) : std::max(tot[3] - tot[2], tfg::kTiny);
+  const double bnorm = std::sqrt(tot[4]);
+  double bn, res;
+  if (o.criterion == TFG_CRIT_RELRES) {
+    bn = bnorm > 0.0 ? 1.0 : 0.0;
+    res = std::sqrt(tot[5]) / bnorm;
+  } else {
+    bn = std::sqrt(tot[0]) / range0;
+    res = std::sqrt(tot[1]) / range0;
+  }
+  int iters = 0, status = TFG_OK;
+  double rho = tot[6];
+  if (bn < tol) {
+    res = 0.0;
+  } else if (!(res < tol)) {
+    const int upd_ops[5] = {tfg::OP_SUM, tfg::OP_MIN, tfg::OP_MAX, tfg::OP_SUM, tfg::OP_SUM};
+    int k = 1;
+    for (; k <= miter; ++k) {
+      HIP_TRY(tfg::launch_cm_pupdate(L, a, prec, k));                  // :145-171
+      TFG_CM(exchange(h->p));                                          // :176
+      HIP_TRY(tfg::launch_spmv_pq(L, a, k));                           // :177-195
+      TFG_CM(reduce(a.pt.spmv, L.grid_spmv, 1, sum1));                 // :197
+      const double pq = tot[0];
+      if (pq == 0.0 || !std::isfinite(pq)) { status = TFG_EBREAKDOWN; break; }
+      TFG_CM(set_field(&h->st->alpha, rho / pq));                      // :198
+      HIP_TRY(tfg::launch_cm_update(L, a, prec));                      // :204-209
+      TFG_CM(reduce(a.pt.upd, L.grid_vec, 5, upd_ops));                // NRMS :38,53-54
+      const double range = norm ? std::max(*norm - 0.0, tfg::kTiny) : std::max(tot[2] - tot[1], tfg::kTiny);
+      res = o.criterion == TFG_CRIT_RELRES ? std::sqrt(tot[4]) / bnorm : std::sqrt(tot[0]) / range;
+      if (!std::isfinite(res)) { status = TFG_EBREAKDOWN; break; }
+      if (res < tol) break;                                            // :220
+      const double rho_new = tot[3];                                   // :150-156 of k+1
+      TFG_CM(set_field(&h->st->beta, rho_new / rho));                  // :165
+      rho = rho_new;
+    }
+    iters = std::min(k, miter);
+  }
+  TFG_CM(exchange(h->x));                                              // :240
+  HIP_TRY(hipEventRecord(h->ev_t1, S));
+  HIP_TRY(hipEventSynchronize(h->ev_t1));
+#undef TFG_CM
+  *niter = iters;
+  *fin_res = res;
+  if (rep) {
+    rep->status = status;
+    rep->iterations = iters;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev_t0, h->ev_t1));
+    rep->solve_ms = ms;
+  }
+  if (o.true_residual) {
+    HIP_TRY(tfg::launch_true_residual(L, a));
+    State* sf = &h->hst[3];
+    HIP_TRY(hipMemcpyAsync(sf, h->st, sizeof(State), hipMemcpyDeviceToHost, S));
+    HIP_TRY(hipStreamSynchronize(S));
+    if (rep) rep->true_relres = sf->aux0;  // this rank's rows only
+  }
+  return status;
+}
+
 }  // namespace tfg
 
 // ===========================================================================
 extern "C" {
+
+int tfg_matrix_set_comm(tfg_matrix* h, tfg_allreduce_fn allreduce, tfg_exchange_fn exchange,
+                        void* ctx) {
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  if ((allreduce == nullptr) != (exchange == nullptr))
+    return fail(TFG_EINVAL, "allreduce and exchange hooks go together");
+  h->allreduce = allreduce;
+  h->exchange = exchange;
+  h->comm_ctx = ctx;
+  return TFG_OK;
+}
 
 const char* tfg_last_error(void) { return tfg::g_err.c_str(); }
 
@@ -593,8 +734,6 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
   if (!handle || !row || !col || !dia) return fail(TFG_EINVAL, "null argument");
   *handle = nullptr;
   if (nt < 1 || ni < 1 || ni > nt) return fail(TFG_EINVAL, "bad sizes ni=%d nt=%d", ni, nt);
-  if (ni != nt)
-    return fail(TFG_ENOTSUP, "buffer cells (ni=%d < nt=%d) need halo exchange hooks", ni, nt);
   { const int rc = tfg::check_csr(nt, nnz, row, col, dia); if (rc != TFG_OK) return rc; }
 
   tfg_matrix* h = new tfg_matrix();
@@ -617,8 +756,9 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
     // Row blocks of the two stage engines (make_blocks reads 1-based starts;
     // only differences matter).
     std::vector<int> blk[2];
-    make_blocks(row, nt, 2048, blk[0]);
-    make_blocks(row, nt, 4096, blk[1]);
+    // products and sums run over the solved rows 1..ni only
+    make_blocks(row, ni, tfg::kCap, blk[0]);
+    make_blocks(row, ni, tfg::kCap2, blk[1]);
     e = hipMalloc(&h->row, sizeof(int) * (size_t)(nt + 1));
     if (e == hipSuccess) e = hipMalloc(&h->col, sizeof(int) * (size_t)nnz);
     for (int b = 0; b < 2 && e == hipSuccess; ++b) {
@@ -774,6 +914,9 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     return fail(TFG_EINVAL, "unknown criterion %d", o.criterion);
   if (o.precision != TFG_FP64 && o.precision != TFG_FP32_IR)
     return fail(TFG_EINVAL, "unknown precision %d", o.precision);
+  if (h->ni < h->nt && !h->allreduce)
+    return fail(TFG_ENOTSUP, "buffer cells (ni=%d < nt=%d) need the tfg_matrix_set_comm hooks",
+                h->ni, h->nt);
   if (rep) {
     memset(rep, 0, sizeof(*rep));
     rep->true_relres = -1.0;
@@ -795,7 +938,12 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
   }
 
   int rc;
-  if (o.precision == TFG_FP32_IR) {
+  if (h->allreduce) {
+    if (method != tfg::NS_NONE || pr == tfg::PREC_ICC || o.precision != TFG_FP64)
+      return fail(TFG_ENOTSUP, "several ranks: FP64 cg / bicg with jacobi or none");
+    rc = tfg::solve_comm(h, pr, dval, db, miter, niter, tol, fin_res, norm, o, rep);
+    if (rc != TFG_OK && rc != TFG_EBREAKDOWN) return rc;
+  } else if (o.precision == TFG_FP32_IR) {
     rc = tfg::solve_ir(h, pr, dval, db, miter, niter, tol, fin_res, norm, o, rep);
     if (rc != TFG_OK && rc != TFG_EBREAKDOWN) return rc;
   } else {
@@ -858,6 +1006,8 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     HIP_TRY(tfg::launch_start(L, a, (method == tfg::NS_CGS || method == tfg::NS_BCGS) ? 5 : 6));
     if (method != tfg::NS_NONE) HIP_TRY(tfg::launch_ns_init(L, a, ns));
 
+    // x update deferred into the next p update when no x range is needed
+    const bool dx = method == tfg::NS_NONE && !icc && (o.criterion == TFG_CRIT_RELRES || norm);
     const int chunk = h->nnz > (1 << 22) ? 8 : 32;
     int k = 1, slot = 0, prev = -1;
     bool stop = false;
@@ -879,11 +1029,11 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
           if (smp) HIP_TRY(hipEventRecord(e1, h->stream));
           HIP_TRY(tfg::launch_update(L, a, tfg::PREC_NONE, k));
         } else {
-          HIP_TRY(tfg::launch_pupdate(L, a, pr, k));
+          HIP_TRY(tfg::launch_pupdate(L, a, pr, k, dx));
           if (smp) HIP_TRY(hipEventRecord(e0, h->stream));
           HIP_TRY(tfg::launch_spmv_pq(L, a, k));
           if (smp) HIP_TRY(hipEventRecord(e1, h->stream));
-          HIP_TRY(tfg::launch_update(L, a, pr, k));
+          HIP_TRY(tfg::launch_update(L, a, pr, k, dx));
         }
         if (smp) sample_iter.push_back(k);
       }
@@ -897,6 +1047,7 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
       prev = slot;
       slot ^= 1;
     }
+    if (dx) HIP_TRY(tfg::launch_fold_x(L, a));
     HIP_TRY(hipEventRecord(h->ev_t1, h->stream));
     State* sf = &h->hst[3];
     HIP_TRY(hipMemcpyAsync(sf, h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));

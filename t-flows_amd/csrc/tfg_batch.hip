@@ -6,7 +6,7 @@
 //
 // Layout: the systems' CSR arrays are concatenated into one block-diagonal
 // matrix (global 0-based rows and columns).  Row blocks (<= 256 rows,
-// <= 2048 nonzeros) never straddle two systems.  Every block-level kernel
+// <= kCap nonzeros) never straddle two systems.  Every block-level kernel
 // runs one workgroup per block of the ACTIVE list (blocks of systems not yet
 // stopped) and publishes one partial per block; a one-workgroup-per-system
 // kernel then sweeps that system's partials in a fixed order and writes its
@@ -373,7 +373,7 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
       if (rc != TFG_OK) return fail(rc, "system %d: %s", s, tfg_last_error());
       for (int i = 0; i < n[s]; ++i) grow[ro + i] = (int)(co + rs[i] - 1);
       for (int j = 0; j < nnz[s]; ++j) gcol[co + j] = (int)(ro + col[co + j] - 1);
-      tfg::make_blocks(rs, n[s], 2048, b1);
+      tfg::make_blocks(rs, n[s], tfg::kCap, b1);
       for (size_t t = 1; t < b1.size(); ++t) {
         blk.push_back((int)(ro + b1[t]));
         bsys.push_back(s);
@@ -520,8 +520,8 @@ int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len, const char
 
 #define TFG_BT_ENGINE(CALL)                                                 \
   do {                                                                      \
-    if (vec) { using E = OneBlock<2048, double>; CALL; }               \
-    else { using E = ScalarStage<2048, double>; CALL; }                     \
+    if (vec) { using E = OneBlock<kCap, double>; CALL; }               \
+    else { using E = ScalarStage<kCap, double>; CALL; }                     \
   } while (0)
 
   HIP_TRY(hipEventRecord(h->ev0, S));

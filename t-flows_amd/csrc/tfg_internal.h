@@ -10,11 +10,18 @@ namespace tfg {
 
 constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kRowsPerBlock = 256; // SpMV rows per staged block (one per thread)
+// Nonzeros per staged block of the stream engines: 2040 (4088) fit 4 (8)
+// 16-byte chunks per thread with the chunk of misalignment at each end.
+constexpr int kCap = 2040;
+constexpr int kCap2 = 4088;
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dealt round robin
 constexpr double kHuge = 1.0e+15;  // Const_Mod.f90:84-93 (HUGE, TINY)
 constexpr double kTiny = 1.0e-15;
 
 enum Prec { PREC_NONE = 0, PREC_JACOBI = 1, PREC_ICC = 2 };
+
+// Reduction operators of the partial sums (same codes as TFG_SUM/MIN/MAX).
+enum RedOp { OP_SUM = 0, OP_MIN = 1, OP_MAX = 2 };
 
 // Scalars of one solve, resident in device memory.  Each field is written by
 // one thread (workgroup 0 of a kernel, or a one-workgroup kernel) and read by
@@ -36,8 +43,11 @@ struct State {
   int niter;        // iteration at which done was raised
   int status;       // TFG_OK or TFG_EBREAKDOWN
   int iter;         // last iteration whose stopping test was evaluated
-  double alpha;     // CGS / BiCGSTAB: alpha of the current iteration
+  double alpha;     // CGS / BiCGSTAB, deferred-x CG: alpha of the current iteration
   double omega;     // BiCGSTAB: omega of the last iteration
+  int fold;         // deferred-x CG: x += alpha p of the last update still to do
+  double beta;      // multi-rank CG: beta from the global reduction
+  int comm;         // multi-rank CG: fn is given (global), not computed
 };
 
 // Partial sums written by one kernel and swept by the next ones.
@@ -149,9 +159,10 @@ int spmv_blocks_per_cu(int kind);
 hipError_t launch_diag_sum(const Launch& L, const CgArgs& a);
 hipError_t launch_prologue(const Launch& L, const CgArgs& a, int prec);
 hipError_t launch_start(const Launch& L, const CgArgs& a, int rho_slot = 6);
-hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k);
+hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k, bool dx = false);
+hipError_t launch_fold_x(const Launch& L, const CgArgs& a);
 hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k);
-hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k);
+hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k, bool dx = false);
 hipError_t launch_finish(const Launch& L, const CgArgs& a, int k, int check_omega = 0);
 
 // DIC preconditioner (tfg_icc.hip).  st = null: unscaled matrix (fn = 1)
@@ -164,6 +175,12 @@ hipError_t launch_icc_rho(const Launch& L, const CgArgs& a, const double* z);
 hipError_t launch_icc_pupdate(const Launch& L, const CgArgs& a, const double* z, int k);
 hipError_t launch_jacobi_apply(hipStream_t s, int n, const int* row, const double* val,
                                const double* r, double* z);
+
+// Multi-rank CG (tfg_comm.hip).
+hipError_t launch_cm_local(hipStream_t s, const double* part, int pstride, int G, int nv,
+                           const int* ops, double* out);
+hipError_t launch_cm_pupdate(const Launch& L, const CgArgs& a, int prec, int k);
+hipError_t launch_cm_update(const Launch& L, const CgArgs& a, int prec);
 
 // Textbook methods (tfg_nonsym.hip).
 hipError_t launch_ns_init(const Launch& L, const CgArgs& a, const NsArgs& s);

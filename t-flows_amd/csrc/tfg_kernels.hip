@@ -74,14 +74,17 @@ template <class E, int PREC>
 __global__ __launch_bounds__(kBlock) void k_prologue(CgArgs a) {
   __shared__ typename E::Smem sm;
   __shared__ double s_tmp[7 * 4], s_out[7];
-  {
+  double fn;
+  if (a.st->comm) {
+    fn = a.st->fn;  // from the global sums (multi-rank solve)
+  } else {
     const int ops1[1] = {OP_SUM};
     gather<1>(a.pt.diag, a.pt.pstride, a.pt.g_vec, ops1, s_tmp, s_out);
+    const double sum_a = s_out[0] / (double)a.n;
+    fn = 1.0 / sum_a;
+    if (lead()) a.st->fn = fn;
+    __syncthreads();
   }
-  const double sum_a = s_out[0] / (double)a.n;
-  const double fn = 1.0 / sum_a;
-  if (lead()) a.st->fn = fn;
-  __syncthreads();
   double v[7] = {0.0, 0.0, kHuge, -kHuge, 0.0, 0.0, 0.0};
   E::template run<true>(
       sm, shape_of(a), a.val, fn, a.x, s_tmp, s_out, [&](int i) { return a.b[i] * fn; },
@@ -140,7 +143,12 @@ __global__ __launch_bounds__(kBlock) void k_start(CgArgs a, int rho_slot) {
 // iteration k-1 (Cg.f90:214-222) and beta = rho_k / rho_{k-1}
 // (Cg.f90:165); then p = z + beta p, or p = z when k = 1 (Cg.f90:158-171),
 // with z = r * d_inv (Prec_Solve.f90:44-49) recomputed on the fly.
-template <int PREC>
+// DX (deferred x, used when the stopping test needs no x range: relres, or
+// NRMS with norm): x += alpha_{k-1} p_{k-1} (Cg.f90:206) is done here, on
+// the p this kernel reads anyway, instead of in k_update -- the same
+// operation on the same operands, one iteration later, 8n bytes fewer per
+// iteration.  The last iteration's term is folded by k_fold_x.
+template <int PREC, bool DX>
 __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
   __shared__ double s_tmp[5 * 4], s_out[5];
   State* st = a.st;
@@ -167,11 +175,14 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
       st->rho[k & 1] = rho;
       st->res = res;
       st->iter = k - 1;
+      if (DX) st->fold = 0;
     }
   }
+  const double alpha = DX && k > 1 ? st->alpha : 0.0;
   const int stride = gridDim.x * kBlock;
   const int npair = a.n >> 1;
   dbl2* p2 = reinterpret_cast<dbl2*>(a.p);
+  dbl2* x2 = reinterpret_cast<dbl2*>(a.x);
   const dbl2* r2 = reinterpret_cast<const dbl2*>(a.r);
   const dbl2* d2 = reinterpret_cast<const dbl2*>(a.ad);
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < npair; t += stride) {
@@ -188,6 +199,13 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
       p2[t] = z;
     } else {
       const dbl2 pv = p2[t];
+      if (DX) {
+        const dbl2 xv = x2[t];
+        dbl2 xo;
+        xo.x = xv.x + alpha * pv.x;
+        xo.y = xv.y + alpha * pv.y;
+        x2[t] = xo;
+      }
       dbl2 o;
       o.x = z.x + beta * pv.x;
       o.y = z.y + beta * pv.y;
@@ -197,6 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
   if ((a.n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
     const int i = a.n - 1;
     const double z = PREC == PREC_JACOBI ? a.r[i] * (1.0 / a.ad[i]) : a.r[i];
+    if (DX && k > 1) a.x[i] = a.x[i] + alpha * a.p[i];
     a.p[i] = (k == 1) ? z : z + beta * a.p[i];
   }
 }
@@ -223,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_pq(CgArgs a, int k) {
 // alpha = rho / (p,q) (Cg.f90:190-198); x += alpha p, r -= alpha q
 // (Cg.f90:204-209); partials of the NRMS sum, the x range, the next (z, r)
 // and ||r||^2 on the new r and x.
-template <int PREC>
+template <int PREC, bool DX>
 __global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
   __shared__ double s_tmp[5 * 4], s_out[5];
   State* st = a.st;
@@ -242,7 +261,27 @@ __global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
     return;
   }
   const double alpha = st->rho[k & 1] / pq;
+  if (DX && lead()) {
+    st->alpha = alpha;
+    st->fold = 1;
+  }
   double v[5] = {0.0, kHuge, -kHuge, 0.0, 0.0};
+  if (DX) {
+    // r -= alpha q and the sums that need no x (x is folded in k_pupdate)
+    const int stride = gridDim.x * kBlock;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) {
+      const double dv = a.ad[i];
+      const double rv = a.r[i] - alpha * a.q[i];
+      a.r[i] = rv;
+      const double z = PREC == PREC_JACOBI ? rv * (1.0 / dv) : rv;
+      v[0] = v[0] + (rv * rv) / (dv * dv);
+      v[3] = v[3] + z * rv;
+      v[4] = v[4] + rv * rv;
+    }
+    const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
+    publish<5>(v, ops, a.pt.upd, a.pt.pstride, s_tmp, s_out);
+    return;
+  }
   auto one = [&](double& xv, double& rv, double pv, double qv, double dv) {
     xv = xv + alpha * pv;
     rv = rv - alpha * qv;
@@ -312,6 +351,15 @@ __global__ __launch_bounds__(kBlock) void k_finish(CgArgs a, int k, int check_om
   }
 }
 
+// The deferred x += alpha_K p_K of the last iteration that updated r.
+__global__ __launch_bounds__(kBlock) void k_fold_x(CgArgs a) {
+  const State* st = a.st;
+  if (!st->fold) return;
+  const double alpha = st->alpha;
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) a.x[i] = a.x[i] + alpha * a.p[i];
+}
+
 // ||b - A x||^2 and ||b||^2 on the unscaled system (FP64), partials.
 template <class E>
 __global__ __launch_bounds__(kBlock) void k_true_residual(CgArgs a) {
@@ -360,12 +408,12 @@ int spmv_blocks_per_cu(int kind) {
   int nb = 0;
   hipError_t e = hipSuccess;
   switch (kind) {
-    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, false, double>>, kBlock, 0); break;
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<4096, false, double>>, kBlock, 0); break;
+    case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, false, double>>, kBlock, 0); break;
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap2, false, double>>, kBlock, 0); break;
     case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<4, double>>, kBlock, 0); break;
     case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<8, double>>, kBlock, 0); break;
-    case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, true, double>>, kBlock, 0); break;
-    case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<2048, true, double, true>>, kBlock, 0); break;
+    case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, true, double>>, kBlock, 0); break;
+    case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, true, double, true>>, kBlock, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<16, double>>, kBlock, 0); break;
   }
   if (e != hipSuccess || nb < 1) return 1;
@@ -390,9 +438,19 @@ hipError_t launch_start(const Launch& L, const CgArgs& a, int rho_slot) {
   return hipGetLastError();
 }
 
-hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k) {
-  if (prec == PREC_JACOBI) k_pupdate<PREC_JACOBI><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
-  else k_pupdate<PREC_NONE><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k, bool dx) {
+  if (prec == PREC_JACOBI) {
+    if (dx) k_pupdate<PREC_JACOBI, true><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+    else k_pupdate<PREC_JACOBI, false><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+  } else {
+    if (dx) k_pupdate<PREC_NONE, true><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+    else k_pupdate<PREC_NONE, false><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fold_x(const Launch& L, const CgArgs& a) {
+  k_fold_x<<<L.grid_vec, kBlock, 0, L.stream>>>(a);
   return hipGetLastError();
 }
 
@@ -401,9 +459,14 @@ hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k) {
   return hipGetLastError();
 }
 
-hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k) {
-  if (prec == PREC_JACOBI) k_update<PREC_JACOBI><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
-  else k_update<PREC_NONE><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k, bool dx) {
+  if (prec == PREC_JACOBI) {
+    if (dx) k_update<PREC_JACOBI, true><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+    else k_update<PREC_JACOBI, false><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+  } else {
+    if (dx) k_update<PREC_NONE, true><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+    else k_update<PREC_NONE, false><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+  }
   return hipGetLastError();
 }
 

@@ -15,7 +15,12 @@
 
 namespace tfg {
 
-enum RedOp { OP_SUM = 0, OP_MIN = 1, OP_MAX = 2 };
+
+// Reduction operators passed by value to a kernel.
+template <int NV>
+struct Ops {
+  int v[NV];
+};
 
 // Thread 0 of workgroup 0: the one thread that writes a kernel's scalars.
 __device__ __forceinline__ bool lead_thread() { return blockIdx.x == 0 && threadIdx.x == 0; }

@@ -426,27 +426,27 @@ struct WaveEngine {
   }
 };
 
-// kind: 0 stage/2048, 1 stage/4096, 2 wave/4, 3 wave/8, 4 wave/16,
-// 5 pipelined stage/2048.  The stage engines fall back to scalar staging
+// kind: 0 stage/kCap, 1 stage/kCap2, 2 wave/4, 3 wave/8, 4 wave/16,
+// 5 pipelined stage/kCap, 6 the same with offset-coded columns.  The stage engines fall back to scalar staging
 // (ScalarStage) when val is not 16-byte aligned or nnz < 4.
 #define TFG_ENGINE_SWITCH_T(L, T, CALL)                                            \
   do {                                                                             \
     switch ((L).kind) {                                                            \
       case 0:                                                                      \
-        if ((L).vec_val) { using E = StreamEngine<2048, false, T>; CALL; }         \
-        else { using E = ScalarStage<2048, T>; CALL; }                             \
+        if ((L).vec_val) { using E = StreamEngine<kCap, false, T>; CALL; }         \
+        else { using E = ScalarStage<kCap, T>; CALL; }                             \
         break;                                                                     \
       case 1:                                                                      \
-        if ((L).vec_val) { using E = StreamEngine<4096, false, T>; CALL; }         \
-        else { using E = ScalarStage<4096, T>; CALL; }                             \
+        if ((L).vec_val) { using E = StreamEngine<kCap2, false, T>; CALL; }         \
+        else { using E = ScalarStage<kCap2, T>; CALL; }                             \
         break;                                                                     \
       case 5:                                                                      \
-        if ((L).vec_val) { using E = StreamEngine<2048, true, T>; CALL; }          \
-        else { using E = ScalarStage<2048, T>; CALL; }                             \
+        if ((L).vec_val) { using E = StreamEngine<kCap, true, T>; CALL; }          \
+        else { using E = ScalarStage<kCap, T>; CALL; }                             \
         break;                                                                     \
       case 6:                                                                      \
-        if ((L).vec_val) { using E = StreamEngine<2048, true, T, true>; CALL; }    \
-        else { using E = ScalarStage<2048, T>; CALL; }                             \
+        if ((L).vec_val) { using E = StreamEngine<kCap, true, T, true>; CALL; }    \
+        else { using E = ScalarStage<kCap, T>; CALL; }                             \
         break;                                                                     \
       case 2: { using E = WaveEngine<4, T>; CALL; } break;                         \
       case 3: { using E = WaveEngine<8, T>; CALL; } break;                         \

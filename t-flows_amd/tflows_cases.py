@@ -129,3 +129,88 @@ def batch_params(count, smin=20, smax=80, rmax=1000.0, seed=DEFAULT_SEED):
 
 def hex_nnz(nx, ny, nz):
     return int(lib().tfc_hex_nnz(nx, ny, nz))
+
+
+@dataclass
+class Faces:
+    """Face data of a hex grid in the reference's layout (Grid_Mod faces_c,
+    Matrix_Type pos / dia): 1-based cells c1, c2 (c2 <= 0 on the boundary),
+    the CSR positions pos (2 x n_faces, 1-based; 0 on the boundary) of the
+    pressure matrix, per-face fc, interpolation weight fw, volume flux, and
+    the boundary flags of a pressure condition on the x = 0 side."""
+    row: np.ndarray
+    col: np.ndarray
+    dia: np.ndarray
+    c1: np.ndarray
+    c2: np.ndarray
+    pos: np.ndarray
+    diag_bc: np.ndarray
+    b_zero: np.ndarray
+    fc: np.ndarray
+    fw: np.ndarray
+    vm: np.ndarray
+    vflux: np.ndarray
+
+    @property
+    def n(self):
+        return len(self.dia)
+
+    @property
+    def n_faces(self):
+        return len(self.c1)
+
+
+def hex_faces(nx, ny=None, nz=None, seed=DEFAULT_SEED, shuffle=True):
+    """Faces of an nx*ny*nz hex grid (cells x slowest, z fastest, as the
+    pressure generators number them) with seeded face data: fc = A/d times
+    U(0.5, 2), fw ~ U(0.3, 0.7), v_m ~ U(0.5, 2) * 1e-3, flux ~ U(-1, 1).
+    Faces are listed in a seeded random order (shuffle) so that assembly
+    order matters."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = nx * ny * nz
+    s = hex_pressure(nx, ny, nz)       # structure: diagonal first, columns ascending
+    idx = np.arange(n).reshape(nx, ny, nz)
+    pairs = [np.stack([idx[:-1].ravel(), idx[1:].ravel()], 1),
+             np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1),
+             np.stack([idx[:, :, :-1].ravel(), idx[:, :, 1:].ravel()], 1)]
+    internal = np.concatenate(pairs)
+    bnd_cells, bnd_x0 = [], []
+    for axis, size in ((0, nx), (1, ny), (2, nz)):
+        for side in (0, size - 1):
+            sl = [slice(None)] * 3
+            sl[axis] = side
+            cells = idx[tuple(sl)].ravel()
+            bnd_cells.append(cells)
+            bnd_x0.append(np.full(cells.size, axis == 0 and side == 0))
+    bnd_cells = np.concatenate(bnd_cells)
+    bnd_x0 = np.concatenate(bnd_x0)
+    ni_, nb = len(internal), len(bnd_cells)
+    c1 = np.concatenate([internal[:, 0], bnd_cells]) + 1
+    c2 = np.concatenate([internal[:, 1] + 1, -(np.arange(nb) + 1)])
+    flag = np.concatenate([np.zeros(ni_, bool), bnd_x0])
+    # CSR positions of (c1, c2) in row c1 and (c2, c1) in row c2
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(s.row))
+    keys = rows * n + (s.col - 1)
+    korder = np.argsort(keys, kind="stable")
+    skeys = keys[korder]
+
+    def position(r, c):
+        q = r.astype(np.int64) * n + c
+        return korder[np.searchsorted(skeys, q)]
+
+    pos = np.zeros((len(c1), 2), dtype=np.int32)
+    pos[:ni_, 0] = position(internal[:, 0], internal[:, 1]) + 1
+    pos[:ni_, 1] = position(internal[:, 1], internal[:, 0]) + 1
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(c1)) if shuffle else np.arange(len(c1))
+    h = 1.0 / nx
+    nf = len(c1)
+    return Faces(row=s.row, col=s.col, dia=s.dia,
+                 c1=np.ascontiguousarray(c1[order], dtype=np.int32),
+                 c2=np.ascontiguousarray(c2[order], dtype=np.int32),
+                 pos=np.ascontiguousarray(pos[order].ravel(), dtype=np.int32),
+                 diag_bc=np.ascontiguousarray(flag[order], dtype=np.int32),
+                 b_zero=np.ascontiguousarray(flag[order], dtype=np.int32),
+                 fc=h * rng.uniform(0.5, 2.0, nf), fw=rng.uniform(0.3, 0.7, nf),
+                 vm=1e-3 * rng.uniform(0.5, 2.0, n), vflux=rng.uniform(-1.0, 1.0, nf))

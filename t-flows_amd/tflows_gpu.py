@@ -52,6 +52,14 @@ class Report(ctypes.Structure):
 _lib = None
 
 
+SUM, MIN, MAX = 0, 1, 2
+# int allreduce(void* ctx, double* values, const int* ops, int count)
+ALLREDUCE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_int), ctypes.c_int)
+# int exchange(void* ctx, double* d_x)
+EXCHANGE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
+
+
 def lib():
     """Load libtflows_gpu.so (raises if it is not built)."""
     global _lib
@@ -79,6 +87,7 @@ def lib():
         L.tfg_memcpy_d2h.argtypes = [P, P, LL]
         L.tfg_synchronize.argtypes = [P]
         L.tfg_stream_copy.argtypes = [P, P, LL, I, ctypes.POINTER(D)]
+        L.tfg_matrix_set_comm.argtypes = [P, ALLREDUCE_FN, EXCHANGE_FN, P]
         L.tfg_batch_create.argtypes = [I, P, P, P, P, P, ctypes.POINTER(P)]
         L.tfg_batch_destroy.argtypes = [P]
         L.tfg_batch_solve.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I, P,
@@ -230,6 +239,28 @@ class Native:
     def spmv(self, dval, dp, dq):
         _check(lib().tfg_spmv(self.h, dval.ptr, dp.ptr, dq.ptr))
         _check(lib().tfg_synchronize(self.h))
+
+    def set_comm(self, allreduce, exchange):
+        """Multi-rank hooks.  allreduce(values: np.ndarray, ops: list) combines
+        in place over the ranks; exchange(d_x: int device address) refreshes
+        the buffer cells.  Both return None (or raise: the solve then fails)."""
+        def ar(ctx, vals, ops, n):
+            try:
+                v = np.ctypeslib.as_array(vals, shape=(n,))
+                allreduce(v, [ops[i] for i in range(n)])
+                return 0
+            except Exception:  # an exception must not cross the C frame
+                return 1
+
+        def ex(ctx, dx):
+            try:
+                exchange(dx)
+                return 0
+            except Exception:
+                return 1
+
+        self._comm = (ALLREDUCE_FN(ar), EXCHANGE_FN(ex))  # keep alive
+        _check(lib().tfg_matrix_set_comm(self.h, self._comm[0], self._comm[1], None))
 
     def prec_apply(self, prec, dval, dr, dz):
         """z = M^-1 r with the reference's preconditioner of the unscaled matrix."""

@@ -43,9 +43,9 @@ def test_structure_validation_on_host(gpu):
     rc = gpu.lib().tfg_matrix_create(2, 2, 5, row.ctypes.data, col.ctypes.data, dia.ctypes.data,
                                      ctypes.byref(h))
     assert rc == gpu.TFG_EINVAL
-    # buffer cells (MPI halo) are not supported by the single-domain library
-    rc, msg = _create(gpu, 1, 2, i32([1, 2, 3]), i32([1, 2]), i32([1, 2]))
-    assert rc == gpu.TFG_ENOTSUP and "halo" in msg
+    # more solved rows than stored rows
+    rc, msg = _create(gpu, 3, 2, i32([1, 2, 3]), i32([1, 2]), i32([1, 2]))
+    assert rc == gpu.TFG_EINVAL and "sizes" in msg
     # null arguments
     assert gpu.lib().tfg_matrix_create(2, 2, 2, None, None, None, ctypes.byref(h)) == gpu.TFG_EINVAL
 
