@@ -382,6 +382,48 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_assembly(args):
+    """Pressure solve fed on the device: 200^3 face data (fc, f, v_m, fluxes)
+    assembled into A % val and b by the face kernel, against uploading the
+    host-built A % val and b (what a host-assembled drop-in call copies)."""
+    import tflows_cases
+    import tflows_gpu as G
+
+    N = args.n
+    t0 = time.perf_counter()
+    f = tflows_cases.hex_faces(N)
+    log(f"bench assembly: {N}^3, {f.n_faces} faces, generated in {time.perf_counter() - t0:.1f} s")
+    nat = G.Native(f.row, f.col, f.dia)
+    faces = G.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    d = {k: G.DeviceArray(getattr(f, k)) for k in ("fc", "fw", "vm", "vflux")}
+    dval, db = G.DeviceArray(np.zeros(len(f.col))), G.DeviceArray(np.zeros(f.n))
+    hval, hb = np.zeros(len(f.col)), np.zeros(f.n)
+    faces.assemble(d["fc"], d["fw"], d["vm"], d["vflux"], dval, db, singular=True)
+    reps = max(3, args.steps)
+    t = time.perf_counter()
+    for _ in range(reps):
+        faces.assemble(d["fc"], d["fw"], d["vm"], d["vflux"], dval, db, singular=True)
+    t_asm = (time.perf_counter() - t) / reps
+    hval[:] = dval.download()
+    hb[:] = db.download()
+    t = time.perf_counter()
+    for _ in range(reps):
+        dval.upload(hval)
+        db.upload(hb)
+    t_up = (time.perf_counter() - t) / reps
+    bytes_up = hval.nbytes + hb.nbytes
+    log(f"bench assembly: device assembly {t_asm * 1e3:.2f} ms, upload {t_up * 1e3:.2f} ms "
+        f"({bytes_up / t_up / 1e9:.1f} GB/s)")
+    print(json.dumps({
+        "metric": "pressure-system assembly on the device vs upload of A % val and b, 200^3",
+        "value": t_asm * 1e3, "unit": "ms", "n_gpus": 1, "steps": reps, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"hex {N}^3 face data, pressure matrix + rhs", "cells": f.n,
+                   "faces": f.n_faces, "nnz": len(f.col)},
+        "upload_ms": t_up * 1e3, "upload_bytes": bytes_up, "speedup_vs_upload": t_up / t_asm,
+    }), flush=True)
+
+
 def calibrate():
     """Known-traffic copy (1 GiB read + 1 GiB write) for counter calibration."""
     import tflows_gpu as G
@@ -404,7 +446,7 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the known-traffic copy kernel (counter calibration)")
     ap.add_argument("--workload", default="twophase200",
-                    choices=["twophase200", "upwind120", "knn4m", "batch256"],
+                    choices=["twophase200", "upwind120", "knn4m", "batch256", "assembly200"],
                     help="the headline solve (default) or one of the other measured workloads")
     ap.add_argument("--summarize-counters", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
                     help="write the traffic summary from four rocprofv3 counter_collection.csv files")
@@ -429,7 +471,8 @@ def main():
               "(the solver is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "twophase200":
-        {"upwind120": bench_upwind, "knn4m": bench_knn_ir, "batch256": bench_batch}[args.workload](args)
+        {"upwind120": bench_upwind, "knn4m": bench_knn_ir, "batch256": bench_batch,
+         "assembly200": bench_assembly}[args.workload](args)
         return
 
     import tflows_cases

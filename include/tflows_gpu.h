@@ -158,6 +158,31 @@ int tfg_spmv_select(tfg_matrix* h, int kind);
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind,
                     int* spmv_grid, int* stage_blocks, int* offsets);
 
+/* ---- pressure-system assembly on the device ------------------------------
+ * Builds A % val and b of the pressure-correction system from face data on
+ * the device, so that a pressure solve copies neither to the GPU: the
+ * matrix lines of the Rhie and Chow face loop (Rhie_And_Chow.f90:184-205),
+ * the right-hand side from the volume fluxes and the boundary diagonal terms
+ * (Compute_Pressure.f90:121-122, 147-169), b = 0 in pressure regions
+ * (:173-178) and, when `singular`, Set_Singular's a_ii * (1 + 1e-6)
+ * (Solver_Mod/Set_Singular.f90:28-31).  Bit-identical to the serial loops.
+ * Structure once per grid: c1, c2 = Grid % faces_c (1-based; c2 <= 0 on the
+ * boundary), pos = A % pos (2 x n_faces, 1-based, pos(1,s) then pos(2,s);
+ * ignored on the boundary), per boundary face diag_bc = 1 for a pressure or
+ * ambient condition, b_zero = 1 in a pressure region. */
+typedef struct tfg_faces tfg_faces;
+int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2,
+                     const int* pos, const int* diag_bc, const int* b_zero,
+                     tfg_faces** faces);
+int tfg_faces_destroy(tfg_faces* faces);
+/* Device pointers: fc = A % fc (n_faces), fw = Grid % f (n_faces), vm =
+ * M % v_m (nt), vflux = v_flux % n (n_faces); val (nnz) and b (nt) out.
+ * Stream-ordered on the matrix handle's stream (a following solve on the
+ * same handle sees the result). */
+int tfg_pressure_assemble(tfg_faces* faces, const double* fc, const double* fw,
+                          const double* vm, const double* vflux, double* val,
+                          double* b, int singular);
+
 /* ---- several MPI ranks -----------------------------------------------------
  * A matrix with buffer cells (ni < nt: rows ni+1..nt are copies of other
  * ranks' cells) is solved with the caller's communication, which stands in

@@ -60,6 +60,7 @@ def lib(omp=False):
         L.or_krylov.argtypes = [P, P, P, I, I, I, P, D, P, P, I, P]
         L.or_transpose_positions.restype = I
         L.or_transpose_positions.argtypes = [P, P]
+        L.or_pressure_assemble.argtypes = [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I]
         _libs[name] = L
     return _libs[name]
 
@@ -203,3 +204,16 @@ def transpose_positions(A):
     sparsity pattern), or None when the pattern is not symmetric."""
     t = np.zeros(len(A.col), dtype=np.int32)
     return t if lib().or_transpose_positions(A.ref, _p(t)) == 0 else None
+
+
+def pressure_assemble(f, singular=False):
+    """Serial pressure-system assembly (Rhie_And_Chow.f90:184-205,
+    Compute_Pressure.f90:121-178, Set_Singular.f90:28-31) from a face set
+    with the fields of tflows_cases.Faces; returns (val, b)."""
+    val = np.zeros(len(f.col))
+    b = np.zeros(len(f.dia))
+    arrs = [np.ascontiguousarray(a) for a in (f.c1, f.c2, f.pos, f.dia, f.diag_bc, f.b_zero,
+                                               f.fc, f.fw, f.vm, f.vflux)]
+    lib().or_pressure_assemble(len(f.dia), len(f.col), len(f.c1), *[_p(a) for a in arrs],
+                               _p(val), _p(b), 1 if singular else 0)
+    return val, b

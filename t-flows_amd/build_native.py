@@ -19,7 +19,7 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["csrc/tfg_api.hip", "csrc/tfg_kernels.hip", "csrc/tfg_ir.hip",
                "csrc/tfg_nonsym.hip", "csrc/tfg_batch.hip", "csrc/tfg_icc.hip",
-               "csrc/tfg_comm.hip"]
+               "csrc/tfg_comm.hip", "csrc/tfg_assembly.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # every product/update is rounded on its own, as in the reference's

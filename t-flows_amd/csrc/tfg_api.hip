@@ -714,6 +714,112 @@ int solve_comm(tfg_matrix* h, int prec, const double* val, const double* b, int 
 // ===========================================================================
 extern "C" {
 
+struct tfg_faces {
+  tfg_matrix* h = nullptr;
+  int n_faces = 0;
+  int *cf_off = nullptr, *cf_face = nullptr, *ent_face = nullptr, *c1 = nullptr, *c2 = nullptr;
+  uint8_t* flags = nullptr;
+};
+
+static void faces_free(tfg_faces* f) {
+  if (!f) return;
+  void* dev[] = {f->cf_off, f->cf_face, f->ent_face, f->c1, f->c2, f->flags};
+  for (void* d : dev)
+    if (d) (void)hipFree(d);
+  delete f;
+}
+
+int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2, const int* pos,
+                     const int* diag_bc, const int* b_zero, tfg_faces** faces) {
+  if (!h || !faces || !c1 || !c2 || !pos || !diag_bc || !b_zero || n_faces < 0)
+    return fail(TFG_EINVAL, "null argument");
+  *faces = nullptr;
+  const int n = h->nt;
+  // host copies of the structure (0-based)
+  std::vector<int> row(n + 1), col(h->nnz);
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(hipMemcpy(row.data(), h->row, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(col.data(), h->col, sizeof(int) * (size_t)h->nnz, hipMemcpyDeviceToHost));
+  std::vector<int> cnt(n + 1, 0), ent(h->nnz, -1), a(n_faces), b(n_faces);
+  std::vector<uint8_t> fl(n_faces);
+  for (int s = 0; s < n_faces; ++s) {
+    const int p = c1[s] - 1, q = c2[s] > 0 ? c2[s] - 1 : -1;
+    if (p < 0 || p >= n || q >= n) return fail(TFG_EINVAL, "face %d: cells out of range", s + 1);
+    a[s] = p;
+    b[s] = q;
+    fl[s] = (uint8_t)(q < 0 ? ((diag_bc[s] ? 1 : 0) | (b_zero[s] ? 2 : 0)) : 0);
+    cnt[p + 1]++;
+    if (q >= 0) {
+      cnt[q + 1]++;
+      const int j1 = pos[2 * s] - 1, j2 = pos[2 * s + 1] - 1;
+      if (j1 <= row[p] || j1 >= row[p + 1] || col[j1] != q || j2 <= row[q] || j2 >= row[q + 1] ||
+          col[j2] != p)
+        return fail(TFG_EINVAL, "face %d: pos does not point at (c1,c2) / (c2,c1)", s + 1);
+      ent[j1] = s;   // the last face that sets the entry wins, as in the loop
+      ent[j2] = s;
+    }
+  }
+  for (int c = 0; c < n; ++c) cnt[c + 1] += cnt[c];
+  std::vector<int> cf(cnt[n]), fill(cnt.begin(), cnt.end() - 1);
+  for (int s = 0; s < n_faces; ++s) {   // increasing face order within each cell
+    cf[fill[a[s]]++] = s << 1;
+    if (b[s] >= 0) cf[fill[b[s]]++] = (s << 1) | 1;
+  }
+  tfg_faces* f = new tfg_faces();
+  f->h = h;
+  f->n_faces = n_faces;
+  hipError_t e = hipSuccess;
+  auto up = [&](void** d, const void* src, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(d, bytes > 0 ? bytes : 4);
+    if (e == hipSuccess && bytes) e = hipMemcpy(*d, src, bytes, hipMemcpyHostToDevice);
+  };
+  up((void**)&f->cf_off, cnt.data(), sizeof(int) * cnt.size());
+  up((void**)&f->cf_face, cf.data(), sizeof(int) * cf.size());
+  up((void**)&f->ent_face, ent.data(), sizeof(int) * ent.size());
+  up((void**)&f->c1, a.data(), sizeof(int) * a.size());
+  up((void**)&f->c2, b.data(), sizeof(int) * b.size());
+  up((void**)&f->flags, fl.data(), fl.size());
+  if (e != hipSuccess) {
+    faces_free(f);
+    return fail(TFG_ENOMEM, "face structure upload: %s", hipGetErrorString(e));
+  }
+  *faces = f;
+  return TFG_OK;
+}
+
+int tfg_faces_destroy(tfg_faces* f) {
+  if (!f) return fail(TFG_EINVAL, "null handle");
+  (void)hipSetDevice(f->h->device);
+  (void)hipStreamSynchronize(f->h->stream);
+  faces_free(f);
+  return TFG_OK;
+}
+
+int tfg_pressure_assemble(tfg_faces* f, const double* fc, const double* fw, const double* vm,
+                          const double* vflux, double* val, double* b, int singular) {
+  if (!f || !fc || !fw || !vm || !vflux || !val || !b) return fail(TFG_EINVAL, "null argument");
+  tfg_matrix* h = f->h;
+  HIP_TRY(hipSetDevice(h->device));
+  tfg::AsmArgs g{};
+  g.n = h->nt;
+  g.row = h->row;
+  g.cf_off = f->cf_off;
+  g.cf_face = f->cf_face;
+  g.ent_face = f->ent_face;
+  g.c1 = f->c1;
+  g.c2 = f->c2;
+  g.flags = f->flags;
+  g.fc = fc;
+  g.fw = fw;
+  g.vm = vm;
+  g.vflux = vflux;
+  g.val = val;
+  g.b = b;
+  g.singular = singular ? 1 : 0;
+  HIP_TRY(tfg::launch_assemble(h->stream, g));
+  return TFG_OK;
+}
+
 int tfg_matrix_set_comm(tfg_matrix* h, tfg_allreduce_fn allreduce, tfg_exchange_fn exchange,
                         void* ctx) {
   if (!h) return fail(TFG_EINVAL, "null handle");

@@ -144,6 +144,25 @@ struct IccArgs {
   const int* off_b_host;  // host: nlev_b + 1 level starts in rows_b
 };
 
+// Pressure-system assembly from face data (tfg_assembly.hip).
+struct AsmArgs {
+  int n;                 // cells (rows) assembled: nt
+  const int* row;        // 0-based CSR row starts (diagonal first)
+  const int* cf_off;     // n+1: each cell's faces in cf_face
+  const int* cf_face;    // face index * 2 + (1 if the cell is the face's c2)
+  const int* ent_face;   // nnz: last face setting each off-diagonal entry, -1 none
+  const int* c1;         // n_faces, 0-based
+  const int* c2;         // n_faces, 0-based; -1 for a boundary face
+  const uint8_t* flags;  // n_faces: 1 boundary diagonal term, 2 b(c1) = 0
+  const double* fc;
+  const double* fw;
+  const double* vm;
+  const double* vflux;
+  double* val;
+  double* b;
+  int singular;
+};
+
 // Launch shapes chosen per matrix.
 struct Launch {
   hipStream_t stream;
@@ -175,6 +194,8 @@ hipError_t launch_icc_rho(const Launch& L, const CgArgs& a, const double* z);
 hipError_t launch_icc_pupdate(const Launch& L, const CgArgs& a, const double* z, int k);
 hipError_t launch_jacobi_apply(hipStream_t s, int n, const int* row, const double* val,
                                const double* r, double* z);
+
+hipError_t launch_assemble(hipStream_t stream, const AsmArgs& g);
 
 // Multi-rank CG (tfg_comm.hip).
 hipError_t launch_cm_local(hipStream_t s, const double* part, int pstride, int G, int nv,

@@ -88,6 +88,9 @@ def lib():
         L.tfg_synchronize.argtypes = [P]
         L.tfg_stream_copy.argtypes = [P, P, LL, I, ctypes.POINTER(D)]
         L.tfg_matrix_set_comm.argtypes = [P, ALLREDUCE_FN, EXCHANGE_FN, P]
+        L.tfg_faces_create.argtypes = [P, I, P, P, P, P, P, ctypes.POINTER(P)]
+        L.tfg_faces_destroy.argtypes = [P]
+        L.tfg_pressure_assemble.argtypes = [P, P, P, P, P, P, P, I]
         L.tfg_batch_create.argtypes = [I, P, P, P, P, P, ctypes.POINTER(P)]
         L.tfg_batch_destroy.argtypes = [P]
         L.tfg_batch_solve.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I, P,
@@ -353,3 +356,34 @@ class Batch:
         ms = np.zeros(self.nsys)
         _check(lib().tfg_batch_times(self.h, _ptr(ms)))
         return ms
+
+
+class Faces:
+    """Face structure of a grid for device-side pressure assembly
+    (tfg_faces_create).  c1, c2, pos, diag_bc, b_zero: the reference's face
+    arrays (1-based; c2 <= 0 on the boundary; pos 2 x n_faces)."""
+
+    def __init__(self, nat, c1, c2, pos, diag_bc, b_zero):
+        self.nat = nat
+        arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (c1, c2, pos, diag_bc, b_zero)]
+        self.n_faces = len(arrs[0])
+        self.h = ctypes.c_void_p()
+        _check(lib().tfg_faces_create(nat.h, self.n_faces, *[_ptr(a) for a in arrs],
+                                      ctypes.byref(self.h)))
+
+    def assemble(self, dfc, dfw, dvm, dvflux, dval, db, singular=False):
+        """A % val and b on the device (DeviceArrays); waits for the result."""
+        _check(lib().tfg_pressure_assemble(self.h, dfc.ptr, dfw.ptr, dvm.ptr, dvflux.ptr,
+                                           dval.ptr, db.ptr, 1 if singular else 0))
+        _check(lib().tfg_synchronize(self.nat.h))
+
+    def close(self):
+        if self.h:
+            lib().tfg_faces_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
