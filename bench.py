@@ -80,15 +80,38 @@ def cpu_baseline(s, gpu_iters, seconds_budget=15.0):
 TRAFFIC = os.path.join(ROOT, "profiles", "spmv_traffic_twophase200.json")
 
 
-def traffic_per_launch(kernel_prefix):
+def format_bytes(kind, nnz, n):
+    """Bytes one SpMV must move in the matrix format the engine streams (the
+    CSR formula counts int32 columns and row pointers; the coded engine
+    streams a 1-byte offset code per nonzero instead of the column, the
+    row-pattern engine a 1-byte pattern id per row instead of columns and row
+    pointers).  Both add 8 B per row of p read and 8 B of q written."""
+    if kind == 7:
+        return 8 * nnz + 17 * n
+    if kind == 6:
+        return 9 * nnz + 20 * n + 4
+    return 12 * nnz + 20 * n + 4
+
+
+# SpMV engine -> text of its template argument in the kernel's full name
+ENGINE_TAG = {0: "StreamEngine<2040, false, double, false>", 1: "StreamEngine<4088,",
+              2: "WaveEngine<4,", 3: "WaveEngine<8,", 4: "WaveEngine<16,",
+              5: "StreamEngine<2040, true, double, false>",
+              6: "StreamEngine<2040, true, double, true>", 7: "PatternEngine<2040,"}
+
+
+def traffic_per_launch(kernel_prefix, kind=None):
     """HBM bytes per launch of the kernel, from the committed counter summary
-    (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel calibration)."""
+    (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel calibration).
+    None when the summary was measured on another SpMV engine than `kind`."""
     if not os.path.exists(TRAFFIC):
         return None, None
     with open(TRAFFIC) as f:
         t = json.load(f)
     k = t["kernels"].get(kernel_prefix)
     if not k:
+        return None, None
+    if kind is not None and ENGINE_TAG[kind] not in k.get("name", ""):
         return None, None
     return k["fetch_kib"] * 1024 * t["fetch_scale"] + k["write_kib"] * 1024 * t["write_scale"], TRAFFIC
 
@@ -106,18 +129,20 @@ def _counter_means(path, counter):
     dispatches that did work (launches after convergence return at once)."""
     import csv
 
-    rows = {}
+    rows, names = {}, {}
     with open(path) as f:
         for r in csv.DictReader(f):
             if r["Counter_Name"] != counter:
                 continue
-            rows.setdefault(_kernel_key(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+            key = _kernel_key(r["Kernel_Name"])
+            rows.setdefault(key, []).append(float(r["Counter_Value"]))
+            names.setdefault(key, r["Kernel_Name"])
     out = {}
     for k, v in rows.items():
         v = np.asarray(v)
         busy = v[v >= 0.5 * v.max()] if v.max() > 0 else v
         out[k] = {"mean_kib": float(busy.mean()), "dispatches": int(busy.size),
-                  "idle_dispatches": int(v.size - busy.size)}
+                  "idle_dispatches": int(v.size - busy.size), "name": names[k]}
     return out
 
 
@@ -138,7 +163,8 @@ def summarize_counters(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, out_p
                         "fetch_kib": cf, "write_kib": cw},
         "kernels": {k: {"fetch_kib": fetch[k]["mean_kib"],
                         "write_kib": write.get(k, {}).get("mean_kib", 0.0),
-                        "dispatches": fetch[k]["dispatches"]} for k in sorted(fetch)},
+                        "dispatches": fetch[k]["dispatches"],
+                        "name": fetch[k]["name"][:160]} for k in sorted(fetch)},
     }
     with open(out_path, "w") as f:
         json.dump(doc, f, indent=1)
@@ -424,6 +450,81 @@ def bench_assembly(args):
     }), flush=True)
 
 
+def bench_icc(args):
+    """The reference's default preconditioner on the headline system: ICC
+    (DIC, level-scheduled, bit-exact sweeps) against Jacobi, CG to relres
+    1e-8 on the 200^3 two-phase system."""
+    import tflows_cases
+    import tflows_gpu as G
+
+    N = args.n
+    s = tflows_cases.hex_pressure(N, rho_out=1000.0)
+    nat = G.Native(s.row, s.col, s.dia)
+    dv, db = G.DeviceArray(s.val), G.DeviceArray(s.b)
+    dx = G.DeviceArray(np.zeros(s.n))
+    runs = {}
+    for prec in ("icc", "jacobi"):
+        opt = G.Options()
+        opt.criterion = G.CRIT_RELRES
+        opt.true_residual = 1
+        dx.upload(np.zeros(s.n))
+        t = time.perf_counter()
+        it, res, rep = nat.solve_native_device("cg", prec, dv, dx, db, 50000, 1e-8, options=opt,
+                                               report=True)
+        dt = time.perf_counter() - t
+        runs[prec] = {"seconds": dt, "iterations": it, "relres": res,
+                      "true_relres": rep["true_relres"], "ms_per_iteration": dt / max(it, 1) * 1e3}
+        log(f"bench icc: {prec}: {dt:.3f} s, {it} iterations, true relres {rep['true_relres']:.2e}")
+    print(json.dumps({
+        "metric": f"time to 1e-8 rel. residual with ICC vs Jacobi, {N}^3 two-phase pressure",
+        "value": runs["icc"]["seconds"], "unit": "s", "n_gpus": 1, "steps": 1,
+        "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"two-phase (1:1000) pressure-Poisson {N}^3, CG relres 1e-8",
+                   "n": s.n, "nnz": s.nnz}, "runs": runs}), flush=True)
+
+
+def bench_poisson40(args):
+    """The small case the reference runs on a CPU: 40^3 Poisson (64,000
+    cells, cell 1 pinned), Jacobi CG to 1e-8 with the reference's NRMS (norm
+    absent and norm = 1) and with relres; GPU against the oracle, both timed,
+    iteration counts side by side."""
+    import tflows_cases
+    import tflows_gpu as G
+    from oracle import oracle
+
+    s = tflows_cases.hex_pressure(40)
+    nat = G.Native(s.row, s.col, s.dia)
+    runs = {}
+    for name, norm, crit in (("nrms", None, 0), ("nrms_norm1", 1.0, 0), ("relres", None, 1)):
+        opt = G.Options()
+        opt.criterion = crit
+        best = None
+        for _ in range(max(3, args.steps)):
+            x = np.zeros(s.n)
+            t = time.perf_counter()
+            it, res = nat.solve_native("cg", "jacobi", s.val.copy(), x, s.b.copy(), 1000, 1e-8,
+                                       norm=norm, options=opt)
+            dt = time.perf_counter() - t
+            best = (dt, it, res, x) if best is None or dt < best[0] else best
+        A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
+        xo = np.zeros(s.n)
+        t = time.perf_counter()
+        it_o, res_o = oracle.solve("cg", "jacobi", A, xo, s.b.copy(), 1000, 1e-8, norm=norm,
+                                   criterion="nrms" if crit == 0 else "relres")
+        dt_o = time.perf_counter() - t
+        runs[name] = {"gpu_seconds": best[0], "gpu_iterations": best[1], "gpu_res": best[2],
+                      "oracle_seconds": dt_o, "oracle_iterations": it_o, "oracle_res": res_o,
+                      "x_rel_diff": float(np.linalg.norm(best[3] - xo) / np.linalg.norm(xo))}
+        log(f"bench poisson40 {name}: gpu {best[0] * 1e3:.1f} ms {best[1]} it; "
+            f"oracle {dt_o * 1e3:.1f} ms {it_o} it")
+    print(json.dumps({
+        "metric": "40^3 Poisson Jacobi-CG to 1e-8 (host arrays, drop-in call), GPU vs 1-thread oracle",
+        "value": runs["nrms"]["gpu_seconds"], "unit": "s", "n_gpus": 1, "steps": max(3, args.steps),
+        "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "hex Poisson 40^3, pinned cell 1, Jacobi, tol 1e-8", "n": s.n,
+                   "nnz": s.nnz}, "runs": runs}), flush=True)
+
+
 def calibrate():
     """Known-traffic copy (1 GiB read + 1 GiB write) for counter calibration."""
     import tflows_gpu as G
@@ -446,7 +547,8 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the known-traffic copy kernel (counter calibration)")
     ap.add_argument("--workload", default="twophase200",
-                    choices=["twophase200", "upwind120", "knn4m", "batch256", "assembly200"],
+                    choices=["twophase200", "upwind120", "knn4m", "batch256", "assembly200",
+                             "icc200", "poisson40"],
                     help="the headline solve (default) or one of the other measured workloads")
     ap.add_argument("--summarize-counters", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
                     help="write the traffic summary from four rocprofv3 counter_collection.csv files")
@@ -454,6 +556,8 @@ def main():
                     help="write a markdown table of a rocprofv3 kernel_stats.csv")
     ap.add_argument("--out", help="output path of --summarize-*")
     ap.add_argument("--command", default="", help="command line recorded in the summary")
+    ap.add_argument("--label", default="200^3 two-phase Jacobi-CG FP64 solve (bench.py --steps 1 --warmup 0)",
+                    help="workload named in the title of a --summarize-stats table")
     args = ap.parse_args()
     if args.calibrate:
         calibrate()
@@ -463,8 +567,7 @@ def main():
         return
     if args.summarize_stats:
         summarize_stats(args.summarize_stats, args.out,
-                        {"workload": "200^3 two-phase Jacobi-CG FP64 solve (bench.py --steps 1 --warmup 0)",
-                         "cmd": args.command})
+                        {"workload": args.label, "cmd": args.command})
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
@@ -472,7 +575,8 @@ def main():
         sys.exit(2)
     if args.workload != "twophase200":
         {"upwind120": bench_upwind, "knn4m": bench_knn_ir, "batch256": bench_batch,
-         "assembly200": bench_assembly}[args.workload](args)
+         "assembly200": bench_assembly, "icc200": bench_icc,
+         "poisson40": bench_poisson40}[args.workload](args)
         return
 
     import tflows_cases
@@ -518,12 +622,14 @@ def main():
     spmv_ms = float(np.mean([r[2]["spmv_ms"] for r in reps]))
     spmv_bytes = 12 * s.nnz + 20 * s.n + 4
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    kind = nat.info()["spmv_kind"]
+    fmt_bytes = format_bytes(kind, s.nnz, s.n)
     total = float(np.sum(times))
     ms_per_step = total / args.steps * 1e3
     iter_bytes = spmv_bytes + 88 * s.n
     solve_gbs = iter_bytes * its[-1] / (times[-1]) / 1e9
 
-    traffic, traffic_src = traffic_per_launch("k_spmv_pq")
+    traffic, traffic_src = traffic_per_launch("k_spmv_pq", kind)
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline leg (oracle, 1 thread)...")
@@ -562,6 +668,8 @@ def main():
             "kernel": "k_spmv_pq (FP64 CSR SpMV with fused p.q)",
             "bytes_per_launch": spmv_bytes,
             "launch_us": spmv_ms * 1e3,
+            "format_bytes_per_launch": fmt_bytes,
+            "format_GBps": fmt_bytes / (spmv_ms * 1e-3) / 1e9,
         },
         "cpu_baseline": cpu,
     }

@@ -36,10 +36,10 @@ extern "C" {
 #define TFG_CRIT_NRMS 0   /* Normalized_Root_Mean_Square < tol (reference)     */
 #define TFG_CRIT_RELRES 1 /* ||b - A x||_2 / ||b||_2 < tol                     */
 
-#define TFG_PATH_AUTO 0        /* pick by size                                 */
+#define TFG_PATH_AUTO 0        /* graph below 2M nonzeros, kernels above       */
 #define TFG_PATH_KERNELS 1     /* one launch per phase, device-side stop flag  */
-#define TFG_PATH_GRAPH 2       /* the same phases captured in a hipGraph       */
-#define TFG_PATH_PERSISTENT 3  /* one cooperative launch for the whole solve   */
+#define TFG_PATH_GRAPH 2       /* chunks of iterations replayed as a hipGraph  */
+                               /* (FP64 cg/bicg, jacobi/none; batches too)     */
 
 typedef struct {
   int precision;      /* TFG_FP64 | TFG_FP32_IR                                */
@@ -139,11 +139,15 @@ int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, d
  * 2040 stage with the next block's loads in flight, 6 the same reading a
  * 1-byte code per nonzero instead of its int32 column (col = row + table[code];
  * available when the matrix has at most 256 distinct col - row offsets, as
- * every structured grid has).  All give bit-identical products; the engine
+ * every structured grid has), 7 the same reading one byte per ROW: the id of
+ * its offset sequence in a table of at most 64 sequences of at most 12
+ * entries (a 7-point box grid has 27), so neither columns nor row pointers
+ * are streamed.  All give bit-identical products; the engine
  * sets the SpMV launch grid and so the tree of the dot products fused into
- * it.  Default: engine 6 when available, else 5, so that a solve is bitwise
+ * it.  Default: engine 7, else 6, else 5 (what the matrix offers), so that a
+ * solve is bitwise
  * reproducible from run to run.  The autotuner is opt-in: tfg_spmv_tune
- * times every engine on d_val (us[7] receives each mean launch time, -1
+ * times every engine on d_val (us[8] receives each mean launch time, -1
  * for an engine the matrix does not offer) and
  * keeps the fastest; TFG_SPMV_AUTOTUNE=1 in the environment makes the first
  * product or solve of a handle do the same; tfg_spmv_select forces an engine
@@ -152,11 +156,11 @@ int tfg_spmv_tune(tfg_matrix* h, const double* d_val, int* kind, float* us);
 int tfg_spmv_select(tfg_matrix* h, int kind);
 
 /* Facts about the handle: sizes, chosen SpMV engine (-1: not chosen yet),
- * its grid, the number of 2040-nonzero row blocks, and the number of
- * distinct col - row offsets when engine 6 is available (else 0).  Any
- * pointer may be NULL. */
+ * its grid, the number of 2040-nonzero row blocks, the number of distinct
+ * col - row offsets when engine 6 is available (else 0) and of distinct row
+ * patterns when engine 7 is (else 0).  Any pointer may be NULL. */
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind,
-                    int* spmv_grid, int* stage_blocks, int* offsets);
+                    int* spmv_grid, int* stage_blocks, int* offsets, int* patterns);
 
 /* ---- pressure-system assembly on the device ------------------------------
  * Builds A % val and b of the pressure-correction system from face data on

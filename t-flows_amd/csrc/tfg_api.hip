@@ -71,6 +71,17 @@ int check_csr(int nt, int nnz, const int* row, const int* col, const int* dia) {
 
 }  // namespace tfg
 
+// What a captured chunk of CG iterations depends on besides the handle.
+struct GraphKey {
+  const void* val;
+  const void* b;
+  int prec, dx, chunk, kind;
+  bool operator==(const GraphKey& o) const {
+    return val == o.val && b == o.b && prec == o.prec && dx == o.dx && chunk == o.chunk &&
+           kind == o.kind;
+  }
+};
+
 struct tfg_matrix {
   int device = 0;
   int ni = 0, nt = 0, nnz = 0;
@@ -120,6 +131,9 @@ struct tfg_matrix {
   double* icc_dinv = nullptr;
   double* icc_y = nullptr;
   double* icc_z = nullptr;
+  // graph path: executable chunk of CG iterations and the operands it holds
+  hipGraphExec_t cg_graph = nullptr;
+  GraphKey cg_key{};
   // multi-rank hooks
   tfg_allreduce_fn allreduce = nullptr;
   tfg_exchange_fn exchange = nullptr;
@@ -129,6 +143,11 @@ struct tfg_matrix {
   uint8_t* code = nullptr;
   int* tab = nullptr;
   int ntab = 0;        // 0: more than 256 distinct offsets, no coded engine
+  // row patterns (structured grids): each row's offset sequence by id
+  uint8_t* pat = nullptr;
+  int* poff = nullptr;
+  int* plen = nullptr;
+  int npat = 0;        // 0: no pattern engine on this matrix
 };
 
 namespace {
@@ -167,7 +186,8 @@ int rows_per_pass(int kind) {
 void size_grids(tfg_matrix* h) {
   for (int k = 0; k < tfg::kKinds; ++k) {
     const int per_cu = tfg::spmv_blocks_per_cu(k);
-    const int work = (k == tfg::K_STAGE2048 || k == tfg::K_PIPE2048 || k == tfg::K_PIPE2048_CODED)
+    const int work = (k == tfg::K_STAGE2048 || k == tfg::K_PIPE2048 || k == tfg::K_PIPE2048_CODED ||
+                      k == tfg::K_PIPE_PATTERN)
                          ? h->nblk[0]
                      : k == tfg::K_STAGE4096 ? h->nblk[1]
                                              : (h->nt + rows_per_pass(k) - 1) / rows_per_pass(k);
@@ -204,6 +224,9 @@ tfg::SpmvShape shape_of(const tfg_matrix* h, int kind) {
   m.code = h->code;
   m.tab = h->tab;
   m.ntab = h->ntab;
+  m.pat = h->pat;
+  m.poff = h->poff;
+  m.plen = h->plen;
   return m;
 }
 
@@ -219,7 +242,17 @@ tfg::CgArgs args_of(const tfg_matrix* h, int kind) {
   a.code = m.code;
   a.tab = m.tab;
   a.ntab = m.ntab;
+  a.pat = m.pat;
+  a.poff = m.poff;
+  a.plen = m.plen;
   return a;
+}
+
+// Engines this matrix offers (6 needs offset codes, 7 row patterns).
+bool kind_available(const tfg_matrix* h, int k) {
+  if (k == tfg::K_PIPE2048_CODED) return h->ntab > 0;
+  if (k == tfg::K_PIPE_PATTERN) return h->npat > 0;
+  return k >= 0 && k < tfg::kKinds;
 }
 
 int alloc_work(tfg_matrix* h) {
@@ -254,11 +287,13 @@ void free_all(tfg_matrix* h) {
   void* dev[] = {h->row, h->col, h->blk[0], h->blk[1], h->val, h->b, h->x, h->r, h->p, h->q, h->ad,
                  h->part, h->st, h->st_in, h->val32, h->e32, h->r32,
                  h->p32, h->q32, h->ad32, h->rt, h->nv[0], h->nv[1], h->nv[2],
-                 h->nv[3], h->valt, h->tpos, h->code, h->tab, h->loc, h->icc_rows[0],
+                 h->nv[3], h->valt, h->tpos, h->code, h->tab, h->pat, h->poff, h->plen,
+                 h->loc, h->icc_rows[0],
                  h->icc_rows[1], h->icc_d, h->icc_dinv, h->icc_y, h->icc_z};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   if (h->hst) (void)hipHostFree(h->hst);
+  if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);
   for (hipEvent_t e : {h->ev_poll[0], h->ev_poll[1], h->ev_t0, h->ev_t1})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->ev_sample)
@@ -280,7 +315,8 @@ int ensure_tuned(tfg_matrix* h, const double* val) {
   if (h->kind >= 0) return TFG_OK;
   const char* at = getenv("TFG_SPMV_AUTOTUNE");
   if (!(at && *at && *at != '0')) {
-    h->kind = h->ntab > 0 ? tfg::K_PIPE2048_CODED : tfg::K_PIPE2048;
+    h->kind = h->npat > 0 ? tfg::K_PIPE_PATTERN
+              : h->ntab > 0 ? tfg::K_PIPE2048_CODED : tfg::K_PIPE2048;
     return TFG_OK;
   }
   return tune_all(h, val);
@@ -291,7 +327,7 @@ int tune_all(tfg_matrix* h, const double* val) {
   float best = 1e30f;
   int best_k = tfg::K_STAGE2048;
   for (int k = 0; k < tfg::kKinds; ++k) {
-    if (k == tfg::K_PIPE2048_CODED && h->ntab == 0) {
+    if (!kind_available(h, k)) {
       h->tune_us[k] = -1.f;  // not available on this matrix
       continue;
     }
@@ -893,6 +929,47 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
       }
       if (e != hipSuccess) { rc = fail(TFG_EHIP, "level upload: %s", hipGetErrorString(e)); break; }
     }
+    // Row patterns when the rows have at most kPatCount distinct offset
+    // sequences of at most kPatMax entries (structured grids)
+    {
+      std::vector<std::vector<int>> seqs;
+      std::vector<uint8_t> pat((size_t)nt + 64, 0);
+      bool ok = nnz >= 64;
+      std::vector<int> cur;
+      int last = -1;
+      for (int i = 0; i < ni && ok; ++i) {
+        const int len = row[i + 1] - row[i];
+        if (len > tfg::kPatMax) { ok = false; break; }
+        cur.assign(len, 0);
+        for (int j = 0; j < len; ++j) cur[j] = col[row[i] - 1 + j] - 1 - i;
+        int id = -1;
+        if (last >= 0 && seqs[last] == cur) id = last;
+        for (size_t t = 0; t < seqs.size() && id < 0; ++t)
+          if (seqs[t] == cur) id = (int)t;
+        if (id < 0) {
+          if ((int)seqs.size() == tfg::kPatCount) { ok = false; break; }
+          id = (int)seqs.size();
+          seqs.push_back(cur);
+        }
+        pat[i] = (uint8_t)id;
+        last = id;
+      }
+      if (ok) {
+        std::vector<int> poff(tfg::kPatCount * tfg::kPatMax, 0), plen(tfg::kPatCount, 1);
+        for (size_t t = 0; t < seqs.size(); ++t) {
+          plen[t] = (int)seqs[t].size();
+          for (size_t j = 0; j < seqs[t].size(); ++j) poff[t * tfg::kPatMax + j] = seqs[t][j];
+        }
+        e = hipMalloc(&h->pat, pat.size());
+        if (e == hipSuccess) e = hipMalloc(&h->poff, sizeof(int) * poff.size());
+        if (e == hipSuccess) e = hipMalloc(&h->plen, sizeof(int) * plen.size());
+        if (e == hipSuccess) e = hipMemcpy(h->pat, pat.data(), pat.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->poff, poff.data(), sizeof(int) * poff.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->plen, plen.data(), sizeof(int) * plen.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { rc = fail(TFG_EHIP, "pattern upload: %s", hipGetErrorString(e)); break; }
+        h->npat = (int)seqs.size();
+      }
+    }
     // Offset codes when the matrix has at most 256 distinct col - row
     // offsets (every structured grid): 1 byte per nonzero instead of 4.
     {
@@ -949,9 +1026,10 @@ int tfg_matrix_destroy(tfg_matrix* h) {
 }
 
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind, int* spmv_grid,
-                    int* stage_blocks, int* offsets) {
+                    int* stage_blocks, int* offsets, int* patterns) {
   if (!h) return fail(TFG_EINVAL, "null handle");
   if (offsets) *offsets = h->ntab;
+  if (patterns) *patterns = h->npat;
   if (n) *n = h->nt;
   if (nnz) *nnz = h->nnz;
   if (spmv_kind) *spmv_kind = h->kind;
@@ -963,8 +1041,8 @@ int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind, int* spmv_g
 int tfg_spmv_select(tfg_matrix* h, int kind) {
   if (!h) return fail(TFG_EINVAL, "null handle");
   if (kind < -1 || kind >= tfg::kKinds) return fail(TFG_EINVAL, "unknown SpMV engine %d", kind);
-  if (kind == tfg::K_PIPE2048_CODED && h->ntab == 0)
-    return fail(TFG_ENOTSUP, "engine 6 needs at most 256 distinct column offsets");
+  if (kind >= 0 && !kind_available(h, kind))
+    return fail(TFG_ENOTSUP, "engine %d is not available on this matrix", kind);
   h->kind = kind;
   return TFG_OK;
 }
@@ -1020,6 +1098,8 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     return fail(TFG_EINVAL, "unknown criterion %d", o.criterion);
   if (o.precision != TFG_FP64 && o.precision != TFG_FP32_IR)
     return fail(TFG_EINVAL, "unknown precision %d", o.precision);
+  if (o.path < TFG_PATH_AUTO || o.path > TFG_PATH_GRAPH)
+    return fail(TFG_EINVAL, "unknown path %d", o.path);
   if (h->ni < h->nt && !h->allreduce)
     return fail(TFG_ENOTSUP, "buffer cells (ni=%d < nt=%d) need the tfg_matrix_set_comm hooks",
                 h->ni, h->nt);
@@ -1084,6 +1164,7 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     s0->criterion = o.criterion;
     s0->has_norm = norm ? 1 : 0;
     s0->norm = norm ? *norm : 0.0;
+    s0->kcur = 1;
     HIP_TRY(hipMemcpyAsync(h->st, s0, sizeof(State), hipMemcpyHostToDevice, h->stream));
 
     const bool timing = o.timing != 0;
@@ -1115,10 +1196,56 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     // x update deferred into the next p update when no x range is needed
     const bool dx = method == tfg::NS_NONE && !icc && (o.criterion == TFG_CRIT_RELRES || norm);
     const int chunk = h->nnz > (1 << 22) ? 8 : 32;
+    // Graph path: chunks of CG iterations replayed as one hipGraph; the
+    // kernels take the iteration number from State::kcur (k = 0) and k_next
+    // advances it.  The executable graph is kept per handle for the same
+    // operands.
+    const bool graph = method == tfg::NS_NONE && !icc && !timing &&
+                       (o.path == TFG_PATH_GRAPH || (o.path == TFG_PATH_AUTO && h->nnz < (1 << 21)));
+    auto enqueue_dev_iterations = [&](int count, bool finish) -> int {
+      for (int c = 0; c < count; ++c) {
+        HIP_TRY(tfg::launch_pupdate(L, a, pr, 0, dx));
+        HIP_TRY(tfg::launch_spmv_pq(L, a, 0));
+        HIP_TRY(tfg::launch_update(L, a, pr, 0, dx));
+        HIP_TRY(tfg::launch_next(L, a));
+      }
+      if (finish) HIP_TRY(tfg::launch_finish(L, a, 0, 0));
+      return TFG_OK;
+    };
+    if (graph) {
+      const GraphKey key{dval, db, pr, dx ? 1 : 0, chunk, h->kind};
+      if (!h->cg_graph || !(h->cg_key == key)) {
+        if (h->cg_graph) {
+          HIP_TRY(hipGraphExecDestroy(h->cg_graph));
+          h->cg_graph = nullptr;
+        }
+        hipGraph_t gr = nullptr;
+        HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        const int rq = enqueue_dev_iterations(chunk, true);
+        hipError_t ge = hipStreamEndCapture(h->stream, &gr);
+        if (rq != TFG_OK) {
+          if (gr) (void)hipGraphDestroy(gr);
+          return rq;
+        }
+        if (ge == hipSuccess) ge = hipGraphInstantiate(&h->cg_graph, gr, nullptr, nullptr, 0);
+        if (gr) (void)hipGraphDestroy(gr);
+        HIP_TRY(ge);
+        h->cg_key = key;
+      }
+    }
     int k = 1, slot = 0, prev = -1;
     bool stop = false;
     while (k <= miter && !stop) {
       const int kend = std::min(miter, k + chunk - 1);
+      if (graph) {
+        if (kend - k + 1 == chunk) {
+          HIP_TRY(hipGraphLaunch(h->cg_graph, h->stream));
+        } else {
+          const int rq = enqueue_dev_iterations(kend - k + 1, true);
+          if (rq != TFG_OK) return rq;
+        }
+        k = kend + 1;
+      }
       for (; k <= kend; ++k) {
         const bool smp = timing && (k % 4 == 1) && (int)sample_iter.size() < max_samples;
         hipEvent_t e0 = smp ? h->ev_sample[2 * sample_iter.size()] : nullptr;
@@ -1143,7 +1270,7 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
         }
         if (smp) sample_iter.push_back(k);
       }
-      HIP_TRY(tfg::launch_finish(L, a, kend, method == tfg::NS_BCGS ? 1 : 0));
+      if (!graph) HIP_TRY(tfg::launch_finish(L, a, kend, method == tfg::NS_BCGS ? 1 : 0));
       HIP_TRY(hipMemcpyAsync(&h->hst[1 + slot], h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
       HIP_TRY(hipEventRecord(h->ev_poll[slot], h->stream));
       if (prev >= 0) {

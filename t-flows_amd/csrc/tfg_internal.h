@@ -14,6 +14,10 @@ constexpr int kRowsPerBlock = 256; // SpMV rows per staged block (one per thread
 // 16-byte chunks per thread with the chunk of misalignment at each end.
 constexpr int kCap = 2040;
 constexpr int kCap2 = 4088;
+// Row-pattern engine: at most kPatCount distinct rows of at most kPatMax
+// entries.
+constexpr int kPatCount = 64;
+constexpr int kPatMax = 12;
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dealt round robin
 constexpr double kHuge = 1.0e+15;  // Const_Mod.f90:84-93 (HUGE, TINY)
 constexpr double kTiny = 1.0e-15;
@@ -48,6 +52,7 @@ struct State {
   int fold;         // deferred-x CG: x += alpha p of the last update still to do
   double beta;      // multi-rank CG: beta from the global reduction
   int comm;         // multi-rank CG: fn is given (global), not computed
+  int kcur;         // graph path: the iteration the next kernels run
 };
 
 // Partial sums written by one kernel and swept by the next ones.
@@ -74,6 +79,9 @@ struct CgArgs {
   const uint8_t* code; // offset codes (coded engine) or null
   const int* tab;
   int ntab;
+  const uint8_t* pat;  // row patterns (pattern engine) or null
+  const int* poff;
+  const int* plen;
   const double* val;   // unscaled values (caller's)
   const double* b;     // unscaled right-hand side (caller's)
   double* x;
@@ -95,6 +103,9 @@ struct SpmvShape {
   const uint8_t* __restrict__ code;  // offset code of each nonzero (coded engine)
   const int* __restrict__ tab;       // offset table: col = row + tab[code]
   int ntab;
+  const uint8_t* __restrict__ pat;   // row pattern of each row (pattern engine)
+  const int* __restrict__ poff;      // kPatCount x kPatMax column offsets
+  const int* __restrict__ plen;      // kPatCount pattern lengths
 };
 
 // Textbook methods for non-symmetric systems (tfg_nonsym.hip).
@@ -113,7 +124,7 @@ struct NsArgs {
 
 // SpMV row engines (see tfg_spmv.h).
 enum SpmvKind { K_STAGE2048 = 0, K_STAGE4096 = 1, K_WAVE4 = 2, K_WAVE8 = 3, K_WAVE16 = 4,
-                K_PIPE2048 = 5, K_PIPE2048_CODED = 6, kKinds = 7 };
+                K_PIPE2048 = 5, K_PIPE2048_CODED = 6, K_PIPE_PATTERN = 7, kKinds = 8 };
 
 // FP32 sweeps of iterative refinement (FP32 vectors, FP64 scalars and
 // reductions).  Reuses the FP64 solve's partial buffers (spmv, upd).
@@ -180,6 +191,7 @@ hipError_t launch_prologue(const Launch& L, const CgArgs& a, int prec);
 hipError_t launch_start(const Launch& L, const CgArgs& a, int rho_slot = 6);
 hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k, bool dx = false);
 hipError_t launch_fold_x(const Launch& L, const CgArgs& a);
+hipError_t launch_next(const Launch& L, const CgArgs& a);
 hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k);
 hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k, bool dx = false);
 hipError_t launch_finish(const Launch& L, const CgArgs& a, int k, int check_omega = 0);

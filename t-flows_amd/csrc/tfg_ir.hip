@@ -151,6 +151,9 @@ __global__ __launch_bounds__(kBlock) void k_ir_spmv_pq(CgArgs a, IrArgs f, int k
   m.code = a.code;
   m.tab = a.tab;
   m.ntab = a.ntab;
+  m.pat = a.pat;
+  m.poff = a.poff;
+  m.plen = a.plen;
   double v[1] = {0.0};
   E::template run<false>(
       sm, m, f.val32, 1.0f, f.p32, s_tmp, s_out, [](int) { return 0.0f; },

@@ -49,6 +49,9 @@ __device__ __forceinline__ SpmvShape shape_of(const CgArgs& a) {
   m.code = a.code;
   m.tab = a.tab;
   m.ntab = a.ntab;
+  m.pat = a.pat;
+  m.poff = a.poff;
+  m.plen = a.plen;
   return m;
 }
 
@@ -148,11 +151,13 @@ __global__ __launch_bounds__(kBlock) void k_start(CgArgs a, int rho_slot) {
 // the p this kernel reads anyway, instead of in k_update -- the same
 // operation on the same operands, one iteration later, 8n bytes fewer per
 // iteration.  The last iteration's term is folded by k_fold_x.
+// k = 0: the iteration number is State::kcur (graph-replayed chunks).
 template <int PREC, bool DX>
 __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
   __shared__ double s_tmp[5 * 4], s_out[5];
   State* st = a.st;
   if (st->done) return;
+  if (k == 0) k = st->kcur;
   double beta = 0.0;
   if (k > 1) {
     const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
@@ -247,6 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
   __shared__ double s_tmp[5 * 4], s_out[5];
   State* st = a.st;
   if (st->done) return;
+  if (k == 0) k = st->kcur;
   {
     const int ops1[1] = {OP_SUM};
     gather<1>(a.pt.spmv, a.pt.pstride, a.pt.g_spmv, ops1, s_tmp, s_out);
@@ -331,6 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(CgArgs a, int k, int check_om
   __shared__ double s_tmp[5 * 4], s_out[5];
   State* st = a.st;
   if (st->done) return;
+  if (k == 0) k = st->kcur - 1;  // after k_next: the iteration just done
   const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
   gather<5>(a.pt.upd, a.pt.pstride, a.pt.g_upd, ops, s_tmp, s_out);
   if (threadIdx.x != 0) return;
@@ -349,6 +356,11 @@ __global__ __launch_bounds__(kBlock) void k_finish(CgArgs a, int k, int check_om
     st->niter = k;
     st->done = 1;
   }
+}
+
+// Advances State::kcur: the last kernel of an iteration in a graph chunk.
+__global__ void k_next(State* st) {
+  if (threadIdx.x == 0) st->kcur = st->kcur + 1;
 }
 
 // The deferred x += alpha_K p_K of the last iteration that updated r.
@@ -414,6 +426,7 @@ int spmv_blocks_per_cu(int kind) {
     case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<8, double>>, kBlock, 0); break;
     case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, true, double>>, kBlock, 0); break;
     case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, true, double, true>>, kBlock, 0); break;
+    case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<PatternEngine<kCap, true, double>>, kBlock, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<16, double>>, kBlock, 0); break;
   }
   if (e != hipSuccess || nb < 1) return 1;
@@ -446,6 +459,11 @@ hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k, boo
     if (dx) k_pupdate<PREC_NONE, true><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
     else k_pupdate<PREC_NONE, false><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_next(const Launch& L, const CgArgs& a) {
+  k_next<<<1, 64, 0, L.stream>>>(a.st);
   return hipGetLastError();
 }
 

@@ -42,6 +42,9 @@ __device__ __forceinline__ SpmvShape shape1(const CgArgs& a) {
   m.code = a.code;
   m.tab = a.tab;
   m.ntab = a.ntab;
+  m.pat = a.pat;
+  m.poff = a.poff;
+  m.plen = a.plen;
   return m;
 }
 

@@ -259,6 +259,120 @@ struct StreamEngine {
   }
 };
 
+// Exclusive prefix sum of v over the workgroup (thread order); s_wave needs
+// kBlock/64 ints.  Contains barriers: every thread must call it.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) s_wave[wid] = inc;
+  __syncthreads();
+  int base = 0;
+  for (int w = 0; w < wid; ++w) base += s_wave[w];
+  __syncthreads();
+  return base + inc - v;
+}
+
+// Rows described by a pattern: the column offsets (col - row, in CSR order)
+// of a row are entry pat[row] of a per-matrix table of at most kPatCount
+// offset sequences of at most kPatMax entries (a 7-point box grid has 27:
+// one per boundary combination).  So neither a column stream nor the row
+// pointers are read, only the values and one byte per row; a row's first
+// nonzero is the block's first plus the prefix sum of the pattern lengths
+// before it (one scan per block).  Otherwise the pipelined stream engine.
+template <int CAP, bool PIPE, typename T>
+struct PatternEngine {
+  static constexpr int EPV = 16 / (int)sizeof(T);
+  static constexpr int NV = (CAP / EPV + 2 + kBlock - 1) / kBlock;
+  struct Smem {
+    T val[CAP + 2 * EPV];
+    int off[kPatCount * kPatMax];
+    int len[kPatCount];
+    int wave[kBlock / 64];
+  };
+  struct Regs {
+    ChunkStream<T, NV> v;
+    int r0, nr, s, e, pat;
+  };
+
+  __device__ __forceinline__ static void fetch(const SpmvShape& m, const T* __restrict__ val,
+                                               int nnz, int lb, Regs& g) {
+    typedef __attribute__((address_space(4))) const int cint;
+    const int r0 = ((cint*)m.blk)[lb], r1 = ((cint*)m.blk)[lb + 1];
+    const int s = ((cint*)m.row)[r0], e = ((cint*)m.row)[r1];
+    g.r0 = r0;
+    g.nr = r1 - r0;
+    g.s = s;
+    g.e = e;
+    g.pat = m.pat[r0 + min((int)threadIdx.x, r1 - r0 - 1)];
+    g.v.fetch(val, nnz, s, e);
+  }
+
+  template <bool SUB, typename Init, typename Epi>
+  __device__ __forceinline__ static void run(Smem& sm, const SpmvShape& m,
+                                             const T* __restrict__ val, T fn, const T* x,
+                                             double* s_tmp, double* s_out, Init init, Epi epi) {
+    (void)s_tmp;
+    (void)s_out;
+    const int tid = threadIdx.x;
+    const int nnz = m.row[m.n];
+    for (int t = tid; t < kPatCount * kPatMax; t += kBlock) sm.off[t] = m.poff[t];
+    for (int t = tid; t < kPatCount; t += kBlock) sm.len[t] = m.plen[t];
+    __syncthreads();
+    typedef __attribute__((address_space(1))) const T gT;
+    gT* xg = (gT*)x;
+    asm volatile("" : "+s"(xg));
+    const XcdWalk w(m.nblk);
+    Regs g;
+    int lb = w.first;
+    if (PIPE && lb < w.end) fetch(m, val, nnz, lb, g);
+    while (lb < w.end) {
+      const int nxt = lb + w.step;
+      if (!PIPE) fetch(m, val, nnz, lb, g);
+      const int r0 = g.r0, nr = g.nr, s = g.s, e = g.e, p = g.pat;
+      g.v.put(sm.val, nnz, s, e);
+      // row starts: prefix sum of the pattern lengths (threads past the
+      // block's rows take its last row, whose start is e - L)
+      const int L = sm.len[p];
+      const int pre = block_exclusive_scan(tid < nr ? L : 0, sm.wave);
+      const int rs = tid < nr ? s + pre : e - L, re = rs + L;
+      const int i = r0 + min(tid, nr - 1);
+      const int ov = s & ~(EPV - 1);
+      const int* po = sm.off + p * kPatMax;
+      T av[8], xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int uu = min(u, L - 1);
+        av[u] = sm.val[rs + uu - ov] * fn;
+        xv[u] = xg[i + po[uu]];
+      }
+      if (PIPE) {
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(m, val, nnz, min(nxt, w.end - 1), g);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      T acc = init(i);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (u < L) acc = SUB ? acc - av[u] * xv[u] : acc + av[u] * xv[u];
+      if (tid < nr) {
+        for (int u = 8; u < L; ++u) {
+          const T a = sm.val[rs + u - ov] * fn;
+          const T xj = x[i + po[u]];
+          acc = SUB ? acc - a * xj : acc + a * xj;
+        }
+        epi(i, acc, xv[0], av[0]);
+      }
+      __syncthreads();
+      lb = nxt;
+    }
+  }
+};
+
 // One given row block lb per call (the batched solver's one-block-per-
 // workgroup launches): StreamEngine staging without the pipelining.
 template <int CAP, typename T>
@@ -427,7 +541,8 @@ struct WaveEngine {
 };
 
 // kind: 0 stage/kCap, 1 stage/kCap2, 2 wave/4, 3 wave/8, 4 wave/16,
-// 5 pipelined stage/kCap, 6 the same with offset-coded columns.  The stage engines fall back to scalar staging
+// 5 pipelined stage/kCap, 6 the same with offset-coded columns, 7 the same
+// with row patterns.  The stage engines fall back to scalar staging
 // (ScalarStage) when val is not 16-byte aligned or nnz < 4.
 #define TFG_ENGINE_SWITCH_T(L, T, CALL)                                            \
   do {                                                                             \
@@ -446,6 +561,10 @@ struct WaveEngine {
         break;                                                                     \
       case 6:                                                                      \
         if ((L).vec_val) { using E = StreamEngine<kCap, true, T, true>; CALL; }    \
+        else { using E = ScalarStage<kCap, T>; CALL; }                             \
+        break;                                                                     \
+      case 7:                                                                      \
+        if ((L).vec_val) { using E = PatternEngine<kCap, true, T>; CALL; }         \
         else { using E = ScalarStage<kCap, T>; CALL; }                             \
         break;                                                                     \
       case 2: { using E = WaveEngine<4, T>; CALL; } break;                         \

@@ -52,6 +52,55 @@
       import :: c_ptr
     end function
 
+    ! Multi-rank hooks: Comm_Mod's Global % Sum/Min/Max_Real and
+    ! Grid % Exchange_Cells_Real stand-ins, passed with c_funloc of bind(C)
+    ! functions of the interfaces below (Tflows_Gpu_Allreduce_If,
+    ! Tflows_Gpu_Exchange_If).
+    integer(c_int) function tfg_matrix_set_comm(handle, allreduce, exchange,  &
+                                                ctx)                        &
+                                          bind(C, name='tfg_matrix_set_comm')
+      import :: c_int, c_ptr, c_funptr
+      type(c_ptr),    value :: handle
+      type(c_funptr), value :: allreduce, exchange
+      type(c_ptr),    value :: ctx
+    end function
+
+    ! Pressure-system assembly on the device from face data
+    integer(c_int) function tfg_faces_create(handle, n_faces, c1, c2, pos,    &
+                                             diag_bc, b_zero, faces)        &
+                                             bind(C, name='tfg_faces_create')
+      import :: c_int, c_ptr
+      type(c_ptr),    value       :: handle
+      integer(c_int), value       :: n_faces
+      integer(c_int), intent(in)  :: c1(*), c2(*), pos(*), diag_bc(*), b_zero(*)
+      type(c_ptr),    intent(out) :: faces
+    end function
+
+    integer(c_int) function tfg_pressure_assemble(faces, fc, fw, vm, vflux,   &
+                                                  val, b, singular)         &
+                                         bind(C, name='tfg_pressure_assemble')
+      import :: c_int, c_ptr
+      type(c_ptr),    value :: faces, fc, fw, vm, vflux, val, b
+      integer(c_int), value :: singular
+    end function
+
+  end interface
+
+  ! Signatures of the two hooks (tflows_gpu.h: tfg_allreduce_fn,
+  ! tfg_exchange_fn); values, ops: count entries; d_x: device address.
+  abstract interface
+    integer(c_int) function Tflows_Gpu_Allreduce_If(ctx, values, ops,       &
+                                                    count) bind(C)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr),    value         :: ctx
+      real(c_double), intent(inout) :: values(*)
+      integer(c_int), intent(in)    :: ops(*)
+      integer(c_int), value         :: count
+    end function
+    integer(c_int) function Tflows_Gpu_Exchange_If(ctx, d_x) bind(C)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_x
+    end function
   end interface
 
   contains

@@ -23,7 +23,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "tflows_gpu.h")
 TFG_OK, TFG_EINVAL, TFG_EHIP, TFG_ENOMEM, TFG_EBREAKDOWN, TFG_ENOTSUP = 0, 1, 2, 3, 4, 5
 FP64, FP32_IR = 0, 1
 CRIT_NRMS, CRIT_RELRES = 0, 1
-PATH_AUTO, PATH_KERNELS, PATH_GRAPH, PATH_PERSISTENT = 0, 1, 2, 3
+PATH_AUTO, PATH_KERNELS, PATH_GRAPH = 0, 1, 2
 
 
 class TFlowsGpuError(RuntimeError):
@@ -70,7 +70,7 @@ def lib():
         P, I, D, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
         L.tfg_matrix_create.argtypes = [I, I, I, P, P, P, ctypes.POINTER(P)]
         L.tfg_matrix_destroy.argtypes = [P]
-        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P, P]
+        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P, P, P]
         L.tfg_solve_native.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I,
                                        P, D, P, P]
         L.tfg_solve_native_ex.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P,
@@ -183,14 +183,16 @@ class Native:
         self.h = h
 
     def info(self):
-        vals = [ctypes.c_int() for _ in range(6)]
+        vals = [ctypes.c_int() for _ in range(7)]
         _check(lib().tfg_matrix_info(self.h, *[ctypes.byref(v) for v in vals]))
-        return dict(zip(["n", "nnz", "spmv_kind", "spmv_grid", "stage_blocks", "offsets"],
-                        [v.value for v in vals]))
+        return dict(zip(["n", "nnz", "spmv_kind", "spmv_grid", "stage_blocks", "offsets",
+                         "patterns"], [v.value for v in vals]))
 
     def kinds(self):
-        """The SpMV engines this matrix offers (engine 6 needs offset codes)."""
-        return list(range(len(self.SPMV_KINDS) if self.info()["offsets"] else 6))
+        """The SpMV engines this matrix offers (6 needs offset codes, 7 row
+        patterns)."""
+        i = self.info()
+        return list(range(6)) + ([6] if i["offsets"] else []) + ([7] if i["patterns"] else [])
 
     def close(self):
         if getattr(self, "h", None):
@@ -276,7 +278,7 @@ class Native:
         _check(lib().tfg_synchronize(self.h))
 
     SPMV_KINDS = ("stage2048", "stage4096", "wave4", "wave8", "wave16", "pipe2048",
-                  "pipe2048c")
+                  "pipe2048c", "pipe_pattern")
 
     def spmv_tune(self, dval):
         """Time every SpMV engine on dval; returns (chosen kind, {name: us})."""

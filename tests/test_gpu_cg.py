@@ -218,3 +218,23 @@ def test_twophase_200_relres(cases, gpu):
     assert res < 1e-8
     assert true < 2e-8
     assert abs(rep["true_relres"] - true) <= 1e-6 * true
+
+
+@pytest.mark.parametrize("norm,crit,miter", [(None, 0, 500), (1.0, 0, 500), (None, 1, 500),
+                                             (None, 0, 45)])
+def test_graph_path_equals_kernel_path(cases, gpu, norm, crit, miter):
+    """Chunks of iterations replayed as a hipGraph give the same bits as
+    launching every kernel (also with the cap falling inside a chunk)."""
+    s = cases.hex_pressure(18, rho_out=50.0)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    out = []
+    for path in (gpu.PATH_KERNELS, gpu.PATH_GRAPH, gpu.PATH_GRAPH, gpu.PATH_KERNELS):
+        o = gpu.Options()
+        o.criterion = crit
+        o.path = path
+        x = np.zeros(s.n)
+        r = nat.solve_native("cg", "jacobi", s.val.copy(), x, s.b.copy(), miter, 1e-9, norm=norm,
+                             options=o)
+        out.append((r, x))
+    for r, x in out[1:]:
+        assert r == out[0][0] and np.array_equal(x, out[0][1])
