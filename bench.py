@@ -85,7 +85,11 @@ def format_bytes(kind, nnz, n):
     CSR formula counts int32 columns and row pointers; the coded engine
     streams a 1-byte offset code per nonzero instead of the column, the
     row-pattern engine a 1-byte pattern id per row instead of columns and row
-    pointers).  Both add 8 B per row of p read and 8 B of q written."""
+    pointers, the symmetric engine that id plus the diagonal and the upper
+    entries only: 8 B per row and per upper entry, 4 nnz + 4 n).  All add
+    8 B per row of p read and 8 B of q written."""
+    if kind == 8:
+        return 4 * nnz + 21 * n
     if kind == 7:
         return 8 * nnz + 17 * n
     if kind == 6:
@@ -97,7 +101,8 @@ def format_bytes(kind, nnz, n):
 ENGINE_TAG = {0: "StreamEngine<2040, false, double, false>", 1: "StreamEngine<4088,",
               2: "WaveEngine<4,", 3: "WaveEngine<8,", 4: "WaveEngine<16,",
               5: "StreamEngine<2040, true, double, false>",
-              6: "StreamEngine<2040, true, double, true>", 7: "PatternEngine<2040,"}
+              6: "StreamEngine<2040, true, double, true>", 7: "PatternEngine<2040,",
+              8: "k_spmv_pq_sym"}
 
 
 def traffic_per_launch(kernel_prefix, kind=None):
@@ -629,7 +634,8 @@ def main():
     iter_bytes = spmv_bytes + 88 * s.n
     solve_gbs = iter_bytes * its[-1] / (times[-1]) / 1e9
 
-    traffic, traffic_src = traffic_per_launch("k_spmv_pq", kind)
+    spmv_kernel = "k_spmv_pq_sym" if kind == 8 and nat.info()["sym_state"] == 1 else "k_spmv_pq"
+    traffic, traffic_src = traffic_per_launch(spmv_kernel, None if spmv_kernel == "k_spmv_pq_sym" else kind)
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline leg (oracle, 1 thread)...")
@@ -665,7 +671,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": os.path.relpath(traffic_src, ROOT) if traffic_src else None,
-            "kernel": "k_spmv_pq (FP64 CSR SpMV with fused p.q)",
+            "kernel": spmv_kernel + " (FP64 SpMV with fused p.q; engine "
+                      + G.Native.SPMV_KINDS[kind] + ")",
             "bytes_per_launch": spmv_bytes,
             "launch_us": spmv_ms * 1e3,
             "format_bytes_per_launch": fmt_bytes,

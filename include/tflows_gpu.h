@@ -142,13 +142,20 @@ int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, d
  * every structured grid has), 7 the same reading one byte per ROW: the id of
  * its offset sequence in a table of at most 64 sequences of at most 12
  * entries (a 7-point box grid has 27), so neither columns nor row pointers
- * are streamed.  All give bit-identical products; the engine
+ * are streamed.  8: symmetric matrices on a row pattern with at most 8
+ * distinct positive offsets and no buffer cells: the values are held once,
+ * as the diagonal and the upper diagonals (8 B per row and per upper entry
+ * instead of 8 B per nonzero), refilled from val at the start of every solve
+ * or product, which also checks that val is symmetric bit for bit; when it
+ * is not, that call uses engine 7.  Engine 8 serves the CG / BiCG product
+ * and tfg_spmv; every other kernel runs engine 7.  All give bit-identical
+ * products; the engine
  * sets the SpMV launch grid and so the tree of the dot products fused into
- * it.  Default: engine 7, else 6, else 5 (what the matrix offers), so that a
- * solve is bitwise
+ * it.  Default: engine 8 at 2M nonzeros and more, else 7, else 6, else 5
+ * (what the matrix offers), so that a solve is bitwise
  * reproducible from run to run.  The autotuner is opt-in: tfg_spmv_tune
- * times every engine on d_val (us[8] receives each mean launch time, -1
- * for an engine the matrix does not offer) and
+ * times every engine on d_val (us[9] receives each mean launch time, -1
+ * for an engine the matrix or d_val does not offer) and
  * keeps the fastest; TFG_SPMV_AUTOTUNE=1 in the environment makes the first
  * product or solve of a handle do the same; tfg_spmv_select forces an engine
  * (-1: default choice again on next use). */
@@ -157,10 +164,14 @@ int tfg_spmv_select(tfg_matrix* h, int kind);
 
 /* Facts about the handle: sizes, chosen SpMV engine (-1: not chosen yet),
  * its grid, the number of 2040-nonzero row blocks, the number of distinct
- * col - row offsets when engine 6 is available (else 0) and of distinct row
- * patterns when engine 7 is (else 0).  Any pointer may be NULL. */
+ * col - row offsets when engine 6 is available (else 0), of distinct row
+ * patterns when engine 7 is (else 0), of upper diagonals when engine 8
+ * is (else 0), and whether the last val engine 8 was set up for was
+ * symmetric (1), not (0: that call used engine 7) or none was (-1).  Any
+ * pointer may be NULL. */
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind,
-                    int* spmv_grid, int* stage_blocks, int* offsets, int* patterns);
+                    int* spmv_grid, int* stage_blocks, int* offsets, int* patterns,
+                    int* sym_diagonals, int* sym_state);
 
 /* ---- pressure-system assembly on the device ------------------------------
  * Builds A % val and b of the pressure-correction system from face data on

@@ -75,10 +75,10 @@ int check_csr(int nt, int nnz, const int* row, const int* col, const int* dia) {
 struct GraphKey {
   const void* val;
   const void* b;
-  int prec, dx, chunk, kind;
+  int prec, dx, chunk, kind, sym;
   bool operator==(const GraphKey& o) const {
     return val == o.val && b == o.b && prec == o.prec && dx == o.dx && chunk == o.chunk &&
-           kind == o.kind;
+           kind == o.kind && sym == o.sym;
   }
 };
 
@@ -148,6 +148,14 @@ struct tfg_matrix {
   int* poff = nullptr;
   int* plen = nullptr;
   int npat = 0;        // 0: no pattern engine on this matrix
+  // symmetric engine: slot of each pattern entry, diagonal + upper diagonals
+  // (allocated on first use), asymmetry flag and its pinned host copy
+  int* pslot = nullptr;
+  int nsym = 0;        // upper diagonals; 0: no symmetric engine
+  double* sym = nullptr;
+  int* sym_flag = nullptr;
+  int* hflag = nullptr;
+  int sym_state = -1;  // last val: 1 symmetric, 0 not, -1 none checked yet
 };
 
 namespace {
@@ -187,7 +195,7 @@ void size_grids(tfg_matrix* h) {
   for (int k = 0; k < tfg::kKinds; ++k) {
     const int per_cu = tfg::spmv_blocks_per_cu(k);
     const int work = (k == tfg::K_STAGE2048 || k == tfg::K_PIPE2048 || k == tfg::K_PIPE2048_CODED ||
-                      k == tfg::K_PIPE_PATTERN)
+                      k == tfg::K_PIPE_PATTERN || k == tfg::K_SYM_DIA)
                          ? h->nblk[0]
                      : k == tfg::K_STAGE4096 ? h->nblk[1]
                                              : (h->nt + rows_per_pass(k) - 1) / rows_per_pass(k);
@@ -210,6 +218,7 @@ tfg::Launch launch_of(const tfg_matrix* h, const void* val, int kind) {
   L.grid_spmv = h->grid[kind];
   L.grid_vec = grid_vec_of(h);
   L.vec_val = ((uintptr_t)val % 16) == 0 && h->nnz >= 4;
+  L.sym = false;
   return L;
 }
 
@@ -227,6 +236,8 @@ tfg::SpmvShape shape_of(const tfg_matrix* h, int kind) {
   m.pat = h->pat;
   m.poff = h->poff;
   m.plen = h->plen;
+  m.pslot = h->pslot;
+  m.sd = h->sym;
   return m;
 }
 
@@ -245,6 +256,8 @@ tfg::CgArgs args_of(const tfg_matrix* h, int kind) {
   a.pat = m.pat;
   a.poff = m.poff;
   a.plen = m.plen;
+  a.pslot = m.pslot;
+  a.sd = m.sd;
   return a;
 }
 
@@ -252,6 +265,7 @@ tfg::CgArgs args_of(const tfg_matrix* h, int kind) {
 bool kind_available(const tfg_matrix* h, int k) {
   if (k == tfg::K_PIPE2048_CODED) return h->ntab > 0;
   if (k == tfg::K_PIPE_PATTERN) return h->npat > 0;
+  if (k == tfg::K_SYM_DIA) return h->nsym > 0;
   return k >= 0 && k < tfg::kKinds;
 }
 
@@ -288,11 +302,13 @@ void free_all(tfg_matrix* h) {
                  h->part, h->st, h->st_in, h->val32, h->e32, h->r32,
                  h->p32, h->q32, h->ad32, h->rt, h->nv[0], h->nv[1], h->nv[2],
                  h->nv[3], h->valt, h->tpos, h->code, h->tab, h->pat, h->poff, h->plen,
+                 h->pslot, h->sym, h->sym_flag,
                  h->loc, h->icc_rows[0],
                  h->icc_rows[1], h->icc_d, h->icc_dinv, h->icc_y, h->icc_z};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   if (h->hst) (void)hipHostFree(h->hst);
+  if (h->hflag) (void)hipHostFree(h->hflag);
   if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);
   for (hipEvent_t e : {h->ev_poll[0], h->ev_poll[1], h->ev_t0, h->ev_t1})
     if (e) (void)hipEventDestroy(e);
@@ -315,11 +331,34 @@ int ensure_tuned(tfg_matrix* h, const double* val) {
   if (h->kind >= 0) return TFG_OK;
   const char* at = getenv("TFG_SPMV_AUTOTUNE");
   if (!(at && *at && *at != '0')) {
-    h->kind = h->npat > 0 ? tfg::K_PIPE_PATTERN
-              : h->ntab > 0 ? tfg::K_PIPE2048_CODED : tfg::K_PIPE2048;
+    h->kind = h->nsym > 0 && h->nnz >= (1 << 21) ? tfg::K_SYM_DIA
+              : h->npat > 0                      ? tfg::K_PIPE_PATTERN
+              : h->ntab > 0                      ? tfg::K_PIPE2048_CODED
+                                                 : tfg::K_PIPE2048;
     return TFG_OK;
   }
   return tune_all(h, val);
+}
+
+// Symmetric engine for one val: fills the diagonal and upper diagonals and
+// reports whether val is symmetric bit for bit (one host wait on a flag).
+// *ok = false: the products must use the row-pattern engine.
+int sym_prepare(tfg_matrix* h, const double* val, bool* ok) {
+  *ok = false;
+  if (h->nsym <= 0) return TFG_OK;
+  if (!h->sym) {
+    HIP_TRY(hipMalloc(&h->sym, sizeof(double) * (size_t)(h->nsym + 1) * (size_t)h->ni));
+    HIP_TRY(hipMalloc(&h->sym_flag, sizeof(int)));
+    HIP_TRY(hipHostMalloc(&h->hflag, sizeof(int), hipHostMallocDefault));
+  }
+  HIP_TRY(hipMemsetAsync(h->sym_flag, 0, sizeof(int), h->stream));
+  HIP_TRY(tfg::launch_sym_pack(h->stream, grid_vec_of(h), shape_of(h, tfg::K_SYM_DIA), val, h->sym,
+                               h->sym_flag));
+  HIP_TRY(hipMemcpyAsync(h->hflag, h->sym_flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  *ok = *h->hflag == 0;
+  h->sym_state = *ok ? 1 : 0;
+  return TFG_OK;
 }
 
 int tune_all(tfg_matrix* h, const double* val) {
@@ -332,6 +371,16 @@ int tune_all(tfg_matrix* h, const double* val) {
       continue;
     }
     tfg::Launch L = launch_of(h, val, k);
+    if (k == tfg::K_SYM_DIA) {
+      bool ok = false;
+      const int rs = sym_prepare(h, val, &ok);
+      if (rs != TFG_OK) return rs;
+      if (!ok) {
+        h->tune_us[k] = -1.f;  // val is not symmetric
+        continue;
+      }
+      L.sym = true;
+    }
     const tfg::SpmvShape m = shape_of(h, k);
     for (int w = 0; w < 2; ++w) HIP_TRY(tfg::launch_spmv_plain(L, m, val, h->p, h->q));
     HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
@@ -968,6 +1017,64 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
         if (e == hipSuccess) e = hipMemcpy(h->plen, plen.data(), sizeof(int) * plen.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) { rc = fail(TFG_EHIP, "pattern upload: %s", hipGetErrorString(e)); break; }
         h->npat = (int)seqs.size();
+        // Symmetric engine: no buffer cells, at most kSymMax distinct
+        // positive offsets, no repeated offset in a row, and every entry
+        // (i, i + d) has its mirror (i + d, i).
+        std::vector<int> up;
+        bool sym_ok = ni == nt;
+        for (const auto& q : seqs) {
+          for (size_t u = 0; u < q.size() && sym_ok; ++u) {
+            if (std::count(q.begin(), q.end(), q[u]) != 1) sym_ok = false;
+            if (q[u] > 0 && std::find(up.begin(), up.end(), q[u]) == up.end()) up.push_back(q[u]);
+          }
+        }
+        if (up.empty() || (int)up.size() > tfg::kSymMax) sym_ok = false;
+        if (sym_ok) {
+          // has[t][k]: pattern t holds offset -up[k]
+          std::vector<char> has(seqs.size() * up.size(), 0);
+          for (size_t t = 0; t < seqs.size(); ++t)
+            for (size_t k = 0; k < up.size(); ++k)
+              has[t * up.size() + k] =
+                  std::find(seqs[t].begin(), seqs[t].end(), -up[k]) != seqs[t].end();
+          std::vector<int> upos(seqs.size() * tfg::kPatMax, -1);  // slot of a positive entry
+          for (size_t t = 0; t < seqs.size(); ++t)
+            for (size_t u = 0; u < seqs[t].size(); ++u)
+              if (seqs[t][u] > 0)
+                upos[t * tfg::kPatMax + u] =
+                    (int)(std::find(up.begin(), up.end(), seqs[t][u]) - up.begin());
+          for (int i = 0; i < ni && sym_ok; ++i) {
+            const int t = pat[i];
+            for (size_t u = 0; u < seqs[t].size(); ++u) {
+              const int k = upos[t * tfg::kPatMax + u];
+              if (k < 0) continue;
+              const long long j = (long long)i + seqs[t][u];
+              if (j >= ni || !has[(size_t)pat[j] * up.size() + k]) { sym_ok = false; break; }
+            }
+          }
+          // lower entries: every (i, i - d) is the mirror of an upper entry
+          // of row i - d (same count of entries on both sides)
+          long long nup = 0, nlow = 0;
+          for (int i = 0; i < ni && sym_ok; ++i)
+            for (int d : seqs[pat[i]]) {
+              if (d > 0) ++nup;
+              else if (d < 0) ++nlow;
+            }
+          if (nup != nlow) sym_ok = false;
+        }
+        if (sym_ok) {
+          std::vector<int> slot(tfg::kPatCount * tfg::kPatMax, 0);
+          for (size_t t = 0; t < seqs.size(); ++t)
+            for (size_t u = 0; u < seqs[t].size(); ++u) {
+              const int d = seqs[t][u];
+              const int k = (int)(std::find(up.begin(), up.end(), d > 0 ? d : -d) - up.begin());
+              slot[t * tfg::kPatMax + u] = d == 0 ? 0 : d > 0 ? k + 1 : -(k + 1);
+            }
+          e = hipMalloc(&h->pslot, sizeof(int) * slot.size());
+          if (e == hipSuccess)
+            e = hipMemcpy(h->pslot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice);
+          if (e != hipSuccess) { rc = fail(TFG_EHIP, "slot upload: %s", hipGetErrorString(e)); break; }
+          h->nsym = (int)up.size();
+        }
       }
     }
     // Offset codes when the matrix has at most 256 distinct col - row
@@ -1008,7 +1115,7 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
     const char* force = getenv("TFG_SPMV_KIND");
     if (rc == TFG_OK && force && *force) {
       const int k = atoi(force);
-      if (k >= 0 && k < tfg::kKinds) h->kind = k;
+      if (kind_available(h, k)) h->kind = k;  // else the default choice
     }
   } while (0);
   if (rc != TFG_OK) {
@@ -1026,8 +1133,11 @@ int tfg_matrix_destroy(tfg_matrix* h) {
 }
 
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind, int* spmv_grid,
-                    int* stage_blocks, int* offsets, int* patterns) {
+                    int* stage_blocks, int* offsets, int* patterns, int* sym_diagonals,
+                    int* sym_state) {
   if (!h) return fail(TFG_EINVAL, "null handle");
+  if (sym_diagonals) *sym_diagonals = h->nsym;
+  if (sym_state) *sym_state = h->sym_state;
   if (offsets) *offsets = h->ntab;
   if (patterns) *patterns = h->npat;
   if (n) *n = h->nt;
@@ -1176,6 +1286,12 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     std::vector<int> sample_iter;
 
     HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
+    // CG products by the symmetric engine when val is symmetric bit for bit
+    if (h->kind == tfg::K_SYM_DIA && method == tfg::NS_NONE) {
+      const int rs = sym_prepare(h, dval, &L.sym);
+      if (rs != TFG_OK) return rs;
+      a.sd = h->sym;
+    }
     HIP_TRY(tfg::launch_diag_sum(L, a));
     const bool icc = pr == tfg::PREC_ICC;
     tfg::IccArgs ic{};
@@ -1213,7 +1329,7 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
       return TFG_OK;
     };
     if (graph) {
-      const GraphKey key{dval, db, pr, dx ? 1 : 0, chunk, h->kind};
+      const GraphKey key{dval, db, pr, dx ? 1 : 0, chunk, h->kind, L.sym ? 1 : 0};
       if (!h->cg_graph || !(h->cg_key == key)) {
         if (h->cg_graph) {
           HIP_TRY(hipGraphExecDestroy(h->cg_graph));
@@ -1350,6 +1466,10 @@ int tfg_spmv(tfg_matrix* h, const double* d_val, const double* d_p, double* d_q)
   HIP_TRY(hipSetDevice(h->device));
   { const int rt = ensure_tuned(h, d_val); if (rt != TFG_OK) return rt; }
   tfg::Launch L = launch_of(h, d_val, h->kind);
+  if (h->kind == tfg::K_SYM_DIA) {
+    const int rs = sym_prepare(h, d_val, &L.sym);
+    if (rs != TFG_OK) return rs;
+  }
   HIP_TRY(tfg::launch_spmv_plain(L, shape_of(h, h->kind), d_val, d_p, d_q));
   return TFG_OK;
 }
@@ -1370,6 +1490,10 @@ int tfg_spmv_bench(tfg_matrix* h, const double* d_val, const double* d_p, double
   HIP_TRY(hipSetDevice(h->device));
   { const int rt = ensure_tuned(h, d_val); if (rt != TFG_OK) return rt; }
   tfg::Launch L = launch_of(h, d_val, h->kind);
+  if (h->kind == tfg::K_SYM_DIA) {
+    const int rs = sym_prepare(h, d_val, &L.sym);
+    if (rs != TFG_OK) return rs;
+  }
   const tfg::SpmvShape m = shape_of(h, h->kind);
   HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
   for (int i = 0; i < reps; ++i) HIP_TRY(tfg::launch_spmv_plain(L, m, d_val, d_p, d_q));

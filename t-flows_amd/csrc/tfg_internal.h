@@ -82,6 +82,8 @@ struct CgArgs {
   const uint8_t* pat;  // row patterns (pattern engine) or null
   const int* poff;
   const int* plen;
+  const int* pslot;    // symmetric engine: value slot of each pattern entry
+  const double* sd;    // symmetric engine: diagonal and upper diagonals
   const double* val;   // unscaled values (caller's)
   const double* b;     // unscaled right-hand side (caller's)
   double* x;
@@ -106,6 +108,12 @@ struct SpmvShape {
   const uint8_t* __restrict__ pat;   // row pattern of each row (pattern engine)
   const int* __restrict__ poff;      // kPatCount x kPatMax column offsets
   const int* __restrict__ plen;      // kPatCount pattern lengths
+  // symmetric engine: sd holds the diagonal (sd[i]) and the upper diagonals
+  // (sd[(k + 1) n + i] = a(i, i + d_k), d_k > 0); pslot gives per pattern
+  // entry where its value lives: 0 diagonal, k + 1 the row's own upper slot
+  // k, -(k + 1) slot k of row i + offset (a lower entry read from its mirror)
+  const int* __restrict__ pslot;
+  const double* __restrict__ sd;
 };
 
 // Textbook methods for non-symmetric systems (tfg_nonsym.hip).
@@ -124,7 +132,11 @@ struct NsArgs {
 
 // SpMV row engines (see tfg_spmv.h).
 enum SpmvKind { K_STAGE2048 = 0, K_STAGE4096 = 1, K_WAVE4 = 2, K_WAVE8 = 3, K_WAVE16 = 4,
-                K_PIPE2048 = 5, K_PIPE2048_CODED = 6, K_PIPE_PATTERN = 7, kKinds = 8 };
+                K_PIPE2048 = 5, K_PIPE2048_CODED = 6, K_PIPE_PATTERN = 7, K_SYM_DIA = 8,
+                kKinds = 9 };
+// Distinct positive column offsets the symmetric engine stores (upper
+// diagonals); a 7-point hex grid has 3.
+constexpr int kSymMax = 8;
 
 // FP32 sweeps of iterative refinement (FP32 vectors, FP64 scalars and
 // reductions).  Reuses the FP64 solve's partial buffers (spmv, upd).
@@ -181,6 +193,7 @@ struct Launch {
   int grid_spmv;   // workgroups of the SpMV kernels (resident, multiple of kXcds)
   int kind;        // SpmvKind
   bool vec_val;    // val is 16-byte aligned (wide nontemporal loads)
+  bool sym;        // products by the symmetric engine (val checked symmetric)
 };
 
 // Resident SpMV workgroups per CU for an engine (occupancy query).
@@ -193,6 +206,10 @@ hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k, boo
 hipError_t launch_fold_x(const Launch& L, const CgArgs& a);
 hipError_t launch_next(const Launch& L, const CgArgs& a);
 hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k);
+// Symmetric engine set-up for one val: sd from val, and *flag = 1 when some
+// lower entry is not bitwise equal to its mirror (then sd must not be used).
+hipError_t launch_sym_pack(hipStream_t stream, int grid, const SpmvShape& m,
+                           const double* val, double* sd, int* flag);
 hipError_t launch_update(const Launch& L, const CgArgs& a, int prec, int k, bool dx = false);
 hipError_t launch_finish(const Launch& L, const CgArgs& a, int k, int check_omega = 0);
 

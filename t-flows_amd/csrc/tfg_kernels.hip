@@ -52,6 +52,8 @@ __device__ __forceinline__ SpmvShape shape_of(const CgArgs& a) {
   m.pat = a.pat;
   m.poff = a.poff;
   m.plen = a.plen;
+  m.pslot = a.pslot;
+  m.sd = a.sd;
   return m;
 }
 
@@ -411,6 +413,65 @@ __global__ __launch_bounds__(kBlock) void k_spmv_plain(SpmvShape m, const double
       [&](int i, double yi, double, double) { y[i] = yi; });
 }
 
+// q = (A*fn) p and the partials of (p, q) with the symmetric engine: the
+// same products as k_spmv_pq, from the diagonal and upper diagonals.
+__global__ __launch_bounds__(kBlock) void k_spmv_pq_sym(CgArgs a, int k) {
+  __shared__ SymEngine::Smem sm;
+  __shared__ double s_tmp[4], s_out[1];
+  (void)k;
+  if (a.st->done) return;
+  const double fn = a.st->fn;
+  double v[1] = {0.0};
+  SymEngine::run<false>(
+      sm, shape_of(a), fn, a.p, [](int) { return 0.0; },
+      [&](int i, double qi, double pi, double) {
+        a.q[i] = qi;
+        v[0] = v[0] + pi * qi;
+      });
+  const int ops[1] = {OP_SUM};
+  publish<1>(v, ops, a.pt.spmv, a.pt.pstride, s_tmp, s_out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_spmv_plain_sym(SpmvShape m, const double* __restrict__ x,
+                                                           double* __restrict__ y) {
+  __shared__ SymEngine::Smem sm;
+  SymEngine::run<false>(
+      sm, m, 1.0, x, [](int) { return 0.0; }, [&](int i, double yi, double, double) { y[i] = yi; });
+}
+
+// Set-up of the symmetric engine for one val, one thread per row: the
+// diagonal and upper entries go to their slots; every lower entry a(i, j)
+// is compared bit for bit with a(j, i), found in row j's pattern (the
+// structure was checked symmetric when the handle was made), and any
+// difference sets *flag.
+__global__ __launch_bounds__(kBlock) void k_sym_pack(SpmvShape m, const double* __restrict__ val,
+                                                     double* __restrict__ sd, int* flag) {
+  __shared__ SymEngine::Smem sm;
+  SymEngine::tables(sm, m);
+  const long long n = m.n;
+  bool bad = false;
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < m.n; i += stride) {
+    const int p = m.pat[i];
+    const int L = sm.len[p];
+    const int rs = m.row[i];
+    for (int u = 0; u < L; ++u) {
+      const int off = sm.off[p * kPatMax + u], sl = sm.slot[p * kPatMax + u];
+      const double v = val[rs + u];
+      if (sl >= 0) {
+        sd[sl * n + i] = v;
+      } else {
+        const int j = i + off, pj = m.pat[j], Lj = sm.len[pj];
+        int mu = -1;
+        for (int t = 0; t < Lj; ++t)
+          if (sm.off[pj * kPatMax + t] == -off) mu = t;
+        if (mu < 0 || __double_as_longlong(val[m.row[j] + mu]) != __double_as_longlong(v)) bad = true;
+      }
+    }
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
 // ---------------------------------------------------------------------------
 // Engine dispatch
 // ---------------------------------------------------------------------------
@@ -427,6 +488,7 @@ int spmv_blocks_per_cu(int kind) {
     case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, true, double>>, kBlock, 0); break;
     case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, true, double, true>>, kBlock, 0); break;
     case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<PatternEngine<kCap, true, double>>, kBlock, 0); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq_sym, kBlock, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<16, double>>, kBlock, 0); break;
   }
   if (e != hipSuccess || nb < 1) return 1;
@@ -473,6 +535,10 @@ hipError_t launch_fold_x(const Launch& L, const CgArgs& a) {
 }
 
 hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k) {
+  if (L.sym) {
+    k_spmv_pq_sym<<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
+    return hipGetLastError();
+  }
   TFG_ENGINE_SWITCH(L, (k_spmv_pq<E><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k)));
   return hipGetLastError();
 }
@@ -516,7 +582,17 @@ hipError_t launch_copy16(hipStream_t stream, int grid, const void* src, void* ds
 
 hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* val,
                              const double* p, double* q) {
+  if (L.sym) {
+    k_spmv_plain_sym<<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
+    return hipGetLastError();
+  }
   TFG_ENGINE_SWITCH(L, (k_spmv_plain<E, false><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, val, nullptr, p, q)));
+  return hipGetLastError();
+}
+
+hipError_t launch_sym_pack(hipStream_t stream, int grid, const SpmvShape& m,
+                           const double* val, double* sd, int* flag) {
+  k_sym_pack<<<grid, kBlock, 0, stream>>>(m, val, sd, flag);
   return hipGetLastError();
 }
 

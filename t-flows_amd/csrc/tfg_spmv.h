@@ -373,6 +373,82 @@ struct PatternEngine {
   }
 };
 
+// Symmetric matrices on a row pattern: a(i, j) = a(j, i) bit for bit, so a
+// row's lower entries are the upper entries of earlier rows.  The values
+// are held once, as the diagonal and kSymMax or fewer upper diagonals
+// (one per distinct positive column offset), each n long: per product 8 B
+// per row and per upper entry instead of 8 B per nonzero.  One row per
+// thread over the same row blocks as the stage engines; entry u of a row
+// with pattern p has column i + off[p][u] and its value in slot[p][u]:
+//   0      diagonal          sd[i]
+//   k + 1  own upper entry   sd[(k + 1) n + i]
+//   -(k+1) lower entry       sd[(k + 1) n + i + off]  (row i+off's upper)
+// The terms are summed in the row's CSR order with the value scaled as the
+// stored one is, so products are bit-identical to the CSR engines'.
+struct SymEngine {
+  struct Smem {
+    int off[kPatCount * kPatMax];
+    int slot[kPatCount * kPatMax];
+    int len[kPatCount];
+  };
+
+  __device__ __forceinline__ static void tables(Smem& sm, const SpmvShape& m) {
+    for (int t = threadIdx.x; t < kPatCount * kPatMax; t += kBlock) {
+      sm.off[t] = m.poff[t];
+      sm.slot[t] = m.pslot[t];
+    }
+    for (int t = threadIdx.x; t < kPatCount; t += kBlock) sm.len[t] = m.plen[t];
+    __syncthreads();
+  }
+
+  template <bool SUB, typename Init, typename Epi>
+  __device__ __forceinline__ static void run(Smem& sm, const SpmvShape& m, double fn,
+                                             const double* x, Init init, Epi epi) {
+    tables(sm, m);
+    typedef __attribute__((address_space(4))) const int cint;
+    typedef __attribute__((address_space(1))) const double gT;
+    gT* xg = (gT*)x;
+    gT* sd = (gT*)m.sd;
+    asm volatile("" : "+s"(xg));
+    const long long n = m.n;
+    const int tid = threadIdx.x;
+    const XcdWalk w(m.nblk);
+    for (int lb = w.first; lb < w.end; lb += w.step) {
+      const int r0 = ((cint*)m.blk)[lb], r1 = ((cint*)m.blk)[lb + 1];
+      // lanes past the block's rows repeat its last row (loads stay in
+      // bounds; their sums are dropped)
+      const int i = min(r0 + tid, r1 - 1);
+      const int p = m.pat[i];
+      const int L = sm.len[p];
+      const int* po = sm.off + p * kPatMax;
+      const int* ps = sm.slot + p * kPatMax;
+      double av[8], xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int uu = min(u, L - 1);
+        const int off = po[uu], sl = ps[uu];
+        const long long at = sl >= 0 ? sl * n + i : -sl * n + i + off;
+        av[u] = sd[at] * fn;
+        xv[u] = xg[i + off];
+      }
+      double acc = init(i);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (u < L) acc = SUB ? acc - av[u] * xv[u] : acc + av[u] * xv[u];
+      if (r0 + tid < r1) {
+        for (int u = 8; u < L; ++u) {
+          const int off = po[u], sl = ps[u];
+          const long long at = sl >= 0 ? sl * n + i : -sl * n + i + off;
+          const double a = sd[at] * fn;
+          const double xj = x[i + off];
+          acc = SUB ? acc - a * xj : acc + a * xj;
+        }
+        epi(i, acc, xv[0], av[0]);
+      }
+    }
+  }
+};
+
 // One given row block lb per call (the batched solver's one-block-per-
 // workgroup launches): StreamEngine staging without the pipelining.
 template <int CAP, typename T>
@@ -542,7 +618,9 @@ struct WaveEngine {
 
 // kind: 0 stage/kCap, 1 stage/kCap2, 2 wave/4, 3 wave/8, 4 wave/16,
 // 5 pipelined stage/kCap, 6 the same with offset-coded columns, 7 the same
-// with row patterns.  The stage engines fall back to scalar staging
+// with row patterns (also kind 8: the symmetric engine serves only the CG
+// product and the plain product, when Launch::sym is set; every other
+// kernel uses the row-pattern engine).  The stage engines fall back to scalar staging
 // (ScalarStage) when val is not 16-byte aligned or nnz < 4.
 #define TFG_ENGINE_SWITCH_T(L, T, CALL)                                            \
   do {                                                                             \
@@ -564,6 +642,7 @@ struct WaveEngine {
         else { using E = ScalarStage<kCap, T>; CALL; }                             \
         break;                                                                     \
       case 7:                                                                      \
+      case 8:                                                                      \
         if ((L).vec_val) { using E = PatternEngine<kCap, true, T>; CALL; }         \
         else { using E = ScalarStage<kCap, T>; CALL; }                             \
         break;                                                                     \

@@ -70,7 +70,7 @@ def lib():
         P, I, D, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
         L.tfg_matrix_create.argtypes = [I, I, I, P, P, P, ctypes.POINTER(P)]
         L.tfg_matrix_destroy.argtypes = [P]
-        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P, P, P]
+        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P, P, P, P, P]
         L.tfg_solve_native.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I,
                                        P, D, P, P]
         L.tfg_solve_native_ex.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P,
@@ -183,16 +183,17 @@ class Native:
         self.h = h
 
     def info(self):
-        vals = [ctypes.c_int() for _ in range(7)]
+        vals = [ctypes.c_int() for _ in range(9)]
         _check(lib().tfg_matrix_info(self.h, *[ctypes.byref(v) for v in vals]))
         return dict(zip(["n", "nnz", "spmv_kind", "spmv_grid", "stage_blocks", "offsets",
-                         "patterns"], [v.value for v in vals]))
+                         "patterns", "sym_diagonals", "sym_state"], [v.value for v in vals]))
 
     def kinds(self):
         """The SpMV engines this matrix offers (6 needs offset codes, 7 row
-        patterns)."""
+        patterns, 8 row patterns on a symmetric structure)."""
         i = self.info()
-        return list(range(6)) + ([6] if i["offsets"] else []) + ([7] if i["patterns"] else [])
+        return (list(range(6)) + ([6] if i["offsets"] else []) + ([7] if i["patterns"] else []) +
+                ([8] if i["sym_diagonals"] else []))
 
     def close(self):
         if getattr(self, "h", None):
@@ -278,7 +279,7 @@ class Native:
         _check(lib().tfg_synchronize(self.h))
 
     SPMV_KINDS = ("stage2048", "stage4096", "wave4", "wave8", "wave16", "pipe2048",
-                  "pipe2048c", "pipe_pattern")
+                  "pipe2048c", "pipe_pattern", "sym_dia")
 
     def spmv_tune(self, dval):
         """Time every SpMV engine on dval; returns (chosen kind, {name: us})."""

@@ -9,6 +9,8 @@ answer by more than 1e-10 by itself; there the bar is widened to twice that
 rounding envelope, measured by re-running the oracle with blocked-pairwise
 sums (printed with every case).
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -118,6 +120,10 @@ def test_every_engine_same_solve(cases, oracle, gpu, which):
         r2 = _gpu_solve(gpu, s, "cg", "jacobi", 500, 1e-9, nat=nat)
         assert r1[0] == r2[0] and np.array_equal(r1[2], r2[2]), kind   # deterministic
         assert abs(r1[0] - it_o) <= 1 and _rel(r1[2], x_o) <= X_TOL, kind
+        if kind == 8:   # the symmetric engine ran (the hex matrix is symmetric)
+            assert nat.info()["sym_state"] == 1
+    if which == "knn10":
+        assert 8 not in nat.kinds()
 
 
 def test_deterministic(cases, gpu):
@@ -238,3 +244,33 @@ def test_graph_path_equals_kernel_path(cases, gpu, norm, crit, miter):
         out.append((r, x))
     for r, x in out[1:]:
         assert r == out[0][0] and np.array_equal(x, out[0][1])
+
+
+def test_symmetric_engine_graph_and_fallback(cases, oracle, gpu):
+    """Engine 8 in both launch paths gives the same bits; a val that is not
+    symmetric bit for bit on the same structure (one lower entry one ulp
+    off) falls back to the row-pattern products in that solve and still
+    meets the parity bar."""
+    s = cases.hex_pressure(18, 16, 14, rho_out=100.0)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    nat.spmv_select(8)
+    out = []
+    for path in (gpu.PATH_KERNELS, gpu.PATH_GRAPH):
+        o = gpu.Options()
+        o.path = path
+        x = np.zeros(s.n)
+        r = nat.solve_native("cg", "jacobi", s.val.copy(), x, s.b.copy(), 800, 1e-9, norm=1.0,
+                             options=o)
+        assert nat.info()["sym_state"] == 1
+        out.append((r, x))
+    assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
+    row0 = s.row - 1
+    i = s.n // 2
+    j = next(t for t in range(row0[i], row0[i + 1]) if s.col[t] - 1 < i)
+    val2 = s.val.copy()
+    val2[j] = np.nextafter(val2[j], 0.0)
+    s2 = dataclasses.replace(s, val=val2)
+    it_o, _, x_o = _oracle_solve(oracle, s2, "cg", "jacobi", 800, 1e-9, norm=1.0)
+    it, res, x, _ = _gpu_solve(gpu, s2, "cg", "jacobi", 800, 1e-9, norm=1.0, nat=nat)
+    assert nat.info()["sym_state"] == 0
+    assert abs(it - it_o) <= 2 and _rel(x, x_o) <= X_TOL

@@ -57,6 +57,45 @@ def test_spmv_bitexact_every_engine(cases, oracle, gpu, idx):
     assert kind in nat.kinds() and us[nat.SPMV_KINDS[kind]] == min(us.values())
 
 
+def test_symmetric_engine(cases, oracle, gpu):
+    """Engine 8 stores the diagonal and upper diagonals once and reads each
+    lower entry from its mirror: bit-identical products on a symmetric val.
+    A val that is not symmetric bit for bit (a lower entry one ulp off, or
+    -0.0 mirroring +0.0) makes that call use the row-pattern engine, with
+    the same exactness; the next symmetric val uses engine 8 again."""
+    s = cases.hex_pressure(24, 20, 16, rho_out=1000.0)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    assert nat.info()["sym_diagonals"] == 3 and 8 in nat.kinds()
+    nat.spmv_select(8)
+    p = np.random.default_rng(5).standard_normal(s.n)
+    dp, dq = gpu.DeviceArray(p), gpu.DeviceArray(np.zeros(s.n))
+    row0 = s.row - 1
+    i = s.n // 3
+    lower = [t for t in range(row0[i], row0[i + 1]) if s.col[t] - 1 < i]
+    upper = [t for t in range(row0[i], row0[i + 1]) if s.col[t] - 1 > i]
+    ulp = s.val.copy()
+    ulp[lower[0]] = np.nextafter(ulp[lower[0]], 0.0)
+    # +0.0 above the diagonal, -0.0 at its mirror
+    zeros = s.val.copy()
+    jj = s.col[upper[0]] - 1
+    mirror = next(t for t in range(row0[jj], row0[jj + 1]) if s.col[t] - 1 == i)
+    zeros[upper[0]], zeros[mirror] = 0.0, -0.0
+    for val, state in ((s.val, 1), (ulp, 0), (zeros, 0), (s.val, 1)):
+        A = oracle.Matrix(s.row, s.col, s.dia, val)
+        dv = gpu.DeviceArray(val)
+        nat.spmv(dv, dp, dq)
+        assert np.array_equal(dq.download(), oracle.spmv(A, p)), state
+        assert nat.info()["sym_state"] == state
+    up = cases.upwind(12)   # symmetric structure, non-symmetric values
+    nat_u = gpu.Native(up.row, up.col, up.dia)
+    nat_u.spmv_select(8)
+    pu = np.random.default_rng(6).standard_normal(up.n)
+    dq_u = gpu.DeviceArray(np.zeros(up.n))
+    nat_u.spmv(gpu.DeviceArray(up.val), gpu.DeviceArray(pu), dq_u)
+    assert nat_u.info()["sym_state"] == 0
+    assert np.array_equal(dq_u.download(), oracle.spmv(oracle.Matrix(up.row, up.col, up.dia, up.val), pu))
+
+
 def test_spmv_misaligned_values(cases, oracle, gpu):
     """val not 16-byte aligned: the scalar staging path must give the same bits."""
     s = cases.hex_pressure(30, 20, 10)
