@@ -148,9 +148,12 @@ struct tfg_matrix {
   int* poff = nullptr;
   int* plen = nullptr;
   int npat = 0;        // 0: no pattern engine on this matrix
+  int pmax = 0;        // longest pattern
   // symmetric engine: slot of each pattern entry, diagonal + upper diagonals
   // (allocated on first use), asymmetry flag and its pinned host copy
   int* pslot = nullptr;
+  int* sorder = nullptr;  // (start, end) row blocks in processing order
+  int nsorder = 0;
   int nsym = 0;        // upper diagonals; 0: no symmetric engine
   double* sym = nullptr;
   int* sym_flag = nullptr;
@@ -195,8 +198,9 @@ void size_grids(tfg_matrix* h) {
   for (int k = 0; k < tfg::kKinds; ++k) {
     const int per_cu = tfg::spmv_blocks_per_cu(k);
     const int work = (k == tfg::K_STAGE2048 || k == tfg::K_PIPE2048 || k == tfg::K_PIPE2048_CODED ||
-                      k == tfg::K_PIPE_PATTERN || k == tfg::K_SYM_DIA)
+                      k == tfg::K_PIPE_PATTERN)
                          ? h->nblk[0]
+                     : k == tfg::K_SYM_DIA ? std::max(1, h->nsorder)
                      : k == tfg::K_STAGE4096 ? h->nblk[1]
                                              : (h->nt + rows_per_pass(k) - 1) / rows_per_pass(k);
     int gs = std::min(work, std::max(1, h->cu_count) * per_cu);
@@ -238,6 +242,9 @@ tfg::SpmvShape shape_of(const tfg_matrix* h, int kind) {
   m.plen = h->plen;
   m.pslot = h->pslot;
   m.sd = h->sym;
+  m.pmax = h->pmax;
+  m.sorder = h->sorder;
+  m.nsorder = h->nsorder;
   return m;
 }
 
@@ -258,6 +265,9 @@ tfg::CgArgs args_of(const tfg_matrix* h, int kind) {
   a.plen = m.plen;
   a.pslot = m.pslot;
   a.sd = m.sd;
+  a.pmax = m.pmax;
+  a.sorder = m.sorder;
+  a.nsorder = m.nsorder;
   return a;
 }
 
@@ -302,7 +312,7 @@ void free_all(tfg_matrix* h) {
                  h->part, h->st, h->st_in, h->val32, h->e32, h->r32,
                  h->p32, h->q32, h->ad32, h->rt, h->nv[0], h->nv[1], h->nv[2],
                  h->nv[3], h->valt, h->tpos, h->code, h->tab, h->pat, h->poff, h->plen,
-                 h->pslot, h->sym, h->sym_flag,
+                 h->pslot, h->sym, h->sym_flag, h->sorder,
                  h->loc, h->icc_rows[0],
                  h->icc_rows[1], h->icc_d, h->icc_dinv, h->icc_y, h->icc_z};
   for (void* d : dev)
@@ -1017,12 +1027,14 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
         if (e == hipSuccess) e = hipMemcpy(h->plen, plen.data(), sizeof(int) * plen.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) { rc = fail(TFG_EHIP, "pattern upload: %s", hipGetErrorString(e)); break; }
         h->npat = (int)seqs.size();
-        // Symmetric engine: no buffer cells, at most kSymMax distinct
-        // positive offsets, no repeated offset in a row, and every entry
-        // (i, i + d) has its mirror (i + d, i).
+        for (const auto& q : seqs) h->pmax = std::max(h->pmax, (int)q.size());
+        // Symmetric engine: no buffer cells, the diagonal first in every
+        // row, at most kSymMax distinct positive offsets, no repeated offset
+        // in a row, and every entry (i, i + d) has its mirror (i + d, i).
         std::vector<int> up;
         bool sym_ok = ni == nt;
         for (const auto& q : seqs) {
+          if (q.empty() || q[0] != 0) sym_ok = false;  // diagonal first
           for (size_t u = 0; u < q.size() && sym_ok; ++u) {
             if (std::count(q.begin(), q.end(), q[u]) != 1) sym_ok = false;
             if (q[u] > 0 && std::find(up.begin(), up.end(), q[u]) == up.end()) up.push_back(q[u]);
@@ -1074,6 +1086,41 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
             e = hipMemcpy(h->pslot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice);
           if (e != hipSuccess) { rc = fail(TFG_EHIP, "slot upload: %s", hipGetErrorString(e)); break; }
           h->nsym = (int)up.size();
+          // Processing order of the symmetric engine: planes of D rows (D =
+          // the largest offset), blocks of at most 256 rows inside a plane,
+          // taken band by band (8 consecutive blocks of a plane) and, within
+          // a band, plane after plane.  The workgroups that run at the same
+          // time then cover one band over neighbouring planes, so most rows
+          // a row reads (one plane or one line away) are read by them too and
+          // stay in L2: 377 instead of 522 MB fetched per product at 200^3.
+          const int D = *std::max_element(up.begin(), up.end());
+          const int bpp = (D + tfg::kRowsPerBlock - 1) / tfg::kRowsPerBlock;
+          constexpr int kBand = 8;
+          std::vector<int> ord;
+          if (D >= kBand * tfg::kRowsPerBlock) {
+            const int planes = (int)(((long long)ni + D - 1) / D);
+            for (int j0 = 0; j0 < bpp; j0 += kBand)
+              for (int x = 0; x < planes; ++x)
+                for (int b = j0; b < std::min(bpp, j0 + kBand); ++b) {
+                  const long long r0 = (long long)x * D + (long long)b * tfg::kRowsPerBlock;
+                  const long long r1 = std::min<long long>(
+                      {r0 + tfg::kRowsPerBlock, (long long)(x + 1) * D, (long long)ni});
+                  if (r0 < r1) {
+                    ord.push_back((int)r0);
+                    ord.push_back((int)r1);
+                  }
+                }
+          } else {
+            for (int r0 = 0; r0 < ni; r0 += tfg::kRowsPerBlock) {
+              ord.push_back(r0);
+              ord.push_back(std::min(ni, r0 + tfg::kRowsPerBlock));
+            }
+          }
+          e = hipMalloc(&h->sorder, sizeof(int) * ord.size());
+          if (e == hipSuccess)
+            e = hipMemcpy(h->sorder, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice);
+          if (e != hipSuccess) { rc = fail(TFG_EHIP, "order upload: %s", hipGetErrorString(e)); break; }
+          h->nsorder = (int)ord.size() / 2;
         }
       }
     }

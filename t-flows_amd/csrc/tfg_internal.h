@@ -83,6 +83,9 @@ struct CgArgs {
   const int* poff;
   const int* plen;
   const int* pslot;    // symmetric engine: value slot of each pattern entry
+  int pmax;            // longest row pattern
+  const int* sorder;   // symmetric engine: (start, end) row blocks in processing order
+  int nsorder;
   const double* sd;    // symmetric engine: diagonal and upper diagonals
   const double* val;   // unscaled values (caller's)
   const double* b;     // unscaled right-hand side (caller's)
@@ -114,6 +117,9 @@ struct SpmvShape {
   // k, -(k + 1) slot k of row i + offset (a lower entry read from its mirror)
   const int* __restrict__ pslot;
   const double* __restrict__ sd;
+  int pmax;                          // longest row pattern
+  const int* __restrict__ sorder;    // symmetric engine: (start, end) row blocks
+  int nsorder;                       //   in processing order, and their count
 };
 
 // Textbook methods for non-symmetric systems (tfg_nonsym.hip).

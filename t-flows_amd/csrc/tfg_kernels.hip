@@ -54,6 +54,9 @@ __device__ __forceinline__ SpmvShape shape_of(const CgArgs& a) {
   m.plen = a.plen;
   m.pslot = a.pslot;
   m.sd = a.sd;
+  m.pmax = a.pmax;
+  m.sorder = a.sorder;
+  m.nsorder = a.nsorder;
   return m;
 }
 
@@ -415,6 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_plain(SpmvShape m, const double
 
 // q = (A*fn) p and the partials of (p, q) with the symmetric engine: the
 // same products as k_spmv_pq, from the diagonal and upper diagonals.
+template <int U>
 __global__ __launch_bounds__(kBlock) void k_spmv_pq_sym(CgArgs a, int k) {
   __shared__ SymEngine::Smem sm;
   __shared__ double s_tmp[4], s_out[1];
@@ -422,21 +426,23 @@ __global__ __launch_bounds__(kBlock) void k_spmv_pq_sym(CgArgs a, int k) {
   if (a.st->done) return;
   const double fn = a.st->fn;
   double v[1] = {0.0};
-  SymEngine::run<false>(
+  SymEngine::run<false, U>(
       sm, shape_of(a), fn, a.p, [](int) { return 0.0; },
-      [&](int i, double qi, double pi, double) {
+      [&](int i, double qi, double pi, double, bool own) {
         a.q[i] = qi;
-        v[0] = v[0] + pi * qi;
+        if (own) v[0] = v[0] + pi * qi;
       });
   const int ops[1] = {OP_SUM};
   publish<1>(v, ops, a.pt.spmv, a.pt.pstride, s_tmp, s_out);
 }
 
+template <int U>
 __global__ __launch_bounds__(kBlock) void k_spmv_plain_sym(SpmvShape m, const double* __restrict__ x,
                                                            double* __restrict__ y) {
   __shared__ SymEngine::Smem sm;
-  SymEngine::run<false>(
-      sm, m, 1.0, x, [](int) { return 0.0; }, [&](int i, double yi, double, double) { y[i] = yi; });
+  SymEngine::run<false, U>(
+      sm, m, 1.0, x, [](int) { return 0.0; },
+      [&](int i, double yi, double, double, bool) { y[i] = yi; });
 }
 
 // Set-up of the symmetric engine for one val, one thread per row: the
@@ -448,7 +454,6 @@ __global__ __launch_bounds__(kBlock) void k_sym_pack(SpmvShape m, const double* 
                                                      double* __restrict__ sd, int* flag) {
   __shared__ SymEngine::Smem sm;
   SymEngine::tables(sm, m);
-  const long long n = m.n;
   bool bad = false;
   const int stride = gridDim.x * kBlock;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < m.n; i += stride) {
@@ -456,10 +461,10 @@ __global__ __launch_bounds__(kBlock) void k_sym_pack(SpmvShape m, const double* 
     const int L = sm.len[p];
     const int rs = m.row[i];
     for (int u = 0; u < L; ++u) {
-      const int off = sm.off[p * kPatMax + u], sl = sm.slot[p * kPatMax + u];
+      const int off = sm.off[p * kPatMax + u];
       const double v = val[rs + u];
-      if (sl >= 0) {
-        sd[sl * n + i] = v;
+      if (off >= 0) {
+        sd[sm.delta[p * kPatMax + u] + i] = v;  // diagonal or own upper slot
       } else {
         const int j = i + off, pj = m.pat[j], Lj = sm.len[pj];
         int mu = -1;
@@ -488,7 +493,7 @@ int spmv_blocks_per_cu(int kind) {
     case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, true, double>>, kBlock, 0); break;
     case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, true, double, true>>, kBlock, 0); break;
     case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<PatternEngine<kCap, true, double>>, kBlock, 0); break;
-    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq_sym, kBlock, 0); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq_sym<7>, kBlock, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<16, double>>, kBlock, 0); break;
   }
   if (e != hipSuccess || nb < 1) return 1;
@@ -536,7 +541,9 @@ hipError_t launch_fold_x(const Launch& L, const CgArgs& a) {
 
 hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k) {
   if (L.sym) {
-    k_spmv_pq_sym<<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
+    if (a.pmax <= 7) k_spmv_pq_sym<7><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
+    else if (a.pmax <= 8) k_spmv_pq_sym<8><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
+    else k_spmv_pq_sym<kPatMax><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
     return hipGetLastError();
   }
   TFG_ENGINE_SWITCH(L, (k_spmv_pq<E><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k)));
@@ -583,7 +590,9 @@ hipError_t launch_copy16(hipStream_t stream, int grid, const void* src, void* ds
 hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* val,
                              const double* p, double* q) {
   if (L.sym) {
-    k_spmv_plain_sym<<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
+    if (m.pmax <= 7) k_spmv_plain_sym<7><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
+    else if (m.pmax <= 8) k_spmv_plain_sym<8><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
+    else k_spmv_plain_sym<kPatMax><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
     return hipGetLastError();
   }
   TFG_ENGINE_SWITCH(L, (k_spmv_plain<E, false><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, val, nullptr, p, q)));

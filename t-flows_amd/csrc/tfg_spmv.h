@@ -378,74 +378,109 @@ struct PatternEngine {
 // are held once, as the diagonal and kSymMax or fewer upper diagonals
 // (one per distinct positive column offset), each n long: per product 8 B
 // per row and per upper entry instead of 8 B per nonzero.  One row per
-// thread over the same row blocks as the stage engines; entry u of a row
+// thread over blocks of at most 256 rows taken in a plane-interleaved order
+// (built with the tables on the host); entry u of a row
 // with pattern p has column i + off[p][u] and its value in slot[p][u]:
 //   0      diagonal          sd[i]
 //   k + 1  own upper entry   sd[(k + 1) n + i]
 //   -(k+1) lower entry       sd[(k + 1) n + i + off]  (row i+off's upper)
 // The terms are summed in the row's CSR order with the value scaled as the
-// stored one is, so products are bit-identical to the CSR engines'.
+// stored one is, so products are bit-identical to the CSR engines'.  The
+// epilogue takes a fifth argument: whether the lane's row is its own (a
+// lane past the block's rows repeats the last row and must not count it
+// twice in a reduction).
 struct SymEngine {
+  // per pattern entry: column offset, and where its value sits relative to
+  // sd + i (delta = slot * n for the row's own entries, |slot| * n + offset
+  // for a lower entry), so that every term is two plain loads
   struct Smem {
+    long long delta[kPatCount * kPatMax];
     int off[kPatCount * kPatMax];
-    int slot[kPatCount * kPatMax];
     int len[kPatCount];
   };
 
   __device__ __forceinline__ static void tables(Smem& sm, const SpmvShape& m) {
+    const long long n = m.n;
     for (int t = threadIdx.x; t < kPatCount * kPatMax; t += kBlock) {
-      sm.off[t] = m.poff[t];
-      sm.slot[t] = m.pslot[t];
+      const int off = m.poff[t], sl = m.pslot[t];
+      sm.off[t] = off;
+      sm.delta[t] = sl >= 0 ? sl * n : -sl * n + off;
     }
     for (int t = threadIdx.x; t < kPatCount; t += kBlock) sm.len[t] = m.plen[t];
     __syncthreads();
   }
 
-  template <bool SUB, typename Init, typename Epi>
-  __device__ __forceinline__ static void run(Smem& sm, const SpmvShape& m, double fn,
-                                             const double* x, Init init, Epi epi) {
-    tables(sm, m);
+  // Block bounds and this lane's pattern id of row block lb (lanes past the
+  // block's rows take its last row, so every load stays in bounds).
+  struct Next {
+    int r0, r1, p;
+  };
+  __device__ __forceinline__ static Next fetch(const SpmvShape& m, int lb) {
     typedef __attribute__((address_space(4))) const int cint;
+    Next g;
+    g.r0 = ((cint*)m.sorder)[2 * lb];
+    g.r1 = ((cint*)m.sorder)[2 * lb + 1];
+    g.p = m.pat[min(g.r0 + (int)threadIdx.x, g.r1 - 1)];
+    return g;
+  }
+
+  // The next block's bounds and pattern ids are loaded after this block's
+  // value and x loads are issued, so they are in flight during its sums
+  // (vector loads complete in issue order): one memory round trip per
+  // block.  Entry 0 of every row is its diagonal (checked on the host), read
+  // with a non-temporal load since no other row uses it.  U terms (the
+  // longest row rounded up to 7, 8 or kPatMax: every load instruction costs
+  // load-pipeline time, so none is issued beyond the longest row) are
+  // loaded unconditionally, entries past the row's length repeating its
+  // last one, so no loop has a variable trip count and the compiler's waits
+  // stay exact.
+  template <bool SUB, int U, typename Init, typename Epi>
+  __device__ __forceinline__ static void step(Smem& sm, const SpmvShape& m, double fn,
+                                              const double* x, const XcdWalk& w, int lb, Next& g,
+                                              Init& init, Epi& epi) {
     typedef __attribute__((address_space(1))) const double gT;
     gT* xg = (gT*)x;
     gT* sd = (gT*)m.sd;
     asm volatile("" : "+s"(xg));
-    const long long n = m.n;
     const int tid = threadIdx.x;
-    const XcdWalk w(m.nblk);
-    for (int lb = w.first; lb < w.end; lb += w.step) {
-      const int r0 = ((cint*)m.blk)[lb], r1 = ((cint*)m.blk)[lb + 1];
-      // lanes past the block's rows repeat its last row (loads stay in
-      // bounds; their sums are dropped)
-      const int i = min(r0 + tid, r1 - 1);
-      const int p = m.pat[i];
-      const int L = sm.len[p];
-      const int* po = sm.off + p * kPatMax;
-      const int* ps = sm.slot + p * kPatMax;
-      double av[8], xv[8];
+    const int r0 = g.r0, r1 = g.r1, p = g.p;
+    const int i = min(r0 + tid, r1 - 1);
+    const int L = sm.len[p];
+    const int* po = sm.off + p * kPatMax;
+    const long long* pd = sm.delta + p * kPatMax;
+    double av[U], xv[U];
+    av[0] = __builtin_nontemporal_load(m.sd + i);
+    xv[0] = xg[i];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int uu = min(u, L - 1);
-        const int off = po[uu], sl = ps[uu];
-        const long long at = sl >= 0 ? sl * n + i : -sl * n + i + off;
-        av[u] = sd[at] * fn;
-        xv[u] = xg[i + off];
-      }
-      double acc = init(i);
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (u < L) acc = SUB ? acc - av[u] * xv[u] : acc + av[u] * xv[u];
-      if (r0 + tid < r1) {
-        for (int u = 8; u < L; ++u) {
-          const int off = po[u], sl = ps[u];
-          const long long at = sl >= 0 ? sl * n + i : -sl * n + i + off;
-          const double a = sd[at] * fn;
-          const double xj = x[i + off];
-          acc = SUB ? acc - a * xj : acc + a * xj;
-        }
-        epi(i, acc, xv[0], av[0]);
-      }
+    for (int u = 1; u < U; ++u) {
+      const int uu = min(u, L - 1);
+      av[u] = sd[pd[uu] + i];
+      xv[u] = xg[i + po[uu]];
     }
+    __builtin_amdgcn_sched_barrier(0);
+    g = fetch(m, min(lb + w.step, w.end - 1));
+    __builtin_amdgcn_sched_barrier(0);
+    double acc = init(i);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double a = av[u] * fn;
+      if (u < L) acc = SUB ? acc - a * xv[u] : acc + a * xv[u];
+    }
+    epi(i, acc, xv[0], av[0] * fn, r0 + tid < r1);
+  }
+
+  template <bool SUB, int U, typename Init, typename Epi>
+  __device__ __forceinline__ static void run(Smem& sm, const SpmvShape& m, double fn,
+                                             const double* x, Init init, Epi epi) {
+    tables(sm, m);
+    const XcdWalk w(m.nsorder);
+    if (w.first >= w.end) return;
+    Next g = fetch(m, w.first);
+    // first block peeled: the loop is entered, like its back edge, with the
+    // next ids and this block's store in flight
+    step<SUB, U>(sm, m, fn, x, w, w.first, g, init, epi);
+    for (int lb = w.first + w.step; lb < w.end; lb += w.step)
+      step<SUB, U>(sm, m, fn, x, w, lb, g, init, epi);
   }
 };
 
