@@ -318,13 +318,21 @@ def bench_knn_ir(args):
         err = float(np.linalg.norm(x - ref[4]) / np.linalg.norm(ref[4]))
         out_runs[name] = {"seconds": dt, "iterations": it, "refinements": rep["refinements"],
                           "relres": res, "true_relres": rep["true_relres"],
+                          "converged": bool(rep["true_relres"] < tol),
                           "error_vs_fp64_solution": err}
         log(f"bench knn: {name} tol {tol}: {dt:.3f} s, {it} iterations, true {rep['true_relres']:.2e}, "
             f"error {err:.2e}")
+    # the value is the fastest form that reached 1e-10 (true residual): on
+    # this pinned, nearly singular Laplacian restarted FP32 sweeps need far
+    # more iterations than FP64 CG and may hit the cap
+    ok = {k: v for k, v in out_runs.items() if v["converged"] and k != "fp32_ir_tighter"}
+    best_name = min(ok, key=lambda k: ok[k]["seconds"]) if ok else "fp64_cg"
     out = {
         "metric": "time to 1e-10 rel. residual, 4M-point kNN Laplacian, FP32 sweeps in FP64 refinement vs FP64 CG",
-        "value": out_runs["fp32_ir"]["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "higher_is_better": False, "dtype": "f32+f64", "data": "synthetic",
+        "value": out_runs[best_name]["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False,
+        "dtype": "f64" if best_name == "fp64_cg" else "f32+f64", "data": "synthetic",
+        "best": best_name,
         "config": {"workload": "kNN 200x200x100 jittered lattice, k 10..18, permuted; Jacobi; relres 1e-10",
                    "n": s.n, "nnz": s.nnz},
         "fp32_ir_speedup_vs_fp64": out_runs["fp64_cg"]["seconds"] / out_runs["fp32_ir"]["seconds"],
