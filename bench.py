@@ -463,6 +463,67 @@ def bench_assembly(args):
     }), flush=True)
 
 
+def bench_momentum(args):
+    """Momentum system of one component fed on the device: 200^3 face and
+    cell data (MINMOD advection with the implicit upwind part, BDF2 inertia,
+    cross diffusion, forces, under-relaxation) assembled into A % val, b and
+    v_m by the face kernel, against uploading A % val and b, and against the
+    oracle's serial face loops (1 thread) on the same data."""
+    import tflows_cases
+    import tflows_gpu as G
+
+    N = args.n
+    kw = dict(component=2, adv="minmod", blend_matrix=True, td="parabolic")
+    reps = max(3, args.steps)
+    timings = {}
+    for order in ("shuffled", "natural"):
+        t0 = time.perf_counter()
+        f = tflows_cases.hex_faces(N, shuffle=order == "shuffled")
+        mf = tflows_cases.momentum_fields(f)
+        log(f"bench momentum: {N}^3, {f.n_faces} faces ({order} order), generated in "
+            f"{time.perf_counter() - t0:.1f} s")
+        nat = G.Native(f.row, f.col, f.dia)
+        faces = G.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+        m = G.Momentum(mf, **kw)
+        dval, db, dvm = (G.DeviceArray(np.zeros(k)) for k in (len(f.col), f.n, f.n))
+        faces.momentum(m, dval, db, dvm)
+        t = time.perf_counter()
+        for _ in range(reps):
+            faces.momentum(m, dval, db, dvm)
+        timings[order] = (time.perf_counter() - t) / reps
+        log(f"bench momentum: {order} face order: {timings[order] * 1e3:.2f} ms")
+    t_asm = timings["natural"]
+    hval, hb = dval.download(), db.download()
+    t = time.perf_counter()
+    for _ in range(reps):
+        dval.upload(hval)
+        db.upload(hb)
+    t_up = (time.perf_counter() - t) / reps
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        t = time.perf_counter()
+        val_o, b_o, _ = oracle.momentum_assemble(f, mf, **kw)
+        t_cpu = time.perf_counter() - t
+        same = bool(np.array_equal(val_o, hval) and np.array_equal(b_o, hb))
+        cpu = {"value": t_cpu * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+               "sample": "oracle (C restatement of the face loops), the whole system once",
+               "bit_identical": same}
+        log(f"bench momentum: oracle {t_cpu * 1e3:.0f} ms, bit-identical {same}")
+    log(f"bench momentum: device assembly {t_asm * 1e3:.2f} ms (faces in grid order), "
+        f"upload {t_up * 1e3:.2f} ms")
+    print(json.dumps({
+        "metric": "momentum-system assembly (one component) on the device vs upload of A % val and b, 200^3",
+        "value": t_asm * 1e3, "unit": "ms", "n_gpus": 1, "steps": reps, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"hex {N}^3 face data, momentum matrix + rhs + v_m, MINMOD, "
+                               "implicit upwind blend, BDF2, faces in grid order",
+                   "cells": f.n, "faces": f.n_faces, "nnz": len(f.col)},
+        "shuffled_faces_ms": timings["shuffled"] * 1e3,
+        "upload_ms": t_up * 1e3, "speedup_vs_upload": t_up / t_asm, "cpu_baseline": cpu,
+    }), flush=True)
+
+
 def bench_icc(args):
     """The reference's default preconditioner on the headline system: ICC
     (DIC, level-scheduled, bit-exact sweeps) against Jacobi, CG to relres
@@ -560,7 +621,7 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the known-traffic copy kernel (counter calibration)")
     ap.add_argument("--workload", default="twophase200",
-                    choices=["twophase200", "upwind120", "knn4m", "batch256", "assembly200",
+                    choices=["twophase200", "upwind120", "knn4m", "batch256", "assembly200", "momentum200",
                              "icc200", "poisson40"],
                     help="the headline solve (default) or one of the other measured workloads")
     ap.add_argument("--summarize-counters", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
@@ -588,7 +649,7 @@ def main():
         sys.exit(2)
     if args.workload != "twophase200":
         {"upwind120": bench_upwind, "knn4m": bench_knn_ir, "batch256": bench_batch,
-         "assembly200": bench_assembly, "icc200": bench_icc,
+         "assembly200": bench_assembly, "momentum200": bench_momentum, "icc200": bench_icc,
          "poisson40": bench_poisson40}[args.workload](args)
         return
 

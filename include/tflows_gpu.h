@@ -198,6 +198,58 @@ int tfg_pressure_assemble(tfg_faces* faces, const double* fc, const double* fw,
                           const double* vm, const double* vflux, double* val,
                           double* b, int singular);
 
+/* ---- momentum assembly on the device ------------------------------------
+ * Matrix, right-hand side and M % v_m of one velocity component from face
+ * data, as Compute_Momentum.f90:119-366 builds them with Numerics_Mod
+ * (Advection_Term with Min_Max and Advection_Scheme, Inertial_Term,
+ * Under_Relax): advection in the chosen scheme with the implicit upwind
+ * part when blend_matrix is set, the diffusion face loop with the caller's
+ * face viscosity and stress (Turb % Face_Vis / Face_Stress), explicit cross
+ * diffusion, inertia, global pressure drop, cell forces, the pressure
+ * gradient, optional surface tension, v_m = vol / a_ii and under-
+ * relaxation.  Bit-identical to the serial loops.  Not included: user
+ * forces (User_Mod_Force), PISO's explicit part, vel_max.
+ * Needs the faces handle of tfg_faces_create (its pos, dia and c2 < 0 for
+ * boundary cells; each boundary cell on one face, each matrix entry set by
+ * one face).  All pointers are device pointers.  Cell arrays point at cell
+ * 1; those marked [b] also hold the boundary cells c = -n_bnd..-1 at offset
+ * c - 1 (Fortran's (-n_bnd:n) array passed as c_loc(a(1))). */
+#define TFG_ADV_CENTRAL 0
+#define TFG_ADV_UPWIND 1
+#define TFG_ADV_LUDS 2
+#define TFG_ADV_QUICK 3
+#define TFG_ADV_SMART 4
+#define TFG_ADV_GAMMA 5
+#define TFG_ADV_MINMOD 6
+#define TFG_ADV_BLENDED 7
+#define TFG_ADV_SUPERBEE 8
+#define TFG_ADV_AVL_SMART 9
+#define TFG_TD_LINEAR 0      /* backward Euler */
+#define TFG_TD_PARABOLIC 1   /* BDF2 */
+#define TFG_TD_NONE 2        /* no inertial term */
+typedef struct {
+  int component;      /* 1, 2, 3: u, v, w (which of s, d, grad are i, j, k) */
+  int n_bnd;          /* boundary cells */
+  int adv_scheme;     /* TFG_ADV_* (ui % adv_scheme) */
+  int blend_matrix;   /* ui % blend_matrix */
+  int td_scheme;      /* TFG_TD_* (ui % td_scheme) */
+  double blend;       /* ui % blends(1), BLENDED only */
+  double dt, urf, p_drop;  /* Flow % dt, ui % urf, bulk % p_drop_i */
+  /* per face */
+  const double *f, *fw, *fc, *v_flux, *vis_eff, *f_stress;  /* f_stress may be NULL */
+  const double *sx, *sy, *sz, *dx, *dy, *dz;
+  const int* wall;    /* 1: boundary face with INFLOW, WALL, CONVECT or WALLFL */
+  /* per cell: phi_n [b] = ui % n, phi_o, phi_oo, gx gy gz [b] = ui % x y z,
+   * uj_i uk_i [b] = the other components' gradients along i, density [b],
+   * vol, p_i = p % x|y|z, cell_f (may be NULL) = Flow % cell_fi, st (may be
+   * NULL) = Vof % surf_fi */
+  const double *phi_n, *phi_o, *phi_oo, *gx, *gy, *gz, *uj_i, *uk_i, *density, *vol, *p_i,
+      *cell_f, *st;
+} tfg_momentum;
+/* val (nnz), b (nt), v_m (nt) out; stream-ordered like tfg_pressure_assemble. */
+int tfg_momentum_assemble(tfg_faces* faces, const tfg_momentum* m, double* val, double* b,
+                          double* v_m);
+
 /* ---- several MPI ranks -----------------------------------------------------
  * A matrix with buffer cells (ni < nt: rows ni+1..nt are copies of other
  * ranks' cells) is solved with the caller's communication, which stands in

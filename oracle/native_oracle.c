@@ -628,3 +628,197 @@ void or_pressure_assemble(int nc, int nnz, int n_faces, const int* c1, const int
   if (singular)                                 /* Set_Singular.f90:28-31 */
     for (int i = 0; i < nc; i++) val[dia[i] - 1] = val[dia[i] - 1] * (1.0 + OR_MICRO);
 }
+
+/* ------------------------------------------------------------------------
+ * Momentum assembly of component m->component (1, 2, 3 = u, v, w), serial in
+ * the reference's loop order.  Cells are 1-based; a boundary cell c < 0 of a
+ * [b] array is element c - 1 of the pointer (the pointer is cell 1).
+ * ---------------------------------------------------------------------- */
+#define OR_FEMTO 1.0e-15
+#define CELL(a, c) ((a)[(c) - 1])
+
+static double or_min(double a, double b) { return b < a ? b : a; }
+static double or_max(double a, double b) { return b > a ? b : a; }
+
+/* Numerics_Mod/Advection_Scheme.f90:22-183: face value phi_f of face s for
+ * every scheme but CENTRAL (which Advection_Term never passes here). */
+static double or_adv_scheme(const or_momentum* m, int s, int c1, int c2,
+                            const double* pmin, const double* pmax) {
+  const double* phi = m->phi_n;
+  double fj, sgn;
+  int c, d;
+  if (m->v_flux[s] > 0.0) {                          /* :43-54 */
+    fj = 1.0 - m->f[s]; c = c1; d = c2; sgn = +1.0;
+  } else {
+    fj = m->f[s]; c = c2; d = c1; sgn = -1.0;
+  }
+  if (m->adv_scheme == OR_ADV_UPWIND) return CELL(phi, c);              /* :103-105 */
+  if (m->adv_scheme == OR_ADV_BLENDED) {                                 /* :107-112 */
+    const double phi_f_c = CELL(phi, c) + fj * sgn * (CELL(phi, c2) - CELL(phi, c1));
+    const double phi_f_u = CELL(phi, c);
+    return m->blend * phi_f_c + (1.0 - m->blend) * phi_f_u;
+  }
+  const double gd = CELL(m->gx, c) * m->dx[s] + CELL(m->gy, c) * m->dy[s] + CELL(m->gz, c) * m->dz[s];
+  double phi_f;
+  if (m->adv_scheme == OR_ADV_GAMMA) {                                   /* :117-139 */
+    const double beta = 0.1;
+    const double phi_star = m->v_flux[s] > 0.0
+                                ? 1.0 - (CELL(phi, d) - CELL(phi, c)) / (2.0 * gd)
+                                : 1.0 + (CELL(phi, d) - CELL(phi, c)) / (2.0 * gd);
+    const double gamma_c = phi_star / beta;
+    if (phi_star < beta && phi_star > 0.0)
+      phi_f = (1.0 - gamma_c * (1.0 - m->f[s])) * CELL(phi, c) + gamma_c * (1.0 - m->f[s]) * CELL(phi, d);
+    else if (phi_star < 1.0 && phi_star >= beta)
+      phi_f = m->f[s] * CELL(phi, c) + (1.0 - m->f[s]) * CELL(phi, d);
+    else
+      phi_f = CELL(phi, c);
+    return phi_f;
+  }
+  const double phi_star = m->v_flux[s] > 0.0 ? CELL(phi, d) - 2.0 * gd    /* :56-64 */
+                                             : CELL(phi, d) + 2.0 * gd;
+  const double phi_u = or_max(CELL(pmin, c), or_min(phi_star, CELL(pmax, c)));
+  double denom = CELL(phi, d) - CELL(phi, c);
+  if (fabs(denom) < OR_FEMTO) denom = denom + copysign(OR_FEMTO, denom);
+  const double rj = (CELL(phi, c) - phi_u) / denom;
+  const double g_d = 0.5 * fj * (1.0 + fj), g_u = 0.5 * fj * (1.0 - fj);
+  double phij = fj;                                                      /* :78-101 */
+  if (m->adv_scheme == OR_ADV_QUICK) {
+    const double alfa = 0.0;
+    phij = (g_d - alfa) + (g_u + alfa) * rj;
+  } else if (m->adv_scheme == OR_ADV_LUDS) {
+    const double alfa = 0.5 * fj * (1 + fj);
+    phij = (g_d - alfa) + (g_u + alfa) * rj;
+  } else if (m->adv_scheme == OR_ADV_MINMOD) {
+    phij = fj * or_max(0.0, or_min(rj, 1.0));
+  } else if (m->adv_scheme == OR_ADV_SMART) {
+    const double beta1 = 3.0, beta2 = 1.0;
+    phij = or_max(0.0, or_min(or_min((beta1 - 1.0) * rj, g_d + g_u * rj), beta2));
+  } else if (m->adv_scheme == OR_ADV_AVL_SMART) {
+    const double beta1 = 1.0 + fj * (2.0 + fj), beta2 = fj * (2.0 - fj);
+    phij = or_max(0.0, or_min(or_min((beta1 - 1.0) * rj, g_d + g_u * rj), beta2));
+  } else if (m->adv_scheme == OR_ADV_SUPERBEE) {
+    phij = 0.5 * or_max(or_max(0.0, or_min(2.0 * rj, 1.0)), or_min(rj, 2.0));
+  }
+  return CELL(phi, c) + phij * sgn * (CELL(phi, c2) - CELL(phi, c1));    /* :114 */
+}
+
+void or_momentum_assemble(int nc, int nnz, int n_faces, const int* c1, const int* c2,
+                          const int* pos, const int* dia, const or_momentum* m, double* val,
+                          double* b, double* v_m) {
+  const int nb = m->n_bnd;
+  double* cross = calloc((size_t)nc, sizeof(double));
+  double* fi = calloc((size_t)nc, sizeof(double));
+  double* advect = calloc((size_t)nc, sizeof(double));
+  double* upwind = calloc((size_t)nc, sizeof(double));
+  double* mn = malloc(sizeof(double) * (size_t)(nb + 1 + nc));
+  double* mx = malloc(sizeof(double) * (size_t)(nb + 1 + nc));
+  double* pmin = mn + nb + 1;       /* cell 1; boundary cells below it */
+  double* pmax = mx + nb + 1;
+  const double* phi = m->phi_n;
+  const double* rho = m->density;
+  /* rotated views of component i (Compute_Momentum.f90:58-98) */
+  const int i = m->component;
+  const double* g[3] = {m->gx, m->gy, m->gz};
+  const double* sv[3] = {m->sx, m->sy, m->sz};
+  const double* dv[3] = {m->dx, m->dy, m->dz};
+  const double *ui_i = g[(i - 1) % 3], *ui_j = g[i % 3], *ui_k = g[(i + 1) % 3];
+  const double *si = sv[(i - 1) % 3], *sj = sv[i % 3], *sk = sv[(i + 1) % 3];
+  const double *di = dv[(i - 1) % 3], *dj = dv[i % 3], *dk = dv[(i + 1) % 3];
+
+  for (int j = 0; j < nnz; j++) val[j] = 0.0;                 /* :119-123 */
+  for (int c = 0; c < nc; c++) b[c] = 0.0;
+
+  /* Advection_Term.f90:35-120 with b = fi */
+  if (m->adv_scheme != OR_ADV_CENTRAL) {                      /* Min_Max.f90:19-34 */
+    for (int c = -nb; c <= nc; c++)
+      if (c != 0) { CELL(pmin, c) = CELL(phi, c); CELL(pmax, c) = CELL(phi, c); }
+    for (int s = 0; s < n_faces; s++) {
+      const int a = c1[s], e = c2[s];
+      CELL(pmin, a) = or_min(CELL(pmin, a), CELL(phi, e));
+      CELL(pmin, e) = or_min(CELL(pmin, e), CELL(phi, a));
+      CELL(pmax, a) = or_max(CELL(pmax, a), CELL(phi, e));
+      CELL(pmax, e) = or_max(CELL(pmax, e), CELL(phi, a));
+    }
+  }
+  for (int pass = 0; pass < 2; pass++) {        /* boundary regions, then domain faces */
+    for (int s = 0; s < n_faces; s++) {
+      const int a = c1[s], e = c2[s];
+      if ((pass == 0) != (e < 0)) continue;
+      double phif = m->f[s] * CELL(phi, a) + (1.0 - m->f[s]) * CELL(phi, e);
+      if (m->adv_scheme != OR_ADV_CENTRAL) phif = or_adv_scheme(m, s, a, e, pmin, pmax);
+      CELL(advect, a) = CELL(advect, a) - m->v_flux[s] * phif * CELL(rho, a);
+      if (e > 0) CELL(advect, e) = CELL(advect, e) + m->v_flux[s] * phif * CELL(rho, e);
+    }
+    if (m->blend_matrix) {
+      for (int s = 0; s < n_faces; s++) {
+        const int a = c1[s], e = c2[s];
+        if ((pass == 0) != (e < 0)) continue;
+        const double up = m->v_flux[s] < 0 ? CELL(phi, e) : CELL(phi, a);
+        CELL(upwind, a) = CELL(upwind, a) - m->v_flux[s] * up * CELL(rho, a);
+        if (e > 0) CELL(upwind, e) = CELL(upwind, e) + m->v_flux[s] * up * CELL(rho, e);
+      }
+    }
+  }
+  for (int c = 0; c < nc; c++) fi[c] = fi[c] + advect[c] - upwind[c];   /* :115-117 */
+
+  /* diffusion, Compute_Momentum.f90:238-301 */
+  for (int s = 0; s < n_faces; s++) {
+    const int a = c1[s], e = c2[s];
+    const double w = m->fw[s], vis_eff = m->vis_eff[s];
+    const double f_stress = m->f_stress ? m->f_stress[s] : 0.0;
+    const double ui_i_f = w * CELL(ui_i, a) + (1.0 - w) * CELL(ui_i, e);
+    const double ui_j_f = w * CELL(ui_j, a) + (1.0 - w) * CELL(ui_j, e);
+    const double ui_k_f = w * CELL(ui_k, a) + (1.0 - w) * CELL(ui_k, e);
+    const double uj_i_f = w * CELL(m->uj_i, a) + (1.0 - w) * CELL(m->uj_i, e);
+    const double uk_i_f = w * CELL(m->uk_i, a) + (1.0 - w) * CELL(m->uk_i, e);
+    const double ui_si = (ui_i_f + ui_i_f) * si[s] + (ui_j_f + uj_i_f) * sj[s] + (ui_k_f + uk_i_f) * sk[s];
+    const double ui_di = ui_i_f * di[s] + ui_j_f * dj[s] + ui_k_f * dk[s];
+    const double f_ex = vis_eff * ui_si;
+    const double m0 = vis_eff * m->fc[s];
+    const double f_im = ui_di * m0;
+    CELL(cross, a) = CELL(cross, a) + f_ex - f_im + f_stress * CELL(rho, a);
+    if (e > 0) CELL(cross, e) = CELL(cross, e) - f_ex + f_im - f_stress * CELL(rho, e);
+    double m12 = m0, m21 = m0;
+    if (m->blend_matrix) {
+      m12 = m12 - or_min(m->v_flux[s], 0.0) * CELL(rho, a);
+      m21 = m21 + or_max(m->v_flux[s], 0.0) * CELL(rho, e);
+    }
+    if (e > 0) {
+      val[pos[2 * s] - 1] = val[pos[2 * s] - 1] - m12;
+      val[CELL(dia, a) - 1] = val[CELL(dia, a) - 1] + m12;
+      val[pos[2 * s + 1] - 1] = val[pos[2 * s + 1] - 1] - m21;
+      val[CELL(dia, e) - 1] = val[CELL(dia, e) - 1] + m21;
+    } else if (e < 0 && m->wall[s]) {
+      val[CELL(dia, a) - 1] = val[CELL(dia, a) - 1] + m12;
+      CELL(fi, a) = CELL(fi, a) + m12 * CELL(phi, e);
+    }
+  }
+  for (int c = 1; c <= nc; c++) CELL(fi, c) = CELL(fi, c) + CELL(cross, c);   /* :306-308 */
+
+  for (int c = 1; c <= nc; c++) {                             /* Inertial_Term.f90:27-41 */
+    const double a0 = CELL(rho, c) * CELL(m->vol, c) / m->dt;
+    double* dd = &val[CELL(dia, c) - 1];
+    if (m->td_scheme == OR_TD_LINEAR) {
+      *dd = *dd + a0;
+      CELL(fi, c) = CELL(fi, c) + a0 * CELL(m->phi_o, c);
+    } else if (m->td_scheme == OR_TD_PARABOLIC) {
+      *dd = *dd + 1.5 * a0;
+      CELL(fi, c) = CELL(fi, c) + 2.0 * a0 * CELL(m->phi_o, c) - 0.5 * a0 * CELL(m->phi_oo, c);
+    }
+  }
+  for (int c = 1; c <= nc; c++) {                  /* Compute_Momentum.f90:322-350 */
+    CELL(fi, c) = CELL(fi, c) + m->p_drop * CELL(m->vol, c);
+    const double cf = m->cell_f ? CELL(m->cell_f, c) : 0.0;
+    CELL(fi, c) = CELL(fi, c) + cf * CELL(m->vol, c);
+    CELL(b, c) = CELL(fi, c) - CELL(m->p_i, c) * CELL(m->vol, c);
+    if (m->st) CELL(b, c) = CELL(b, c) + CELL(m->st, c) * CELL(m->vol, c);
+  }
+  for (int c = 1; c <= nc; c++)                                 /* :361-363 */
+    CELL(v_m, c) = CELL(m->vol, c) / val[CELL(dia, c) - 1];
+  for (int c = 1; c <= nc; c++) {                               /* Under_Relax.f90:18-24 */
+    double* dd = &val[CELL(dia, c) - 1];
+    *dd = *dd / m->urf;
+    CELL(b, c) = CELL(b, c) + *dd * (1.0 - m->urf) * CELL(phi, c);
+  }
+  free(cross); free(fi); free(advect); free(upwind); free(mn); free(mx);
+}

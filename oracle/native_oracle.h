@@ -72,6 +72,28 @@ void or_pressure_assemble(int nc, int nnz, int n_faces, const int* c1, const int
                           const double* vm, const double* vflux, double* val, double* b,
                           int singular);
 
+/* Momentum assembly of one velocity component from face data
+ * (Compute_Momentum.f90:119-366 with Numerics_Mod Advection_Term, Min_Max,
+ * Advection_Scheme, Inertial_Term, Under_Relax), serial in the reference's
+ * loop order; see native_oracle.c.  Cell arrays point at cell 1; those
+ * marked [b] also hold the boundary cells c = -n_bnd..-1 at offset c - 1. */
+enum { OR_ADV_CENTRAL = 0, OR_ADV_UPWIND = 1, OR_ADV_LUDS = 2, OR_ADV_QUICK = 3,
+       OR_ADV_SMART = 4, OR_ADV_GAMMA = 5, OR_ADV_MINMOD = 6, OR_ADV_BLENDED = 7,
+       OR_ADV_SUPERBEE = 8, OR_ADV_AVL_SMART = 9 };
+enum { OR_TD_LINEAR = 0, OR_TD_PARABOLIC = 1, OR_TD_NONE = 2 };
+typedef struct {
+  int component, n_bnd, adv_scheme, blend_matrix, td_scheme;
+  double blend, dt, urf, p_drop;
+  const double *f, *fw, *fc, *v_flux, *vis_eff, *f_stress;   /* per face; f_stress may be NULL */
+  const double *sx, *sy, *sz, *dx, *dy, *dz;                  /* per face */
+  const int* wall;                                            /* per face */
+  const double *phi_n, *phi_o, *phi_oo, *gx, *gy, *gz, *uj_i, *uk_i, *density, *vol, *p_i,
+      *cell_f, *st;  /* [b]: phi_n gx gy gz uj_i uk_i density; cell_f, st may be NULL */
+} or_momentum;
+void or_momentum_assemble(int nc, int nnz, int n_faces, const int* c1, const int* c2,
+                          const int* pos, const int* dia, const or_momentum* m, double* val,
+                          double* b, double* v_m);
+
 /* 0: the reference's serial summation (default); 1, 2: blocked-pairwise
  * summation (blocks of 64, 1000) of every global reduction, to measure
  * rounding sensitivity. */

@@ -61,6 +61,7 @@ def lib(omp=False):
         L.or_transpose_positions.restype = I
         L.or_transpose_positions.argtypes = [P, P]
         L.or_pressure_assemble.argtypes = [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I]
+        L.or_momentum_assemble.argtypes = [I, I, I, P, P, P, P, P, P, P, P]
         _libs[name] = L
     return _libs[name]
 
@@ -217,3 +218,59 @@ def pressure_assemble(f, singular=False):
     lib().or_pressure_assemble(len(f.dia), len(f.col), len(f.c1), *[_p(a) for a in arrs],
                                _p(val), _p(b), 1 if singular else 0)
     return val, b
+
+
+# advection schemes (Numerics_Mod/Advection_Scheme_Code.f90 names) and time
+# schemes of the momentum assembly
+ADV = {"central": 0, "upwind": 1, "luds": 2, "quick": 3, "smart": 4, "gamma": 5,
+       "minmod": 6, "blended": 7, "superbee": 8, "avl_smart": 9}
+TD = {"linear": 0, "parabolic": 1, "none": 2}
+
+
+class _Momentum(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_int) for k in ("component", "n_bnd", "adv_scheme", "blend_matrix",
+                                              "td_scheme")] +
+                [(k, ctypes.c_double) for k in ("blend", "dt", "urf", "p_drop")] +
+                [(k, ctypes.c_void_p) for k in ("f", "fw", "fc", "v_flux", "vis_eff", "f_stress",
+                                                "sx", "sy", "sz", "dx", "dy", "dz", "wall",
+                                                "phi_n", "phi_o", "phi_oo", "gx", "gy", "gz",
+                                                "uj_i", "uk_i", "density", "vol", "p_i",
+                                                "cell_f", "st")])
+
+
+MOMENTUM_EXT = ("phi_n", "gx", "gy", "gz", "uj_i", "uk_i", "density")
+
+
+def momentum_assemble(f, mf, component=1, adv="central", blend_matrix=True, td="linear",
+                      blend=0.8, dt=1e-2, urf=0.7, p_drop=0.5, stress=True, cell_force=True,
+                      surface_tension=False):
+    """Serial momentum assembly of one component (Compute_Momentum.f90
+    :119-366 with Numerics_Mod) on a Faces set f and tflows_cases
+    MomentumFields mf; returns (val, b, v_m)."""
+    keep = []
+
+    def ptr(a, ext=False):
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        return a.ctypes.data + ((mf.n_bnd + 1) * a.itemsize if ext else 0)
+
+    m = _Momentum()
+    m.component, m.n_bnd, m.adv_scheme = component, mf.n_bnd, ADV[adv]
+    m.blend_matrix, m.td_scheme = int(blend_matrix), TD[td]
+    m.blend, m.dt, m.urf, m.p_drop = blend, dt, urf, p_drop
+    for k in ("f", "fw", "fc", "v_flux", "vis_eff", "sx", "sy", "sz", "dx", "dy", "dz", "phi_o",
+              "phi_oo", "vol", "p_i"):
+        setattr(m, k, ptr(getattr(mf, k)))
+    m.wall = ptr(np.ascontiguousarray(mf.wall, dtype=np.int32))
+    for k in MOMENTUM_EXT:
+        setattr(m, k, ptr(getattr(mf, k), ext=True))
+    m.f_stress = ptr(mf.f_stress) if stress else None
+    m.cell_f = ptr(mf.cell_f) if cell_force else None
+    m.st = ptr(mf.st) if surface_tension else None
+    val = np.zeros(len(f.col))
+    b = np.zeros(len(f.dia))
+    vm = np.zeros(len(f.dia))
+    arrs = [np.ascontiguousarray(a) for a in (f.c1, f.c2, f.pos, f.dia)]
+    lib().or_momentum_assemble(len(f.dia), len(f.col), len(f.c1), *[_p(a) for a in arrs],
+                               ctypes.byref(m), _p(val), _p(b), _p(vm))
+    return val, b, vm

@@ -814,11 +814,17 @@ struct tfg_faces {
   int n_faces = 0;
   int *cf_off = nullptr, *cf_face = nullptr, *ent_face = nullptr, *c1 = nullptr, *c2 = nullptr;
   uint8_t* flags = nullptr;
+  int* c2b = nullptr;        // c2 as given (1-based, boundary cells < 0)
+  int max_bnd = 0;           // largest boundary cell number
+  bool shared_entry = false; // some matrix entry set by two faces
+  bool shared_bnd = false;   // some boundary cell on two faces
+  double *pmin = nullptr, *pmax = nullptr;  // momentum scratch (nt each), on first use
 };
 
 static void faces_free(tfg_faces* f) {
   if (!f) return;
-  void* dev[] = {f->cf_off, f->cf_face, f->ent_face, f->c1, f->c2, f->flags};
+  void* dev[] = {f->cf_off, f->cf_face, f->ent_face, f->c1, f->c2, f->flags, f->c2b, f->pmin,
+                 f->pmax};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   delete f;
@@ -837,6 +843,12 @@ int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2, c
   HIP_TRY(hipMemcpy(col.data(), h->col, sizeof(int) * (size_t)h->nnz, hipMemcpyDeviceToHost));
   std::vector<int> cnt(n + 1, 0), ent(h->nnz, -1), a(n_faces), b(n_faces);
   std::vector<uint8_t> fl(n_faces);
+  bool shared_entry = false;
+  int max_bnd = 0;
+  for (int s = 0; s < n_faces; ++s)
+    if (c2[s] < 0) max_bnd = std::max(max_bnd, -c2[s]);
+  std::vector<char> bnd_seen((size_t)max_bnd + 1, 0);
+  bool shared_bnd = false;
   for (int s = 0; s < n_faces; ++s) {
     const int p = c1[s] - 1, q = c2[s] > 0 ? c2[s] - 1 : -1;
     if (p < 0 || p >= n || q >= n) return fail(TFG_EINVAL, "face %d: cells out of range", s + 1);
@@ -850,8 +862,12 @@ int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2, c
       if (j1 <= row[p] || j1 >= row[p + 1] || col[j1] != q || j2 <= row[q] || j2 >= row[q + 1] ||
           col[j2] != p)
         return fail(TFG_EINVAL, "face %d: pos does not point at (c1,c2) / (c2,c1)", s + 1);
+      if (ent[j1] >= 0 || ent[j2] >= 0) shared_entry = true;
       ent[j1] = s;   // the last face that sets the entry wins, as in the loop
       ent[j2] = s;
+    } else if (c2[s] < 0) {
+      if (bnd_seen[-c2[s]]) shared_bnd = true;
+      bnd_seen[-c2[s]] = 1;
     }
   }
   for (int c = 0; c < n; ++c) cnt[c + 1] += cnt[c];
@@ -874,11 +890,91 @@ int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2, c
   up((void**)&f->c1, a.data(), sizeof(int) * a.size());
   up((void**)&f->c2, b.data(), sizeof(int) * b.size());
   up((void**)&f->flags, fl.data(), fl.size());
+  up((void**)&f->c2b, c2, sizeof(int) * (size_t)n_faces);
+  f->max_bnd = max_bnd;
+  f->shared_entry = shared_entry;
+  f->shared_bnd = shared_bnd;
   if (e != hipSuccess) {
     faces_free(f);
     return fail(TFG_ENOMEM, "face structure upload: %s", hipGetErrorString(e));
   }
   *faces = f;
+  return TFG_OK;
+}
+
+int tfg_momentum_assemble(tfg_faces* f, const tfg_momentum* m, double* val, double* b,
+                          double* v_m) {
+  if (!f || !m || !val || !b || !v_m) return fail(TFG_EINVAL, "null argument");
+  const double* need[] = {m->f, m->fw, m->fc, m->v_flux, m->vis_eff, m->sx, m->sy, m->sz, m->dx,
+                          m->dy, m->dz, m->phi_n, m->phi_o, m->phi_oo, m->gx, m->gy, m->gz,
+                          m->uj_i, m->uk_i, m->density, m->vol, m->p_i};
+  for (const double* p : need)
+    if (!p) return fail(TFG_EINVAL, "null momentum array");
+  if (!m->wall) return fail(TFG_EINVAL, "null momentum array");
+  if (m->component < 1 || m->component > 3) return fail(TFG_EINVAL, "component %d", m->component);
+  if (m->adv_scheme < TFG_ADV_CENTRAL || m->adv_scheme > TFG_ADV_AVL_SMART)
+    return fail(TFG_ENOTSUP, "advection scheme %d", m->adv_scheme);
+  if (m->td_scheme < TFG_TD_LINEAR || m->td_scheme > TFG_TD_NONE)
+    return fail(TFG_EINVAL, "time scheme %d", m->td_scheme);
+  if (m->n_bnd < f->max_bnd)
+    return fail(TFG_EINVAL, "n_bnd %d below the faces' boundary cell %d", m->n_bnd, f->max_bnd);
+  if (f->shared_entry || f->shared_bnd)
+    return fail(TFG_ENOTSUP, "a matrix entry or a boundary cell belongs to two faces");
+  tfg_matrix* h = f->h;
+  HIP_TRY(hipSetDevice(h->device));
+  if (m->adv_scheme != TFG_ADV_CENTRAL && !f->pmin) {
+    HIP_TRY(hipMalloc(&f->pmin, sizeof(double) * (size_t)std::max(1, h->nt)));
+    HIP_TRY(hipMalloc(&f->pmax, sizeof(double) * (size_t)std::max(1, h->nt)));
+  }
+  tfg::MomArgs g{};
+  g.n = h->nt;
+  g.nb = m->n_bnd;
+  g.row = h->row;
+  g.cf_off = f->cf_off;
+  g.cf_face = f->cf_face;
+  g.ent_face = f->ent_face;
+  g.c1 = f->c1;
+  g.c2b = f->c2b;
+  g.comp = m->component;
+  g.adv = m->adv_scheme;
+  g.blend_matrix = m->blend_matrix ? 1 : 0;
+  g.td = m->td_scheme;
+  g.blend = m->blend;
+  g.dt = m->dt;
+  g.urf = m->urf;
+  g.p_drop = m->p_drop;
+  g.f = m->f;
+  g.fw = m->fw;
+  g.fc = m->fc;
+  g.v_flux = m->v_flux;
+  g.vis_eff = m->vis_eff;
+  g.f_stress = m->f_stress;
+  g.sv[0] = m->sx;
+  g.sv[1] = m->sy;
+  g.sv[2] = m->sz;
+  g.dv[0] = m->dx;
+  g.dv[1] = m->dy;
+  g.dv[2] = m->dz;
+  g.wall = m->wall;
+  g.phi = m->phi_n;
+  g.phi_o = m->phi_o;
+  g.phi_oo = m->phi_oo;
+  g.g[0] = m->gx;
+  g.g[1] = m->gy;
+  g.g[2] = m->gz;
+  g.uj_i = m->uj_i;
+  g.uk_i = m->uk_i;
+  g.rho = m->density;
+  g.vol = m->vol;
+  g.p_i = m->p_i;
+  g.cell_f = m->cell_f;
+  g.st = m->st;
+  g.pmin = f->pmin;
+  g.pmax = f->pmax;
+  g.val = val;
+  g.b = b;
+  g.v_m = v_m;
+  HIP_TRY(tfg::launch_momentum(h->stream, g));
   return TFG_OK;
 }
 

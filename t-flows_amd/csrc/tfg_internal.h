@@ -192,6 +192,28 @@ struct AsmArgs {
   int singular;
 };
 
+// Momentum assembly (tfg_assembly.hip).  Cell arrays point at cell 1 (a
+// 1-based cell C is element C - 1; boundary cells C < 0 below it).
+struct MomArgs {
+  int n, nb;
+  const int* row;
+  const int* cf_off;
+  const int* cf_face;
+  const int* ent_face;
+  const int* c1;         // 0-based
+  const int* c2b;        // 1-based as given (boundary cell < 0)
+  int comp, adv, blend_matrix, td;
+  double blend, dt, urf, p_drop;
+  const double *f, *fw, *fc, *v_flux, *vis_eff, *f_stress;
+  const double *sv[3], *dv[3];
+  const int* wall;
+  const double *phi, *phi_o, *phi_oo, *g[3], *uj_i, *uk_i, *rho, *vol, *p_i, *cell_f, *st;
+  double* pmin;          // n scratch values: min / max of phi over each cell's faces
+  double* pmax;
+  double *val, *b, *v_m;
+};
+hipError_t launch_momentum(hipStream_t stream, const MomArgs& m);
+
 // Launch shapes chosen per matrix.
 struct Launch {
   hipStream_t stream;

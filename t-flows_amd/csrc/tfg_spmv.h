@@ -339,7 +339,7 @@ struct PatternEngine {
       // block's rows take its last row, whose start is e - L)
       const int L = sm.len[p];
       const int pre = block_exclusive_scan(tid < nr ? L : 0, sm.wave);
-      const int rs = tid < nr ? s + pre : e - L, re = rs + L;
+      const int rs = tid < nr ? s + pre : e - L;
       const int i = r0 + min(tid, nr - 1);
       const int ov = s & ~(EPV - 1);
       const int* po = sm.off + p * kPatMax;

@@ -13,6 +13,19 @@
   implicit none
 !==============================================================================!
 
+  ! tfg_momentum (tflows_gpu.h): one velocity component's momentum assembly
+  ! (Compute_Momentum.f90:119-366).  Device addresses; cell arrays at cell 1,
+  ! e.g. c_loc(ui % n(1)) for the (-n_bnd:n) arrays phi_n, gx, gy, gz, uj_i,
+  ! uk_i and density.
+  type, bind(C) :: Tflows_Gpu_Momentum
+    integer(c_int) :: component, n_bnd, adv_scheme, blend_matrix, td_scheme
+    real(c_double) :: blend, dt, urf, p_drop
+    type(c_ptr)    :: f, fw, fc, v_flux, vis_eff, f_stress
+    type(c_ptr)    :: sx, sy, sz, dx, dy, dz, wall
+    type(c_ptr)    :: phi_n, phi_o, phi_oo, gx, gy, gz, uj_i, uk_i, density,  &
+                      vol, p_i, cell_f, st
+  end type
+
   interface
 
     integer(c_int) function tfg_matrix_create(ni, nt, nnz, row, col, dia,  &
@@ -82,6 +95,13 @@
       import :: c_int, c_ptr
       type(c_ptr),    value :: faces, fc, fw, vm, vflux, val, b
       integer(c_int), value :: singular
+    end function
+
+    integer(c_int) function tfg_momentum_assemble(faces, m, val, b, v_m)    &
+                                         bind(C, name='tfg_momentum_assemble')
+      import :: c_int, c_ptr, Tflows_Gpu_Momentum
+      type(c_ptr),               value      :: faces, val, b, v_m
+      type(Tflows_Gpu_Momentum), intent(in) :: m
     end function
 
   end interface

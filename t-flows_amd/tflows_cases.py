@@ -214,3 +214,75 @@ def hex_faces(nx, ny=None, nz=None, seed=DEFAULT_SEED, shuffle=True):
                  b_zero=np.ascontiguousarray(flag[order], dtype=np.int32),
                  fc=h * rng.uniform(0.5, 2.0, nf), fw=rng.uniform(0.3, 0.7, nf),
                  vm=1e-3 * rng.uniform(0.5, 2.0, n), vflux=rng.uniform(-1.0, 1.0, nf))
+
+
+@dataclass
+class MomentumFields:
+    """Face and cell data of one momentum assembly on a Faces set, in the
+    reference's layout.  Per face (n_faces): interpolation factors f
+    (Grid % f) and fw (Grid % fw), M % fc, v_flux % n, face viscosity, face
+    stress, the face vectors s (Grid % sx, sy, sz) and d (dx, dy, dz), and
+    `wall` = 1 on boundary faces with an INFLOW / WALL / CONVECT / WALLFL
+    condition.  Per cell: arrays marked ext hold the boundary cells as well,
+    laid out as Fortran's (-n_bnd:n) so that cell c is element c + n_bnd
+    (element n_bnd, cell 0, is unused)."""
+    n_bnd: int
+    f: np.ndarray
+    fw: np.ndarray
+    fc: np.ndarray
+    v_flux: np.ndarray
+    vis_eff: np.ndarray
+    f_stress: np.ndarray
+    sx: np.ndarray
+    sy: np.ndarray
+    sz: np.ndarray
+    dx: np.ndarray
+    dy: np.ndarray
+    dz: np.ndarray
+    wall: np.ndarray
+    phi_n: np.ndarray    # ext
+    gx: np.ndarray       # ext
+    gy: np.ndarray       # ext
+    gz: np.ndarray       # ext
+    uj_i: np.ndarray     # ext
+    uk_i: np.ndarray     # ext
+    density: np.ndarray  # ext
+    phi_o: np.ndarray
+    phi_oo: np.ndarray
+    vol: np.ndarray
+    p_i: np.ndarray
+    cell_f: np.ndarray
+    st: np.ndarray
+
+
+def momentum_fields(faces, seed=DEFAULT_SEED, rho_ratio=1000.0):
+    """Seeded momentum data on a Faces set (hex_faces): velocities U(-1, 1),
+    gradients U(-1, 1) / h, density 1 or rho_ratio per cell (a random half of
+    the boundary cells included), viscosity-like face data ~ 1e-3, time-step
+    and force terms of unit size."""
+    rng = np.random.default_rng(seed + 17)
+    nf = faces.n_faces
+    n = faces.n
+    nb = int(-faces.c2.min()) if faces.c2.min() < 0 else 0
+    h = faces.fc.mean()
+
+    def ext(scale=1.0):
+        a = scale * rng.uniform(-1.0, 1.0, nb + 1 + n)
+        a[nb] = 0.0
+        return a
+
+    dens = np.where(rng.uniform(0, 1, nb + 1 + n) < 0.5, 1.0, rho_ratio)
+    dens[nb] = 0.0
+    return MomentumFields(
+        n_bnd=nb, f=rng.uniform(0.3, 0.7, nf), fw=rng.uniform(0.3, 0.7, nf),
+        fc=faces.fc.copy(), v_flux=rng.uniform(-1.0, 1.0, nf) * h * h,
+        vis_eff=1e-3 * rng.uniform(0.5, 2.0, nf), f_stress=1e-4 * rng.uniform(-1.0, 1.0, nf),
+        sx=h * h * rng.uniform(-1.0, 1.0, nf), sy=h * h * rng.uniform(-1.0, 1.0, nf),
+        sz=h * h * rng.uniform(-1.0, 1.0, nf), dx=h * rng.uniform(-1.0, 1.0, nf),
+        dy=h * rng.uniform(-1.0, 1.0, nf), dz=h * rng.uniform(-1.0, 1.0, nf),
+        wall=np.ascontiguousarray((faces.c2 < 0) & (rng.uniform(0, 1, nf) < 0.7), dtype=np.int32),
+        phi_n=ext(), gx=ext(1.0 / h), gy=ext(1.0 / h), gz=ext(1.0 / h), uj_i=ext(1.0 / h),
+        uk_i=ext(1.0 / h), density=dens, phi_o=rng.uniform(-1.0, 1.0, n),
+        phi_oo=rng.uniform(-1.0, 1.0, n), vol=h ** 3 * rng.uniform(0.5, 2.0, n),
+        p_i=rng.uniform(-1.0, 1.0, n) / h, cell_f=rng.uniform(-1.0, 1.0, n),
+        st=rng.uniform(-1.0, 1.0, n))

@@ -91,6 +91,7 @@ def lib():
         L.tfg_faces_create.argtypes = [P, I, P, P, P, P, P, ctypes.POINTER(P)]
         L.tfg_faces_destroy.argtypes = [P]
         L.tfg_pressure_assemble.argtypes = [P, P, P, P, P, P, P, I]
+        L.tfg_momentum_assemble.argtypes = [P, P, P, P, P]
         L.tfg_batch_create.argtypes = [I, P, P, P, P, P, ctypes.POINTER(P)]
         L.tfg_batch_destroy.argtypes = [P]
         L.tfg_batch_solve.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I, P,
@@ -380,6 +381,12 @@ class Faces:
                                            dval.ptr, db.ptr, 1 if singular else 0))
         _check(lib().tfg_synchronize(self.nat.h))
 
+    def momentum(self, m, dval, db, dvm):
+        """Momentum matrix, right-hand side and v_m of one component on the
+        device (tfg_momentum_assemble); m is a Momentum; waits for the result."""
+        _check(lib().tfg_momentum_assemble(self.h, ctypes.byref(m.s), dval.ptr, db.ptr, dvm.ptr))
+        _check(lib().tfg_synchronize(self.nat.h))
+
     def close(self):
         if self.h:
             lib().tfg_faces_destroy(self.h)
@@ -390,3 +397,51 @@ class Faces:
             self.close()
         except Exception:
             pass
+
+
+ADV = {"central": 0, "upwind": 1, "luds": 2, "quick": 3, "smart": 4, "gamma": 5,
+       "minmod": 6, "blended": 7, "superbee": 8, "avl_smart": 9}
+TD = {"linear": 0, "parabolic": 1, "none": 2}
+
+
+class _MomentumS(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_int) for k in ("component", "n_bnd", "adv_scheme", "blend_matrix",
+                                              "td_scheme")] +
+                [(k, ctypes.c_double) for k in ("blend", "dt", "urf", "p_drop")] +
+                [(k, ctypes.c_void_p) for k in ("f", "fw", "fc", "v_flux", "vis_eff", "f_stress",
+                                                "sx", "sy", "sz", "dx", "dy", "dz", "wall",
+                                                "phi_n", "phi_o", "phi_oo", "gx", "gy", "gz",
+                                                "uj_i", "uk_i", "density", "vol", "p_i",
+                                                "cell_f", "st")])
+
+
+class Momentum:
+    """tfg_momentum on device copies of a set of momentum fields (the
+    attributes of tflows_cases.MomentumFields; the boundary-extended cell
+    arrays are laid out (-n_bnd:n) and passed at cell 1)."""
+
+    EXT = ("phi_n", "gx", "gy", "gz", "uj_i", "uk_i", "density")
+    FACE = ("f", "fw", "fc", "v_flux", "vis_eff", "sx", "sy", "sz", "dx", "dy", "dz")
+    CELL = ("phi_o", "phi_oo", "vol", "p_i")
+
+    def __init__(self, mf, component=1, adv="central", blend_matrix=True, td="linear", blend=0.8,
+                 dt=1e-2, urf=0.7, p_drop=0.5, stress=True, cell_force=True,
+                 surface_tension=False):
+        s = self.s = _MomentumS()
+        self.dev = {}
+        s.component, s.n_bnd, s.adv_scheme = component, mf.n_bnd, ADV[adv]
+        s.blend_matrix, s.td_scheme = int(blend_matrix), TD[td]
+        s.blend, s.dt, s.urf, s.p_drop = blend, dt, urf, p_drop
+        for k in self.FACE + self.CELL:
+            setattr(s, k, self._up(k, getattr(mf, k)))
+        s.wall = self._up("wall", np.ascontiguousarray(mf.wall, dtype=np.int32))
+        for k in self.EXT:
+            setattr(s, k, self._up(k, getattr(mf, k), (mf.n_bnd + 1) * 8))
+        s.f_stress = self._up("f_stress", mf.f_stress) if stress else None
+        s.cell_f = self._up("cell_f", mf.cell_f) if cell_force else None
+        s.st = self._up("st", mf.st) if surface_tension else None
+
+    def _up(self, key, a, offset=0):
+        d = DeviceArray(np.ascontiguousarray(a))
+        self.dev[key] = d
+        return d.ptr.value + offset

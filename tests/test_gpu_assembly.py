@@ -52,3 +52,107 @@ def test_faces_validation(cases, gpu):
     with pytest.raises(gpu.TFlowsGpuError) as e:
         gpu.Faces(nat, f.c1, f.c2, pos, f.diag_bc, f.b_zero)
     assert e.value.code == gpu.TFG_EINVAL and "pos" in str(e.value)
+
+
+# ---- momentum (tfg_momentum_assemble) ------------------------------------
+
+def _momentum_case(cases, dims, seed=5):
+    f = cases.hex_faces(*dims, seed=seed)
+    return f, cases.momentum_fields(f, seed=seed)
+
+
+def _momentum_gpu(gpu, f, mf, **kw):
+    nat = gpu.Native(f.row, f.col, f.dia)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    m = gpu.Momentum(mf, **kw)
+    dval, db, dvm = (gpu.DeviceArray(np.full(k, np.nan)) for k in (len(f.col), f.n, f.n))
+    faces.momentum(m, dval, db, dvm)
+    return nat, dval.download(), db.download(), dvm.download()
+
+
+MOMENTUM_CASES = [
+    # component, scheme, blend_matrix, time scheme, stress, cell force, surface tension
+    (1, "central", True, "linear", True, True, False),
+    (2, "upwind", True, "parabolic", False, False, True),
+    (3, "luds", False, "linear", True, True, True),
+    (1, "quick", True, "none", True, False, False),
+    (2, "smart", True, "linear", True, True, False),
+    (3, "gamma", True, "parabolic", True, True, False),
+    (1, "minmod", False, "linear", False, True, False),
+    (2, "blended", True, "linear", True, True, False),
+    (3, "superbee", True, "parabolic", True, False, True),
+    (1, "avl_smart", True, "linear", True, True, False),
+]
+
+
+@pytest.mark.parametrize("comp,adv,blend,td,stress,cf,st", MOMENTUM_CASES,
+                         ids=[f"{c[1]}-u{c[0]}" for c in MOMENTUM_CASES])
+def test_momentum_bitexact(cases, oracle, gpu, comp, adv, blend, td, stress, cf, st):
+    """val, b and v_m of every scheme, time scheme and optional term equal
+    the serial loops bit for bit (faces in a shuffled order)."""
+    f, mf = _momentum_case(cases, (11, 9, 7))
+    kw = dict(component=comp, adv=adv, blend_matrix=blend, td=td, stress=stress,
+              cell_force=cf, surface_tension=st)
+    print(f"momentum case {kw} n={f.n} faces={f.n_faces}", flush=True)
+    val_o, b_o, vm_o = oracle.momentum_assemble(f, mf, **kw)
+    _, val, b, vm = _momentum_gpu(gpu, f, mf, **kw)
+    assert np.array_equal(val, val_o), np.abs(val - val_o).max()
+    assert np.array_equal(b, b_o), np.abs(b - b_o).max()
+    assert np.array_equal(vm, vm_o), np.abs(vm - vm_o).max()
+
+
+def test_momentum_boundary_faces_numbered_first(cases, oracle, gpu):
+    """T-Flows numbers the boundary faces first: the advection loops'
+    boundary-then-domain order is then plain face order."""
+    f = cases.hex_faces(9, 8, 7, shuffle=False)
+    order = np.argsort(f.c2 > 0, kind="stable")
+    pos = f.pos.reshape(-1, 2)[order].ravel()
+    g = type(f)(**{**f.__dict__, "c1": f.c1[order], "c2": f.c2[order], "pos": pos,
+                   "diag_bc": f.diag_bc[order], "b_zero": f.b_zero[order], "fc": f.fc[order],
+                   "fw": f.fw[order], "vflux": f.vflux[order]})
+    mf = cases.momentum_fields(g)
+    kw = dict(adv="superbee", td="parabolic")
+    val_o, b_o, vm_o = oracle.momentum_assemble(g, mf, **kw)
+    _, val, b, vm = _momentum_gpu(gpu, g, mf, **kw)
+    assert np.array_equal(val, val_o) and np.array_equal(b, b_o) and np.array_equal(vm, vm_o)
+
+
+def test_momentum_system_solves(cases, oracle, gpu):
+    """The device-assembled momentum system (non-symmetric with the implicit
+    upwind part) solved on the device with BiCGSTAB, against the oracle's
+    BiCGSTAB on the host-assembled one."""
+    f, mf = _momentum_case(cases, (14, 12, 10))
+    kw = dict(adv="minmod", td="linear")
+    val_o, b_o, _ = oracle.momentum_assemble(f, mf, **kw)
+    nat, val, b, _ = _momentum_gpu(gpu, f, mf, **kw)
+    x = np.zeros(f.n)
+    it, res = nat.solve_native("bcgs", "jacobi", val.copy(), x, b.copy(), 200, 1e-10, norm=1.0)
+    A = oracle.Matrix(f.row, f.col, f.dia, val_o.copy())
+    xo = np.zeros(f.n)
+    it_o, res_o = oracle.solve("bcgs", "jacobi", A, xo, b_o.copy(), 200, 1e-10, norm=1.0)
+    print(f"momentum solve: gpu {it} its res {res:.3e}; oracle {it_o} its res {res_o:.3e}",
+          flush=True)
+    assert res < 1e-10 and abs(it - it_o) <= 2
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_momentum_validation(cases, gpu):
+    f, mf = _momentum_case(cases, (4, 3, 3))
+    nat = gpu.Native(f.row, f.col, f.dia)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    dval, db, dvm = (gpu.DeviceArray(np.zeros(k)) for k in (len(f.col), f.n, f.n))
+    m = gpu.Momentum(mf)
+    m.s.n_bnd = mf.n_bnd - 1                     # below the largest boundary cell
+    with pytest.raises(gpu.TFlowsGpuError) as e:
+        faces.momentum(m, dval, db, dvm)
+    assert e.value.code == gpu.TFG_EINVAL
+    m = gpu.Momentum(mf)
+    m.s.adv_scheme = 10                          # not a face-value scheme
+    with pytest.raises(gpu.TFlowsGpuError) as e:
+        faces.momentum(m, dval, db, dvm)
+    assert e.value.code == gpu.TFG_ENOTSUP
+    m = gpu.Momentum(mf)
+    m.s.vol = None
+    with pytest.raises(gpu.TFlowsGpuError) as e:
+        faces.momentum(m, dval, db, dvm)
+    assert e.value.code == gpu.TFG_EINVAL
