@@ -429,20 +429,25 @@ def bench_assembly(args):
     import tflows_gpu as G
 
     N = args.n
-    t0 = time.perf_counter()
-    f = tflows_cases.hex_faces(N)
-    log(f"bench assembly: {N}^3, {f.n_faces} faces, generated in {time.perf_counter() - t0:.1f} s")
-    nat = G.Native(f.row, f.col, f.dia)
-    faces = G.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
-    d = {k: G.DeviceArray(getattr(f, k)) for k in ("fc", "fw", "vm", "vflux")}
-    dval, db = G.DeviceArray(np.zeros(len(f.col))), G.DeviceArray(np.zeros(f.n))
-    hval, hb = np.zeros(len(f.col)), np.zeros(f.n)
-    faces.assemble(d["fc"], d["fw"], d["vm"], d["vflux"], dval, db, singular=True)
     reps = max(3, args.steps)
-    t = time.perf_counter()
-    for _ in range(reps):
+    timings = {}
+    for order in ("shuffled", "natural"):
+        t0 = time.perf_counter()
+        f = tflows_cases.hex_faces(N, shuffle=order == "shuffled")
+        log(f"bench assembly: {N}^3, {f.n_faces} faces ({order} order), generated in "
+            f"{time.perf_counter() - t0:.1f} s")
+        nat = G.Native(f.row, f.col, f.dia)
+        faces = G.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+        d = {k: G.DeviceArray(getattr(f, k)) for k in ("fc", "fw", "vm", "vflux")}
+        dval, db = G.DeviceArray(np.zeros(len(f.col))), G.DeviceArray(np.zeros(f.n))
         faces.assemble(d["fc"], d["fw"], d["vm"], d["vflux"], dval, db, singular=True)
-    t_asm = (time.perf_counter() - t) / reps
+        t = time.perf_counter()
+        for _ in range(reps):
+            faces.assemble(d["fc"], d["fw"], d["vm"], d["vflux"], dval, db, singular=True)
+        timings[order] = (time.perf_counter() - t) / reps
+        log(f"bench assembly: {order} face order: {timings[order] * 1e3:.2f} ms")
+    t_asm = timings["natural"]
+    hval, hb = np.zeros(len(f.col)), np.zeros(f.n)
     hval[:] = dval.download()
     hb[:] = db.download()
     t = time.perf_counter()
@@ -457,8 +462,9 @@ def bench_assembly(args):
         "metric": "pressure-system assembly on the device vs upload of A % val and b, 200^3",
         "value": t_asm * 1e3, "unit": "ms", "n_gpus": 1, "steps": reps, "higher_is_better": False,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"hex {N}^3 face data, pressure matrix + rhs", "cells": f.n,
-                   "faces": f.n_faces, "nnz": len(f.col)},
+        "config": {"workload": f"hex {N}^3 face data, pressure matrix + rhs, faces in grid order",
+                   "cells": f.n, "faces": f.n_faces, "nnz": len(f.col)},
+        "shuffled_faces_ms": timings["shuffled"] * 1e3,
         "upload_ms": t_up * 1e3, "upload_bytes": bytes_up, "speedup_vs_upload": t_up / t_asm,
     }), flush=True)
 
