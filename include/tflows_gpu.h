@@ -166,12 +166,32 @@ int tfg_spmv_select(tfg_matrix* h, int kind);
  * its grid, the number of 2040-nonzero row blocks, the number of distinct
  * col - row offsets when engine 6 is available (else 0), of distinct row
  * patterns when engine 7 is (else 0), of upper diagonals when engine 8
- * is (else 0), and whether the last val engine 8 was set up for was
- * symmetric (1), not (0: that call used engine 7) or none was (-1).  Any
- * pointer may be NULL. */
+ * is (else 0), whether the last val engine 8 was set up for was
+ * symmetric (1), not (0: that call used engine 7) or none was (-1), and the
+ * internal ordering (TFG_ORDER_*; the engine figures are then those of the
+ * renumbered matrix).  Any pointer may be NULL. */
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind,
                     int* spmv_grid, int* stage_blocks, int* offsets, int* patterns,
-                    int* sym_diagonals, int* sym_state);
+                    int* sym_diagonals, int* sym_state, int* ordering);
+
+/* ---- internal ordering ----------------------------------------------------
+ * No counterpart in the reference.  A matrix whose numbering has no locality
+ * (randomly numbered cells, as a permuted kNN or a badly ordered unstructured
+ * grid) makes every gather of the SpMV a cache miss.  TFG_ORDER_RCM
+ * renumbers rows and columns internally by reverse Cuthill-McKee (computed
+ * once, on the host); every call then permutes val, b and x in and x (or
+ * the product) out, so callers keep their own numbering.  Each row keeps
+ * its entries in the caller's order, so products are bit-identical; only
+ * the order of the global dot products changes, as between SpMV engines.
+ * tfg_matrix_create applies it by itself to a matrix of 65536 rows or more
+ * without row patterns whose numbering has no locality (mean |col - row| of
+ * the off-diagonals above nt/64).  "icc" (an ordering-dependent
+ * factorisation) and face assembly always run in the caller's order;
+ * communication hooks drop the ordering; not with buffer cells
+ * (TFG_ENOTSUP).  TFG_ORDER_NATURAL drops it. */
+#define TFG_ORDER_NATURAL 0
+#define TFG_ORDER_RCM 1
+int tfg_matrix_reorder(tfg_matrix* h, int order);
 
 /* ---- pressure-system assembly on the device ------------------------------
  * Builds A % val and b of the pressure-correction system from face data on
@@ -189,6 +209,8 @@ typedef struct tfg_faces tfg_faces;
 int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2,
                      const int* pos, const int* diag_bc, const int* b_zero,
                      tfg_faces** faces);
+/* The faces handle uses its matrix handle: assemble only while both exist.
+ * Destroying it needs only itself, before or after the matrix handle. */
 int tfg_faces_destroy(tfg_faces* faces);
 /* Device pointers: fc = A % fc (n_faces), fw = Grid % f (n_faces), vm =
  * M % v_m (nt), vflux = v_flux % n (n_faces); val (nnz) and b (nt) out.

@@ -159,6 +159,12 @@ struct tfg_matrix {
   int* sym_flag = nullptr;
   int* hflag = nullptr;
   int sym_state = -1;  // last val: 1 symmetric, 0 not, -1 none checked yet
+  // internal ordering (tfg_matrix_reorder): the handle of the renumbered
+  // matrix, new <- old maps of values (vmap) and rows (iperm), old <- new of
+  // rows (perm), and permuted copies of val, b, x and an output vector
+  tfg_matrix* inner = nullptr;
+  int *vmap = nullptr, *perm = nullptr, *iperm = nullptr;
+  double *pval = nullptr, *pb = nullptr, *px = nullptr, *pq = nullptr;
 };
 
 namespace {
@@ -319,12 +325,17 @@ void free_all(tfg_matrix* h) {
     if (d) (void)hipFree(d);
   if (h->hst) (void)hipHostFree(h->hst);
   if (h->hflag) (void)hipHostFree(h->hflag);
+  if (h->inner) free_all(h->inner);
+  for (void* d : {(void*)h->vmap, (void*)h->perm, (void*)h->iperm, (void*)h->pval,
+                  (void*)h->pb, (void*)h->px, (void*)h->pq})
+    if (d) (void)hipFree(d);
   if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);
   for (hipEvent_t e : {h->ev_poll[0], h->ev_poll[1], h->ev_t0, h->ev_t1})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->ev_sample)
     if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
+  (void)hipGetLastError();  // release errors are dropped, not left for later calls
   delete h;
 }
 
@@ -809,8 +820,13 @@ int solve_comm(tfg_matrix* h, int prec, const double* val, const double* b, int 
 // ===========================================================================
 extern "C" {
 
+// set while tfg_matrix_reorder creates its inner handle (no automatic
+// ordering of that one)
+static thread_local bool g_inner_create = false;
+
 struct tfg_faces {
   tfg_matrix* h = nullptr;
+  int device = 0;            // of h, kept so that destroying needs no h
   int n_faces = 0;
   int *cf_off = nullptr, *cf_face = nullptr, *ent_face = nullptr, *c1 = nullptr, *c2 = nullptr;
   uint8_t* flags = nullptr;
@@ -821,12 +837,17 @@ struct tfg_faces {
   double *pmin = nullptr, *pmax = nullptr;  // momentum scratch (nt each), on first use
 };
 
+// hipFree waits for the device, so no stream of the matrix handle (which
+// may be gone already) is needed; errors of the release are dropped here
+// and cleared, so that no later call reports them as its own
 static void faces_free(tfg_faces* f) {
   if (!f) return;
+  (void)hipSetDevice(f->device);
   void* dev[] = {f->cf_off, f->cf_face, f->ent_face, f->c1, f->c2, f->flags, f->c2b, f->pmin,
                  f->pmax};
   for (void* d : dev)
     if (d) (void)hipFree(d);
+  (void)hipGetLastError();
   delete f;
 }
 
@@ -878,6 +899,7 @@ int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2, c
   }
   tfg_faces* f = new tfg_faces();
   f->h = h;
+  f->device = h->device;
   f->n_faces = n_faces;
   hipError_t e = hipSuccess;
   auto up = [&](void** d, const void* src, size_t bytes) {
@@ -980,8 +1002,6 @@ int tfg_momentum_assemble(tfg_faces* f, const tfg_momentum* m, double* val, doub
 
 int tfg_faces_destroy(tfg_faces* f) {
   if (!f) return fail(TFG_EINVAL, "null handle");
-  (void)hipSetDevice(f->h->device);
-  (void)hipStreamSynchronize(f->h->stream);
   faces_free(f);
   return TFG_OK;
 }
@@ -1016,6 +1036,10 @@ int tfg_matrix_set_comm(tfg_matrix* h, tfg_allreduce_fn allreduce, tfg_exchange_
   if (!h) return fail(TFG_EINVAL, "null handle");
   if ((allreduce == nullptr) != (exchange == nullptr))
     return fail(TFG_EINVAL, "allreduce and exchange hooks go together");
+  if (allreduce && h->inner) {   // the multi-rank path runs in the caller's order
+    const int rc = tfg_matrix_reorder(h, TFG_ORDER_NATURAL);
+    if (rc != TFG_OK) return rc;
+  }
   h->allreduce = allreduce;
   h->exchange = exchange;
   h->comm_ctx = ctx;
@@ -1260,12 +1284,179 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
       const int k = atoi(force);
       if (kind_available(h, k)) h->kind = k;  // else the default choice
     }
+    // A large matrix without row patterns whose numbering has no locality
+    // (mean |col - row| of the off-diagonals above nt/64; a random numbering
+    // gives about nt/3) gets the internal RCM ordering.
+    if (rc == TFG_OK && !g_inner_create && h->npat == 0 && ni == nt && nt >= (1 << 16)) {
+      double dist = 0.0;
+      long long cnt = 0;
+      for (int i = 0; i < ni; ++i)
+        for (int j = row[i] - 1; j < row[i + 1] - 1; ++j)
+          if (col[j] - 1 != i) {
+            dist += std::abs(col[j] - 1 - i);
+            ++cnt;
+          }
+      if (cnt > 0 && dist / (double)cnt > nt / 64.0) (void)tfg_matrix_reorder(h, TFG_ORDER_RCM);
+    }
   } while (0);
   if (rc != TFG_OK) {
     free_all(h);
     return rc;
   }
   *handle = h;
+  return TFG_OK;
+}
+
+namespace {
+
+// Reverse Cuthill-McKee order of the graph of rows (0-based CSR): for each
+// connected part, a breadth-first sweep from a pseudo-peripheral row
+// (George and Liu: repeat the sweep from a least-degree row of the last
+// level while the depth grows), neighbours in increasing degree; the whole
+// order reversed.  Returns new -> old.
+std::vector<int> rcm_order(int n, const std::vector<int>& row, const std::vector<int>& col) {
+  std::vector<int> deg(n), order, stamp(n, -1), level(n, 0);
+  for (int i = 0; i < n; ++i) deg[i] = row[i + 1] - row[i];
+  std::vector<int> by_deg(n);
+  for (int i = 0; i < n; ++i) by_deg[i] = i;
+  std::stable_sort(by_deg.begin(), by_deg.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+  std::vector<char> done(n, 0);
+  order.reserve(n);
+  std::vector<int> queue, nb;
+  queue.reserve(n);
+  int sweep = 0;
+  // breadth-first levels from s over rows not yet ordered; returns depth and
+  // a least-degree row of the last level
+  auto levels = [&](int s, int& last) -> int {
+    ++sweep;
+    queue.clear();
+    queue.push_back(s);
+    stamp[s] = sweep;
+    level[s] = 0;
+    int depth = 0;
+    for (size_t h = 0; h < queue.size(); ++h) {
+      const int u = queue[h];
+      for (int k = row[u]; k < row[u + 1]; ++k) {
+        const int v = col[k];
+        if (v == u || done[v] || stamp[v] == sweep) continue;
+        stamp[v] = sweep;
+        level[v] = level[u] + 1;
+        depth = std::max(depth, level[v]);
+        queue.push_back(v);
+      }
+    }
+    last = -1;
+    for (int u : queue)
+      if (level[u] == depth && (last < 0 || deg[u] < deg[last])) last = u;
+    return depth;
+  };
+  size_t cand = 0;
+  while ((int)order.size() < n) {
+    while (done[by_deg[cand]]) ++cand;
+    int s = by_deg[cand], last = -1;
+    int depth = levels(s, last);
+    for (int it = 0; it < 4; ++it) {
+      int last2 = -1;
+      const int d2 = levels(last, last2);
+      if (d2 <= depth) break;
+      s = last;
+      depth = d2;
+      last = last2;
+    }
+    const size_t start = order.size();
+    order.push_back(s);
+    done[s] = 1;
+    for (size_t h = start; h < order.size(); ++h) {
+      const int u = order[h];
+      nb.clear();
+      for (int k = row[u]; k < row[u + 1]; ++k)
+        if (!done[col[k]]) {
+          done[col[k]] = 1;
+          nb.push_back(col[k]);
+        }
+      std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+      order.insert(order.end(), nb.begin(), nb.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+bool is_icc(const char* prec, int prec_len) { return fstring(prec, prec_len) == "icc"; }
+
+// permuted copies of a value array (by vmap) and vectors (by iperm) on the
+// inner handle's stream; back by perm
+hipError_t to_inner_val(tfg_matrix* h, const double* val) {
+  return tfg::launch_gather(h->inner->stream, val, h->vmap, h->pval, h->nnz);
+}
+hipError_t to_inner_vec(tfg_matrix* h, const double* v, double* dst) {
+  return tfg::launch_gather(h->inner->stream, v, h->iperm, dst, h->nt);
+}
+hipError_t from_inner_vec(tfg_matrix* h, const double* src, double* v) {
+  return tfg::launch_gather(h->inner->stream, src, h->perm, v, h->nt);
+}
+
+}  // namespace
+
+int tfg_matrix_reorder(tfg_matrix* h, int order) {
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  if (order != TFG_ORDER_NATURAL && order != TFG_ORDER_RCM)
+    return fail(TFG_EINVAL, "unknown ordering %d", order);
+  HIP_TRY(hipSetDevice(h->device));
+  if (h->inner) {
+    free_all(h->inner);
+    h->inner = nullptr;
+    for (void** d : {(void**)&h->vmap, (void**)&h->perm, (void**)&h->iperm, (void**)&h->pval,
+                     (void**)&h->pb, (void**)&h->px, (void**)&h->pq}) {
+      if (*d) HIP_TRY(hipFree(*d));
+      *d = nullptr;
+    }
+  }
+  if (order == TFG_ORDER_NATURAL) return TFG_OK;
+  if (h->ni < h->nt || h->allreduce)
+    return fail(TFG_ENOTSUP, "internal ordering with buffer cells or communication hooks");
+  const int n = h->nt, nnz = h->nnz;
+  std::vector<int> row(n + 1), col(nnz);
+  HIP_TRY(hipMemcpy(row.data(), h->row, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(col.data(), h->col, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToHost));
+  const std::vector<int> iperm = rcm_order(n, row, col);
+  std::vector<int> perm(n);
+  for (int k = 0; k < n; ++k) perm[iperm[k]] = k;
+  // the renumbered CSR, 1-based: row k is old row iperm[k], entries in the
+  // caller's order (diagonal first), columns renumbered
+  std::vector<int> row1(n + 1), col1(nnz), dia1(n), vmap(nnz);
+  row1[0] = 1;
+  for (int k = 0; k < n; ++k) {
+    const int o = iperm[k];
+    int w = row1[k] - 1;
+    for (int j = row[o]; j < row[o + 1]; ++j, ++w) {
+      col1[w] = perm[col[j]] + 1;
+      vmap[w] = j;
+    }
+    row1[k + 1] = w + 1;
+    dia1[k] = row1[k];
+  }
+  tfg_matrix* in = nullptr;
+  g_inner_create = true;
+  const int rc = tfg_matrix_create(n, n, nnz, row1.data(), col1.data(), dia1.data(), &in);
+  g_inner_create = false;
+  if (rc != TFG_OK) return rc;
+  h->inner = in;
+  auto up = [&](int** d, const std::vector<int>& v) -> hipError_t {
+    hipError_t e = hipMalloc(d, sizeof(int) * std::max<size_t>(1, v.size()));
+    if (e == hipSuccess) e = hipMemcpy(*d, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice);
+    return e;
+  };
+  hipError_t e = up(&h->vmap, vmap);
+  if (e == hipSuccess) e = up(&h->perm, perm);
+  if (e == hipSuccess) e = up(&h->iperm, iperm);
+  if (e == hipSuccess) e = hipMalloc(&h->pval, sizeof(double) * (size_t)std::max(1, nnz));
+  for (double** d : {&h->pb, &h->px, &h->pq})
+    if (e == hipSuccess) e = hipMalloc(d, sizeof(double) * (size_t)std::max(1, n));
+  if (e != hipSuccess) {
+    (void)tfg_matrix_reorder(h, TFG_ORDER_NATURAL);
+    return fail(TFG_ENOMEM, "ordering buffers: %s", hipGetErrorString(e));
+  }
   return TFG_OK;
 }
 
@@ -1277,8 +1468,12 @@ int tfg_matrix_destroy(tfg_matrix* h) {
 
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind, int* spmv_grid,
                     int* stage_blocks, int* offsets, int* patterns, int* sym_diagonals,
-                    int* sym_state) {
+                    int* sym_state, int* ordering) {
   if (!h) return fail(TFG_EINVAL, "null handle");
+  if (ordering) *ordering = h->inner ? TFG_ORDER_RCM : TFG_ORDER_NATURAL;
+  if (h->inner)
+    return tfg_matrix_info(h->inner, n, nnz, spmv_kind, spmv_grid, stage_blocks, offsets,
+                           patterns, sym_diagonals, sym_state, nullptr);
   if (sym_diagonals) *sym_diagonals = h->nsym;
   if (sym_state) *sym_state = h->sym_state;
   if (offsets) *offsets = h->ntab;
@@ -1293,6 +1488,7 @@ int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind, int* spmv_g
 
 int tfg_spmv_select(tfg_matrix* h, int kind) {
   if (!h) return fail(TFG_EINVAL, "null handle");
+  if (h->inner) return tfg_spmv_select(h->inner, kind);
   if (kind < -1 || kind >= tfg::kKinds) return fail(TFG_EINVAL, "unknown SpMV engine %d", kind);
   if (kind >= 0 && !kind_available(h, kind))
     return fail(TFG_ENOTSUP, "engine %d is not available on this matrix", kind);
@@ -1303,6 +1499,10 @@ int tfg_spmv_select(tfg_matrix* h, int kind) {
 int tfg_spmv_tune(tfg_matrix* h, const double* d_val, int* kind, float* us) {
   if (!h || !d_val) return fail(TFG_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
+  if (h->inner) {
+    HIP_TRY(to_inner_val(h, d_val));
+    return tfg_spmv_tune(h->inner, h->pval, kind, us);
+  }
   int rc = tune_all(h, d_val);
   if (rc != TFG_OK) return rc;
   if (kind) *kind = h->kind;
@@ -1331,6 +1531,35 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
   // Beyond the reference: the textbook methods under their PETSc names
   // ('cgs', 'bcgs' / 'bicgstab'), and textbook BiCG for 'bicg' when
   // opt->textbook_bicg is set.
+  if (h->inner && !is_icc(prec, prec_len)) {
+    // internal ordering: permute in, solve the renumbered system, permute x
+    // out.  ICC depends on the ordering, so it runs in the caller's order.
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t S = h->inner->stream;
+    const size_t n = (size_t)h->nt;
+    const double *dval = val, *db = b;
+    double* dx = x;
+    if (!device_ptrs) {
+      if (!h->val) HIP_TRY(hipMalloc(&h->val, sizeof(double) * (size_t)h->nnz));
+      if (!h->b) HIP_TRY(hipMalloc(&h->b, sizeof(double) * n));
+      HIP_TRY(hipMemcpyAsync(h->val, val, sizeof(double) * (size_t)h->nnz, hipMemcpyHostToDevice, S));
+      HIP_TRY(hipMemcpyAsync(h->b, b, sizeof(double) * n, hipMemcpyHostToDevice, S));
+      HIP_TRY(hipMemcpyAsync(h->x, x, sizeof(double) * n, hipMemcpyHostToDevice, S));
+      dval = h->val;
+      db = h->b;
+      dx = h->x;
+    }
+    HIP_TRY(to_inner_val(h, dval));
+    HIP_TRY(to_inner_vec(h, db, h->pb));
+    HIP_TRY(to_inner_vec(h, dx, h->px));
+    const int rc = tfg_solve_native_ex(h->inner, solver, solver_len, prec, prec_len, h->pval,
+                                       h->px, h->pb, miter, niter, tol, fin_res, norm, 1, opt, rep);
+    if (rc != TFG_OK && rc != TFG_EBREAKDOWN) return rc;
+    HIP_TRY(from_inner_vec(h, h->px, dx));
+    if (!device_ptrs) HIP_TRY(hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, S));
+    HIP_TRY(hipStreamSynchronize(S));
+    return rc;
+  }
   tfg_options o{};
   if (opt) o = *opt;
   const std::string sv = fstring(solver, solver_len);
@@ -1589,6 +1818,14 @@ int tfg_prec_apply(tfg_matrix* h, const char* prec, int prec_len, const double* 
                    const double* d_r, double* d_z) {
   if (!h || !d_val || !d_r || !d_z) return fail(TFG_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
+  if (h->inner && !is_icc(prec, prec_len)) {
+    HIP_TRY(to_inner_val(h, d_val));
+    HIP_TRY(to_inner_vec(h, d_r, h->pb));
+    const int rc = tfg_prec_apply(h->inner, prec, prec_len, h->pval, h->pb, h->pq);
+    if (rc != TFG_OK) return rc;
+    HIP_TRY(from_inner_vec(h, h->pq, d_z));
+    return TFG_OK;
+  }
   const std::string pv = fstring(prec, prec_len);
   if (pv == "icc") {
     const int ri = ensure_icc(h);
@@ -1607,6 +1844,14 @@ int tfg_prec_apply(tfg_matrix* h, const char* prec, int prec_len, const double* 
 int tfg_spmv(tfg_matrix* h, const double* d_val, const double* d_p, double* d_q) {
   if (!h || !d_val || !d_p || !d_q) return fail(TFG_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
+  if (h->inner) {
+    HIP_TRY(to_inner_val(h, d_val));
+    HIP_TRY(to_inner_vec(h, d_p, h->px));
+    const int rc = tfg_spmv(h->inner, h->pval, h->px, h->pq);
+    if (rc != TFG_OK) return rc;
+    HIP_TRY(from_inner_vec(h, h->pq, d_q));
+    return TFG_OK;
+  }
   { const int rt = ensure_tuned(h, d_val); if (rt != TFG_OK) return rt; }
   tfg::Launch L = launch_of(h, d_val, h->kind);
   if (h->kind == tfg::K_SYM_DIA) {
@@ -1621,6 +1866,15 @@ int tfg_residual(tfg_matrix* h, const double* d_val, const double* d_b, const do
                  double* d_r) {
   if (!h || !d_val || !d_b || !d_x || !d_r) return fail(TFG_EINVAL, "null argument");
   HIP_TRY(hipSetDevice(h->device));
+  if (h->inner) {
+    HIP_TRY(to_inner_val(h, d_val));
+    HIP_TRY(to_inner_vec(h, d_b, h->pb));
+    HIP_TRY(to_inner_vec(h, d_x, h->px));
+    const int rc = tfg_residual(h->inner, h->pval, h->pb, h->px, h->pq);
+    if (rc != TFG_OK) return rc;
+    HIP_TRY(from_inner_vec(h, h->pq, d_r));
+    return TFG_OK;
+  }
   { const int rt = ensure_tuned(h, d_val); if (rt != TFG_OK) return rt; }
   tfg::Launch L = launch_of(h, d_val, h->kind);
   HIP_TRY(tfg::launch_residual_plain(L, shape_of(h, h->kind), d_val, d_b, d_x, d_r));
@@ -1631,6 +1885,11 @@ int tfg_spmv_bench(tfg_matrix* h, const double* d_val, const double* d_p, double
                    double* ms_avg) {
   if (!h || !ms_avg || reps < 1) return fail(TFG_EINVAL, "bad argument");
   HIP_TRY(hipSetDevice(h->device));
+  if (h->inner) {   // times the renumbered matrix's product alone
+    HIP_TRY(to_inner_val(h, d_val));
+    HIP_TRY(to_inner_vec(h, d_p, h->px));
+    return tfg_spmv_bench(h->inner, h->pval, h->px, h->pq, reps, ms_avg);
+  }
   { const int rt = ensure_tuned(h, d_val); if (rt != TFG_OK) return rt; }
   tfg::Launch L = launch_of(h, d_val, h->kind);
   if (h->kind == tfg::K_SYM_DIA) {
@@ -1696,6 +1955,7 @@ int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, d
 int tfg_synchronize(tfg_matrix* h) {
   if (!h) HIP_TRY(hipDeviceSynchronize());
   else HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h && h->inner) HIP_TRY(hipStreamSynchronize(h->inner->stream));
   return TFG_OK;
 }
 

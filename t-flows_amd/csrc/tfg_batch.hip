@@ -340,6 +340,7 @@ void batch_free(tfg_batch* h) {
   for (hipEvent_t e : {h->ev0, h->ev1, h->ev_poll[0], h->ev_poll[1]})
     if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
+  (void)hipGetLastError();  // release errors are dropped, not left for later calls
   delete h;
 }
 

@@ -214,6 +214,10 @@ struct MomArgs {
 };
 hipError_t launch_momentum(hipStream_t stream, const MomArgs& m);
 
+// dst[k] = src[map[k]], k < count (permutation of values and vectors)
+hipError_t launch_gather(hipStream_t stream, const double* src, const int* map, double* dst,
+                         long long count);
+
 // Launch shapes chosen per matrix.
 struct Launch {
   hipStream_t stream;

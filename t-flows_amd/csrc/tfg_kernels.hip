@@ -20,6 +20,7 @@
 // to the reference's serial arithmetic; only the order of the global dot
 // products differs (deterministic trees here).
 #include "tfg_spmv.h"
+#include <algorithm>
 
 namespace tfg {
 
@@ -596,6 +597,22 @@ hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* 
     return hipGetLastError();
   }
   TFG_ENGINE_SWITCH(L, (k_spmv_plain<E, false><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, val, nullptr, p, q)));
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(const double* __restrict__ src,
+                                                   const int* __restrict__ map,
+                                                   double* __restrict__ dst, long long count) {
+  const long long stride = (long long)gridDim.x * kBlock;
+  for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < count; k += stride)
+    dst[k] = src[map[k]];
+}
+
+hipError_t launch_gather(hipStream_t stream, const double* src, const int* map, double* dst,
+                         long long count) {
+  if (count <= 0) return hipSuccess;
+  const long long blocks = (count + kBlock - 1) / kBlock;
+  k_gather<<<(int)std::min<long long>(blocks, 4096), kBlock, 0, stream>>>(src, map, dst, count);
   return hipGetLastError();
 }
 

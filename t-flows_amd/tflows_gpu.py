@@ -11,6 +11,7 @@ There is no CPU fallback: if libtflows_gpu.so is missing or no GPU is
 present, every call raises.
 """
 import ctypes
+import weakref
 import os
 import re
 
@@ -24,6 +25,7 @@ TFG_OK, TFG_EINVAL, TFG_EHIP, TFG_ENOMEM, TFG_EBREAKDOWN, TFG_ENOTSUP = 0, 1, 2,
 FP64, FP32_IR = 0, 1
 CRIT_NRMS, CRIT_RELRES = 0, 1
 PATH_AUTO, PATH_KERNELS, PATH_GRAPH = 0, 1, 2
+ORDER_NATURAL, ORDER_RCM = 0, 1
 
 
 class TFlowsGpuError(RuntimeError):
@@ -70,7 +72,7 @@ def lib():
         P, I, D, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
         L.tfg_matrix_create.argtypes = [I, I, I, P, P, P, ctypes.POINTER(P)]
         L.tfg_matrix_destroy.argtypes = [P]
-        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P, P, P, P, P]
+        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P, P, P, P, P, P]
         L.tfg_solve_native.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I,
                                        P, D, P, P]
         L.tfg_solve_native_ex.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P,
@@ -92,6 +94,7 @@ def lib():
         L.tfg_faces_destroy.argtypes = [P]
         L.tfg_pressure_assemble.argtypes = [P, P, P, P, P, P, P, I]
         L.tfg_momentum_assemble.argtypes = [P, P, P, P, P]
+        L.tfg_matrix_reorder.argtypes = [P, I]
         L.tfg_batch_create.argtypes = [I, P, P, P, P, P, ctypes.POINTER(P)]
         L.tfg_batch_destroy.argtypes = [P]
         L.tfg_batch_solve.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I, P,
@@ -182,12 +185,20 @@ class Native:
         _check(lib().tfg_matrix_create(self.ni, self.nt, self.nnz, _ptr(self.row), _ptr(self.col),
                                        _ptr(self.dia), ctypes.byref(h)))
         self.h = h
+        # handles built on this one (Faces), closed before it: a garbage
+        # collector may finalise the two in either order
+        self._dependents = weakref.WeakSet()
 
     def info(self):
-        vals = [ctypes.c_int() for _ in range(9)]
+        vals = [ctypes.c_int() for _ in range(10)]
         _check(lib().tfg_matrix_info(self.h, *[ctypes.byref(v) for v in vals]))
         return dict(zip(["n", "nnz", "spmv_kind", "spmv_grid", "stage_blocks", "offsets",
-                         "patterns", "sym_diagonals", "sym_state"], [v.value for v in vals]))
+                         "patterns", "sym_diagonals", "sym_state", "ordering"],
+                        [v.value for v in vals]))
+
+    def reorder(self, order):
+        """Internal ordering: ORDER_NATURAL or ORDER_RCM (tfg_matrix_reorder)."""
+        _check(lib().tfg_matrix_reorder(self.h, int(order)))
 
     def kinds(self):
         """The SpMV engines this matrix offers (6 needs offset codes, 7 row
@@ -197,6 +208,8 @@ class Native:
                 ([8] if i["sym_diagonals"] else []))
 
     def close(self):
+        for d in list(getattr(self, "_dependents", ())):
+            d.close()
         if getattr(self, "h", None):
             lib().tfg_matrix_destroy(self.h)
             self.h = None
@@ -374,6 +387,7 @@ class Faces:
         self.h = ctypes.c_void_p()
         _check(lib().tfg_faces_create(nat.h, self.n_faces, *[_ptr(a) for a in arrs],
                                       ctypes.byref(self.h)))
+        nat._dependents.add(self)
 
     def assemble(self, dfc, dfw, dvm, dvflux, dval, db, singular=False):
         """A % val and b on the device (DeviceArrays); waits for the result."""
