@@ -279,16 +279,33 @@ __global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
   }
   double v[5] = {0.0, kHuge, -kHuge, 0.0, 0.0};
   if (DX) {
-    // r -= alpha q and the sums that need no x (x is folded in k_pupdate)
-    const int stride = gridDim.x * kBlock;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) {
-      const double dv = a.ad[i];
-      const double rv = a.r[i] - alpha * a.q[i];
-      a.r[i] = rv;
+    // r -= alpha q and the sums that need no x (x is folded in k_pupdate),
+    // two rows per thread with 16-byte accesses
+    auto sums = [&](double rv, double dv) {
       const double z = PREC == PREC_JACOBI ? rv * (1.0 / dv) : rv;
       v[0] = v[0] + (rv * rv) / (dv * dv);
       v[3] = v[3] + z * rv;
       v[4] = v[4] + rv * rv;
+    };
+    const int stride = gridDim.x * kBlock;
+    const int npair = a.n >> 1;
+    dbl2* r2 = reinterpret_cast<dbl2*>(a.r);
+    const dbl2* q2 = reinterpret_cast<const dbl2*>(a.q);
+    const dbl2* d2 = reinterpret_cast<const dbl2*>(a.ad);
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < npair; t += stride) {
+      const dbl2 rv = r2[t], qv = q2[t], dv = d2[t];
+      dbl2 ro;
+      ro.x = rv.x - alpha * qv.x;
+      ro.y = rv.y - alpha * qv.y;
+      r2[t] = ro;
+      sums(ro.x, dv.x);
+      sums(ro.y, dv.y);
+    }
+    if ((a.n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+      const int i = a.n - 1;
+      const double rv = a.r[i] - alpha * a.q[i];
+      a.r[i] = rv;
+      sums(rv, a.ad[i]);
     }
     const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
     publish<5>(v, ops, a.pt.upd, a.pt.pstride, s_tmp, s_out);
