@@ -274,3 +274,49 @@ def test_symmetric_engine_graph_and_fallback(cases, oracle, gpu):
     it, res, x, _ = _gpu_solve(gpu, s2, "cg", "jacobi", 800, 1e-9, norm=1.0, nat=nat)
     assert nat.info()["sym_state"] == 0
     assert abs(it - it_o) <= 2 and _rel(x, x_o) <= X_TOL
+
+
+def _tiny(n):
+    """SPD tridiagonal system of n rows (1-based CSR, diagonal first)."""
+    row, col, val = [1], [], []
+    for i in range(n):
+        cols = [i] + ([i - 1] if i > 0 else []) + ([i + 1] if i + 1 < n else [])
+        vals = [2.0 + 0.5 * i] + [-1.0] * (len(cols) - 1)
+        col += [c + 1 for c in cols]
+        val += vals
+        row.append(len(col) + 1)
+    row = np.array(row, np.int32)
+    return row, np.array(col, np.int32), row[:-1].copy(), np.array(val), \
+        np.linspace(1.0, 2.0, n) if n > 1 else np.array([3.0])
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5])
+def test_tiny_systems(oracle, gpu, n):
+    """1..5 rows (fewer than 4 nonzeros for n = 1): every engine's product
+    and CG / BiCG with every preconditioner and criterion against the
+    oracle."""
+    row, col, dia, val, b = _tiny(n)
+    print(f"tiny case n={n} nnz={len(col)}", flush=True)
+    A = oracle.Matrix(row, col, dia, val)
+    nat = gpu.Native(row, col, dia)
+    p = np.arange(1.0, n + 1.0)
+    dv, dp, dq = gpu.DeviceArray(val), gpu.DeviceArray(p), gpu.DeviceArray(np.zeros(n))
+    for kind in nat.kinds():
+        nat.spmv_select(kind)
+        nat.spmv(dv, dp, dq)
+        assert np.array_equal(dq.download(), oracle.spmv(A, p)), kind
+    nat.spmv_select(-1)
+    for solver in ("cg", "bicg"):
+        for prec in ("jacobi", "none", "icc"):
+            for crit in (0, 1):
+                Ao = oracle.Matrix(row, col, dia, val.copy())
+                xo = np.zeros(n)
+                it_o, res_o = oracle.solve(solver, prec, Ao, xo, b.copy(), 50, 1e-12,
+                                           criterion="relres" if crit else "nrms")
+                o = gpu.Options()
+                o.criterion = crit
+                x = np.zeros(n)
+                it, res = nat.solve_native(solver, prec, val.copy(), x, b.copy(), 50, 1e-12,
+                                           options=o)
+                assert abs(it - it_o) <= 2 and res < 1e-12, (solver, prec, crit, it, it_o, res)
+                assert np.allclose(x, xo, rtol=1e-12, atol=0.0), (solver, prec, crit)
