@@ -272,6 +272,38 @@ typedef struct {
 int tfg_momentum_assemble(tfg_faces* faces, const tfg_momentum* m, double* val, double* b,
                           double* v_m);
 
+/* ---- Rhie-Chow face fluxes on the device ----------------------------------
+ * The volume flux of every domain face (c2 > 0), Rhie_And_Chow.f90:86-259:
+ * the interpolated cell velocities through the face plus the pressure
+ * smoothing a12 (p1 - p2) + fc (grad p v_m)_f . d, with the optional VOF
+ * surface-tension term, Choi's time-step correction and Gu's body-force
+ * correction.  Each face on its own: bit-identical to the serial loop.
+ * Boundary faces are not written.  The matrix lines of the same loop are
+ * tfg_pressure_assemble's.  Device pointers; cell arrays hold the cells
+ * 1..n (Cells_In_Domain_And_Buffers), face arrays n_faces entries. */
+typedef struct {
+  /* per face: Grid % f, A % fc, Grid % sx sy sz, Grid % dx dy dz */
+  const double *f, *fc, *sx, *sy, *sz, *dx, *dy, *dz;
+  /* per cell: u % n, v % n, w % n, p % n, p % x y z, M % v_m */
+  const double *u, *v, *w, *p, *px, *py, *pz, *v_m;
+  /* Choi's correction (choi != 0, applied when the caller's
+   * Time % Curr_Dt() - Time % First_Dt() > 3 holds): u/v/w % o and oo,
+   * density, v_flux % o and oo, dt, and w_o, w_oo (1, 0 for LINEAR, 2,
+   * -0.5 for PARABOLIC) */
+  int choi;
+  const double *u_o, *u_oo, *v_o, *v_oo, *w_o_, *w_oo_, *density, *flux_o, *flux_oo;
+  double dt, w_o, w_oo;
+  /* Gu's correction (gu != 0): Flow % cell_fx fy fz (cells), face_fx fy fz */
+  int gu;
+  const double *cell_fx, *cell_fy, *cell_fz, *face_fx, *face_fy, *face_fz;
+  /* surface tension (vof != 0 and sigma > TINY = 1e-15): Vof % curv, fun % n, fun % x y z */
+  int vof;
+  double sigma;
+  const double *curv, *col, *col_x, *col_y, *col_z;
+} tfg_rhie_chow;
+/* v_flux (n_faces) out on the domain faces; stream-ordered. */
+int tfg_rhie_chow_fluxes(tfg_faces* faces, const tfg_rhie_chow* rc, double* v_flux);
+
 /* ---- several MPI ranks -----------------------------------------------------
  * A matrix with buffer cells (ni < nt: rows ni+1..nt are copies of other
  * ranks' cells) is solved with the caller's communication, which stands in

@@ -822,3 +822,65 @@ void or_momentum_assemble(int nc, int nnz, int n_faces, const int* c1, const int
   }
   free(cross); free(fi); free(advect); free(upwind); free(mn); free(mx);
 }
+
+/* ------------------------------------------------------------------------
+ * Rhie-Chow volume fluxes, Rhie_And_Chow.f90:86-259 (the matrix lines of
+ * the same face loop are or_pressure_assemble's).  Cells 1-based.
+ * ---------------------------------------------------------------------- */
+void or_rhie_chow_fluxes(int nc, int n_faces, const int* c1, const int* c2,
+                         const or_rhie_chow* r, double* v_flux) {
+  const int vof = r->vof && r->sigma > 1.0e-15;      /* TINY = FEMTO, :104-105 */
+  double* pst = malloc(sizeof(double) * 3 * (size_t)nc);
+  double* uc = malloc(sizeof(double) * 3 * (size_t)nc);
+  double* tm = malloc(sizeof(double) * (size_t)nc);
+  for (int c = 0; c < nc; c++) {                              /* :86-91, 107-112 */
+    pst[3 * c] = r->px[c];
+    pst[3 * c + 1] = r->py[c];
+    pst[3 * c + 2] = r->pz[c];
+    if (vof) {
+      pst[3 * c] = pst[3 * c] - r->sigma * r->curv[c] * r->col_x[c];
+      pst[3 * c + 1] = pst[3 * c + 1] - r->sigma * r->curv[c] * r->col_y[c];
+      pst[3 * c + 2] = pst[3 * c + 2] - r->sigma * r->curv[c] * r->col_z[c];
+    }
+    uc[3 * c] = r->u[c];                                      /* :130-134 */
+    uc[3 * c + 1] = r->v[c];
+    uc[3 * c + 2] = r->w[c];
+    if (r->choi) {                                            /* :139-163 */
+      tm[c] = r->v_m[c] * r->density[c] / r->dt;
+      uc[3 * c] = uc[3 * c] - (r->w_o * r->u_o[c] + r->w_oo * r->u_oo[c]) * tm[c];
+      uc[3 * c + 1] = uc[3 * c + 1] - (r->w_o * r->v_o[c] + r->w_oo * r->v_oo[c]) * tm[c];
+      uc[3 * c + 2] = uc[3 * c + 2] - (r->w_o * r->w_o_[c] + r->w_oo * r->w_oo_[c]) * tm[c];
+    }
+    if (r->gu) {                                              /* :169-179 */
+      uc[3 * c] = uc[3 * c] - r->v_m[c] * r->cell_fx[c];
+      uc[3 * c + 1] = uc[3 * c + 1] - r->v_m[c] * r->cell_fy[c];
+      uc[3 * c + 2] = uc[3 * c + 2] - r->v_m[c] * r->cell_fz[c];
+    }
+  }
+  for (int s = 0; s < n_faces; s++) {                         /* :184-259 */
+    if (c2[s] <= 0) continue;
+    const int a = c1[s] - 1, b = c2[s] - 1;
+    const double fs = r->f[s];
+    double pst_d = r->p[a] - r->p[b];                         /* :94-98 */
+    if (vof) {                                                /* :115-123 */
+      const double curv_f = fs * r->curv[a] + (1.0 - fs) * r->curv[b];
+      pst_d = pst_d - r->sigma * curv_f * (r->col[a] - r->col[b]);
+    }
+    const double u_f = fs * uc[3 * a] + (1.0 - fs) * uc[3 * b];
+    const double v_f = fs * uc[3 * a + 1] + (1.0 - fs) * uc[3 * b + 1];
+    const double w_f = fs * uc[3 * a + 2] + (1.0 - fs) * uc[3 * b + 2];
+    const double a12 = r->fc[s] * (fs * r->v_m[a] + (1.0 - fs) * r->v_m[b]);
+    const double px_f = (fs * (pst[3 * a] * r->v_m[a]) + (1.0 - fs) * (pst[3 * b] * r->v_m[b])) * r->dx[s];
+    const double py_f = (fs * (pst[3 * a + 1] * r->v_m[a]) + (1.0 - fs) * (pst[3 * b + 1] * r->v_m[b])) * r->dy[s];
+    const double pz_f = (fs * (pst[3 * a + 2] * r->v_m[a]) + (1.0 - fs) * (pst[3 * b + 2] * r->v_m[b])) * r->dz[s];
+    double fl = (u_f * r->sx[s] + v_f * r->sy[s] + w_f * r->sz[s]) + a12 * pst_d +
+                r->fc[s] * (px_f + py_f + pz_f);
+    if (r->choi)                                              /* :235-246 */
+      fl = fl + (fs * tm[a] + (1.0 - fs) * tm[b]) * (r->w_o * r->flux_o[s] + r->w_oo * r->flux_oo[s]);
+    if (r->gu)                                                /* :250-256 */
+      fl = fl + (fs * r->v_m[a] + (1.0 - fs) * r->v_m[b]) *
+                    (r->face_fx[s] * r->sx[s] + r->face_fy[s] * r->sy[s] + r->face_fz[s] * r->sz[s]);
+    v_flux[s] = fl;
+  }
+  free(pst); free(uc); free(tm);
+}

@@ -94,6 +94,23 @@ void or_momentum_assemble(int nc, int nnz, int n_faces, const int* c1, const int
                           const int* pos, const int* dia, const or_momentum* m, double* val,
                           double* b, double* v_m);
 
+/* Rhie-Chow volume fluxes of the domain faces (Rhie_And_Chow.f90:77-259),
+ * serial; cell arrays 1-based at cell 1.  Same fields as tfg_rhie_chow. */
+typedef struct {
+  const double *f, *fc, *sx, *sy, *sz, *dx, *dy, *dz;
+  const double *u, *v, *w, *p, *px, *py, *pz, *v_m;
+  int choi;
+  const double *u_o, *u_oo, *v_o, *v_oo, *w_o_, *w_oo_, *density, *flux_o, *flux_oo;
+  double dt, w_o, w_oo;
+  int gu;
+  const double *cell_fx, *cell_fy, *cell_fz, *face_fx, *face_fy, *face_fz;
+  int vof;
+  double sigma;
+  const double *curv, *col, *col_x, *col_y, *col_z;
+} or_rhie_chow;
+void or_rhie_chow_fluxes(int nc, int n_faces, const int* c1, const int* c2,
+                         const or_rhie_chow* r, double* v_flux);
+
 /* 0: the reference's serial summation (default); 1, 2: blocked-pairwise
  * summation (blocks of 64, 1000) of every global reduction, to measure
  * rounding sensitivity. */

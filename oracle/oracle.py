@@ -62,6 +62,7 @@ def lib(omp=False):
         L.or_transpose_positions.argtypes = [P, P]
         L.or_pressure_assemble.argtypes = [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I]
         L.or_momentum_assemble.argtypes = [I, I, I, P, P, P, P, P, P, P, P]
+        L.or_rhie_chow_fluxes.argtypes = [I, I, P, P, P, P]
         _libs[name] = L
     return _libs[name]
 
@@ -274,3 +275,42 @@ def momentum_assemble(f, mf, component=1, adv="central", blend_matrix=True, td="
     lib().or_momentum_assemble(len(f.dia), len(f.col), len(f.c1), *[_p(a) for a in arrs],
                                ctypes.byref(m), _p(val), _p(b), _p(vm))
     return val, b, vm
+
+
+RC_PTRS = ("f", "fc", "sx", "sy", "sz", "dx", "dy", "dz", "u", "v", "w", "p", "px", "py", "pz",
+           "v_m")
+RC_CHOI = ("u_o", "u_oo", "v_o", "v_oo", "w_o_", "w_oo_", "density", "flux_o", "flux_oo")
+RC_GU = ("cell_fx", "cell_fy", "cell_fz", "face_fx", "face_fy", "face_fz")
+RC_VOF = ("curv", "col", "col_x", "col_y", "col_z")
+
+
+class _RhieChow(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_void_p) for k in RC_PTRS] + [("choi", ctypes.c_int)] +
+                [(k, ctypes.c_void_p) for k in RC_CHOI] +
+                [(k, ctypes.c_double) for k in ("dt", "w_o", "w_oo")] + [("gu", ctypes.c_int)] +
+                [(k, ctypes.c_void_p) for k in RC_GU[:3] + RC_GU[3:]] +
+                [("vof", ctypes.c_int), ("sigma", ctypes.c_double)] +
+                [(k, ctypes.c_void_p) for k in RC_VOF])
+
+
+def rhie_chow_fluxes(f, rf, v_flux, choi=False, dt=1e-2, w_o=1.0, w_oo=0.0, gu=False, vof=False,
+                     sigma=0.07):
+    """Serial Rhie-Chow fluxes (Rhie_And_Chow.f90:86-259) of the domain faces
+    of Faces set f with tflows_cases.RhieChowFields rf; returns a copy of
+    v_flux with those faces replaced."""
+    keep = []
+
+    def ptr(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data
+    r = _RhieChow()
+    for k in RC_PTRS + RC_CHOI + RC_GU + RC_VOF:
+        setattr(r, k, ptr(getattr(rf, k)))
+    r.choi, r.gu, r.vof = int(choi), int(gu), int(vof)
+    r.dt, r.w_o, r.w_oo, r.sigma = dt, w_o, w_oo, sigma
+    out = np.array(v_flux, dtype=np.float64)
+    arrs = [np.ascontiguousarray(a) for a in (f.c1, f.c2)]
+    lib().or_rhie_chow_fluxes(len(f.dia), len(f.c1), *[_p(a) for a in arrs], ctypes.byref(r),
+                              _p(out))
+    return out

@@ -1000,6 +1000,41 @@ int tfg_momentum_assemble(tfg_faces* f, const tfg_momentum* m, double* val, doub
   return TFG_OK;
 }
 
+int tfg_rhie_chow_fluxes(tfg_faces* f, const tfg_rhie_chow* r, double* v_flux) {
+  if (!f || !r || !v_flux) return fail(TFG_EINVAL, "null argument");
+  const double* need[] = {r->f, r->fc, r->sx, r->sy, r->sz, r->dx, r->dy, r->dz, r->u, r->v, r->w,
+                          r->p, r->px, r->py, r->pz, r->v_m};
+  for (const double* q : need)
+    if (!q) return fail(TFG_EINVAL, "null Rhie-Chow array");
+  if (r->choi) {
+    const double* choi[] = {r->u_o, r->u_oo, r->v_o, r->v_oo, r->w_o_, r->w_oo_, r->density,
+                            r->flux_o, r->flux_oo};
+    for (const double* q : choi)
+      if (!q) return fail(TFG_EINVAL, "null array of Choi's correction");
+  }
+  if (r->gu) {
+    const double* gu[] = {r->cell_fx, r->cell_fy, r->cell_fz, r->face_fx, r->face_fy, r->face_fz};
+    for (const double* q : gu)
+      if (!q) return fail(TFG_EINVAL, "null array of Gu's correction");
+  }
+  const bool vof = r->vof && r->sigma > 1.0e-15;   // Rhie_And_Chow.f90:104-105, TINY
+  if (vof) {
+    const double* st[] = {r->curv, r->col, r->col_x, r->col_y, r->col_z};
+    for (const double* q : st)
+      if (!q) return fail(TFG_EINVAL, "null surface-tension array");
+  }
+  HIP_TRY(hipSetDevice(f->device));
+  tfg::RcArgs a{};
+  a.n_faces = f->n_faces;
+  a.c1 = f->c1;
+  a.c2 = f->c2;
+  a.r = *r;
+  a.vof = vof ? 1 : 0;
+  a.v_flux = v_flux;
+  HIP_TRY(tfg::launch_rhie_chow(f->h->stream, a));
+  return TFG_OK;
+}
+
 int tfg_faces_destroy(tfg_faces* f) {
   if (!f) return fail(TFG_EINVAL, "null handle");
   faces_free(f);

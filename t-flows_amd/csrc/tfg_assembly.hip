@@ -278,6 +278,86 @@ __global__ __launch_bounds__(kBlock) void k_momentum(MomArgs m) {
   m.b[c] = bb;
 }
 
+// ---------------------------------------------------------------------------
+// Rhie-Chow volume fluxes (Rhie_And_Chow.f90:86-259), one thread per face.
+// The cell quantities (the corrected velocities u_c, the pressure terms pst)
+// are formed for each of the face's two cells with the serial loop's
+// operations, so both faces of a cell see the same bits.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct RcCell {
+  double u, v, w, px, py, pz, tm;
+};
+
+__device__ __forceinline__ RcCell rc_cell(const RcArgs& a, int c) {
+  const tfg_rhie_chow& r = a.r;
+  RcCell o;
+  o.px = r.px[c];
+  o.py = r.py[c];
+  o.pz = r.pz[c];
+  if (a.vof) {                                            // :107-112
+    o.px = o.px - r.sigma * r.curv[c] * r.col_x[c];
+    o.py = o.py - r.sigma * r.curv[c] * r.col_y[c];
+    o.pz = o.pz - r.sigma * r.curv[c] * r.col_z[c];
+  }
+  o.u = r.u[c];                                           // :130-134
+  o.v = r.v[c];
+  o.w = r.w[c];
+  o.tm = 0.0;
+  if (r.choi) {                                           // :139-163
+    o.tm = r.v_m[c] * r.density[c] / r.dt;
+    o.u = o.u - (r.w_o * r.u_o[c] + r.w_oo * r.u_oo[c]) * o.tm;
+    o.v = o.v - (r.w_o * r.v_o[c] + r.w_oo * r.v_oo[c]) * o.tm;
+    o.w = o.w - (r.w_o * r.w_o_[c] + r.w_oo * r.w_oo_[c]) * o.tm;
+  }
+  if (r.gu) {                                             // :169-179
+    o.u = o.u - r.v_m[c] * r.cell_fx[c];
+    o.v = o.v - r.v_m[c] * r.cell_fy[c];
+    o.w = o.w - r.v_m[c] * r.cell_fz[c];
+  }
+  return o;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_rhie_chow(RcArgs a) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= a.n_faces) return;
+  const int c1 = a.c1[s], c2 = a.c2[s];
+  if (c2 < 0) return;                                     // domain faces only
+  const tfg_rhie_chow& r = a.r;
+  const RcCell A = rc_cell(a, c1), B = rc_cell(a, c2);
+  const double fs = r.f[s];
+  double pst_d = r.p[c1] - r.p[c2];                       // :94-98
+  if (a.vof) {                                            // :115-123
+    const double curv_f = fs * r.curv[c1] + (1.0 - fs) * r.curv[c2];
+    pst_d = pst_d - r.sigma * curv_f * (r.col[c1] - r.col[c2]);
+  }
+  const double u_f = fs * A.u + (1.0 - fs) * B.u;
+  const double v_f = fs * A.v + (1.0 - fs) * B.v;
+  const double w_f = fs * A.w + (1.0 - fs) * B.w;
+  const double vm1 = r.v_m[c1], vm2 = r.v_m[c2];
+  const double a12 = r.fc[s] * (fs * vm1 + (1.0 - fs) * vm2);
+  const double px_f = (fs * (A.px * vm1) + (1.0 - fs) * (B.px * vm2)) * r.dx[s];
+  const double py_f = (fs * (A.py * vm1) + (1.0 - fs) * (B.py * vm2)) * r.dy[s];
+  const double pz_f = (fs * (A.pz * vm1) + (1.0 - fs) * (B.pz * vm2)) * r.dz[s];
+  double fl = (u_f * r.sx[s] + v_f * r.sy[s] + w_f * r.sz[s]) + a12 * pst_d +
+              r.fc[s] * (px_f + py_f + pz_f);             // :226-230
+  if (r.choi)                                             // :235-246
+    fl = fl + (fs * A.tm + (1.0 - fs) * B.tm) * (r.w_o * r.flux_o[s] + r.w_oo * r.flux_oo[s]);
+  if (r.gu)                                               // :250-256
+    fl = fl + (fs * vm1 + (1.0 - fs) * vm2) *
+                  (r.face_fx[s] * r.sx[s] + r.face_fy[s] * r.sy[s] + r.face_fz[s] * r.sz[s]);
+  a.v_flux[s] = fl;
+}
+
+hipError_t launch_rhie_chow(hipStream_t stream, const RcArgs& a) {
+  if (a.n_faces <= 0) return hipSuccess;
+  k_rhie_chow<<<(a.n_faces + kBlock - 1) / kBlock, kBlock, 0, stream>>>(a);
+  return hipGetLastError();
+}
+
 hipError_t launch_momentum(hipStream_t stream, const MomArgs& m) {
   const int grid = (m.n + kBlock - 1) / kBlock;
   if (m.adv != TFG_ADV_CENTRAL) {

@@ -214,6 +214,17 @@ struct MomArgs {
 };
 hipError_t launch_momentum(hipStream_t stream, const MomArgs& m);
 
+// Rhie-Chow face fluxes (tfg_assembly.hip); c1 0-based, c2 0-based or -1.
+struct RcArgs {
+  int n_faces;
+  const int* c1;
+  const int* c2;
+  tfg_rhie_chow r;
+  int vof;               // surface tension term active
+  double* v_flux;
+};
+hipError_t launch_rhie_chow(hipStream_t stream, const RcArgs& a);
+
 // dst[k] = src[map[k]], k < count (permutation of values and vectors)
 hipError_t launch_gather(hipStream_t stream, const double* src, const int* map, double* dst,
                          long long count);

@@ -26,6 +26,21 @@
                       vol, p_i, cell_f, st
   end type
 
+  ! tfg_rhie_chow (tflows_gpu.h): the Rhie-Chow volume fluxes
+  ! (Rhie_And_Chow.f90:86-259); device addresses of cell (1..n) and face arrays.
+  type, bind(C) :: Tflows_Gpu_Rhie_Chow
+    type(c_ptr)    :: f, fc, sx, sy, sz, dx, dy, dz
+    type(c_ptr)    :: u, v, w, p, px, py, pz, v_m
+    integer(c_int) :: choi
+    type(c_ptr)    :: u_o, u_oo, v_o, v_oo, w_o_, w_oo_, density, flux_o, flux_oo
+    real(c_double) :: dt, w_o, w_oo
+    integer(c_int) :: gu
+    type(c_ptr)    :: cell_fx, cell_fy, cell_fz, face_fx, face_fy, face_fz
+    integer(c_int) :: vof
+    real(c_double) :: sigma
+    type(c_ptr)    :: curv, col, col_x, col_y, col_z
+  end type
+
   interface
 
     integer(c_int) function tfg_matrix_create(ni, nt, nnz, row, col, dia,  &
@@ -95,6 +110,13 @@
       import :: c_int, c_ptr
       type(c_ptr),    value :: faces, fc, fw, vm, vflux, val, b
       integer(c_int), value :: singular
+    end function
+
+    integer(c_int) function tfg_rhie_chow_fluxes(faces, rc, v_flux)          &
+                                         bind(C, name='tfg_rhie_chow_fluxes')
+      import :: c_int, c_ptr, Tflows_Gpu_Rhie_Chow
+      type(c_ptr),                value      :: faces, v_flux
+      type(Tflows_Gpu_Rhie_Chow), intent(in) :: rc
     end function
 
     integer(c_int) function tfg_momentum_assemble(faces, m, val, b, v_m)    &

@@ -286,3 +286,89 @@ def momentum_fields(faces, seed=DEFAULT_SEED, rho_ratio=1000.0):
         phi_oo=rng.uniform(-1.0, 1.0, n), vol=h ** 3 * rng.uniform(0.5, 2.0, n),
         p_i=rng.uniform(-1.0, 1.0, n) / h, cell_f=rng.uniform(-1.0, 1.0, n),
         st=rng.uniform(-1.0, 1.0, n))
+
+
+@dataclass
+class RhieChowFields:
+    """Face and cell data of the Rhie-Chow flux loop on a hex Faces set.
+    Per face: Grid % f, A % fc, the area vector s and the centre-to-centre
+    vector d (c1 to c2) of the uniform grid; per cell (1..n): velocities,
+    pressure and its gradient, v_m, and the fields of Choi's and Gu's
+    corrections and of the surface-tension term."""
+    f: np.ndarray
+    fc: np.ndarray
+    sx: np.ndarray
+    sy: np.ndarray
+    sz: np.ndarray
+    dx: np.ndarray
+    dy: np.ndarray
+    dz: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    p: np.ndarray
+    px: np.ndarray
+    py: np.ndarray
+    pz: np.ndarray
+    v_m: np.ndarray
+    u_o: np.ndarray
+    u_oo: np.ndarray
+    v_o: np.ndarray
+    v_oo: np.ndarray
+    w_o_: np.ndarray
+    w_oo_: np.ndarray
+    density: np.ndarray
+    flux_o: np.ndarray
+    flux_oo: np.ndarray
+    cell_fx: np.ndarray
+    cell_fy: np.ndarray
+    cell_fz: np.ndarray
+    face_fx: np.ndarray
+    face_fy: np.ndarray
+    face_fz: np.ndarray
+    curv: np.ndarray
+    col: np.ndarray
+    col_x: np.ndarray
+    col_y: np.ndarray
+    col_z: np.ndarray
+
+
+def cell_centres(nx, ny, nz):
+    """Centres of the cells of the unit cube's nx*ny*nz hex grid (x slowest,
+    z fastest, as the generators number them)."""
+    i = np.arange(nx * ny * nz)
+    return np.stack([(i // (ny * nz) + 0.5) / nx, ((i // nz) % ny + 0.5) / ny,
+                     (i % nz + 0.5) / nz], 1)
+
+
+def rhie_chow_fields(faces, nx, ny, nz, seed=DEFAULT_SEED):
+    """Seeded fields on hex_faces(nx, ny, nz): the grid's own geometry
+    (d = centre(c2) - centre(c1), s = area normal, f = 1/2, fc = |s|/|d|)
+    and random cell and face data of unit size."""
+    rng = np.random.default_rng(seed + 29)
+    n, nf = faces.n, faces.n_faces
+    xc = cell_centres(nx, ny, nz)
+    internal = faces.c2 > 0
+    d = np.zeros((nf, 3))
+    d[internal] = xc[faces.c2[internal] - 1] - xc[faces.c1[internal] - 1]
+    h = np.array([1.0 / nx, 1.0 / ny, 1.0 / nz])
+    area = np.array([h[1] * h[2], h[0] * h[2], h[0] * h[1]])
+    axis = np.argmax(np.abs(d), 1)
+    sv = np.zeros((nf, 3))
+    sv[np.arange(nf), axis] = area[axis] * np.where(internal, 1.0, 0.0)
+    fc = np.where(internal, np.linalg.norm(sv, axis=1) / np.maximum(np.linalg.norm(d, axis=1), 1e-300), 0.0)
+
+    def cells(scale=1.0):
+        return scale * rng.uniform(-1.0, 1.0, n)
+
+    def faces_(scale=1.0):
+        return scale * rng.uniform(-1.0, 1.0, nf)
+    return RhieChowFields(
+        f=np.full(nf, 0.5), fc=fc, sx=sv[:, 0].copy(), sy=sv[:, 1].copy(), sz=sv[:, 2].copy(),
+        dx=d[:, 0].copy(), dy=d[:, 1].copy(), dz=d[:, 2].copy(),
+        u=cells(), v=cells(), w=cells(), p=cells(), px=cells(), py=cells(), pz=cells(),
+        v_m=1e-2 * rng.uniform(0.5, 2.0, n), u_o=cells(), u_oo=cells(), v_o=cells(),
+        v_oo=cells(), w_o_=cells(), w_oo_=cells(), density=rng.uniform(1.0, 10.0, n),
+        flux_o=faces_(area.max()), flux_oo=faces_(area.max()), cell_fx=cells(), cell_fy=cells(),
+        cell_fz=cells(), face_fx=faces_(), face_fy=faces_(), face_fz=faces_(), curv=cells(10.0),
+        col=rng.uniform(0.0, 1.0, n), col_x=cells(), col_y=cells(), col_z=cells())

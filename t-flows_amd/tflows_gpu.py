@@ -95,6 +95,7 @@ def lib():
         L.tfg_pressure_assemble.argtypes = [P, P, P, P, P, P, P, I]
         L.tfg_momentum_assemble.argtypes = [P, P, P, P, P]
         L.tfg_matrix_reorder.argtypes = [P, I]
+        L.tfg_rhie_chow_fluxes.argtypes = [P, P, P]
         L.tfg_batch_create.argtypes = [I, P, P, P, P, P, ctypes.POINTER(P)]
         L.tfg_batch_destroy.argtypes = [P]
         L.tfg_batch_solve.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I, P,
@@ -401,6 +402,12 @@ class Faces:
         _check(lib().tfg_momentum_assemble(self.h, ctypes.byref(m.s), dval.ptr, db.ptr, dvm.ptr))
         _check(lib().tfg_synchronize(self.nat.h))
 
+    def rhie_chow(self, rc, dflux):
+        """Rhie-Chow volume fluxes of the domain faces into dflux
+        (tfg_rhie_chow_fluxes); rc is a RhieChow; waits for the result."""
+        _check(lib().tfg_rhie_chow_fluxes(self.h, ctypes.byref(rc.s), dflux.ptr))
+        _check(lib().tfg_synchronize(self.nat.h))
+
     def close(self):
         if self.h:
             lib().tfg_faces_destroy(self.h)
@@ -459,3 +466,36 @@ class Momentum:
         d = DeviceArray(np.ascontiguousarray(a))
         self.dev[key] = d
         return d.ptr.value + offset
+
+
+RC_PTRS = ("f", "fc", "sx", "sy", "sz", "dx", "dy", "dz", "u", "v", "w", "p", "px", "py", "pz",
+           "v_m")
+RC_CHOI = ("u_o", "u_oo", "v_o", "v_oo", "w_o_", "w_oo_", "density", "flux_o", "flux_oo")
+RC_GU = ("cell_fx", "cell_fy", "cell_fz", "face_fx", "face_fy", "face_fz")
+RC_VOF = ("curv", "col", "col_x", "col_y", "col_z")
+
+
+class _RhieChowS(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_void_p) for k in RC_PTRS] + [("choi", ctypes.c_int)] +
+                [(k, ctypes.c_void_p) for k in RC_CHOI] +
+                [(k, ctypes.c_double) for k in ("dt", "w_o", "w_oo")] + [("gu", ctypes.c_int)] +
+                [(k, ctypes.c_void_p) for k in RC_GU] +
+                [("vof", ctypes.c_int), ("sigma", ctypes.c_double)] +
+                [(k, ctypes.c_void_p) for k in RC_VOF])
+
+
+class RhieChow:
+    """tfg_rhie_chow on device copies of tflows_cases.RhieChowFields; the
+    arrays of a switched-off correction are not uploaded (NULL)."""
+
+    def __init__(self, rf, choi=False, dt=1e-2, w_o=1.0, w_oo=0.0, gu=False, vof=False,
+                 sigma=0.07):
+        s = self.s = _RhieChowS()
+        self.dev = {}
+        for k in RC_PTRS + (RC_CHOI if choi else ()) + (RC_GU if gu else ()) + \
+                (RC_VOF if vof else ()):
+            d = DeviceArray(np.ascontiguousarray(getattr(rf, k), dtype=np.float64))
+            self.dev[k] = d
+            setattr(s, k, d.ptr.value)
+        s.choi, s.gu, s.vof = int(choi), int(gu), int(vof)
+        s.dt, s.w_o, s.w_oo, s.sigma = dt, w_o, w_oo, sigma

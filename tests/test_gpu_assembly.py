@@ -156,3 +156,66 @@ def test_momentum_validation(cases, gpu):
     with pytest.raises(gpu.TFlowsGpuError) as e:
         faces.momentum(m, dval, db, dvm)
     assert e.value.code == gpu.TFG_EINVAL
+
+
+# ---- Rhie-Chow fluxes (tfg_rhie_chow_fluxes) -------------------------------
+
+@pytest.mark.parametrize("choi,gu,vof,td", [(False, False, False, "linear"),
+                                            (True, False, False, "linear"),
+                                            (True, True, True, "parabolic"),
+                                            (False, True, False, "linear"),
+                                            (False, False, True, "linear")])
+def test_rhie_chow_bitexact(cases, oracle, gpu, choi, gu, vof, td):
+    """Every domain face's flux equals the serial loop's bit for bit; the
+    boundary faces keep their values."""
+    dims = (13, 11, 9)
+    f = cases.hex_faces(*dims)
+    rf = cases.rhie_chow_fields(f, *dims)
+    w_o, w_oo = (1.0, 0.0) if td == "linear" else (2.0, -0.5)
+    kw = dict(choi=choi, dt=3e-3, w_o=w_o, w_oo=w_oo, gu=gu, vof=vof, sigma=0.07)
+    print(f"rhie-chow case {dims} {kw}", flush=True)
+    base = np.random.default_rng(4).uniform(-1.0, 1.0, f.n_faces)
+    ref = oracle.rhie_chow_fluxes(f, rf, base, **kw)
+    nat = gpu.Native(f.row, f.col, f.dia)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    dflux = gpu.DeviceArray(base)
+    faces.rhie_chow(gpu.RhieChow(rf, **kw), dflux)
+    assert np.array_equal(dflux.download(), ref)
+
+
+def test_rhie_chow_then_pressure_system(cases, oracle, gpu):
+    """Momentum v_m -> Rhie-Chow fluxes -> pressure system, all on the
+    device, against the same chain of serial loops."""
+    dims = (10, 9, 8)
+    f = cases.hex_faces(*dims)
+    mf = cases.momentum_fields(f)
+    rf = cases.rhie_chow_fields(f, *dims)
+    _, _, vm_o = oracle.momentum_assemble(f, mf)
+    nat, _, _, vm = _momentum_gpu(gpu, f, mf)
+    assert np.array_equal(vm, vm_o)
+    import dataclasses
+    rf = dataclasses.replace(rf, v_m=vm_o)
+    flux_o = oracle.rhie_chow_fluxes(f, rf, f.vflux)
+    g = dataclasses.replace(f, vm=vm_o, vflux=flux_o)
+    val_o, b_o = oracle.pressure_assemble(g, singular=True)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    dflux = gpu.DeviceArray(f.vflux)
+    faces.rhie_chow(gpu.RhieChow(rf), dflux)
+    assert np.array_equal(dflux.download(), flux_o)
+    d = {k: gpu.DeviceArray(getattr(g, k)) for k in ("fc", "fw", "vm")}
+    dval, db = gpu.DeviceArray(np.zeros(len(f.col))), gpu.DeviceArray(np.zeros(f.n))
+    faces.assemble(d["fc"], d["fw"], d["vm"], dflux, dval, db, singular=True)
+    assert np.array_equal(dval.download(), val_o) and np.array_equal(db.download(), b_o)
+
+
+def test_rhie_chow_validation(cases, gpu):
+    dims = (4, 3, 3)
+    f = cases.hex_faces(*dims)
+    rf = cases.rhie_chow_fields(f, *dims)
+    nat = gpu.Native(f.row, f.col, f.dia)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    rc = gpu.RhieChow(rf)
+    rc.s.choi = 1                                # its arrays were not given
+    with pytest.raises(gpu.TFlowsGpuError) as e:
+        faces.rhie_chow(rc, gpu.DeviceArray(np.zeros(f.n_faces)))
+    assert e.value.code == gpu.TFG_EINVAL
