@@ -1720,9 +1720,9 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     const bool dx = method == tfg::NS_NONE && !icc && (o.criterion == TFG_CRIT_RELRES || norm);
     const int chunk = h->nnz > (1 << 22) ? 8 : 32;
     // Graph path: chunks of CG iterations replayed as one hipGraph; the
-    // kernels take the iteration number from State::kcur (k = 0) and k_next
-    // advances it.  The executable graph is kept per handle for the same
-    // operands.
+    // kernels take the iteration number from State::kcur (k = 0), which the
+    // SpMV advances: three launches per iteration.  The executable graph is
+    // kept per handle for the same operands.
     const bool graph = method == tfg::NS_NONE && !icc && !timing &&
                        (o.path == TFG_PATH_GRAPH || (o.path == TFG_PATH_AUTO && h->nnz < (1 << 21)));
     auto enqueue_dev_iterations = [&](int count, bool finish) -> int {
@@ -1730,7 +1730,6 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
         HIP_TRY(tfg::launch_pupdate(L, a, pr, 0, dx));
         HIP_TRY(tfg::launch_spmv_pq(L, a, 0));
         HIP_TRY(tfg::launch_update(L, a, pr, 0, dx));
-        HIP_TRY(tfg::launch_next(L, a));
       }
       if (finish) HIP_TRY(tfg::launch_finish(L, a, 0, 0));
       return TFG_OK;

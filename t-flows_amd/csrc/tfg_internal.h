@@ -247,7 +247,6 @@ hipError_t launch_prologue(const Launch& L, const CgArgs& a, int prec);
 hipError_t launch_start(const Launch& L, const CgArgs& a, int rho_slot = 6);
 hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k, bool dx = false);
 hipError_t launch_fold_x(const Launch& L, const CgArgs& a);
-hipError_t launch_next(const Launch& L, const CgArgs& a);
 hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k);
 // Symmetric engine set-up for one val: sd from val, and *flag = 1 when some
 // lower entry is not bitwise equal to its mirror (then sd must not be used).

@@ -236,8 +236,10 @@ template <class E>
 __global__ __launch_bounds__(kBlock) void k_spmv_pq(CgArgs a, int k) {
   __shared__ typename E::Smem sm;
   __shared__ double s_tmp[4], s_out[1];
-  (void)k;
   if (a.st->done) return;
+  // graph mode (k = 0): the iteration counter moves on here; no workgroup
+  // of this kernel reads it, k_update reads kcur - 1
+  if (k == 0 && lead()) a.st->kcur = a.st->kcur + 1;
   const double fn = a.st->fn;
   double v[1] = {0.0};
   E::template run<false>(
@@ -258,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
   __shared__ double s_tmp[5 * 4], s_out[5];
   State* st = a.st;
   if (st->done) return;
-  if (k == 0) k = st->kcur;
+  if (k == 0) k = st->kcur - 1;  // the SpMV of this iteration advanced it
   {
     const int ops1[1] = {OP_SUM};
     gather<1>(a.pt.spmv, a.pt.pstride, a.pt.g_spmv, ops1, s_tmp, s_out);
@@ -360,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(CgArgs a, int k, int check_om
   __shared__ double s_tmp[5 * 4], s_out[5];
   State* st = a.st;
   if (st->done) return;
-  if (k == 0) k = st->kcur - 1;  // after k_next: the iteration just done
+  if (k == 0) k = st->kcur - 1;  // the iteration just done
   const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
   gather<5>(a.pt.upd, a.pt.pstride, a.pt.g_upd, ops, s_tmp, s_out);
   if (threadIdx.x != 0) return;
@@ -379,11 +381,6 @@ __global__ __launch_bounds__(kBlock) void k_finish(CgArgs a, int k, int check_om
     st->niter = k;
     st->done = 1;
   }
-}
-
-// Advances State::kcur: the last kernel of an iteration in a graph chunk.
-__global__ void k_next(State* st) {
-  if (threadIdx.x == 0) st->kcur = st->kcur + 1;
 }
 
 // The deferred x += alpha_K p_K of the last iteration that updated r.
@@ -440,8 +437,8 @@ template <int U>
 __global__ __launch_bounds__(kBlock) void k_spmv_pq_sym(CgArgs a, int k) {
   __shared__ SymEngine::Smem sm;
   __shared__ double s_tmp[4], s_out[1];
-  (void)k;
   if (a.st->done) return;
+  if (k == 0 && lead()) a.st->kcur = a.st->kcur + 1;  // as in k_spmv_pq
   const double fn = a.st->fn;
   double v[1] = {0.0};
   SymEngine::run<false, U>(
@@ -544,11 +541,6 @@ hipError_t launch_pupdate(const Launch& L, const CgArgs& a, int prec, int k, boo
     if (dx) k_pupdate<PREC_NONE, true><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
     else k_pupdate<PREC_NONE, false><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
   }
-  return hipGetLastError();
-}
-
-hipError_t launch_next(const Launch& L, const CgArgs& a) {
-  k_next<<<1, 64, 0, L.stream>>>(a.st);
   return hipGetLastError();
 }
 
