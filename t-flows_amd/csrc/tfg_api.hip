@@ -154,6 +154,7 @@ struct tfg_matrix {
   int* pslot = nullptr;
   int* sorder = nullptr;  // (start, end) row blocks in processing order
   int nsorder = 0;
+  tfg::SymDiag sdg{};     // symmetric engine geometry (offsets, stride, mode)
   int nsym = 0;        // upper diagonals; 0: no symmetric engine
   double* sym = nullptr;
   int* sym_flag = nullptr;
@@ -251,6 +252,7 @@ tfg::SpmvShape shape_of(const tfg_matrix* h, int kind) {
   m.pmax = h->pmax;
   m.sorder = h->sorder;
   m.nsorder = h->nsorder;
+  m.sdg = h->sdg;
   return m;
 }
 
@@ -274,6 +276,7 @@ tfg::CgArgs args_of(const tfg_matrix* h, int kind) {
   a.pmax = m.pmax;
   a.sorder = m.sorder;
   a.nsorder = m.nsorder;
+  a.sdg = m.sdg;
   return a;
 }
 
@@ -368,7 +371,10 @@ int sym_prepare(tfg_matrix* h, const double* val, bool* ok) {
   *ok = false;
   if (h->nsym <= 0) return TFG_OK;
   if (!h->sym) {
-    HIP_TRY(hipMalloc(&h->sym, sizeof(double) * (size_t)(h->nsym + 1) * (size_t)h->ni));
+    // zero once: entries a row lacks and the gaps between slots stay zero
+    const size_t bytes = sizeof(double) * (size_t)(h->nsym + 1) * (size_t)h->sdg.stride;
+    HIP_TRY(hipMalloc(&h->sym, bytes));
+    HIP_TRY(hipMemsetAsync(h->sym, 0, bytes, h->stream));
     HIP_TRY(hipMalloc(&h->sym_flag, sizeof(int)));
     HIP_TRY(hipHostMalloc(&h->hflag, sizeof(int), hipHostMallocDefault));
   }
@@ -1184,18 +1190,22 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
         h->npat = (int)seqs.size();
         for (const auto& q : seqs) h->pmax = std::max(h->pmax, (int)q.size());
         // Symmetric engine: no buffer cells, the diagonal first in every
-        // row, at most kSymMax distinct positive offsets, no repeated offset
-        // in a row, and every entry (i, i + d) has its mirror (i + d, i).
+        // row and the other columns ascending (the order the engine sums
+        // in), at most kSymMax distinct positive offsets, and every entry
+        // (i, i + d) has its mirror (i + d, i).
         std::vector<int> up;
         bool sym_ok = ni == nt;
         for (const auto& q : seqs) {
           if (q.empty() || q[0] != 0) sym_ok = false;  // diagonal first
+          for (size_t u = 2; u < q.size() && sym_ok; ++u)
+            if (!(q[u - 1] < q[u])) sym_ok = false;    // then ascending
           for (size_t u = 0; u < q.size() && sym_ok; ++u) {
             if (std::count(q.begin(), q.end(), q[u]) != 1) sym_ok = false;
             if (q[u] > 0 && std::find(up.begin(), up.end(), q[u]) == up.end()) up.push_back(q[u]);
           }
         }
         if (up.empty() || (int)up.size() > tfg::kSymMax) sym_ok = false;
+        std::sort(up.begin(), up.end());
         if (sym_ok) {
           // has[t][k]: pattern t holds offset -up[k]
           std::vector<char> has(seqs.size() * up.size(), 0);
@@ -1241,41 +1251,73 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
             e = hipMemcpy(h->pslot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice);
           if (e != hipSuccess) { rc = fail(TFG_EHIP, "slot upload: %s", hipGetErrorString(e)); break; }
           h->nsym = (int)up.size();
+          // Slot stride: n and then at least D + 2 zeros, even, so that the
+          // lower entries of the first rows read zeros.  Two rows per thread
+          // when the offsets are 1 and even ones and n is even.
+          const int D = up.back();
+          const long long gap = ((long long)D + 2 + 1) & ~1LL;
+          h->sdg = tfg::SymDiag{};
+          h->sdg.stride = ((long long)ni + gap + 1) & ~1LL;
+          h->sdg.k = h->nsym;
+          for (int k = 0; k < h->nsym; ++k) h->sdg.off[k] = up[k];
+          bool pair = up[0] == 1 && (ni % 2) == 0 && h->nsym <= tfg::kSymPairMax;
+          for (int k = 1; k < h->nsym; ++k) pair = pair && (up[k] % 2) == 0;
+          h->sdg.pair = pair ? 1 : 0;
           // Processing order of the symmetric engine: planes of D rows (D =
-          // the largest offset), blocks of at most 256 rows inside a plane,
-          // taken band by band (8 consecutive blocks of a plane) and, within
-          // a band, plane after plane.  The workgroups that run at the same
-          // time then cover one band over neighbouring planes, so most rows
-          // a row reads (one plane or one line away) are read by them too and
-          // stay in L2: 377 instead of 522 MB fetched per product at 200^3.
-          const int D = *std::max_element(up.begin(), up.end());
-          const int bpp = (D + tfg::kRowsPerBlock - 1) / tfg::kRowsPerBlock;
+          // the largest offset; plane starts rounded down to even rows),
+          // blocks of at most R rows (256, or 512 in pair mode) inside a
+          // plane, taken band by band (8 consecutive blocks of a plane) and,
+          // within a band, plane after plane.  The workgroups that run at
+          // the same time then cover one band over neighbouring planes, so
+          // most rows a row reads (one plane or one line away) are read by
+          // them too and stay in L2: 377 instead of 522 MB fetched per
+          // product at 200^3.
+          const int R = pair ? 2 * tfg::kRowsPerBlock : tfg::kRowsPerBlock;
+          const long long bpp = ((long long)D + 1 + R - 1) / R;  // +1: a plane may gain a row
           constexpr int kBand = 8;
           std::vector<int> ord;
-          if (D >= kBand * tfg::kRowsPerBlock) {
-            const int planes = (int)(((long long)ni + D - 1) / D);
-            for (int j0 = 0; j0 < bpp; j0 += kBand)
-              for (int x = 0; x < planes; ++x)
-                for (int b = j0; b < std::min(bpp, j0 + kBand); ++b) {
-                  const long long r0 = (long long)x * D + (long long)b * tfg::kRowsPerBlock;
-                  const long long r1 = std::min<long long>(
-                      {r0 + tfg::kRowsPerBlock, (long long)(x + 1) * D, (long long)ni});
+          if (D >= kBand * R) {
+            const long long planes = ((long long)ni + D - 1) / D;
+            auto start = [&](long long x) { return std::min<long long>(ni, (x * D) & ~1LL); };
+            for (long long j0 = 0; j0 < bpp; j0 += kBand)
+              for (long long x = 0; x < planes; ++x)
+                for (long long b = j0; b < std::min<long long>(bpp, j0 + kBand); ++b) {
+                  const long long r0 = start(x) + b * R;
+                  const long long r1 = std::min<long long>({r0 + R, start(x + 1), (long long)ni});
                   if (r0 < r1) {
                     ord.push_back((int)r0);
                     ord.push_back((int)r1);
                   }
                 }
           } else {
-            for (int r0 = 0; r0 < ni; r0 += tfg::kRowsPerBlock) {
+            for (int r0 = 0; r0 < ni; r0 += R) {
               ord.push_back(r0);
-              ord.push_back(std::min(ni, r0 + tfg::kRowsPerBlock));
+              ord.push_back(std::min(ni, r0 + R));
             }
           }
-          e = hipMalloc(&h->sorder, sizeof(int) * ord.size());
-          if (e == hipSuccess)
-            e = hipMemcpy(h->sorder, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice);
-          if (e != hipSuccess) { rc = fail(TFG_EHIP, "order upload: %s", hipGetErrorString(e)); break; }
-          h->nsorder = (int)ord.size() / 2;
+          // every row once; pair mode: even starts and lengths
+          {
+            std::vector<char> seen((size_t)ni, 0);
+            bool cover = true;
+            for (size_t t = 0; t + 1 < ord.size() && cover; t += 2) {
+              if (pair && ((ord[t] | (ord[t + 1] - ord[t])) & 1)) cover = false;
+              for (int r = ord[t]; r < ord[t + 1] && cover; ++r) {
+                if (seen[r]) cover = false;
+                seen[r] = 1;
+              }
+            }
+            for (int r = 0; r < ni && cover; ++r) cover = seen[r] != 0;
+            // (not expected) no symmetric engine; the kernels address a
+            // slot with 32-bit byte offsets
+            if (!cover || h->sdg.stride >= (1LL << 29)) h->nsym = 0;
+          }
+          if (h->nsym > 0) {
+            e = hipMalloc(&h->sorder, sizeof(int) * ord.size());
+            if (e == hipSuccess)
+              e = hipMemcpy(h->sorder, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) { rc = fail(TFG_EHIP, "order upload: %s", hipGetErrorString(e)); break; }
+            h->nsorder = (int)ord.size() / 2;
+          }
         }
       }
     }

@@ -18,6 +18,11 @@ constexpr int kCap2 = 4088;
 // entries.
 constexpr int kPatCount = 64;
 constexpr int kPatMax = 12;
+// Distinct positive column offsets the symmetric engine stores (upper
+// diagonals; a 7-point hex grid has 3), and the most it takes two rows per
+// thread with.
+constexpr int kSymMax = 8;
+constexpr int kSymPairMax = 4;
 constexpr int kXcds = 8;           // MI355X: 8 XCDs, workgroups dealt round robin
 constexpr double kHuge = 1.0e+15;  // Const_Mod.f90:84-93 (HUGE, TINY)
 constexpr double kTiny = 1.0e-15;
@@ -69,6 +74,15 @@ struct Partials {
   int g_upd;        // grid of the kernel that produces upd (method dependent)
 };
 
+// Symmetric engine geometry (by value in the kernel arguments).
+struct SymDiag {
+  long long stride;   // slot stride of sd, even
+  int k;              // upper diagonals, 1..kSymMax
+  int pair;           // 1: two rows per thread (d_0 = 1, other d_k even, n even,
+                      //    k <= kSymPairMax; blocks of 512 rows at even starts)
+  int off[kSymMax];   // d_0 < d_1 < ... < d_{k-1}
+};
+
 // Everything a CG phase kernel needs.  Device pointers.
 struct CgArgs {
   int n;               // rows solved (ni)
@@ -87,6 +101,7 @@ struct CgArgs {
   const int* sorder;   // symmetric engine: (start, end) row blocks in processing order
   int nsorder;
   const double* sd;    // symmetric engine: diagonal and upper diagonals
+  SymDiag sdg;         // symmetric engine: offsets, slot stride, row pairing
   const double* val;   // unscaled values (caller's)
   const double* b;     // unscaled right-hand side (caller's)
   double* x;
@@ -112,14 +127,17 @@ struct SpmvShape {
   const int* __restrict__ poff;      // kPatCount x kPatMax column offsets
   const int* __restrict__ plen;      // kPatCount pattern lengths
   // symmetric engine: sd holds the diagonal (sd[i]) and the upper diagonals
-  // (sd[(k + 1) n + i] = a(i, i + d_k), d_k > 0); pslot gives per pattern
-  // entry where its value lives: 0 diagonal, k + 1 the row's own upper slot
-  // k, -(k + 1) slot k of row i + offset (a lower entry read from its mirror)
+  // (sd[(k + 1) S + i] = a(i, i + d_k), 0 < d_0 < d_1 < ..., zero where row
+  // i has no such entry; S = sdg.stride leaves the largest d_k + 2 zeros after
+  // every slot); pslot gives per pattern entry its slot: 0 diagonal, k + 1
+  // the row's own upper slot k, -(k + 1) a lower entry (slot k of row
+  // i + offset), for the set-up kernel
   const int* __restrict__ pslot;
   const double* __restrict__ sd;
   int pmax;                          // longest row pattern
   const int* __restrict__ sorder;    // symmetric engine: (start, end) row blocks
   int nsorder;                       //   in processing order, and their count
+  SymDiag sdg;
 };
 
 // Textbook methods for non-symmetric systems (tfg_nonsym.hip).
@@ -140,10 +158,6 @@ struct NsArgs {
 enum SpmvKind { K_STAGE2048 = 0, K_STAGE4096 = 1, K_WAVE4 = 2, K_WAVE8 = 3, K_WAVE16 = 4,
                 K_PIPE2048 = 5, K_PIPE2048_CODED = 6, K_PIPE_PATTERN = 7, K_SYM_DIA = 8,
                 kKinds = 9 };
-// Distinct positive column offsets the symmetric engine stores (upper
-// diagonals); a 7-point hex grid has 3.
-constexpr int kSymMax = 8;
-
 // FP32 sweeps of iterative refinement (FP32 vectors, FP64 scalars and
 // reductions).  Reuses the FP64 solve's partial buffers (spmv, upd).
 struct IrArgs {

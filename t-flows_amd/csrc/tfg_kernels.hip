@@ -21,6 +21,7 @@
 // products differs (deterministic trees here).
 #include "tfg_spmv.h"
 #include <algorithm>
+#include <type_traits>
 
 namespace tfg {
 
@@ -58,6 +59,7 @@ __device__ __forceinline__ SpmvShape shape_of(const CgArgs& a) {
   m.pmax = a.pmax;
   m.sorder = a.sorder;
   m.nsorder = a.nsorder;
+  m.sdg = a.sdg;
   return m;
 }
 
@@ -431,65 +433,109 @@ __global__ __launch_bounds__(kBlock) void k_spmv_plain(SpmvShape m, const double
       [&](int i, double yi, double, double) { y[i] = yi; });
 }
 
-// q = (A*fn) p and the partials of (p, q) with the symmetric engine: the
-// same products as k_spmv_pq, from the diagonal and upper diagonals.
-template <int U>
-__global__ __launch_bounds__(kBlock) void k_spmv_pq_sym(CgArgs a, int k) {
-  __shared__ SymEngine::Smem sm;
+// q = (A*fn) p and the partials of (p, q) with the symmetric engine (K
+// upper diagonals, mode as in SymEngine): the same products as k_spmv_pq.
+// Six waves per SIMD leave the pair mode the registers to have all its
+// loads in flight at once (at eight it waits for the first ones midway).
+template <int K, int MODE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 6)))
+void k_spmv_pq_sym(CgArgs a, int k) {
   __shared__ double s_tmp[4], s_out[1];
   if (a.st->done) return;
   if (k == 0 && lead()) a.st->kcur = a.st->kcur + 1;  // as in k_spmv_pq
   const double fn = a.st->fn;
   double v[1] = {0.0};
-  SymEngine::run<false, U>(
-      sm, shape_of(a), fn, a.p, [](int) { return 0.0; },
+  SymEngine::run<false, K, MODE>(
+      shape_of(a), fn, a.p, [](int) { return 0.0; },
       [&](int i, double qi, double pi, double, bool own) {
         a.q[i] = qi;
-        if (own) v[0] = v[0] + pi * qi;
+        v[0] = v[0] + (own ? pi * qi : 0.0);
+      },
+      [&](int i, double q0, double q1, double p0, double p1, bool own) {
+        dbl2 o;
+        o.x = q0;
+        o.y = q1;
+        *(dbl2*)(a.q + i) = o;
+        v[0] = v[0] + (own ? p0 * q0 : 0.0);
+        v[0] = v[0] + (own ? p1 * q1 : 0.0);
       });
   const int ops[1] = {OP_SUM};
   publish<1>(v, ops, a.pt.spmv, a.pt.pstride, s_tmp, s_out);
 }
 
-template <int U>
-__global__ __launch_bounds__(kBlock) void k_spmv_plain_sym(SpmvShape m, const double* __restrict__ x,
-                                                           double* __restrict__ y) {
-  __shared__ SymEngine::Smem sm;
-  SymEngine::run<false, U>(
-      sm, m, 1.0, x, [](int) { return 0.0; },
-      [&](int i, double yi, double, double, bool) { y[i] = yi; });
+template <int K, int MODE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 6)))
+void k_spmv_plain_sym(SpmvShape m, const double* __restrict__ x, double* __restrict__ y) {
+  SymEngine::run<false, K, MODE>(
+      m, 1.0, x, [](int) { return 0.0; },
+      [&](int i, double yi, double, double, bool) { y[i] = yi; },
+      [&](int i, double y0, double y1, double, double, bool) {
+        dbl2 o;
+        o.x = y0;
+        o.y = y1;
+        *(dbl2*)(y + i) = o;
+      });
 }
 
 // Set-up of the symmetric engine for one val, one thread per row: the
-// diagonal and upper entries go to their slots; every lower entry a(i, j)
-// is compared bit for bit with a(j, i), found in row j's pattern (the
-// structure was checked symmetric when the handle was made), and any
-// difference sets *flag.
+// diagonal and upper entries go to their slots (entries a row lacks stay
+// zero from the allocation); every lower entry a(i, j) is compared bit for
+// bit with a(j, i), found in row j's pattern (the structure was checked
+// symmetric when the handle was made), and any difference sets *flag.
 __global__ __launch_bounds__(kBlock) void k_sym_pack(SpmvShape m, const double* __restrict__ val,
                                                      double* __restrict__ sd, int* flag) {
-  __shared__ SymEngine::Smem sm;
-  SymEngine::tables(sm, m);
+  __shared__ int s_off[kPatCount * kPatMax], s_slot[kPatCount * kPatMax], s_len[kPatCount];
+  for (int t = threadIdx.x; t < kPatCount * kPatMax; t += kBlock) {
+    s_off[t] = m.poff[t];
+    s_slot[t] = m.pslot[t];
+  }
+  for (int t = threadIdx.x; t < kPatCount; t += kBlock) s_len[t] = m.plen[t];
+  __syncthreads();
+  const long long S = m.sdg.stride;
   bool bad = false;
   const int stride = gridDim.x * kBlock;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < m.n; i += stride) {
     const int p = m.pat[i];
-    const int L = sm.len[p];
+    const int L = s_len[p];
     const int rs = m.row[i];
     for (int u = 0; u < L; ++u) {
-      const int off = sm.off[p * kPatMax + u];
+      const int off = s_off[p * kPatMax + u];
       const double v = val[rs + u];
       if (off >= 0) {
-        sd[sm.delta[p * kPatMax + u] + i] = v;  // diagonal or own upper slot
+        sd[s_slot[p * kPatMax + u] * S + i] = v;  // diagonal (slot 0) or own upper slot
       } else {
-        const int j = i + off, pj = m.pat[j], Lj = sm.len[pj];
+        const int j = i + off, pj = m.pat[j], Lj = s_len[pj];
         int mu = -1;
         for (int t = 0; t < Lj; ++t)
-          if (sm.off[pj * kPatMax + t] == -off) mu = t;
+          if (s_off[pj * kPatMax + t] == -off) mu = t;
         if (mu < 0 || __double_as_longlong(val[m.row[j] + mu]) != __double_as_longlong(v)) bad = true;
       }
     }
   }
   if (bad) atomicOr(flag, 1);
+}
+
+// Symmetric-engine instantiation for the runtime K: f(integral_constant<K>).
+template <typename F>
+hipError_t sym_dispatch(int K, F f) {
+  switch (K) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Mode of a symmetric product (SymEngine): pairs need 16-byte aligned x, y.
+int sym_mode(const SymDiag& g, const void* x, const void* y) {
+  if (!g.pair) return 0;
+  return (((uintptr_t)x | (uintptr_t)y) % 16) == 0 ? 1 : 2;
 }
 
 // ---------------------------------------------------------------------------
@@ -508,7 +554,7 @@ int spmv_blocks_per_cu(int kind) {
     case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, true, double>>, kBlock, 0); break;
     case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<StreamEngine<kCap, true, double, true>>, kBlock, 0); break;
     case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<PatternEngine<kCap, true, double>>, kBlock, 0); break;
-    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq_sym<7>, kBlock, 0); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq_sym<3, 1>, kBlock, 0); break;
     default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spmv_pq<WaveEngine<16, double>>, kBlock, 0); break;
   }
   if (e != hipSuccess || nb < 1) return 1;
@@ -551,10 +597,17 @@ hipError_t launch_fold_x(const Launch& L, const CgArgs& a) {
 
 hipError_t launch_spmv_pq(const Launch& L, const CgArgs& a, int k) {
   if (L.sym) {
-    if (a.pmax <= 7) k_spmv_pq_sym<7><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
-    else if (a.pmax <= 8) k_spmv_pq_sym<8><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
-    else k_spmv_pq_sym<kPatMax><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
-    return hipGetLastError();
+    const int mode = sym_mode(a.sdg, a.p, a.q);
+    return sym_dispatch(a.sdg.k, [&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      if constexpr (K <= kSymPairMax)
+        if (mode == 1) {
+          k_spmv_pq_sym<K, 1><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
+          return;
+        }
+      if (mode == 2) k_spmv_pq_sym<K, 2><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
+      else k_spmv_pq_sym<K, 0><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k);
+    });
   }
   TFG_ENGINE_SWITCH(L, (k_spmv_pq<E><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, k)));
   return hipGetLastError();
@@ -600,10 +653,17 @@ hipError_t launch_copy16(hipStream_t stream, int grid, const void* src, void* ds
 hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* val,
                              const double* p, double* q) {
   if (L.sym) {
-    if (m.pmax <= 7) k_spmv_plain_sym<7><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
-    else if (m.pmax <= 8) k_spmv_plain_sym<8><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
-    else k_spmv_plain_sym<kPatMax><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
-    return hipGetLastError();
+    const int mode = sym_mode(m.sdg, p, q);
+    return sym_dispatch(m.sdg.k, [&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      if constexpr (K <= kSymPairMax)
+        if (mode == 1) {
+          k_spmv_plain_sym<K, 1><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
+          return;
+        }
+      if (mode == 2) k_spmv_plain_sym<K, 2><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
+      else k_spmv_plain_sym<K, 0><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, p, q);
+    });
   }
   TFG_ENGINE_SWITCH(L, (k_spmv_plain<E, false><<<L.grid_spmv, kBlock, 0, L.stream>>>(m, val, nullptr, p, q)));
   return hipGetLastError();

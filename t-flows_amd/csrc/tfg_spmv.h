@@ -373,114 +373,162 @@ struct PatternEngine {
   }
 };
 
-// Symmetric matrices on a row pattern: a(i, j) = a(j, i) bit for bit, so a
-// row's lower entries are the upper entries of earlier rows.  The values
-// are held once, as the diagonal and kSymMax or fewer upper diagonals
-// (one per distinct positive column offset), each n long: per product 8 B
-// per row and per upper entry instead of 8 B per nonzero.  One row per
-// thread over blocks of at most 256 rows taken in a plane-interleaved order
-// (built with the tables on the host); entry u of a row
-// with pattern p has column i + off[p][u] and its value in slot[p][u]:
-//   0      diagonal          sd[i]
-//   k + 1  own upper entry   sd[(k + 1) n + i]
-//   -(k+1) lower entry       sd[(k + 1) n + i + off]  (row i+off's upper)
-// The terms are summed in the row's CSR order with the value scaled as the
-// stored one is, so products are bit-identical to the CSR engines'.  The
-// epilogue takes a fifth argument: whether the lane's row is its own (a
-// lane past the block's rows repeats the last row and must not count it
-// twice in a reduction).
+// Symmetric matrices whose entries lie on a few diagonals: a(i, j) =
+// a(j, i) bit for bit, and every column offset j - i is 0 or +-d_k for one
+// set 0 < d_0 < ... < d_{K-1} (K <= kSymMax).  The values are held once, as
+// the diagonal and the K upper diagonals, each zero where a row has no such
+// entry (SpmvShape::sd): per product 8 B per row and per upper diagonal
+// instead of 8 B per nonzero, and no per-row structure at all.  Row i sums
+//   a_ii x_i,
+//   a(i, i - d_k) x_{i - d_k} for k = K-1 .. 0 (the mirror: slot k of row
+//     i - d_k),
+//   a(i, i + d_k) x_{i + d_k} for k = 0 .. K-1,
+// which is the CSR order (diagonal first, then columns ascending; checked
+// when the handle is made), with each value scaled as the stored one is.  A
+// missing entry adds a zero product and leaves the sum as it was, so for
+// finite x the products are bit-identical to the CSR engines' (the x index
+// of a missing entry is clamped into the vector).  Row blocks come in a
+// plane-interleaved order built on the host (sorder).
+// Modes: 0 one row per thread, blocks of at most 256 rows; 1 two rows per
+// thread (d_0 = 1, the other offsets even, n even; blocks of at most 512
+// rows at even starts): rows i and i + 1 (i even) read 16-byte pairs, the
+// +-1 neighbours from the pairs at i - 2 and i + 2 and every other term
+// from one aligned pair; 2 one row per thread over mode 1's blocks (x or y
+// not 16-byte aligned).  The epilogue takes a fifth argument: whether the
+// lane's row is its own (a lane past the block's rows repeats the last row
+// and must not count it twice in a reduction).
 struct SymEngine {
-  // per pattern entry: column offset, and where its value sits relative to
-  // sd + i (delta = slot * n for the row's own entries, |slot| * n + offset
-  // for a lower entry), so that every term is two plain loads
-  struct Smem {
-    long long delta[kPatCount * kPatMax];
-    int off[kPatCount * kPatMax];
-    int len[kPatCount];
-  };
-
-  __device__ __forceinline__ static void tables(Smem& sm, const SpmvShape& m) {
-    const long long n = m.n;
-    for (int t = threadIdx.x; t < kPatCount * kPatMax; t += kBlock) {
-      const int off = m.poff[t], sl = m.pslot[t];
-      sm.off[t] = off;
-      sm.delta[t] = sl >= 0 ? sl * n : -sl * n + off;
-    }
-    for (int t = threadIdx.x; t < kPatCount; t += kBlock) sm.len[t] = m.plen[t];
-    __syncthreads();
-  }
-
-  // Block bounds and this lane's pattern id of row block lb (lanes past the
-  // block's rows take its last row, so every load stays in bounds).
-  struct Next {
-    int r0, r1, p;
-  };
-  __device__ __forceinline__ static Next fetch(const SpmvShape& m, int lb) {
+  __device__ __forceinline__ static int2 ids(const SpmvShape& m, int lb) {
     typedef __attribute__((address_space(4))) const int cint;
-    Next g;
-    g.r0 = ((cint*)m.sorder)[2 * lb];
-    g.r1 = ((cint*)m.sorder)[2 * lb + 1];
-    g.p = m.pat[min(g.r0 + (int)threadIdx.x, g.r1 - 1)];
-    return g;
+    return make_int2(((cint*)m.sorder)[2 * lb], ((cint*)m.sorder)[2 * lb + 1]);
   }
 
-  // The next block's bounds and pattern ids are loaded after this block's
-  // value and x loads are issued, so they are in flight during its sums
-  // (vector loads complete in issue order): one memory round trip per
-  // block.  Entry 0 of every row is its diagonal (checked on the host), read
-  // with a non-temporal load since no other row uses it.  U terms (the
-  // longest row rounded up to 7, 8 or kPatMax: every load instruction costs
-  // load-pipeline time, so none is issued beyond the longest row) are
-  // loaded unconditionally, entries past the row's length repeating its
-  // last one, so no loop has a variable trip count and the compiler's waits
-  // stay exact.
-  template <bool SUB, int U, typename Init, typename Epi>
-  __device__ __forceinline__ static void step(Smem& sm, const SpmvShape& m, double fn,
-                                              const double* x, const XcdWalk& w, int lb, Next& g,
-                                              Init& init, Epi& epi) {
-    typedef __attribute__((address_space(1))) const double gT;
-    gT* xg = (gT*)x;
-    gT* sd = (gT*)m.sd;
-    asm volatile("" : "+s"(xg));
-    const int tid = threadIdx.x;
-    const int r0 = g.r0, r1 = g.r1, p = g.p;
-    const int i = min(r0 + tid, r1 - 1);
-    const int L = sm.len[p];
-    const int* po = sm.off + p * kPatMax;
-    const long long* pd = sm.delta + p * kPatMax;
-    double av[U], xv[U];
-    av[0] = __builtin_nontemporal_load(m.sd + i);
-    xv[0] = xg[i];
-#pragma unroll
-    for (int u = 1; u < U; ++u) {
-      const int uu = min(u, L - 1);
-      av[u] = sd[pd[uu] + i];
-      xv[u] = xg[i + po[uu]];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    g = fetch(m, min(lb + w.step, w.end - 1));
-    __builtin_amdgcn_sched_barrier(0);
-    double acc = init(i);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const double a = av[u] * fn;
-      if (u < L) acc = SUB ? acc - a * xv[u] : acc + a * xv[u];
-    }
-    epi(i, acc, xv[0], av[0] * fn, r0 + tid < r1);
+  template <bool SUB>
+  __device__ __forceinline__ static double term(double acc, double v, double fn, double x) {
+    const double a = v * fn;
+    return SUB ? acc - a * x : acc + a * x;
   }
 
-  template <bool SUB, int U, typename Init, typename Epi>
-  __device__ __forceinline__ static void run(Smem& sm, const SpmvShape& m, double fn,
-                                             const double* x, Init init, Epi epi) {
-    tables(sm, m);
+  // Loads at a workgroup-uniform base plus a 32-bit byte offset (one VGPR
+  // per address; sdg.stride < 2^29 is checked on the host).
+  typedef __attribute__((address_space(1))) const char gB;
+  template <typename T>
+  __device__ __forceinline__ static T at(gB* base, unsigned byte) {
+    return *(__attribute__((address_space(1))) const T*)(base + byte);
+  }
+  template <typename T>  // non-temporal: no other row reads it
+  __device__ __forceinline__ static T at_nt(gB* base, unsigned byte) {
+    return __builtin_nontemporal_load((__attribute__((address_space(1))) const T*)(base + byte));
+  }
+
+  // rows r0 + h * kBlock + tid, h < H, of block [r0, r1)
+  template <bool SUB, int K, int H, typename Init, typename Epi>
+  __device__ __forceinline__ static void rows(const SpmvShape& m, double fn, const double* x,
+                                              int r0, int r1, Init& init, Epi& epi) {
+    gB* xb = (gB*)x;
+    gB* sb = (gB*)m.sd;
+    const long long S8 = 8 * m.sdg.stride;
+    const unsigned lastb = 8u * (unsigned)(m.n - 1);
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const int t = r0 + h * kBlock + (int)threadIdx.x;
+      const int i = min(t, r1 - 1);
+      const unsigned ib = 8u * (unsigned)i;
+      double lo[K], up[K], xl[K], xu[K];
+      const double d = at_nt<double>(sb, ib);
+      const double x0 = at<double>(xb, ib);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const unsigned ob = 8u * (unsigned)m.sdg.off[k];
+        gB* slot = sb + (k + 1) * S8;
+        up[k] = at<double>(slot, ib);
+        lo[k] = at<double>(slot - ob, ib);  // in the zeros after slot k - 1 for i < d_k
+        xu[k] = at<double>(xb, min(ib + ob, lastb));
+        xl[k] = at<double>(xb, max(ib, ob) - ob);
+      }
+      double acc = init(i);
+      acc = term<SUB>(acc, d, fn, x0);
+#pragma unroll
+      for (int k = K - 1; k >= 0; --k) acc = term<SUB>(acc, lo[k], fn, xl[k]);
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc = term<SUB>(acc, up[k], fn, xu[k]);
+      epi(i, acc, x0, d * fn, t < r1);
+    }
+  }
+
+  // rows i, i + 1 (i = r0 + 2 tid, even) of block [r0, r1), r1 - r0 even;
+  // epi2(i, y_i, y_{i+1}, x_i, x_{i+1}, own)
+  template <bool SUB, int K, typename Init, typename Epi2>
+  __device__ __forceinline__ static void pairs(const SpmvShape& m, double fn, const double* x,
+                                               int r0, int r1, Init& init, Epi2& epi2) {
+    gB* xb = (gB*)x;
+    gB* sb = (gB*)m.sd;
+    const long long S8 = 8 * m.sdg.stride;
+    const int t = r0 + 2 * (int)threadIdx.x;
+    const int i = min(t, r1 - 2);
+    const unsigned ib = 8u * (unsigned)i;          // 16-byte aligned
+    const unsigned lastb = 8u * (unsigned)(m.n - 2);
+    // every offset first, then all loads: an address register written
+    // between the loads can land on a pending load's destination and make
+    // the compiler wait for it
+    unsigned ou[K], ol[K];
+    ou[0] = min(ib + 16, lastb);
+    ol[0] = max(ib, 16u) - 16;
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+      const unsigned ob = 8u * (unsigned)m.sdg.off[k];
+      ou[k] = min(ib + ob, lastb);
+      ol[k] = max(ib, ob) - ob;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    dbl2 lo[K], up[K], xl[K], xu[K];
+    const dbl2 d = at_nt<dbl2>(sb, ib);
+    const dbl2 x0 = at<dbl2>(xb, ib);
+    up[0] = at<dbl2>(sb + S8, ib);        // a(i, i+1), a(i+1, i+2)
+    lo[0] = at<dbl2>(sb + S8 - 16, ib);   // .y = a(i-1, i)
+    xu[0] = at<dbl2>(xb, ou[0]);          // .x = x_{i+2}
+    xl[0] = at<dbl2>(xb, ol[0]);          // .y = x_{i-1}
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+      const unsigned ob = 8u * (unsigned)m.sdg.off[k];
+      gB* slot = sb + (k + 1) * S8;
+      up[k] = at<dbl2>(slot, ib);
+      lo[k] = at<dbl2>(slot - ob, ib);
+      xu[k] = at<dbl2>(xb, ou[k]);
+      xl[k] = at<dbl2>(xb, ol[k]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double a0 = init(i);
+    a0 = term<SUB>(a0, d.x, fn, x0.x);
+#pragma unroll
+    for (int k = K - 1; k >= 1; --k) a0 = term<SUB>(a0, lo[k].x, fn, xl[k].x);
+    a0 = term<SUB>(a0, lo[0].y, fn, xl[0].y);
+    a0 = term<SUB>(a0, up[0].x, fn, x0.y);
+#pragma unroll
+    for (int k = 1; k < K; ++k) a0 = term<SUB>(a0, up[k].x, fn, xu[k].x);
+    double a1 = init(i + 1);
+    a1 = term<SUB>(a1, d.y, fn, x0.y);
+#pragma unroll
+    for (int k = K - 1; k >= 1; --k) a1 = term<SUB>(a1, lo[k].y, fn, xl[k].y);
+    a1 = term<SUB>(a1, up[0].x, fn, x0.x);
+    a1 = term<SUB>(a1, up[0].y, fn, xu[0].x);
+#pragma unroll
+    for (int k = 1; k < K; ++k) a1 = term<SUB>(a1, up[k].y, fn, xu[k].y);
+    epi2(i, a0, a1, x0.x, x0.y, t < r1);
+  }
+
+  template <bool SUB, int K, int MODE, typename Init, typename Epi, typename Epi2>
+  __device__ __forceinline__ static void run(const SpmvShape& m, double fn, const double* x,
+                                             Init init, Epi epi, Epi2 epi2) {
     const XcdWalk w(m.nsorder);
     if (w.first >= w.end) return;
-    Next g = fetch(m, w.first);
-    // first block peeled: the loop is entered, like its back edge, with the
-    // next ids and this block's store in flight
-    step<SUB, U>(sm, m, fn, x, w, w.first, g, init, epi);
-    for (int lb = w.first + w.step; lb < w.end; lb += w.step)
-      step<SUB, U>(sm, m, fn, x, w, lb, g, init, epi);
+    int2 b = ids(m, w.first);
+    for (int lb = w.first; lb < w.end; lb += w.step) {
+      const int2 nb = ids(m, min(lb + w.step, w.end - 1));  // in flight during this block
+      if constexpr (MODE == 1) pairs<SUB, K>(m, fn, x, b.x, b.y, init, epi2);
+      else rows<SUB, K, MODE == 2 ? 2 : 1>(m, fn, x, b.x, b.y, init, epi);
+      b = nb;
+    }
   }
 };
 

@@ -96,6 +96,46 @@ def test_symmetric_engine(cases, oracle, gpu):
     assert np.array_equal(dq_u.download(), oracle.spmv(oracle.Matrix(up.row, up.col, up.dia, up.val), pu))
 
 
+# grid, whether two rows per thread apply (offsets 1 and even ones, n even)
+SYM_GRIDS = [((24, 20, 16), True), ((1, 30, 20), True), ((10, 8, 5), False), ((9, 7, 5), False)]
+
+
+@pytest.mark.parametrize("dims,pair", SYM_GRIDS, ids=[f"{d[0]}x{d[1]}x{d[2]}" for d, _ in SYM_GRIDS])
+def test_symmetric_engine_modes(cases, oracle, gpu, dims, pair):
+    """Engine 8 takes two rows per thread when the column offsets are 1 and
+    even ones and n is even (24x20x16: 1, 16, 320; 1x30x20: 1, 20), and one
+    row per thread otherwise (10x8x5: offset 5; 9x7x5: n odd); with x or y
+    not 16-byte aligned a pair matrix runs one row per thread over the same
+    blocks.  Every mode gives the CSR product bit for bit, and a CG solve
+    on it meets the parity bar against the oracle."""
+    s = cases.hex_pressure(*dims, rho_out=100.0)
+    offs = sorted({int(c) - 1 - i for i in range(s.n)
+                   for c in s.col[s.row[i] - 1:s.row[i + 1] - 1] if int(c) - 1 > i})
+    assert pair == (offs[0] == 1 and all(o % 2 == 0 for o in offs[1:]) and s.n % 2 == 0)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    assert nat.info()["sym_diagonals"] == len(offs) and 8 in nat.kinds()
+    nat.spmv_select(8)
+    p = np.random.default_rng(11).standard_normal(s.n)
+    ref = oracle.spmv(oracle.Matrix(s.row, s.col, s.dia, s.val), p)
+    dv = gpu.DeviceArray(s.val)
+    dq = gpu.DeviceArray(np.zeros(s.n))
+    nat.spmv(dv, gpu.DeviceArray(p), dq)
+    assert nat.info()["sym_state"] == 1
+    assert np.array_equal(dq.download(), ref)
+    bp = gpu.DeviceArray(np.concatenate([[0.0], p]))
+    bq = gpu.DeviceArray(np.zeros(s.n + 1))
+    nat.spmv(dv, _View(bp, 8), _View(bq, 8))
+    assert np.array_equal(bq.download()[1:], ref)
+    A = oracle.Matrix(s.row, s.col, s.dia, s.val)
+    x_o = np.zeros(s.n)
+    it_o, _ = oracle.solve("cg", "jacobi", A, x_o, s.b.copy(), 2000, 1e-9)
+    x = np.zeros(s.n)
+    it, _ = nat.solve_native("cg", "jacobi", s.val.copy(), x, s.b.copy(), 2000, 1e-9)
+    assert nat.info()["sym_state"] == 1
+    assert abs(it - it_o) <= 2
+    assert np.linalg.norm(x - x_o) <= 1e-10 * np.linalg.norm(x_o)
+
+
 def test_spmv_misaligned_values(cases, oracle, gpu):
     """val not 16-byte aligned: the scalar staging path must give the same bits."""
     s = cases.hex_pressure(30, 20, 10)
