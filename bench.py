@@ -80,16 +80,16 @@ def cpu_baseline(s, gpu_iters, seconds_budget=15.0):
 TRAFFIC = os.path.join(ROOT, "profiles", "spmv_traffic_twophase200.json")
 
 
-def format_bytes(kind, nnz, n):
+def format_bytes(kind, nnz, n, sym_diagonals=0):
     """Bytes one SpMV must move in the matrix format the engine streams (the
     CSR formula counts int32 columns and row pointers; the coded engine
     streams a 1-byte offset code per nonzero instead of the column, the
     row-pattern engine a 1-byte pattern id per row instead of columns and row
-    pointers, the symmetric engine that id plus the diagonal and the upper
-    entries only: 8 B per row and per upper entry, 4 nnz + 4 n).  All add
-    8 B per row of p read and 8 B of q written."""
+    pointers, the symmetric engine the diagonal and its K upper diagonals,
+    zero-padded: 8 (K + 1) n).  All add 8 B per row of p read and 8 B of q
+    written."""
     if kind == 8:
-        return 4 * nnz + 21 * n
+        return 8 * (sym_diagonals + 1) * n + 16 * n
     if kind == 7:
         return 8 * nnz + 17 * n
     if kind == 6:
@@ -723,7 +723,7 @@ def main():
     spmv_bytes = 12 * s.nnz + 20 * s.n + 4
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     kind = nat.info()["spmv_kind"]
-    fmt_bytes = format_bytes(kind, s.nnz, s.n)
+    fmt_bytes = format_bytes(kind, s.nnz, s.n, nat.info()["sym_diagonals"])
     total = float(np.sum(times))
     ms_per_step = total / args.steps * 1e3
     iter_bytes = spmv_bytes + 88 * s.n
