@@ -131,6 +131,9 @@ struct tfg_matrix {
   double* icc_dinv = nullptr;
   double* icc_y = nullptr;
   double* icc_z = nullptr;
+  bool box = false;               // box grid: tiled DIC sweeps (bx, box_toff)
+  tfg::BoxIcc bx{};
+  std::vector<int> box_toff;
   // graph path: executable chunk of CG iterations and the operands it holds
   hipGraphExec_t cg_graph = nullptr;
   GraphKey cg_key{};
@@ -323,7 +326,9 @@ void free_all(tfg_matrix* h) {
                  h->nv[3], h->valt, h->tpos, h->code, h->tab, h->pat, h->poff, h->plen,
                  h->pslot, h->sym, h->sym_flag, h->sorder,
                  h->loc, h->icc_rows[0],
-                 h->icc_rows[1], h->icc_d, h->icc_dinv, h->icc_y, h->icc_z};
+                 h->icc_rows[1], h->icc_d, h->icc_dinv, h->icc_y, h->icc_z,
+                 (void*)h->bx.tiles, h->bx.dg, h->bx.lo[0], h->bx.lo[1],
+                 h->bx.lo[2], h->bx.up[0], h->bx.up[1], h->bx.up[2]};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   if (h->hst) (void)hipHostFree(h->hst);
@@ -493,6 +498,13 @@ int ensure_icc(tfg_matrix* h) {
   const size_t n = (size_t)h->nt;
   for (double** v : {&h->icc_d, &h->icc_dinv, &h->icc_y, &h->icc_z})
     if (!*v) HIP_TRY(hipMalloc(v, n * sizeof(double)));
+  if (h->box && !h->bx.dg) {  // zero once: entries a row lacks stay zero
+    for (double** v : {&h->bx.dg, &h->bx.lo[0], &h->bx.lo[1], &h->bx.lo[2], &h->bx.up[0],
+                       &h->bx.up[1], &h->bx.up[2]}) {
+      HIP_TRY(hipMalloc(v, n * sizeof(double)));
+      HIP_TRY(hipMemsetAsync(*v, 0, n * sizeof(double), h->stream));
+    }
+  }
   return TFG_OK;
 }
 
@@ -510,7 +522,85 @@ tfg::IccArgs icc_args(const tfg_matrix* h) {
   c.nlev_b = (int)h->icc_off[1].size() - 1;
   c.off_f_host = h->icc_off[0].data();
   c.off_b_host = h->icc_off[1].data();
+  c.box = h->box ? 1 : 0;
+  c.bx = h->bx;
+  c.toff_host = h->box_toff.data();
   return c;
+}
+
+// Box grid of the tiled DIC sweeps: rows (x, y, z) of an nx*ny*nz grid, x
+// slowest, every row stored diagonal first and then by ascending column,
+// every off-diagonal entry a box neighbour (offsets 1, nz, ny*nz, inside the
+// box), no buffer cells.  The sides come from the distinct positive offsets
+// d0 = 1 < d1 < d2: nz = d1, ny = d2 / d1 (fewer offsets: a 2-D or 1-D
+// grid).  Sets h->box and the tile tables; false leaves the row-level
+// sweeps.
+int box_setup(tfg_matrix* h, const int* row0, const int* col0) {
+  const int n = h->ni;
+  if (n != h->nt || n < 2) return TFG_OK;
+  std::vector<long long> offs;
+  for (int i = 0; i < n; ++i) {
+    if (row0[i + 1] <= row0[i] || col0[row0[i]] != i) return TFG_OK;  // diagonal first
+    for (int j = row0[i] + 1; j < row0[i + 1]; ++j) {
+      if (j > row0[i] + 1 && !(col0[j - 1] < col0[j])) return TFG_OK;  // ascending
+      const long long d = (long long)col0[j] - i;
+      if (d > 0 && std::find(offs.begin(), offs.end(), d) == offs.end()) {
+        if (offs.size() == 3) return TFG_OK;
+        offs.push_back(d);
+      }
+    }
+  }
+  std::sort(offs.begin(), offs.end());
+  if (offs.empty() || offs[0] != 1) return TFG_OK;
+  long long nz = n, ny = 1, nx = 1;
+  if (offs.size() >= 2) {
+    nz = offs[1];
+    if (n % nz) return TFG_OK;
+    ny = n / nz;
+  }
+  if (offs.size() == 3) {
+    if (offs[2] % offs[1]) return TFG_OK;
+    ny = offs[2] / offs[1];
+    if (n % offs[2]) return TFG_OK;
+    nx = n / offs[2];
+  }
+  const long long nyz = ny * nz;
+  for (int i = 0; i < n; ++i) {
+    const long long x = i / nyz, y = (i / nz) % ny, z = i % nz;
+    for (int j = row0[i] + 1; j < row0[i + 1]; ++j) {
+      const long long d = (long long)col0[j] - i;
+      const bool ok = (d == 1 && z + 1 < nz) || (d == -1 && z > 0) || (d == nz && y + 1 < ny) ||
+                      (d == -nz && y > 0) || (d == nyz && x + 1 < nx) || (d == -nyz && x > 0);
+      if (!ok) return TFG_OK;
+    }
+  }
+  tfg::BoxIcc& b = h->bx;
+  b = tfg::BoxIcc{};
+  b.nx = (int)nx;
+  b.ny = (int)ny;
+  b.nz = (int)nz;
+  b.tx = std::min(b.nx, tfg::kBoxT);
+  b.ty = std::min(b.ny, tfg::kBoxT);
+  b.tz = std::min(b.nz, tfg::kBoxT);
+  b.ntx = (b.nx + b.tx - 1) / b.tx;
+  b.nty = (b.ny + b.ty - 1) / b.ty;
+  b.ntz = (b.nz + b.tz - 1) / b.tz;
+  std::vector<int> tiles;
+  h->box_toff.assign(1, 0);
+  for (int l = 0; l <= b.ntx + b.nty + b.ntz - 3; ++l) {
+    for (int X = 0; X < b.ntx; ++X)
+      for (int Y = 0; Y < b.nty; ++Y) {
+        const int Z = l - X - Y;
+        if (Z >= 0 && Z < b.ntz) tiles.push_back((X * b.nty + Y) * b.ntz + Z);
+      }
+    h->box_toff.push_back((int)tiles.size());
+  }
+  int* dt = nullptr;
+  HIP_TRY(hipMalloc(&dt, sizeof(int) * tiles.size()));
+  b.tiles = dt;
+  HIP_TRY(hipMemcpy(dt, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice));
+  h->box = true;
+  return TFG_OK;
 }
 
 }  // namespace
@@ -1148,6 +1238,8 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
           e = hipMemcpy(h->icc_rows[f], rows.data(), sizeof(int) * (size_t)nt, hipMemcpyHostToDevice);
       }
       if (e != hipSuccess) { rc = fail(TFG_EHIP, "level upload: %s", hipGetErrorString(e)); break; }
+      rc = box_setup(h, r0.data(), c0.data());
+      if (rc != TFG_OK) break;
     }
     // Row patterns when the rows have at most kPatCount distinct offset
     // sequences of at most kPatMax entries (structured grids)

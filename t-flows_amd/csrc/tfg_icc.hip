@@ -77,6 +77,176 @@ __global__ __launch_bounds__(kBlock) void k_icc_bwd(IccArgs c, const double* __r
   z[i] = sum * c.d[i];
 }
 
+// ---------------------------------------------------------------------------
+// Box grids: tiled sweeps
+// ---------------------------------------------------------------------------
+// On a box grid (rows (x, y, z), x slowest; every entry a box neighbour) row
+// i's lower entries are, in CSR order, (x-1), (y-1), (z-1), and its upper
+// entries from the end of the row down (x+1), (y+1), (z+1).  In sweep
+// coordinates (u, v, w) = (x, y, z) for the form and the forward sweep and
+// (nx-1-x, ny-1-y, nz-1-z) for the backward sweep, a row reads the finished
+// values at u-1, v-1, w-1, in that order.  The grid is cut into tiles of at
+// most kBoxT^3 cells; a tile needs the tiles before it along u, v and w, so
+// the tiles of one level X + Y + Z run in one launch, one workgroup each
+// (37 launches per sweep at 200^3 instead of 598 row levels).  A workgroup
+// loads the three finished faces next to its tile into LDS and runs the
+// tile's own levels lu + lv + lw with a barrier between them, one cell per
+// thread and level.  Every row does the reference's operations in the
+// reference's order; an entry the row lacks is a zero in the diagonals and
+// takes a zero product away, so for finite values the results are the same
+// bits as the row-level sweeps'.
+
+// the seven diagonals of val (entries a row lacks stay zero from the
+// allocation)
+__global__ __launch_bounds__(kBlock) void k_box_pack(IccArgs c, const double* __restrict__ val) {
+  const BoxIcc& b = c.bx;
+  const int nyz = b.ny * b.nz;
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < c.n; i += stride) {
+    const int rs = c.row[i], re = c.row[i + 1];
+    b.dg[i] = val[rs];
+    for (int j = rs + 1; j < re; ++j) {
+      const int off = c.col[j] - i;
+      const double v = val[j];
+      if (off == -nyz) b.lo[0][i] = v;
+      else if (off == -b.nz) b.lo[1][i] = v;
+      else if (off == -1) b.lo[2][i] = v;
+      else if (off == 1) b.up[2][i] = v;
+      else if (off == b.nz) b.up[1][i] = v;
+      else if (off == nyz) b.up[0][i] = v;
+    }
+  }
+}
+
+// MODE 0 form: d_inv, d from a_ii and the lower entries   Prec_Form.f90:68-78
+// MODE 1 forward: y from r                                Prec_Solve.f90:64-75
+// MODE 2 backward: z from y                               Prec_Solve.f90:77-92
+// The value the neighbours read (d, y, z) is `out`; the tiles of level
+// toff[l] .. are blockIdx.x + lo.
+template <int MODE>
+__global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const State* st,
+                                                           const double* __restrict__ in,
+                                                           double* __restrict__ out, int lo) {
+  __shared__ double s[(kBoxT + 1) * (kBoxT + 1) * (kBoxT + 1)];
+  if (MODE != 0 && st && st->done) return;
+  const BoxIcc& b = c.bx;
+  const double fn = st ? st->fn : 1.0;
+  const int tid = threadIdx.x;
+  const int tile = b.tiles[lo + blockIdx.x];
+  const int X = tile / (b.nty * b.ntz), Y = (tile / b.ntz) % b.nty, Z = tile % b.ntz;
+  const int u0 = X * b.tx, v0 = Y * b.ty, w0 = Z * b.tz;
+  const int SW = 1, SV = b.tz + 1, SU = (b.ty + 1) * (b.tz + 1);  // LDS strides
+  auto row_of = [&](int u, int v, int w) {
+    const int x = MODE == 2 ? b.nx - 1 - u : u;
+    const int y = MODE == 2 ? b.ny - 1 - v : v;
+    const int z = MODE == 2 ? b.nz - 1 - w : w;
+    return (x * b.ny + y) * b.nz + z;
+  };
+  // finished faces at u0-1, v0-1, w0-1 (zero outside the grid)
+  for (int t = tid; t < b.ty * b.tz; t += kBoxThreads) {
+    const int lv = t / b.tz, lw = t % b.tz, v = v0 + lv, w = w0 + lw;
+    s[(lv + 1) * SV + (lw + 1) * SW] = (u0 > 0 && v < b.ny && w < b.nz) ? out[row_of(u0 - 1, v, w)] : 0.0;
+  }
+  for (int t = tid; t < b.tx * b.tz; t += kBoxThreads) {
+    const int lu = t / b.tz, lw = t % b.tz, u = u0 + lu, w = w0 + lw;
+    s[(lu + 1) * SU + (lw + 1) * SW] = (v0 > 0 && u < b.nx && w < b.nz) ? out[row_of(u, v0 - 1, w)] : 0.0;
+  }
+  for (int t = tid; t < b.tx * b.ty; t += kBoxThreads) {
+    const int lu = t / b.ty, lv = t % b.ty, u = u0 + lu, v = v0 + lv;
+    s[(lu + 1) * SU + (lv + 1) * SV] = (w0 > 0 && u < b.nx && v < b.ny) ? out[row_of(u, v, w0 - 1)] : 0.0;
+  }
+  // this thread's cells, in the tile's natural order (w fastest: the loads
+  // coalesce); level l below computes every cell with lu + lv + lw = l
+  constexpr int M = kBoxT * kBoxT * kBoxT / kBoxThreads;
+  const int ncell = b.tx * b.ty * b.tz;
+  int lev[M], pos[M], ri[M];
+  double c0[M], c1[M], c2[M], a0[M], dd[M], res[M], res2[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int sidx = tid + m * kBoxThreads;
+    lev[m] = -1;
+    pos[m] = 0;
+    ri[m] = 0;
+    c0[m] = c1[m] = c2[m] = a0[m] = dd[m] = res[m] = res2[m] = 0.0;
+    if (sidx < ncell) {
+      const int lw = sidx % b.tz, lv = (sidx / b.tz) % b.ty, lu = sidx / (b.tz * b.ty);
+      const int u = u0 + lu, v = v0 + lv, w = w0 + lw;
+      if (u < b.nx && v < b.ny && w < b.nz) {
+        const int i = row_of(u, v, w);
+        lev[m] = lu + lv + lw;
+        pos[m] = (lu + 1) * SU + (lv + 1) * SV + (lw + 1) * SW;
+        ri[m] = i;
+        const double* const* cf = MODE == 2 ? b.up : b.lo;
+        c0[m] = cf[0][i];
+        c1[m] = cf[1][i];
+        c2[m] = cf[2][i];
+        if (MODE == 0) {
+          a0[m] = b.dg[i];
+        } else if (MODE == 1) {
+          a0[m] = in[i];
+          dd[m] = c.d[i];
+        } else {
+          a0[m] = in[i] * c.dinv[i];
+          dd[m] = c.d[i];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // results stay in registers until the last level: a barrier waits for
+  // the workgroup's outstanding global stores
+  const int nlev = b.tx + b.ty + b.tz - 2;
+  for (int l = 0; l < nlev; ++l) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      if (lev[m] == l) {
+        const int p = pos[m];
+        const double n0 = s[p - SU], n1 = s[p - SV], n2 = s[p - SW];
+        double r;
+        if (MODE == 0) {
+          double sum = a0[m] * fn;
+          double a = c0[m] * fn;
+          sum = sum - n0 * a * a;
+          a = c1[m] * fn;
+          sum = sum - n1 * a * a;
+          a = c2[m] * fn;
+          sum = sum - n2 * a * a;
+          res2[m] = sum;
+          r = 1.0 / sum;
+        } else {
+          double sum = a0[m];
+          sum = sum - (c0[m] * fn) * n0;
+          sum = sum - (c1[m] * fn) * n1;
+          sum = sum - (c2[m] * fn) * n2;
+          r = sum * dd[m];
+        }
+        s[p] = r;
+        res[m] = r;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    if (lev[m] >= 0) {
+      out[ri[m]] = res[m];
+      if (MODE == 0) c.dinv[ri[m]] = res2[m];
+    }
+  }
+}
+
+hipError_t launch_box_sweep(hipStream_t s, const IccArgs& c, int mode, const State* st,
+                            const double* in, double* out) {
+  const int nl = c.bx.ntx + c.bx.nty + c.bx.ntz - 2;
+  for (int l = 0; l < nl; ++l) {
+    const int lo = c.toff_host[l], cnt = c.toff_host[l + 1] - lo;
+    if (mode == 0) k_box_sweep<0><<<cnt, kBoxThreads, 0, s>>>(c, st, in, out, lo);
+    else if (mode == 1) k_box_sweep<1><<<cnt, kBoxThreads, 0, s>>>(c, st, in, out, lo);
+    else k_box_sweep<2><<<cnt, kBoxThreads, 0, s>>>(c, st, in, out, lo);
+  }
+  return hipGetLastError();
+}
+
 // CG with ICC, iteration k: the stopping test of iteration k-1 on the update
 // partials (one workgroup), before the sweeps (Cg.f90:214-222).
 __global__ __launch_bounds__(kBlock) void k_icc_check(CgArgs a, int k) {
@@ -148,6 +318,11 @@ hipError_t launch_jacobi_apply(hipStream_t s, int n, const int* row, const doubl
 }
 
 hipError_t launch_icc_form(hipStream_t s, const IccArgs& c, const double* val, const State* st) {
+  if (c.box) {
+    const int grid = std::max(1, std::min((c.n + kBlock - 1) / kBlock, 4096));
+    k_box_pack<<<grid, kBlock, 0, s>>>(c, val);
+    return launch_box_sweep(s, c, 0, st, nullptr, c.d);
+  }
   for (int l = 0; l < c.nlev_f; ++l) {
     const int lo = c.off_f_host[l], hi = c.off_f_host[l + 1];
     k_icc_form<<<(hi - lo + kBlock - 1) / kBlock, kBlock, 0, s>>>(c, val, st, lo, hi);
@@ -157,6 +332,11 @@ hipError_t launch_icc_form(hipStream_t s, const IccArgs& c, const double* val, c
 
 hipError_t launch_icc_apply(hipStream_t s, const IccArgs& c, const double* val, const State* st,
                             const double* r, double* z) {
+  if (c.box) {  // the diagonals were packed from val by launch_icc_form
+    const hipError_t e = launch_box_sweep(s, c, 1, st, r, c.y);
+    if (e != hipSuccess) return e;
+    return launch_box_sweep(s, c, 2, st, c.y, z);
+  }
   for (int l = 0; l < c.nlev_f; ++l) {
     const int lo = c.off_f_host[l], hi = c.off_f_host[l + 1];
     k_icc_fwd<<<(hi - lo + kBlock - 1) / kBlock, kBlock, 0, s>>>(c, val, st, r, lo, hi);

@@ -173,6 +173,22 @@ struct IrArgs {
 
 // Level-scheduled DIC (tfg_icc.hip).  Device arrays except the two level
 // offset tables, which the host reads to size each level's launch.
+// DIC sweeps of a box grid (tfg_icc.hip): rows (x, y, z), x slowest, with
+// every entry a box neighbour (offsets 1, nz, ny*nz).  The values sit in
+// seven zero-padded diagonals; the sweeps run over tiles of at most
+// kBoxT^3 cells, one workgroup per tile and one launch per tile level.
+constexpr int kBoxT = 16;
+constexpr int kBoxThreads = 1024;
+struct BoxIcc {
+  int nx, ny, nz;
+  int tx, ty, tz;           // tile sides (<= kBoxT)
+  int ntx, nty, ntz;        // tiles per side
+  const int* tiles;         // tile ids X * nty * ntz + Y * ntz + Z, sorted by X + Y + Z
+  double* dg;               // a_ii
+  double* lo[3];            // a(i, i - off) for off = ny*nz, nz, 1 (zero when absent)
+  double* up[3];            // a(i, i + off), same order
+};
+
 struct IccArgs {
   int n;
   const int* row;       // 0-based
@@ -185,6 +201,9 @@ struct IccArgs {
   int nlev_f, nlev_b;
   const int* off_f_host;  // host: nlev_f + 1 level starts in rows_f
   const int* off_b_host;  // host: nlev_b + 1 level starts in rows_b
+  int box;                // 1: the tiled sweeps of a box grid
+  BoxIcc bx;
+  const int* toff_host;   // host: tile-level starts in bx.tiles (ntx + nty + ntz - 1 levels)
 };
 
 // Pressure-system assembly from face data (tfg_assembly.hip).
