@@ -1476,6 +1476,7 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
   return TFG_OK;
 }
 
+extern "C++" {  // C++ helpers inside the C-ABI block
 namespace {
 
 // Reverse Cuthill-McKee order of the graph of rows (0-based CSR): for each
@@ -1566,6 +1567,7 @@ hipError_t from_inner_vec(tfg_matrix* h, const double* src, double* v) {
 }
 
 }  // namespace
+}  // extern "C++"
 
 int tfg_matrix_reorder(tfg_matrix* h, int order) {
   if (!h) return fail(TFG_EINVAL, "null handle");
