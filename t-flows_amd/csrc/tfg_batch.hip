@@ -35,6 +35,19 @@ namespace {
 // Partial rows of the batch (each nblk long).
 enum { BP_DIAG = 0, BP_PRO = 1, BP_SPMV = 8, BP_UPD = 9, BP_ROWS = 14 };
 
+// A system whose entries lie on its diagonal and k <= kBatchSym upper
+// diagonals and their mirrors (a hex grid: k = 3), each row stored diagonal
+// first and then by ascending column.  Its products read the values once,
+// from sd + base: the diagonal, then upper diagonal j at (j + 1) * stride,
+// zero where a row lacks the entry, each followed by at least off[k-1] + 2
+// zeros (so a lower entry of an early row reads zeros).
+constexpr int kBatchSym = 3;
+struct SysDiag {
+  long long base, stride;
+  int k;                    // 0: the system runs the CSR product
+  int off[kBatchSym];       // ascending
+};
+
 struct BatchArgs {
   int nsys, nblk, nact;
   SpmvShape m;              // concatenated CSR; m.blk = all row blocks
@@ -52,6 +65,9 @@ struct BatchArgs {
   double* part;             // [BP_ROWS][nblk]
   State* st;                // one per system
   unsigned long long* clk;  // [0] start; [1 + s] stop of system s (device clock)
+  const SysDiag* sdg;       // per system; null: every system runs the CSR product
+  double* sd;               // the diagonals of the systems with k > 0
+  int* bad;                 // per system: 1 when this solve's val is not symmetric
 };
 
 __device__ __forceinline__ double bmeasure(const State* st, double nrms, double xmin,
@@ -205,6 +221,34 @@ __global__ __launch_bounds__(kBlock) void k_bt_pupdate(BatchArgs a) {
   a.p[i] = st->iter == 0 ? z : z + st->aux1 * a.p[i];
 }
 
+// Row i of a system on diagonals (SysDiag): a_ii x_i, the lower entries from
+// their mirrors for j = k-1 .. 0, then the upper entries for j = 0 .. k-1:
+// the CSR order, so the product is the CSR engine's bit for bit (a missing
+// entry adds a zero product; the x index of one is clamped into the system).
+template <int K>
+__device__ __forceinline__ double bt_dia_row(const BatchArgs& a, const SysDiag& g, int i, int r0,
+                                             int r1, double fn) {
+  const double* sd = a.sd + g.base;
+  const long long li = i - r0;
+  double lo[K], up[K], xl[K], xu[K];
+  const double d = sd[li];
+  const double x0 = a.p[i];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int o = g.off[j];
+    up[j] = sd[(j + 1) * g.stride + li];
+    lo[j] = sd[(j + 1) * g.stride + li - o];
+    xu[j] = a.p[min(i + o, r1 - 1)];
+    xl[j] = a.p[max(i - o, r0)];
+  }
+  double acc = 0.0 + (d * fn) * x0;
+#pragma unroll
+  for (int j = K - 1; j >= 0; --j) acc = acc + (lo[j] * fn) * xl[j];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc = acc + (up[j] * fn) * xu[j];
+  return acc;
+}
+
 // q = (A fn) p and the block's (p, q)                       Cg.f90:176-195
 template <class E>
 __global__ __launch_bounds__(kBlock) void k_bt_spmv(BatchArgs a) {
@@ -212,10 +256,27 @@ __global__ __launch_bounds__(kBlock) void k_bt_spmv(BatchArgs a) {
   __shared__ double s_tmp[4], s_out[1];
   const int bl = active_block(a);
   if (bl < 0) return;
-  const State* st = &a.st[a.bsys[bl]];
+  const int s = a.bsys[bl];
+  const State* st = &a.st[s];
   if (st->done) return;
   const double fn = st->fn;
   double v[1] = {0.0};
+  const SysDiag g = a.sdg ? a.sdg[s] : SysDiag{};
+  if (g.k > 0 && !a.bad[s]) {
+    const int i = a.m.blk[bl] + (int)threadIdx.x;
+    if (i < a.m.blk[bl + 1]) {
+      const int r0 = a.srow[s], r1 = a.srow[s + 1];
+      double qi = 0.0;
+      if (g.k == 1) qi = bt_dia_row<1>(a, g, i, r0, r1, fn);
+      else if (g.k == 2) qi = bt_dia_row<2>(a, g, i, r0, r1, fn);
+      else qi = bt_dia_row<3>(a, g, i, r0, r1, fn);
+      a.q[i] = qi;
+      v[0] = v[0] + a.p[i] * qi;
+    }
+    const int ops[1] = {OP_SUM};
+    block_publish<1>(a, BP_SPMV, bl, v, ops, s_tmp, s_out);
+    return;
+  }
   E::template one<false>(
       sm, a.m, a.val, fn, a.p, s_tmp, s_out, bl, [](int) { return 0.0; },
       [&](int i, double qi, double pi, double) {
@@ -224,6 +285,41 @@ __global__ __launch_bounds__(kBlock) void k_bt_spmv(BatchArgs a) {
       });
   const int ops[1] = {OP_SUM};
   block_publish<1>(a, BP_SPMV, bl, v, ops, s_tmp, s_out);
+}
+
+// The diagonals of every system with k > 0 from val, one thread per row:
+// the diagonal and the upper entries to their slots; every lower entry
+// a(i, j) compared bit for bit with a(j, i) from row j, a difference
+// setting bad[s] (that system then runs the CSR product in this solve).
+__global__ __launch_bounds__(kBlock) void k_bt_sympack(BatchArgs a) {
+  const int bl = blockIdx.x;
+  const int s = a.bsys[bl];
+  const SysDiag g = a.sdg[s];
+  if (g.k == 0) return;
+  const int i = a.m.blk[bl] + (int)threadIdx.x;
+  if (i >= a.m.blk[bl + 1]) return;
+  const long long li = i - a.srow[s];
+  double* sd = a.sd + g.base;
+  bool bad = false;
+  for (int j = a.m.row[i]; j < a.m.row[i + 1]; ++j) {
+    const int c = a.m.col[j];
+    const double v = a.val[j];
+    if (c == i) {
+      sd[li] = v;
+    } else if (c > i) {
+      for (int t = 0; t < g.k; ++t)
+        if (c - i == g.off[t]) sd[(t + 1) * g.stride + li] = v;
+    } else {
+      bool found = false;
+      for (int t = a.m.row[c]; t < a.m.row[c + 1]; ++t)
+        if (a.m.col[t] == i) {
+          found = true;
+          if (__double_as_longlong(a.val[t]) != __double_as_longlong(v)) bad = true;
+        }
+      if (!found) bad = true;
+    }
+  }
+  if (bad) atomicOr(&a.bad[s], 1);
 }
 
 // alpha = rho / (p, q) per system                            Cg.f90:190-198
@@ -319,6 +415,11 @@ struct tfg_batch {
          *ad = nullptr, *part = nullptr;
   tfg::State* st = nullptr;
   tfg::State* hst = nullptr;          // pinned, 2 x nsys (polling, double-buffered)
+  tfg::SysDiag* sdg = nullptr;        // per system (device); null: no system on diagonals
+  double* sd = nullptr;               // their diagonals (zeroed at the first solve)
+  long long sd_len = 0;
+  bool sd_zeroed = false;
+  int* bad = nullptr;                 // per system, per solve
   unsigned long long* clk = nullptr;  // device, nsys + 1
   std::vector<double> done_ms;        // stop time of each system, last solve
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_poll[2] = {nullptr, nullptr};
@@ -328,12 +429,49 @@ namespace {
 
 using tfg::fail;
 
+// Whether system s (1-based CSR of n rows) lies on its diagonal and at most
+// kBatchSym upper diagonals and their mirrors, each row stored diagonal
+// first and then by ascending column; g.k = 0 if not.
+void sys_diag(int n, const int* row, const int* col, tfg::SysDiag& g) {
+  g = tfg::SysDiag{};
+  std::vector<int> off;
+  long long nup = 0, nlow = 0;
+  for (int i = 0; i < n; ++i) {
+    const int rs = row[i] - 1, re = row[i + 1] - 1;
+    if (re <= rs || col[rs] - 1 != i) return;
+    for (int j = rs + 1; j < re; ++j) {
+      if (j > rs + 1 && !(col[j - 1] < col[j])) return;
+      const int d = col[j] - 1 - i;
+      if (d > 0) {
+        ++nup;
+        if (std::find(off.begin(), off.end(), d) == off.end()) {
+          if ((int)off.size() == tfg::kBatchSym) return;
+          off.push_back(d);
+        }
+        // the mirror (i + d, i) in row i + d
+        const int c = i + d;
+        bool found = false;
+        for (int t = row[c] - 1; t < row[c + 1] - 1 && !found; ++t) found = col[t] - 1 == i;
+        if (!found) return;
+      } else {
+        ++nlow;
+      }
+    }
+  }
+  if (off.empty() || nup != nlow) return;
+  std::sort(off.begin(), off.end());
+  g.k = (int)off.size();
+  for (int j = 0; j < g.k; ++j) g.off[j] = off[j];
+  const long long gap = ((long long)off.back() + 2 + 1) & ~1LL;
+  g.stride = ((long long)n + gap + 1) & ~1LL;
+}
+
 void batch_free(tfg_batch* h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* dev[] = {h->row, h->col, h->blk, h->bsys, h->sblk, h->srow, h->act, h->val, h->b,
-                 h->x, h->r, h->p, h->q, h->ad, h->part, h->st, h->clk};
+                 h->x, h->r, h->p, h->q, h->ad, h->part, h->st, h->clk, h->sdg, h->sd, h->bad};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   if (h->hst) (void)hipHostFree(h->hst);
@@ -362,6 +500,9 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
   if (NT > INT_MAX / 2 || NNZ > INT_MAX - 64) return fail(TFG_EINVAL, "batch too large for int32 indices");
   // validate every system and build the global arrays
   std::vector<int> grow((size_t)NT + 1), gcol((size_t)NNZ), blk, bsys, sblk, srow;
+  std::vector<tfg::SysDiag> sdg(nsys);
+  long long sd_len = 0;
+  int nsd = 0;
   {
     long long ro = 0, co = 0, rpos = 0, dpos = 0;
     std::vector<int> b1;
@@ -372,6 +513,12 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
       const int* rs = row + rpos;
       const int rc = tfg::check_csr(n[s], nnz[s], rs, col + co, dia + dpos);
       if (rc != TFG_OK) return fail(rc, "system %d: %s", s, tfg_last_error());
+      sys_diag(n[s], rs, col + co, sdg[s]);
+      if (sdg[s].k > 0) {
+        sdg[s].base = sd_len;
+        sd_len += (sdg[s].k + 1) * sdg[s].stride;
+        ++nsd;
+      }
       for (int i = 0; i < n[s]; ++i) grow[ro + i] = (int)(co + rs[i] - 1);
       for (int j = 0; j < nnz[s]; ++j) gcol[co + j] = (int)(ro + col[co + j] - 1);
       tfg::make_blocks(rs, n[s], tfg::kCap, b1);
@@ -414,6 +561,12 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
     dmal((void**)&h->part, sizeof(double) * (size_t)tfg::BP_ROWS * bsys.size());
     dmal((void**)&h->st, sizeof(tfg::State) * (size_t)nsys);
     dmal((void**)&h->clk, sizeof(unsigned long long) * (size_t)(nsys + 1));
+    if (nsd > 0) {
+      dmal((void**)&h->sdg, sizeof(tfg::SysDiag) * (size_t)nsys);
+      dmal((void**)&h->sd, sizeof(double) * (size_t)sd_len);
+      dmal((void**)&h->bad, sizeof(int) * (size_t)nsys);
+      h->sd_len = sd_len;
+    }
     if (e != hipSuccess) { rc = fail(TFG_ENOMEM, "batch allocation: %s", hipGetErrorString(e)); break; }
     e = hipHostMalloc((void**)&h->hst, 2 * sizeof(tfg::State) * (size_t)nsys, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreate(&h->ev0);
@@ -429,6 +582,7 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
     up(h->bsys, bsys.data(), sizeof(int) * bsys.size());
     up(h->sblk, sblk.data(), sizeof(int) * sblk.size());
     up(h->srow, srow.data(), sizeof(int) * srow.size());
+    if (h->sdg) up(h->sdg, sdg.data(), sizeof(tfg::SysDiag) * sdg.size());
     if (e != hipSuccess) { rc = fail(TFG_EHIP, "batch upload: %s", hipGetErrorString(e)); break; }
   } while (0);
   if (rc != TFG_OK) {
@@ -515,6 +669,20 @@ int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len, const char
   a.part = h->part;
   a.st = h->st;
   a.clk = h->clk;
+  // Systems on diagonals take their products from the diagonals, refilled
+  // from val here; TFG_BATCH_SPMV=csr keeps every system on the CSR product
+  // (the same bits).
+  const char* force = getenv("TFG_BATCH_SPMV");
+  if (h->sdg && !(force && std::string(force) == "csr")) {
+    if (!h->sd_zeroed) {  // entries a row lacks and the gaps stay zero
+      HIP_TRY(hipMemsetAsync(h->sd, 0, sizeof(double) * (size_t)h->sd_len, h->stream));
+      h->sd_zeroed = true;
+    }
+    HIP_TRY(hipMemsetAsync(h->bad, 0, sizeof(int) * (size_t)h->nsys, h->stream));
+    a.sdg = h->sdg;
+    a.sd = h->sd;
+    a.bad = h->bad;
+  }
   const bool vec = ((uintptr_t)dval % 16) == 0 && h->nnz >= 4;
   const hipStream_t S = h->stream;
   const int B = kBlock;
@@ -527,6 +695,7 @@ int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len, const char
 
   HIP_TRY(hipEventRecord(h->ev0, S));
   k_bt_clock<<<1, B, 0, S>>>(a);
+  if (a.sdg) k_bt_sympack<<<h->nblk, B, 0, S>>>(a);
   k_bt_diag<<<h->nblk, B, 0, S>>>(a);
   k_bt_fn<<<h->nsys, B, 0, S>>>(a);
   if (pr == PREC_JACOBI) TFG_BT_ENGINE((k_bt_prologue<E, PREC_JACOBI><<<h->nblk, B, 0, S>>>(a)));

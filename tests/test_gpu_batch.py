@@ -152,3 +152,30 @@ def test_batch_many_medium(cases, gpu):
         r = s.b - s.scipy() @ xs
         assert niter[k] < 5000 and fin[k] < 1e-8
         assert np.linalg.norm(r) / np.linalg.norm(s.b) < 1e-7
+
+
+def test_batch_diagonal_products_equal_csr(cases, gpu, monkeypatch):
+    """Systems on a few diagonals (hex grids) take their products from
+    zero-padded diagonals, the others (kNN; upwind, whose values are not
+    symmetric) from the CSR rows, in one batch.  Forcing every system onto
+    the CSR product (TFG_BATCH_SPMV=csr) gives the same bits: niter, fin_res
+    and x.  A hex system whose val is made asymmetric by one ulp falls back
+    to the CSR product for that solve, again with the same bits."""
+    systems = _systems(cases, SPECS)
+    bt = _batch(gpu, systems)
+    val, b = _concat(systems, "val"), _concat(systems, "b")
+    row0 = systems[1].row - 1
+    i = systems[1].n // 2
+    j = next(t for t in range(row0[i], row0[i + 1]) if systems[1].col[t] - 1 < i)
+    val_asym = val.copy()
+    off = int(sum(s.nnz for s in systems[:1]))
+    val_asym[off + j] = np.nextafter(val_asym[off + j], 0.0)
+    for v in (val, val_asym):
+        out = []
+        for mode in ("", "csr"):
+            monkeypatch.setenv("TFG_BATCH_SPMV", mode)
+            x = np.zeros(int(bt.offsets[-1]))
+            niter, fin, _ = bt.solve("cg", "jacobi", v, x, b, 3000, 1e-9)
+            out.append((niter.copy(), fin.copy(), x))
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+        assert np.array_equal(out[0][2], out[1][2])
