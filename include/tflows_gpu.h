@@ -167,12 +167,13 @@ int tfg_spmv_select(tfg_matrix* h, int kind);
  * col - row offsets when engine 6 is available (else 0), of distinct row
  * patterns when engine 7 is (else 0), of upper diagonals when engine 8
  * is (else 0), whether the last val engine 8 was set up for was
- * symmetric (1), not (0: that call used engine 7) or none was (-1), and the
+ * symmetric (1), not (0: that call used engine 7) or none was (-1), the
  * internal ordering (TFG_ORDER_*; the engine figures are then those of the
- * renumbered matrix).  Any pointer may be NULL. */
+ * renumbered matrix), and the tiles of the ICC sweeps on a box grid (0: the
+ * row-level sweeps).  Any pointer may be NULL. */
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind,
                     int* spmv_grid, int* stage_blocks, int* offsets, int* patterns,
-                    int* sym_diagonals, int* sym_state, int* ordering);
+                    int* sym_diagonals, int* sym_state, int* ordering, int* icc_tiles);
 
 /* ---- internal ordering ----------------------------------------------------
  * No counterpart in the reference.  A matrix whose numbering has no locality

@@ -1639,12 +1639,14 @@ int tfg_matrix_destroy(tfg_matrix* h) {
 
 int tfg_matrix_info(tfg_matrix* h, int* n, int* nnz, int* spmv_kind, int* spmv_grid,
                     int* stage_blocks, int* offsets, int* patterns, int* sym_diagonals,
-                    int* sym_state, int* ordering) {
+                    int* sym_state, int* ordering, int* icc_tiles) {
   if (!h) return fail(TFG_EINVAL, "null handle");
   if (ordering) *ordering = h->inner ? TFG_ORDER_RCM : TFG_ORDER_NATURAL;
+  // ICC runs in the caller's order, on this handle
+  if (icc_tiles) *icc_tiles = h->box ? h->bx.ntx * h->bx.nty * h->bx.ntz : 0;
   if (h->inner)
     return tfg_matrix_info(h->inner, n, nnz, spmv_kind, spmv_grid, stage_blocks, offsets,
-                           patterns, sym_diagonals, sym_state, nullptr);
+                           patterns, sym_diagonals, sym_state, nullptr, nullptr);
   if (sym_diagonals) *sym_diagonals = h->nsym;
   if (sym_state) *sym_state = h->sym_state;
   if (offsets) *offsets = h->ntab;

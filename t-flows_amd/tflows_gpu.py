@@ -72,7 +72,7 @@ def lib():
         P, I, D, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
         L.tfg_matrix_create.argtypes = [I, I, I, P, P, P, ctypes.POINTER(P)]
         L.tfg_matrix_destroy.argtypes = [P]
-        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P, P, P, P, P, P]
+        L.tfg_matrix_info.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P]
         L.tfg_solve_native.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P, I,
                                        P, D, P, P]
         L.tfg_solve_native_ex.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P,
@@ -191,10 +191,10 @@ class Native:
         self._dependents = weakref.WeakSet()
 
     def info(self):
-        vals = [ctypes.c_int() for _ in range(10)]
+        vals = [ctypes.c_int() for _ in range(11)]
         _check(lib().tfg_matrix_info(self.h, *[ctypes.byref(v) for v in vals]))
         return dict(zip(["n", "nnz", "spmv_kind", "spmv_grid", "stage_blocks", "offsets",
-                         "patterns", "sym_diagonals", "sym_state", "ordering"],
+                         "patterns", "sym_diagonals", "sym_state", "ordering", "icc_tiles"],
                         [v.value for v in vals]))
 
     def reorder(self, order):
