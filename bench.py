@@ -772,6 +772,11 @@ def main():
             "launch_us": spmv_ms * 1e3,
             "format_bytes_per_launch": fmt_bytes,
             "format_GBps": fmt_bytes / (spmv_ms * 1e-3) / 1e9,
+            # the HBM bytes the kernel moves (counters) per launch duration:
+            # the symmetric engine moves fewer bytes than the CSR formula
+            # counts, so `frac` can exceed 1 while this stays below it
+            "traffic_GBps": traffic / (spmv_ms * 1e-3) / 1e9 if traffic else None,
+            "traffic_frac": traffic / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
         },
         "cpu_baseline": cpu,
     }
