@@ -181,3 +181,63 @@ def test_spmv_long_rows(oracle, gpu):
             assert q[0] == q_ref[0]
         else:           # the stage engines sum a row longer than the stage as a tree
             assert abs(q[0] - q_ref[0]) <= 1e-14 * np.abs(val[:n] * p[col[:n] - 1]).sum()
+
+
+def _nine_point(nx, ny, seed=3):
+    """Symmetric 2-D 9-point system (1-based CSR, diagonal first, then
+    ascending columns): upper offsets 1, ny-1, ny, ny+1."""
+    rng = np.random.default_rng(seed)
+    n = nx * ny
+    w = {}
+    for x in range(nx):
+        for y in range(ny):
+            i = x * ny + y
+            for dx, dy in ((0, 1), (1, -1), (1, 0), (1, 1)):
+                if 0 <= x + dx < nx and 0 <= y + dy < ny:
+                    j = (x + dx) * ny + y + dy
+                    w[(i, j)] = w[(j, i)] = -rng.uniform(0.5, 1.5)
+    rows = [[] for _ in range(n)]
+    for (i, j), v in w.items():
+        rows[i].append((j, v))
+    row, col, val = [1], [], []
+    for i in range(n):
+        ent = sorted(rows[i])
+        col += [i + 1] + [j + 1 for j, _ in ent]
+        val += [-sum(v for _, v in ent) + 0.1] + [v for _, v in ent]
+        row.append(len(col) + 1)
+    row = np.array(row, np.int32)
+    return row, np.array(col, np.int32), row[:-1].copy(), np.array(val)
+
+
+def test_symmetric_engine_nine_point(oracle, gpu):
+    """A 2-D 9-point stencil: four upper diagonals (1, ny-1, ny, ny+1; odd
+    offsets, so one row per thread) in engine 8, bit-exact against the
+    oracle's CSR product; a CG solve on it meets the parity bar; and its ICC
+    (not a box grid: diagonal neighbours) runs the row-level sweeps, also
+    bit-exact."""
+    row, col, dia, val = _nine_point(36, 30)
+    n = len(dia)
+    nat = gpu.Native(row, col, dia)
+    assert nat.info()["sym_diagonals"] == 4 and nat.info()["icc_tiles"] == 0
+    nat.spmv_select(8)
+    A = oracle.Matrix(row, col, dia, val)
+    p = np.random.default_rng(12).standard_normal(n)
+    dv, dq = gpu.DeviceArray(val), gpu.DeviceArray(np.zeros(n))
+    nat.spmv(dv, gpu.DeviceArray(p), dq)
+    assert nat.info()["sym_state"] == 1
+    assert np.array_equal(dq.download(), oracle.spmv(A, p))
+    b = np.random.default_rng(13).standard_normal(n)
+    x_o = np.zeros(n)
+    # the oracle's solve scales its matrix in place and back (as Cg.f90
+    # does), which can move a value by an ulp: a copy of its own
+    it_o, _ = oracle.solve("cg", "jacobi", oracle.Matrix(row, col, dia, val.copy()), x_o,
+                           b.copy(), 2000, 1e-10)
+    x = np.zeros(n)
+    it, _ = nat.solve_native("cg", "jacobi", val.copy(), x, b.copy(), 2000, 1e-10)
+    assert abs(it - it_o) <= 2
+    assert np.linalg.norm(x - x_o) <= 1e-10 * np.linalg.norm(x_o)
+    d, dinv = oracle.prec_form(A, "icc")
+    z_ref = oracle.prec_solve(A, d, dinv, b, "icc")
+    dz = gpu.DeviceArray(np.zeros(n))
+    nat.prec_apply("icc", dv, gpu.DeviceArray(b), dz)
+    assert np.array_equal(dz.download(), z_ref)
