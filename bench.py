@@ -719,6 +719,15 @@ def main():
             f"true {rep['true_relres']:.3e}, spmv {rep['spmv_ms'] * 1e3:.1f} us "
             f"({rep['spmv_samples']} samples)")
     its = [r[0] for r in reps]
+    # the drop-in call with host arrays (val, b and x0 uploaded, x
+    # downloaded: PCIe inside the call), once, outside the timed steps
+    opt_h = G.Options()
+    opt_h.criterion = G.CRIT_RELRES
+    xh = np.zeros(s.n)
+    t = time.perf_counter()
+    it_h, _ = nat.solve_native("cg", "jacobi", s.val, xh, s.b, miter, tol, options=opt_h)
+    host_s = time.perf_counter() - t
+    log(f"bench: drop-in call with host arrays: {host_s:.3f} s, {it_h} iterations")
     spmv_ms = float(np.mean([r[2]["spmv_ms"] for r in reps]))
     spmv_bytes = 12 * s.nnz + 20 * s.n + 4
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
@@ -756,7 +765,8 @@ def main():
             "tol": tol, "iterations": its[-1], "true_relres": reps[-1][2]["true_relres"],
             "spmv_engine": G.Native.SPMV_KINDS[nat.info()["spmv_kind"]],
             "spmv_grid": nat.info()["spmv_grid"],
-            "solve_GBps_algorithmic": solve_gbs, "parallelism": "single GPU",
+            "solve_GBps_algorithmic": solve_gbs,
+            "drop_in_host_arrays_s": host_s, "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
