@@ -219,10 +219,15 @@ void size_grids(tfg_matrix* h) {
   }
 }
 
+// Grid of the element-wise kernels: at most 3 workgroups per CU.  Every
+// workgroup of the next phase sweeps all of their partials (5 per
+// workgroup), so more workgroups cost more than they hide: at 200^3 the
+// solve took 0.469 / 0.468 / 0.473 / 0.494 / 0.545 s with 2 / 3 / 4 / 8 /
+// 16 per CU.
 int grid_vec_of(const tfg_matrix* h) {
   const int pairs = (h->nt + 1) / 2;
   const int gv = (pairs + tfg::kBlock - 1) / tfg::kBlock;
-  return std::max(1, std::min(gv, std::max(1, h->cu_count) * 4));
+  return std::max(1, std::min(gv, std::max(1, h->cu_count) * 3));
 }
 
 tfg::Launch launch_of(const tfg_matrix* h, const void* val, int kind) {
