@@ -246,6 +246,27 @@ def test_graph_path_equals_kernel_path(cases, gpu, norm, crit, miter):
         assert r == out[0][0] and np.array_equal(x, out[0][1])
 
 
+@pytest.mark.parametrize("kind", [5, 6, 7, 0, 8])
+def test_graph_path_every_engine(cases, gpu, kind):
+    """With every SpMV engine the graph path gives the kernel path's bits,
+    with deferred x (relres, NRMS with norm) and without (NRMS), and with
+    the cap inside a chunk."""
+    s = cases.hex_pressure(16, 14, 12, rho_out=30.0)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    nat.spmv_select(kind)
+    for norm, crit, miter in ((None, 1, 400), (None, 0, 400), (1.0, 0, 37)):
+        out = []
+        for path in (gpu.PATH_KERNELS, gpu.PATH_GRAPH):
+            o = gpu.Options()
+            o.criterion = crit
+            o.path = path
+            x = np.zeros(s.n)
+            r = nat.solve_native("cg", "jacobi", s.val.copy(), x, s.b.copy(), miter, 1e-9,
+                                 norm=norm, options=o)
+            out.append((r, x))
+        assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1]), (kind, crit, miter)
+
+
 def test_symmetric_engine_graph_and_fallback(cases, oracle, gpu):
     """Engine 8 in both launch paths gives the same bits; a val that is not
     symmetric bit for bit on the same structure (one lower entry one ulp
