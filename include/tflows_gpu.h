@@ -336,7 +336,12 @@ int tfg_matrix_set_comm(tfg_matrix* h, tfg_allreduce_fn allreduce,
  * niter and fin_res).  Systems that stop are masked and dropped from the
  * launches; a chunk of iterations is replayed as a hipGraph (opt->path =
  * TFG_PATH_KERNELS launches kernels one by one instead).  FP64, "cg"/"bicg",
- * jacobi or none. */
+ * jacobi or none.  A system whose entries lie on its diagonal and at most 3
+ * upper diagonals and their mirrors (a hex grid), rows stored diagonal first
+ * then by ascending column, takes its products from zero-padded diagonals
+ * when its val is symmetric bit for bit (checked per solve), with the CSR
+ * product's bits; TFG_BATCH_SPMV=csr in the environment keeps every system
+ * on CSR. */
 typedef struct tfg_batch tfg_batch;
 
 /* n[s], nnz[s]: rows and nonzeros of system s; row, col, dia: the systems'
