@@ -671,7 +671,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported "
-              "(the solver is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(the solver runs on one GPU; the design notes describe how it would shard by rows)", file=sys.stderr)
         sys.exit(2)
     if args.workload != "twophase200":
         {"upwind120": bench_upwind, "knn4m": bench_knn_ir, "batch256": bench_batch,
