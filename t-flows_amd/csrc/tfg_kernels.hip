@@ -166,6 +166,24 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
   State* st = a.st;
   if (st->done) return;
   if (k == 0) k = st->kcur;
+  const int stride = gridDim.x * kBlock;
+  const int npair = a.n >> 1;
+  dbl2* p2 = reinterpret_cast<dbl2*>(a.p);
+  dbl2* x2 = reinterpret_cast<dbl2*>(a.x);
+  const dbl2* r2 = reinterpret_cast<const dbl2*>(a.r);
+  const dbl2* d2 = reinterpret_cast<const dbl2*>(a.ad);
+  // this thread's first pair, loaded before the sweep of the partials so
+  // that the two memory round trips overlap (small systems are bound by
+  // them)
+  const int t0 = blockIdx.x * kBlock + threadIdx.x;
+  const int tp = npair > 0 ? min(t0, npair - 1) : 0;
+  dbl2 rv0 = {0.0, 0.0}, dv0 = {1.0, 1.0}, pv0 = {0.0, 0.0}, xv0 = {0.0, 0.0};
+  if (npair > 0) {
+    rv0 = r2[tp];
+    if (PREC == PREC_JACOBI) dv0 = d2[tp];
+    pv0 = p2[tp];
+    if (DX) xv0 = x2[tp];
+  }
   double beta = 0.0;
   if (k > 1) {
     const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
@@ -192,17 +210,9 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
     }
   }
   const double alpha = DX && k > 1 ? st->alpha : 0.0;
-  const int stride = gridDim.x * kBlock;
-  const int npair = a.n >> 1;
-  dbl2* p2 = reinterpret_cast<dbl2*>(a.p);
-  dbl2* x2 = reinterpret_cast<dbl2*>(a.x);
-  const dbl2* r2 = reinterpret_cast<const dbl2*>(a.r);
-  const dbl2* d2 = reinterpret_cast<const dbl2*>(a.ad);
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < npair; t += stride) {
-    const dbl2 rv = r2[t];
+  auto pair = [&](int t, dbl2 rv, dbl2 dv, dbl2 pv, dbl2 xv) {
     dbl2 z;
     if (PREC == PREC_JACOBI) {
-      const dbl2 dv = d2[t];
       z.x = rv.x * (1.0 / dv.x);
       z.y = rv.y * (1.0 / dv.y);
     } else {
@@ -211,9 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
     if (k == 1) {
       p2[t] = z;
     } else {
-      const dbl2 pv = p2[t];
       if (DX) {
-        const dbl2 xv = x2[t];
         dbl2 xo;
         xo.x = xv.x + alpha * pv.x;
         xo.y = xv.y + alpha * pv.y;
@@ -224,6 +232,14 @@ __global__ __launch_bounds__(kBlock) void k_pupdate(CgArgs a, int k) {
       o.y = z.y + beta * pv.y;
       p2[t] = o;
     }
+  };
+  if (t0 < npair) pair(t0, rv0, dv0, pv0, xv0);
+  for (int t = t0 + stride; t < npair; t += stride) {
+    const dbl2 rv = r2[t];
+    const dbl2 dv = PREC == PREC_JACOBI ? d2[t] : dbl2{1.0, 1.0};
+    const dbl2 pv = k == 1 ? dbl2{0.0, 0.0} : p2[t];
+    const dbl2 xv = DX && k > 1 ? x2[t] : dbl2{0.0, 0.0};
+    pair(t, rv, dv, pv, xv);
   }
   if ((a.n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
     const int i = a.n - 1;
@@ -263,6 +279,27 @@ __global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
   State* st = a.st;
   if (st->done) return;
   if (k == 0) k = st->kcur - 1;  // the SpMV of this iteration advanced it
+  const int stride = gridDim.x * kBlock;
+  const int npair = a.n >> 1;
+  dbl2* x2 = reinterpret_cast<dbl2*>(a.x);
+  dbl2* r2 = reinterpret_cast<dbl2*>(a.r);
+  const dbl2* p2 = reinterpret_cast<const dbl2*>(a.p);
+  const dbl2* q2 = reinterpret_cast<const dbl2*>(a.q);
+  const dbl2* d2 = reinterpret_cast<const dbl2*>(a.ad);
+  // this thread's first pair, loaded before the sweep of the (p, q)
+  // partials so that the two memory round trips overlap
+  const int t0 = blockIdx.x * kBlock + threadIdx.x;
+  const int tp = npair > 0 ? min(t0, npair - 1) : 0;
+  dbl2 rv0 = {0.0, 0.0}, qv0 = {0.0, 0.0}, dv0 = {1.0, 1.0}, xv0 = {0.0, 0.0}, pv0 = {0.0, 0.0};
+  if (npair > 0) {
+    rv0 = r2[tp];
+    qv0 = q2[tp];
+    dv0 = d2[tp];
+    if (!DX) {
+      xv0 = x2[tp];
+      pv0 = p2[tp];
+    }
+  }
   {
     const int ops1[1] = {OP_SUM};
     gather<1>(a.pt.spmv, a.pt.pstride, a.pt.g_spmv, ops1, s_tmp, s_out);
@@ -291,20 +328,16 @@ __global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
       v[3] = v[3] + z * rv;
       v[4] = v[4] + rv * rv;
     };
-    const int stride = gridDim.x * kBlock;
-    const int npair = a.n >> 1;
-    dbl2* r2 = reinterpret_cast<dbl2*>(a.r);
-    const dbl2* q2 = reinterpret_cast<const dbl2*>(a.q);
-    const dbl2* d2 = reinterpret_cast<const dbl2*>(a.ad);
-    for (int t = blockIdx.x * kBlock + threadIdx.x; t < npair; t += stride) {
-      const dbl2 rv = r2[t], qv = q2[t], dv = d2[t];
+    auto pair = [&](int t, dbl2 rv, dbl2 qv, dbl2 dv) {
       dbl2 ro;
       ro.x = rv.x - alpha * qv.x;
       ro.y = rv.y - alpha * qv.y;
       r2[t] = ro;
       sums(ro.x, dv.x);
       sums(ro.y, dv.y);
-    }
+    };
+    if (t0 < npair) pair(t0, rv0, qv0, dv0);
+    for (int t = t0 + stride; t < npair; t += stride) pair(t, r2[t], q2[t], d2[t]);
     if ((a.n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
       const int i = a.n - 1;
       const double rv = a.r[i] - alpha * a.q[i];
@@ -325,16 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
     v[3] = v[3] + z * rv;
     v[4] = v[4] + rv * rv;
   };
-  const int stride = gridDim.x * kBlock;
-  const int npair = a.n >> 1;
-  dbl2* x2 = reinterpret_cast<dbl2*>(a.x);
-  dbl2* r2 = reinterpret_cast<dbl2*>(a.r);
-  const dbl2* p2 = reinterpret_cast<const dbl2*>(a.p);
-  const dbl2* q2 = reinterpret_cast<const dbl2*>(a.q);
-  const dbl2* d2 = reinterpret_cast<const dbl2*>(a.ad);
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < npair; t += stride) {
-    const dbl2 xv = x2[t], rv = r2[t];
-    const dbl2 pv = p2[t], qv = q2[t], dv = d2[t];
+  auto pair = [&](int t, dbl2 xv, dbl2 rv, dbl2 pv, dbl2 qv, dbl2 dv) {
     double x0 = xv.x, x1 = xv.y, r0 = rv.x, r1 = rv.y;
     one(x0, r0, pv.x, qv.x, dv.x);
     one(x1, r1, pv.y, qv.y, dv.y);
@@ -345,7 +369,9 @@ __global__ __launch_bounds__(kBlock) void k_update(CgArgs a, int k) {
     ro.y = r1;
     x2[t] = xo;
     r2[t] = ro;
-  }
+  };
+  if (t0 < npair) pair(t0, xv0, rv0, pv0, qv0, dv0);
+  for (int t = t0 + stride; t < npair; t += stride) pair(t, x2[t], r2[t], p2[t], q2[t], d2[t]);
   if ((a.n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
     const int i = a.n - 1;
     double xv = a.x[i], rv = a.r[i];
