@@ -151,7 +151,6 @@ struct tfg_matrix {
   int* poff = nullptr;
   int* plen = nullptr;
   int npat = 0;        // 0: no pattern engine on this matrix
-  int pmax = 0;        // longest pattern
   // symmetric engine: slot of each pattern entry, diagonal + upper diagonals
   // (allocated on first use), asymmetry flag and its pinned host copy
   int* pslot = nullptr;
@@ -257,7 +256,6 @@ tfg::SpmvShape shape_of(const tfg_matrix* h, int kind) {
   m.plen = h->plen;
   m.pslot = h->pslot;
   m.sd = h->sym;
-  m.pmax = h->pmax;
   m.sorder = h->sorder;
   m.nsorder = h->nsorder;
   m.sdg = h->sdg;
@@ -281,7 +279,6 @@ tfg::CgArgs args_of(const tfg_matrix* h, int kind) {
   a.plen = m.plen;
   a.pslot = m.pslot;
   a.sd = m.sd;
-  a.pmax = m.pmax;
   a.sorder = m.sorder;
   a.nsorder = m.nsorder;
   a.sdg = m.sdg;
@@ -1285,7 +1282,6 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
         if (e == hipSuccess) e = hipMemcpy(h->plen, plen.data(), sizeof(int) * plen.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) { rc = fail(TFG_EHIP, "pattern upload: %s", hipGetErrorString(e)); break; }
         h->npat = (int)seqs.size();
-        for (const auto& q : seqs) h->pmax = std::max(h->pmax, (int)q.size());
         // Symmetric engine: no buffer cells, the diagonal first in every
         // row and the other columns ascending (the order the engine sums
         // in), at most kSymMax distinct positive offsets, and every entry

@@ -97,7 +97,6 @@ struct CgArgs {
   const int* poff;
   const int* plen;
   const int* pslot;    // symmetric engine: value slot of each pattern entry
-  int pmax;            // longest row pattern
   const int* sorder;   // symmetric engine: (start, end) row blocks in processing order
   int nsorder;
   const double* sd;    // symmetric engine: diagonal and upper diagonals
@@ -134,7 +133,6 @@ struct SpmvShape {
   // i + offset), for the set-up kernel
   const int* __restrict__ pslot;
   const double* __restrict__ sd;
-  int pmax;                          // longest row pattern
   const int* __restrict__ sorder;    // symmetric engine: (start, end) row blocks
   int nsorder;                       //   in processing order, and their count
   SymDiag sdg;

@@ -56,7 +56,6 @@ __device__ __forceinline__ SpmvShape shape_of(const CgArgs& a) {
   m.plen = a.plen;
   m.pslot = a.pslot;
   m.sd = a.sd;
-  m.pmax = a.pmax;
   m.sorder = a.sorder;
   m.nsorder = a.nsorder;
   m.sdg = a.sdg;
