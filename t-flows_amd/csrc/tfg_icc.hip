@@ -88,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_icc_bwd(IccArgs c, const double* __r
 // values at u-1, v-1, w-1, in that order.  The grid is cut into tiles of at
 // most kBoxT^3 cells; a tile needs the tiles before it along u, v and w, so
 // the tiles of one level X + Y + Z run in one launch, one workgroup each
-// (37 launches per sweep at 200^3 instead of 598 row levels).  A workgroup
+// (73 launches per sweep at 200^3 instead of 598 row levels).  A workgroup
 // loads the three finished faces next to its tile into LDS and runs the
 // tile's own levels lu + lv + lw with a barrier between them, one cell per
 // thread and level.  Every row does the reference's operations in the

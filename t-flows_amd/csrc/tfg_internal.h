@@ -174,9 +174,12 @@ struct IrArgs {
 // DIC sweeps of a box grid (tfg_icc.hip): rows (x, y, z), x slowest, with
 // every entry a box neighbour (offsets 1, nz, ny*nz).  The values sit in
 // seven zero-padded diagonals; the sweeps run over tiles of at most
-// kBoxT^3 cells, one workgroup per tile and one launch per tile level.
-constexpr int kBoxT = 16;
-constexpr int kBoxThreads = 1024;
+// kBoxT^3 cells, one workgroup per tile (one thread per cell) and one
+// launch per tile level.  Measured at 200^3 (ICC-CG solve): sides 4, 6, 7, 8,
+// 9, 10, 12, 16 ran 1.89, 1.59, 1.52, 1.34, 1.43, 1.35, 1.57, 1.78 s:
+// smaller tiles take more launches, larger ones more barriers.
+constexpr int kBoxT = 8;
+constexpr int kBoxThreads = 512;
 struct BoxIcc {
   int nx, ny, nz;
   int tx, ty, tz;           // tile sides (<= kBoxT)

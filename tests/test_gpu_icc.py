@@ -11,8 +11,8 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-# box grids run the tiled sweeps (tiles of up to 16^3: 40x33x18 has 3x3x2
-# tiles, partial along every axis; 1x50x37 is a 2-D grid of 4x3 tiles);
+# box grids run the tiled sweeps (tiles of up to 8^3: 40x33x18 has 5x5x3
+# tiles, partial along y and z; 1x50x37 is a 2-D grid of 7x5 tiles);
 # kNN runs the row-level sweeps
 @pytest.mark.parametrize("which", ["poisson12", "twophase10x8x6", "knn8", "upwind9", "line7",
                                    "twophase40x33x18", "plane1x50x37"])
@@ -28,8 +28,8 @@ def test_prec_apply_bitexact(cases, oracle, gpu, which):
     r = np.random.default_rng(5).standard_normal(s.n)
     A = oracle.Matrix(s.row, s.col, s.dia, s.val)
     nat = gpu.Native(s.row, s.col, s.dia)
-    tiles = {"poisson12": 1, "twophase10x8x6": 1, "knn8": 0, "upwind9": 1, "line7": 1,
-             "twophase40x33x18": 18, "plane1x50x37": 12}[which]
+    tiles = {"poisson12": 8, "twophase10x8x6": 2, "knn8": 0, "upwind9": 8, "line7": 1,
+             "twophase40x33x18": 75, "plane1x50x37": 35}[which]
     assert nat.info()["icc_tiles"] == tiles
     dv, dr, dz = gpu.DeviceArray(s.val), gpu.DeviceArray(r), gpu.DeviceArray(np.zeros(s.n))
     for prec in ("icc", "jacobi", "none"):
