@@ -30,7 +30,7 @@
  * serial left-to-right order.  1 (2) sums blocks of 64 (1000) terms left to
  * right and combines the block sums pairwise: equally valid parallel orders,
  * used only to measure how far rounding alone moves a solve (the envelope
- * the GPU's own order is judged against).  Not part of the reference. */
+ * the GPU's own order is compared against).  Not part of the reference. */
 static int g_sum_order = 0;
 
 void or_set_sum_order(int order) { g_sum_order = order; }
