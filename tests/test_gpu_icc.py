@@ -12,10 +12,11 @@ def _rel(a, b):
 
 
 # box grids run the tiled sweeps (tiles of up to 8^3: 40x33x18 has 5x5x3
-# tiles, partial along y and z; 1x50x37 is a 2-D grid of 7x5 tiles);
+# tiles, partial along y and z; 17x19x21 has 3x3x3, partial along every
+# axis; 1x50x37 is a 2-D grid of 7x5 tiles);
 # kNN runs the row-level sweeps
 @pytest.mark.parametrize("which", ["poisson12", "twophase10x8x6", "knn8", "upwind9", "line7",
-                                   "twophase40x33x18", "plane1x50x37"])
+                                   "twophase40x33x18", "twophase17x19x21", "plane1x50x37"])
 def test_prec_apply_bitexact(cases, oracle, gpu, which):
     s = {"poisson12": lambda: cases.hex_pressure(12),
          "twophase10x8x6": lambda: cases.hex_pressure(10, 8, 6, rho_out=1000.0),
@@ -23,13 +24,15 @@ def test_prec_apply_bitexact(cases, oracle, gpu, which):
          "upwind9": lambda: cases.upwind(9),
          "line7": lambda: cases.hex_pressure(1, 1, 7),
          "twophase40x33x18": lambda: cases.hex_pressure(40, 33, 18, rho_out=1000.0),
+         "twophase17x19x21": lambda: cases.hex_pressure(17, 19, 21, rho_out=1000.0),
          "plane1x50x37": lambda: cases.hex_pressure(1, 50, 37, rho_out=100.0)}[which]()
     print(f"icc apply case {s.name} n={s.n} nnz={s.nnz}", flush=True)
     r = np.random.default_rng(5).standard_normal(s.n)
     A = oracle.Matrix(s.row, s.col, s.dia, s.val)
     nat = gpu.Native(s.row, s.col, s.dia)
     tiles = {"poisson12": 8, "twophase10x8x6": 2, "knn8": 0, "upwind9": 8, "line7": 1,
-             "twophase40x33x18": 75, "plane1x50x37": 35}[which]
+             "twophase40x33x18": 75, "twophase17x19x21": 27,
+             "plane1x50x37": 35}[which]
     assert nat.info()["icc_tiles"] == tiles
     dv, dr, dz = gpu.DeviceArray(s.val), gpu.DeviceArray(r), gpu.DeviceArray(np.zeros(s.n))
     for prec in ("icc", "jacobi", "none"):
