@@ -581,9 +581,9 @@ int box_setup(tfg_matrix* h, const int* row0, const int* col0) {
   b.nx = (int)nx;
   b.ny = (int)ny;
   b.nz = (int)nz;
-  b.tx = std::min(b.nx, tfg::kBoxT);
-  b.ty = std::min(b.ny, tfg::kBoxT);
-  b.tz = std::min(b.nz, tfg::kBoxT);
+  b.tx = std::min(b.nx, tfg::kBoxTX);
+  b.ty = std::min(b.ny, tfg::kBoxTY);
+  b.tz = std::min(b.nz, tfg::kBoxTZ);
   b.ntx = (b.nx + b.tx - 1) / b.tx;
   b.nty = (b.ny + b.ty - 1) / b.ty;
   b.ntz = (b.nz + b.tz - 1) / b.tz;

@@ -86,9 +86,9 @@ __global__ __launch_bounds__(kBlock) void k_icc_bwd(IccArgs c, const double* __r
 // coordinates (u, v, w) = (x, y, z) for the form and the forward sweep and
 // (nx-1-x, ny-1-y, nz-1-z) for the backward sweep, a row reads the finished
 // values at u-1, v-1, w-1, in that order.  The grid is cut into tiles of at
-// most kBoxT^3 cells; a tile needs the tiles before it along u, v and w, so
+// most kBoxTX x kBoxTY x kBoxTZ cells; a tile needs the tiles before it along u, v and w, so
 // the tiles of one level X + Y + Z run in one launch, one workgroup each
-// (73 launches per sweep at 200^3 instead of 598 row levels).  A workgroup
+// (61 launches per sweep at 200^3 instead of 598 row levels).  A workgroup
 // loads the three finished faces next to its tile into LDS and runs the
 // tile's own levels lu + lv + lw with a barrier between them, one cell per
 // thread and level.  Every row does the reference's operations in the
@@ -127,7 +127,7 @@ template <int MODE>
 __global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const State* st,
                                                            const double* __restrict__ in,
                                                            double* __restrict__ out, int lo) {
-  __shared__ double s[(kBoxT + 1) * (kBoxT + 1) * (kBoxT + 1)];
+  __shared__ double s[(kBoxTX + 1) * (kBoxTY + 1) * (kBoxTZ + 1)];
   if (MODE != 0 && st && st->done) return;
   const BoxIcc& b = c.bx;
   const double fn = st ? st->fn : 1.0;
@@ -157,7 +157,8 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const Stat
   }
   // this thread's cells, in the tile's natural order (w fastest: the loads
   // coalesce); level l below computes every cell with lu + lv + lw = l
-  constexpr int M = kBoxT * kBoxT * kBoxT / kBoxThreads;
+  constexpr int M = kBoxTX * kBoxTY * kBoxTZ / kBoxThreads;
+  static_assert(M * kBoxThreads == kBoxTX * kBoxTY * kBoxTZ, "every cell has a thread");
   const int ncell = b.tx * b.ty * b.tz;
   int lev[M], pos[M], ri[M];
   double c0[M], c1[M], c2[M], a0[M], dd[M], res[M], res2[M];

@@ -174,15 +174,17 @@ struct IrArgs {
 // DIC sweeps of a box grid (tfg_icc.hip): rows (x, y, z), x slowest, with
 // every entry a box neighbour (offsets 1, nz, ny*nz).  The values sit in
 // seven zero-padded diagonals; the sweeps run over tiles of at most
-// kBoxT^3 cells, one workgroup per tile (one thread per cell) and one
-// launch per tile level.  Measured at 200^3 (ICC-CG solve): sides 4, 6, 7, 8,
-// 9, 10, 12, 16 ran 1.89, 1.59, 1.52, 1.34, 1.43, 1.35, 1.57, 1.78 s:
-// smaller tiles take more launches, larger ones more barriers.
-constexpr int kBoxT = 8;
-constexpr int kBoxThreads = 512;
+// kBoxTX x kBoxTY x kBoxTZ cells (z is the fastest axis), one workgroup
+// per tile (one thread per cell) and one launch per tile level.  Measured at
+// 200^3 (ICC-CG solve): cubes of side 4, 6, 7, 8, 9, 10, 12, 16 ran 1.89,
+// 1.59, 1.52, 1.34, 1.43, 1.35, 1.57, 1.78 s (smaller tiles take more
+// launches, larger ones more barriers); 8x8x16 1.32, 16x8x8 1.33, 8x8x12
+// 1.36, 8x16x16 1.47 s.
+constexpr int kBoxTX = 8, kBoxTY = 8, kBoxTZ = 16;
+constexpr int kBoxThreads = 1024;
 struct BoxIcc {
   int nx, ny, nz;
-  int tx, ty, tz;           // tile sides (<= kBoxT)
+  int tx, ty, tz;           // tile sides (<= kBoxTX, kBoxTY, kBoxTZ)
   int ntx, nty, ntz;        // tiles per side
   const int* tiles;         // tile ids X * nty * ntz + Y * ntz + Z, sorted by X + Y + Z
   double* dg;               // a_ii

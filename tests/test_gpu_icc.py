@@ -11,9 +11,9 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-# box grids run the tiled sweeps (tiles of up to 8^3: 40x33x18 has 5x5x3
-# tiles, partial along y and z; 17x19x21 has 3x3x3, partial along every
-# axis; 1x50x37 is a 2-D grid of 7x5 tiles);
+# box grids run the tiled sweeps (tiles of up to 8x8x16: 40x33x18 has 5x5x2
+# tiles, partial along y and z; 17x19x21 has 3x3x2, partial along every
+# axis; 1x50x37 is a 2-D grid of 7x3 tiles);
 # kNN runs the row-level sweeps
 @pytest.mark.parametrize("which", ["poisson12", "twophase10x8x6", "knn8", "upwind9", "line7",
                                    "twophase40x33x18", "twophase17x19x21", "plane1x50x37"])
@@ -30,9 +30,9 @@ def test_prec_apply_bitexact(cases, oracle, gpu, which):
     r = np.random.default_rng(5).standard_normal(s.n)
     A = oracle.Matrix(s.row, s.col, s.dia, s.val)
     nat = gpu.Native(s.row, s.col, s.dia)
-    tiles = {"poisson12": 8, "twophase10x8x6": 2, "knn8": 0, "upwind9": 8, "line7": 1,
-             "twophase40x33x18": 75, "twophase17x19x21": 27,
-             "plane1x50x37": 35}[which]
+    tiles = {"poisson12": 4, "twophase10x8x6": 2, "knn8": 0, "upwind9": 4, "line7": 1,
+             "twophase40x33x18": 50, "twophase17x19x21": 18,
+             "plane1x50x37": 21}[which]
     assert nat.info()["icc_tiles"] == tiles
     dv, dr, dz = gpu.DeviceArray(s.val), gpu.DeviceArray(r), gpu.DeviceArray(np.zeros(s.n))
     for prec in ("icc", "jacobi", "none"):
