@@ -89,12 +89,13 @@ __global__ __launch_bounds__(kBlock) void k_icc_bwd(IccArgs c, const double* __r
 // most kBoxTX x kBoxTY x kBoxTZ cells; a tile needs the tiles before it along u, v and w, so
 // the tiles of one level X + Y + Z run in one launch, one workgroup each
 // (61 launches per sweep at 200^3 instead of 598 row levels).  A workgroup
-// loads the three finished faces next to its tile into LDS and runs the
-// tile's own levels lu + lv + lw with a barrier between them, one cell per
-// thread and level.  Every row does the reference's operations in the
-// reference's order; an entry the row lacks is a zero in the diagonals and
-// takes a zero product away, so for finite values the results are the same
-// bits as the row-level sweeps'.
+// loads the three finished faces next to its tile and its cells' values
+// into LDS, one thread per cell; one wave then runs the tile's own levels
+// lu + lv + lw, a lane per (u, v) pencil, with no barrier between them.
+// Every row does the reference's operations in the reference's order; an
+// entry the row lacks is a zero in the diagonals and takes a zero product
+// away, so for finite values the results are the same bits as the
+// row-level sweeps'.
 
 // the seven diagonals of val (entries a row lacks stay zero from the
 // allocation)
@@ -127,7 +128,15 @@ template <int MODE>
 __global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const State* st,
                                                            const double* __restrict__ in,
                                                            double* __restrict__ out, int lo) {
-  __shared__ double s[(kBoxTX + 1) * (kBoxTY + 1) * (kBoxTZ + 1)];
+  constexpr int TX = kBoxTX, TY = kBoxTY, TZ = kBoxTZ;
+  static_assert(TX * TY == 64, "one lane of the level wave per (u, v) pencil");
+  static_assert(TX * TY * TZ == kBoxThreads, "one thread per cell when staging");
+  // LDS strides of the tile with its faces and the pencil stride of the
+  // staged cell data, padded so that the lanes of one level fall on
+  // different banks
+  constexpr int SW = 1, SV = TZ + 2, SU = (TY + 1) * SV + 2, P = TZ + 2;
+  __shared__ double s[(TX + 1) * SU];
+  __shared__ double cv[5][TX * TY * P];  // c0, c1, c2, a0 (r or y), d; MODE 0: d_inv in slot 4
   if (MODE != 0 && st && st->done) return;
   const BoxIcc& b = c.bx;
   const double fn = st ? st->fn : 1.0;
@@ -135,7 +144,6 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const Stat
   const int tile = b.tiles[lo + blockIdx.x];
   const int X = tile / (b.nty * b.ntz), Y = (tile / b.ntz) % b.nty, Z = tile % b.ntz;
   const int u0 = X * b.tx, v0 = Y * b.ty, w0 = Z * b.tz;
-  const int SW = 1, SV = b.tz + 1, SU = (b.ty + 1) * (b.tz + 1);  // LDS strides
   auto row_of = [&](int u, int v, int w) {
     const int x = MODE == 2 ? b.nx - 1 - u : u;
     const int y = MODE == 2 ? b.ny - 1 - v : v;
@@ -155,84 +163,70 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const Stat
     const int lu = t / b.ty, lv = t % b.ty, u = u0 + lu, v = v0 + lv;
     s[(lu + 1) * SU + (lv + 1) * SV] = (w0 > 0 && u < b.nx && v < b.ny) ? out[row_of(u, v, w0 - 1)] : 0.0;
   }
-  // this thread's cells, in the tile's natural order (w fastest: the loads
-  // coalesce); level l below computes every cell with lu + lv + lw = l
-  constexpr int M = kBoxTX * kBoxTY * kBoxTZ / kBoxThreads;
-  static_assert(M * kBoxThreads == kBoxTX * kBoxTY * kBoxTZ, "every cell has a thread");
-  const int ncell = b.tx * b.ty * b.tz;
-  int lev[M], pos[M], ri[M];
-  double c0[M], c1[M], c2[M], a0[M], dd[M], res[M], res2[M];
-#pragma unroll
-  for (int m = 0; m < M; ++m) {
-    const int sidx = tid + m * kBoxThreads;
-    lev[m] = -1;
-    pos[m] = 0;
-    ri[m] = 0;
-    c0[m] = c1[m] = c2[m] = a0[m] = dd[m] = res[m] = res2[m] = 0.0;
-    if (sidx < ncell) {
-      const int lw = sidx % b.tz, lv = (sidx / b.tz) % b.ty, lu = sidx / (b.tz * b.ty);
-      const int u = u0 + lu, v = v0 + lv, w = w0 + lw;
-      if (u < b.nx && v < b.ny && w < b.nz) {
-        const int i = row_of(u, v, w);
-        lev[m] = lu + lv + lw;
-        pos[m] = (lu + 1) * SU + (lv + 1) * SV + (lw + 1) * SW;
-        ri[m] = i;
-        const double* const* cf = MODE == 2 ? b.up : b.lo;
-        c0[m] = cf[0][i];
-        c1[m] = cf[1][i];
-        c2[m] = cf[2][i];
-        if (MODE == 0) {
-          a0[m] = b.dg[i];
-        } else if (MODE == 1) {
-          a0[m] = in[i];
-          dd[m] = c.d[i];
-        } else {
-          a0[m] = in[i] * c.dinv[i];
-          dd[m] = c.d[i];
-        }
-      }
+  auto in_tile = [&](int lu, int lv, int lw) {
+    return lu < b.tx && lv < b.ty && lw < b.tz && u0 + lu < b.nx && v0 + lv < b.ny && w0 + lw < b.nz;
+  };
+  // stage this thread's cell (the tile's natural order, w fastest: the
+  // loads coalesce)
+  const double* const* cf = MODE == 2 ? b.up : b.lo;
+  const int su = tid / (TY * TZ), sv = (tid / TZ) % TY, sw = tid % TZ;
+  const bool mine = in_tile(su, sv, sw);
+  const int sq = (su * TY + sv) * P + sw;
+  int si = 0;
+  if (mine) {
+    si = row_of(u0 + su, v0 + sv, w0 + sw);
+    cv[0][sq] = cf[0][si];
+    cv[1][sq] = cf[1][si];
+    cv[2][sq] = cf[2][si];
+    if (MODE == 0) {
+      cv[3][sq] = b.dg[si];
+    } else if (MODE == 1) {
+      cv[3][sq] = in[si];
+      cv[4][sq] = c.d[si];
+    } else {
+      cv[3][sq] = in[si] * c.dinv[si];
+      cv[4][sq] = c.d[si];
     }
   }
   __syncthreads();
-  // results stay in registers until the last level: a barrier waits for
-  // the workgroup's outstanding global stores
-  const int nlev = b.tx + b.ty + b.tz - 2;
-  for (int l = 0; l < nlev; ++l) {
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      if (lev[m] == l) {
-        const int p = pos[m];
+  // one wave runs the levels: at level l the lane of pencil (pu, pv)
+  // computes cell w = l - pu - pv, so a level needs no barrier
+  if (tid < TX * TY) {
+    const int pu = tid / TY, pv = tid % TY;
+    const int nlev = b.tx + b.ty + b.tz - 2;
+    for (int l = 0; l < nlev; ++l) {
+      const int lw = l - pu - pv;
+      if (lw >= 0 && in_tile(pu, pv, lw)) {
+        const int q = (pu * TY + pv) * P + lw;
+        const int p = (pu + 1) * SU + (pv + 1) * SV + (lw + 1) * SW;
         const double n0 = s[p - SU], n1 = s[p - SV], n2 = s[p - SW];
         double r;
         if (MODE == 0) {
-          double sum = a0[m] * fn;
-          double a = c0[m] * fn;
+          double sum = cv[3][q] * fn;
+          double a = cv[0][q] * fn;
           sum = sum - n0 * a * a;
-          a = c1[m] * fn;
+          a = cv[1][q] * fn;
           sum = sum - n1 * a * a;
-          a = c2[m] * fn;
+          a = cv[2][q] * fn;
           sum = sum - n2 * a * a;
-          res2[m] = sum;
+          cv[4][q] = sum;
           r = 1.0 / sum;
         } else {
-          double sum = a0[m];
-          sum = sum - (c0[m] * fn) * n0;
-          sum = sum - (c1[m] * fn) * n1;
-          sum = sum - (c2[m] * fn) * n2;
-          r = sum * dd[m];
+          double sum = cv[3][q];
+          sum = sum - (cv[0][q] * fn) * n0;
+          sum = sum - (cv[1][q] * fn) * n1;
+          sum = sum - (cv[2][q] * fn) * n2;
+          r = sum * cv[4][q];
         }
         s[p] = r;
-        res[m] = r;
       }
+      __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
   }
-#pragma unroll
-  for (int m = 0; m < M; ++m) {
-    if (lev[m] >= 0) {
-      out[ri[m]] = res[m];
-      if (MODE == 0) c.dinv[ri[m]] = res2[m];
-    }
+  __syncthreads();
+  if (mine) {
+    out[si] = s[(su + 1) * SU + (sv + 1) * SV + (sw + 1) * SW];
+    if (MODE == 0) c.dinv[si] = cv[4][sq];
   }
 }
 

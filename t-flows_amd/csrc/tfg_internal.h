@@ -179,7 +179,8 @@ struct IrArgs {
 // 200^3 (ICC-CG solve): cubes of side 4, 6, 7, 8, 9, 10, 12, 16 ran 1.89,
 // 1.59, 1.52, 1.34, 1.43, 1.35, 1.57, 1.78 s (smaller tiles take more
 // launches, larger ones more barriers); 8x8x16 1.32, 16x8x8 1.33, 8x8x12
-// 1.36, 8x16x16 1.47 s.
+// 1.36, 8x16x16 1.47 s.  The level loop runs on one wave, a lane per (x, y)
+// pencil, so kBoxTX * kBoxTY is 64.
 constexpr int kBoxTX = 8, kBoxTY = 8, kBoxTZ = 16;
 constexpr int kBoxThreads = 1024;
 struct BoxIcc {
