@@ -8,11 +8,14 @@ Workload (one step = one full solve, device-resident inputs):
   ||b - A x||_2 / ||b||_2 < 1e-8.
 
 Prints ONE JSON line.  `roofline` is the SpMV kernel (the dominant one):
-algorithmic bytes per launch 12*nnz + 20*n + 4 over the mean launch duration
-measured with HIP events on the library's stream during the timed solves.
+the bytes its engine must move per launch in its own matrix format (for the
+symmetric-diagonal engine 8 (K + 1) n + 16 n: the diagonal and K upper
+diagonals once, p read, q written) over the mean launch duration measured
+with HIP events on the library's stream during the timed solves; the CSR
+formula of the metric (12 nnz + 20 n + 4) is reported beside it.
 `cpu_baseline` is the oracle (the C restatement of Native_Mod, 1 thread)
-timed on a bounded number of CG iterations of the same system and scaled to
-the GPU's iteration count.
+solving the same system to convergence, timed; `parity` compares its
+iterations and solution with the GPU's.
 """
 import argparse
 import json
@@ -33,48 +36,62 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def _time_oracle(s, iters, omp):
+def _oracle_full_solve(s, omp, progress):
+    """The oracle's Jacobi-CG on system s from x0 = 0 to relres 1e-8; returns
+    (seconds, niter, x)."""
     from oracle import oracle
 
-    A = oracle.Matrix(s.row, s.col, s.dia, s.val)
+    A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
     x = np.zeros(s.n)
     t0 = time.perf_counter()
-    oracle.solve("cg", "jacobi", A, x, s.b.copy(), iters, 1e-30, criterion="relres", omp=omp)
-    return time.perf_counter() - t0
+    it, _ = oracle.solve("cg", "jacobi", A, x, s.b.copy(), 50000, 1e-8, criterion="relres",
+                         omp=omp, progress=progress)
+    return time.perf_counter() - t0, it, x
 
 
-def cpu_baseline(s, gpu_iters, seconds_budget=15.0):
-    """Oracle CG (FP64) on the same system for a bounded number of iterations,
-    1 thread (the reference's default OMP=no build) and all host cores (its
-    OMP=yes analogue); returns the projected time to the GPU's iteration
-    count."""
-    # marginal time per iteration (the solve's set-up excluded), then a sample
-    # of about seconds_budget
-    t2, t6 = _time_oracle(s, 2, False), _time_oracle(s, 6, False)
-    probe = max((t6 - t2) / 4.0, 1e-6)
-    iters = int(max(3, min(1000, (seconds_budget - t2) / probe)))
-    dt = _time_oracle(s, iters, False)
-    per_iter = dt / iters
+def cpu_baseline(s, x_gpu, gpu_iters):
+    """The oracle (the C restatement of Native_Mod's Cg.f90) solves the same
+    system to convergence on the host, timed once each: built with OpenMP on
+    every host core (the reference's OMP=yes analogue), then single-threaded
+    (its default OMP=no build, the reported value).  Both solutions are
+    compared with the GPU's: the 1-thread oracle is the parity reference
+    (iterations, relative x difference), and the OpenMP build, whose sums run
+    in another order, measures how far rounding alone moves the answer at
+    this size."""
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
-    probe_omp = _time_oracle(s, 2, True) / 2.0
-    iters_omp = int(max(3, min(400, seconds_budget / 2 / max(probe_omp, 1e-6))))
-    dt_omp = _time_oracle(s, iters_omp, True)
-    return {
-        "value": per_iter * gpu_iters,
-        "unit": "s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"oracle Jacobi-CG (C restatement of Native_Mod Cg.f90, FP64, 1 thread) on the "
-                   f"same 200^3 system for {iters} iterations ({dt:.1f} s incl. normalisation "
-                   f"and initial residual), {per_iter * 1e3:.1f} ms/iteration scaled to the GPU's "
-                   f"{gpu_iters} iterations"),
-        "ms_per_iteration": per_iter * 1e3,
-        "all_cores": {
-            "value": dt_omp / iters_omp * gpu_iters, "cores": cores,
-            "sample": f"same oracle built with OpenMP, {iters_omp} iterations in {dt_omp:.1f} s",
-            "ms_per_iteration": dt_omp / iters_omp * 1e3,
-        },
+    log(f"bench: CPU baseline: oracle Jacobi-CG to relres 1e-8 on {cores} threads (OpenMP build)")
+    t_omp, it_omp, x_omp = _oracle_full_solve(s, True, 500)
+    log(f"bench: CPU baseline: OpenMP oracle {t_omp:.1f} s, {it_omp} iterations")
+    log("bench: CPU baseline: oracle Jacobi-CG to relres 1e-8, 1 thread")
+    t1, it1, x1 = _oracle_full_solve(s, False, 250)
+    log(f"bench: CPU baseline: 1-thread oracle {t1:.1f} s, {it1} iterations")
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    err, env = rel(x_gpu, x1), rel(x_omp, x1)
+    cpu = {
+        "value": t1, "unit": "s", "cores": 1, "kind": "port",
+        "sample": (f"the whole solve, measured: oracle Jacobi-CG (C restatement of Native_Mod "
+                   f"Cg.f90, FP64, 1 thread) on the same 200^3 system from x0 = 0 to relres "
+                   f"1e-8, {it1} iterations"),
+        "iterations": it1, "ms_per_iteration": t1 / max(it1, 1) * 1e3,
+        "all_cores": {"value": t_omp, "cores": cores, "iterations": it_omp,
+                      "sample": "the same whole solve, oracle built with OpenMP"},
     }
+    parity = {
+        "reference": "1-thread oracle (C restatement of Cg.f90), same system and inputs",
+        "oracle_iterations": it1, "gpu_iterations": gpu_iters,
+        "iterations_diff": gpu_iters - it1,
+        "x_rel_err_gpu_vs_oracle": err,
+        "oracle_openmp_iterations": it_omp,
+        "x_rel_envelope_openmp_vs_serial_oracle": env,
+        "within_bar": bool(abs(gpu_iters - it1) <= max(2, abs(it_omp - it1))
+                           and err <= max(1e-10, env)),
+    }
+    log(f"bench: parity: gpu {gpu_iters} vs oracle {it1} iterations (OpenMP oracle {it_omp}); "
+        f"x rel err {err:.2e} (OpenMP-vs-serial envelope {env:.2e})")
+    return cpu, parity
 
 
 TRAFFIC = os.path.join(ROOT, "profiles", "spmv_traffic_twophase200.json")
@@ -95,6 +112,33 @@ def format_bytes(kind, nnz, n, sym_diagonals=0):
     if kind == 6:
         return 9 * nnz + 20 * n + 4
     return 12 * nnz + 20 * n + 4
+
+
+FORMAT_FORMULA = {8: "8 (K + 1) n + 16 n (diagonal and K upper diagonals, p read, q written)",
+                  7: "8 nnz + 17 n", 6: "9 nnz + 20 n + 4", 5: "12 nnz + 20 n + 4",
+                  0: "12 nnz + 20 n + 4", 1: "12 nnz + 20 n + 4", 2: "12 nnz + 20 n + 4",
+                  3: "12 nnz + 20 n + 4", 4: "12 nnz + 20 n + 4"}
+
+
+def spmv_roofline(kind, nnz, n, sym_diagonals, launch_ms, traffic=None):
+    """Roofline of one SpMV launch: the bytes the engine that ran must move
+    in its own matrix format (the symmetric engine stores the values once)
+    over the launch duration, against the HBM peak; the same launch priced by
+    the metric's CSR formula (12 nnz + 20 n + 4) beside it, which can exceed
+    the peak because the engine's format moves fewer bytes than CSR."""
+    fmt = format_bytes(kind, nnz, n, sym_diagonals)
+    csr = 12 * nnz + 20 * n + 4
+    sec = launch_ms * 1e-3
+    return {
+        "bound": "hbm", "achieved": fmt / sec / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": fmt / sec / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+        "bytes_per_launch": fmt, "bytes_formula": FORMAT_FORMULA[kind],
+        "launch_us": launch_ms * 1e3,
+        "traffic_GBps": traffic / sec / 1e9 if traffic else None,
+        "traffic_frac": traffic / sec / 1e9 / HBM_PEAK_GBS if traffic else None,
+        "csr_equivalent": {"bytes_per_launch": csr, "GBps": csr / sec / 1e9,
+                           "frac": csr / sec / 1e9 / HBM_PEAK_GBS},
+    }
 
 
 # SpMV engine -> text of its template argument in the kernel's full name
@@ -643,7 +687,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=200, help="grid side (default 200 -> 8M cells)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the known-traffic copy kernel (counter calibration)")
     ap.add_argument("--workload", default="twophase200",
@@ -719,6 +762,7 @@ def main():
             f"true {rep['true_relres']:.3e}, spmv {rep['spmv_ms'] * 1e3:.1f} us "
             f"({rep['spmv_samples']} samples)")
     its = [r[0] for r in reps]
+    x_gpu = dx.download()
     # the drop-in call with host arrays (val, b and x0 uploaded, x
     # downloaded: PCIe inside the call), once, outside the timed steps
     opt_h = G.Options()
@@ -729,21 +773,18 @@ def main():
     host_s = time.perf_counter() - t
     log(f"bench: drop-in call with host arrays: {host_s:.3f} s, {it_h} iterations")
     spmv_ms = float(np.mean([r[2]["spmv_ms"] for r in reps]))
-    spmv_bytes = 12 * s.nnz + 20 * s.n + 4
-    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     kind = nat.info()["spmv_kind"]
-    fmt_bytes = format_bytes(kind, s.nnz, s.n, nat.info()["sym_diagonals"])
+    sym_ran = kind == 8 and nat.info()["sym_state"] == 1
     total = float(np.sum(times))
     ms_per_step = total / args.steps * 1e3
-    iter_bytes = spmv_bytes + 88 * s.n
+    iter_bytes = 12 * s.nnz + 20 * s.n + 4 + 88 * s.n
     solve_gbs = iter_bytes * its[-1] / (times[-1]) / 1e9
 
-    spmv_kernel = "k_spmv_pq_sym" if kind == 8 and nat.info()["sym_state"] == 1 else "k_spmv_pq"
-    traffic, traffic_src = traffic_per_launch(spmv_kernel, None if spmv_kernel == "k_spmv_pq_sym" else kind)
-    cpu = None
+    spmv_kernel = "k_spmv_pq_sym" if sym_ran else "k_spmv_pq"
+    traffic, traffic_src = traffic_per_launch(spmv_kernel, None if sym_ran else kind)
+    cpu = parity = None
     if not args.no_cpu:
-        log("bench: CPU baseline leg (oracle, 1 thread)...")
-        cpu = cpu_baseline(s, its[-1], args.cpu_seconds)
+        cpu, parity = cpu_baseline(s, x_gpu, its[-1])
 
     out = {
         "metric": "time to 1e-8 rel. residual, 8M-cell pressure solve; SpMV HBM GB/s vs 8 TB/s",
@@ -768,27 +809,15 @@ def main():
             "solve_GBps_algorithmic": solve_gbs,
             "drop_in_host_arrays_s": host_s, "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "traffic_source": os.path.relpath(traffic_src, ROOT) if traffic_src else None,
-            "kernel": spmv_kernel + " (FP64 SpMV with fused p.q; engine "
-                      + G.Native.SPMV_KINDS[kind] + ")",
-            "bytes_per_launch": spmv_bytes,
-            "launch_us": spmv_ms * 1e3,
-            "format_bytes_per_launch": fmt_bytes,
-            "format_GBps": fmt_bytes / (spmv_ms * 1e-3) / 1e9,
-            # the HBM bytes the kernel moves (counters) per launch duration:
-            # the symmetric engine moves fewer bytes than the CSR formula
-            # counts, so `frac` can exceed 1 while this stays below it
-            "traffic_GBps": traffic / (spmv_ms * 1e-3) / 1e9 if traffic else None,
-            "traffic_frac": traffic / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
-        },
+        "roofline": dict(
+            spmv_roofline(kind if (kind != 8 or sym_ran) else 7, s.nnz, s.n,
+                          nat.info()["sym_diagonals"], spmv_ms, traffic),
+            traffic_source=os.path.relpath(traffic_src, ROOT) if traffic_src else None,
+            kernel=spmv_kernel + " (FP64 SpMV with fused p.q; engine "
+            + G.Native.SPMV_KINDS[kind] + ")",
+            launch_timing="mean of sampled in-solve launches, hipEvents on the library's stream"),
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     print(json.dumps(out), flush=True)
 

@@ -18,6 +18,7 @@
 #include "native_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -28,12 +29,20 @@
 
 /* Summation order of the global reductions.  0 (default) is the reference's
  * serial left-to-right order.  1 (2) sums blocks of 64 (1000) terms left to
- * right and combines the block sums pairwise: equally valid parallel orders,
- * used only to measure how far rounding alone moves a solve (the envelope
- * the GPU's own order is compared against).  Not part of the reference. */
+ * right and combines the block sums pairwise; 3 sums serially from the last
+ * term to the first: equally valid orders, used only to measure how far
+ * rounding alone moves a solve (the envelope the GPU's own order is compared
+ * against).  Not part of the reference. */
 static int g_sum_order = 0;
 
 void or_set_sum_order(int order) { g_sum_order = order; }
+
+/* Progress line on stderr every `every` CG iterations (0: none), so that a
+ * long solve (the full-size CPU baseline) reports as it goes.  Not part of
+ * the reference. */
+static int g_progress = 0;
+
+void or_set_progress(int every) { g_progress = every; }
 
 static double pairwise(const double* s, int n) {
   if (n == 1) return s[0];
@@ -41,9 +50,16 @@ static double pairwise(const double* s, int n) {
   return pairwise(s, h) + pairwise(s + h, n - h);
 }
 
-/* sum_i f(i) in the blocked order; kind: 0 a*b, 1 (a*a)/(b*b), 2 a */
+/* sum_i f(i) in the blocked order (3: serial from the last term to the
+ * first); kind: 0 a*b, 1 (a*a)/(b*b), 2 a */
 static double blocked(const double* a, const double* b, int n, int kind) {
   if (n <= 0) return 0.0;
+  if (g_sum_order == 3) {
+    double s = 0.0;
+    for (int i = n - 1; i >= 0; i--)
+      s = s + (kind == 0 ? a[i] * b[i] : (kind == 1 ? (a[i] * a[i]) / (b[i] * b[i]) : a[i]));
+    return s;
+  }
   const int bs = g_sum_order == 2 ? 1000 : 64;
   int nb = (n + bs - 1) / bs;
   double* part = (double*)malloc(sizeof(double) * (size_t)nb);
@@ -299,6 +315,10 @@ int or_cg_ex(or_matrix* A, double* x, double* b, int prec, int miter,
       r1[i] = r1[i] - alfa * q1[i];
     }
     res = measure(A, ni, r1, x, norm, crit, bnorm);  /* :214-218 */
+    if (g_progress > 0 && iter % g_progress == 0) {
+      fprintf(stderr, "oracle cg: iteration %d, residual %.3e\n", iter, res);
+      fflush(stderr);
+    }
     if (res < tol) goto done;                   /* :220 */
     rho_old = rho;                              /* :222 */
   }

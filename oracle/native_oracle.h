@@ -115,6 +115,7 @@ void or_rhie_chow_fluxes(int nc, int n_faces, const int* c1, const int* c2,
  * summation (blocks of 64, 1000) of every global reduction, to measure
  * rounding sensitivity. */
 void or_set_sum_order(int order);
+void or_set_progress(int every);
 
 #ifdef __cplusplus
 }

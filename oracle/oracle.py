@@ -56,6 +56,7 @@ def lib(omp=False):
         L.or_cg_ex.restype = I
         L.or_cg_ex.argtypes = [P, P, P, I, I, P, D, P, P, I]
         L.or_set_sum_order.argtypes = [I]
+        L.or_set_progress.argtypes = [I]
         L.or_krylov.restype = I
         L.or_krylov.argtypes = [P, P, P, I, I, I, P, D, P, P, I, P]
         L.or_transpose_positions.restype = I
@@ -145,14 +146,15 @@ class BreakdownError(ArithmeticError):
 
 
 def solve(solver, prec, A, x, b, miter, tol, norm=None, criterion="nrms", omp=False,
-          sum_order=0, omegas=None):
+          sum_order=0, omegas=None, progress=0):
     """Solve_Native.f90:2-46 restated: dispatches 'cg' / 'bicg', plus the
     textbook methods 'bicg_textbook', 'cgs', 'bcgs' ('bicgstab').
 
     x and b are updated in place like the reference's intent(out)/(inout)
     arguments.  Returns (niter, fin_res).  sum_order=1 switches every global
     reduction to a blocked-pairwise order (not the reference's): it measures
-    how far rounding alone moves the answer."""
+    how far rounding alone moves the answer.  progress > 0 prints a line on
+    stderr every that many CG iterations."""
     assert x.dtype == np.float64 and x.flags.c_contiguous and x.shape == (A.nt,)
     assert b.dtype == np.float64 and b.flags.c_contiguous and b.shape == (A.nt,)
     niter = ctypes.c_int(0)
@@ -161,10 +163,12 @@ def solve(solver, prec, A, x, b, miter, tol, norm=None, criterion="nrms", omp=Fa
     nref = ctypes.byref(nv) if nv is not None else None
     L = lib(omp)
     L.or_set_sum_order(int(sum_order))
+    L.or_set_progress(int(progress))
     try:
         return _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin, omegas)
     finally:
         L.or_set_sum_order(0)
+        L.or_set_progress(0)
 
 
 def _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin, omegas=None):

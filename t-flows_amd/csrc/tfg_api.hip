@@ -318,10 +318,17 @@ int alloc_work(tfg_matrix* h) {
   return TFG_OK;
 }
 
+// Teardown in dependency order: wait for the handle's work, then the
+// executable graph (its kernel nodes hold this handle's buffers as
+// arguments) and the inner handle, then the memory, events and stream.
 void free_all(tfg_matrix* h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);
+  h->cg_graph = nullptr;
+  if (h->inner) free_all(h->inner);
+  h->inner = nullptr;
   void* dev[] = {h->row, h->col, h->blk[0], h->blk[1], h->val, h->b, h->x, h->r, h->p, h->q, h->ad,
                  h->part, h->st, h->st_in, h->val32, h->e32, h->r32,
                  h->p32, h->q32, h->ad32, h->rt, h->nv[0], h->nv[1], h->nv[2],
@@ -335,11 +342,9 @@ void free_all(tfg_matrix* h) {
     if (d) (void)hipFree(d);
   if (h->hst) (void)hipHostFree(h->hst);
   if (h->hflag) (void)hipHostFree(h->hflag);
-  if (h->inner) free_all(h->inner);
   for (void* d : {(void*)h->vmap, (void*)h->perm, (void*)h->iperm, (void*)h->pval,
                   (void*)h->pb, (void*)h->px, (void*)h->pq})
     if (d) (void)hipFree(d);
-  if (h->cg_graph) (void)hipGraphExecDestroy(h->cg_graph);
   for (hipEvent_t e : {h->ev_poll[0], h->ev_poll[1], h->ev_t0, h->ev_t1})
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->ev_sample)

@@ -220,7 +220,11 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const Stat
         }
         s[p] = r;
       }
+      // the next level reads cells other lanes of this wave stored: release
+      // this level's LDS stores and acquire the others' at wavefront scope
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
   __syncthreads();
@@ -232,6 +236,15 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const Stat
 
 hipError_t launch_box_sweep(hipStream_t s, const IccArgs& c, int mode, const State* st,
                             const double* in, double* out) {
+  // the level loop hands cells between the lanes of one wave of 64
+  static const int wave = [] {
+    int dev = 0, w = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&w, hipDeviceAttributeWarpSize, dev) != hipSuccess)
+      return 0;
+    return w;
+  }();
+  if (wave != 64) return hipErrorInvalidDeviceFunction;
   const int nl = c.bx.ntx + c.bx.nty + c.bx.ntz - 2;
   for (int l = 0; l < nl; ++l) {
     const int lo = c.toff_host[l], cnt = c.toff_host[l + 1] - lo;

@@ -58,3 +58,20 @@ def test_kernel_stats_table(tmp_path):
     text = out.read_text()
     assert "rocprofv3 --kernel-trace --stats -- python3 bench.py" in text
     assert "| `void tfg::k_spmv_pq_sym<3, 1>(tfg::CgArgs, int)` | 10 | 0.9 | 85.1 | 84.0 | 90.0 | 61.5 |" in text
+
+
+def test_roofline_prices_the_engine_format():
+    # the symmetric engine's launch at the in-solve time measured on the
+    # 200^3 system (88.4 us): its own 384 MB give a fraction below 1, the
+    # CSR formula's 829 MB a "fraction" above 1, reported apart
+    n, nnz = 8_000_000, 55_760_000
+    r = bench.spmv_roofline(8, nnz, n, 3, 0.0884)
+    assert r["bytes_per_launch"] == 384_000_000
+    assert r["frac"] == pytest.approx(384e6 / 88.4e-6 / 8e12)
+    assert 0 < r["frac"] <= 1.0
+    assert r["csr_equivalent"]["bytes_per_launch"] == 829_120_004
+    assert r["csr_equivalent"]["frac"] > 1.0
+    # a CSR engine is priced by the CSR formula itself
+    c = bench.spmv_roofline(5, nnz, n, 0, 0.150)
+    assert c["bytes_per_launch"] == c["csr_equivalent"]["bytes_per_launch"]
+    assert c["frac"] <= 1.0

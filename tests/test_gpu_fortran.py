@@ -7,6 +7,8 @@ import subprocess
 import numpy as np
 import pytest
 
+import parity_bar
+
 pytestmark = pytest.mark.gpu
 
 DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
@@ -42,16 +44,17 @@ def test_fortran_driver(cases, oracle, gpu, tmp_path, solver, prec, norm):
     it_p, res_p = nat.solve_native(solver, prec, s.val.copy(), xp, s.b.copy(), 2000, 1e-8, norm=norm)
     assert (niter, fin_res) == (it_p, res_p) and np.array_equal(x, xp)
     # and agrees with the oracle: +-2 iterations, x within 1e-10 relative or
-    # twice the spread that reordering the oracle's sums causes by itself
-    A = oracle.Matrix(s.row, s.col, s.dia, s.val)
-    xo, xe = x0.copy(), x0.copy()
-    it_o, res_o = oracle.solve(solver, prec, A, xo, s.b.copy(), 2000, 1e-8, norm=norm)
-    oracle.solve(solver, prec, A, xe, s.b.copy(), 2000, 1e-8, norm=norm, sum_order=1)
-    env = np.linalg.norm(xe - xo) / np.linalg.norm(xo)
-    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
-    print(f"  niter {niter} (oracle {it_o}), x rel err {err:.2e}, envelope {env:.2e}", flush=True)
-    assert abs(niter - it_o) <= 2
-    assert err <= max(1e-10, 2.0 * env)
+    # the spread that reordering the oracle's sums causes by itself
+
+    def solve(so):
+        A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
+        xo = x0.copy()
+        it, _ = oracle.solve(solver, prec, A, xo, s.b.copy(), 2000, 1e-8, norm=norm, sum_order=so)
+        return it, xo
+
+    it_o, xo = solve(0)
+    env, spread = parity_bar.envelope(solve, it_o, xo)
+    parity_bar.check(f"fortran {solver} {prec} norm={norm}", niter, x, it_o, xo, env, spread)
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built (no amdflang)")

@@ -4,6 +4,8 @@ solves at the FP64 parity bar."""
 import numpy as np
 import pytest
 
+import parity_bar
+
 pytestmark = pytest.mark.gpu
 
 
@@ -54,24 +56,22 @@ CASES = [
 def test_icc_cg_parity(cases, oracle, gpu, name, make, norm, miter, tol):
     s = make(cases)
     print(f"icc-cg case {name} n={s.n} nnz={s.nnz}", flush=True)
-    res = []
-    for so in (0, 1, 2):
+
+    def solve(so):
         A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
         x = np.zeros(s.n)
         it, fr = oracle.solve("cg", "icc", A, x, s.b.copy(), miter, tol, norm=norm, sum_order=so)
-        res.append((it, fr, x))
-    it_o, res_o, x_o = res[0]
-    env = max(_rel(res[1][2], x_o), _rel(res[2][2], x_o))
+        return it, fr, x
+
+    it_o, res_o, x_o = solve(0)
+    env, spread = parity_bar.envelope(lambda so: solve(so)[::2], it_o, x_o)
     nat = gpu.Native(s.row, s.col, s.dia)
     x = np.zeros(s.n)
     it_g, res_g = nat.solve_native("cg", "icc", s.val.copy(), x, s.b.copy(), miter, tol, norm=norm)
-    err = _rel(x, x_o)
-    print(f"  oracle niter={it_o} res={res_o:.3e}; gpu niter={it_g} res={res_g:.3e}; "
-          f"err {err:.2e} envelope {env:.2e}", flush=True)
+    print(f"  oracle niter={it_o} res={res_o:.3e}; gpu niter={it_g} res={res_g:.3e}", flush=True)
     assert it_o < miter
-    it_env = max(abs(res[1][0] - it_o), abs(res[2][0] - it_o))
-    assert abs(it_g - it_o) <= max(2, 2 * it_env) and res_g < tol
-    assert err <= max(1e-10, 2.0 * env)
+    parity_bar.check(f"icc-cg {name}", it_g, x, it_o, x_o, env, spread)
+    assert res_g < tol
 
 
 def test_icc_bicg_equals_cg_and_caps(cases, oracle, gpu):

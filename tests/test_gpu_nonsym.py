@@ -1,13 +1,15 @@
 """Textbook BiCG / CGS / BiCGSTAB on the GPU against the oracle.
 
 Same bar as the FP64 CG drop-in: iteration count within +-2 of the oracle's,
-x within 1e-10 relative of the oracle's, or within twice the spread that
-reordering the oracle's own global sums causes (the GPU's dot products use a
-different, fixed order; every product row and element-wise update is the
-same arithmetic).
+x within 1e-10 relative of the oracle's, or within the spread that
+reordering the oracle's own global sums causes (parity_bar.py; the GPU's dot
+products use a different, fixed order; every product row and element-wise
+update is the same arithmetic).
 """
 import numpy as np
 import pytest
+
+import parity_bar
 
 pytestmark = pytest.mark.gpu
 
@@ -60,16 +62,15 @@ def test_parity(cases, oracle, gpu, method, prec, name, make, norm, miter, tol, 
     print(f"case {name} {method}/{prec} n={s.n} nnz={s.nnz}", flush=True)
     ci = 0 if crit == "nrms" else 1
     it_o, res_o, x_o = _oracle(oracle, s, method, prec, miter, tol, norm, crit)
-    it_e, _, x_e = _oracle(oracle, s, method, prec, miter, tol, norm, crit, sum_order=1)
-    it_f, _, x_f = _oracle(oracle, s, method, prec, miter, tol, norm, crit, sum_order=2)
+    env, spread = parity_bar.envelope(
+        lambda so: _oracle(oracle, s, method, prec, miter, tol, norm, crit, sum_order=so)[::2],
+        it_o, x_o)
     it_g, res_g, x_g, rep = _gpu(gpu, s, method, prec, miter, tol, norm, ci)
-    env, err = max(_rel(x_e, x_o), _rel(x_f, x_o)), _rel(x_g, x_o)
     print(f"  oracle niter={it_o} res={res_o:.3e}; gpu niter={it_g} res={res_g:.3e}; "
-          f"x rel err={err:.2e} (envelope {env:.2e}, niter {it_e}); "
           f"true relres {rep['true_relres']:.2e}", flush=True)
     assert it_o < miter
-    assert abs(it_g - it_o) <= max(2, 2 * max(abs(it_e - it_o), abs(it_f - it_o))) and res_g < tol
-    assert err <= max(X_TOL, 2.0 * env)
+    parity_bar.check(f"{method} {prec} {name}", it_g, x_g, it_o, x_o, env, spread)
+    assert res_g < tol
 
 
 @pytest.mark.parametrize("method", ["bicg_textbook", "cgs", "bcgs"])
@@ -165,4 +166,4 @@ def test_upwind120_relres(cases, gpu):
         it, res, x, rep = _gpu(gpu, s, method, "jacobi", 2000, 1e-8, criterion=1, nat=nat)
         print(f"upwind120 {method}: {it} iterations, relres {res:.2e}, "
               f"true {rep['true_relres']:.2e}, {rep['solve_ms']:.1f} ms", flush=True)
-        assert it < 2000 and res < 1e-8 and rep["true_relres"] < 1e-7
+        assert it < 2000 and res < 1e-8 and rep["true_relres"] < 1.01e-8

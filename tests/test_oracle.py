@@ -277,14 +277,18 @@ def test_values_round_trip(cases, oracle):
     assert np.all(np.abs(b - s.b) <= 2 * np.spacing(np.abs(s.b)))
 
 
-def test_reordered_sums_stay_close(cases, oracle):
-    """The blocked-pairwise mode (rounding envelope) solves the same problem."""
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_reordered_sums_stay_close(cases, oracle, order):
+    """Each other summation order (the rounding envelope) solves the same
+    problem, and differs from the reference order only by rounding."""
     s = cases.hex_pressure(12, rho_out=10.0)
     xa, xb = np.zeros(s.n), np.zeros(s.n)
     ia, _ = oracle.solve("cg", "jacobi", _mat(oracle, s), xa, s.b.copy(), 1000, 1e-9)
-    ib, _ = oracle.solve("cg", "jacobi", _mat(oracle, s), xb, s.b.copy(), 1000, 1e-9, sum_order=1)
+    ib, _ = oracle.solve("cg", "jacobi", _mat(oracle, s), xb, s.b.copy(), 1000, 1e-9,
+                         sum_order=order)
     assert abs(ia - ib) <= 1
     assert np.linalg.norm(xa - xb) <= 1e-9 * np.linalg.norm(xa)
+    assert not np.array_equal(xa, xb)
 
 
 # ------------------------------------------------------------- fixtures -----
