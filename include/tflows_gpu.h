@@ -36,6 +36,11 @@ extern "C" {
 #define TFG_CRIT_NRMS 0   /* Normalized_Root_Mean_Square < tol (reference)     */
 #define TFG_CRIT_RELRES 1 /* ||b - A x||_2 / ||b||_2 < tol                     */
 
+#define TFG_DEFLATE_AUTO 0  /* FP32_IR: deflate the constant vector when the   */
+                            /* values are symmetric and nearly singular       */
+#define TFG_DEFLATE_OFF 1
+#define TFG_DEFLATE_ON 2
+
 #define TFG_PATH_AUTO 0        /* graph below 2M nonzeros, kernels above       */
 #define TFG_PATH_KERNELS 1     /* one launch per phase, device-side stop flag  */
 #define TFG_PATH_GRAPH 2       /* chunks of iterations replayed as a hipGraph  */
@@ -45,12 +50,15 @@ typedef struct {
   int precision;      /* TFG_FP64 | TFG_FP32_IR                                */
   int criterion;      /* TFG_CRIT_NRMS | TFG_CRIT_RELRES                       */
   int path;           /* TFG_PATH_*                                            */
-  int inner_max;      /* FP32_IR: inner iterations per refinement step (0: 500)*/
-  double inner_tol;   /* FP32_IR: inner residual reduction per step (0: 1e-4)  */
+  int inner_max;      /* FP32_IR: > 0: a refinement at least every inner_max  */
+                      /* FP32 iterations (0: only when due, see inner_tol)     */
+  double inner_tol;   /* FP32_IR: refinement once the FP32 residual fell by   */
+                      /* this factor since the last one (0: 1e-3)              */
   int timing;         /* 1: time sampled SpMV launches with hipEvents          */
   int true_residual;  /* 1: recompute ||b - A x||_2/||b||_2 in FP64 at exit    */
   int textbook_bicg;  /* 1: "bicg" is textbook BiCG (with A^T); 0: the        */
                       /* reference's Bicg.f90, which equals CG                 */
+  int deflation;      /* FP32_IR: TFG_DEFLATE_AUTO | _OFF | _ON               */
 } tfg_options;
 
 typedef struct {
@@ -61,6 +69,7 @@ typedef struct {
   double solve_ms;      /* device time of the solve (hipEvents)               */
   double spmv_ms;       /* mean duration of the sampled SpMV launches         */
   int spmv_samples;     /* number of SpMV launches timed                      */
+  int deflated;         /* FP32_IR: 1 when the constant vector was deflated   */
 } tfg_report;
 
 typedef struct tfg_matrix tfg_matrix;
@@ -108,6 +117,19 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len,
 int tfg_spmv(tfg_matrix* h, const double* d_val, const double* d_p, double* d_q);
 int tfg_residual(tfg_matrix* h, const double* d_val, const double* d_b,
                  const double* d_x, double* d_r);
+
+/* Building block of the FP32 refinement form (no counterpart in the
+ * reference): the FP32 product in the row-excess form,
+ *   y_i = s_i x_i + sum_{j != i} a_ij (x_j - x_i),
+ * of the FP32 matrix made from d_val scaled by fn: s_i = fl32(sum_j a_ij fn)
+ * (each product rounded in FP64, summed in CSR order in FP64) and
+ * fl32(a_ij fn) off the diagonal; the terms are added in CSR order.  d_x,
+ * d_y: FP32 device vectors of nt entries.  Runs the symmetric FP32 engine
+ * when the matrix offers the symmetric engine and d_val is symmetric bit for
+ * bit (then d_x and d_y must be 16-byte aligned), else a CSR engine; not on
+ * an internally ordered matrix or with buffer cells (TFG_ENOTSUP).
+ * Stream-ordered. */
+int tfg_spmv_fp32(tfg_matrix* h, const double* d_val, double fn, const float* d_x, float* d_y);
 
 /* z = M^-1 r with the reference's preconditioner of the unscaled matrix
  * (Prec_Form.f90 + Prec_Solve.f90): "icc" (DIC, level-scheduled, bit-exact

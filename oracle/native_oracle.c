@@ -904,3 +904,224 @@ void or_rhie_chow_fluxes(int nc, int n_faces, const int* c1, const int* c2,
   }
   free(pst); free(uc); free(tm);
 }
+
+/* ==========================================================================
+ * FP32 sweeps inside FP64 refinement: the library's TFG_FP32_IR form.  No
+ * counterpart in the reference, which solves in FP64 only; restated here so
+ * that the GPU's FP32 product is checked bit for bit and its solves step by
+ * step.  The FP64 parts (normalisation, prologue, true residual, stopping
+ * test) are Cg.f90's, as or_cg_ex above.
+ * ========================================================================== */
+
+/* Neumaier's compensated sum: the running sum and the rounding errors of
+ * its additions, added at the end. */
+typedef struct {
+  double s, c;
+} or_csum;
+
+static void csum_add(or_csum* a, double v) {
+  const double t = a->s + v;
+  if (fabs(a->s) >= fabs(v)) a->c = a->c + ((a->s - t) + v);
+  else a->c = a->c + ((v - t) + a->s);
+  a->s = t;
+}
+
+/* Row sum s_i = sum_j a_ij fn of row i (the products rounded in FP64),
+ * compensated, in CSR order: the row's exact sum to within an FP64
+ * rounding, even where it cancels to the rounding level of the entries. */
+static double row_excess(const or_matrix* A, int i, double fn) {
+  or_csum a = {0.0, 0.0};
+  for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) csum_add(&a, A->val[j] * fn);
+  return a.s + a.c;
+}
+
+/* FP32 matrix of the row-excess form, from the FP64 values scaled by fn:
+ * the diagonal slot of row i holds fl32(s_i), every other slot
+ * fl32(a_ij fn).  Rows 0..ni-1. */
+void or_lap32_form(const or_matrix* A, int ni, double fn, float* v32) {
+  for (int i = 0; i < ni; i++) {
+    for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) v32[j] = (float)(A->val[j] * fn);
+    v32[A->dia[i] - 1] = (float)row_excess(A, i, fn);
+  }
+}
+
+/* y_i = s_i x_i + sum_{j != i} a_ij (x_j - x_i), FP32, terms in CSR order
+ * (diagonal first).  Exact in exact arithmetic (it is A x rewritten); in
+ * FP32 a vector constant over a row's cells costs no rounding beyond s_i's,
+ * so the nearly singular pressure systems keep their small eigenvalues. */
+void or_lap32_spmv(const or_matrix* A, int ni, const float* v32, const float* x, float* y) {
+  for (int i = 0; i < ni; i++) {
+    const float xi = x[i];
+    float acc = v32[A->dia[i] - 1] * xi;
+    for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) {
+      const int k = A->col[j] - 1;
+      if (k == i) continue;
+      acc = acc + v32[j] * (x[k] - xi);
+    }
+    y[i] = acc;
+  }
+}
+
+static double dot32(const float* a, const float* b, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s = s + (double)a[i] * (double)b[i];
+  return s;
+}
+
+/* FP32 Jacobi / unpreconditioned CG in the row-excess form with FP64
+ * reliable updates; with `deflate`, the constant vector w is deflated
+ * (deflated PCG, Saad, Yeung, Erhel and Guyomarc'h 2000): E = w'Aw =
+ * sum_i s_i, every direction p = z + beta p - w (s'z) / E, and every
+ * restart begins with the coarse correction x += c w, c = w'(b - A x) / E
+ * computed as (sum_i b_i - sum_i s_i x_i) / E: with the compensated row
+ * sums s_i and a compensated sum of b this is accurate, where summing the
+ * residual itself would carry its rows' rounding errors, which 1 / E
+ * amplifies on a nearly singular matrix.  The Krylov vectors r, p, q, the
+ * correction e and the product are FP32; alpha, beta and every dot product
+ * are FP64 (each FP32 product is exact in FP64).  Every `check` iterations
+ * (and at miter) the FP32 residual is looked at; a refinement follows when
+ * it has dropped by `delta` since the last restart, when its estimate of
+ * the stopping measure is below tol, when `force` iterations (> 0) have
+ * passed, or at miter: x += e in FP64, the FP64 true residual b - A x, the
+ * stopping test on it, and the FP32 CG restarts from it.  niter counts FP32
+ * iterations; fin_res is the measure of the last FP64 true residual.
+ * Returns 0, 1 on breakdown ((p, q) = 0 or not finite: x keeps the
+ * correction so far), -2 out of memory. */
+int or_ir_cg(or_matrix* A, double* x, double* b, int prec, int miter, int* niter,
+             double tol, double* fin_res, const double* norm, int crit, double delta,
+             int check, int force, int deflate, int* refinements) {
+  const int nt = A->nt, ni = A->ni;
+  const long long nnz = A->row[nt] - 1;
+  double* r64 = (double*)calloc((size_t)nt, sizeof(double));
+  double* s64 = (double*)calloc((size_t)nt, sizeof(double));
+  float* v32 = (float*)malloc(sizeof(float) * (size_t)(nnz > 0 ? nnz : 1));
+  float* rd = (float*)malloc(sizeof(float) * (size_t)nt);
+  float* r = (float*)calloc((size_t)nt, sizeof(float));
+  float* z = (float*)calloc((size_t)nt, sizeof(float));
+  float* p = (float*)calloc((size_t)nt, sizeof(float));
+  float* q = (float*)calloc((size_t)nt, sizeof(float));
+  float* e = (float*)calloc((size_t)nt, sizeof(float));
+  if (!r64 || !s64 || !v32 || !rd || !r || !z || !p || !q || !e) {
+    free(r64); free(s64); free(v32); free(rd); free(r); free(z); free(p); free(q); free(e);
+    return -2;
+  }
+  double res = 0.0, rho = 0.0, rref = 0.0;
+  int iter = 0, nref = 0, brk = 0;
+  if (check < 1) check = 1;
+
+  const double fn = normalise(A, b);
+  or_lap32_form(A, ni, 1.0, v32);             /* values already scaled */
+  double E = 0.0;
+  or_csum bsum = {0.0, 0.0};
+  for (int i = 0; i < ni; i++) {
+    rd[i] = (float)(1.0 / A->val[A->dia[i] - 1]);
+    s64[i] = row_excess(A, i, 1.0);
+    E = E + s64[i];
+    csum_add(&bsum, b[i]);
+  }
+  if (!(E > 0.0)) deflate = 0;
+  const double bnorm = sqrt(sumsq(b, ni));
+  const double bnrm2 = crit == OR_CRIT_RELRES ? (bnorm > 0.0 ? 1.0 : 0.0)
+                                              : or_nrms(A, ni, b, x, norm);
+  if (bnrm2 < tol) goto done;
+  or_residual(A, ni, r64, b, x);
+  res = measure(A, ni, r64, x, norm, crit, bnorm);
+  if (res < tol) goto done;
+
+  {
+    int klast = 0;
+    double xmin = 0.0, xmax = 0.0;
+    for (int k = 0; k <= miter; k++) {
+      if (k == 0 || klast == k) {             /* start / restart from r64 */
+        xmin = HUGE_VAL;                      /* x range of the true residual */
+        xmax = -HUGE_VAL;
+        double sx = 0.0;
+        for (int i = 0; i < ni; i++) {
+          xmin = fmin(xmin, x[i]);
+          xmax = fmax(xmax, x[i]);
+          sx = sx + s64[i] * x[i];
+        }
+        double c = 0.0;
+        if (deflate) {                        /* x += c w: then w'r = 0 */
+          c = ((bsum.s - sx) + bsum.c) / E;
+          for (int i = 0; i < ni; i++) x[i] = x[i] + c;
+        }
+        for (int i = 0; i < ni; i++) {        /* r - c A w = r - c s */
+          r[i] = (float)(r64[i] - c * s64[i]);
+          z[i] = prec == OR_PREC_JACOBI ? r[i] * rd[i] : r[i];
+        }
+        float mu = 0.0f;                      /* p = z - w (s'z) / E */
+        if (deflate) {
+          double g = 0.0;
+          for (int i = 0; i < ni; i++) g = g + (double)(float)s64[i] * (double)z[i];
+          mu = (float)(g / E);
+        }
+        for (int i = 0; i < ni; i++) p[i] = z[i] - mu;
+        rref = dot32(r, r, ni);
+        rho = dot32(z, r, ni);
+        continue;
+      }
+      or_lap32_spmv(A, ni, v32, p, q);
+      const double pq = dot32(p, q, ni);
+      if (pq == 0.0 || !isfinite(pq)) {       /* breakdown: keep the correction */
+        for (int i = 0; i < ni; i++) x[i] = x[i] + (double)e[i];
+        iter = k;
+        brk = 1;
+        goto done;
+      }
+      const double alpha = rho / pq;
+      const float a32 = (float)alpha;
+      for (int i = 0; i < ni; i++) {
+        e[i] = e[i] + a32 * p[i];
+        r[i] = r[i] - a32 * q[i];
+        z[i] = prec == OR_PREC_JACOBI ? r[i] * rd[i] : r[i];
+      }
+      const double rz = dot32(z, r, ni), rr = dot32(r, r, ni);
+      iter = k;
+      if (k % check == 0 || k == miter) {
+        double est;
+        if (crit == OR_CRIT_RELRES) {
+          est = sqrt(rr) / bnorm;
+        } else {
+          double s = 0.0;
+          for (int i = 0; i < ni; i++) {
+            const double t = (double)(r[i] * rd[i]);
+            s = s + t * t;
+          }
+          const double range = norm ? fmax(*norm - 0.0, OR_TINY) : fmax(xmax - xmin, OR_TINY);
+          est = sqrt(s) / range;
+        }
+        if (rr <= delta * delta * rref || est < tol || (force > 0 && k - klast >= force) ||
+            k == miter || !isfinite(rr)) {
+          for (int i = 0; i < ni; i++) {
+            x[i] = x[i] + (double)e[i];
+            e[i] = 0.0f;
+          }
+          or_residual(A, ni, r64, b, x);
+          res = measure(A, ni, r64, x, norm, crit, bnorm);
+          nref++;
+          if (res < tol || !isfinite(res) || k == miter) goto done;
+          klast = k;
+          k--;                                /* restart at the top, same k */
+          continue;
+        }
+      }
+      const float beta = (float)(rz / rho);
+      rho = rz;
+      float mu = 0.0f;
+      if (deflate) {
+        double g = 0.0;
+        for (int i = 0; i < ni; i++) g = g + (double)(float)s64[i] * (double)z[i];
+        mu = (float)(g / E);
+      }
+      for (int i = 0; i < ni; i++) p[i] = (z[i] + beta * p[i]) - mu;
+    }
+  }
+done:
+  denormalise(A, b, fn);
+  *fin_res = res;
+  *niter = iter;
+  if (refinements) *refinements = nref;
+  free(r64); free(s64); free(v32); free(rd); free(r); free(z); free(p); free(q); free(e);
+  return brk ? 1 : 0;
+}

@@ -117,6 +117,16 @@ void or_rhie_chow_fluxes(int nc, int n_faces, const int* c1, const int* c2,
 void or_set_sum_order(int order);
 void or_set_progress(int every);
 
+/* FP32 sweeps inside FP64 refinement (the library's TFG_FP32_IR form; no
+ * counterpart in the reference).  or_lap32_form: the FP32 row-excess matrix
+ * from val * fn; or_lap32_spmv: its product; or_ir_cg: the solve (see the
+ * definition). */
+void or_lap32_form(const or_matrix* A, int ni, double fn, float* v32);
+void or_lap32_spmv(const or_matrix* A, int ni, const float* v32, const float* x, float* y);
+int or_ir_cg(or_matrix* A, double* x, double* b, int prec, int miter, int* niter,
+             double tol, double* fin_res, const double* norm, int crit, double delta,
+             int check, int force, int deflate, int* refinements);
+
 #ifdef __cplusplus
 }
 #endif

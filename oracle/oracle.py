@@ -57,6 +57,10 @@ def lib(omp=False):
         L.or_cg_ex.argtypes = [P, P, P, I, I, P, D, P, P, I]
         L.or_set_sum_order.argtypes = [I]
         L.or_set_progress.argtypes = [I]
+        L.or_lap32_form.argtypes = [P, I, D, P]
+        L.or_lap32_spmv.argtypes = [P, I, P, P, P]
+        L.or_ir_cg.restype = I
+        L.or_ir_cg.argtypes = [P, P, P, I, I, P, D, P, P, I, D, I, I, I, P]
         L.or_krylov.restype = I
         L.or_krylov.argtypes = [P, P, P, I, I, I, P, D, P, P, I, P]
         L.or_transpose_positions.restype = I
@@ -203,6 +207,42 @@ def _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin, om
     if rc != 0:
         raise MemoryError("oracle scratch allocation failed")
     return niter.value, fin.value
+
+
+def lap32_form(A, fn=1.0):
+    """FP32 row-excess matrix of the FP32 sweeps: s_i = fl32(sum_j a_ij fn)
+    in the diagonal slot, fl32(a_ij fn) elsewhere."""
+    v = np.zeros(len(A.col), dtype=np.float32)
+    lib().or_lap32_form(A.ref, A.ni, float(fn), _p(v))
+    return v
+
+
+def lap32_spmv(A, v32, x):
+    """y = s_i x_i + sum_{j != i} a_ij (x_j - x_i) in FP32, CSR order."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros(A.nt, dtype=np.float32)
+    lib().or_lap32_spmv(A.ref, A.ni, _p(np.ascontiguousarray(v32, dtype=np.float32)), _p(x), _p(y))
+    return y
+
+
+def ir_solve(prec, A, x, b, miter, tol, norm=None, criterion="nrms", delta=0.1, check=1,
+             force=0, deflate=0):
+    """FP32 CG (row-excess product) inside FP64 refinement, restated from the
+    library's TFG_FP32_IR form: refinement decisions every `check`
+    iterations, when the FP32 residual fell by `delta`, its estimate is below
+    tol, `force` iterations passed, or at miter; `deflate` deflates the
+    constant vector.  Returns (niter, fin_res, refinements)."""
+    niter, fin, nref = ctypes.c_int(0), ctypes.c_double(0.0), ctypes.c_int(0)
+    nv = ctypes.c_double(norm) if norm is not None else None
+    rc = lib().or_ir_cg(A.ref, _p(x), _p(b), PREC.get(prec, 0), int(miter), ctypes.byref(niter),
+                        float(tol), ctypes.byref(fin), ctypes.byref(nv) if nv is not None else None,
+                        CRIT[criterion], float(delta), int(check), int(force), int(deflate),
+                        ctypes.byref(nref))
+    if rc == 1:
+        raise BreakdownError(niter.value, fin.value)
+    if rc != 0:
+        raise MemoryError("oracle scratch allocation failed")
+    return niter.value, fin.value, nref.value
 
 
 def transpose_positions(A):

@@ -111,8 +111,18 @@ struct tfg_matrix {
   hipEvent_t ev_poll[2] = {nullptr, nullptr};
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
   std::vector<hipEvent_t> ev_sample;
-  // FP32 inner-solve work (allocated on first use)
+  // FP32 sweeps of the refinement form (allocated on first use): the FP32
+  // matrix (CSR val32, or the symmetric storage sd32 with its 4-row blocks),
+  // row sums, 1 / a_ii, vectors
   float* val32 = nullptr;
+  float* sd32 = nullptr;
+  long long stride32 = 0;
+  int* sorder32 = nullptr;
+  int nsorder32 = 0;
+  int quad32 = 0;
+  int grid32 = 0;
+  double* s64 = nullptr;
+  float* s32 = nullptr;
   float* e32 = nullptr;
   float* r32 = nullptr;
   float* p32 = nullptr;
@@ -173,8 +183,8 @@ struct tfg_matrix {
 namespace {
 
 // Rows of the partials buffer: diag 1, prologue 7, spmv 1, update 5, aux 2,
-// ext 6.
-constexpr int kPartRows = 22;
+// ext 6, irc 4.
+constexpr int kPartRows = 26;
 
 tfg::Partials partials_of(const tfg_matrix* h, const tfg::Launch& L) {
   tfg::Partials pt;
@@ -185,6 +195,7 @@ tfg::Partials partials_of(const tfg_matrix* h, const tfg::Launch& L) {
   pt.upd = pt.spmv + 1 * (size_t)h->pstride;
   pt.aux = pt.upd + 5 * (size_t)h->pstride;
   pt.ext = pt.aux + 2 * (size_t)h->pstride;
+  pt.irc = pt.ext + 6 * (size_t)h->pstride;
   pt.g_vec = L.grid_vec;
   pt.g_spmv = L.grid_spmv;
   pt.g_upd = L.grid_vec;
@@ -330,8 +341,8 @@ void free_all(tfg_matrix* h) {
   if (h->inner) free_all(h->inner);
   h->inner = nullptr;
   void* dev[] = {h->row, h->col, h->blk[0], h->blk[1], h->val, h->b, h->x, h->r, h->p, h->q, h->ad,
-                 h->part, h->st, h->st_in, h->val32, h->e32, h->r32,
-                 h->p32, h->q32, h->ad32, h->rt, h->nv[0], h->nv[1], h->nv[2],
+                 h->part, h->st, h->st_in, h->val32, h->sd32, h->sorder32, h->s64, h->s32, h->e32,
+                 h->r32, h->p32, h->q32, h->ad32, h->rt, h->nv[0], h->nv[1], h->nv[2],
                  h->nv[3], h->valt, h->tpos, h->code, h->tab, h->pat, h->poff, h->plen,
                  h->pslot, h->sym, h->sym_flag, h->sorder,
                  h->loc, h->icc_rows[0],
@@ -438,14 +449,9 @@ int tune_all(tfg_matrix* h, const double* val) {
   return TFG_OK;
 }
 
-// Work vectors of the textbook methods; for BiCG also the transposed-entry
-// map (built once per handle on the device) and the transposed values.
-int ensure_nonsym(tfg_matrix* h, int method, const double* val, const tfg::Launch& L) {
-  const size_t n = (size_t)h->nt;
-  if (!h->rt) HIP_TRY(hipMalloc(&h->rt, n * sizeof(double)));
-  for (double*& v : h->nv)
-    if (!v) HIP_TRY(hipMalloc(&v, n * sizeof(double)));
-  if (method != tfg::NS_BICG) return TFG_OK;
+// Position of every entry's transposed entry (built once per handle on the
+// device); h->tpos_state -1 when the pattern is not symmetric.
+int ensure_tpos(tfg_matrix* h) {
   if (h->tpos_state == 0) {
     HIP_TRY(hipMalloc(&h->tpos, sizeof(int) * (size_t)h->nnz));
     int* bad = nullptr;
@@ -459,6 +465,18 @@ int ensure_nonsym(tfg_matrix* h, int method, const double* val, const tfg::Launc
     HIP_TRY(e);
     h->tpos_state = hb ? -1 : 1;
   }
+  return TFG_OK;
+}
+
+// Work vectors of the textbook methods; for BiCG also the transposed-entry
+// map and the transposed values.
+int ensure_nonsym(tfg_matrix* h, int method, const double* val, const tfg::Launch& L) {
+  const size_t n = (size_t)h->nt;
+  if (!h->rt) HIP_TRY(hipMalloc(&h->rt, n * sizeof(double)));
+  for (double*& v : h->nv)
+    if (!v) HIP_TRY(hipMalloc(&v, n * sizeof(double)));
+  if (method != tfg::NS_BICG) return TFG_OK;
+  { const int rt = ensure_tpos(h); if (rt != TFG_OK) return rt; }
   if (h->tpos_state < 0)
     return fail(TFG_ENOTSUP, "textbook BiCG needs a structurally symmetric matrix");
   if (!h->valt) HIP_TRY(hipMalloc(&h->valt, sizeof(double) * (size_t)h->nnz));
@@ -533,6 +551,54 @@ tfg::IccArgs icc_args(const tfg_matrix* h) {
   c.bx = h->bx;
   c.toff_host = h->box_toff.data();
   return c;
+}
+
+// Processing order of the symmetric engines: planes of D rows (D = the
+// largest offset; plane starts rounded down to a multiple of `align` rows),
+// blocks of at most R rows inside a plane, taken band by band (8
+// consecutive blocks of a plane) and, within a band, plane after plane.
+// The workgroups that run at the same time then cover one band over
+// neighbouring planes, so most rows a row reads (one plane or one line
+// away) are read by them too and stay in L2: 377 instead of 522 MB fetched
+// per FP64 product at 200^3.  (start, end) pairs; every row exactly once,
+// starts and lengths multiples of align (ni a multiple of align); empty if
+// that fails.
+std::vector<int> sym_blocks(int ni, int D, int R, int align) {
+  // a plane may gain up to align - 1 rows (at least one is allowed for)
+  const long long bpp = ((long long)D + std::max(align, 2) - 1 + R - 1) / R;
+  constexpr int kBand = 8;
+  const long long amask = ~((long long)align - 1);
+  std::vector<int> ord;
+  if (D >= kBand * R) {
+    const long long planes = ((long long)ni + D - 1) / D;
+    auto start = [&](long long x) { return std::min<long long>(ni, (x * D) & amask); };
+    for (long long j0 = 0; j0 < bpp; j0 += kBand)
+      for (long long x = 0; x < planes; ++x)
+        for (long long b = j0; b < std::min<long long>(bpp, j0 + kBand); ++b) {
+          const long long r0 = start(x) + b * R;
+          const long long r1 = std::min<long long>({r0 + R, start(x + 1), (long long)ni});
+          if (r0 < r1) {
+            ord.push_back((int)r0);
+            ord.push_back((int)r1);
+          }
+        }
+  } else {
+    for (int r0 = 0; r0 < ni; r0 += R) {
+      ord.push_back(r0);
+      ord.push_back(std::min(ni, r0 + R));
+    }
+  }
+  std::vector<char> seen((size_t)ni, 0);
+  for (size_t t = 0; t + 1 < ord.size(); t += 2) {
+    if ((ord[t] % align) || ((ord[t + 1] - ord[t]) % align)) return {};
+    for (int r = ord[t]; r < ord[t + 1]; ++r) {
+      if (seen[r]) return {};
+      seen[r] = 1;
+    }
+  }
+  for (int r = 0; r < ni; ++r)
+    if (!seen[r]) return {};
+  return ord;
 }
 
 // Box grid of the tiled DIC sweeps: rows (x, y, z) of an nx*ny*nz grid, x
@@ -610,26 +676,78 @@ int box_setup(tfg_matrix* h, const int* row0, const int* col0) {
   return TFG_OK;
 }
 
+// FP32 sweeps of the symmetric storage: zero-padded slots (the row sums
+// and the upper diagonals, refilled per solve) and their row blocks, four
+// rows per thread when the offsets allow.
+int ensure_sym32(tfg_matrix* h) {
+  if (h->sd32) return TFG_OK;
+  const int D = h->sdg.off[h->nsym - 1];
+  h->stride32 = ((long long)h->ni + D + 4 + 3) & ~3LL;
+  bool quad = h->sdg.off[0] == 1 && (h->ni % 4) == 0 && h->nsym <= tfg::kSymPairMax;
+  for (int k = 1; k < h->nsym; ++k) quad = quad && (h->sdg.off[k] % 4) == 0;
+  std::vector<int> ord = sym_blocks(h->ni, D, quad ? 4 * tfg::kRowsPerBlock : tfg::kRowsPerBlock,
+                                    quad ? 4 : 1);
+  if (ord.empty() || 4 * h->stride32 * (h->nsym + 1) >= (1LL << 31))
+    return fail(TFG_ENOTSUP, "FP32 symmetric storage does not fit");
+  const size_t bytes = sizeof(float) * (size_t)(h->nsym + 1) * (size_t)h->stride32;
+  HIP_TRY(hipMalloc(&h->sd32, bytes));
+  HIP_TRY(hipMemsetAsync(h->sd32, 0, bytes, h->stream));
+  HIP_TRY(hipMalloc(&h->sorder32, sizeof(int) * ord.size()));
+  HIP_TRY(hipMemcpy(h->sorder32, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+  h->nsorder32 = (int)ord.size() / 2;
+  h->quad32 = quad ? 1 : 0;
+  const int per_cu = std::max(1, tfg::spmv_blocks_per_cu(tfg::K_SYM_DIA));
+  int g = std::min(h->nsorder32, std::max(1, h->cu_count) * per_cu);
+  if (g >= tfg::kXcds) g -= g % tfg::kXcds;
+  h->grid32 = std::max(1, std::min(g, h->pstride));
+  return TFG_OK;
+}
+
 }  // namespace
 
 namespace tfg {
 
-// FP32 inner CG inside FP64 iterative refinement (kernels in tfg_ir.hip).
-// niter = total inner iterations (capped by miter); fin_res = the criterion
-// on the last FP64 true residual.
+// FP32 sweeps inside FP64 refinement (kernels in tfg_ir.hip): the CG of
+// Cg.f90 in FP32 on the row-excess matrix, with the constant vector
+// deflated, and FP64 refinements (true residual, stopping test, restart)
+// decided on the device every chunk of `check` iterations.  niter = FP32
+// iterations; fin_res = the criterion on the last FP64 true residual.
 int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int miter,
              int* niter, double tol, double* fin_res, const double* norm,
              const tfg_options& o, tfg_report* rep) {
   const size_t n = (size_t)h->nt;
-  if (!h->val32) HIP_TRY(hipMalloc(&h->val32, sizeof(float) * (size_t)h->nnz));
-  if (!h->ad32) HIP_TRY(hipMalloc(&h->ad32, sizeof(float) * n));
-  if (!h->r32) HIP_TRY(hipMalloc(&h->r32, sizeof(float) * n));
-  if (!h->e32) HIP_TRY(hipMalloc(&h->e32, sizeof(float) * n));
-  if (!h->p32) HIP_TRY(hipMalloc(&h->p32, sizeof(float) * n));
-  if (!h->q32) HIP_TRY(hipMalloc(&h->q32, sizeof(float) * n));
   { const int rt = ensure_tuned(h, val); if (rt != TFG_OK) return rt; }
-  const tfg::Launch L = launch_of(h, val, h->kind);
-  const tfg::Launch L32 = launch_of(h, h->val32, h->kind);
+  for (float** v : {&h->ad32, &h->s32, &h->r32, &h->e32, &h->p32, &h->q32})
+    if (!*v) HIP_TRY(hipMalloc(v, sizeof(float) * std::max<size_t>(n, 4)));
+  if (!h->s64) HIP_TRY(hipMalloc(&h->s64, sizeof(double) * std::max<size_t>(n, 4)));
+  // the FP32 product: the symmetric storage when the matrix offers it and
+  // val is symmetric bit for bit, else a CSR engine on val32
+  bool sym = false;
+  if (h->nsym > 0) {
+    const int rs = sym_prepare(h, val, &sym);
+    if (rs != TFG_OK) return rs;
+  }
+  if (sym) {
+    const int r3 = ensure_sym32(h);
+    if (r3 != TFG_OK) return r3;
+  } else if (!h->val32) {
+    HIP_TRY(hipMalloc(&h->val32, sizeof(float) * (size_t)std::max(h->nnz, 4)));
+  }
+  tfg::Launch L32 = launch_of(h, h->val32, h->kind);
+  if (sym) {
+    L32.kind = tfg::K_SYM_DIA;
+    L32.grid_spmv = h->grid32;
+  } else {
+    if (L32.kind != tfg::K_STAGE2048 && L32.kind != tfg::K_PIPE2048_CODED &&
+        L32.kind != tfg::K_PIPE_PATTERN)
+      L32.kind = tfg::K_PIPE2048;
+    L32.grid_spmv = h->grid[L32.kind];
+    L32.vec_val = h->nnz >= 4;
+  }
+  // one grid for every product-shaped kernel of this solve (their partials
+  // are swept with it)
+  tfg::Launch L = launch_of(h, val, h->kind);
+  L.grid_spmv = L32.grid_spmv;
   tfg::CgArgs a = args_of(h, h->kind);
   a.val = val;
   a.b = b;
@@ -643,20 +761,29 @@ int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int mi
   tfg::IrArgs f{};
   f.n = h->ni;
   f.val32 = h->val32;
-  f.dinv32 = h->ad32;
+  f.sd32 = h->sd32;
+  f.stride32 = h->stride32;
+  f.sorder32 = h->sorder32;
+  f.nsorder32 = h->nsorder32;
+  f.quad = h->quad32;
+  f.s64 = h->s64;
+  f.s32 = h->s32;
+  f.rd32 = h->ad32;
   f.r32 = h->r32;
-  f.e32 = h->e32;
   f.p32 = h->p32;
   f.q32 = h->q32;
-  f.st_in = h->st_in;
-  const int ipr = prec == tfg::PREC_JACOBI ? tfg::PREC_JACOBI : tfg::PREC_NONE;
-  // Refinement (and restart of the FP32 CG) when its recursive residual has
-  // dropped by inner_tol (default 1e-4) since the last refinement, and every
-  // inner_max iterations when inner_max > 0.  (Keeping the FP32 direction
-  // across refinements -- residual replacement -- stagnates on the nearly
-  // singular pinned-pressure systems, so every refinement restarts.)
-  const double delta = o.inner_tol > 0.0 ? o.inner_tol : 1e-4;
-  const int restart = o.inner_max > 0 ? o.inner_max : 0;
+  f.e32 = h->e32;
+  f.prec = prec == tfg::PREC_JACOBI ? tfg::PREC_JACOBI : tfg::PREC_NONE;
+  if (o.deflation < TFG_DEFLATE_AUTO || o.deflation > TFG_DEFLATE_ON)
+    return fail(TFG_EINVAL, "unknown deflation mode %d", o.deflation);
+  // deflation needs symmetric values: the symmetric storage checked them;
+  // a CSR matrix is checked against its transpose map
+  bool check_sym = false;
+  if (!sym && o.deflation != TFG_DEFLATE_OFF) {
+    const int rt = ensure_tpos(h);
+    if (rt != TFG_OK) return rt;
+    check_sym = h->tpos_state > 0;
+  }
 
   State* s0 = &h->hst[0];
   memset(s0, 0, sizeof(State));
@@ -664,6 +791,13 @@ int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int mi
   s0->criterion = o.criterion;
   s0->has_norm = norm ? 1 : 0;
   s0->norm = norm ? *norm : 0.0;
+  s0->kcur = 1;
+  s0->refine = 1;   // the start runs the restart kernels
+  s0->miter = std::max(miter, 0);
+  s0->delta = o.inner_tol > 0.0 ? o.inner_tol : 1e-3;
+  s0->force = std::max(o.inner_max, 0);
+  s0->defl_mode = o.deflation;
+  s0->sym_vals = (sym || check_sym) ? 1 : 0;
   HIP_TRY(hipMemcpyAsync(h->st, s0, sizeof(State), hipMemcpyHostToDevice, h->stream));
 
   const bool timing = o.timing != 0;
@@ -673,116 +807,77 @@ int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int mi
     for (auto& e : h->ev_sample) HIP_TRY(hipEventCreate(&e));
   }
   std::vector<int> sample_k;
-  int status = TFG_OK, refinements = 0;
-  State* sf = &h->hst[3];
+  const int ipr = f.prec;
+  const bool nrms = o.criterion != TFG_CRIT_RELRES;
+  // the deflated update reads s; when deflation is off for certain, it
+  // need not
+  const bool defl_kernels = o.deflation != TFG_DEFLATE_OFF && (sym || check_sym);
 
   HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
   HIP_TRY(tfg::launch_diag_sum(L, a));
   HIP_TRY(tfg::launch_prologue(L, a, ipr));
-  HIP_TRY(tfg::launch_ir_convert(L, a, f, ipr, h->nnz));
-  HIP_TRY(tfg::launch_ir_check(L, a, 0));
-  HIP_TRY(tfg::launch_ir_zero(L, f));
-  HIP_TRY(tfg::launch_ir_replace(L, a, f, ipr));
+  HIP_TRY(tfg::launch_start(L, a));
+  HIP_TRY(tfg::launch_ir_convert(L, L32, a, f, h->nnz));
+  if (check_sym)
+    HIP_TRY(tfg::launch_ir_symcheck(h->stream, L.grid_vec, val, h->tpos, h->nnz, h->st));
+  HIP_TRY(tfg::launch_ir_restart(L, a, f));
 
-  const int chunk = h->nnz > (1 << 22) ? 8 : 32;
-  int k = 1, first = 1, slot = 0, prev = -1, last_refine = 0;
-  bool stop = false, ignore_next = false;
-  // refinement: fold e32 into x, FP64 true residual, stopping test (st->iter
-  // records the iteration count K), residual replacement
-  auto refine = [&](int K, bool restart_dir) -> int {
-    HIP_TRY(tfg::launch_ir_correct(L, a, f));
-    HIP_TRY(tfg::launch_prologue(L, a, ipr));
-    HIP_TRY(tfg::launch_ir_check(L, a, K));
-    HIP_TRY(tfg::launch_ir_replace(L, a, f, ipr));
-    ++refinements;
-    last_refine = K;
-    if (restart_dir) first = 1;
-    return TFG_OK;
-  };
+  const int check = 8;  // refinement decisions at chunk ends
+  int k = 1, slot = 0, prev = -1;
+  bool stop = false;
   while (k <= miter && !stop) {
-    int kend = std::min(miter, k + chunk - 1);
-    if (restart) kend = std::min(kend, last_refine + restart);
+    const int kend = std::min(miter, k + check - 1);
     for (; k <= kend; ++k) {
-      HIP_TRY(tfg::launch_ir_pupdate(L, a, f, ipr, k, first));
-      first = 0;
+      HIP_TRY(tfg::launch_ir_pupdate(L, a, f));
       const bool take = timing && (k % 4 == 1) && (int)sample_k.size() < max_samples;
       if (take) HIP_TRY(hipEventRecord(h->ev_sample[2 * sample_k.size()], h->stream));
-      HIP_TRY(tfg::launch_ir_spmv_pq(L32, a, f, k));
+      HIP_TRY(tfg::launch_ir_spmv(L32, a, f));
       if (take) {
         HIP_TRY(hipEventRecord(h->ev_sample[2 * sample_k.size() + 1], h->stream));
         sample_k.push_back(k);
       }
-      HIP_TRY(tfg::launch_ir_update(L, a, f, ipr, k));
+      HIP_TRY(tfg::launch_ir_update(L, a, f, defl_kernels, nrms));
     }
-    HIP_TRY(tfg::launch_ir_finish(L, a, f, kend, delta));
-    if (restart && kend == last_refine + restart && kend < miter) {
-      const int rr = refine(kend, true);
-      if (rr != TFG_OK) return rr;
-      ignore_next = true;
-    }
-    HIP_TRY(hipMemcpyAsync(&h->hst[1 + slot], h->st_in, sizeof(State), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(tfg::launch_ir_refine(L, a, f));
+    HIP_TRY(hipMemcpyAsync(&h->hst[1 + slot], h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipEventRecord(h->ev_poll[slot], h->stream));
     if (prev >= 0) {
       HIP_TRY(hipEventSynchronize(h->ev_poll[prev]));
-      const State& sp = h->hst[1 + prev];
-      if (sp.done) {
-        stop = true;
-      } else if (ignore_next) {
-        ignore_next = false;  // this flag predates the refinement just queued
-      } else if (sp.aux0 != 0.0 && k <= miter) {
-        const int rr = refine(k - 1, true);
-        if (rr != TFG_OK) return rr;
-        ignore_next = true;
-      }
+      if (h->hst[1 + prev].done) stop = true;
     }
     prev = slot;
     slot ^= 1;
   }
-  // final fold of the FP32 correction and the stopping measure on the
-  // FP64 true residual (a no-op fold when the last refinement converged)
+  State* sf = &h->hst[3];
   HIP_TRY(hipMemcpyAsync(sf, h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  int iters;
-  if (sf->done) {
-    iters = sf->iter;
-  } else {
-    iters = std::max(0, std::min(k - 1, miter));
-    HIP_TRY(tfg::launch_ir_correct(L, a, f));
-    HIP_TRY(tfg::launch_prologue(L, a, ipr));
-    HIP_TRY(tfg::launch_ir_check(L, a, iters));
-    ++refinements;
+  int status = sf->status;
+  if (status == TFG_EBREAKDOWN) {  // the correction since the last refinement
+    HIP_TRY(tfg::launch_ir_fold(L, a, f));
   }
-  HIP_TRY(hipMemcpyAsync(&h->hst[4], h->st_in, sizeof(State), hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->hst[4].status == TFG_EBREAKDOWN) status = TFG_EBREAKDOWN;
-  double sample_ms = 0.0;
-  int samples = 0;
-  if (timing) {
-    for (size_t q = 0; q < sample_k.size(); ++q) {
-      if (sample_k[q] > iters) break;
-      float e = 0.f;
-      HIP_TRY(hipEventElapsedTime(&e, h->ev_sample[2 * q], h->ev_sample[2 * q + 1]));
-      sample_ms += e;
-      ++samples;
-    }
-  }
-  const int outer = refinements;
-  const long long total = iters;
   HIP_TRY(hipEventRecord(h->ev_t1, h->stream));
-  HIP_TRY(hipMemcpyAsync(sf, h->st, sizeof(State), hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  if (sf->status == TFG_EBREAKDOWN) status = TFG_EBREAKDOWN;
-  *niter = (int)std::min<long long>(total, INT32_MAX);
+  HIP_TRY(hipEventSynchronize(h->ev_t1));
+  *niter = sf->done ? sf->niter : std::max(miter, 0);
   *fin_res = sf->res;
   if (rep) {
     rep->status = status;
     rep->iterations = *niter;
-    rep->refinements = outer;
+    rep->refinements = sf->nrefine;
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, h->ev_t0, h->ev_t1));
     rep->solve_ms = ms;
-    rep->spmv_ms = samples ? sample_ms / samples : 0.0;
-    rep->spmv_samples = samples;
+    double tot = 0.0;
+    int cnt = 0;
+    for (size_t q = 0; q < sample_k.size(); ++q) {
+      if (sample_k[q] > *niter) break;
+      float e = 0.f;
+      HIP_TRY(hipEventElapsedTime(&e, h->ev_sample[2 * q], h->ev_sample[2 * q + 1]));
+      tot += e;
+      ++cnt;
+    }
+    rep->spmv_ms = cnt ? tot / cnt : 0.0;
+    rep->spmv_samples = cnt;
+    rep->deflated = sf->defl_e > 0.0 ? 1 : 0;
   }
   if (o.true_residual) {
     HIP_TRY(tfg::launch_true_residual(L, a));
@@ -1371,44 +1466,10 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
           // them too and stay in L2: 377 instead of 522 MB fetched per
           // product at 200^3.
           const int R = pair ? 2 * tfg::kRowsPerBlock : tfg::kRowsPerBlock;
-          const long long bpp = ((long long)D + 1 + R - 1) / R;  // +1: a plane may gain a row
-          constexpr int kBand = 8;
-          std::vector<int> ord;
-          if (D >= kBand * R) {
-            const long long planes = ((long long)ni + D - 1) / D;
-            auto start = [&](long long x) { return std::min<long long>(ni, (x * D) & ~1LL); };
-            for (long long j0 = 0; j0 < bpp; j0 += kBand)
-              for (long long x = 0; x < planes; ++x)
-                for (long long b = j0; b < std::min<long long>(bpp, j0 + kBand); ++b) {
-                  const long long r0 = start(x) + b * R;
-                  const long long r1 = std::min<long long>({r0 + R, start(x + 1), (long long)ni});
-                  if (r0 < r1) {
-                    ord.push_back((int)r0);
-                    ord.push_back((int)r1);
-                  }
-                }
-          } else {
-            for (int r0 = 0; r0 < ni; r0 += R) {
-              ord.push_back(r0);
-              ord.push_back(std::min(ni, r0 + R));
-            }
-          }
-          // every row once; pair mode: even starts and lengths
-          {
-            std::vector<char> seen((size_t)ni, 0);
-            bool cover = true;
-            for (size_t t = 0; t + 1 < ord.size() && cover; t += 2) {
-              if (pair && ((ord[t] | (ord[t + 1] - ord[t])) & 1)) cover = false;
-              for (int r = ord[t]; r < ord[t + 1] && cover; ++r) {
-                if (seen[r]) cover = false;
-                seen[r] = 1;
-              }
-            }
-            for (int r = 0; r < ni && cover; ++r) cover = seen[r] != 0;
-            // (not expected) no symmetric engine; the kernels address a
-            // slot with 32-bit byte offsets
-            if (!cover || h->sdg.stride >= (1LL << 29)) h->nsym = 0;
-          }
+          std::vector<int> ord = sym_blocks(ni, D, R, pair ? 2 : 1);
+          // (not expected) no symmetric engine; the kernels address a slot
+          // with 32-bit byte offsets
+          if (ord.empty() || h->sdg.stride >= (1LL << 29)) h->nsym = 0;
           if (h->nsym > 0) {
             e = hipMalloc(&h->sorder, sizeof(int) * ord.size());
             if (e == hipSuccess)
@@ -2016,6 +2077,65 @@ int tfg_prec_apply(tfg_matrix* h, const char* prec, int prec_len, const double* 
   } else {
     HIP_TRY(hipMemcpyAsync(d_z, d_r, sizeof(double) * (size_t)h->ni, hipMemcpyDeviceToDevice, h->stream));
   }
+  return TFG_OK;
+}
+
+int tfg_spmv_fp32(tfg_matrix* h, const double* d_val, double fn, const float* d_x, float* d_y) {
+  if (!h || !d_val || !d_x || !d_y) return fail(TFG_EINVAL, "null argument");
+  if (h->inner) return fail(TFG_ENOTSUP, "FP32 product on an internally ordered matrix");
+  if (h->ni < h->nt) return fail(TFG_ENOTSUP, "FP32 product with buffer cells");
+  HIP_TRY(hipSetDevice(h->device));
+  { const int rt = ensure_tuned(h, d_val); if (rt != TFG_OK) return rt; }
+  for (float** v : {&h->ad32, &h->s32})
+    if (!*v) HIP_TRY(hipMalloc(v, sizeof(float) * std::max<size_t>((size_t)h->nt, 4)));
+  if (!h->s64) HIP_TRY(hipMalloc(&h->s64, sizeof(double) * std::max<size_t>((size_t)h->nt, 4)));
+  bool sym = false;
+  if (h->nsym > 0) {
+    const int rs = sym_prepare(h, d_val, &sym);
+    if (rs != TFG_OK) return rs;
+  }
+  if (sym) {
+    const int r3 = ensure_sym32(h);
+    if (r3 != TFG_OK) return r3;
+    if (h->quad32 && (((uintptr_t)d_x | (uintptr_t)d_y) % 16))
+      return fail(TFG_EINVAL, "FP32 product: x and y must be 16-byte aligned");
+  } else if (!h->val32) {
+    HIP_TRY(hipMalloc(&h->val32, sizeof(float) * (size_t)std::max(h->nnz, 4)));
+  }
+  tfg::Launch L32 = launch_of(h, h->val32, h->kind);
+  if (sym) {
+    L32.kind = tfg::K_SYM_DIA;
+    L32.grid_spmv = h->grid32;
+  } else {
+    if (L32.kind != tfg::K_STAGE2048 && L32.kind != tfg::K_PIPE2048_CODED &&
+        L32.kind != tfg::K_PIPE_PATTERN)
+      L32.kind = tfg::K_PIPE2048;
+    L32.grid_spmv = h->grid[L32.kind];
+    L32.vec_val = h->nnz >= 4;
+  }
+  tfg::Launch L = launch_of(h, d_val, h->kind);
+  tfg::CgArgs a = args_of(h, h->kind);
+  a.val = d_val;
+  a.pt = partials_of(h, L);
+  a.st = h->st;
+  tfg::IrArgs f{};
+  f.n = h->ni;
+  f.val32 = h->val32;
+  f.sd32 = h->sd32;
+  f.stride32 = h->stride32;
+  f.sorder32 = h->sorder32;
+  f.nsorder32 = h->nsorder32;
+  f.quad = h->quad32;
+  f.s64 = h->s64;
+  f.s32 = h->s32;
+  f.rd32 = h->ad32;
+  State* s0 = &h->hst[0];
+  memset(s0, 0, sizeof(State));
+  s0->fn = fn;
+  HIP_TRY(hipMemcpyAsync(h->st, s0, sizeof(State), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(tfg::launch_ir_convert(L, L32, a, f, h->nnz));
+  tfg::SpmvShape m = shape_of(h, L32.kind == tfg::K_SYM_DIA ? h->kind : L32.kind);
+  HIP_TRY(tfg::launch_spmv_lap32(L32, m, f, d_x, d_y));
   return TFG_OK;
 }
 

@@ -58,6 +58,19 @@ struct State {
   double beta;      // multi-rank CG: beta from the global reduction
   int comm;         // multi-rank CG: fn is given (global), not computed
   int kcur;         // graph path: the iteration the next kernels run
+  // FP32 sweeps in FP64 refinement (tfg_ir.hip)
+  double defl_e;    // w'Aw of the deflation vector w = 1 (0: no deflation)
+  double rref;      // ||r32||^2 at the last restart
+  double xmin, xmax;  // x range at the last refinement (NRMS estimate)
+  double delta;     // refinement when ||r32|| fell by this since the restart
+  int refine;       // 1: the refinement kernels of this chunk end run
+  int kref;         // iteration of the last restart (0: the start)
+  int miter;        // iteration cap
+  int force;        // > 0: a refinement at least every `force` iterations
+  int defl_mode;    // 0 auto, 1 off, 2 on (TFG_DEFLATE_*)
+  int sym_vals;     // the values are symmetric (deflation allowed); cleared by
+                    // any thread of the check that finds an asymmetric pair
+  int nrefine;      // refinements (true residuals) so far
 };
 
 // Partial sums written by one kernel and swept by the next ones.
@@ -68,6 +81,7 @@ struct Partials {
   double* upd;      // [5][pstride]  NRMS sum, x min, x max, (z,r), (r,r)
   double* aux;      // [2][pstride]  true-residual sums
   double* ext;      // [6][pstride]  BiCGSTAB: sums of s, then (t,s), (t,t)
+  double* irc;      // [4][pstride]  FP32 refinement form: conversion sums
   int pstride;
   int g_vec;        // grid of the element-wise kernels (producers of diag)
   int g_spmv;       // grid of the SpMV kernels (producers of pro/spmv/aux)
@@ -156,17 +170,25 @@ struct NsArgs {
 enum SpmvKind { K_STAGE2048 = 0, K_STAGE4096 = 1, K_WAVE4 = 2, K_WAVE8 = 3, K_WAVE16 = 4,
                 K_PIPE2048 = 5, K_PIPE2048_CODED = 6, K_PIPE_PATTERN = 7, K_SYM_DIA = 8,
                 kKinds = 9 };
-// FP32 sweeps of iterative refinement (FP32 vectors, FP64 scalars and
-// reductions).  Reuses the FP64 solve's partial buffers (spmv, upd).
+// FP32 sweeps of iterative refinement (tfg_ir.hip): FP32 vectors and
+// matrix in the row-excess form, FP64 scalars, reductions, solution and
+// true residual.  Partial sums go to the FP64 solve's buffers.
 struct IrArgs {
   int n;
-  const float* val32;   // float(a_ij * fn)
-  const float* dinv32;  // float(1 / (a_ii * fn)) (Jacobi) or 1
+  float* val32;         // CSR engines: fl32(a_ij fn), the diagonal slot s_i
+  float* sd32;          // symmetric engine: slot 0 s_i, slot k+1 fl32(a(i, i+d_k) fn)
+  long long stride32;   // slot stride of sd32 (floats)
+  const int* sorder32;  // symmetric engine: (start, end) blocks of 4-row groups
+  int nsorder32;
+  int quad;             // symmetric engine: four rows per thread
+  double* s64;          // s_i = sum_j a_ij fn (compensated, CSR order)
+  float* s32;           // fl32(s_i)
+  float* rd32;          // fl32(1 / (a_ii fn))
   float* r32;
-  float* e32;
   float* p32;
   float* q32;
-  State* st_in;         // inner scalars
+  float* e32;           // FP32 correction since the last refinement
+  int prec;             // PREC_JACOBI or PREC_NONE
 };
 
 // Level-scheduled DIC (tfg_icc.hip).  Device arrays except the two level
@@ -326,21 +348,28 @@ hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* 
                              const double* p, double* q);
 hipError_t launch_copy16(hipStream_t stream, int grid, const void* src, void* dst, long long n16);
 
-// Iterative refinement (tfg_ir.hip).
-hipError_t launch_ir_convert(const Launch& L, const CgArgs& a, const IrArgs& f, int prec,
-                             int nnz);
-hipError_t launch_ir_check(const Launch& L, const CgArgs& a, int outer);
-hipError_t launch_ir_replace(const Launch& L, const CgArgs& a, const IrArgs& f, int prec);
-hipError_t launch_ir_zero(const Launch& L, const IrArgs& f);
-hipError_t launch_ir_pupdate(const Launch& L, const CgArgs& a, const IrArgs& f, int prec, int k,
-                             int first);
-hipError_t launch_ir_spmv_pq(const Launch& L, const CgArgs& a, const IrArgs& f, int k);
-hipError_t launch_ir_update(const Launch& L, const CgArgs& a, const IrArgs& f, int prec, int k);
-hipError_t launch_ir_finish(const Launch& L, const CgArgs& a, const IrArgs& f, int k,
-                            double delta);
-hipError_t launch_ir_correct(const Launch& L, const CgArgs& a, const IrArgs& f);
-hipError_t launch_spmv_plain32(const Launch& L, const SpmvShape& m, const float* val,
-                               const float* p, float* q);
+// FP32 sweeps in FP64 refinement (tfg_ir.hip).  L: the FP64 engine (the
+// true residual); L32: the FP32 product's engine (kind 8 = the symmetric
+// FP32 engine on f.sd32, else a CSR engine on f.val32).
+hipError_t launch_ir_convert(const Launch& L, const Launch& L32, const CgArgs& a,
+                             const IrArgs& f, int nnz);
+// State::sym_vals = 0 when some val[j] differs from val[tpos[j]] bitwise.
+hipError_t launch_ir_symcheck(hipStream_t s, int grid, const double* val, const int* tpos,
+                              long long nnz, State* st);
+hipError_t launch_ir_pupdate(const Launch& L, const CgArgs& a, const IrArgs& f);
+hipError_t launch_ir_spmv(const Launch& L32, const CgArgs& a, const IrArgs& f);
+hipError_t launch_ir_update(const Launch& L, const CgArgs& a, const IrArgs& f, bool defl,
+                            bool nrms);
+// Chunk end: fold (when a refinement is due), FP64 true residual, stopping
+// test and restart; launch_ir_restart is the last two alone (the start).
+hipError_t launch_ir_refine(const Launch& L, const CgArgs& a, const IrArgs& f);
+hipError_t launch_ir_restart(const Launch& L, const CgArgs& a, const IrArgs& f);
+// x += e after a breakdown.
+hipError_t launch_ir_fold(const Launch& L, const CgArgs& a, const IrArgs& f);
+// The FP32 row-excess product y = s_i x_i + sum_{j != i} a_ij (x_j - x_i)
+// of an FP32 matrix already converted (building block, tfg_spmv_fp32).
+hipError_t launch_spmv_lap32(const Launch& L32, const SpmvShape& m, const IrArgs& f,
+                             const float* x, float* y);
 hipError_t launch_residual_plain(const Launch& L, const SpmvShape& m, const double* val,
                                  const double* b, const double* x, double* r);
 

@@ -113,13 +113,14 @@ struct ChunkStream {
 
 // Sum of a row longer than the stage: strided partial sums combined in a
 // fixed order (rows this long do not occur in T-Flows grids).
-template <bool SUB, typename T, typename Init, typename Epi>
+template <bool SUB, bool LAP = false, typename T, typename Init, typename Epi>
 __device__ __forceinline__ void long_row(const SpmvShape& m, const T* __restrict__ val, T fn,
                                          const T* __restrict__ x, int r0, int s, int e,
                                          double* s_tmp, double* s_out, Init& init, Epi& epi) {
   double part[1] = {0.0};
   for (int j = s + (int)threadIdx.x; j < e; j += kBlock)
-    part[0] = part[0] + (double)((val[j] * fn) * x[m.col[j]]);
+    part[0] = part[0] + (double)((val[j] * fn) * (LAP && j != s ? x[m.col[j]] - x[r0]
+                                                                 : x[m.col[j]]));
   const int ops[1] = {OP_SUM};
   block_reduce<1>(part, ops, s_tmp, s_out);
   if (threadIdx.x == 0) {
@@ -128,6 +129,15 @@ __device__ __forceinline__ void long_row(const SpmvShape& m, const T* __restrict
     epi(r0, SUB ? a0 - sum : a0 + sum, x[r0], val[s] * fn);
   }
   __syncthreads();
+}
+
+// Term of a row sum: a x_j, or in the row-excess form (LAP, the FP32 sweeps
+// of iterative refinement) a (x_j - x_i) for every entry after the first,
+// the diagonal, whose value is then the row's excess s_i = sum_j a_ij and
+// takes x_i: s_i x_i + sum_{j != i} a_ij (x_j - x_i) = sum_j a_ij x_j.
+template <bool LAP, typename T>
+__device__ __forceinline__ T row_term(T a, T xj, T xi, bool first) {
+  return (LAP && !first) ? a * (xj - xi) : a * xj;
 }
 
 // ---------------------------------------------------------------------------
@@ -185,7 +195,7 @@ struct StreamEngine {
       return sm.col[j - oc];
   }
 
-  template <bool SUB, typename Init, typename Epi>
+  template <bool SUB, bool LAP = false, typename Init, typename Epi>
   __device__ __forceinline__ static void run(Smem& sm, const SpmvShape& m,
                                              const T* __restrict__ val, T fn, const T* x,
                                              double* s_tmp, double* s_out, Init init, Epi epi) {
@@ -210,7 +220,7 @@ struct StreamEngine {
       if (!PIPE) fetch(m, val, nnz, lb, g);
       const int r0 = g.r0, nr = g.nr, s = g.s, e = g.e, rs = g.rs, re = g.re;
       if (e - s > CAP) {
-        long_row<SUB>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
+        long_row<SUB, LAP>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
         if (PIPE) fetch(m, val, nnz, min(nxt, w.end - 1), g);
         lb = nxt;
         continue;
@@ -244,12 +254,15 @@ struct StreamEngine {
       T acc = init(i);
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (rs + u < re) acc = SUB ? acc - av[u] * xv[u] : acc + av[u] * xv[u];
+        if (rs + u < re) {
+          const T t = row_term<LAP>(av[u], xv[u], xv[0], u == 0);
+          acc = SUB ? acc - t : acc + t;
+        }
       if (tid < nr) {
         for (int j = rs + 8; j < re; ++j) {
           const T a = sm.val[j - ov] * fn;
-          const T xj = x[column(sm, i, j, oc)];
-          acc = SUB ? acc - a * xj : acc + a * xj;
+          const T t = row_term<LAP>(a, x[column(sm, i, j, oc)], xv[0], false);
+          acc = SUB ? acc - t : acc + t;
         }
         epi(i, acc, xv[0], av[0]);
       }
@@ -312,7 +325,7 @@ struct PatternEngine {
     g.v.fetch(val, nnz, s, e);
   }
 
-  template <bool SUB, typename Init, typename Epi>
+  template <bool SUB, bool LAP = false, typename Init, typename Epi>
   __device__ __forceinline__ static void run(Smem& sm, const SpmvShape& m,
                                              const T* __restrict__ val, T fn, const T* x,
                                              double* s_tmp, double* s_out, Init init, Epi epi) {
@@ -358,12 +371,15 @@ struct PatternEngine {
       T acc = init(i);
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (u < L) acc = SUB ? acc - av[u] * xv[u] : acc + av[u] * xv[u];
+        if (u < L) {
+          const T t = row_term<LAP>(av[u], xv[u], xv[0], u == 0);
+          acc = SUB ? acc - t : acc + t;
+        }
       if (tid < nr) {
         for (int u = 8; u < L; ++u) {
           const T a = sm.val[rs + u - ov] * fn;
-          const T xj = x[i + po[u]];
-          acc = SUB ? acc - a * xj : acc + a * xj;
+          const T t = row_term<LAP>(a, x[i + po[u]], xv[0], false);
+          acc = SUB ? acc - t : acc + t;
         }
         epi(i, acc, xv[0], av[0]);
       }
@@ -532,6 +548,137 @@ struct SymEngine {
   }
 };
 
+// FP32 row-excess product on the symmetric storage of the FP32 sweeps of
+// iterative refinement: slot 0 of sd32 holds s_i = sum_j a_ij, slot k + 1
+// the upper diagonal a(i, i + d_k), zero where row i has no such entry; the
+// slot stride S leaves at least the largest offset + 4 zeros after every
+// slot (lower entries of the first rows read them).  Row i sums
+//   s_i x_i,
+//   a(i - d_k, i) (x_{i - d_k} - x_i) for k = K-1 .. 0 (the mirror),
+//   a(i, i + d_k) (x_{i + d_k} - x_i) for k = 0 .. K-1,
+// the CSR order of the row, so the products equal the CSR engines' in the
+// row-excess form bit for bit (a missing entry adds a zero term; its x index
+// is clamped into the vector).  Modes: 0 one row per thread, blocks of at
+// most 256 rows; 1 four rows per thread with 16-byte loads (d_0 = 1, the
+// other offsets multiples of 4, n a multiple of 4; blocks of at most 1024
+// rows at multiples of 4): rows i .. i + 3 read every term as one aligned
+// quadruple, the +-1 neighbours from the quadruples at i - 4 and i + 4.
+// epi(i, y_i, x_i, own), epi4(i, y[4], x[4], own).
+struct SymEngine32 {
+  typedef float flt4 __attribute__((ext_vector_type(4)));
+  typedef __attribute__((address_space(1))) const char gB;
+  template <typename V>
+  __device__ __forceinline__ static V at(gB* base, unsigned byte) {
+    return *(__attribute__((address_space(1))) const V*)(base + byte);
+  }
+  template <typename V>  // non-temporal: no other row reads it
+  __device__ __forceinline__ static V at_nt(gB* base, unsigned byte) {
+    return __builtin_nontemporal_load((__attribute__((address_space(1))) const V*)(base + byte));
+  }
+  __device__ __forceinline__ static float term(float acc, float a, float xj, float xi) {
+    return acc + a * (xj - xi);
+  }
+
+  template <int K, typename Epi>
+  __device__ __forceinline__ static void rows(const SymDiag& g, const float* sd, long long S, int n,
+                                              const float* x, int r0, int r1, Epi& epi) {
+    gB* xb = (gB*)x;
+    gB* sb = (gB*)sd;
+    const long long S4 = 4 * S;
+    const unsigned lastb = 4u * (unsigned)(n - 1);
+    const int t = r0 + (int)threadIdx.x;
+    const int i = min(t, r1 - 1);
+    const unsigned ib = 4u * (unsigned)i;
+    float lo[K], up[K], xl[K], xu[K];
+    const float s = at_nt<float>(sb, ib);
+    const float x0 = at<float>(xb, ib);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const unsigned ob = 4u * (unsigned)g.off[k];
+      gB* slot = sb + (k + 1) * S4;
+      up[k] = at<float>(slot, ib);
+      lo[k] = at<float>(slot - ob, ib);  // in the zeros after slot k for i < d_k
+      xu[k] = at<float>(xb, min(ib + ob, lastb));
+      xl[k] = at<float>(xb, max(ib, ob) - ob);
+    }
+    float acc = s * x0;
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k) acc = term(acc, lo[k], xl[k], x0);
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc = term(acc, up[k], xu[k], x0);
+    epi(i, acc, x0, t < r1);
+  }
+
+  template <int K, typename Epi4>
+  __device__ __forceinline__ static void quads(const SymDiag& g, const float* sd, long long S,
+                                               int n, const float* x, int r0, int r1, Epi4& epi4) {
+    gB* xb = (gB*)x;
+    gB* sb = (gB*)sd;
+    const long long S4 = 4 * S;
+    const int t = r0 + 4 * (int)threadIdx.x;
+    const int i = min(t, r1 - 4);
+    const unsigned ib = 4u * (unsigned)i;           // 16-byte aligned
+    const unsigned lastb = 4u * (unsigned)(n - 4);
+    // every offset first, then all loads (as in SymEngine::pairs)
+    unsigned ou[K], ol[K];
+    ou[0] = min(ib + 16, lastb);
+    ol[0] = max(ib, 16u) - 16;
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+      const unsigned ob = 4u * (unsigned)g.off[k];
+      ou[k] = min(ib + ob, lastb);
+      ol[k] = max(ib, ob) - ob;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    flt4 lo[K], up[K], xl[K], xu[K];
+    const flt4 s = at_nt<flt4>(sb, ib);
+    const flt4 x0 = at<flt4>(xb, ib);
+    up[0] = at<flt4>(sb + S4, ib);        // a(i, i+1) .. a(i+3, i+4)
+    lo[0] = at<flt4>(sb + S4 - 16, ib);   // .w = a(i-1, i)
+    xu[0] = at<flt4>(xb, ou[0]);          // .x = x_{i+4}
+    xl[0] = at<flt4>(xb, ol[0]);          // .w = x_{i-1}
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+      const unsigned ob = 4u * (unsigned)g.off[k];
+      gB* slot = sb + (k + 1) * S4;
+      up[k] = at<flt4>(slot, ib);
+      lo[k] = at<flt4>(slot - ob, ib);
+      xu[k] = at<flt4>(xb, ou[k]);
+      xl[k] = at<flt4>(xb, ol[k]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    float y[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float acc = s[c] * x0[c];
+#pragma unroll
+      for (int k = K - 1; k >= 1; --k) acc = term(acc, lo[k][c], xl[k][c], x0[c]);
+      // offset 1: the row's lower neighbour, then its upper one
+      if (c == 0) acc = term(acc, lo[0][3], xl[0][3], x0[0]);
+      else acc = term(acc, up[0][c - 1], x0[c - 1], x0[c]);
+      if (c == 3) acc = term(acc, up[0][3], xu[0][0], x0[3]);
+      else acc = term(acc, up[0][c], x0[c + 1], x0[c]);
+#pragma unroll
+      for (int k = 1; k < K; ++k) acc = term(acc, up[k][c], xu[k][c], x0[c]);
+      y[c] = acc;
+    }
+    epi4(i, y, x0, t < r1);
+  }
+
+  template <int K, int MODE, typename Epi, typename Epi4>
+  __device__ __forceinline__ static void run(const SymDiag& g, const float* sd, long long S, int n,
+                                             const int* sorder, int nsorder, const float* x,
+                                             Epi epi, Epi4 epi4) {
+    typedef __attribute__((address_space(4))) const int cint;
+    const XcdWalk w(nsorder);
+    for (int lb = w.first; lb < w.end; lb += w.step) {
+      const int r0 = ((cint*)sorder)[2 * lb], r1 = ((cint*)sorder)[2 * lb + 1];
+      if constexpr (MODE == 1) quads<K>(g, sd, S, n, x, r0, r1, epi4);
+      else rows<K>(g, sd, S, n, x, r0, r1, epi);
+    }
+  }
+};
+
 // One given row block lb per call (the batched solver's one-block-per-
 // workgroup launches): StreamEngine staging without the pipelining.
 template <int CAP, typename T>
@@ -579,7 +726,7 @@ struct ScalarStage {
     int col[CAP];
   };
 
-  template <bool SUB, typename Init, typename Epi>
+  template <bool SUB, bool LAP = false, typename Init, typename Epi>
   __device__ __forceinline__ static void run(Smem& sm, const SpmvShape& m,
                                              const T* __restrict__ val, T fn,
                                              const T* __restrict__ x, double* s_tmp,
@@ -590,7 +737,7 @@ struct ScalarStage {
       const int r0 = m.blk[lb], nr = m.blk[lb + 1] - r0;
       const int s = m.row[r0], e = m.row[r0 + nr];
       if (e - s > CAP) {
-        long_row<SUB>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
+        long_row<SUB, LAP>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
         continue;
       }
       for (int t = tid; t < e - s; t += kBlock) {
@@ -601,12 +748,13 @@ struct ScalarStage {
       if (tid < nr) {
         const int i = r0 + tid;
         const int rs = m.row[i], re = m.row[i + 1];
+        const T xi = x[sm.col[rs - s]];
         T acc = init(i);
         for (int j = rs; j < re; ++j) {
-          const T a = sm.val[j - s] * fn;
-          acc = SUB ? acc - a * x[sm.col[j - s]] : acc + a * x[sm.col[j - s]];
+          const T t = row_term<LAP>(sm.val[j - s] * fn, x[sm.col[j - s]], xi, j == rs);
+          acc = SUB ? acc - t : acc + t;
         }
-        epi(i, acc, x[sm.col[rs - s]], sm.val[rs - s] * fn);
+        epi(i, acc, xi, sm.val[rs - s] * fn);
       }
       __syncthreads();
     }

@@ -23,6 +23,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "tflows_gpu.h")
 
 TFG_OK, TFG_EINVAL, TFG_EHIP, TFG_ENOMEM, TFG_EBREAKDOWN, TFG_ENOTSUP = 0, 1, 2, 3, 4, 5
 FP64, FP32_IR = 0, 1
+DEFLATE_AUTO, DEFLATE_OFF, DEFLATE_ON = 0, 1, 2
 CRIT_NRMS, CRIT_RELRES = 0, 1
 PATH_AUTO, PATH_KERNELS, PATH_GRAPH = 0, 1, 2
 ORDER_NATURAL, ORDER_RCM = 0, 1
@@ -38,14 +39,15 @@ class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("criterion", ctypes.c_int),
                 ("path", ctypes.c_int), ("inner_max", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("timing", ctypes.c_int),
-                ("true_residual", ctypes.c_int), ("textbook_bicg", ctypes.c_int)]
+                ("true_residual", ctypes.c_int), ("textbook_bicg", ctypes.c_int),
+                ("deflation", ctypes.c_int)]
 
 
 class Report(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int),
                 ("refinements", ctypes.c_int), ("true_relres", ctypes.c_double),
                 ("solve_ms", ctypes.c_double), ("spmv_ms", ctypes.c_double),
-                ("spmv_samples", ctypes.c_int)]
+                ("spmv_samples", ctypes.c_int), ("deflated", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -78,6 +80,7 @@ def lib():
         L.tfg_solve_native_ex.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P, P,
                                           I, P, D, P, P, I, P, P]
         L.tfg_spmv.argtypes = [P, P, P, P]
+        L.tfg_spmv_fp32.argtypes = [P, P, D, P, P]
         L.tfg_prec_apply.argtypes = [P, ctypes.c_char_p, I, P, P, P]
         L.tfg_residual.argtypes = [P, P, P, P, P]
         L.tfg_spmv_bench.argtypes = [P, P, P, P, I, ctypes.POINTER(D)]
@@ -259,6 +262,11 @@ class Native:
     # -- building blocks ----------------------------------------------------
     def spmv(self, dval, dp, dq):
         _check(lib().tfg_spmv(self.h, dval.ptr, dp.ptr, dq.ptr))
+        _check(lib().tfg_synchronize(self.h))
+
+    def spmv_fp32(self, dval, fn, dx32, dy32):
+        """FP32 row-excess product of the refinement form on val * fn."""
+        _check(lib().tfg_spmv_fp32(self.h, dval.ptr, float(fn), dx32.ptr, dy32.ptr))
         _check(lib().tfg_synchronize(self.h))
 
     def set_comm(self, allreduce, exchange):
