@@ -36,17 +36,69 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def _oracle_full_solve(s, omp, progress):
-    """The oracle's Jacobi-CG on system s from x0 = 0 to relres 1e-8; returns
+def _oracle_full_solve(s, omp, progress, tol=1e-8):
+    """The oracle's Jacobi-CG on system s from x0 = 0 to relres tol; returns
     (seconds, niter, x)."""
     from oracle import oracle
 
     A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
     x = np.zeros(s.n)
     t0 = time.perf_counter()
-    it, _ = oracle.solve("cg", "jacobi", A, x, s.b.copy(), 50000, 1e-8, criterion="relres",
+    it, _ = oracle.solve("cg", "jacobi", A, x, s.b.copy(), 50000, tol, criterion="relres",
                          omp=omp, progress=progress)
     return time.perf_counter() - t0, it, x
+
+
+FP32_TOLS = (1e-8, 5e-9, 2e-9, 1e-9)
+
+
+def fp32_leg(G, nat, s, dv, db, dx, tol, miter):
+    """The FP32 refinement form (FP32 sweeps, FP64 true residual and
+    solution) on the headline system at the headline tolerance and tighter
+    ones; each solve timed twice, the second kept.  Returns
+    {tol: (seconds, iterations, refinements, true relres, x)}."""
+    runs = {}
+    opt = G.Options()
+    opt.precision = G.FP32_IR
+    opt.criterion = G.CRIT_RELRES
+    opt.true_residual = 1
+    for t in FP32_TOLS:
+        for rep_ in range(2):
+            dx.upload(np.zeros(s.n))
+            G.lib().tfg_synchronize(nat.h)
+            t0 = time.perf_counter()
+            it, res, rep = nat.solve_native_device("cg", "jacobi", dv, dx, db, miter, t,
+                                                   options=opt, report=True)
+            dt = time.perf_counter() - t0
+        runs[t] = (dt, it, rep["refinements"], rep["true_relres"], dx.download(), rep["deflated"])
+        log(f"bench: FP32 form tol {t:.0e}: {dt:.3f} s, {it} FP32 iterations, "
+            f"{rep['refinements']} refinements, true relres {rep['true_relres']:.2e}, "
+            f"deflated {rep['deflated']}")
+    return runs
+
+
+def fp32_summary(runs, x64, x_ref, t64, tol):
+    """Errors of the FP64 CG and of the FP32 form against the reference
+    solution, and the FP32 form's speed-up at the loosest tolerance whose
+    error is no worse than FP64 CG's at the headline one."""
+    def rel(a):
+        return float(np.linalg.norm(a - x_ref) / np.linalg.norm(x_ref))
+
+    err64 = rel(x64)
+    table = {f"{t:.0e}": {"seconds": v[0], "iterations": v[1], "refinements": v[2],
+                          "true_relres": v[3], "x_rel_err_vs_reference": rel(v[4]),
+                          "deflated": bool(v[5])} for t, v in runs.items()}
+    ok = [t for t in FP32_TOLS if rel(runs[t][4]) <= err64]
+    pick = ok[0] if ok else None
+    out = {"fp64_cg_x_rel_err_vs_reference": err64, "fp64_cg_seconds": t64, "runs": table,
+           "tolerance_at_fp64_error": pick,
+           "speedup_at_same_error": (t64 / runs[pick][0]) if pick is not None else None,
+           "speedup_at_same_tolerance": t64 / runs[tol][0]}
+    log(f"bench: FP32 form: FP64 CG error {err64:.2e}; "
+        + "; ".join(f"tol {k}: {v['seconds']:.3f} s err {v['x_rel_err_vs_reference']:.2e}"
+                    for k, v in table.items())
+        + (f"; same error at tol {pick:.0e}: {t64 / runs[pick][0]:.2f}x" if pick else ""))
+    return out
 
 
 def cpu_baseline(s, x_gpu, gpu_iters):
@@ -62,6 +114,9 @@ def cpu_baseline(s, x_gpu, gpu_iters):
     log(f"bench: CPU baseline: oracle Jacobi-CG to relres 1e-8 on {cores} threads (OpenMP build)")
     t_omp, it_omp, x_omp = _oracle_full_solve(s, True, 500)
     log(f"bench: CPU baseline: OpenMP oracle {t_omp:.1f} s, {it_omp} iterations")
+    log("bench: reference solution: OpenMP oracle Jacobi-CG to relres 1e-12")
+    t_ref, it_ref, x_ref = _oracle_full_solve(s, True, 500, tol=1e-12)
+    log(f"bench: reference solution: {t_ref:.1f} s, {it_ref} iterations")
     log("bench: CPU baseline: oracle Jacobi-CG to relres 1e-8, 1 thread")
     t1, it1, x1 = _oracle_full_solve(s, False, 250)
     log(f"bench: CPU baseline: 1-thread oracle {t1:.1f} s, {it1} iterations")
@@ -88,10 +143,14 @@ def cpu_baseline(s, x_gpu, gpu_iters):
         "x_rel_envelope_openmp_vs_serial_oracle": env,
         "within_bar": bool(abs(gpu_iters - it1) <= max(2, abs(it_omp - it1))
                            and err <= max(1e-10, env)),
+        "reference_solution": {"solver": "OpenMP oracle Jacobi-CG", "relres": 1e-12,
+                               "iterations": it_ref},
+        "x_rel_err_gpu_vs_reference": rel(x_gpu, x_ref),
+        "x_rel_err_oracle_vs_reference": rel(x1, x_ref),
     }
     log(f"bench: parity: gpu {gpu_iters} vs oracle {it1} iterations (OpenMP oracle {it_omp}); "
         f"x rel err {err:.2e} (OpenMP-vs-serial envelope {env:.2e})")
-    return cpu, parity
+    return cpu, parity, x_ref
 
 
 TRAFFIC = os.path.join(ROOT, "profiles", "spmv_traffic_twophase200.json")
@@ -763,6 +822,8 @@ def main():
             f"({rep['spmv_samples']} samples)")
     its = [r[0] for r in reps]
     x_gpu = dx.download()
+    # the FP32 refinement form on the same system, outside the timed steps
+    fp32_runs = fp32_leg(G, nat, s, dv, db, dx, tol, miter)
     # the drop-in call with host arrays (val, b and x0 uploaded, x
     # downloaded: PCIe inside the call), once, outside the timed steps
     opt_h = G.Options()
@@ -782,9 +843,10 @@ def main():
 
     spmv_kernel = "k_spmv_pq_sym" if sym_ran else "k_spmv_pq"
     traffic, traffic_src = traffic_per_launch(spmv_kernel, None if sym_ran else kind)
-    cpu = parity = None
+    cpu = parity = fp32 = None
     if not args.no_cpu:
-        cpu, parity = cpu_baseline(s, x_gpu, its[-1])
+        cpu, parity, x_ref = cpu_baseline(s, x_gpu, its[-1])
+        fp32 = fp32_summary(fp32_runs, x_gpu, x_ref, total / args.steps, tol)
 
     out = {
         "metric": "time to 1e-8 rel. residual, 8M-cell pressure solve; SpMV HBM GB/s vs 8 TB/s",
@@ -818,6 +880,7 @@ def main():
             launch_timing="mean of sampled in-solve launches, hipEvents on the library's stream"),
         "cpu_baseline": cpu,
         "parity": parity,
+        "fp32_refinement": fp32,
     }
     print(json.dumps(out), flush=True)
 

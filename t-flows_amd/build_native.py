@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["csrc/tfg_api.hip", "csrc/tfg_kernels.hip", "csrc/tfg_ir.hip",
+HIP_SOURCES = ["csrc/tfg_api.hip", "csrc/tfg_kernels.hip", "csrc/tfg_ir.hip", "csrc/tfg_ir_ns.hip",
                "csrc/tfg_nonsym.hip", "csrc/tfg_batch.hip", "csrc/tfg_icc.hip",
                "csrc/tfg_comm.hip", "csrc/tfg_assembly.hip"]
 HIP_FLAGS = [

@@ -128,6 +128,8 @@ struct tfg_matrix {
   float* p32 = nullptr;
   float* q32 = nullptr;
   float* ad32 = nullptr;
+  float* ns32[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // textbook methods
+  float* valt32 = nullptr;                                         // BiCG: A^T
   // textbook BiCG / CGS / BiCGSTAB work (allocated on first use)
   double* rt = nullptr;
   double* nv[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -342,7 +344,8 @@ void free_all(tfg_matrix* h) {
   h->inner = nullptr;
   void* dev[] = {h->row, h->col, h->blk[0], h->blk[1], h->val, h->b, h->x, h->r, h->p, h->q, h->ad,
                  h->part, h->st, h->st_in, h->val32, h->sd32, h->sorder32, h->s64, h->s32, h->e32,
-                 h->r32, h->p32, h->q32, h->ad32, h->rt, h->nv[0], h->nv[1], h->nv[2],
+                 h->r32, h->p32, h->q32, h->ad32, h->ns32[0], h->ns32[1], h->ns32[2],
+                 h->ns32[3], h->ns32[4], h->valt32, h->rt, h->nv[0], h->nv[1], h->nv[2],
                  h->nv[3], h->valt, h->tpos, h->code, h->tab, h->pat, h->poff, h->plen,
                  h->pslot, h->sym, h->sym_flag, h->sorder,
                  h->loc, h->icc_rows[0],
@@ -885,6 +888,166 @@ int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int mi
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (rep) rep->true_relres = sf->aux0;
   }
+  return status;
+}
+
+// FP32 sweeps of the textbook methods inside FP64 refinement (kernels in
+// tfg_ir_ns.hip): each refinement restarts the method in FP32 on the FP64
+// true residual and runs it until its residual has fallen by inner_tol
+// (default 1e-4; every inner_max iterations at the latest when > 0), then
+// x += e in FP64, the true residual and the stopping test.  The host waits
+// at every chunk of 8 inner iterations (the inner solve's length is decided
+// there).  niter = FP32 iterations; fin_res = the criterion on the last
+// FP64 true residual.
+int solve_ir_ns(tfg_matrix* h, int method, int prec, const double* val, const double* b,
+                int miter, int* niter, double tol, double* fin_res, const double* norm,
+                const tfg_options& o, tfg_report* rep) {
+  const size_t n = (size_t)h->nt;
+  { const int rt = ensure_tuned(h, val); if (rt != TFG_OK) return rt; }
+  for (float** v : {&h->ad32, &h->s32, &h->r32, &h->e32, &h->p32, &h->q32, &h->ns32[0],
+                    &h->ns32[1], &h->ns32[2], &h->ns32[3], &h->ns32[4]})
+    if (!*v) HIP_TRY(hipMalloc(v, sizeof(float) * std::max<size_t>(n, 4)));
+  if (!h->s64) HIP_TRY(hipMalloc(&h->s64, sizeof(double) * std::max<size_t>(n, 4)));
+  if (!h->val32) HIP_TRY(hipMalloc(&h->val32, sizeof(float) * (size_t)std::max(h->nnz, 4)));
+  if (method == tfg::NS_BICG) {
+    { const int rt = ensure_tpos(h); if (rt != TFG_OK) return rt; }
+    if (h->tpos_state < 0)
+      return fail(TFG_ENOTSUP, "textbook BiCG needs a structurally symmetric matrix");
+    if (!h->valt32) HIP_TRY(hipMalloc(&h->valt32, sizeof(float) * (size_t)std::max(h->nnz, 4)));
+  }
+  tfg::Launch L32 = launch_of(h, h->val32, h->kind);
+  if (L32.kind != tfg::K_STAGE2048 && L32.kind != tfg::K_PIPE2048_CODED &&
+      L32.kind != tfg::K_PIPE_PATTERN)
+    L32.kind = tfg::K_PIPE2048;
+  L32.grid_spmv = h->grid[L32.kind];
+  L32.vec_val = h->nnz >= 4;
+  tfg::Launch L = launch_of(h, val, h->kind);
+  L.grid_spmv = L32.grid_spmv;
+  tfg::CgArgs a = args_of(h, h->kind);
+  a.val = val;
+  a.b = b;
+  a.x = h->x;
+  a.r = h->r;
+  a.p = h->p;
+  a.q = h->q;
+  a.ad = h->ad;
+  a.pt = partials_of(h, L);
+  a.st = h->st;
+  tfg::IrArgs f{};
+  f.n = h->ni;
+  f.val32 = h->val32;
+  f.s64 = h->s64;
+  f.s32 = h->s32;
+  f.rd32 = h->ad32;
+  f.r32 = h->r32;
+  f.p32 = h->p32;
+  f.q32 = h->q32;
+  f.e32 = h->e32;
+  f.prec = prec == tfg::PREC_JACOBI ? tfg::PREC_JACOBI : tfg::PREC_NONE;
+  tfg::NsArgs32 s{};
+  s.valt = h->valt32;
+  s.rt = h->ns32[0];
+  s.v1 = h->ns32[1];
+  s.v2 = h->ns32[2];
+  s.v3 = h->ns32[3];
+  s.v4 = h->ns32[4];
+  tfg::CgArgs ai = a;  // the inner solve's partial grids
+  ai.pt.g_upd = method == tfg::NS_CGS ? L32.grid_spmv : L.grid_vec;
+  const double tol_in = o.inner_tol > 0.0 ? o.inner_tol : 1e-4;
+
+  State* s0 = &h->hst[0];
+  memset(s0, 0, sizeof(State));
+  s0->tol = tol;
+  s0->criterion = o.criterion;
+  s0->has_norm = norm ? 1 : 0;
+  s0->norm = norm ? *norm : 0.0;
+  s0->refine = 1;   // the FP64 residual kernel runs at every refinement
+  HIP_TRY(hipMemcpyAsync(h->st, s0, sizeof(State), hipMemcpyHostToDevice, h->stream));
+  State* sf = &h->hst[3];
+  State* si = &h->hst[4];
+  const bool timing = o.timing != 0;
+  const int max_samples = 256;
+  if (timing && h->ev_sample.empty()) {
+    h->ev_sample.resize(2 * max_samples, nullptr);
+    for (auto& e : h->ev_sample) HIP_TRY(hipEventCreate(&e));
+  }
+  std::vector<int> sample_k;
+  auto read = [&](State* dst, const State* src) -> int {
+    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(State), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return TFG_OK;
+  };
+#define TFG_NS(expr) do { const int rc_ = (expr); if (rc_ != TFG_OK) return rc_; } while (0)
+
+  HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
+  HIP_TRY(tfg::launch_diag_sum(L, a));
+  HIP_TRY(tfg::launch_prologue(L, a, f.prec));
+  HIP_TRY(tfg::launch_start(L, a));
+  HIP_TRY(tfg::launch_ir_convert(L, L32, a, f, h->nnz));
+  if (method == tfg::NS_BICG) HIP_TRY(tfg::launch_ns32_valt(L, a, h->tpos, h->valt32));
+  TFG_NS(read(sf, h->st));
+  int total = 0, status = TFG_OK;
+  const int inner_cap = o.inner_max > 0 ? o.inner_max : INT32_MAX;
+  while (!sf->done) {
+    HIP_TRY(tfg::launch_ns32_restart(L, a, f, s, h->st_in, tol_in, method));
+    int k = 1;
+    const int kmax = std::min(miter - total, inner_cap);
+    bool inner_done = false;
+    while (k <= kmax && !inner_done) {
+      const int kend = std::min(kmax, k + 7);
+      for (; k <= kend; ++k) {
+        const bool take = timing && (total + k) % 4 == 1 && (int)sample_k.size() < max_samples;
+        hipEvent_t e0 = take ? h->ev_sample[2 * sample_k.size()] : nullptr;
+        hipEvent_t e1 = take ? h->ev_sample[2 * sample_k.size() + 1] : nullptr;
+        HIP_TRY(tfg::launch_ns32_iteration(L, L32, ai, f, s, h->st_in, method, k, e0, e1));
+        if (take) sample_k.push_back(total + k);
+      }
+      TFG_NS(read(si, h->st_in));
+      inner_done = si->done != 0;
+    }
+    const int steps = si->done ? si->niter : (k - 1);
+    total += steps;
+    HIP_TRY(tfg::launch_ns32_fold_test(L, a, f, total, std::max(miter, 0)));
+    TFG_NS(read(sf, h->st));
+    if (sf->done) break;
+    // an inner breakdown, or an inner solve that could not take a step
+    // (its FP32 residual is zero): no refinement can make progress
+    if (si->status == TFG_EBREAKDOWN) {
+      status = TFG_EBREAKDOWN;
+      break;
+    }
+    if (steps == 0) break;
+  }
+  if (sf->status == TFG_EBREAKDOWN) status = TFG_EBREAKDOWN;
+  HIP_TRY(hipEventRecord(h->ev_t1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev_t1));
+  *niter = sf->done ? sf->niter : total;
+  *fin_res = sf->res;
+  if (rep) {
+    rep->status = status;
+    rep->iterations = *niter;
+    rep->refinements = sf->nrefine;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev_t0, h->ev_t1));
+    rep->solve_ms = ms;
+    double tot = 0.0;
+    int cnt = 0;
+    for (size_t q = 0; q < sample_k.size(); ++q) {
+      if (sample_k[q] > *niter) break;
+      float e = 0.f;
+      HIP_TRY(hipEventElapsedTime(&e, h->ev_sample[2 * q], h->ev_sample[2 * q + 1]));
+      tot += e;
+      ++cnt;
+    }
+    rep->spmv_ms = cnt ? tot / cnt : 0.0;
+    rep->spmv_samples = cnt;
+  }
+  if (o.true_residual) {
+    HIP_TRY(tfg::launch_true_residual(L, a));
+    TFG_NS(read(sf, h->st));
+    if (rep) rep->true_relres = sf->aux0;
+  }
+#undef TFG_NS
   return status;
 }
 
@@ -1814,8 +1977,7 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
   const int pr = pv == "jacobi" ? tfg::PREC_JACOBI : (pv == "icc" ? tfg::PREC_ICC : tfg::PREC_NONE);
   if (pr == tfg::PREC_ICC && (method != tfg::NS_NONE || o.precision != TFG_FP64))
     return fail(TFG_ENOTSUP, "preconditioner 'icc' runs with FP64 cg / bicg only");
-  if (method != tfg::NS_NONE && o.precision != TFG_FP64)
-    return fail(TFG_ENOTSUP, "%s runs in FP64 only", sv.c_str());
+
   if (o.criterion != TFG_CRIT_NRMS && o.criterion != TFG_CRIT_RELRES)
     return fail(TFG_EINVAL, "unknown criterion %d", o.criterion);
   if (o.precision != TFG_FP64 && o.precision != TFG_FP32_IR)
@@ -1850,6 +2012,9 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
     if (method != tfg::NS_NONE || pr == tfg::PREC_ICC || o.precision != TFG_FP64)
       return fail(TFG_ENOTSUP, "several ranks: FP64 cg / bicg with jacobi or none");
     rc = tfg::solve_comm(h, pr, dval, db, miter, niter, tol, fin_res, norm, o, rep);
+    if (rc != TFG_OK && rc != TFG_EBREAKDOWN) return rc;
+  } else if (o.precision == TFG_FP32_IR && method != tfg::NS_NONE) {
+    rc = tfg::solve_ir_ns(h, method, pr, dval, db, miter, niter, tol, fin_res, norm, o, rep);
     if (rc != TFG_OK && rc != TFG_EBREAKDOWN) return rc;
   } else if (o.precision == TFG_FP32_IR) {
     rc = tfg::solve_ir(h, pr, dval, db, miter, niter, tol, fin_res, norm, o, rep);

@@ -366,6 +366,30 @@ hipError_t launch_ir_refine(const Launch& L, const CgArgs& a, const IrArgs& f);
 hipError_t launch_ir_restart(const Launch& L, const CgArgs& a, const IrArgs& f);
 // x += e after a breakdown.
 hipError_t launch_ir_fold(const Launch& L, const CgArgs& a, const IrArgs& f);
+// The FP64 true residual of a refinement (when State::refine) with its
+// stopping sums and (s, x) (pt.ext).
+hipError_t launch_ir_resid(const Launch& L, const CgArgs& a, const IrArgs& f);
+
+// FP32 sweeps of the textbook methods (tfg_ir_ns.hip); roles of the vectors
+// per method are listed there.
+struct NsArgs32 {
+  const float* valt;   // BiCG: the FP32 transpose in the row-excess form
+  float* rt;           // shadow residual
+  float* v1;
+  float* v2;
+  float* v3;
+  float* v4;
+};
+hipError_t launch_ns32_valt(const Launch& L, const CgArgs& a, const int* tpos, float* valt);
+// inner restart from the FP64 true residual a.r (inner State `in`)
+hipError_t launch_ns32_restart(const Launch& L, const CgArgs& a, const IrArgs& f,
+                               const NsArgs32& s, State* in, double tol_in, int method);
+hipError_t launch_ns32_iteration(const Launch& L, const Launch& L32, const CgArgs& a,
+                                 const IrArgs& f, const NsArgs32& s, State* in, int method,
+                                 int k, hipEvent_t ev0, hipEvent_t ev1);
+// x += e, FP64 true residual, stopping test (niter = iters; at miter stop)
+hipError_t launch_ns32_fold_test(const Launch& L, const CgArgs& a, const IrArgs& f, int iters,
+                                 int miter);
 // The FP32 row-excess product y = s_i x_i + sum_{j != i} a_ij (x_j - x_i)
 // of an FP32 matrix already converted (building block, tfg_spmv_fp32).
 hipError_t launch_spmv_lap32(const Launch& L32, const SpmvShape& m, const IrArgs& f,

@@ -654,9 +654,13 @@ hipError_t launch_ir_refine(const Launch& L, const CgArgs& a, const IrArgs& f) {
   return launch_ir_restart(L, a, f);
 }
 
-hipError_t launch_ir_restart(const Launch& L, const CgArgs& a, const IrArgs& f) {
+hipError_t launch_ir_resid(const Launch& L, const CgArgs& a, const IrArgs& f) {
   TFG_ENGINE_SWITCH_T(L, double, (k_ir_resid<E><<<L.grid_spmv, kBlock, 0, L.stream>>>(a, f.s64)));
-  hipError_t e = hipGetLastError();
+  return hipGetLastError();
+}
+
+hipError_t launch_ir_restart(const Launch& L, const CgArgs& a, const IrArgs& f) {
+  hipError_t e = launch_ir_resid(L, a, f);
   if (e != hipSuccess) return e;
   if (f.prec == PREC_JACOBI) k_ir_restart<PREC_JACOBI><<<L.grid_vec, kBlock, 0, L.stream>>>(a, f);
   else k_ir_restart<PREC_NONE><<<L.grid_vec, kBlock, 0, L.stream>>>(a, f);
