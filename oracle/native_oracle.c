@@ -722,6 +722,26 @@ static double or_adv_scheme(const or_momentum* m, int s, int c1, int c2,
   return CELL(phi, c) + phij * sgn * (CELL(phi, c2) - CELL(phi, c1));    /* :114 */
 }
 
+double or_adv_face(int scheme, double f, double flux, const double* phi, const double* grad,
+                   const double* d, const double* pmin, const double* pmax, double blend) {
+  const double gx[2] = {grad[0], grad[3]}, gy[2] = {grad[1], grad[4]}, gz[2] = {grad[2], grad[5]};
+  or_momentum m;
+  memset(&m, 0, sizeof(m));
+  m.adv_scheme = scheme;
+  m.blend = blend;
+  m.f = &f;
+  m.v_flux = &flux;
+  m.dx = &d[0];
+  m.dy = &d[1];
+  m.dz = &d[2];
+  m.phi_n = phi;
+  m.gx = gx;
+  m.gy = gy;
+  m.gz = gz;
+  if (scheme == OR_ADV_CENTRAL) return f * phi[0] + (1.0 - f) * phi[1];
+  return or_adv_scheme(&m, 0, 1, 2, pmin, pmax);
+}
+
 void or_momentum_assemble(int nc, int nnz, int n_faces, const int* c1, const int* c2,
                           const int* pos, const int* dia, const or_momentum* m, double* val,
                           double* b, double* v_m) {

@@ -93,6 +93,14 @@ typedef struct {
 void or_momentum_assemble(int nc, int nnz, int n_faces, const int* c1, const int* c2,
                           const int* pos, const int* dia, const or_momentum* m, double* val,
                           double* b, double* v_m);
+/* Face value phi_f of one face between cells 1 (c1) and 2 (c2) by the
+ * advection-scheme routine of the momentum assembly (the limiter alone, for
+ * its tests): f = Grid % f(s), flux = v_flux(s), phi[2] the cell values,
+ * grad[6] the two cells' gradients (x, y, z of cell 1, then of cell 2),
+ * d[3] = Grid % d{x,y,z}(s), pmin[2] / pmax[2] the cells' neighbour bounds,
+ * blend = phi % blends(1) (BLENDED only). */
+double or_adv_face(int scheme, double f, double flux, const double* phi, const double* grad,
+                   const double* d, const double* pmin, const double* pmax, double blend);
 
 /* Rhie-Chow volume fluxes of the domain faces (Rhie_And_Chow.f90:77-259),
  * serial; cell arrays 1-based at cell 1.  Same fields as tfg_rhie_chow. */

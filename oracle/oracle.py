@@ -67,6 +67,8 @@ def lib(omp=False):
         L.or_transpose_positions.argtypes = [P, P]
         L.or_pressure_assemble.argtypes = [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I]
         L.or_momentum_assemble.argtypes = [I, I, I, P, P, P, P, P, P, P, P]
+        L.or_adv_face.restype = D
+        L.or_adv_face.argtypes = [I, D, D, P, P, P, P, P, D]
         L.or_rhie_chow_fluxes.argtypes = [I, I, P, P, P, P]
         _libs[name] = L
     return _libs[name]
@@ -284,6 +286,18 @@ class _Momentum(ctypes.Structure):
 
 
 MOMENTUM_EXT = ("phi_n", "gx", "gy", "gz", "uj_i", "uk_i", "density")
+
+
+def adv_face(adv, f, flux, phi, grad, d, pmin=(-1e300, -1e300), pmax=(1e300, 1e300),
+             blend=0.8):
+    """Face value of one face between cells 1 and 2 by the momentum
+    assembly's advection-scheme routine (Advection_Scheme.f90): f is
+    Grid % f(s) (the weight of cell 1), flux the face's volume flux, phi the
+    two cells' values, grad their gradients (6 values: cell 1 then cell 2),
+    d the vector from cell 1 to cell 2, pmin / pmax the cells' bounds."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (phi, grad, d, pmin, pmax)]
+    return lib().or_adv_face(ADV[adv], float(f), float(flux), *[_p(a) for a in arrs],
+                             float(blend))
 
 
 def momentum_assemble(f, mf, component=1, adv="central", blend_matrix=True, td="linear",
