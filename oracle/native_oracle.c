@@ -28,14 +28,25 @@
 #define OR_MICRO 1.0e-6
 
 /* Summation order of the global reductions.  0 (default) is the reference's
- * serial left-to-right order.  1 (2) sums blocks of 64 (1000) terms left to
- * right and combines the block sums pairwise; 3 sums serially from the last
- * term to the first: equally valid orders, used only to measure how far
- * rounding alone moves a solve (the envelope the GPU's own order is compared
- * against).  Not part of the reference. */
+ * serial left-to-right order.  1, 2, 4, 5 (6) sum blocks of 64, 1000, 256,
+ * 4096 (1) terms left to right and combine the block sums pairwise; 3 sums
+ * serially from the last term to the first.  7 (8) keeps the reference's
+ * order but takes the normalisation factor fn one ulp below (above) the
+ * reference's: the sum of the diagonal that fn comes from rounds differently
+ * in another order, and on ill-conditioned systems one ulp of fn alone moves
+ * the iteration count by as much as reordering the dot products.  9 (10) is
+ * the fully pairwise order 6 with fn one ulp below (above) its own.  Equally
+ * valid answers, used only to measure how far rounding alone moves a solve
+ * (the envelope the GPU's own order is compared against).  Not part of the
+ * reference. */
 static int g_sum_order = 0;
 
 void or_set_sum_order(int order) { g_sum_order = order; }
+
+/* the reductions run in a blocked (not the serial) order */
+static int reordered(void) {
+  return (g_sum_order >= 1 && g_sum_order <= 6) || g_sum_order == 9 || g_sum_order == 10;
+}
 
 /* Progress line on stderr every `every` CG iterations (0: none), so that a
  * long solve (the full-size CPU baseline) reports as it goes.  Not part of
@@ -60,7 +71,11 @@ static double blocked(const double* a, const double* b, int n, int kind) {
       s = s + (kind == 0 ? a[i] * b[i] : (kind == 1 ? (a[i] * a[i]) / (b[i] * b[i]) : a[i]));
     return s;
   }
-  const int bs = g_sum_order == 2 ? 1000 : 64;
+  const int bs = g_sum_order == 2   ? 1000
+                 : g_sum_order == 4 ? 256
+                 : g_sum_order == 5 ? 4096
+                 : g_sum_order == 6 || g_sum_order == 9 || g_sum_order == 10 ? 1
+                                    : 64;
   int nb = (n + bs - 1) / bs;
   double* part = (double*)malloc(sizeof(double) * (size_t)nb);
   for (int k = 0; k < nb; k++) {
@@ -80,7 +95,7 @@ static double blocked(const double* a, const double* b, int n, int kind) {
 double or_nrms(const or_matrix* A, int ni, const double* r, const double* x,
                const double* norm) {
   double rms = 0.0;
-  if (g_sum_order) {
+  if (reordered()) {
     double* dg = (double*)malloc(sizeof(double) * (size_t)(ni > 0 ? ni : 1));
     for (int i = 0; i < ni; i++) dg[i] = A->val[A->dia[i] - 1];
     rms = blocked(r, dg, ni, 1);
@@ -201,7 +216,7 @@ void or_prec_solve(const or_matrix* A, int ni, const double* d,
 static double normalise(or_matrix* A, double* b) {
   double sum_a = 0.0;
   int sum_n = 0;
-  if (g_sum_order) {
+  if (reordered()) {
     double* dg = (double*)malloc(sizeof(double) * (size_t)A->ni);
     for (int i = 0; i < A->ni; i++) dg[i] = A->val[A->dia[i] - 1];
     sum_a = blocked(dg, NULL, A->ni, 2);
@@ -216,6 +231,8 @@ static double normalise(or_matrix* A, double* b) {
   }
   sum_a = sum_a / (double)sum_n;
   double fn = 1.0 / sum_a;
+  if (g_sum_order == 7 || g_sum_order == 9) fn = nextafter(fn, 0.0);
+  if (g_sum_order == 8 || g_sum_order == 10) fn = nextafter(fn, INFINITY);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < A->nt; i++) {
     for (int j = A->row[i] - 1; j < A->row[i + 1] - 1; j++) A->val[j] = A->val[j] * fn;
@@ -233,7 +250,7 @@ static void denormalise(or_matrix* A, double* b, double fn) {
 }
 
 static double sumsq(const double* v, int n) {
-  if (g_sum_order) return blocked(v, v, n, 0);
+  if (reordered()) return blocked(v, v, n, 0);
   double s = 0.0;
   for (int i = 0; i < n; i++) s = s + v[i] * v[i];
   return s;
@@ -286,7 +303,7 @@ int or_cg_ex(or_matrix* A, double* x, double* b, int prec, int miter,
   for (iter = 1; iter <= miter; iter++) {       /* :139 */
     or_prec_solve(A, ni, d, d_inv, q1, r1, prec);   /* :145 */
     rho = 0.0;                                  /* :150-156 */
-    if (g_sum_order) {
+    if (reordered()) {
       rho = blocked(q1, r1, ni, 0);
     } else {
 #pragma omp parallel for schedule(static) reduction(+ : rho)
@@ -302,7 +319,7 @@ int or_cg_ex(or_matrix* A, double* x, double* b, int prec, int miter,
     }
     or_spmv(A, ni, p1, q1);                     /* :176-185 */
     alfa = 0.0;                                 /* :190-198 */
-    if (g_sum_order) {
+    if (reordered()) {
       alfa = blocked(p1, q1, ni, 0);
     } else {
 #pragma omp parallel for schedule(static) reduction(+ : alfa)
@@ -461,7 +478,7 @@ static void spmv_t(const or_matrix* A, int ni, const int* tpos, const double* p,
 }
 
 static double dot(const double* a, const double* b, int n) {
-  if (g_sum_order) return blocked(a, b, n, 0);
+  if (reordered()) return blocked(a, b, n, 0);
   double s = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : s)
   for (int i = 0; i < n; i++) s = s + a[i] * b[i];

@@ -130,7 +130,8 @@ __device__ __forceinline__ double ir_measure(const State* st, double nrms, doubl
 // rounded in FP64, summed in CSR order), the FP32 matrix (CSR: fl32(a_ij fn)
 // with s_i in the diagonal slot; symmetric storage: s_i in slot 0, the upper
 // entries in their slots), s32, rd32 = fl32(1 / (a_ii fn)); partials of
-// sum s_i and sum a_ii fn.
+// sum s_i, sum a_ii fn and (when a.b is given: a solve, not the product
+// building block) the double-double sum of b_i fn.
 template <bool SYM>
 __global__ __launch_bounds__(kBlock) void k_ir_convert(CgArgs a, IrArgs f) {
   __shared__ double s_tmp[2 * 4], s_out[2];
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_convert(CgArgs a, IrArgs f) {
     f.rd32[i] = (float)(1.0 / di);
     v[0] = v[0] + s;
     v[1] = v[1] + di;
-    bs = dd_add(bs, DD{a.b[i] * fn, 0.0});
+    if (a.b) bs = dd_add(bs, DD{a.b[i] * fn, 0.0});
   }
   const int ops[2] = {OP_SUM, OP_SUM};
   publish<2>(v, ops, a.pt.irc, a.pt.pstride, s_tmp, s_out);

@@ -277,7 +277,7 @@ def test_values_round_trip(cases, oracle):
     assert np.all(np.abs(b - s.b) <= 2 * np.spacing(np.abs(s.b)))
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 6, 7, 8, 9, 10])
 def test_reordered_sums_stay_close(cases, oracle, order):
     """Each other summation order (the rounding envelope) solves the same
     problem, and differs from the reference order only by rounding."""
@@ -289,6 +289,26 @@ def test_reordered_sums_stay_close(cases, oracle, order):
     assert abs(ia - ib) <= 1
     assert np.linalg.norm(xa - xb) <= 1e-9 * np.linalg.norm(xa)
     assert not np.array_equal(xa, xb)
+
+
+@pytest.mark.parametrize("order,ulps", [(7, -1), (8, 1)])
+def test_fn_ulp_orders(cases, oracle, order, ulps):
+    """Orders 7 and 8 are the reference's order with fn one ulp away: the
+    scaled values are the reference's with that fn (checked through the
+    first residual of a one-iteration solve with no preconditioner, whose
+    NRMS is a closed form in fn)."""
+    s = cases.hex_pressure(5, rho_out=10.0)
+    fn = 1.0 / (sum(s.val[s.dia - 1].tolist()) / s.n)  # serial sum, Cg.f90:62-76
+    d = s.val[s.dia - 1]
+    for o, f in ((0, fn), (order, np.nextafter(fn, np.inf if ulps > 0 else 0.0))):
+        x = np.zeros(s.n)
+        _, res = oracle.solve("cg", "none", _mat(oracle, s), x, s.b.copy(), 0, 1e-30, norm=1.0,
+                              sum_order=o)
+        # r0 = b fn (x0 = 0); NRMS = sqrt(sum (r_i / (a_ii fn))^2) / 1
+        r = s.b * f
+        dd = d * f
+        exp = np.sqrt(sum(((r * r) / (dd * dd)).tolist()))
+        assert res == exp, (o, res, exp)
 
 
 # ------------------------------------------------------------- fixtures -----
