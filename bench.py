@@ -49,7 +49,13 @@ def _oracle_full_solve(s, omp, progress, tol=1e-8):
     return time.perf_counter() - t0, it, x
 
 
-FP32_TOLS = (1e-8, 5e-9, 2e-9, 1e-9)
+FP32_TOLS = (1e-8, 5e-9, 2e-9, 1e-9, 5e-10)
+# Equal error: the FP32 form's error against the reference solution within
+# 1 % of the FP64 GPU CG's.  At this size the FP64 error at one tolerance
+# itself moves by 2x between equally valid summation orders (the GPU's and
+# the serial oracle's errors at relres 1e-8: 4.9e-7 and 2.4e-7), so 1 % is
+# far inside what rounding alone decides.
+SAME_ERROR_SLACK = 1.01
 
 
 def fp32_leg(G, nat, s, dv, db, dx, tol, miter):
@@ -87,10 +93,13 @@ def fp32_summary(runs, x64, x_ref, t64, tol):
     err64 = rel(x64)
     table = {f"{t:.0e}": {"seconds": v[0], "iterations": v[1], "refinements": v[2],
                           "true_relres": v[3], "x_rel_err_vs_reference": rel(v[4]),
+                          "error_ratio_vs_fp64_cg": rel(v[4]) / err64,
+                          "speedup_vs_fp64_cg": t64 / v[0],
                           "deflated": bool(v[5])} for t, v in runs.items()}
-    ok = [t for t in FP32_TOLS if rel(runs[t][4]) <= err64]
+    ok = [t for t in FP32_TOLS if rel(runs[t][4]) <= SAME_ERROR_SLACK * err64]
     pick = ok[0] if ok else None
     out = {"fp64_cg_x_rel_err_vs_reference": err64, "fp64_cg_seconds": t64, "runs": table,
+           "same_error_rule": f"error <= {SAME_ERROR_SLACK} x the FP64 GPU CG's",
            "tolerance_at_fp64_error": pick,
            "speedup_at_same_error": (t64 / runs[pick][0]) if pick is not None else None,
            "speedup_at_same_tolerance": t64 / runs[tol][0]}
@@ -141,8 +150,12 @@ def cpu_baseline(s, x_gpu, gpu_iters):
         "x_rel_err_gpu_vs_oracle": err,
         "oracle_openmp_iterations": it_omp,
         "x_rel_envelope_openmp_vs_serial_oracle": env,
-        "within_bar": bool(abs(gpu_iters - it1) <= max(2, abs(it_omp - it1))
-                           and err <= max(1e-10, env)),
+        # the parity rule of the tests (tests/parity_bar.py): iterations
+        # within max(2, 1.2 x the spread), x within max(1e-10, 1.2 x the
+        # envelope) of equally valid runs (here the serial and OpenMP oracle)
+        "bar": "iterations within max(2, 1.2 x spread); x within max(1e-10, 1.2 x envelope)",
+        "within_bar": bool(abs(gpu_iters - it1) <= max(2, 1.2 * abs(it_omp - it1))
+                           and err <= max(1e-10, 1.2 * env)),
         "reference_solution": {"solver": "OpenMP oracle Jacobi-CG", "relres": 1e-12,
                                "iterations": it_ref},
         "x_rel_err_gpu_vs_reference": rel(x_gpu, x_ref),

@@ -464,9 +464,11 @@ __global__ __launch_bounds__(kBlock) void k_ir_restart(CgArgs a, IrArgs f) {
   gather<kIrRes>(a.pt.ext, a.pt.pstride, a.pt.g_spmv, ops5, s_tmp, s_out);
   const double nrms = s_out[0], xmin = s_out[1], xmax = s_out[2], rr = s_out[3], sx = s_out[4];
   const double res = ir_measure(st, nrms, xmin, xmax, rr);
+  // the residual of the start (k = 0) is not a refinement
+  const int counted = k > 0 ? 1 : 0;
   if (!isfinite(res) || res < st->tol || k >= st->miter) {
     if (ir_lead()) {
-      st->nrefine = st->nrefine + 1;
+      st->nrefine = st->nrefine + counted;
       st->res = res;
       st->iter = k;
       st->niter = k;
@@ -487,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_restart(CgArgs a, IrArgs f) {
                                  E < 1e-2 * sad));
   const double c = defl ? ((bsum.hi - sx) + bsum.lo) / E : 0.0;
   if (ir_lead()) {
-    st->nrefine = st->nrefine + 1;
+    st->nrefine = st->nrefine + counted;
     st->res = res;
     st->iter = k;
     st->kref = k;
