@@ -387,6 +387,31 @@ int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len,
 int tfg_batch_times(tfg_batch* h, double* ms);
 int tfg_batch_info(tfg_batch* h, int* nsys, int* n, int* nnz, int* blocks);
 
+/* ---- allocation-free solve ----------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory during a solve (code inside HIP graph capture of their own, tight
+ * serving loops, harnesses that forbid hipMalloc): tfg_solve_native_ex with
+ * device pointers whose work vectors (r, p, q, the scaled diagonal), partial
+ * sums, solver state and the symmetric engine's diagonals live in one static
+ * device array of TFG_NOALLOC_MAX_BYTES per GPU, held from the library's
+ * first tfg_matrix_create on that GPU until the process ends.  The call
+ * makes no hipMalloc / hipFree and no other allocating call; x is iterated in
+ * place.  Same kernels, SpMV engine, launch shapes and so the same bits as
+ * tfg_solve_native_ex's FP64 path.  Scope: "cg" / "bicg" (the reference's),
+ * jacobi or none, FP64, no buffer cells, no internal ordering, no timing
+ * (TFG_ENOTSUP otherwise); a system needing more scratch than the maximum
+ * (about 32 bytes per row, plus 8 (K + 1) per row with the symmetric engine:
+ * roughly 0.75M rows of a 7-point grid) returns TFG_EINVAL with the bytes it
+ * needs.  Every call on one GPU shares that GPU's array: calls that overlap
+ * in time on one GPU (two threads, two streams) race and must be serialised
+ * by the caller. */
+#define TFG_NOALLOC_MAX_BYTES (48LL << 20)
+int tfg_solve_native_noalloc(tfg_matrix* h, const char* solver, int solver_len,
+                             const char* prec, int prec_len, const double* d_val,
+                             double* d_x, const double* d_b, int miter, int* niter,
+                             double tol, double* fin_res, const double* norm,
+                             const tfg_options* opt, tfg_report* rep);
+
 const char* tfg_last_error(void);
 int tfg_version(void);
 

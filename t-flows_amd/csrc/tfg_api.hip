@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -70,6 +71,42 @@ int check_csr(int nt, int nnz, const int* row, const int* col, const int* dia) {
 }
 
 }  // namespace tfg
+
+// Scratch of the allocation-free solve (tfg_solve_native_noalloc): one
+// static device array, of which every GPU holds its own copy.  In host code
+// the array's name is a host-side stand-in, not a device address, so it is
+// never passed to a kernel or a copy: set-up resolves each device's copy
+// once, with that device current, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[TFG_NOALLOC_MAX_BYTES];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kScratchDevices] = {};
+
+// Device address of `device`'s scratch array, resolved on its first call
+// (tfg_matrix_create, with `device` current).
+int scratch_setup(int device) {
+  if (device < 0 || device >= kScratchDevices)
+    return fail(TFG_EINVAL, "device %d: the scratch table holds %d devices", device,
+                kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return TFG_OK;
+  void* ptr = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return fail(TFG_EHIP, "no device address for the scratch array on device %d", device);
+  g_scratch[device] = static_cast<unsigned char*>(ptr);
+  return TFG_OK;
+}
+
+unsigned char* scratch_of(int device) {
+  if (device < 0 || device >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch[device];
+}
+
+}  // namespace
 
 // What a captured chunk of CG iterations depends on besides the handle.
 struct GraphKey {
@@ -188,10 +225,11 @@ namespace {
 // ext 6, irc 4.
 constexpr int kPartRows = 26;
 
-tfg::Partials partials_of(const tfg_matrix* h, const tfg::Launch& L) {
+// The partial rows in the handle's buffer, or in `base` (same layout).
+tfg::Partials partials_of(const tfg_matrix* h, const tfg::Launch& L, double* base = nullptr) {
   tfg::Partials pt;
   pt.pstride = h->pstride;
-  pt.diag = h->part;
+  pt.diag = base ? base : h->part;
   pt.pro = pt.diag + 1 * (size_t)h->pstride;
   pt.spmv = pt.pro + 7 * (size_t)h->pstride;
   pt.upd = pt.spmv + 1 * (size_t)h->pstride;
@@ -377,14 +415,20 @@ void free_all(tfg_matrix* h) {
 // 5 timed launches each, keeps the fastest).  TFG_SPMV_KIND=<0..5> forces one.
 int tune_all(tfg_matrix* h, const double* val);
 
+// The engine chosen without timing: the symmetric one at 2M nonzeros and
+// more, else the most compact format the matrix offers.
+int default_kind(const tfg_matrix* h) {
+  return h->nsym > 0 && h->nnz >= (1 << 21) ? tfg::K_SYM_DIA
+         : h->npat > 0                      ? tfg::K_PIPE_PATTERN
+         : h->ntab > 0                      ? tfg::K_PIPE2048_CODED
+                                            : tfg::K_PIPE2048;
+}
+
 int ensure_tuned(tfg_matrix* h, const double* val) {
   if (h->kind >= 0) return TFG_OK;
   const char* at = getenv("TFG_SPMV_AUTOTUNE");
   if (!(at && *at && *at != '0')) {
-    h->kind = h->nsym > 0 && h->nnz >= (1 << 21) ? tfg::K_SYM_DIA
-              : h->npat > 0                      ? tfg::K_PIPE_PATTERN
-              : h->ntab > 0                      ? tfg::K_PIPE2048_CODED
-                                                 : tfg::K_PIPE2048;
+    h->kind = default_kind(h);
     return TFG_OK;
   }
   return tune_all(h, val);
@@ -1468,6 +1512,8 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
     if (e != hipSuccess) { rc = fail(TFG_EHIP, "device attribute: %s", hipGetErrorString(e)); break; }
     e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { rc = fail(TFG_EHIP, "stream: %s", hipGetErrorString(e)); break; }
+    rc = scratch_setup(h->device);
+    if (rc != TFG_OK) break;
 
     std::vector<int> tmp((size_t)std::max(nnz, nt + 1));
     // Row blocks of the two stage engines (make_blocks reads 1-based starts;
@@ -2214,6 +2260,162 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
   else
     HIP_TRY(hipMemcpyAsync(x, h->x, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  if (rc == TFG_EBREAKDOWN) return fail(TFG_EBREAKDOWN, "Krylov breakdown at iteration %d", *niter);
+  return TFG_OK;
+}
+
+// The FP64 CG / reference-BiCG of tfg_solve_native_ex (device pointers,
+// kernel path: the graph path gives the same bits) with its device work in
+// the static scratch array instead of the handle's buffers; nothing is
+// allocated (the pinned polling states and events are the handle's, made at
+// create).  Layout of the scratch, each piece 256-byte aligned: r, p, q, the
+// scaled diagonal (nt doubles each), the partial sums (kPartRows x pstride),
+// the State, then with the symmetric engine its diagonals ((K + 1) x stride)
+// and the asymmetry flag.  What must start at zero (the partials, the
+// diagonals' gaps, the flag) is cleared by every call, since the array keeps
+// the previous call's contents.
+int tfg_solve_native_noalloc(tfg_matrix* h, const char* solver, int solver_len, const char* prec,
+                             int prec_len, const double* d_val, double* d_x, const double* d_b,
+                             int miter, int* niter, double tol, double* fin_res,
+                             const double* norm, const tfg_options* opt, tfg_report* rep) {
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  if (!d_val || !d_x || !d_b || !niter || !fin_res) return fail(TFG_EINVAL, "null argument");
+  tfg_options o{};
+  if (opt) o = *opt;
+  const std::string sv = fstring(solver, solver_len);
+  if (sv == "cgs" || sv == "bcgs" || sv == "bicgstab" || (sv == "bicg" && o.textbook_bicg))
+    return fail(TFG_ENOTSUP, "the allocation-free solve runs the reference's cg / bicg only");
+  if (sv != "cg" && sv != "bicg") return fail(TFG_EINVAL, "Unknown native solver: %s", sv.c_str());
+  const std::string pv = fstring(prec, prec_len);
+  const int pr = pv == "jacobi" ? tfg::PREC_JACOBI : (pv == "icc" ? tfg::PREC_ICC : tfg::PREC_NONE);
+  if (pr == tfg::PREC_ICC)
+    return fail(TFG_ENOTSUP, "the allocation-free solve takes jacobi or none, not icc");
+  if (o.precision != TFG_FP64)
+    return fail(TFG_ENOTSUP, "the allocation-free solve runs in FP64 only");
+  if (o.criterion != TFG_CRIT_NRMS && o.criterion != TFG_CRIT_RELRES)
+    return fail(TFG_EINVAL, "unknown criterion %d", o.criterion);
+  if (o.path < TFG_PATH_AUTO || o.path > TFG_PATH_GRAPH)
+    return fail(TFG_EINVAL, "unknown path %d", o.path);
+  if (o.timing) return fail(TFG_ENOTSUP, "the allocation-free solve does not sample products");
+  if (h->inner) return fail(TFG_ENOTSUP, "the allocation-free solve needs a matrix without "
+                                         "internal ordering");
+  if (h->ni < h->nt || h->allreduce)
+    return fail(TFG_ENOTSUP, "the allocation-free solve needs a matrix without buffer cells");
+  if (rep) {
+    memset(rep, 0, sizeof(*rep));
+    rep->true_relres = -1.0;
+  }
+  unsigned char* base = scratch_of(h->device);
+  if (!base) return fail(TFG_EHIP, "the scratch array of device %d is not set up", h->device);
+
+  // the handle's engine, else the one chosen without timing (not stored:
+  // a later tfg_solve_native_ex may still time the engines)
+  const int kind = h->kind >= 0 ? h->kind : default_kind(h);
+  const bool symk = kind == tfg::K_SYM_DIA;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t n = (size_t)h->nt;
+  const size_t vec = al(n * sizeof(double));
+  const size_t partb = al(sizeof(double) * kPartRows * (size_t)h->pstride);
+  const size_t stb = al(sizeof(State));
+  const size_t sdb = symk ? al(sizeof(double) * (size_t)(h->nsym + 1) * (size_t)h->sdg.stride) : 0;
+  const size_t flb = symk ? 256 : 0;
+  const size_t need = 4 * vec + partb + stb + sdb + flb;
+  if (need > (size_t)TFG_NOALLOC_MAX_BYTES)
+    return fail(TFG_EINVAL,
+                "the allocation-free solve of %d rows needs %zu bytes of scratch; it holds %lld "
+                "(TFG_NOALLOC_MAX_BYTES)", h->nt, need, (long long)TFG_NOALLOC_MAX_BYTES);
+  unsigned char* cur = base;
+  auto take = [&cur](size_t b) { unsigned char* p = cur; cur += b; return p; };
+  double* r = reinterpret_cast<double*>(take(vec));
+  double* p = reinterpret_cast<double*>(take(vec));
+  double* q = reinterpret_cast<double*>(take(vec));
+  double* ad = reinterpret_cast<double*>(take(vec));
+  double* part = reinterpret_cast<double*>(take(partb));
+  State* st = reinterpret_cast<State*>(take(stb));
+  double* sd = symk ? reinterpret_cast<double*>(take(sdb)) : nullptr;
+  int* flag = symk ? reinterpret_cast<int*>(take(flb)) : nullptr;
+
+  HIP_TRY(hipSetDevice(h->device));
+  hipStream_t S = h->stream;
+  HIP_TRY(hipMemsetAsync(part, 0, partb, S));
+  tfg::Launch L = launch_of(h, d_val, kind);
+  tfg::CgArgs a = args_of(h, kind);
+  a.val = d_val;
+  a.b = d_b;
+  a.x = d_x;
+  a.r = r;
+  a.p = p;
+  a.q = q;
+  a.ad = ad;
+  a.pt = partials_of(h, L, part);
+  a.st = st;
+  if (symk) {
+    // the diagonals' absent entries and slot gaps must read zero
+    HIP_TRY(hipMemsetAsync(sd, 0, sdb, S));
+    HIP_TRY(hipMemsetAsync(flag, 0, sizeof(int), S));
+    HIP_TRY(tfg::launch_sym_pack(S, grid_vec_of(h), shape_of(h, tfg::K_SYM_DIA), d_val, sd, flag));
+    int asym = 1;
+    HIP_TRY(hipMemcpyAsync(&asym, flag, sizeof(int), hipMemcpyDeviceToHost, S));
+    HIP_TRY(hipStreamSynchronize(S));
+    L.sym = asym == 0;
+    h->sym_state = L.sym ? 1 : 0;
+    a.sd = sd;
+  }
+
+  State* s0 = &h->hst[0];
+  memset(s0, 0, sizeof(State));
+  s0->tol = tol;
+  s0->criterion = o.criterion;
+  s0->has_norm = norm ? 1 : 0;
+  s0->norm = norm ? *norm : 0.0;
+  s0->kcur = 1;
+  HIP_TRY(hipMemcpyAsync(st, s0, sizeof(State), hipMemcpyHostToDevice, S));
+  HIP_TRY(hipEventRecord(h->ev_t0, S));
+  HIP_TRY(tfg::launch_diag_sum(L, a));
+  HIP_TRY(tfg::launch_prologue(L, a, pr));
+  HIP_TRY(tfg::launch_start(L, a, 6));
+  const bool dx = o.criterion == TFG_CRIT_RELRES || norm;
+  const int chunk = h->nnz > (1 << 22) ? 8 : 32;
+  int k = 1, slot = 0, prev = -1;
+  bool stop = false;
+  while (k <= miter && !stop) {
+    const int kend = std::min(miter, k + chunk - 1);
+    for (; k <= kend; ++k) {
+      HIP_TRY(tfg::launch_pupdate(L, a, pr, k, dx));
+      HIP_TRY(tfg::launch_spmv_pq(L, a, k));
+      HIP_TRY(tfg::launch_update(L, a, pr, k, dx));
+    }
+    HIP_TRY(tfg::launch_finish(L, a, kend, 0));
+    HIP_TRY(hipMemcpyAsync(&h->hst[1 + slot], st, sizeof(State), hipMemcpyDeviceToHost, S));
+    HIP_TRY(hipEventRecord(h->ev_poll[slot], S));
+    if (prev >= 0) {
+      HIP_TRY(hipEventSynchronize(h->ev_poll[prev]));
+      if (h->hst[1 + prev].done) stop = true;
+    }
+    prev = slot;
+    slot ^= 1;
+  }
+  if (dx) HIP_TRY(tfg::launch_fold_x(L, a));
+  HIP_TRY(hipEventRecord(h->ev_t1, S));
+  State* sf = &h->hst[3];
+  HIP_TRY(hipMemcpyAsync(sf, st, sizeof(State), hipMemcpyDeviceToHost, S));
+  HIP_TRY(hipStreamSynchronize(S));
+  *niter = sf->done ? sf->niter : std::max(miter, 0);
+  *fin_res = sf->res;
+  const int rc = sf->status;
+  if (rep) {
+    rep->status = rc;
+    rep->iterations = *niter;
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev_t0, h->ev_t1));
+    rep->solve_ms = ms;
+  }
+  if (o.true_residual) {
+    HIP_TRY(tfg::launch_true_residual(L, a));
+    HIP_TRY(hipMemcpyAsync(sf, st, sizeof(State), hipMemcpyDeviceToHost, S));
+    HIP_TRY(hipStreamSynchronize(S));
+    if (rep) rep->true_relres = sf->aux0;
+  }
   if (rc == TFG_EBREAKDOWN) return fail(TFG_EBREAKDOWN, "Krylov breakdown at iteration %d", *niter);
   return TFG_OK;
 }

@@ -105,6 +105,8 @@ def lib():
                                       D, P, P, I, P, P]
         L.tfg_batch_times.argtypes = [P, P]
         L.tfg_batch_info.argtypes = [P, P, P, P, P]
+        L.tfg_solve_native_noalloc.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P,
+                                               P, I, P, D, P, P, P, P]
         L.tfg_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -254,6 +256,27 @@ class Native:
                                        ctypes.byref(niter), float(tol), ctypes.byref(fin),
                                        ctypes.byref(nv) if nv is not None else None,
                                        1 if device else 0, ctypes.byref(opt), ctypes.byref(rep))
+        _check(rc)
+        if report:
+            return niter.value, fin.value, rep.as_dict()
+        return niter.value, fin.value
+
+    def solve_native_noalloc(self, solver, prec, dval, dx, db, miter, tol, norm=None,
+                             options=None, report=False):
+        """The allocation-free solve (tfg_solve_native_noalloc) on DeviceArray
+        operands: the FP64 CG of solve_native_device with its device work in the
+        library's static scratch array; x is iterated in place."""
+        niter = ctypes.c_int(0)
+        fin = ctypes.c_double(0.0)
+        nv = ctypes.c_double(norm) if norm is not None else None
+        opt = options if options is not None else Options()
+        rep = Report()
+        s, p = solver.encode(), prec.encode()
+        rc = lib().tfg_solve_native_noalloc(self.h, s, len(s), p, len(p), dval.ptr, dx.ptr,
+                                            db.ptr, int(miter), ctypes.byref(niter), float(tol),
+                                            ctypes.byref(fin),
+                                            ctypes.byref(nv) if nv is not None else None,
+                                            ctypes.byref(opt), ctypes.byref(rep))
         _check(rc)
         if report:
             return niter.value, fin.value, rep.as_dict()
