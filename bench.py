@@ -50,6 +50,7 @@ def _oracle_full_solve(s, omp, progress, tol=1e-8):
 
 
 FP32_TOLS = (1e-8, 5e-9, 2e-9, 1e-9, 5e-10)
+KNN_FP32_TOLS = (1e-10, 5e-11, 2e-11, 1e-11, 5e-12)
 # Equal error: the FP32 form's error against the reference solution within
 # 1 % of the FP64 GPU CG's.  At this size the FP64 error at one tolerance
 # itself moves by 2x between equally valid summation orders (the GPU's and
@@ -340,13 +341,23 @@ def bench_upwind(args):
     dx = G.DeviceArray(np.zeros(s.n))
     runs = {}
     spmv_bytes = 12 * s.nnz + 20 * s.n + 4
-    for name, solver, textbook in (("reference_bicg", "bicg", 0), ("bicg", "bicg", 1),
-                                   ("cgs", "cgs", 0), ("bicgstab", "bcgs", 0)):
+    # reference solution: FP64 BiCGSTAB to relres 1e-13
+    ro = G.Options()
+    ro.criterion = G.CRIT_RELRES
+    dx.upload(np.zeros(s.n))
+    nat.solve_native_device("bcgs", "jacobi", dv, dx, db, 5000, 1e-13, options=ro)
+    x_ref = dx.download()
+    for name, solver, textbook, prec32 in (("reference_bicg", "bicg", 0, 0), ("bicg", "bicg", 1, 0),
+                                           ("cgs", "cgs", 0, 0), ("bicgstab", "bcgs", 0, 0),
+                                           ("bicg_fp32_ir", "bicg", 1, 1),
+                                           ("cgs_fp32_ir", "cgs", 0, 1),
+                                           ("bicgstab_fp32_ir", "bcgs", 0, 1)):
         opt = G.Options()
         opt.criterion = G.CRIT_RELRES
         opt.timing = 1
         opt.true_residual = 1
         opt.textbook_bicg = textbook
+        opt.precision = G.FP32_IR if prec32 else G.FP64
         best = None
         for _ in range(args.steps + args.warmup):
             dx.upload(np.zeros(s.n))
@@ -360,15 +371,21 @@ def bench_upwind(args):
                                                           "spmv_ms": 0.0}, str(e)
             dt = time.perf_counter() - t
             if best is None or dt < best[0]:
-                best = (dt, it, res, rep, status)
-        dt, it, res, rep, status = best
+                best = (dt, it, res, rep, status, dx.download())
+        dt, it, res, rep, status, xs = best
         runs[name] = {"seconds": dt, "iterations": it, "relres": res,
                       "true_relres": rep["true_relres"], "status": status,
                       "converged": bool(it >= 0 and res < 1e-8),
+                      "x_rel_err_vs_reference": float(np.linalg.norm(xs - x_ref) /
+                                                      np.linalg.norm(x_ref)),
                       "spmv_GBps": spmv_bytes / (rep["spmv_ms"] * 1e-3) / 1e9 if rep["spmv_ms"] else None}
         log(f"bench upwind{N}: {name}: {dt:.3f} s, {it} iterations, relres {res:.2e}, "
             f"true {rep['true_relres']:.2e} ({status})")
     cpu = None
+    for m in ("bicg", "cgs", "bicgstab"):
+        r32, r64 = runs[f"{m}_fp32_ir"], runs[m]
+        r32["speedup_vs_fp64"] = r64["seconds"] / r32["seconds"]
+        r32["error_ratio_vs_fp64"] = r32["x_rel_err_vs_reference"] / max(r64["x_rel_err_vs_reference"], 1e-300)
     if not args.no_cpu:
         cpu = {k: _oracle_ms_per_iter(s, sv, "jacobi", 20) for k, sv in
                (("reference_bicg", "bicg"), ("bicg", "bicg_textbook"), ("cgs", "cgs"),
@@ -423,33 +440,43 @@ def bench_knn_ir(args):
                 best = (dt, it, res, rep, dx.download())
         return best
 
+    t = time.perf_counter()
     nat.reorder(G.ORDER_RCM)
+    t_order = time.perf_counter() - t
     ref = run(G.FP64, 1e-13)
     out_runs = {}
-    # FP64 CG in the caller's random numbering and with the internal RCM
-    # ordering; FP32 sweeps in FP64 refinement (refinement whenever the inner
-    # residual fell by 1e-4, and the restart-every-50 form, both capped at
-    # 20000 inner iterations) on the RCM ordering
-    for name, prec, tol, im, order in (("fp64_cg", G.FP64, 1e-10, 0, G.ORDER_NATURAL),
-                                       ("fp64_cg_rcm", G.FP64, 1e-10, 0, G.ORDER_RCM),
-                                       ("fp32_ir_rcm", G.FP32_IR, 1e-10, 0, G.ORDER_RCM),
-                                       ("fp32_ir_restart50_rcm", G.FP32_IR, 1e-10, 50,
-                                        G.ORDER_RCM)):
-        t = time.perf_counter()
-        nat.reorder(order)
-        t_order = time.perf_counter() - t
+    # FP64 CG with the internal RCM ordering; FP32 sweeps in FP64 refinement
+    # on it (refinement whenever the inner residual fell by 1e-3, at 1e-10
+    # and tighter, to find the tolerance of equal error, and the
+    # restart-every-50 form of the configuration, capped at 20000 inner
+    # iterations); then FP64 CG in the caller's random numbering
+    legs = [("fp64_cg_rcm", G.FP64, 1e-10, 0, G.ORDER_RCM)]
+    legs += [(f"fp32_ir_rcm_{t:.0e}", G.FP32_IR, t, 0, G.ORDER_RCM) for t in KNN_FP32_TOLS]
+    legs += [("fp32_ir_restart50_rcm", G.FP32_IR, 1e-10, 50, G.ORDER_RCM),
+             ("fp64_cg", G.FP64, 1e-10, 0, G.ORDER_NATURAL)]
+    for name, prec, tol, im, order in legs:
+        if order == G.ORDER_NATURAL:
+            nat.reorder(order)
         dt, it, res, rep, x = run(prec, tol, im, 100000 if prec == G.FP64 else 20000)
         err = float(np.linalg.norm(x - ref[4]) / np.linalg.norm(ref[4]))
-        out_runs[name] = {"seconds": dt, "iterations": it, "refinements": rep["refinements"],
+        out_runs[name] = {"seconds": dt, "tol": tol, "iterations": it,
+                          "refinements": rep["refinements"],
                           "relres": res, "true_relres": rep["true_relres"],
-                          "converged": bool(rep["true_relres"] < tol),
-                          "error_vs_fp64_solution": err,
-                          "ordering_s": t_order if order == G.ORDER_RCM else 0.0}
+                          "converged": bool(rep["true_relres"] < 1e-10),
+                          "error_vs_fp64_solution": err}
         log(f"bench knn: {name} tol {tol}: {dt:.3f} s, {it} iterations, true {rep['true_relres']:.2e}, "
             f"error {err:.2e}")
-    # the value is the fastest form that reached 1e-10 (true residual): on
-    # this pinned, nearly singular Laplacian restarted FP32 sweeps need far
-    # more iterations than FP64 CG and may hit the cap
+    base = out_runs["fp64_cg_rcm"]
+    for v in out_runs.values():
+        v["speedup_vs_fp64_cg_rcm"] = base["seconds"] / v["seconds"]
+        v["error_ratio_vs_fp64_cg_rcm"] = v["error_vs_fp64_solution"] / base["error_vs_fp64_solution"]
+    # equal error: the loosest FP32 tolerance whose error is within 1 % of
+    # FP64 CG's at 1e-10 (the FP32 form stops with more error in the
+    # smooth modes at one tolerance)
+    same = [f"fp32_ir_rcm_{t:.0e}" for t in KNN_FP32_TOLS
+            if out_runs[f"fp32_ir_rcm_{t:.0e}"]["error_ratio_vs_fp64_cg_rcm"] <= SAME_ERROR_SLACK]
+    same_name = same[0] if same else None
+    # the value is the fastest form that reached 1e-10 (true residual)
     ok = {k: v for k, v in out_runs.items() if v["converged"]}
     best_name = min(ok, key=lambda k: ok[k]["seconds"]) if ok else "fp64_cg"
     out = {
@@ -461,7 +488,14 @@ def bench_knn_ir(args):
         "config": {"workload": "kNN 200x200x100 jittered lattice, k 10..18, permuted; Jacobi; relres 1e-10",
                    "n": s.n, "nnz": s.nnz},
         "rcm_speedup": out_runs["fp64_cg"]["seconds"] / out_runs["fp64_cg_rcm"]["seconds"],
-        "reference_solution": {"tol": 1e-13, "iterations": ref[1], "seconds": ref[0]},
+        "ordering_s": t_order,
+        "reference_solution": {"solver": "FP64 GPU CG", "tol": 1e-13, "iterations": ref[1],
+                               "seconds": ref[0]},
+        "fp32_speedup_at_same_tolerance": out_runs["fp32_ir_rcm_1e-10"]["speedup_vs_fp64_cg_rcm"],
+        "same_error_rule": f"error <= {SAME_ERROR_SLACK} x FP64 CG's at 1e-10",
+        "fp32_at_same_error": same_name,
+        "fp32_speedup_at_same_error": (out_runs[same_name]["speedup_vs_fp64_cg_rcm"]
+                                       if same_name else None),
         "runs": out_runs,
     }
     print(json.dumps(out), flush=True)
@@ -469,9 +503,11 @@ def bench_knn_ir(args):
 
 def bench_batch(args):
     """Batched workload: 256 two-phase pressure systems, sides U{20..80},
-    density ratios logU(1, 1000), own seeded RHS each; FP64 Jacobi CG to
-    relres 1e-8 per system, solved as one batch (masks, hipGraph chunks) and
-    one after another through the single-system call."""
+    density ratios logU(1, 1000), own seeded RHS each, relres 1e-8 per
+    system, solved as one batch (masks, hipGraph chunks): FP32 sweeps inside
+    FP64 refinement (the configuration's form) and plain FP64 CG, against the
+    256 FP64 solves one after another through the single-system call (skipped
+    with --no-sequential)."""
     import tflows_cases
     import tflows_gpu as G
 
@@ -488,60 +524,90 @@ def bench_batch(args):
     b = np.concatenate([s.b for s in systems])
     dv, db = G.DeviceArray(val), G.DeviceArray(b)
     dx = G.DeviceArray(np.zeros(total))
-    opt = G.Options()
-    opt.criterion = G.CRIT_RELRES
-    best = None
-    for _ in range(args.steps + args.warmup):
-        dx.upload(np.zeros(total))
-        t = time.perf_counter()
-        niter, fin, rep = bt.solve("cg", "jacobi", dv, dx, db, 20000, 1e-8, options=opt,
-                                   device=True)
-        dt = time.perf_counter() - t
-        if best is None or dt < best[0]:
-            best = (dt, niter.copy(), fin.copy(), rep, bt.times())
-    dt, niter, fin, rep, done_ms = best
-    # algorithmic bytes of the whole batch: every system's iterations at
-    # 12 nnz + 108 n each (the single-system CG iteration)
-    batch_bytes = float(sum(int(it) * (12 * s.nnz + 108 * s.n) for it, s in zip(niter, systems)))
-    log(f"bench batch: {dt:.3f} s wall, device {rep['solve_ms']:.1f} ms, max niter {niter.max()}, "
-        f"{batch_bytes / (rep['solve_ms'] * 1e-3) / 1e9:.0f} GB/s algorithmic")
-    # one after another through the single-system drop-in (device-resident)
-    seq_ms, seq_it = [], []
-    for k, s in enumerate(systems):
-        nat = G.Native(s.row, s.col, s.dia)
-        sv, sb, sx = G.DeviceArray(s.val), G.DeviceArray(s.b), G.DeviceArray(np.zeros(s.n))
-        o = G.Options()
-        o.criterion = G.CRIT_RELRES
-        it, res, r = nat.solve_native_device("cg", "jacobi", sv, sx, sb, 20000, 1e-8, options=o,
-                                             report=True)
-        seq_ms.append(r["solve_ms"])
-        seq_it.append(it)
-        nat.close()
-    seq_ms = np.array(seq_ms)
-    seq_done = np.cumsum(seq_ms)
+
+    def batched(precision):
+        opt = G.Options()
+        opt.criterion = G.CRIT_RELRES
+        opt.precision = precision
+        best = None
+        for _ in range(args.steps + args.warmup):
+            dx.upload(np.zeros(total))
+            t = time.perf_counter()
+            niter, fin, rep = bt.solve("cg", "jacobi", dv, dx, db, 20000, 1e-8, options=opt,
+                                       device=True)
+            dt = time.perf_counter() - t
+            if best is None or dt < best[0]:
+                best = (dt, niter.copy(), fin.copy(), rep, bt.times(), dx.download())
+        return best
+
+    def relres(x):
+        worst = 0.0
+        for k, s in enumerate(systems):
+            xs = x[bt.offsets[k]:bt.offsets[k + 1]]
+            worst = max(worst, float(np.linalg.norm(s.b - s.scipy() @ xs) / np.linalg.norm(s.b)))
+        return worst
+
+    legs = {}
+    for name, prec in (("fp32_ir", G.FP32_IR), ("fp64", G.FP64)):
+        dt, niter, fin, rep, done_ms, x = batched(prec)
+        # algorithmic bytes per iteration in CSR format: FP64 CG at
+        # 12 nnz + 108 n (the single-system CG iteration); FP32 sweeps at
+        # 8 nnz + 56 n (FP32 value and int32 column per nonzero; row start,
+        # p gathered and q written per row; p update 24 n, r update 20 n)
+        per = (lambda s: 12 * s.nnz + 108 * s.n) if prec == G.FP64 else \
+              (lambda s: 8 * s.nnz + 56 * s.n)
+        bbytes = float(sum(int(it) * per(s) for it, s in zip(niter, systems)))
+        legs[name] = {
+            "device_ms": rep["solve_ms"], "wall_s": dt,
+            "solves_per_s": count / (rep["solve_ms"] * 1e-3),
+            "iterations_total": int(niter.sum()), "iterations_max": int(niter.max()),
+            "iterations_median": float(np.median(niter)),
+            "p50_system_ms": float(np.percentile(done_ms, 50)),
+            "p99_system_ms": float(np.percentile(done_ms, 99)),
+            "all_converged": bool(np.all(fin < 1e-8)),
+            "worst_true_relres": relres(x),
+            "GBps_algorithmic": bbytes / (rep["solve_ms"] * 1e-3) / 1e9,
+            "autotuner": bt.tuning(prec),
+        }
+        log(f"bench batch {name}: {dt:.3f} s wall, device {rep['solve_ms']:.1f} ms, "
+            f"{legs[name]['solves_per_s']:.1f} solves/s, max niter {niter.max()}, p50/p99 "
+            f"{legs[name]['p50_system_ms']:.0f}/{legs[name]['p99_system_ms']:.0f} ms, worst true "
+            f"relres {legs[name]['worst_true_relres']:.2e}")
+    seq = None
+    if not args.no_sequential:
+        # one after another through the single-system drop-in (device-resident)
+        seq_ms, seq_it = [], []
+        for k, s in enumerate(systems):
+            nat = G.Native(s.row, s.col, s.dia)
+            sv, sb, sx = G.DeviceArray(s.val), G.DeviceArray(s.b), G.DeviceArray(np.zeros(s.n))
+            o = G.Options()
+            o.criterion = G.CRIT_RELRES
+            it, res, r = nat.solve_native_device("cg", "jacobi", sv, sx, sb, 20000, 1e-8, options=o,
+                                                 report=True)
+            seq_ms.append(r["solve_ms"])
+            seq_it.append(it)
+            nat.close()
+        seq_ms = np.array(seq_ms)
+        seq_done = np.cumsum(seq_ms)
+        seq = {"device_ms": float(seq_ms.sum()),
+               "solves_per_s": count / (seq_ms.sum() * 1e-3),
+               "p50_system_ms": float(np.percentile(seq_ms, 50)),
+               "p99_system_ms": float(np.percentile(seq_ms, 99)),
+               "p50_completion_ms": float(np.percentile(seq_done, 50)),
+               "p99_completion_ms": float(np.percentile(seq_done, 99)),
+               "iterations_total": int(np.sum(seq_it))}
+    head = legs["fp32_ir"]
     out = {
-        "metric": "batched solves/s: 256 two-phase pressure systems (sides 20..80), FP64 Jacobi CG to relres 1e-8",
-        "value": count / (rep["solve_ms"] * 1e-3), "unit": "solves/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f64",
+        "metric": "batched solves/s: 256 two-phase pressure systems (sides 20..80), FP32 Jacobi CG in FP64 refinement to relres 1e-8",
+        "value": head["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "256 hex pressure systems, sides U{20..80}, rho ratio logU(1,1000)",
                    "cells": int(total), "nnz": int(sum(s.nnz for s in systems)),
-                   "blocks": bt.info()["blocks"], "max_iterations": int(niter.max()),
-                   "all_converged": bool(np.all(fin < 1e-8))},
-        "batch": {"device_ms": rep["solve_ms"], "wall_s": dt,
-                  "GBps_algorithmic": batch_bytes / (rep["solve_ms"] * 1e-3) / 1e9,
-                  "iterations_total": int(niter.sum()), "iterations_median": float(np.median(niter)),
-                  "p50_system_ms": float(np.percentile(done_ms, 50)),
-                  "p99_system_ms": float(np.percentile(done_ms, 99))},
-        "sequential": {"device_ms": float(seq_ms.sum()),
-                       "GBps_algorithmic": batch_bytes / (seq_ms.sum() * 1e-3) / 1e9,
-                       "solves_per_s": count / (seq_ms.sum() * 1e-3),
-                       "p50_system_ms": float(np.percentile(seq_ms, 50)),
-                       "p99_system_ms": float(np.percentile(seq_ms, 99)),
-                       "p50_completion_ms": float(np.percentile(seq_done, 50)),
-                       "p99_completion_ms": float(np.percentile(seq_done, 99)),
-                       "same_iterations": bool(np.all(np.abs(np.array(seq_it) - niter) <= 2))},
-        "speedup_vs_sequential": float(seq_ms.sum() / rep["solve_ms"]),
+                   "blocks": bt.info()["blocks"]},
+        "batch_fp32_ir": head, "batch_fp64": legs["fp64"], "sequential_fp64": seq,
+        "fp32_vs_fp64_batch": legs["fp64"]["device_ms"] / head["device_ms"],
+        "speedup_vs_sequential_fp64": (seq["device_ms"] / head["device_ms"]) if seq else None,
     }
     print(json.dumps(out), flush=True)
 
@@ -742,14 +808,22 @@ def bench_poisson40(args):
 
 
 def calibrate():
-    """Known-traffic copy (1 GiB read + 1 GiB write) for counter calibration."""
+    """Known-traffic copy (1 GiB read + 1 GiB write) for counter calibration,
+    and the HBM ceilings of the three access mixes: 1 GiB copied, read, and
+    written (16-byte accesses, 4 workgroups per CU, mean of 5 launches after
+    one warm-up)."""
     import tflows_gpu as G
 
     nbytes = 1 << 30
     src, dst = G.DeviceArray(nbytes, np.uint8), G.DeviceArray(nbytes, np.uint8)
     ms = G.stream_copy(dst, src, nbytes, reps=5)
+    ms_read = G.stream_bench(dst, src, nbytes, 1, reps=5)
+    ms_write = G.stream_bench(dst, None, nbytes, 2, reps=5)
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
-                      "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+                      "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9,
+                      "read_ms": ms_read, "read_GBps": nbytes / (ms_read * 1e-3) / 1e9,
+                      "write_ms": ms_write, "write_GBps": nbytes / (ms_write * 1e-3) / 1e9,
+                      "peak_GBps": HBM_PEAK_GBS}), flush=True)
 
 
 def main():
@@ -759,6 +833,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=200, help="grid side (default 200 -> 8M cells)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batch workload: skip the one-after-another comparison leg")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the known-traffic copy kernel (counter calibration)")
     ap.add_argument("--workload", default="twophase200",

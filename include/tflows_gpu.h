@@ -155,6 +155,12 @@ int tfg_synchronize(tfg_matrix* h);
  * pointers), launched reps+1 times on the default stream; *ms_avg = mean of
  * the last reps.  Its known traffic calibrates the HBM byte counters. */
 int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, double* ms_avg);
+/* The same with mode 0 copy (as above), 1 read-only sweep of d_src (one sum
+ * per workgroup written to d_dst, which must hold 8 x 4 x CUs bytes), 2
+ * write-only fill of d_dst (d_src unused): the HBM ceilings of the three
+ * access mixes. */
+int tfg_stream_bench(void* d_dst, const void* d_src, long long bytes, int mode, int reps,
+                     double* ms_avg);
 
 /* SpMV engines: 0 LDS-staged rows (2040 nonzeros per block), 1 the same
  * with 4088, 2/3/4 sub-wavefront rows with 4/8/16 lanes per row, 5 the
@@ -375,7 +381,10 @@ int tfg_batch_destroy(tfg_batch* handle);
 
 /* val, x, b: concatenated like the structure (host or device pointers);
  * niter[nsys], fin_res[nsys] out; norm: NULL or nsys values.  rep (may be
- * NULL): iterations = the largest niter, solve_ms = device time of the batch. */
+ * NULL): iterations = the largest niter, solve_ms = device time of the batch.
+ * opt->precision TFG_FP32_IR runs every system in the FP32-sweep form of
+ * tfg_solve_native_ex (niter = FP32 iterations, fin_res = the measure of the
+ * last FP64 true residual), jacobi or none. */
 int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len,
                     const char* prec, int prec_len, const double* val, double* x,
                     double* b, int miter, int* niter, double tol, double* fin_res,
@@ -386,6 +395,20 @@ int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len,
  * each system (device clock). */
 int tfg_batch_times(tfg_batch* h, double* ms);
 int tfg_batch_info(tfg_batch* h, int* nsys, int* n, int* nnz, int* blocks);
+
+/* The batch autotuner (no counterpart in the reference).  On the first
+ * solve of a handle in each precision, unless TFG_BATCH_TUNE=0 in the
+ * environment or miter < 32, short timed probes (the first 32 iterations
+ * from the caller's x0, restored afterwards) choose the iterations each
+ * replayed graph holds (FP64: 8, 16 or 32; the FP32 form keeps 8, its
+ * refinement cadence) and, per system-size class (bit 0: < 32768 rows,
+ * bit 1: < 262144, bit 2: larger), whether systems on diagonals take their
+ * products from the diagonals (bit set) or the CSR rows.  Every choice gives
+ * the same bits.  Reports the choice for precision (TFG_FP64 | TFG_FP32_IR):
+ * chunk and diag_classes -1 before the first solve; probes timed and the
+ * best probe's device time.  Any pointer may be NULL. */
+int tfg_batch_tuning(tfg_batch* h, int precision, int* chunk, int* diag_classes, int* probes,
+                     double* probe_ms);
 
 /* ---- allocation-free solve ----------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device

@@ -2594,19 +2594,33 @@ int tfg_memcpy_d2h(void* dst, const void* src, long long bytes) {
 }
 
 int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, double* ms_avg) {
-  if (!d_dst || !d_src || bytes < 16 || (bytes % 16) || reps < 1 || !ms_avg)
-    return fail(TFG_EINVAL, "bad copy request");
-  if (((uintptr_t)d_dst % 16) || ((uintptr_t)d_src % 16)) return fail(TFG_EINVAL, "copy needs 16-byte alignment");
+  return tfg_stream_bench(d_dst, d_src, bytes, 0, reps, ms_avg);
+}
+
+int tfg_stream_bench(void* d_dst, const void* d_src, long long bytes, int mode, int reps,
+                     double* ms_avg) {
+  if (mode < 0 || mode > 2) return fail(TFG_EINVAL, "unknown sweep mode %d", mode);
+  if (!d_dst || (mode != 2 && !d_src) || bytes < 16 || (bytes % 16) || reps < 1 || !ms_avg)
+    return fail(TFG_EINVAL, "bad sweep request");
+  if (((uintptr_t)d_dst % 16) || ((uintptr_t)d_src % 16)) return fail(TFG_EINVAL, "sweeps need 16-byte alignment");
   int dev = 0, cus = 0;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int grid = std::max(1, cus * 4);
+  if (mode == 1 && bytes < (long long)grid * 8)
+    return fail(TFG_EINVAL, "the read sweep writes %d sums to d_dst: it needs %lld bytes", grid,
+                (long long)grid * 8);
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  const int grid = std::max(1, cus * 4);
-  HIP_TRY(tfg::launch_copy16(nullptr, grid, d_src, d_dst, bytes / 16));
+  auto once = [&]() -> hipError_t {
+    if (mode == 0) return tfg::launch_copy16(nullptr, grid, d_src, d_dst, bytes / 16);
+    if (mode == 1) return tfg::launch_read16(nullptr, grid, d_src, static_cast<double*>(d_dst), bytes / 16);
+    return tfg::launch_fill16(nullptr, grid, d_dst, bytes / 16);
+  };
+  HIP_TRY(once());
   HIP_TRY(hipEventRecord(e0, nullptr));
-  for (int r = 0; r < reps; ++r) HIP_TRY(tfg::launch_copy16(nullptr, grid, d_src, d_dst, bytes / 16));
+  for (int r = 0; r < reps; ++r) HIP_TRY(once());
   HIP_TRY(hipEventRecord(e1, nullptr));
   HIP_TRY(hipEventSynchronize(e1));
   float ms = 0.f;

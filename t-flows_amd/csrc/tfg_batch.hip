@@ -33,7 +33,29 @@ namespace tfg {
 namespace {
 
 // Partial rows of the batch (each nblk long).
-enum { BP_DIAG = 0, BP_PRO = 1, BP_SPMV = 8, BP_UPD = 9, BP_ROWS = 14 };
+// FP32 form: BP_UPD + 0..3 (z, r), (s, z), (r, r), sum (r / a_ii)^2 of the
+// FP32 residual; BP_IRC + 0..3 sum s_i, sum a_ii fn, sum b_i fn (hi, lo);
+// BP_RES + 0..4 the FP64 true residual's NRMS sum, x min, x max, (r, r),
+// (s, x).
+enum { BP_DIAG = 0, BP_PRO = 1, BP_SPMV = 8, BP_UPD = 9, BP_IRC = 14, BP_RES = 18, BP_ROWS = 23 };
+
+// FP32 sweeps in FP64 refinement: refinement decisions every kBt32Check
+// iterations (and at the cap), as the single-system form takes them.
+constexpr int kBt32Check = 8;
+
+// FP32 arrays of the refinement form (allocated on the first FP32 solve).
+struct Bt32 {
+  float* val32;   // every system: fl32(a_ij fn), the diagonal slot s_i (row-excess form)
+  float* sd32;    // systems on diagonals: slot 0 s_i, slot t + 1 upper diagonal t (SysDiag layout)
+  double* s64;    // s_i = sum_j a_ij fn, compensated, CSR order
+  float* s32;     // fl32(s_i)
+  float* rd32;    // fl32(1 / (a_ii fn))
+  float* r32;
+  float* p32;
+  float* q32;
+  float* e32;     // FP32 correction since the last refinement
+  int* asym;      // per system: 1 when its values are not symmetric bit for bit
+};
 
 // A system whose entries lie on its diagonal and k <= kBatchSym upper
 // diagonals and their mirrors (a hex grid: k = 3), each row stored diagonal
@@ -68,7 +90,20 @@ struct BatchArgs {
   const SysDiag* sdg;       // per system; null: every system runs the CSR product
   double* sd;               // the diagonals of the systems with k > 0
   int* bad;                 // per system: 1 when this solve's val is not symmetric
+  Bt32 f;                   // FP32 form
+  const uint8_t* scls;      // size class of each system (0: < 32768 rows, 1: < 262144, 2)
+  int diag_mask;            // bit c: systems of class c on diagonals use the diagonals
 };
+
+// Launch choices of a batched solve (chosen by the batch autotuner).
+struct BatchCfg {
+  int chunk;      // iterations per replayed graph (FP64)
+  int diag_mask;  // BatchArgs::diag_mask
+};
+
+__device__ __forceinline__ bool on_diagonals(const BatchArgs& a, int s, const SysDiag& g) {
+  return g.k > 0 && !a.bad[s] && ((a.diag_mask >> a.scls[s]) & 1);
+}
 
 __device__ __forceinline__ double bmeasure(const State* st, double nrms, double xmin,
                                            double xmax, double rr) {
@@ -262,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_spmv(BatchArgs a) {
   const double fn = st->fn;
   double v[1] = {0.0};
   const SysDiag g = a.sdg ? a.sdg[s] : SysDiag{};
-  if (g.k > 0 && !a.bad[s]) {
+  if (on_diagonals(a, s, g)) {
     const int i = a.m.blk[bl] + (int)threadIdx.x;
     if (i < a.m.blk[bl + 1]) {
       const int r0 = a.srow[s], r1 = a.srow[s + 1];
@@ -399,6 +434,446 @@ __global__ __launch_bounds__(kBlock) void k_bt_check(BatchArgs a) {
   }
 }
 
+// ===========================================================================
+// FP32 sweeps inside FP64 refinement (TFG_FP32_IR), system by system.  No
+// counterpart in the reference.  Each system runs the single-system form of
+// tfg_ir.hip: the CG of Cg.f90 in FP32 on the row-excess matrix (row i of
+// the product is s_i x_i + sum_{j != i} a_ij (x_j - x_i), s_i = sum_j a_ij
+// fn), FP64 alpha, beta and dot products, the constant vector deflated when
+// the system's values are symmetric and nearly singular, and every
+// kBt32Check iterations a refinement when due (the FP32 residual fell by
+// delta since the last restart, its estimate of the stopping measure is
+// below tol, `force` iterations passed, or the cap): x += e in FP64, the
+// FP64 true residual, the stopping test on it, the deflation's coarse
+// correction and a restart.  Block kernels do the vector work; one
+// workgroup per system sweeps that system's block partials and sets its
+// scalars.  Per iteration: pupdate | product + (p, q) | alpha | update +
+// sums | check (beta, mu, refinement due?); per chunk end: fold | true
+// residual | test | restart | restart scalars.
+// ===========================================================================
+
+// Conversion (all blocks, once per solve): s_i, the FP32 matrix, 1 / a_ii;
+// partials of sum s_i, sum a_ii fn and the double-double sum of b_i fn.
+__global__ __launch_bounds__(kBlock) void k_bt32_convert(BatchArgs a) {
+  __shared__ double s_tmp[2 * 4], s_out[2];
+  __shared__ DD s_dd[kBlock / 64 + 1];
+  const int bl = blockIdx.x;
+  const int s = a.bsys[bl];
+  const double fn = a.st[s].fn;
+  const SysDiag g = a.sdg ? a.sdg[s] : SysDiag{};
+  const bool dia = g.k > 0 && !a.bad[s];
+  double v[2] = {0.0, 0.0};
+  DD bs{0.0, 0.0};
+  const int i = a.m.blk[bl] + (int)threadIdx.x;
+  if (i < a.m.blk[bl + 1]) {
+    const int rs = a.m.row[i], re = a.m.row[i + 1];
+    const long long li = i - a.srow[s];
+    CSum cs{0.0, 0.0};
+    for (int j = rs; j < re; ++j) {
+      const double aij = a.val[j] * fn;
+      cs.add(aij);
+      if (j == rs) continue;
+      a.f.val32[j] = (float)aij;
+      if (dia) {
+        const int d = a.m.col[j] - i;
+#pragma unroll
+        for (int t = 0; t < kBatchSym; ++t)
+          if (t < g.k && d == g.off[t]) a.f.sd32[g.base + (t + 1) * g.stride + li] = (float)aij;
+      }
+    }
+    const double sx = cs.s + cs.c;
+    const double di = a.val[rs] * fn;
+    a.f.val32[rs] = (float)sx;
+    if (dia) a.f.sd32[g.base + li] = (float)sx;
+    a.f.s64[i] = sx;
+    a.f.s32[i] = (float)sx;
+    a.f.rd32[i] = (float)(1.0 / di);
+    v[0] = sx;
+    v[1] = di;
+    bs = DD{a.b[i] * fn, 0.0};
+  }
+  const int ops[2] = {OP_SUM, OP_SUM};
+  block_publish<2>(a, BP_IRC, bl, v, ops, s_tmp, s_out);
+  const DD t = dd_block(bs, s_dd);
+  if (threadIdx.x == 0) {
+    a.part[(size_t)(BP_IRC + 2) * a.nblk + bl] = t.hi;
+    a.part[(size_t)(BP_IRC + 3) * a.nblk + bl] = t.lo;
+  }
+}
+
+// Values symmetric bit for bit, for the systems on the CSR product (those on
+// diagonals were checked by k_bt_sympack): every lower entry a(i, c) is
+// compared with a(c, i) from row c; a difference or a missing mirror sets
+// f.asym[s].
+__global__ __launch_bounds__(kBlock) void k_bt32_symcheck(BatchArgs a) {
+  const int bl = blockIdx.x;
+  const int s = a.bsys[bl];
+  if (a.sdg && a.sdg[s].k > 0) return;
+  const int i = a.m.blk[bl] + (int)threadIdx.x;
+  if (i >= a.m.blk[bl + 1]) return;
+  bool bad = false;
+  for (int j = a.m.row[i]; j < a.m.row[i + 1]; ++j) {
+    const int c = a.m.col[j];
+    if (c >= i) continue;
+    bool found = false;
+    for (int t = a.m.row[c]; t < a.m.row[c + 1]; ++t)
+      if (a.m.col[t] == i) {
+        found = true;
+        if (__double_as_longlong(a.val[t]) != __double_as_longlong(a.val[j])) bad = true;
+      }
+    if (!found) bad = true;
+  }
+  if (bad) atomicOr(&a.f.asym[s], 1);
+}
+
+// Per system: the conversion sums and the deflation choice (automatic for a
+// system whose values are symmetric bit for bit and whose row sums add to
+// less than 1e-2 of the diagonal's sum, as the single-system form chooses).
+__global__ __launch_bounds__(kBlock) void k_bt32_sysinit(BatchArgs a) {
+  __shared__ double s_tmp[2 * 4], s_out[2];
+  __shared__ DD s_dd[kBlock / 64 + 1];
+  const int s = blockIdx.x;
+  State* st = &a.st[s];
+  const int ops[2] = {OP_SUM, OP_SUM};
+  sys_gather<2>(a, BP_IRC, s, ops, s_tmp, s_out);
+  DD b{0.0, 0.0};
+  for (int bl = a.sblk[s] + (int)threadIdx.x; bl < a.sblk[s + 1]; bl += kBlock)
+    b = dd_add(b, DD{a.part[(size_t)(BP_IRC + 2) * a.nblk + bl],
+                     a.part[(size_t)(BP_IRC + 3) * a.nblk + bl]});
+  b = dd_block(b, s_dd);
+  if (threadIdx.x != 0) return;
+  const double E = s_out[0], sad = s_out[1];
+  const SysDiag g = a.sdg ? a.sdg[s] : SysDiag{};
+  const bool sym = g.k > 0 ? !a.bad[s] : !a.f.asym[s];
+  const bool defl = E > 0.0 && (st->defl_mode == TFG_DEFLATE_ON ||
+                                (st->defl_mode == TFG_DEFLATE_AUTO && sym && E < 1e-2 * sad));
+  st->dsum[0] = E;
+  st->dsum[1] = sad;
+  st->dsum[2] = b.hi;
+  st->dsum[3] = b.lo;
+  st->defl_e = defl ? E : 0.0;
+  st->refine = 1;  // the start runs the restart kernels
+  st->kref = 0;
+  st->nrefine = 0;
+}
+
+// x += e + alpha p (the correction with the last iteration's step) when a
+// refinement is due
+__global__ __launch_bounds__(kBlock) void k_bt32_fold(BatchArgs a) {
+  const int bl = active_block(a);
+  if (bl < 0) return;
+  const State* st = &a.st[a.bsys[bl]];
+  if (st->done || !st->refine) return;
+  const int i = a.m.blk[bl] + threadIdx.x;
+  if (i >= a.m.blk[bl + 1]) return;
+  const float ap = (float)st->alpha;
+  a.x[i] = a.x[i] + (double)(a.f.e32[i] + ap * a.f.p32[i]);
+  a.f.e32[i] = 0.0f;
+}
+
+// The FP64 true residual r = b fn - (A fn) x of a refinement, its stopping
+// sums and (s, x)
+template <class E>
+__global__ __launch_bounds__(kBlock) void k_bt32_resid(BatchArgs a) {
+  __shared__ typename E::Smem sm;
+  __shared__ double s_tmp[5 * 4], s_out[5];
+  const int bl = active_block(a);
+  if (bl < 0) return;
+  const State* st = &a.st[a.bsys[bl]];
+  if (st->done || !st->refine) return;
+  const double fn = st->fn;
+  double v[5] = {0.0, kHuge, -kHuge, 0.0, 0.0};
+  E::template one<true>(
+      sm, a.m, a.val, fn, a.x, s_tmp, s_out, bl, [&](int i) { return a.b[i] * fn; },
+      [&](int i, double ri, double xi, double di) {
+        a.r[i] = ri;
+        v[0] = v[0] + (ri * ri) / (di * di);
+        v[1] = fmin(v[1], xi);
+        v[2] = fmax(v[2], xi);
+        v[3] = v[3] + ri * ri;
+        v[4] = v[4] + a.f.s64[i] * xi;
+      });
+  const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
+  block_publish<5>(a, BP_RES, bl, v, ops, s_tmp, s_out);
+}
+
+// Per system: the stopping test on the true residual (niter = the FP32
+// iterations so far); otherwise the coarse correction c = w'(b - A x) / E
+// = (sum b - (s, x)) / E of the restart
+__global__ __launch_bounds__(kBlock) void k_bt32_test(BatchArgs a) {
+  __shared__ double s_tmp[5 * 4], s_out[5];
+  const int s = blockIdx.x;
+  State* st = &a.st[s];
+  if (st->done || !st->refine) return;
+  const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
+  sys_gather<5>(a, BP_RES, s, ops, s_tmp, s_out);
+  if (threadIdx.x != 0) return;
+  const int k = st->iter;
+  const double xmin = s_out[1], xmax = s_out[2], sx = s_out[4];
+  const double res = bmeasure(st, s_out[0], xmin, xmax, s_out[3]);
+  const int counted = k > 0 ? 1 : 0;  // the start's residual is not a refinement
+  st->nrefine = st->nrefine + counted;
+  st->res = res;
+  if (!isfinite(res) || res < st->tol || k >= st->miter) {
+    st->niter = k;
+    st->status = isfinite(res) ? TFG_OK : TFG_EBREAKDOWN;
+    st->done = 1;
+    stamp(a, s);
+    return;
+  }
+  st->kref = k;
+  st->xmin = xmin;
+  st->xmax = xmax;
+  st->corr = st->defl_e > 0.0 ? ((st->dsum[2] - sx) + st->dsum[3]) / st->defl_e : 0.0;
+}
+
+// Restart: x += c w, r32 = fl32(r - c s), e = 0, and the sums of the
+// restarted residual
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_bt32_restart(BatchArgs a) {
+  __shared__ double s_tmp[4 * 4], s_out[4];
+  const int bl = active_block(a);
+  if (bl < 0) return;
+  const State* st = &a.st[a.bsys[bl]];
+  if (st->done || !st->refine) return;
+  const double c = st->corr;
+  const bool defl = st->defl_e > 0.0;
+  const bool nrm = st->criterion != TFG_CRIT_RELRES;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  const int i = a.m.blk[bl] + (int)threadIdx.x;
+  if (i < a.m.blk[bl + 1]) {
+    const float sv = a.f.s32[i], d = a.f.rd32[i];
+    if (defl) a.x[i] = a.x[i] + c;
+    const float r = (float)(a.r[i] - c * a.f.s64[i]);
+    a.f.r32[i] = r;
+    a.f.e32[i] = 0.0f;
+    const float z = PREC == PREC_JACOBI ? r * d : r;
+    v[0] = (double)z * (double)r;
+    v[1] = defl ? (double)sv * (double)z : 0.0;
+    v[2] = (double)r * (double)r;
+    if (nrm) {
+      const double t = (double)(r * d);
+      v[3] = t * t;
+    }
+  }
+  const int ops[4] = {OP_SUM, OP_SUM, OP_SUM, OP_SUM};
+  block_publish<4>(a, BP_UPD, bl, v, ops, s_tmp, s_out);
+}
+
+// Per system: the scalars the restarted iteration starts from
+__global__ __launch_bounds__(kBlock) void k_bt32_rstart(BatchArgs a) {
+  __shared__ double s_tmp[4 * 4], s_out[4];
+  const int s = blockIdx.x;
+  State* st = &a.st[s];
+  if (st->done || !st->refine) return;
+  const int ops[4] = {OP_SUM, OP_SUM, OP_SUM, OP_SUM};
+  sys_gather<4>(a, BP_UPD, s, ops, s_tmp, s_out);
+  if (threadIdx.x != 0) return;
+  const int k = st->iter;
+  const double rho = s_out[0];
+  if (!isfinite(rho)) {
+    st->status = TFG_EBREAKDOWN;
+    st->niter = k;
+    st->done = 1;
+    stamp(a, s);
+    return;
+  }
+  st->rho[(k + 1) & 1] = rho;
+  st->rref = s_out[2];
+  st->aux1 = 0.0;                                                 // beta
+  st->aux2 = st->defl_e > 0.0 ? s_out[1] / st->defl_e : 0.0;      // mu
+  st->refine = 0;
+}
+
+// p = z + beta p - mu (a fresh start after a restart: p = z - mu), with the
+// deferred e += alpha_{k-1} p_{k-1} on the p read anyway
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_bt32_pupdate(BatchArgs a) {
+  const int bl = active_block(a);
+  if (bl < 0) return;
+  const State* st = &a.st[a.bsys[bl]];
+  if (st->done) return;
+  const int i = a.m.blk[bl] + threadIdx.x;
+  if (i >= a.m.blk[bl + 1]) return;
+  const bool fresh = st->iter == st->kref;
+  const float mu = (float)st->aux2;
+  const float r = a.f.r32[i];
+  const float z = PREC == PREC_JACOBI ? r * a.f.rd32[i] : r;
+  if (fresh) {
+    a.f.p32[i] = z - mu;
+  } else {
+    const float beta = (float)st->aux1, ap = (float)st->alpha;
+    const float pv = a.f.p32[i];
+    a.f.e32[i] = a.f.e32[i] + ap * pv;
+    a.f.p32[i] = (z + beta * pv) - mu;
+  }
+}
+
+// Row i of a system on diagonals in the row-excess form: s_i x_i, the lower
+// entries (x_{i-d} - x_i) for j = k-1 .. 0, then the upper ones: the CSR
+// order of the row-excess CSR product, so the same bits (a missing entry is
+// a zero times a finite difference)
+template <int K>
+__device__ __forceinline__ float bt32_dia_row(const BatchArgs& a, const SysDiag& g, int i, int r0,
+                                              int r1) {
+  const float* sd = a.f.sd32 + g.base;
+  const long long li = i - r0;
+  float lo[K], up[K], xl[K], xu[K];
+  const float sv = sd[li];
+  const float x0 = a.f.p32[i];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const int o = g.off[j];
+    up[j] = sd[(j + 1) * g.stride + li];
+    lo[j] = sd[(j + 1) * g.stride + li - o];
+    xu[j] = a.f.p32[min(i + o, r1 - 1)];
+    xl[j] = a.f.p32[max(i - o, r0)];
+  }
+  float acc = sv * x0;
+#pragma unroll
+  for (int j = K - 1; j >= 0; --j) acc = acc + lo[j] * (xl[j] - x0);
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc = acc + up[j] * (xu[j] - x0);
+  return acc;
+}
+
+// q = A32 p and the block's (p, q)
+template <class E>
+__global__ __launch_bounds__(kBlock) void k_bt32_spmv(BatchArgs a) {
+  __shared__ typename E::Smem sm;
+  __shared__ double s_tmp[4], s_out[1];
+  const int bl = active_block(a);
+  if (bl < 0) return;
+  const int s = a.bsys[bl];
+  const State* st = &a.st[s];
+  if (st->done) return;
+  double v[1] = {0.0};
+  const SysDiag g = a.sdg ? a.sdg[s] : SysDiag{};
+  if (on_diagonals(a, s, g)) {
+    const int i = a.m.blk[bl] + (int)threadIdx.x;
+    if (i < a.m.blk[bl + 1]) {
+      const int r0 = a.srow[s], r1 = a.srow[s + 1];
+      float qi;
+      if (g.k == 1) qi = bt32_dia_row<1>(a, g, i, r0, r1);
+      else if (g.k == 2) qi = bt32_dia_row<2>(a, g, i, r0, r1);
+      else qi = bt32_dia_row<3>(a, g, i, r0, r1);
+      a.f.q32[i] = qi;
+      v[0] = (double)a.f.p32[i] * (double)qi;
+    }
+    const int ops[1] = {OP_SUM};
+    block_publish<1>(a, BP_SPMV, bl, v, ops, s_tmp, s_out);
+    return;
+  }
+  E::template one<false, true>(
+      sm, a.m, a.f.val32, 1.0f, a.f.p32, s_tmp, s_out, bl, [](int) { return 0.0f; },
+      [&](int i, float qi, float pi, float) {
+        a.f.q32[i] = qi;
+        v[0] = v[0] + (double)pi * (double)qi;
+      });
+  const int ops[1] = {OP_SUM};
+  block_publish<1>(a, BP_SPMV, bl, v, ops, s_tmp, s_out);
+}
+
+// alpha = rho_k / (p, q) per system (FP64)
+__global__ __launch_bounds__(kBlock) void k_bt32_alpha(BatchArgs a) {
+  __shared__ double s_tmp[4], s_out[1];
+  const int s = blockIdx.x;
+  State* st = &a.st[s];
+  if (st->done) return;
+  const int ops[1] = {OP_SUM};
+  sys_gather<1>(a, BP_SPMV, s, ops, s_tmp, s_out);
+  if (threadIdx.x != 0) return;
+  const int k = st->iter + 1;
+  const double pq = s_out[0];
+  if (pq == 0.0 || !isfinite(pq)) {
+    st->status = TFG_EBREAKDOWN;
+    st->niter = k;
+    st->done = 1;
+    stamp(a, s);
+    return;
+  }
+  st->alpha = st->rho[k & 1] / pq;
+}
+
+// r -= alpha q and the sums (z, r), (s, z), (r, r), NRMS of the FP32 residual
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_bt32_update(BatchArgs a) {
+  __shared__ double s_tmp[4 * 4], s_out[4];
+  const int bl = active_block(a);
+  if (bl < 0) return;
+  const State* st = &a.st[a.bsys[bl]];
+  if (st->done) return;
+  const float a32 = (float)st->alpha;
+  const bool defl = st->defl_e > 0.0;
+  const bool nrm = st->criterion != TFG_CRIT_RELRES;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  const int i = a.m.blk[bl] + (int)threadIdx.x;
+  if (i < a.m.blk[bl + 1]) {
+    const float d = a.f.rd32[i];
+    const float r = a.f.r32[i] - a32 * a.f.q32[i];
+    a.f.r32[i] = r;
+    const float z = PREC == PREC_JACOBI ? r * d : r;
+    v[0] = (double)z * (double)r;
+    v[1] = defl ? (double)a.f.s32[i] * (double)z : 0.0;
+    v[2] = (double)r * (double)r;
+    if (nrm) {
+      const double t = (double)(r * d);
+      v[3] = t * t;
+    }
+  }
+  const int ops[4] = {OP_SUM, OP_SUM, OP_SUM, OP_SUM};
+  block_publish<4>(a, BP_UPD, bl, v, ops, s_tmp, s_out);
+}
+
+// Per system, end of iteration k: beta and mu of the next iteration and, at
+// a chunk end (every kBt32Check iterations) or the cap, whether a
+// refinement is due
+__global__ __launch_bounds__(kBlock) void k_bt32_check(BatchArgs a) {
+  __shared__ double s_tmp[4 * 4], s_out[4];
+  const int s = blockIdx.x;
+  State* st = &a.st[s];
+  if (st->done) return;
+  const int ops[4] = {OP_SUM, OP_SUM, OP_SUM, OP_SUM};
+  sys_gather<4>(a, BP_UPD, s, ops, s_tmp, s_out);
+  if (threadIdx.x != 0) return;
+  const int k = st->iter + 1;
+  st->iter = k;
+  const double rho = s_out[0], rr = s_out[2];
+  if (!isfinite(rho)) {
+    st->status = TFG_EBREAKDOWN;
+    st->niter = k;
+    st->done = 1;
+    stamp(a, s);
+    return;
+  }
+  st->aux1 = rho / st->rho[k & 1];
+  st->rho[(k + 1) & 1] = rho;
+  st->aux2 = st->defl_e > 0.0 ? s_out[1] / st->defl_e : 0.0;
+  if (k % kBt32Check == 0 || k >= st->miter) {
+    double est;
+    if (st->criterion == TFG_CRIT_RELRES) {
+      est = sqrt(rr) / st->bnorm;
+    } else {
+      const double range = st->has_norm ? fmax(st->norm - 0.0, kTiny) : fmax(st->xmax - st->xmin, kTiny);
+      est = sqrt(s_out[3]) / range;
+    }
+    st->refine = (rr <= st->delta * st->delta * st->rref || est < st->tol ||
+                  (st->force > 0 && k - st->kref >= st->force) || k >= st->miter ||
+                  !isfinite(rr)) ? 1 : 0;
+  }
+}
+
+// x += e of every system that broke down (no step pending); all blocks
+__global__ __launch_bounds__(kBlock) void k_bt32_fold_e(BatchArgs a) {
+  const int bl = blockIdx.x;
+  const State* st = &a.st[a.bsys[bl]];
+  if (st->status != TFG_EBREAKDOWN) return;
+  const int i = a.m.blk[bl] + threadIdx.x;
+  if (i >= a.m.blk[bl + 1]) return;
+  a.x[i] = a.x[i] + (double)a.f.e32[i];
+  a.f.e32[i] = 0.0f;
+}
+
 }  // namespace tfg
 
 // ===========================================================================
@@ -423,6 +898,17 @@ struct tfg_batch {
   unsigned long long* clk = nullptr;  // device, nsys + 1
   std::vector<double> done_ms;        // stop time of each system, last solve
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_poll[2] = {nullptr, nullptr};
+  tfg::Bt32 f{};                      // FP32 form (allocated on first use)
+  bool sd32_zeroed = false;
+  uint8_t* scls = nullptr;            // size class of each system (device)
+  bool class_diag[3] = {false, false, false};  // a class holds a system on diagonals
+  // batch autotuner, per precision (FP64, FP32): choice, probes, best probe
+  bool tuned[2] = {false, false};
+  tfg::BatchCfg cfg[2] = {{16, 7}, {16, 7}};
+  int probes[2] = {0, 0};
+  double probe_ms[2] = {0.0, 0.0};
+  double* x0 = nullptr;               // the caller's x0 while probing
+  std::vector<unsigned long long> clk_host;  // device clock stamps of the last solve
 };
 
 namespace {
@@ -471,7 +957,9 @@ void batch_free(tfg_batch* h) {
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* dev[] = {h->row, h->col, h->blk, h->bsys, h->sblk, h->srow, h->act, h->val, h->b,
-                 h->x, h->r, h->p, h->q, h->ad, h->part, h->st, h->clk, h->sdg, h->sd, h->bad};
+                 h->x, h->r, h->p, h->q, h->ad, h->part, h->st, h->clk, h->sdg, h->sd, h->bad,
+                 h->f.val32, h->f.sd32, h->f.s64, h->f.s32, h->f.rd32, h->f.r32, h->f.p32,
+                 h->f.q32, h->f.e32, h->f.asym, h->scls, h->x0};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   if (h->hst) (void)hipHostFree(h->hst);
@@ -480,6 +968,295 @@ void batch_free(tfg_batch* h) {
   if (h->stream) (void)hipStreamDestroy(h->stream);
   (void)hipGetLastError();  // release errors are dropped, not left for later calls
   delete h;
+}
+
+}  // namespace
+
+namespace {
+
+using tfg::BatchCfg;
+
+// One batched solve from h->x with the launch choices `cfg` (the graph's
+// iterations and the size classes whose systems on diagonals take their
+// products from the diagonals); leaves the final states in h->hst[0..nsys),
+// the device clock stamps in h->clk_host and the device time in *ms.
+int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const double* dval,
+              const double* db, int miter, double tol, const double* norm, const BatchCfg& cfg,
+              float* ms) {
+  using namespace tfg;
+  // per-system state
+  for (int s = 0; s < h->nsys; ++s) {
+    State& q = h->hst[s];
+    memset(&q, 0, sizeof(State));
+    q.tol = tol;
+    q.criterion = o.criterion;
+    q.has_norm = norm ? 1 : 0;
+    q.norm = norm ? norm[s] : 0.0;
+    q.miter = std::max(miter, 0);
+    q.delta = o.inner_tol > 0.0 ? o.inner_tol : 1e-3;
+    q.force = std::max(o.inner_max, 0);
+    q.defl_mode = o.deflation;
+  }
+  HIP_TRY(hipMemcpyAsync(h->st, h->hst, sizeof(State) * (size_t)h->nsys, hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+
+  BatchArgs a{};
+  a.nsys = h->nsys;
+  a.nblk = h->nblk;
+  a.m.n = h->nt;
+  a.m.row = h->row;
+  a.m.col = h->col;
+  a.m.blk = h->blk;
+  a.m.nblk = h->nblk;
+  a.bsys = h->bsys;
+  a.sblk = h->sblk;
+  a.srow = h->srow;
+  a.act = h->act;
+  a.val = dval;
+  a.b = db;
+  a.x = h->x;
+  a.r = h->r;
+  a.p = h->p;
+  a.q = h->q;
+  a.ad = h->ad;
+  a.part = h->part;
+  a.st = h->st;
+  a.clk = h->clk;
+  // Systems on diagonals take their products from the diagonals, refilled
+  // from val here; TFG_BATCH_SPMV=csr keeps every system on the CSR product
+  // (the same bits).
+  const char* force = getenv("TFG_BATCH_SPMV");
+  if (h->sdg && !(force && std::string(force) == "csr")) {
+    if (!h->sd_zeroed) {  // entries a row lacks and the gaps stay zero
+      HIP_TRY(hipMemsetAsync(h->sd, 0, sizeof(double) * (size_t)h->sd_len, h->stream));
+      h->sd_zeroed = true;
+    }
+    if (fp32 && !h->sd32_zeroed) {
+      HIP_TRY(hipMemsetAsync(h->f.sd32, 0, sizeof(float) * (size_t)h->sd_len, h->stream));
+      h->sd32_zeroed = true;
+    }
+    HIP_TRY(hipMemsetAsync(h->bad, 0, sizeof(int) * (size_t)h->nsys, h->stream));
+    a.sdg = h->sdg;
+    a.sd = h->sd;
+    a.bad = h->bad;
+  }
+  a.f = h->f;
+  a.scls = h->scls;
+  a.diag_mask = cfg.diag_mask;
+  const bool vec = ((uintptr_t)dval % 16) == 0 && h->nnz >= 4;
+  const bool vec32 = h->nnz >= 4;  // val32 is the library's, 16-byte aligned
+  const hipStream_t S = h->stream;
+  const int B = kBlock;
+
+#define TFG_BT_ENGINE(CALL)                                                 \
+  do {                                                                      \
+    if (vec) { using E = OneBlock<kCap, double>; CALL; }               \
+    else { using E = ScalarStage<kCap, double>; CALL; }                     \
+  } while (0)
+
+  HIP_TRY(hipEventRecord(h->ev0, S));
+  k_bt_clock<<<1, B, 0, S>>>(a);
+  if (a.sdg) k_bt_sympack<<<h->nblk, B, 0, S>>>(a);
+  k_bt_diag<<<h->nblk, B, 0, S>>>(a);
+  k_bt_fn<<<h->nsys, B, 0, S>>>(a);
+  if (pr == PREC_JACOBI) TFG_BT_ENGINE((k_bt_prologue<E, PREC_JACOBI><<<h->nblk, B, 0, S>>>(a)));
+  else TFG_BT_ENGINE((k_bt_prologue<E, PREC_NONE><<<h->nblk, B, 0, S>>>(a)));
+  k_bt_start<<<h->nsys, B, 0, S>>>(a);
+  if (fp32) {
+    k_bt32_symcheck<<<h->nblk, B, 0, S>>>(a);
+    k_bt32_convert<<<h->nblk, B, 0, S>>>(a);
+    k_bt32_sysinit<<<h->nsys, B, 0, S>>>(a);
+  }
+  HIP_TRY(hipGetLastError());
+
+  // active list: every block; rebuilt from the polled state as systems stop
+  std::vector<int> act(h->nblk);
+  for (int t = 0; t < h->nblk; ++t) act[t] = t;
+  a.nact = h->nblk;
+  HIP_TRY(hipMemcpyAsync(h->act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, S));
+  HIP_TRY(hipStreamSynchronize(S));
+
+#define TFG_BT32_ENGINE(CALL)                                               \
+  do {                                                                      \
+    if (vec32) { using E = OneBlock<kCap, float>; CALL; }                   \
+    else { using E = ScalarStage<kCap, float>; CALL; }                      \
+  } while (0)
+
+  const bool use_graph = o.path != TFG_PATH_KERNELS;
+  const int chunk = fp32 ? kBt32Check : cfg.chunk;
+  hipGraphExec_t exec = nullptr;
+  // FP32 form: the true residual, test and restart of the systems whose
+  // refinement is due (all of them at the start)
+  auto enqueue_refine = [&](int G) {
+    TFG_BT_ENGINE((k_bt32_resid<E><<<G, B, 0, S>>>(a)));
+    k_bt32_test<<<a.nsys, B, 0, S>>>(a);
+    if (pr == PREC_JACOBI) k_bt32_restart<PREC_JACOBI><<<G, B, 0, S>>>(a);
+    else k_bt32_restart<PREC_NONE><<<G, B, 0, S>>>(a);
+    k_bt32_rstart<<<a.nsys, B, 0, S>>>(a);
+  };
+  auto enqueue_chunk = [&](int count) -> hipError_t {
+    const int G = kXcds * ((a.nact + kXcds - 1) / kXcds);
+    if (fp32) {
+      for (int c = 0; c < count; ++c) {
+        if (pr == PREC_JACOBI) k_bt32_pupdate<PREC_JACOBI><<<G, B, 0, S>>>(a);
+        else k_bt32_pupdate<PREC_NONE><<<G, B, 0, S>>>(a);
+        TFG_BT32_ENGINE((k_bt32_spmv<E><<<G, B, 0, S>>>(a)));
+        k_bt32_alpha<<<a.nsys, B, 0, S>>>(a);
+        if (pr == PREC_JACOBI) k_bt32_update<PREC_JACOBI><<<G, B, 0, S>>>(a);
+        else k_bt32_update<PREC_NONE><<<G, B, 0, S>>>(a);
+        k_bt32_check<<<a.nsys, B, 0, S>>>(a);
+      }
+      k_bt32_fold<<<G, B, 0, S>>>(a);
+      enqueue_refine(G);
+      return hipGetLastError();
+    }
+    for (int c = 0; c < count; ++c) {
+      if (pr == PREC_JACOBI) k_bt_pupdate<PREC_JACOBI><<<G, B, 0, S>>>(a);
+      else k_bt_pupdate<PREC_NONE><<<G, B, 0, S>>>(a);
+      TFG_BT_ENGINE((k_bt_spmv<E><<<G, B, 0, S>>>(a)));
+      k_bt_alpha<<<a.nsys, B, 0, S>>>(a);
+      if (pr == PREC_JACOBI) k_bt_update<PREC_JACOBI><<<G, B, 0, S>>>(a);
+      else k_bt_update<PREC_NONE><<<G, B, 0, S>>>(a);
+      k_bt_check<<<a.nsys, B, 0, S>>>(a);
+    }
+    return hipGetLastError();
+  };
+  if (fp32) {  // the start: FP64 residual, test, first restart
+    enqueue_refine(kXcds * ((a.nact + kXcds - 1) / kXcds));
+    HIP_TRY(hipGetLastError());
+  }
+  auto build_graph = [&]() -> int {
+    if (exec) {
+      HIP_TRY(hipGraphExecDestroy(exec));
+      exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal));
+    const hipError_t e1 = enqueue_chunk(chunk);
+    hipError_t e2 = hipStreamEndCapture(S, &g);
+    if (e1 != hipSuccess) e2 = e1;
+    if (e2 == hipSuccess) e2 = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    HIP_TRY(e2);
+    return TFG_OK;
+  };
+  int rc = TFG_OK;
+  if (use_graph) rc = build_graph();
+  if (rc != TFG_OK) return rc;
+
+  // Chunks of iterations; the state of chunk c is read while chunk c+1
+  // runs (stopped systems are masked on the device, so the lag only costs
+  // the masked launches of one chunk).
+  int k = 0, slot = 0, prev = -1;
+  std::vector<int> nact;
+  while (k < miter) {
+    const int cnt = std::min(chunk, miter - k);
+    if (use_graph && cnt == chunk) HIP_TRY(hipGraphLaunch(exec, S));
+    else HIP_TRY(enqueue_chunk(cnt));
+    k += cnt;
+    State* poll = h->hst + (size_t)slot * h->nsys;
+    HIP_TRY(hipMemcpyAsync(poll, h->st, sizeof(State) * (size_t)h->nsys, hipMemcpyDeviceToHost, S));
+    HIP_TRY(hipEventRecord(h->ev_poll[slot], S));
+    if (prev >= 0) {
+      HIP_TRY(hipEventSynchronize(h->ev_poll[prev]));
+      const State* ps = h->hst + (size_t)prev * h->nsys;
+      int running = 0;
+      nact.clear();
+      for (int s = 0; s < h->nsys; ++s) {
+        if (!ps[s].done) {
+          ++running;
+          for (int t = h->h_sblk[s]; t < h->h_sblk[s + 1]; ++t) nact.push_back(t);
+        }
+      }
+      if (!running) break;
+      // compaction once an eighth of the active blocks have stopped (the
+      // copy is stream-ordered after the chunk already queued)
+      if ((int)nact.size() * 8 <= a.nact * 7) {
+        HIP_TRY(hipMemcpyAsync(h->act, nact.data(), sizeof(int) * nact.size(), hipMemcpyHostToDevice, S));
+        HIP_TRY(hipStreamSynchronize(S));
+        a.nact = (int)nact.size();
+        if (use_graph) {
+          rc = build_graph();
+          if (rc != TFG_OK) return rc;
+        }
+      }
+    }
+    prev = slot;
+    slot ^= 1;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (fp32) {  // the correction of systems that broke down
+    k_bt32_fold_e<<<h->nblk, B, 0, S>>>(a);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(h->ev1, S));
+  HIP_TRY(hipMemcpyAsync(h->hst, h->st, sizeof(State) * (size_t)h->nsys, hipMemcpyDeviceToHost, S));
+  h->clk_host.resize(h->nsys + 1);
+  HIP_TRY(hipMemcpyAsync(h->clk_host.data(), h->clk, sizeof(unsigned long long) * h->clk_host.size(),
+                         hipMemcpyDeviceToHost, S));
+  HIP_TRY(hipStreamSynchronize(S));
+  HIP_TRY(hipEventElapsedTime(ms, h->ev0, h->ev1));
+  return TFG_OK;
+#undef TFG_BT_ENGINE
+#undef TFG_BT32_ENGINE
+}
+
+// The batch autotuner: short timed probes (the first kTuneIters iterations
+// from the caller's x0, restored before every probe and before the solve)
+// pick the graph's iterations (FP64: 8, 16 or 32; the FP32 form refines
+// every 8 and keeps 8) and, per size class present, whether systems on
+// diagonals take their products from the diagonals or the CSR rows.  Every
+// choice gives the same bits, so a tuned solve is reproducible; the result
+// is kept per handle and precision.  TFG_BATCH_TUNE=0 keeps the defaults
+// (16 iterations, diagonals for every class).
+constexpr int kTuneIters = 32;
+
+int batch_tune(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const double* dval,
+               const double* db, int miter, double tol, const double* norm) {
+  const int p = fp32 ? 1 : 0;
+  if (h->tuned[p]) return TFG_OK;
+  h->cfg[p] = BatchCfg{16, 7};
+  h->probes[p] = 0;
+  h->probe_ms[p] = 0.0;
+  const char* env = getenv("TFG_BATCH_TUNE");
+  if ((env && *env == '0') || miter < kTuneIters) {
+    h->tuned[p] = true;
+    return TFG_OK;
+  }
+  const size_t N = (size_t)h->nt;
+  if (!h->x0) HIP_TRY(hipMalloc(&h->x0, sizeof(double) * N));
+  HIP_TRY(hipMemcpyAsync(h->x0, h->x, sizeof(double) * N, hipMemcpyDeviceToDevice, h->stream));
+  double best = 1e300;
+  BatchCfg pick = h->cfg[p];
+  auto probe = [&](const BatchCfg& c) -> int {
+    HIP_TRY(hipMemcpyAsync(h->x, h->x0, sizeof(double) * N, hipMemcpyDeviceToDevice, h->stream));
+    float ms = 0.f;
+    const int rc = batch_run(h, pr, fp32, o, dval, db, kTuneIters, tol, norm, c, &ms);
+    if (rc != TFG_OK) return rc;
+    ++h->probes[p];
+    if (ms < best) {
+      best = ms;
+      pick = c;
+    }
+    return TFG_OK;
+  };
+  for (int ch : fp32 ? std::vector<int>{16} : std::vector<int>{8, 16, 32}) {
+    const int rc = probe(BatchCfg{ch, 7});
+    if (rc != TFG_OK) return rc;
+  }
+  if (h->sdg) {
+    const BatchCfg base = pick;
+    for (int c = 0; c < 3; ++c) {
+      if (!h->class_diag[c]) continue;  // no system on diagonals in this class
+      const int rc = probe(BatchCfg{base.chunk, base.diag_mask & ~(1 << c)});
+      if (rc != TFG_OK) return rc;
+    }
+  }
+  h->cfg[p] = pick;
+  h->probe_ms[p] = best;
+  h->tuned[p] = true;
+  HIP_TRY(hipMemcpyAsync(h->x, h->x0, sizeof(double) * N, hipMemcpyDeviceToDevice, h->stream));
+  return TFG_OK;
 }
 
 }  // namespace
@@ -561,6 +1338,7 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
     dmal((void**)&h->part, sizeof(double) * (size_t)tfg::BP_ROWS * bsys.size());
     dmal((void**)&h->st, sizeof(tfg::State) * (size_t)nsys);
     dmal((void**)&h->clk, sizeof(unsigned long long) * (size_t)(nsys + 1));
+    dmal((void**)&h->scls, (size_t)nsys);
     if (nsd > 0) {
       dmal((void**)&h->sdg, sizeof(tfg::SysDiag) * (size_t)nsys);
       dmal((void**)&h->sd, sizeof(double) * (size_t)sd_len);
@@ -583,6 +1361,15 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
     up(h->sblk, sblk.data(), sizeof(int) * sblk.size());
     up(h->srow, srow.data(), sizeof(int) * srow.size());
     if (h->sdg) up(h->sdg, sdg.data(), sizeof(tfg::SysDiag) * sdg.size());
+    {
+      // size classes of the autotuner: < 32768 rows, < 262144, larger
+      std::vector<uint8_t> cls(nsys);
+      for (int q = 0; q < nsys; ++q) {
+        cls[q] = n[q] < (1 << 15) ? 0 : (n[q] < (1 << 18) ? 1 : 2);
+        if (sdg[q].k > 0) h->class_diag[cls[q]] = true;
+      }
+      up(h->scls, cls.data(), (size_t)nsys);
+    }
     if (e != hipSuccess) { rc = fail(TFG_EHIP, "batch upload: %s", hipGetErrorString(e)); break; }
   } while (0);
   if (rc != TFG_OK) {
@@ -615,7 +1402,11 @@ int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len, const char
   if (opt) o = *opt;
   if (o.criterion != TFG_CRIT_NRMS && o.criterion != TFG_CRIT_RELRES)
     return fail(TFG_EINVAL, "unknown criterion %d", o.criterion);
-  if (o.precision != TFG_FP64) return fail(TFG_ENOTSUP, "batched solves run in FP64");
+  if (o.precision != TFG_FP64 && o.precision != TFG_FP32_IR)
+    return fail(TFG_EINVAL, "unknown precision %d", o.precision);
+  if (o.deflation < TFG_DEFLATE_AUTO || o.deflation > TFG_DEFLATE_ON)
+    return fail(TFG_EINVAL, "unknown deflation mode %d", o.deflation);
+  const bool fp32 = o.precision == TFG_FP32_IR;
   if (rep) {
     memset(rep, 0, sizeof(*rep));
     rep->true_relres = -1.0;
@@ -635,161 +1426,21 @@ int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len, const char
   } else {
     HIP_TRY(hipMemcpyAsync(h->x, x, sizeof(double) * N, hipMemcpyDeviceToDevice, h->stream));
   }
-  // per-system state
-  for (int s = 0; s < h->nsys; ++s) {
-    State& q = h->hst[s];
-    memset(&q, 0, sizeof(State));
-    q.tol = tol;
-    q.criterion = o.criterion;
-    q.has_norm = norm ? 1 : 0;
-    q.norm = norm ? norm[s] : 0.0;
+  if (fp32) {  // the FP32 form's arrays, once per handle
+    const size_t nn = std::max<size_t>(N, 4);
+    if (!h->f.val32) HIP_TRY(hipMalloc(&h->f.val32, sizeof(float) * (size_t)std::max(h->nnz, 4)));
+    if (!h->f.s64) HIP_TRY(hipMalloc(&h->f.s64, sizeof(double) * nn));
+    for (float** v : {&h->f.s32, &h->f.rd32, &h->f.r32, &h->f.p32, &h->f.q32, &h->f.e32})
+      if (!*v) HIP_TRY(hipMalloc(v, sizeof(float) * nn));
+    if (h->sdg && !h->f.sd32) HIP_TRY(hipMalloc(&h->f.sd32, sizeof(float) * (size_t)h->sd_len));
+    if (!h->f.asym) HIP_TRY(hipMalloc(&h->f.asym, sizeof(int) * (size_t)h->nsys));
+    HIP_TRY(hipMemsetAsync(h->f.asym, 0, sizeof(int) * (size_t)h->nsys, h->stream));
   }
-  HIP_TRY(hipMemcpyAsync(h->st, h->hst, sizeof(State) * (size_t)h->nsys, hipMemcpyHostToDevice, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
-
-  BatchArgs a{};
-  a.nsys = h->nsys;
-  a.nblk = h->nblk;
-  a.m.n = h->nt;
-  a.m.row = h->row;
-  a.m.col = h->col;
-  a.m.blk = h->blk;
-  a.m.nblk = h->nblk;
-  a.bsys = h->bsys;
-  a.sblk = h->sblk;
-  a.srow = h->srow;
-  a.act = h->act;
-  a.val = dval;
-  a.b = db;
-  a.x = h->x;
-  a.r = h->r;
-  a.p = h->p;
-  a.q = h->q;
-  a.ad = h->ad;
-  a.part = h->part;
-  a.st = h->st;
-  a.clk = h->clk;
-  // Systems on diagonals take their products from the diagonals, refilled
-  // from val here; TFG_BATCH_SPMV=csr keeps every system on the CSR product
-  // (the same bits).
-  const char* force = getenv("TFG_BATCH_SPMV");
-  if (h->sdg && !(force && std::string(force) == "csr")) {
-    if (!h->sd_zeroed) {  // entries a row lacks and the gaps stay zero
-      HIP_TRY(hipMemsetAsync(h->sd, 0, sizeof(double) * (size_t)h->sd_len, h->stream));
-      h->sd_zeroed = true;
-    }
-    HIP_TRY(hipMemsetAsync(h->bad, 0, sizeof(int) * (size_t)h->nsys, h->stream));
-    a.sdg = h->sdg;
-    a.sd = h->sd;
-    a.bad = h->bad;
-  }
-  const bool vec = ((uintptr_t)dval % 16) == 0 && h->nnz >= 4;
+  { const int rt = batch_tune(h, pr, fp32, o, dval, db, miter, tol, norm); if (rt != TFG_OK) return rt; }
+  float ms = 0.f;
+  { const int rr = batch_run(h, pr, fp32, o, dval, db, miter, tol, norm, h->cfg[fp32 ? 1 : 0], &ms);
+    if (rr != TFG_OK) return rr; }
   const hipStream_t S = h->stream;
-  const int B = kBlock;
-
-#define TFG_BT_ENGINE(CALL)                                                 \
-  do {                                                                      \
-    if (vec) { using E = OneBlock<kCap, double>; CALL; }               \
-    else { using E = ScalarStage<kCap, double>; CALL; }                     \
-  } while (0)
-
-  HIP_TRY(hipEventRecord(h->ev0, S));
-  k_bt_clock<<<1, B, 0, S>>>(a);
-  if (a.sdg) k_bt_sympack<<<h->nblk, B, 0, S>>>(a);
-  k_bt_diag<<<h->nblk, B, 0, S>>>(a);
-  k_bt_fn<<<h->nsys, B, 0, S>>>(a);
-  if (pr == PREC_JACOBI) TFG_BT_ENGINE((k_bt_prologue<E, PREC_JACOBI><<<h->nblk, B, 0, S>>>(a)));
-  else TFG_BT_ENGINE((k_bt_prologue<E, PREC_NONE><<<h->nblk, B, 0, S>>>(a)));
-  k_bt_start<<<h->nsys, B, 0, S>>>(a);
-  HIP_TRY(hipGetLastError());
-
-  // active list: every block; rebuilt from the polled state as systems stop
-  std::vector<int> act(h->nblk);
-  for (int t = 0; t < h->nblk; ++t) act[t] = t;
-  a.nact = h->nblk;
-  HIP_TRY(hipMemcpyAsync(h->act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, S));
-  HIP_TRY(hipStreamSynchronize(S));
-
-  const bool use_graph = o.path != TFG_PATH_KERNELS;
-  const int chunk = 16;
-  hipGraphExec_t exec = nullptr;
-  auto enqueue_chunk = [&](int count) -> hipError_t {
-    const int G = kXcds * ((a.nact + kXcds - 1) / kXcds);
-    for (int c = 0; c < count; ++c) {
-      if (pr == PREC_JACOBI) k_bt_pupdate<PREC_JACOBI><<<G, B, 0, S>>>(a);
-      else k_bt_pupdate<PREC_NONE><<<G, B, 0, S>>>(a);
-      TFG_BT_ENGINE((k_bt_spmv<E><<<G, B, 0, S>>>(a)));
-      k_bt_alpha<<<a.nsys, B, 0, S>>>(a);
-      if (pr == PREC_JACOBI) k_bt_update<PREC_JACOBI><<<G, B, 0, S>>>(a);
-      else k_bt_update<PREC_NONE><<<G, B, 0, S>>>(a);
-      k_bt_check<<<a.nsys, B, 0, S>>>(a);
-    }
-    return hipGetLastError();
-  };
-  auto build_graph = [&]() -> int {
-    if (exec) {
-      HIP_TRY(hipGraphExecDestroy(exec));
-      exec = nullptr;
-    }
-    hipGraph_t g = nullptr;
-    HIP_TRY(hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal));
-    const hipError_t e1 = enqueue_chunk(chunk);
-    hipError_t e2 = hipStreamEndCapture(S, &g);
-    if (e1 != hipSuccess) e2 = e1;
-    if (e2 == hipSuccess) e2 = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    if (g) (void)hipGraphDestroy(g);
-    HIP_TRY(e2);
-    return TFG_OK;
-  };
-  int rc = TFG_OK;
-  if (use_graph) rc = build_graph();
-  if (rc != TFG_OK) return rc;
-
-  // Chunks of iterations; the state of chunk c is read while chunk c+1
-  // runs (stopped systems are masked on the device, so the lag only costs
-  // the masked launches of one chunk).
-  int k = 0, slot = 0, prev = -1;
-  std::vector<int> nact;
-  while (k < miter) {
-    const int cnt = std::min(chunk, miter - k);
-    if (use_graph && cnt == chunk) HIP_TRY(hipGraphLaunch(exec, S));
-    else HIP_TRY(enqueue_chunk(cnt));
-    k += cnt;
-    State* poll = h->hst + (size_t)slot * h->nsys;
-    HIP_TRY(hipMemcpyAsync(poll, h->st, sizeof(State) * (size_t)h->nsys, hipMemcpyDeviceToHost, S));
-    HIP_TRY(hipEventRecord(h->ev_poll[slot], S));
-    if (prev >= 0) {
-      HIP_TRY(hipEventSynchronize(h->ev_poll[prev]));
-      const State* ps = h->hst + (size_t)prev * h->nsys;
-      int running = 0;
-      nact.clear();
-      for (int s = 0; s < h->nsys; ++s) {
-        if (!ps[s].done) {
-          ++running;
-          for (int t = h->h_sblk[s]; t < h->h_sblk[s + 1]; ++t) nact.push_back(t);
-        }
-      }
-      if (!running) break;
-      // compaction once an eighth of the active blocks have stopped (the
-      // copy is stream-ordered after the chunk already queued)
-      if ((int)nact.size() * 8 <= a.nact * 7) {
-        HIP_TRY(hipMemcpyAsync(h->act, nact.data(), sizeof(int) * nact.size(), hipMemcpyHostToDevice, S));
-        HIP_TRY(hipStreamSynchronize(S));
-        a.nact = (int)nact.size();
-        if (use_graph) {
-          rc = build_graph();
-          if (rc != TFG_OK) return rc;
-        }
-      }
-    }
-    prev = slot;
-    slot ^= 1;
-  }
-  if (exec) (void)hipGraphExecDestroy(exec);
-  HIP_TRY(hipEventRecord(h->ev1, S));
-  HIP_TRY(hipMemcpyAsync(h->hst, h->st, sizeof(State) * (size_t)h->nsys, hipMemcpyDeviceToHost, S));
-  std::vector<unsigned long long> clk(h->nsys + 1);
-  HIP_TRY(hipMemcpyAsync(clk.data(), h->clk, sizeof(unsigned long long) * clk.size(), hipMemcpyDeviceToHost, S));
   if (!device_ptrs)
     HIP_TRY(hipMemcpyAsync(x, h->x, sizeof(double) * N, hipMemcpyDeviceToHost, S));
   else
@@ -797,8 +1448,7 @@ int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len, const char
   HIP_TRY(hipStreamSynchronize(S));
   int rate_khz = 0;
   HIP_TRY(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, h->device));
-  float ms = 0.f;
-  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  const std::vector<unsigned long long>& clk = h->clk_host;
   int status = TFG_OK, maxit = 0;
   for (int s = 0; s < h->nsys; ++s) {
     const State& q = h->hst[s];
@@ -815,7 +1465,19 @@ int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len, const char
   }
   if (status == TFG_EBREAKDOWN) return fail(TFG_EBREAKDOWN, "Krylov breakdown in a batched system");
   return TFG_OK;
-#undef TFG_BT_ENGINE
+}
+
+int tfg_batch_tuning(tfg_batch* h, int precision, int* chunk, int* diag_classes, int* probes,
+                     double* probe_ms) {
+  if (!h) return fail(TFG_EINVAL, "null handle");
+  if (precision != TFG_FP64 && precision != TFG_FP32_IR)
+    return fail(TFG_EINVAL, "unknown precision %d", precision);
+  const int p = precision == TFG_FP32_IR ? 1 : 0;
+  if (chunk) *chunk = !h->tuned[p] ? -1 : (p ? tfg::kBt32Check : h->cfg[p].chunk);
+  if (diag_classes) *diag_classes = h->tuned[p] ? h->cfg[p].diag_mask : -1;
+  if (probes) *probes = h->probes[p];
+  if (probe_ms) *probe_ms = h->probe_ms[p];
+  return TFG_OK;
 }
 
 int tfg_batch_times(tfg_batch* h, double* ms) {

@@ -71,6 +71,11 @@ struct State {
   int sym_vals;     // the values are symmetric (deflation allowed); cleared by
                     // any thread of the check that finds an asymmetric pair
   int nrefine;      // refinements (true residuals) so far
+  // batched FP32 sweeps (tfg_batch.hip): per solve sum_i s_i, sum_i a_ii fn,
+  // sum_i b_i fn as a double-double (hi, lo); the coarse correction c of
+  // the pending restart
+  double dsum[4];
+  double corr;
 };
 
 // Partial sums written by one kernel and swept by the next ones.
@@ -347,6 +352,9 @@ hipError_t launch_true_residual(const Launch& L, const CgArgs& a);
 hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* val,
                              const double* p, double* q);
 hipError_t launch_copy16(hipStream_t stream, int grid, const void* src, void* dst, long long n16);
+// out: one double per workgroup (grid entries)
+hipError_t launch_read16(hipStream_t stream, int grid, const void* src, double* out, long long n16);
+hipError_t launch_fill16(hipStream_t stream, int grid, void* dst, long long n16);
 
 // FP32 sweeps in FP64 refinement (tfg_ir.hip).  L: the FP64 engine (the
 // true residual); L32: the FP32 product's engine (kind 8 = the symmetric

@@ -51,52 +51,6 @@ constexpr int kIrRes = 5;
 
 __device__ __forceinline__ bool ir_lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
-// Compensated (Neumaier) addition: the running sum and the rounding errors
-// of its additions, added at the end.
-struct CSum {
-  double s, c;
-  __device__ __forceinline__ void add(double v) {
-    const double t = s + v;
-    if (fabs(s) >= fabs(v)) c = c + ((s - t) + v);
-    else c = c + ((v - t) + s);
-    s = t;
-  }
-};
-
-// Double-double sums for the one reduction that must not lose digits to
-// cancellation (sum_i b_i of a right-hand side that sums to nearly zero):
-// hi + lo with |lo| <= ulp(hi) / 2; additions by Knuth's two-sum.
-struct DD {
-  double hi, lo;
-};
-__device__ __forceinline__ DD dd_add(DD a, DD b) {
-  const double s = a.hi + b.hi;
-  const double bb = s - a.hi;
-  const double err = (a.hi - (s - bb)) + (b.hi - bb);
-  const double lo = err + (a.lo + b.lo);
-  const double hi = s + lo;
-  return DD{hi, lo - (hi - s)};
-}
-// Sum over the workgroup; on return s_dd[0] holds it for every thread
-// (s_dd: kBlock / 64 + 1 entries).
-__device__ __forceinline__ DD dd_block(DD v, DD* s_dd) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const DD o{__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)};
-    v = dd_add(v, o);
-  }
-  if (lane == 0) s_dd[1 + wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    DD t = s_dd[1];
-    for (int w = 1; w < kBlock / 64; ++w) t = dd_add(t, s_dd[1 + w]);
-    s_dd[0] = t;
-  }
-  __syncthreads();
-  return s_dd[0];
-}
-
 __device__ __forceinline__ SpmvShape ir_shape(const CgArgs& a) {
   SpmvShape m;
   m.n = a.n;

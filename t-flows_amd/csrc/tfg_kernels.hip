@@ -670,8 +670,41 @@ __global__ __launch_bounds__(kBlock) void k_copy16(const dbl2* __restrict__ src,
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+// Read-only sweep: every thread sums its 16-byte chunks; one sum per
+// workgroup lands in out[blockIdx.x] (so the loads are not dead code).
+__global__ __launch_bounds__(kBlock) void k_read16(const dbl2* __restrict__ src,
+                                                   double* __restrict__ out, long long n16) {
+  __shared__ double s_tmp[4], s_out[1];
+  const long long stride = (long long)gridDim.x * kBlock;
+  double v[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) {
+    const dbl2 c = src[i];
+    v[0] = v[0] + (c.x + c.y);
+  }
+  const int ops[1] = {OP_SUM};
+  block_reduce<1>(v, ops, s_tmp, s_out);
+  if (threadIdx.x == 0) out[blockIdx.x] = s_out[0];
+}
+
+// Write-only fill of 16-byte chunks.
+__global__ __launch_bounds__(kBlock) void k_fill16(dbl2* __restrict__ dst, long long n16) {
+  const long long stride = (long long)gridDim.x * kBlock;
+  const dbl2 z = {1.0, 2.0};
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) dst[i] = z;
+}
+
 hipError_t launch_copy16(hipStream_t stream, int grid, const void* src, void* dst, long long n16) {
   k_copy16<<<grid, kBlock, 0, stream>>>(reinterpret_cast<const dbl2*>(src), reinterpret_cast<dbl2*>(dst), n16);
+  return hipGetLastError();
+}
+
+hipError_t launch_read16(hipStream_t stream, int grid, const void* src, double* out, long long n16) {
+  k_read16<<<grid, kBlock, 0, stream>>>(reinterpret_cast<const dbl2*>(src), out, n16);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill16(hipStream_t stream, int grid, void* dst, long long n16) {
+  k_fill16<<<grid, kBlock, 0, stream>>>(reinterpret_cast<dbl2*>(dst), n16);
   return hipGetLastError();
 }
 

@@ -685,7 +685,7 @@ template <int CAP, typename T>
 struct OneBlock {
   using SE = StreamEngine<CAP, false, T>;
   using Smem = typename SE::Smem;
-  template <bool SUB, typename Init, typename Epi>
+  template <bool SUB, bool LAP = false, typename Init, typename Epi>
   __device__ __forceinline__ static void one(Smem& sm, const SpmvShape& m,
                                              const T* __restrict__ val, T fn, const T* x,
                                              double* s_tmp, double* s_out, int lb, Init init,
@@ -697,7 +697,7 @@ struct OneBlock {
     SE::fetch(m, val, nnz, lb, g);
     const int r0 = g.r0, nr = g.nr, s = g.s, e = g.e, rs = g.rs, re = g.re;
     if (e - s > CAP) {
-      long_row<SUB>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
+      long_row<SUB, LAP>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
       return;
     }
     g.v.put(sm.val, nnz, s, e);
@@ -706,12 +706,13 @@ struct OneBlock {
     const int ov = s & ~(EPV - 1), oc = s & ~(SE::EPC - 1);
     if (tid < nr) {
       const int i = r0 + tid;
+      const T xi = x[sm.col[rs - oc]];
       T acc = init(i);
       for (int j = rs; j < re; ++j) {
-        const T a = sm.val[j - ov] * fn;
-        acc = SUB ? acc - a * x[sm.col[j - oc]] : acc + a * x[sm.col[j - oc]];
+        const T t = row_term<LAP>(sm.val[j - ov] * fn, x[sm.col[j - oc]], xi, j == rs);
+        acc = SUB ? acc - t : acc + t;
       }
-      epi(i, acc, x[sm.col[rs - oc]], sm.val[rs - ov] * fn);
+      epi(i, acc, xi, sm.val[rs - ov] * fn);
     }
     __syncthreads();
   }
@@ -761,7 +762,7 @@ struct ScalarStage {
   }
 
   // one given row block lb (see OneBlock)
-  template <bool SUB, typename Init, typename Epi>
+  template <bool SUB, bool LAP = false, typename Init, typename Epi>
   __device__ __forceinline__ static void one(Smem& sm, const SpmvShape& m,
                                              const T* __restrict__ val, T fn,
                                              const T* __restrict__ x, double* s_tmp,
@@ -770,7 +771,7 @@ struct ScalarStage {
     const int r0 = m.blk[lb], nr = m.blk[lb + 1] - r0;
     const int s = m.row[r0], e = m.row[r0 + nr];
     if (e - s > CAP) {
-      long_row<SUB>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
+      long_row<SUB, LAP>(m, val, fn, x, r0, s, e, s_tmp, s_out, init, epi);
       return;
     }
     for (int t = tid; t < e - s; t += kBlock) {
@@ -781,12 +782,13 @@ struct ScalarStage {
     if (tid < nr) {
       const int i = r0 + tid;
       const int rs = m.row[i], re = m.row[i + 1];
+      const T xi = x[sm.col[rs - s]];
       T acc = init(i);
       for (int j = rs; j < re; ++j) {
-        const T a = sm.val[j - s] * fn;
-        acc = SUB ? acc - a * x[sm.col[j - s]] : acc + a * x[sm.col[j - s]];
+        const T t = row_term<LAP>(sm.val[j - s] * fn, x[sm.col[j - s]], xi, j == rs);
+        acc = SUB ? acc - t : acc + t;
       }
-      epi(i, acc, x[sm.col[rs - s]], sm.val[rs - s] * fn);
+      epi(i, acc, xi, sm.val[rs - s] * fn);
     }
     __syncthreads();
   }

@@ -92,6 +92,7 @@ def lib():
         L.tfg_memcpy_d2h.argtypes = [P, P, LL]
         L.tfg_synchronize.argtypes = [P]
         L.tfg_stream_copy.argtypes = [P, P, LL, I, ctypes.POINTER(D)]
+        L.tfg_stream_bench.argtypes = [P, P, LL, I, I, ctypes.POINTER(D)]
         L.tfg_matrix_set_comm.argtypes = [P, ALLREDUCE_FN, EXCHANGE_FN, P]
         L.tfg_faces_create.argtypes = [P, I, P, P, P, P, P, ctypes.POINTER(P)]
         L.tfg_faces_destroy.argtypes = [P]
@@ -105,6 +106,7 @@ def lib():
                                       D, P, P, I, P, P]
         L.tfg_batch_times.argtypes = [P, P]
         L.tfg_batch_info.argtypes = [P, P, P, P, P]
+        L.tfg_batch_tuning.argtypes = [P, I, P, P, P, P]
         L.tfg_solve_native_noalloc.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P,
                                                P, I, P, D, P, P, P, P]
         L.tfg_last_error.restype = ctypes.c_char_p
@@ -172,6 +174,15 @@ def stream_copy(dst, src, nbytes, reps=10):
     """Mean time (ms) of a plain device copy of nbytes (calibration kernel)."""
     ms = ctypes.c_double()
     _check(lib().tfg_stream_copy(dst.ptr, src.ptr, int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
+
+
+def stream_bench(dst, src, nbytes, mode, reps=10):
+    """Mean time (ms) of a sweep over nbytes: mode 0 copy, 1 read-only (src;
+    per-workgroup sums into dst), 2 write-only (dst)."""
+    ms = ctypes.c_double()
+    _check(lib().tfg_stream_bench(dst.ptr, src.ptr if src is not None else None, int(nbytes),
+                                  int(mode), int(reps), ctypes.byref(ms)))
     return ms.value
 
 
@@ -400,6 +411,17 @@ class Batch:
                                    ctypes.byref(opt), ctypes.byref(rep))
         _check(rc)
         return niter, fin, rep.as_dict()
+
+    def tuning(self, precision=0):
+        """The batch autotuner's choice for a precision: chunk (iterations per
+        replayed graph), diag_classes (bit c: systems of size class c on
+        diagonals use the diagonal products), probes timed, best probe ms."""
+        c, d, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        _check(lib().tfg_batch_tuning(self.h, int(precision), ctypes.byref(c), ctypes.byref(d),
+                                      ctypes.byref(p), ctypes.byref(ms)))
+        return {"chunk": c.value, "diag_classes": d.value, "probes": p.value,
+                "best_probe_ms": ms.value}
 
     def times(self):
         ms = np.zeros(self.nsys)
