@@ -580,6 +580,8 @@ int ensure_icc(tfg_matrix* h) {
   return TFG_OK;
 }
 
+
+
 tfg::IccArgs icc_args(const tfg_matrix* h) {
   tfg::IccArgs c{};
   c.n = h->ni;

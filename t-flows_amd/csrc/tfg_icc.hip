@@ -122,103 +122,121 @@ __global__ __launch_bounds__(kBlock) void k_box_pack(IccArgs c, const double* __
 // MODE 0 form: d_inv, d from a_ii and the lower entries   Prec_Form.f90:68-78
 // MODE 1 forward: y from r                                Prec_Solve.f90:64-75
 // MODE 2 backward: z from y                               Prec_Solve.f90:77-92
-// The value the neighbours read (d, y, z) is `out`; the tiles of level
-// toff[l] .. are blockIdx.x + lo.
+// The value the neighbours read (d, y, z) is `out`.  One tile per call of
+// the pieces below, by a workgroup of kBoxThreads threads: its cells'
+// values into LDS, the three finished faces next to it, the tile's levels
+// on one wave, the results back to HBM.
 template <int MODE>
-__global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const State* st,
-                                                           const double* __restrict__ in,
-                                                           double* __restrict__ out, int lo) {
-  constexpr int TX = kBoxTX, TY = kBoxTY, TZ = kBoxTZ;
+struct BoxTile {
+  static constexpr int TX = kBoxTX, TY = kBoxTY, TZ = kBoxTZ;
   static_assert(TX * TY == 64, "one lane of the level wave per (u, v) pencil");
   static_assert(TX * TY * TZ == kBoxThreads, "one thread per cell when staging");
+  static_assert(TX * TZ >= TY * TZ && TX * TZ >= TX * TY, "the largest face is the v face");
   // LDS strides of the tile with its faces and the pencil stride of the
   // staged cell data, padded so that the lanes of one level fall on
   // different banks
-  constexpr int SW = 1, SV = TZ + 2, SU = (TY + 1) * SV + 2, P = TZ + 2;
-  __shared__ double s[(TX + 1) * SU];
-  __shared__ double cv[5][TX * TY * P];  // c0, c1, c2, a0 (r or y), d; MODE 0: d_inv in slot 4
-  if (MODE != 0 && st && st->done) return;
-  const BoxIcc& b = c.bx;
-  const double fn = st ? st->fn : 1.0;
-  const int tid = threadIdx.x;
-  const int tile = b.tiles[lo + blockIdx.x];
-  const int X = tile / (b.nty * b.ntz), Y = (tile / b.ntz) % b.nty, Z = tile % b.ntz;
-  const int u0 = X * b.tx, v0 = Y * b.ty, w0 = Z * b.tz;
-  auto row_of = [&](int u, int v, int w) {
+  static constexpr int SW = 1, SV = TZ + 2, SU = (TY + 1) * SV + 2, P = TZ + 2;
+  struct Smem {
+    double s[(TX + 1) * SU];
+    double cv[5][TX * TY * P];  // c0, c1, c2, a0 (r or y), d; MODE 0: d_inv in slot 4
+  };
+  int u0, v0, w0;
+  int su, sv, sw, sq, si;
+  bool mine;
+
+  __device__ __forceinline__ BoxTile(const BoxIcc& b, int tile) {
+    const int X = tile / (b.nty * b.ntz), Y = (tile / b.ntz) % b.nty, Z = tile % b.ntz;
+    u0 = X * b.tx;
+    v0 = Y * b.ty;
+    w0 = Z * b.tz;
+    const int tid = threadIdx.x;
+    su = tid / (TY * TZ);
+    sv = (tid / TZ) % TY;
+    sw = tid % TZ;
+    mine = in_tile(b, su, sv, sw);
+    sq = (su * TY + sv) * P + sw;
+    si = mine ? row_of(b, u0 + su, v0 + sv, w0 + sw) : 0;
+  }
+  __device__ __forceinline__ static int row_of(const BoxIcc& b, int u, int v, int w) {
     const int x = MODE == 2 ? b.nx - 1 - u : u;
     const int y = MODE == 2 ? b.ny - 1 - v : v;
     const int z = MODE == 2 ? b.nz - 1 - w : w;
     return (x * b.ny + y) * b.nz + z;
-  };
-  // finished faces at u0-1, v0-1, w0-1 (zero outside the grid)
-  for (int t = tid; t < b.ty * b.tz; t += kBoxThreads) {
-    const int lv = t / b.tz, lw = t % b.tz, v = v0 + lv, w = w0 + lw;
-    s[(lv + 1) * SV + (lw + 1) * SW] = (u0 > 0 && v < b.ny && w < b.nz) ? out[row_of(u0 - 1, v, w)] : 0.0;
   }
-  for (int t = tid; t < b.tx * b.tz; t += kBoxThreads) {
-    const int lu = t / b.tz, lw = t % b.tz, u = u0 + lu, w = w0 + lw;
-    s[(lu + 1) * SU + (lw + 1) * SW] = (v0 > 0 && u < b.nx && w < b.nz) ? out[row_of(u, v0 - 1, w)] : 0.0;
+  __device__ __forceinline__ bool in_tile(const BoxIcc& b, int lu, int lv, int lw) const {
+    return lu < b.tx && lv < b.ty && lw < b.tz && u0 + lu < b.nx && v0 + lv < b.ny &&
+           w0 + lw < b.nz;
   }
-  for (int t = tid; t < b.tx * b.ty; t += kBoxThreads) {
-    const int lu = t / b.ty, lv = t % b.ty, u = u0 + lu, v = v0 + lv;
-    s[(lu + 1) * SU + (lv + 1) * SV] = (w0 > 0 && u < b.nx && v < b.ny) ? out[row_of(u, v, w0 - 1)] : 0.0;
-  }
-  auto in_tile = [&](int lu, int lv, int lw) {
-    return lu < b.tx && lv < b.ty && lw < b.tz && u0 + lu < b.nx && v0 + lv < b.ny && w0 + lw < b.nz;
-  };
-  // stage this thread's cell (the tile's natural order, w fastest: the
-  // loads coalesce)
-  const double* const* cf = MODE == 2 ? b.up : b.lo;
-  const int su = tid / (TY * TZ), sv = (tid / TZ) % TY, sw = tid % TZ;
-  const bool mine = in_tile(su, sv, sw);
-  const int sq = (su * TY + sv) * P + sw;
-  int si = 0;
-  if (mine) {
-    si = row_of(u0 + su, v0 + sv, w0 + sw);
-    cv[0][sq] = cf[0][si];
-    cv[1][sq] = cf[1][si];
-    cv[2][sq] = cf[2][si];
+  // this thread's cell (the tile's natural order, w fastest: the loads
+  // coalesce); inputs written by earlier kernels only
+  __device__ __forceinline__ void stage_cells(const IccArgs& c, const double* __restrict__ in,
+                                              Smem& m) const {
+    const BoxIcc& b = c.bx;
+    const double* const* cf = MODE == 2 ? b.up : b.lo;
+    if (!mine) return;
+    m.cv[0][sq] = cf[0][si];
+    m.cv[1][sq] = cf[1][si];
+    m.cv[2][sq] = cf[2][si];
     if (MODE == 0) {
-      cv[3][sq] = b.dg[si];
+      m.cv[3][sq] = b.dg[si];
     } else if (MODE == 1) {
-      cv[3][sq] = in[si];
-      cv[4][sq] = c.d[si];
+      m.cv[3][sq] = in[si];
+      m.cv[4][sq] = c.d[si];
     } else {
-      cv[3][sq] = in[si] * c.dinv[si];
-      cv[4][sq] = c.d[si];
+      m.cv[3][sq] = in[si] * c.dinv[si];
+      m.cv[4][sq] = c.d[si];
     }
   }
-  __syncthreads();
+  // finished faces at u0-1, v0-1, w0-1 (zero outside the grid)
+  __device__ __forceinline__ void stage_faces(const BoxIcc& b, const double* out, Smem& m) const {
+    const int tid = threadIdx.x;
+    for (int t = tid; t < b.ty * b.tz; t += kBoxThreads) {
+      const int lv = t / b.tz, lw = t % b.tz, v = v0 + lv, w = w0 + lw;
+      m.s[(lv + 1) * SV + (lw + 1) * SW] =
+          (u0 > 0 && v < b.ny && w < b.nz) ? out[row_of(b, u0 - 1, v, w)] : 0.0;
+    }
+    for (int t = tid; t < b.tx * b.tz; t += kBoxThreads) {
+      const int lu = t / b.tz, lw = t % b.tz, u = u0 + lu, w = w0 + lw;
+      m.s[(lu + 1) * SU + (lw + 1) * SW] =
+          (v0 > 0 && u < b.nx && w < b.nz) ? out[row_of(b, u, v0 - 1, w)] : 0.0;
+    }
+    for (int t = tid; t < b.tx * b.ty; t += kBoxThreads) {
+      const int lu = t / b.ty, lv = t % b.ty, u = u0 + lu, v = v0 + lv;
+      m.s[(lu + 1) * SU + (lv + 1) * SV] =
+          (w0 > 0 && u < b.nx && v < b.ny) ? out[row_of(b, u, v, w0 - 1)] : 0.0;
+    }
+  }
   // one wave runs the levels: at level l the lane of pencil (pu, pv)
   // computes cell w = l - pu - pv, so a level needs no barrier
-  if (tid < TX * TY) {
-    const int pu = tid / TY, pv = tid % TY;
+  __device__ __forceinline__ void levels(const BoxIcc& b, double fn, Smem& m) const {
+    if (threadIdx.x >= TX * TY) return;
+    const int pu = threadIdx.x / TY, pv = threadIdx.x % TY;
     const int nlev = b.tx + b.ty + b.tz - 2;
     for (int l = 0; l < nlev; ++l) {
       const int lw = l - pu - pv;
-      if (lw >= 0 && in_tile(pu, pv, lw)) {
+      if (lw >= 0 && in_tile(b, pu, pv, lw)) {
         const int q = (pu * TY + pv) * P + lw;
         const int p = (pu + 1) * SU + (pv + 1) * SV + (lw + 1) * SW;
-        const double n0 = s[p - SU], n1 = s[p - SV], n2 = s[p - SW];
+        const double n0 = m.s[p - SU], n1 = m.s[p - SV], n2 = m.s[p - SW];
         double r;
         if (MODE == 0) {
-          double sum = cv[3][q] * fn;
-          double a = cv[0][q] * fn;
+          double sum = m.cv[3][q] * fn;
+          double a = m.cv[0][q] * fn;
           sum = sum - n0 * a * a;
-          a = cv[1][q] * fn;
+          a = m.cv[1][q] * fn;
           sum = sum - n1 * a * a;
-          a = cv[2][q] * fn;
+          a = m.cv[2][q] * fn;
           sum = sum - n2 * a * a;
-          cv[4][q] = sum;
+          m.cv[4][q] = sum;
           r = 1.0 / sum;
         } else {
-          double sum = cv[3][q];
-          sum = sum - (cv[0][q] * fn) * n0;
-          sum = sum - (cv[1][q] * fn) * n1;
-          sum = sum - (cv[2][q] * fn) * n2;
-          r = sum * cv[4][q];
+          double sum = m.cv[3][q];
+          sum = sum - (m.cv[0][q] * fn) * n0;
+          sum = sum - (m.cv[1][q] * fn) * n1;
+          sum = sum - (m.cv[2][q] * fn) * n2;
+          r = sum * m.cv[4][q];
         }
-        s[p] = r;
+        m.s[p] = r;
       }
       // the next level reads cells other lanes of this wave stored: release
       // this level's LDS stores and acquire the others' at wavefront scope
@@ -227,11 +245,31 @@ __global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const Stat
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
-  __syncthreads();
-  if (mine) {
-    out[si] = s[(su + 1) * SU + (sv + 1) * SV + (sw + 1) * SW];
-    if (MODE == 0) c.dinv[si] = cv[4][sq];
+  __device__ __forceinline__ void store(const IccArgs& c, double* out, const Smem& m) const {
+    if (!mine) return;
+    out[si] = m.s[(su + 1) * SU + (sv + 1) * SV + (sw + 1) * SW];
+    if (MODE == 0) c.dinv[si] = m.cv[4][sq];
   }
+};
+
+// One launch per tile level: the tiles of level toff[l] .. are
+// blockIdx.x + lo.
+template <int MODE>
+__global__ __launch_bounds__(kBoxThreads) void k_box_sweep(IccArgs c, const State* st,
+                                                           const double* __restrict__ in,
+                                                           double* __restrict__ out, int lo) {
+  using T = BoxTile<MODE>;
+  __shared__ typename T::Smem m;
+  if (MODE != 0 && st && st->done) return;
+  const BoxIcc& b = c.bx;
+  const double fn = st ? st->fn : 1.0;
+  const T t(b, b.tiles[lo + blockIdx.x]);
+  t.stage_faces(b, out, m);
+  t.stage_cells(c, in, m);
+  __syncthreads();
+  t.levels(b, fn, m);
+  __syncthreads();
+  t.store(c, out, m);
 }
 
 hipError_t launch_box_sweep(hipStream_t s, const IccArgs& c, int mode, const State* st,

@@ -104,3 +104,40 @@ def test_icc_fewer_iterations_than_jacobi(cases, gpu):
         out[prec] = (it, rep["solve_ms"])
         print(f"  {prec}: {it} iterations, {rep['solve_ms']:.1f} ms", flush=True)
     assert out["icc"][0] < out["jacobi"][0]
+
+
+@pytest.mark.skip(reason="the single-launch box sweeps (a persistent grid taking tiles from a "
+                         "queue, waiting on tile flags) hung the first prec_apply on the GPU twice, "
+                         "the second time after the flag write-back fix; no further cause was "
+                         "found, so the code path was taken out and the sweeps run one launch "
+                         "per tile level")
+@pytest.mark.parametrize("which", ["twophase40x33x18", "twophase17x19x21", "plane1x50x37",
+                                   "twophase64"])
+def test_single_launch_sweeps_equal_level_launches(cases, gpu, monkeypatch, which):
+    """The box sweeps in one launch (a persistent grid taking tiles from a
+    queue in level order, each waiting on its predecessors' flags) give the
+    bits of one launch per tile level, for z = M^-1 r and for an ICC-CG
+    solve; the 64^3 grid has 8x8x4 tiles, more than the grid holds at once."""
+    s = {"twophase40x33x18": lambda: cases.hex_pressure(40, 33, 18, rho_out=1000.0),
+         "twophase17x19x21": lambda: cases.hex_pressure(17, 19, 21, rho_out=1000.0),
+         "plane1x50x37": lambda: cases.hex_pressure(1, 50, 37, rho_out=100.0),
+         "twophase64": lambda: cases.hex_pressure(64, rho_out=1000.0)}[which]()
+    r = np.random.default_rng(9).standard_normal(s.n)
+    out = {}
+    for mode in ("levels", "single"):
+        monkeypatch.setenv("TFG_ICC_SWEEP", mode)
+        nat = gpu.Native(s.row, s.col, s.dia)
+        dv, dr, dz = gpu.DeviceArray(s.val), gpu.DeviceArray(r), gpu.DeviceArray(np.zeros(s.n))
+        zs = []
+        for _ in range(3):  # repeated sweeps: new epochs over the same flags
+            nat.prec_apply("icc", dv, dr, dz)
+            zs.append(dz.download())
+        assert all(np.array_equal(z, zs[0]) for z in zs)
+        x = np.zeros(s.n)
+        it, res = nat.solve_native("cg", "icc", s.val.copy(), x, s.b.copy(), 2000, 1e-9)
+        print(f"ICC {which} {mode}: {it} iterations, res {res:.3e}", flush=True)
+        out[mode] = (zs[0], it, res, x)
+        nat.close()
+    a, b = out["levels"], out["single"]
+    assert np.array_equal(a[0], b[0])
+    assert a[1] == b[1] and a[2] == b[2] and np.array_equal(a[3], b[3])
