@@ -346,9 +346,12 @@ int tfg_rhie_chow_fluxes(tfg_faces* faces, const tfg_rhie_chow* rc, double* v_fl
  *     Grid_Mod/Parallel/Exchange_Cells_Real.f90:2-129); d_x[0..ni-1] is current.
  * Both return 0 on success; anything else aborts the solve (TFG_EINVAL).
  * Called from the thread that runs the solve, with the device idle.  Every
- * rank makes the same calls in the same order.  FP64 cg / bicg with jacobi
- * or none; x's buffer cells must be current on entry, as in the reference,
- * and are refreshed at exit (Cg.f90:240). */
+ * rank makes the same calls in the same order.  FP64 cg / bicg with jacobi,
+ * icc or none; "icc" is the reference's per-rank DIC over rows 1..ni
+ * (buffer columns left out, Prec_Solve.f90:89), so it needs each row's
+ * diagonal first and its other columns ascending in the rank's own
+ * numbering.  x's buffer cells must be current on entry, as in the
+ * reference, and are refreshed at exit (Cg.f90:240). */
 #define TFG_SUM 0
 #define TFG_MIN 1
 #define TFG_MAX 2

@@ -43,9 +43,26 @@ static int g_sum_order = 0;
 
 void or_set_sum_order(int order) { g_sum_order = order; }
 
+/* Ranks of a multi-rank run on one global matrix: part[i] is the rank that
+ * owns row i (NULL: one rank).  With parts set, "icc" is the reference's
+ * per-rank DIC: each rank forms and applies it over its own rows only
+ * (Prec_Form.f90:68-78 and Prec_Solve.f90:64-92 on the rank's matrix, whose
+ * buffer columns the recurrences never read, :89), which on the global
+ * matrix means skipping every coupling between rows of different ranks.
+ * Sum order 11 is the reductions of such a run: each rank sums its rows
+ * serially (its own loop) and the rank totals are added in rank order (the
+ * allreduce); ranks must then own contiguous runs of rows.  Test
+ * infrastructure, not part of the reference. */
+static const int* g_parts = NULL;
+
+void or_set_parts(const int* part) { g_parts = part; }
+
+static int other_rank(int i, int k) { return g_parts && g_parts[k] != g_parts[i]; }
+
 /* the reductions run in a blocked (not the serial) order */
 static int reordered(void) {
-  return (g_sum_order >= 1 && g_sum_order <= 6) || g_sum_order == 9 || g_sum_order == 10;
+  return (g_sum_order >= 1 && g_sum_order <= 6) || g_sum_order == 9 || g_sum_order == 10 ||
+         (g_sum_order == 11 && g_parts);
 }
 
 /* Progress line on stderr every `every` CG iterations (0: none), so that a
@@ -65,6 +82,17 @@ static double pairwise(const double* s, int n) {
  * first); kind: 0 a*b, 1 (a*a)/(b*b), 2 a */
 static double blocked(const double* a, const double* b, int n, int kind) {
   if (n <= 0) return 0.0;
+  if (g_sum_order == 11) {                      /* per rank, then over ranks */
+    double s = 0.0;
+    for (int i = 0; i < n;) {
+      const int rank = g_parts[i];
+      double t = 0.0;
+      for (; i < n && g_parts[i] == rank; i++)
+        t = t + (kind == 0 ? a[i] * b[i] : (kind == 1 ? (a[i] * a[i]) / (b[i] * b[i]) : a[i]));
+      s = s + t;
+    }
+    return s;
+  }
   if (g_sum_order == 3) {
     double s = 0.0;
     for (int i = n - 1; i >= 0; i--)
@@ -167,6 +195,7 @@ void or_prec_form(const or_matrix* A, int ni, double* d, double* d_inv, int prec
       for (int j = A->row[i]; j < A->row[i + 1] - 1; j++) { /* a_row(i)+1 .. */
         int k = A->col[j] - 1;
         if (k > i) break;
+        if (other_rank(i, k)) continue;
         sum = sum - d[k] * A->val[j] * A->val[j];
       }
       d_inv[i] = sum;
@@ -190,6 +219,7 @@ void or_prec_solve(const or_matrix* A, int ni, const double* d,
       for (int j = A->row[i]; j < A->row[i + 1] - 1; j++) {
         int k = A->col[j] - 1;
         if (k > i) break;
+        if (other_rank(i, k)) continue;
         sum = sum - A->val[j] * x[k];
       }
       x[i] = sum * d[i];
@@ -201,7 +231,7 @@ void or_prec_solve(const or_matrix* A, int ni, const double* d,
       for (int j = A->row[i + 1] - 2; j >= A->row[i]; j--) { /* down to diag+1 */
         int k = A->col[j] - 1;
         if (k < i) break;
-        if (k < ni) sum = sum - A->val[j] * x[k];
+        if (k < ni && !other_rank(i, k)) sum = sum - A->val[j] * x[k];
       }
       x[i] = sum * d[i];
     }

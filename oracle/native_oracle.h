@@ -124,6 +124,7 @@ void or_rhie_chow_fluxes(int nc, int n_faces, const int* c1, const int* c2,
  * rounding sensitivity. */
 void or_set_sum_order(int order);
 void or_set_progress(int every);
+void or_set_parts(const int* part);
 
 /* FP32 sweeps inside FP64 refinement (the library's TFG_FP32_IR form; no
  * counterpart in the reference).  or_lap32_form: the FP32 row-excess matrix

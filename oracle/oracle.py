@@ -57,6 +57,7 @@ def lib(omp=False):
         L.or_cg_ex.argtypes = [P, P, P, I, I, P, D, P, P, I]
         L.or_set_sum_order.argtypes = [I]
         L.or_set_progress.argtypes = [I]
+        L.or_set_parts.argtypes = [P]
         L.or_lap32_form.argtypes = [P, I, D, P]
         L.or_lap32_spmv.argtypes = [P, I, P, P, P]
         L.or_ir_cg.restype = I
@@ -129,17 +130,39 @@ def spmv(A, p, omp=False):
     return q
 
 
-def prec_form(A, prec):
+class _Parts:
+    """Sets the ranks of the rows (or_set_parts) for the duration of a call;
+    parts=None leaves one rank."""
+
+    def __init__(self, L, A, parts):
+        self.L = L
+        self.parts = None if parts is None else np.ascontiguousarray(parts, dtype=np.int32)
+        if self.parts is not None:
+            assert self.parts.shape == (A.nt,)
+
+    def __enter__(self):
+        if self.parts is not None:
+            self.L.or_set_parts(self.parts.ctypes.data)
+
+    def __exit__(self, *exc):
+        self.L.or_set_parts(None)
+
+
+def prec_form(A, prec, parts=None):
+    """Prec_Form.f90:47-89; parts (rank of each row): the per-rank DIC."""
     d = np.zeros(A.nt)
     dinv = np.zeros(A.nt)
-    lib().or_prec_form(A.ref, A.ni, _p(d), _p(dinv), PREC[prec])
+    with _Parts(lib(), A, parts):
+        lib().or_prec_form(A.ref, A.ni, _p(d), _p(dinv), PREC[prec])
     return d, dinv
 
 
-def prec_solve(A, d, dinv, b, prec):
+def prec_solve(A, d, dinv, b, prec, parts=None):
+    """Prec_Solve.f90:44-103; parts as for prec_form."""
     d, dinv, b = _f64(d, A.nt), _f64(dinv, A.nt), _f64(b, A.nt)
     x = np.zeros(A.nt)
-    lib().or_prec_solve(A.ref, A.ni, _p(d), _p(dinv), _p(x), _p(b), PREC[prec])
+    with _Parts(lib(), A, parts):
+        lib().or_prec_solve(A.ref, A.ni, _p(d), _p(dinv), _p(x), _p(b), PREC[prec])
     return x
 
 
@@ -152,7 +175,7 @@ class BreakdownError(ArithmeticError):
 
 
 def solve(solver, prec, A, x, b, miter, tol, norm=None, criterion="nrms", omp=False,
-          sum_order=0, omegas=None, progress=0):
+          sum_order=0, omegas=None, progress=0, parts=None):
     """Solve_Native.f90:2-46 restated: dispatches 'cg' / 'bicg', plus the
     textbook methods 'bicg_textbook', 'cgs', 'bcgs' ('bicgstab').
 
@@ -160,7 +183,10 @@ def solve(solver, prec, A, x, b, miter, tol, norm=None, criterion="nrms", omp=Fa
     arguments.  Returns (niter, fin_res).  sum_order=1 switches every global
     reduction to a blocked-pairwise order (not the reference's): it measures
     how far rounding alone moves the answer.  progress > 0 prints a line on
-    stderr every that many CG iterations."""
+    stderr every that many CG iterations.  parts (the rank owning each row)
+    emulates a multi-rank run on the global matrix: "icc" becomes the
+    per-rank DIC, and sum_order=11 sums each rank's rows serially and adds
+    the rank totals in rank order (contiguous ranks)."""
     assert x.dtype == np.float64 and x.flags.c_contiguous and x.shape == (A.nt,)
     assert b.dtype == np.float64 and b.flags.c_contiguous and b.shape == (A.nt,)
     niter = ctypes.c_int(0)
@@ -168,10 +194,14 @@ def solve(solver, prec, A, x, b, miter, tol, norm=None, criterion="nrms", omp=Fa
     nv = ctypes.c_double(norm) if norm is not None else None
     nref = ctypes.byref(nv) if nv is not None else None
     L = lib(omp)
+    if sum_order == 11:
+        assert parts is not None, "sum order 11 sums per rank: parts needed"
     L.or_set_sum_order(int(sum_order))
     L.or_set_progress(int(progress))
     try:
-        return _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin, omegas)
+        with _Parts(L, A, parts):
+            return _solve(L, solver, prec, A, x, b, miter, tol, nref, criterion, niter, fin,
+                          omegas)
     finally:
         L.or_set_sum_order(0)
         L.or_set_progress(0)

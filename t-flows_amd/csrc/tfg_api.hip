@@ -1099,7 +1099,12 @@ int solve_ir_ns(tfg_matrix* h, int method, int prec, const double* val, const do
 
 // One rank of a multi-rank FP64 CG (Cg.f90 with Comm_Mod's calls): every
 // global scalar goes through the caller's allreduce, p's buffer cells through
-// its exchange before each product, x's at exit.
+// its exchange before each product, x's at exit.  ICC is the reference's
+// per-rank DIC: Prec_Form / Prec_Solve run over this rank's rows 1..ni only
+// (the forward recurrences stop at the first column above the row, which a
+// buffer cell always is; the backward one skips buffer columns,
+// Prec_Solve.f90:89), so M is block diagonal over the ranks and only
+// rho = (z, r) crosses them.
 int solve_comm(tfg_matrix* h, int prec, const double* val, const double* b, int miter,
                int* niter, double tol, double* fin_res, const double* norm,
                const tfg_options& o, tfg_report* rep) {
@@ -1146,6 +1151,13 @@ int solve_comm(tfg_matrix* h, int prec, const double* val, const double* b, int 
     return TFG_OK;
   };
 #define TFG_CM(expr) do { const int rc_ = (expr); if (rc_ != TFG_OK) return rc_; } while (0)
+  const bool icc = prec == tfg::PREC_ICC;
+  tfg::IccArgs ic{};
+  if (icc) {
+    const int ri = ensure_icc(h);
+    if (ri != TFG_OK) return ri;
+    ic = icc_args(h);
+  }
   HIP_TRY(hipEventRecord(h->ev_t0, S));
   // fn = 1 / (sum_a / sum_n) over all ranks                     Cg.f90:62-76
   HIP_TRY(tfg::launch_diag_sum(L, a));
@@ -1158,7 +1170,10 @@ int solve_comm(tfg_matrix* h, int prec, const double* val, const double* b, int 
     TFG_CM(set_field(&h->st->fn, 1.0 / (vals[0] / vals[1])));
   }
   // r0 = b - A x and the start sums                            Cg.f90:95-123
-  HIP_TRY(tfg::launch_prologue(L, a, prec));
+  // (ICC: the products and updates run unpreconditioned, z from the sweeps)
+  const int pk = icc ? tfg::PREC_NONE : prec;
+  HIP_TRY(tfg::launch_prologue(L, a, pk));
+  if (icc) HIP_TRY(tfg::launch_icc_form(S, ic, val, h->st));         // Cg.f90:89
   const int pro_ops[7] = {tfg::OP_SUM, tfg::OP_SUM, tfg::OP_MIN, tfg::OP_MAX,
                           tfg::OP_SUM, tfg::OP_SUM, tfg::OP_SUM};
   TFG_CM(reduce(a.pt.pro, L.grid_spmv, 7, pro_ops));
@@ -1180,22 +1195,31 @@ int solve_comm(tfg_matrix* h, int prec, const double* val, const double* b, int 
     const int upd_ops[5] = {tfg::OP_SUM, tfg::OP_MIN, tfg::OP_MAX, tfg::OP_SUM, tfg::OP_SUM};
     int k = 1;
     for (; k <= miter; ++k) {
-      HIP_TRY(tfg::launch_cm_pupdate(L, a, prec, k));                  // :145-171
+      if (icc) {
+        HIP_TRY(tfg::launch_icc_apply(S, ic, val, h->st, h->r, h->icc_z));  // :145
+        HIP_TRY(tfg::launch_icc_rho(L, a, h->icc_z));
+        TFG_CM(reduce(a.pt.ext, L.grid_vec, 1, sum1));                 // :150-156
+        if (k > 1) TFG_CM(set_field(&h->st->beta, tot[0] / rho));     // :165
+        rho = tot[0];
+      }
+      HIP_TRY(tfg::launch_cm_pupdate(L, a, prec, h->icc_z, k));        // :145-171
       TFG_CM(exchange(h->p));                                          // :176
       HIP_TRY(tfg::launch_spmv_pq(L, a, k));                           // :177-195
       TFG_CM(reduce(a.pt.spmv, L.grid_spmv, 1, sum1));                 // :197
       const double pq = tot[0];
       if (pq == 0.0 || !std::isfinite(pq)) { status = TFG_EBREAKDOWN; break; }
       TFG_CM(set_field(&h->st->alpha, rho / pq));                      // :198
-      HIP_TRY(tfg::launch_cm_update(L, a, prec));                      // :204-209
+      HIP_TRY(tfg::launch_cm_update(L, a, pk));                        // :204-209
       TFG_CM(reduce(a.pt.upd, L.grid_vec, 5, upd_ops));                // NRMS :38,53-54
       const double range = norm ? std::max(*norm - 0.0, tfg::kTiny) : std::max(tot[2] - tot[1], tfg::kTiny);
       res = o.criterion == TFG_CRIT_RELRES ? std::sqrt(tot[4]) / bnorm : std::sqrt(tot[0]) / range;
       if (!std::isfinite(res)) { status = TFG_EBREAKDOWN; break; }
       if (res < tol) break;                                            // :220
-      const double rho_new = tot[3];                                   // :150-156 of k+1
-      TFG_CM(set_field(&h->st->beta, rho_new / rho));                  // :165
-      rho = rho_new;
+      if (!icc) {
+        const double rho_new = tot[3];                                 // :150-156 of k+1
+        TFG_CM(set_field(&h->st->beta, rho_new / rho));                // :165
+        rho = rho_new;
+      }
     }
     iters = std::min(k, miter);
   }
@@ -2057,8 +2081,8 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
 
   int rc;
   if (h->allreduce) {
-    if (method != tfg::NS_NONE || pr == tfg::PREC_ICC || o.precision != TFG_FP64)
-      return fail(TFG_ENOTSUP, "several ranks: FP64 cg / bicg with jacobi or none");
+    if (method != tfg::NS_NONE || o.precision != TFG_FP64)
+      return fail(TFG_ENOTSUP, "several ranks: FP64 cg / bicg with jacobi, icc or none");
     rc = tfg::solve_comm(h, pr, dval, db, miter, niter, tol, fin_res, norm, o, rep);
     if (rc != TFG_OK && rc != TFG_EBREAKDOWN) return rc;
   } else if (o.precision == TFG_FP32_IR && method != tfg::NS_NONE) {

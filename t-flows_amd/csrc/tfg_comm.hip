@@ -26,12 +26,15 @@ __global__ __launch_bounds__(kBlock) void k_cm_local(const double* part, int pst
 
 // p = z + beta p with the global beta of State (p = z when k = 1)
 //                                                    Cg.f90:145, 158-171
+// ICC: z is this rank's DIC sweep of r (Prec_Solve.f90:64-92 over rows
+// 1..ni, buffer cells left out), passed in zv.
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_cm_pupdate(CgArgs a, int k) {
+__global__ __launch_bounds__(kBlock) void k_cm_pupdate(CgArgs a, const double* __restrict__ zv,
+                                                       int k) {
   const double beta = a.st->beta;
   const int stride = gridDim.x * kBlock;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) {
-    const double z = PREC == PREC_JACOBI ? a.r[i] * (1.0 / a.ad[i]) : a.r[i];
+    const double z = PREC == PREC_ICC ? zv[i] : (PREC == PREC_JACOBI ? a.r[i] * (1.0 / a.ad[i]) : a.r[i]);
     a.p[i] = k == 1 ? z : z + beta * a.p[i];
   }
 }
@@ -81,9 +84,15 @@ hipError_t launch_cm_local(hipStream_t s, const double* part, int pstride, int G
   return hipGetLastError();
 }
 
-hipError_t launch_cm_pupdate(const Launch& L, const CgArgs& a, int prec, int k) {
-  if (prec == PREC_JACOBI) k_cm_pupdate<PREC_JACOBI><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
-  else k_cm_pupdate<PREC_NONE><<<L.grid_vec, kBlock, 0, L.stream>>>(a, k);
+hipError_t launch_cm_pupdate(const Launch& L, const CgArgs& a, int prec, const double* z, int k) {
+  if (prec == PREC_ICC) {
+    if (!z) return hipErrorInvalidValue;
+    k_cm_pupdate<PREC_ICC><<<L.grid_vec, kBlock, 0, L.stream>>>(a, z, k);
+  } else if (prec == PREC_JACOBI) {
+    k_cm_pupdate<PREC_JACOBI><<<L.grid_vec, kBlock, 0, L.stream>>>(a, nullptr, k);
+  } else {
+    k_cm_pupdate<PREC_NONE><<<L.grid_vec, kBlock, 0, L.stream>>>(a, nullptr, k);
+  }
   return hipGetLastError();
 }
 

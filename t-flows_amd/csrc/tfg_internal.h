@@ -335,7 +335,7 @@ hipError_t launch_assemble(hipStream_t stream, const AsmArgs& g);
 // Multi-rank CG (tfg_comm.hip).
 hipError_t launch_cm_local(hipStream_t s, const double* part, int pstride, int G, int nv,
                            const int* ops, double* out);
-hipError_t launch_cm_pupdate(const Launch& L, const CgArgs& a, int prec, int k);
+hipError_t launch_cm_pupdate(const Launch& L, const CgArgs& a, int prec, const double* z, int k);
 hipError_t launch_cm_update(const Launch& L, const CgArgs& a, int prec);
 
 // Textbook methods (tfg_nonsym.hip).

@@ -6,7 +6,9 @@ reference's layout: owned cells first, then the buffer cells its rows read
 combine the two ranks' values in rank order (allreduce) and copy buffer
 cells between the handles' device vectors (exchange).  The two ranks must
 agree with each other and, together, with the oracle's one-domain solve at
-the FP64 parity bar.
+the FP64 parity bar.  ICC is per rank in the reference (each rank's DIC over
+its own rows), so its oracle is the global solve with the rows' ranks given:
+per-rank DIC, and the reductions summed per rank, then over the ranks.
 """
 import ctypes
 import threading
@@ -15,43 +17,9 @@ import numpy as np
 import pytest
 
 import parity_bar
+import ranks as rank_split
 
 pytestmark = pytest.mark.gpu
-
-
-def _split(s, nx):
-    """Two ranks of a hex system ordered x slowest: (ni, row, col, dia, val,
-    b, glob) per rank, glob = global ids of the local cells."""
-    n = s.n
-    n0 = n * (nx // 2) // nx
-    row0 = s.row - 1
-    ranks = []
-    for r, (lo, hi) in enumerate(((0, n0), (n0, n))):
-        cols = []
-        for i in range(lo, hi):
-            cols.extend(s.col[row0[i]:row0[i + 1]] - 1)
-        cols = np.array(cols)
-        buf = np.unique(cols[(cols < lo) | (cols >= hi)])
-        glob = np.concatenate([np.arange(lo, hi), buf])
-        loc = {g: k for k, g in enumerate(glob)}
-        ni, nt = hi - lo, len(glob)
-        row, col, val = [1], [], []
-        for i in range(lo, hi):
-            for j in range(row0[i], row0[i + 1]):
-                col.append(loc[s.col[j] - 1] + 1)
-                val.append(s.val[j])
-            row.append(len(col) + 1)
-        for k in range(ni, nt):          # buffer rows: never solved
-            col.append(k + 1)
-            val.append(1.0)
-            row.append(len(col) + 1)
-        row = np.array(row, dtype=np.int32)
-        col = np.array(col, dtype=np.int32)
-        b = np.zeros(nt)
-        b[:ni] = s.b[lo:hi]
-        ranks.append(dict(ni=ni, nt=nt, row=row, col=col, dia=row[:-1].copy(),
-                          val=np.array(val), b=b, glob=glob))
-    return ranks
 
 
 class _Comm:
@@ -88,7 +56,7 @@ class _Comm:
                                     buf.nbytes) == 0
 
 
-def _run(gpu, ranks, prec, miter, tol, norm, criterion):
+def _run(gpu, ranks, prec, miter, tol, norm, criterion, solver="cg"):
     comm = _Comm(gpu, ranks)
     out = [None, None]
 
@@ -100,8 +68,8 @@ def _run(gpu, ranks, prec, miter, tol, norm, criterion):
         opt = gpu.Options()
         opt.criterion = criterion
         try:
-            it, res = nat.solve_native("cg", prec, me["val"].copy(), x, me["b"].copy(), miter, tol,
-                                       norm=norm, options=opt)
+            it, res = nat.solve_native(solver, prec, me["val"].copy(), x, me["b"].copy(), miter,
+                                       tol, norm=norm, options=opt)
             out[r] = (it, res, x)
         except Exception as e:   # noqa: BLE001 - reported by the caller
             out[r] = e
@@ -120,7 +88,7 @@ def _run(gpu, ranks, prec, miter, tol, norm, criterion):
 def test_two_ranks_match_one_domain(cases, oracle, gpu, prec, criterion, norm):
     nx = 12
     s = cases.hex_pressure(nx, 10, 8, rho_out=100.0)
-    ranks = _split(s, nx)
+    ranks = rank_split.split(s, nx)
     print(f"two ranks: ni {ranks[0]['ni']}/{ranks[1]['ni']}, nt {ranks[0]['nt']}/{ranks[1]['nt']}",
           flush=True)
     out = _run(gpu, ranks, prec, 2000, 1e-8, norm, criterion)
@@ -150,9 +118,53 @@ def test_two_ranks_match_one_domain(cases, oracle, gpu, prec, criterion, norm):
         assert np.array_equal(xr[rk["ni"]:], x[rk["glob"][rk["ni"]:]])
 
 
+@pytest.mark.parametrize("solver,criterion,norm", [("cg", 0, None), ("cg", 1, None),
+                                                   ("bicg", 0, 1.0)])
+def test_two_ranks_icc_match_per_rank_oracle(cases, oracle, gpu, solver, criterion, norm):
+    """Prec_Form.f90:68-78 / Prec_Solve.f90:64-92 on each rank's own rows."""
+    nx = 12
+    s = cases.hex_pressure(nx, 10, 8, rho_out=100.0)
+    ranks = rank_split.split(s, nx, local_order=True)
+    parts = rank_split.parts(ranks)
+    print(f"two ranks icc {solver}: ni {ranks[0]['ni']}/{ranks[1]['ni']}, "
+          f"nt {ranks[0]['nt']}/{ranks[1]['nt']}", flush=True)
+    out = _run(gpu, ranks, "icc", 2000, 1e-8, norm, criterion, solver)
+    for o in out:
+        assert not isinstance(o, Exception), o
+    (it0, res0, x0), (it1, res1, x1) = out
+    assert it0 == it1 and res0 == res1
+    x = np.zeros(s.n)
+    for rk, xr in zip(ranks, (x0, x1)):
+        x[rk["glob"][:rk["ni"]]] = xr[:rk["ni"]]
+    cname = "nrms" if criterion == 0 else "relres"
+
+    def solve(so, per_rank=True):
+        A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
+        xo = np.zeros(s.n)
+        it, r = oracle.solve(solver, "icc", A, xo, s.b.copy(), 2000, 1e-8, norm=norm,
+                             criterion=cname, sum_order=so, parts=parts if per_rank else None)
+        return it, r, xo
+
+    it_o, res_o, x_o = solve(11)     # each rank's serial sums, added in rank order
+    env, spread = parity_bar.envelope(lambda so: solve(so)[::2], it_o, x_o)
+    print(f"  ranks niter {it0} res {res0:.3e}; oracle niter {it_o} res {res_o:.3e}", flush=True)
+    err = parity_bar.check(f"two ranks icc {solver} {cname} norm={norm}", it0, x, it_o, x_o, env,
+                           spread)
+    assert res0 < 1e-8
+    # the one-domain DIC is another preconditioner: its run lies outside the bar
+    it_1, _, x_1 = solve(0, per_rank=False)
+    far = parity_bar.rel(x, x_1)
+    print(f"  one-domain icc: niter {it_1}, x differs by {far:.2e} (per-rank err {err:.2e})",
+          flush=True)
+    assert (abs(it_1 - it0) > max(2, parity_bar.FACTOR * spread)
+            or far > max(parity_bar.X_TOL, parity_bar.FACTOR * env))
+    for rk, xr in zip(ranks, (x0, x1)):
+        assert np.array_equal(xr[rk["ni"]:], x[rk["glob"][rk["ni"]:]])
+
+
 def test_buffer_cells_need_hooks(cases, gpu):
     s = cases.hex_pressure(6, 4, 4)
-    rk = _split(s, 6)[0]
+    rk = rank_split.split(s, 6)[0]
     nat = gpu.Native(rk["row"], rk["col"], rk["dia"], ni=rk["ni"])
     with pytest.raises(gpu.TFlowsGpuError) as e:
         nat.solve_native("cg", "jacobi", rk["val"].copy(), np.zeros(rk["nt"]), rk["b"].copy(), 10,

@@ -14,7 +14,10 @@ operand, wrong index, wrong step length) fails:
                    dense projection; exact-arithmetic iteration counts on
                    matrices with few distinct eigenvalues
   prec icc         the DIC identity diag((D+L) D^-1 (D+U)) = diag(A) and the
-                   dense solve with that M
+                   dense solve with that M; per rank (the rows' ranks given),
+                   each rank's own DIC on its matrix with buffer cells, bit
+                   for bit, and the identity on A without cross-rank terms
+  sum order 11     closed forms of fn and NRMS summed per rank
   bicg             bit-identical to cg (Bicg.f90 runs the same recurrence)
   conventions      early exits, iteration cap, optional norm (Cg.f90)
 plus the committed fixtures of tests/golden (made by tests/golden/make_golden.py
@@ -208,6 +211,78 @@ def test_icc_dic_identity(cases, oracle):
     rhs = np.cos(np.arange(s.n))
     z = oracle.prec_solve(M, d, dinv, rhs, "icc")
     assert np.allclose(Mm @ z, rhs, atol=1e-12 * np.abs(rhs).max())
+
+
+def test_icc_per_rank_is_each_ranks_own_dic(cases, oracle):
+    """With the rows' ranks given, "icc" on the global matrix is the DIC each
+    rank forms on its own matrix (ni < nt, buffer columns never read by the
+    recurrences, Prec_Solve.f90:89), to the bit; the preconditioner is the
+    DIC of A without the couplings between ranks (block diagonal), checked
+    by the DIC identity on that matrix."""
+    import ranks as rank_split
+    nx = 6
+    s = cases.hex_pressure(nx, 5, 4, rho_out=30.0)
+    rks = rank_split.split(s, nx, local_order=True)
+    parts = rank_split.parts(rks)
+    M = _mat(oracle, s)
+    d, dinv = oracle.prec_form(M, "icc", parts=parts)
+    rhs = np.cos(np.arange(s.n) * 0.7) + 0.1
+    z = oracle.prec_solve(M, d, dinv, rhs, "icc", parts=parts)
+    for rk in rks:
+        ni, g = rk["ni"], rk["glob"]
+        Ml = oracle.Matrix(rk["row"], rk["col"], rk["dia"], rk["val"], ni=ni)
+        dl, dinvl = oracle.prec_form(Ml, "icc")
+        assert np.array_equal(dl[:ni], d[g[:ni]]) and np.array_equal(dinvl[:ni], dinv[g[:ni]])
+        rl = rhs[g].copy()
+        rl[ni:] = 1e300                   # buffer cells: never read
+        zl = oracle.prec_solve(Ml, dl, dinvl, rl, "icc")
+        assert np.array_equal(zl[:ni], z[g[:ni]])
+    # the DIC identity on A without the couplings between ranks
+    A = s.scipy().toarray()
+    same = parts[:, None] == parts[None, :]
+    Ab = np.where(same, A, 0.0)
+    L = np.tril(Ab, -1)
+    P = np.diag(dinv)
+    Mm = (P + L) @ np.diag(1.0 / dinv) @ (P + L.T)
+    assert np.allclose(np.diag(Mm), np.diag(A), rtol=1e-13)
+    assert np.all(Mm[~same] == 0.0)
+    mask = Ab != 0
+    assert np.allclose(Mm[mask], Ab[mask], rtol=1e-12, atol=1e-15)
+    assert np.allclose(Mm @ z, rhs, atol=1e-12 * np.abs(rhs).max())
+    # one rank: the ordinary DIC, to the bit
+    d1, dinv1 = oracle.prec_form(M, "icc", parts=np.zeros(s.n, dtype=np.int32))
+    d0, dinv0 = oracle.prec_form(M, "icc")
+    assert np.array_equal(d1, d0) and np.array_equal(dinv1, dinv0)
+    assert not np.array_equal(d, d0)
+
+
+def test_sum_order_per_rank(cases, oracle):
+    """Sum order 11 sums each rank's rows serially and adds the rank totals
+    (a two-rank allreduce): checked through fn (Cg.f90:62-76) and the first
+    NRMS of a solve that stops before iterating, both closed forms; with
+    one rank it is the reference's serial order, to the bit."""
+    s = cases.hex_pressure(7, 4, 3, rho_out=10.0)
+    n0 = 40
+    parts = (np.arange(s.n) >= n0).astype(np.int32)
+    dg = s.val[s.dia - 1]
+
+    def serial(v):
+        t = 0.0
+        for e in v.tolist():
+            t = t + e
+        return t
+
+    def first_res(order, p):
+        x = np.zeros(s.n)
+        _, res = oracle.solve("cg", "none", _mat(oracle, s), x, s.b.copy(), 0, 1e-30, norm=1.0,
+                              sum_order=order, parts=p)
+        return res
+
+    fn = 1.0 / ((serial(dg[:n0]) + serial(dg[n0:])) / s.n)
+    r, dd = s.b * fn, dg * fn
+    t = (r * r) / (dd * dd)
+    assert first_res(11, parts) == np.sqrt(serial(t[:n0]) + serial(t[n0:]))
+    assert first_res(11, np.zeros(s.n, dtype=np.int32)) == first_res(0, None)
 
 
 def test_jacobi_and_none_prec(cases, oracle):
