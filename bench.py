@@ -649,7 +649,7 @@ def bench_assembly(args):
         log(f"bench assembly: {N}^3, {f.n_faces} faces ({order} order), generated in "
             f"{time.perf_counter() - t0:.1f} s")
         nat = G.Native(f.row, f.col, f.dia)
-        faces = G.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+        faces = G.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_region)
         d = {k: G.DeviceArray(getattr(f, k)) for k in ("fc", "fw", "vm", "vflux")}
         dval, db = G.DeviceArray(np.zeros(len(f.col))), G.DeviceArray(np.zeros(f.n))
         faces.assemble(d["fc"], d["fw"], d["vm"], d["vflux"], dval, db, singular=True)
@@ -712,7 +712,7 @@ def bench_momentum(args):
         log(f"bench momentum: {N}^3, {f.n_faces} faces ({order} order), generated in "
             f"{time.perf_counter() - t0:.1f} s")
         nat = G.Native(f.row, f.col, f.dia)
-        faces = G.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+        faces = G.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_region)
         m = G.Momentum(mf, **kw)
         dval, db, dvm = (G.DeviceArray(np.zeros(k)) for k in (len(f.col), f.n, f.n))
         faces.momentum(m, dval, db, dvm)

@@ -227,16 +227,24 @@ int tfg_matrix_reorder(tfg_matrix* h, int order);
  * the device, so that a pressure solve copies neither to the GPU: the
  * matrix lines of the Rhie and Chow face loop (Rhie_And_Chow.f90:184-205),
  * the right-hand side from the volume fluxes and the boundary diagonal terms
- * (Compute_Pressure.f90:121-122, 147-169), b = 0 in pressure regions
- * (:173-178) and, when `singular`, Set_Singular's a_ii * (1 + 1e-6)
+ * (Compute_Pressure.f90:121-122, 147-169), the boundary regions' b
+ * (:171-189) and, when `singular`, Set_Singular's a_ii * (1 + 1e-6)
  * (Solver_Mod/Set_Singular.f90:28-31).  Bit-identical to the serial loops.
  * Structure once per grid: c1, c2 = Grid % faces_c (1-based; c2 <= 0 on the
  * boundary), pos = A % pos (2 x n_faces, 1-based, pos(1,s) then pos(2,s);
  * ignored on the boundary), per boundary face diag_bc = 1 for a pressure or
- * ambient condition, b_zero = 1 in a pressure region. */
+ * ambient condition (Bnd_Cond_Type of c2) and b_region = the type of the
+ * face's region: TFG_REGION_PRESSURE (b(c1) = 0, :173-178),
+ * TFG_REGION_AMBIENT (b(c1) = fc(s) v_m(c1) pp % n(c2), :180-187) or
+ * TFG_REGION_NONE.  The regions' assignments of b follow face order, which
+ * is the reference's region loop when each region's faces form one range
+ * and the ranges ascend with the region number (Faces_In_Region). */
+#define TFG_REGION_NONE 0
+#define TFG_REGION_PRESSURE 1
+#define TFG_REGION_AMBIENT 2
 typedef struct tfg_faces tfg_faces;
 int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2,
-                     const int* pos, const int* diag_bc, const int* b_zero,
+                     const int* pos, const int* diag_bc, const int* b_region,
                      tfg_faces** faces);
 /* The faces handle uses its matrix handle: assemble only while both exist.
  * Destroying it needs only itself, before or after the matrix handle. */
@@ -248,6 +256,19 @@ int tfg_faces_destroy(tfg_faces* faces);
 int tfg_pressure_assemble(tfg_faces* faces, const double* fc, const double* fw,
                           const double* vm, const double* vflux, double* val,
                           double* b, int singular);
+/* The same with pp = pp % n (device, pointing at cell 1; boundary cell c < 0
+ * at pp[c - 1], Fortran's (-n_bnd:n) array passed as c_loc(pp(1))), which
+ * ambient regions read; tfg_pressure_assemble (pp = NULL) returns
+ * TFG_EINVAL when the faces have ambient regions. */
+int tfg_pressure_assemble_ex(tfg_faces* faces, const double* fc, const double* fw,
+                             const double* vm, const double* vflux, const double* pp,
+                             double* val, double* b, int singular);
+/* vel_max = max(MICRO, max over cells 1..nt of sqrt(u^2 + v^2 + w^2)), the
+ * norm of the momentum solves before its Global % Max_Real
+ * (Compute_Momentum.f90:201-204), from device u, v, w (nt each); the result
+ * to host memory (waits for the handle's stream). */
+int tfg_vel_max(tfg_matrix* h, const double* u, const double* v, const double* w,
+                double* vel_max);
 
 /* ---- momentum assembly on the device ------------------------------------
  * Matrix, right-hand side and M % v_m of one velocity component from face
@@ -259,7 +280,7 @@ int tfg_pressure_assemble(tfg_faces* faces, const double* fc, const double* fw,
  * diffusion, inertia, global pressure drop, cell forces, the pressure
  * gradient, optional surface tension, v_m = vol / a_ii and under-
  * relaxation.  Bit-identical to the serial loops.  Not included: user
- * forces (User_Mod_Force), PISO's explicit part, vel_max.
+ * forces (User_Mod_Force) and PISO's explicit part; vel_max is tfg_vel_max.
  * Needs the faces handle of tfg_faces_create (its pos, dia and c2 < 0 for
  * boundary cells; each boundary cell on one face, each matrix entry set by
  * one face).  All pointers are device pointers.  Cell arrays point at cell

@@ -654,17 +654,22 @@ done:
  * (Compute_Pressure.f90:121-122), the matrix lines of the Rhie and Chow face
  * loop over internal faces (Rhie_And_Chow.f90:184-205), the right-hand side
  * from the volume fluxes and the diagonal terms of boundary faces with a
- * pressure / ambient condition (Compute_Pressure.f90:147-169), b = 0 in
- * pressure regions (:173-178), and Set_Singular's a_ii * (1 + MICRO)
+ * pressure / ambient condition (Compute_Pressure.f90:147-169), the boundary
+ * regions' b (:171-189: b = 0 in pressure regions, b = fc v_m(c1) pp(c2) in
+ * ambient ones, region by region), and Set_Singular's a_ii * (1 + MICRO)
  * (Solver_Mod/Set_Singular.f90:28-31) when `singular`.  Serial, in face order.
  * c1, c2: 1-based cells (c2 <= 0: boundary face); pos: 2 x n_faces 1-based
- * CSR positions of (c1,c2) and (c2,c1); diag_bc / b_zero: per-face flags.
+ * CSR positions of (c1,c2) and (c2,c1); diag_bc: per-face flag; b_region:
+ * the type of the face's region (0 none, 1 pressure, 2 ambient); region: the
+ * region number of each face (1.., 0 for internal faces): Faces_In_Region
+ * are the faces of a number in increasing order; pp: pp % n at cell 1
+ * (boundary cell c at pp[c - 1]), read by ambient regions only.
  * ------------------------------------------------------------------------ */
 void or_pressure_assemble(int nc, int nnz, int n_faces, const int* c1, const int* c2,
                           const int* pos, const int* dia, const int* diag_bc,
-                          const int* b_zero, const double* fc, const double* fw,
-                          const double* vm, const double* vflux, double* val, double* b,
-                          int singular) {
+                          const int* b_region, const int* region, const double* fc,
+                          const double* fw, const double* vm, const double* vflux,
+                          const double* pp, double* val, double* b, int singular) {
   for (int j = 0; j < nnz; j++) val[j] = 0.0;
   for (int i = 0; i < nc; i++) b[i] = 0.0;
   for (int s = 0; s < n_faces; s++) {           /* Rhie_And_Chow.f90:184-205 */
@@ -690,10 +695,32 @@ void or_pressure_assemble(int nc, int nnz, int n_faces, const int* c1, const int
       }
     }
   }
-  for (int s = 0; s < n_faces; s++)             /* :173-178, pressure regions */
-    if (b_zero[s]) b[c1[s] - 1] = 0.0;
+  int nreg = 0;                                 /* :171-189, Boundary_Regions() */
+  for (int s = 0; s < n_faces; s++) nreg = region[s] > nreg ? region[s] : nreg;
+  for (int reg = 1; reg <= nreg; reg++)
+    for (int s = 0; s < n_faces; s++) {         /* Faces_In_Region(reg) */
+      if (region[s] != reg) continue;
+      const int a = c1[s] - 1;
+      if (b_region[s] == 1) {                   /* :173-178, pressure */
+        b[a] = 0.0;
+      } else if (b_region[s] == 2) {            /* :180-187, ambient */
+        const double a12 = fc[s] * vm[a];
+        b[a] = a12 * pp[c2[s] - 1];
+      }
+    }
   if (singular)                                 /* Set_Singular.f90:28-31 */
     for (int i = 0; i < nc; i++) val[dia[i] - 1] = val[dia[i] - 1] * (1.0 + OR_MICRO);
+}
+
+/* Compute_Momentum.f90:201-204: vel_max = max(MICRO, max_c |u_c|) over
+ * cells 1..nc (before the reference's Global % Max_Real). */
+double or_vel_max(int nc, const double* u, const double* v, const double* w) {
+  double vel_max = OR_MICRO;
+  for (int c = 0; c < nc; c++) {
+    const double m = sqrt(u[c] * u[c] + v[c] * v[c] + w[c] * w[c]);
+    vel_max = m > vel_max ? m : vel_max;
+  }
+  return vel_max;
 }
 
 /* ------------------------------------------------------------------------

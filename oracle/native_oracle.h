@@ -68,9 +68,10 @@ int or_transpose_positions(const or_matrix* A, int* tpos);
  * order; see native_oracle.c. */
 void or_pressure_assemble(int nc, int nnz, int n_faces, const int* c1, const int* c2,
                           const int* pos, const int* dia, const int* diag_bc,
-                          const int* b_zero, const double* fc, const double* fw,
-                          const double* vm, const double* vflux, double* val, double* b,
-                          int singular);
+                          const int* b_region, const int* region, const double* fc,
+                          const double* fw, const double* vm, const double* vflux,
+                          const double* pp, double* val, double* b, int singular);
+double or_vel_max(int nc, const double* u, const double* v, const double* w);
 
 /* Momentum assembly of one velocity component from face data
  * (Compute_Momentum.f90:119-366 with Numerics_Mod Advection_Term, Min_Max,

@@ -66,7 +66,9 @@ def lib(omp=False):
         L.or_krylov.argtypes = [P, P, P, I, I, I, P, D, P, P, I, P]
         L.or_transpose_positions.restype = I
         L.or_transpose_positions.argtypes = [P, P]
-        L.or_pressure_assemble.argtypes = [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I]
+        L.or_pressure_assemble.argtypes = [I, I, I] + [P] * 14 + [I]
+        L.or_vel_max.restype = D
+        L.or_vel_max.argtypes = [I, P, P, P]
         L.or_momentum_assemble.argtypes = [I, I, I, P, P, P, P, P, P, P, P]
         L.or_adv_face.restype = D
         L.or_adv_face.argtypes = [I, D, D, P, P, P, P, P, D]
@@ -290,11 +292,19 @@ def pressure_assemble(f, singular=False):
     with the fields of tflows_cases.Faces; returns (val, b)."""
     val = np.zeros(len(f.col))
     b = np.zeros(len(f.dia))
-    arrs = [np.ascontiguousarray(a) for a in (f.c1, f.c2, f.pos, f.dia, f.diag_bc, f.b_zero,
-                                               f.fc, f.fw, f.vm, f.vflux)]
+    arrs = [np.ascontiguousarray(a) for a in (f.c1, f.c2, f.pos, f.dia, f.diag_bc, f.b_region,
+                                               f.region, f.fc, f.fw, f.vm, f.vflux)]
+    pp = np.ascontiguousarray(f.pp, dtype=np.float64) if f.pp is not None else None
+    pp_cell1 = pp.ctypes.data + 8 * (f.n_bnd + 1) if pp is not None else None
     lib().or_pressure_assemble(len(f.dia), len(f.col), len(f.c1), *[_p(a) for a in arrs],
-                               _p(val), _p(b), 1 if singular else 0)
+                               pp_cell1, _p(val), _p(b), 1 if singular else 0)
     return val, b
+
+
+def vel_max(u, v, w):
+    """Compute_Momentum.f90:201-204 (before Global % Max_Real)."""
+    u, v, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w))
+    return lib().or_vel_max(len(u), _p(u), _p(v), _p(w))
 
 
 # advection schemes (Numerics_Mod/Advection_Scheme_Code.f90 names) and time

@@ -191,6 +191,7 @@ struct tfg_matrix {
   tfg_exchange_fn exchange = nullptr;
   void* comm_ctx = nullptr;
   double* loc = nullptr;   // device: this rank's totals of a reduction
+  double* vmax = nullptr;  // device: tfg_vel_max's partials and result
   // offset-coded columns (structured grids): col = row + tab[code]
   uint8_t* code = nullptr;
   int* tab = nullptr;
@@ -386,7 +387,7 @@ void free_all(tfg_matrix* h) {
                  h->ns32[3], h->ns32[4], h->valt32, h->rt, h->nv[0], h->nv[1], h->nv[2],
                  h->nv[3], h->valt, h->tpos, h->code, h->tab, h->pat, h->poff, h->plen,
                  h->pslot, h->sym, h->sym_flag, h->sorder,
-                 h->loc, h->icc_rows[0],
+                 h->loc, h->vmax, h->icc_rows[0],
                  h->icc_rows[1], h->icc_d, h->icc_dinv, h->icc_y, h->icc_z,
                  (void*)h->bx.tiles, h->bx.dg, h->bx.lo[0], h->bx.lo[1],
                  h->bx.lo[2], h->bx.up[0], h->bx.up[1], h->bx.up[2]};
@@ -1259,6 +1260,7 @@ struct tfg_faces {
   tfg_matrix* h = nullptr;
   int device = 0;            // of h, kept so that destroying needs no h
   int n_faces = 0;
+  int n_ambient = 0;         // boundary faces of ambient regions
   int *cf_off = nullptr, *cf_face = nullptr, *ent_face = nullptr, *c1 = nullptr, *c2 = nullptr;
   uint8_t* flags = nullptr;
   int* c2b = nullptr;        // c2 as given (1-based, boundary cells < 0)
@@ -1283,8 +1285,8 @@ static void faces_free(tfg_faces* f) {
 }
 
 int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2, const int* pos,
-                     const int* diag_bc, const int* b_zero, tfg_faces** faces) {
-  if (!h || !faces || !c1 || !c2 || !pos || !diag_bc || !b_zero || n_faces < 0)
+                     const int* diag_bc, const int* b_region, tfg_faces** faces) {
+  if (!h || !faces || !c1 || !c2 || !pos || !diag_bc || !b_region || n_faces < 0)
     return fail(TFG_EINVAL, "null argument");
   *faces = nullptr;
   const int n = h->nt;
@@ -1301,12 +1303,20 @@ int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2, c
     if (c2[s] < 0) max_bnd = std::max(max_bnd, -c2[s]);
   std::vector<char> bnd_seen((size_t)max_bnd + 1, 0);
   bool shared_bnd = false;
+  int n_ambient = 0;
   for (int s = 0; s < n_faces; ++s) {
     const int p = c1[s] - 1, q = c2[s] > 0 ? c2[s] - 1 : -1;
     if (p < 0 || p >= n || q >= n) return fail(TFG_EINVAL, "face %d: cells out of range", s + 1);
+    if (b_region[s] < TFG_REGION_NONE || b_region[s] > TFG_REGION_AMBIENT)
+      return fail(TFG_EINVAL, "face %d: region code %d", s + 1, b_region[s]);
     a[s] = p;
     b[s] = q;
-    fl[s] = (uint8_t)(q < 0 ? ((diag_bc[s] ? 1 : 0) | (b_zero[s] ? 2 : 0)) : 0);
+    uint8_t code = 0;
+    if (q < 0 && c2[s] < 0) {   // a boundary cell: its region's assignment of b(c1)
+      code = b_region[s] == TFG_REGION_PRESSURE ? 2 : (b_region[s] == TFG_REGION_AMBIENT ? 4 : 0);
+      if (code == 4) ++n_ambient;
+    }
+    fl[s] = (uint8_t)(q < 0 ? ((diag_bc[s] ? 1 : 0) | code) : 0);
     cnt[p + 1]++;
     if (q >= 0) {
       cnt[q + 1]++;
@@ -1345,6 +1355,7 @@ int tfg_faces_create(tfg_matrix* h, int n_faces, const int* c1, const int* c2, c
   up((void**)&f->flags, fl.data(), fl.size());
   up((void**)&f->c2b, c2, sizeof(int) * (size_t)n_faces);
   f->max_bnd = max_bnd;
+  f->n_ambient = n_ambient;
   f->shared_entry = shared_entry;
   f->shared_bnd = shared_bnd;
   if (e != hipSuccess) {
@@ -1474,7 +1485,15 @@ int tfg_faces_destroy(tfg_faces* f) {
 
 int tfg_pressure_assemble(tfg_faces* f, const double* fc, const double* fw, const double* vm,
                           const double* vflux, double* val, double* b, int singular) {
+  return tfg_pressure_assemble_ex(f, fc, fw, vm, vflux, nullptr, val, b, singular);
+}
+
+int tfg_pressure_assemble_ex(tfg_faces* f, const double* fc, const double* fw, const double* vm,
+                             const double* vflux, const double* pp, double* val, double* b,
+                             int singular) {
   if (!f || !fc || !fw || !vm || !vflux || !val || !b) return fail(TFG_EINVAL, "null argument");
+  if (f->n_ambient > 0 && !pp)
+    return fail(TFG_EINVAL, "%d faces of ambient regions need pp % n", f->n_ambient);
   tfg_matrix* h = f->h;
   HIP_TRY(hipSetDevice(h->device));
   tfg::AsmArgs g{};
@@ -1486,6 +1505,8 @@ int tfg_pressure_assemble(tfg_faces* f, const double* fc, const double* fw, cons
   g.c1 = f->c1;
   g.c2 = f->c2;
   g.flags = f->flags;
+  g.c2b = f->c2b;
+  g.pp = pp;
   g.fc = fc;
   g.fw = fw;
   g.vm = vm;
@@ -1494,6 +1515,19 @@ int tfg_pressure_assemble(tfg_faces* f, const double* fc, const double* fw, cons
   g.b = b;
   g.singular = singular ? 1 : 0;
   HIP_TRY(tfg::launch_assemble(h->stream, g));
+  return TFG_OK;
+}
+
+int tfg_vel_max(tfg_matrix* h, const double* u, const double* v, const double* w,
+                double* vel_max) {
+  if (!h || !u || !v || !w || !vel_max) return fail(TFG_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  const int G = std::max(1, std::min((h->nt + tfg::kBlock - 1) / tfg::kBlock, 1024));
+  if (!h->vmax) HIP_TRY(hipMalloc(&h->vmax, sizeof(double) * 1025));
+  HIP_TRY(tfg::launch_vel_max(h->stream, h->nt, u, v, w, h->vmax, G, h->vmax + 1024));
+  HIP_TRY(hipMemcpyAsync(vel_max, h->vmax + 1024, sizeof(double), hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return TFG_OK;
 }
 

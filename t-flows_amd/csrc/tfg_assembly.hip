@@ -8,13 +8,18 @@
 //   all faces s (in order): b(c1) -= flux(s), b(c2) += flux(s) (internal);
 //     boundary faces with a pressure / ambient condition:
 //     val(dia(c1)) += fc(s) v_m(c1)             Compute_Pressure.f90:147-169
-//   b(c1) = 0 in pressure regions                Compute_Pressure.f90:173-178
+//   boundary regions: b(c1) = 0 in pressure regions, b(c1) = fc(s) v_m(c1)
+//     pp(c2) in ambient regions, assigned region by region and face by
+//     face                                        Compute_Pressure.f90:173-188
 //   optionally a_ii *= (1 + MICRO)              Solver_Mod/Set_Singular.f90:28-31
 // The face loop scatters into the diagonal and b, so it is turned around:
 // one thread per row walks, in increasing face order, the faces of its cell
 // (a list built once per structure) and forms its diagonal and b entry as
 // the serial loop would; each off-diagonal entry takes -a12 of the last face
-// that sets it.  Every value is bit-identical to the serial assembly.
+// that sets it.  The region assignments of b follow face order, which is the
+// region loop's order when each region's faces are one range and the ranges
+// ascend with the region number (Faces_In_Region, as T-Flows numbers its
+// boundary faces).  Every value is bit-identical to the serial assembly.
 #include "tflows_gpu.h"
 #include "tfg_spmv.h"
 
@@ -37,8 +42,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
     const int s = g.cf_face[t] >> 1;
     if (g.c2[s] >= 0) diag = diag + face_a12(g, s);
   }
-  double bc = 0.0;
-  bool zero = false;
+  double bc = 0.0, bset = 0.0;
+  bool set = false;
   for (int t = fs0; t < fs1; ++t) {
     const int e = g.cf_face[t], s = e >> 1;
     const double flux = g.vflux[s];
@@ -46,8 +51,16 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
       bc = (e & 1) ? bc + flux : bc - flux;
     } else {
       bc = bc - flux;
-      if (g.flags[s] & 1) diag = diag + g.fc[s] * g.vm[c];
-      if (g.flags[s] & 2) zero = true;
+      const int fl = g.flags[s];
+      if (fl & 1) diag = diag + g.fc[s] * g.vm[c];
+      if (fl & 2) {
+        bset = 0.0;
+        set = true;
+      } else if (fl & 4) {
+        const double a12 = g.fc[s] * g.vm[c];
+        bset = a12 * g.pp[g.c2b[s] - 1];
+        set = true;
+      }
     }
   }
   if (g.singular) diag = diag * (1.0 + 1.0e-6);
@@ -57,11 +70,42 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs g) {
     const int s = g.ent_face[j];
     g.val[j] = s < 0 ? 0.0 : -face_a12(g, s);
   }
-  g.b[c] = zero ? 0.0 : bc;
+  g.b[c] = set ? bset : bc;
 }
 
 hipError_t launch_assemble(hipStream_t stream, const AsmArgs& g) {
   k_assemble<<<(g.n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(g);
+  return hipGetLastError();
+}
+
+// vel_max = max(MICRO, max_c sqrt(u_c^2 + v_c^2 + w_c^2))  Compute_Momentum.f90:201-204
+// (a maximum: the same bits in any order).  One workgroup per chunk of
+// cells, the chunk maxima combined by the second kernel.
+__global__ __launch_bounds__(kBlock) void k_vel_max(int n, const double* __restrict__ u,
+                                                    const double* __restrict__ v,
+                                                    const double* __restrict__ w,
+                                                    double* __restrict__ part) {
+  __shared__ double s_tmp[4], s_out[1];
+  double m[1] = {1.0e-6};
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    m[0] = fmax(m[0], sqrt(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]));
+  const int ops[1] = {OP_MAX};
+  publish<1>(m, ops, part, 1, s_tmp, s_out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_vel_max_final(const double* __restrict__ part,
+                                                          int G, double* out) {
+  __shared__ double s_tmp[4], s_out[1];
+  const int ops[1] = {OP_MAX};
+  gather<1>(part, 1, G, ops, s_tmp, s_out);
+  if (threadIdx.x == 0) out[0] = s_out[0];
+}
+
+hipError_t launch_vel_max(hipStream_t stream, int n, const double* u, const double* v,
+                          const double* w, double* part, int G, double* out) {
+  k_vel_max<<<G, kBlock, 0, stream>>>(n, u, v, w, part);
+  k_vel_max_final<<<1, kBlock, 0, stream>>>(part, G, out);
   return hipGetLastError();
 }
 

@@ -246,7 +246,10 @@ struct AsmArgs {
   const int* ent_face;   // nnz: last face setting each off-diagonal entry, -1 none
   const int* c1;         // n_faces, 0-based
   const int* c2;         // n_faces, 0-based; -1 for a boundary face
-  const uint8_t* flags;  // n_faces: 1 boundary diagonal term, 2 b(c1) = 0
+  const uint8_t* flags;  // n_faces: 1 boundary diagonal term, 2 b(c1) = 0 (pressure
+                         // region), 4 b(c1) = fc v_m(c1) pp(c2) (ambient region)
+  const int* c2b;        // n_faces: c2 as given (boundary cells < 0)
+  const double* pp;      // pp % n at cell 1 (boundary cell c at pp[c - 1]); ambient only
   const double* fc;
   const double* fw;
   const double* vm;
@@ -331,6 +334,8 @@ hipError_t launch_jacobi_apply(hipStream_t s, int n, const int* row, const doubl
                                const double* r, double* z);
 
 hipError_t launch_assemble(hipStream_t stream, const AsmArgs& g);
+hipError_t launch_vel_max(hipStream_t stream, int n, const double* u, const double* v,
+                          const double* w, double* part, int G, double* out);
 
 // Multi-rank CG (tfg_comm.hip).
 hipError_t launch_cm_local(hipStream_t s, const double* part, int pstride, int G, int nv,

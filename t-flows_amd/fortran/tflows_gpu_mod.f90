@@ -95,12 +95,12 @@
 
     ! Pressure-system assembly on the device from face data
     integer(c_int) function tfg_faces_create(handle, n_faces, c1, c2, pos,    &
-                                             diag_bc, b_zero, faces)        &
+                                             diag_bc, b_region, faces)        &
                                              bind(C, name='tfg_faces_create')
       import :: c_int, c_ptr
       type(c_ptr),    value       :: handle
       integer(c_int), value       :: n_faces
-      integer(c_int), intent(in)  :: c1(*), c2(*), pos(*), diag_bc(*), b_zero(*)
+      integer(c_int), intent(in)  :: c1(*), c2(*), pos(*), diag_bc(*), b_region(*)
       type(c_ptr),    intent(out) :: faces
     end function
 
@@ -110,6 +110,24 @@
       import :: c_int, c_ptr
       type(c_ptr),    value :: faces, fc, fw, vm, vflux, val, b
       integer(c_int), value :: singular
+    end function
+
+    ! The same with pp % n for ambient regions: pp = c_loc(pp % n(1))
+    integer(c_int) function tfg_pressure_assemble_ex(faces, fc, fw, vm,      &
+                                                     vflux, pp, val, b,      &
+                                                     singular)               &
+                                      bind(C, name='tfg_pressure_assemble_ex')
+      import :: c_int, c_ptr
+      type(c_ptr),    value :: faces, fc, fw, vm, vflux, pp, val, b
+      integer(c_int), value :: singular
+    end function
+
+    ! vel_max of Compute_Momentum from device u, v, w (before Max_Real)
+    integer(c_int) function tfg_vel_max(handle, u, v, w, vel_max)            &
+                                         bind(C, name='tfg_vel_max')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr),    value       :: handle, u, v, w
+      real(c_double), intent(out) :: vel_max
     end function
 
     integer(c_int) function tfg_rhie_chow_fluxes(faces, rc, v_flux)          &

@@ -136,8 +136,12 @@ class Faces:
     """Face data of a hex grid in the reference's layout (Grid_Mod faces_c,
     Matrix_Type pos / dia): 1-based cells c1, c2 (c2 <= 0 on the boundary),
     the CSR positions pos (2 x n_faces, 1-based; 0 on the boundary) of the
-    pressure matrix, per-face fc, interpolation weight fw, volume flux, and
-    the boundary flags of a pressure condition on the x = 0 side."""
+    pressure matrix, per-face fc, interpolation weight fw, volume flux, the
+    boundary flags (diag_bc, b_region: 1 pressure, 2 ambient region) and the
+    region number of each face (0 internal, 1..6 the sides x0, x1, y0, y1,
+    z0, z1); pp = pp % n over the boundary cells and the cells, laid out as
+    Fortran's (-n_bnd:n) (cell c at element c + n_bnd), when a side is
+    ambient."""
     row: np.ndarray
     col: np.ndarray
     dia: np.ndarray
@@ -145,27 +149,36 @@ class Faces:
     c2: np.ndarray
     pos: np.ndarray
     diag_bc: np.ndarray
-    b_zero: np.ndarray
+    b_region: np.ndarray
     fc: np.ndarray
     fw: np.ndarray
     vm: np.ndarray
     vflux: np.ndarray
+    region: np.ndarray = None
+    pp: np.ndarray = None
 
     @property
     def n(self):
         return len(self.dia)
 
     @property
+    def n_bnd(self):
+        return int(max(0, -self.c2.min())) if len(self.c2) else 0
+
+    @property
     def n_faces(self):
         return len(self.c1)
 
 
-def hex_faces(nx, ny=None, nz=None, seed=DEFAULT_SEED, shuffle=True):
+def hex_faces(nx, ny=None, nz=None, seed=DEFAULT_SEED, shuffle=True, ambient=False):
     """Faces of an nx*ny*nz hex grid (cells x slowest, z fastest, as the
     pressure generators number them) with seeded face data: fc = A/d times
     U(0.5, 2), fw ~ U(0.3, 0.7), v_m ~ U(0.5, 2) * 1e-3, flux ~ U(-1, 1).
-    Faces are listed in a seeded random order (shuffle) so that assembly
-    order matters."""
+    The x = 0 side is a pressure region.  Faces are listed in a seeded
+    random order (shuffle) so that assembly order matters.  ambient: the
+    y = 0 side is an ambient region too (pp ~ U(-1, 1) on its boundary
+    cells) and the boundary faces come first, side by side in region order,
+    as T-Flows numbers them (the internal faces after them, shuffled)."""
     ny = nx if ny is None else ny
     nz = nx if nz is None else nz
     n = nx * ny * nz
@@ -175,20 +188,25 @@ def hex_faces(nx, ny=None, nz=None, seed=DEFAULT_SEED, shuffle=True):
              np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1),
              np.stack([idx[:, :, :-1].ravel(), idx[:, :, 1:].ravel()], 1)]
     internal = np.concatenate(pairs)
-    bnd_cells, bnd_x0 = [], []
+    bnd_cells, bnd_reg = [], []
     for axis, size in ((0, nx), (1, ny), (2, nz)):
         for side in (0, size - 1):
             sl = [slice(None)] * 3
             sl[axis] = side
             cells = idx[tuple(sl)].ravel()
             bnd_cells.append(cells)
-            bnd_x0.append(np.full(cells.size, axis == 0 and side == 0))
+            bnd_reg.append(np.full(cells.size, 2 * axis + (1 if side == 0 else 2)))
     bnd_cells = np.concatenate(bnd_cells)
-    bnd_x0 = np.concatenate(bnd_x0)
+    bnd_reg = np.concatenate(bnd_reg)
     ni_, nb = len(internal), len(bnd_cells)
     c1 = np.concatenate([internal[:, 0], bnd_cells]) + 1
     c2 = np.concatenate([internal[:, 1] + 1, -(np.arange(nb) + 1)])
-    flag = np.concatenate([np.zeros(ni_, bool), bnd_x0])
+    region = np.concatenate([np.zeros(ni_, int), bnd_reg])
+    code = np.zeros(len(c1), int)
+    code[region == 1] = 1                      # x = 0: pressure
+    if ambient:
+        code[region == 3] = 2                  # y = 0: ambient
+    flag = code > 0
     # CSR positions of (c1, c2) in row c1 and (c2, c1) in row c2
     rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(s.row))
     keys = rows * n + (s.col - 1)
@@ -204,16 +222,23 @@ def hex_faces(nx, ny=None, nz=None, seed=DEFAULT_SEED, shuffle=True):
     pos[:ni_, 1] = position(internal[:, 1], internal[:, 0]) + 1
     rng = np.random.default_rng(seed)
     order = rng.permutation(len(c1)) if shuffle else np.arange(len(c1))
+    if ambient:   # boundary faces first in region order, then the internal ones
+        order = np.concatenate([np.arange(ni_, ni_ + nb), order[order < ni_]])
     h = 1.0 / nx
     nf = len(c1)
-    return Faces(row=s.row, col=s.col, dia=s.dia,
-                 c1=np.ascontiguousarray(c1[order], dtype=np.int32),
-                 c2=np.ascontiguousarray(c2[order], dtype=np.int32),
-                 pos=np.ascontiguousarray(pos[order].ravel(), dtype=np.int32),
-                 diag_bc=np.ascontiguousarray(flag[order], dtype=np.int32),
-                 b_zero=np.ascontiguousarray(flag[order], dtype=np.int32),
-                 fc=h * rng.uniform(0.5, 2.0, nf), fw=rng.uniform(0.3, 0.7, nf),
-                 vm=1e-3 * rng.uniform(0.5, 2.0, n), vflux=rng.uniform(-1.0, 1.0, nf))
+    f = Faces(row=s.row, col=s.col, dia=s.dia,
+              c1=np.ascontiguousarray(c1[order], dtype=np.int32),
+              c2=np.ascontiguousarray(c2[order], dtype=np.int32),
+              pos=np.ascontiguousarray(pos[order].ravel(), dtype=np.int32),
+              diag_bc=np.ascontiguousarray(flag[order], dtype=np.int32),
+              b_region=np.ascontiguousarray(code[order], dtype=np.int32),
+              fc=h * rng.uniform(0.5, 2.0, nf), fw=rng.uniform(0.3, 0.7, nf),
+              vm=1e-3 * rng.uniform(0.5, 2.0, n), vflux=rng.uniform(-1.0, 1.0, nf),
+              region=np.ascontiguousarray(region[order], dtype=np.int32))
+    if ambient:
+        f.pp = rng.uniform(-1.0, 1.0, nb + 1 + n)
+        f.pp[nb] = 0.0                         # cell 0: unused
+    return f
 
 
 @dataclass

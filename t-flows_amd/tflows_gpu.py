@@ -97,6 +97,8 @@ def lib():
         L.tfg_faces_create.argtypes = [P, I, P, P, P, P, P, ctypes.POINTER(P)]
         L.tfg_faces_destroy.argtypes = [P]
         L.tfg_pressure_assemble.argtypes = [P, P, P, P, P, P, P, I]
+        L.tfg_pressure_assemble_ex.argtypes = [P, P, P, P, P, P, P, P, I]
+        L.tfg_vel_max.argtypes = [P, P, P, P, ctypes.POINTER(ctypes.c_double)]
         L.tfg_momentum_assemble.argtypes = [P, P, P, P, P]
         L.tfg_matrix_reorder.argtypes = [P, I]
         L.tfg_rhie_chow_fluxes.argtypes = [P, P, P]
@@ -249,6 +251,12 @@ class Native:
                 raise TypeError(f"{name} must be a contiguous float64 array of length {n}")
         return self._solve(solver, prec, _ptr(val), _ptr(x), _ptr(b), miter, tol, norm, options,
                            report, device=False)
+
+    def vel_max(self, du, dv, dw):
+        """max(MICRO, max |u|) over the handle's nt cells (tfg_vel_max)."""
+        out = ctypes.c_double(0.0)
+        _check(lib().tfg_vel_max(self.h, du.ptr, dv.ptr, dw.ptr, ctypes.byref(out)))
+        return out.value
 
     def solve_native_device(self, solver, prec, dval, dx, db, miter, tol, norm=None,
                             options=None, report=False):
@@ -431,22 +439,28 @@ class Batch:
 
 class Faces:
     """Face structure of a grid for device-side pressure assembly
-    (tfg_faces_create).  c1, c2, pos, diag_bc, b_zero: the reference's face
+    (tfg_faces_create).  c1, c2, pos, diag_bc, b_region: the reference's face
     arrays (1-based; c2 <= 0 on the boundary; pos 2 x n_faces)."""
 
-    def __init__(self, nat, c1, c2, pos, diag_bc, b_zero):
+    def __init__(self, nat, c1, c2, pos, diag_bc, b_region):
         self.nat = nat
-        arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (c1, c2, pos, diag_bc, b_zero)]
+        arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (c1, c2, pos, diag_bc, b_region)]
         self.n_faces = len(arrs[0])
         self.h = ctypes.c_void_p()
         _check(lib().tfg_faces_create(nat.h, self.n_faces, *[_ptr(a) for a in arrs],
                                       ctypes.byref(self.h)))
         nat._dependents.add(self)
 
-    def assemble(self, dfc, dfw, dvm, dvflux, dval, db, singular=False):
-        """A % val and b on the device (DeviceArrays); waits for the result."""
-        _check(lib().tfg_pressure_assemble(self.h, dfc.ptr, dfw.ptr, dvm.ptr, dvflux.ptr,
-                                           dval.ptr, db.ptr, 1 if singular else 0))
+    def assemble(self, dfc, dfw, dvm, dvflux, dval, db, singular=False, dpp=None, n_bnd=0):
+        """A % val and b on the device (DeviceArrays); waits for the result.
+        dpp: pp % n laid out (-n_bnd:n) (ambient regions: tfg_pressure_assemble_ex)."""
+        if dpp is None:
+            _check(lib().tfg_pressure_assemble(self.h, dfc.ptr, dfw.ptr, dvm.ptr, dvflux.ptr,
+                                               dval.ptr, db.ptr, 1 if singular else 0))
+        else:
+            _check(lib().tfg_pressure_assemble_ex(self.h, dfc.ptr, dfw.ptr, dvm.ptr, dvflux.ptr,
+                                                  dpp.ptr.value + 8 * (n_bnd + 1), dval.ptr, db.ptr,
+                                                  1 if singular else 0))
         _check(lib().tfg_synchronize(self.nat.h))
 
     def momentum(self, m, dval, db, dvm):

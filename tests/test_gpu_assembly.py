@@ -1,7 +1,8 @@
 """Pressure-system assembly on the device (tfg_faces_create /
-tfg_pressure_assemble) against the oracle's serial face loop: A % val and b
-bit-identical, and the solve on the device-assembled system identical to the
-solve on the host-built one."""
+tfg_pressure_assemble / _ex) against the oracle's serial face loop: A % val
+and b bit-identical (pressure and ambient regions), and the solve on the
+device-assembled system identical to the solve on the host-built one;
+vel_max (tfg_vel_max); momentum assembly and Rhie-Chow fluxes."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 def _assemble_gpu(gpu, f, singular):
     nat = gpu.Native(f.row, f.col, f.dia)
-    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_region)
     d = {k: gpu.DeviceArray(getattr(f, k)) for k in ("fc", "fw", "vm", "vflux")}
     dval, db = gpu.DeviceArray(np.zeros(len(f.col))), gpu.DeviceArray(np.zeros(f.n))
     faces.assemble(d["fc"], d["fw"], d["vm"], d["vflux"], dval, db, singular=singular)
@@ -29,9 +30,42 @@ def test_assembly_bitexact(cases, oracle, gpu, dims, singular):
     assert np.array_equal(b, b_o), np.abs(b - b_o).max()
 
 
+@pytest.mark.parametrize("singular", [False, True])
+def test_ambient_assembly_bitexact(cases, oracle, gpu, singular):
+    """Compute_Pressure.f90:171-189 with a pressure and an ambient side
+    (corner cells on both: the later region's value)."""
+    f = cases.hex_faces(12, 10, 8, ambient=True)
+    print(f"ambient assembly n={f.n} faces={f.n_faces} singular={singular}", flush=True)
+    val_o, b_o = oracle.pressure_assemble(f, singular=singular)
+    nat = gpu.Native(f.row, f.col, f.dia)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_region)
+    d = {k: gpu.DeviceArray(getattr(f, k)) for k in ("fc", "fw", "vm", "vflux", "pp")}
+    dval, db = gpu.DeviceArray(np.zeros(len(f.col))), gpu.DeviceArray(np.zeros(f.n))
+    faces.assemble(d["fc"], d["fw"], d["vm"], d["vflux"], dval, db, singular=singular,
+                   dpp=d["pp"], n_bnd=f.n_bnd)
+    assert np.array_equal(dval.download(), val_o)
+    assert np.array_equal(db.download(), b_o)
+    # without pp % n the ambient faces are refused
+    with pytest.raises(gpu.TFlowsGpuError) as e:
+        faces.assemble(d["fc"], d["fw"], d["vm"], d["vflux"], dval, db)
+    assert e.value.code == gpu.TFG_EINVAL and "ambient" in str(e.value)
+
+
+def test_vel_max(oracle, gpu, cases):
+    """Compute_Momentum.f90:201-204 over the handle's cells."""
+    s = cases.hex_pressure(20, 18, 16)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    rng = np.random.default_rng(8)
+    u, v, w = (rng.uniform(-2, 2, s.n) for _ in range(3))
+    du, dv, dw = (gpu.DeviceArray(a) for a in (u, v, w))
+    assert nat.vel_max(du, dv, dw) == oracle.vel_max(u, v, w)
+    z = gpu.DeviceArray(np.zeros(s.n))
+    assert nat.vel_max(z, z, z) == 1e-6
+
+
 def test_solve_on_assembled_system(cases, oracle, gpu):
     f = cases.hex_faces(16, 14, 12)
-    f.b_zero[:] = 0
+    f.b_region[:] = 0
     val_o, b_o = oracle.pressure_assemble(f, singular=True)
     nat, faces, dval, db = _assemble_gpu(gpu, f, True)
     dx = gpu.DeviceArray(np.zeros(f.n))
@@ -46,11 +80,16 @@ def test_solve_on_assembled_system(cases, oracle, gpu):
 def test_faces_validation(cases, gpu):
     f = cases.hex_faces(4, 3, 3)
     nat = gpu.Native(f.row, f.col, f.dia)
+    bad = f.b_region.copy()
+    bad[np.flatnonzero(f.c2 < 0)[0]] = 3
+    with pytest.raises(gpu.TFlowsGpuError) as e:
+        gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, bad)
+    assert e.value.code == gpu.TFG_EINVAL and "region" in str(e.value)
     pos = f.pos.copy()
     k = int(np.nonzero(f.c2 > 0)[0][0])
     pos[2 * k] = f.dia[f.c1[k] - 1]          # points at the diagonal
     with pytest.raises(gpu.TFlowsGpuError) as e:
-        gpu.Faces(nat, f.c1, f.c2, pos, f.diag_bc, f.b_zero)
+        gpu.Faces(nat, f.c1, f.c2, pos, f.diag_bc, f.b_region)
     assert e.value.code == gpu.TFG_EINVAL and "pos" in str(e.value)
 
 
@@ -63,7 +102,7 @@ def _momentum_case(cases, dims, seed=5):
 
 def _momentum_gpu(gpu, f, mf, **kw):
     nat = gpu.Native(f.row, f.col, f.dia)
-    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_region)
     m = gpu.Momentum(mf, **kw)
     dval, db, dvm = (gpu.DeviceArray(np.full(k, np.nan)) for k in (len(f.col), f.n, f.n))
     faces.momentum(m, dval, db, dvm)
@@ -108,7 +147,7 @@ def test_momentum_boundary_faces_numbered_first(cases, oracle, gpu):
     order = np.argsort(f.c2 > 0, kind="stable")
     pos = f.pos.reshape(-1, 2)[order].ravel()
     g = type(f)(**{**f.__dict__, "c1": f.c1[order], "c2": f.c2[order], "pos": pos,
-                   "diag_bc": f.diag_bc[order], "b_zero": f.b_zero[order], "fc": f.fc[order],
+                   "diag_bc": f.diag_bc[order], "b_region": f.b_region[order], "fc": f.fc[order],
                    "fw": f.fw[order], "vflux": f.vflux[order]})
     mf = cases.momentum_fields(g)
     kw = dict(adv="superbee", td="parabolic")
@@ -139,7 +178,7 @@ def test_momentum_system_solves(cases, oracle, gpu):
 def test_momentum_validation(cases, gpu):
     f, mf = _momentum_case(cases, (4, 3, 3))
     nat = gpu.Native(f.row, f.col, f.dia)
-    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_region)
     dval, db, dvm = (gpu.DeviceArray(np.zeros(k)) for k in (len(f.col), f.n, f.n))
     m = gpu.Momentum(mf)
     m.s.n_bnd = mf.n_bnd - 1                     # below the largest boundary cell
@@ -177,7 +216,7 @@ def test_rhie_chow_bitexact(cases, oracle, gpu, choi, gu, vof, td):
     base = np.random.default_rng(4).uniform(-1.0, 1.0, f.n_faces)
     ref = oracle.rhie_chow_fluxes(f, rf, base, **kw)
     nat = gpu.Native(f.row, f.col, f.dia)
-    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_region)
     dflux = gpu.DeviceArray(base)
     faces.rhie_chow(gpu.RhieChow(rf, **kw), dflux)
     assert np.array_equal(dflux.download(), ref)
@@ -198,7 +237,7 @@ def test_rhie_chow_then_pressure_system(cases, oracle, gpu):
     flux_o = oracle.rhie_chow_fluxes(f, rf, f.vflux)
     g = dataclasses.replace(f, vm=vm_o, vflux=flux_o)
     val_o, b_o = oracle.pressure_assemble(g, singular=True)
-    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_region)
     dflux = gpu.DeviceArray(f.vflux)
     faces.rhie_chow(gpu.RhieChow(rf), dflux)
     assert np.array_equal(dflux.download(), flux_o)
@@ -213,7 +252,7 @@ def test_rhie_chow_validation(cases, gpu):
     f = cases.hex_faces(*dims)
     rf = cases.rhie_chow_fields(f, *dims)
     nat = gpu.Native(f.row, f.col, f.dia)
-    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_zero)
+    faces = gpu.Faces(nat, f.c1, f.c2, f.pos, f.diag_bc, f.b_region)
     rc = gpu.RhieChow(rf)
     rc.s.choi = 1                                # its arrays were not given
     with pytest.raises(gpu.TFlowsGpuError) as e:
