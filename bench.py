@@ -830,9 +830,10 @@ def bench_poisson40(args):
 
 def calibrate():
     """Known-traffic copy (1 GiB read + 1 GiB write) for counter calibration,
-    and the HBM ceilings of the three access mixes: 1 GiB copied, read, and
+    the HBM ceilings of the three access mixes: 1 GiB copied, read, and
     written (16-byte accesses, 4 workgroups per CU, mean of 5 launches after
-    one warm-up)."""
+    one warm-up), and the cost of a grid-wide barrier against a kernel
+    boundary."""
     import tflows_gpu as G
 
     nbytes = 1 << 30
@@ -840,11 +841,20 @@ def calibrate():
     ms = G.stream_copy(dst, src, nbytes, reps=5)
     ms_read = G.stream_bench(dst, src, nbytes, 1, reps=5)
     ms_write = G.stream_bench(dst, None, nbytes, 2, reps=5)
+    # phase separation: a grid-wide barrier in one cooperative launch against
+    # a kernel boundary, 1000 of each, at the grid sizes of the solve's kernels
+    sync = {}
+    for grid in (64, 128, 256, 768, 2048):
+        try:
+            b, k = G.sync_bench(grid, 1000)
+            sync[str(grid)] = {"grid_barrier_us": b, "kernel_boundary_us": k}
+        except G.TFlowsGpuError as e:   # not resident at once on this part
+            sync[str(grid)] = {"error": str(e)}
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
                       "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9,
                       "read_ms": ms_read, "read_GBps": nbytes / (ms_read * 1e-3) / 1e9,
                       "write_ms": ms_write, "write_GBps": nbytes / (ms_write * 1e-3) / 1e9,
-                      "peak_GBps": HBM_PEAK_GBS}), flush=True)
+                      "peak_GBps": HBM_PEAK_GBS, "sync_workgroups": sync}), flush=True)
 
 
 def main():

@@ -161,6 +161,13 @@ int tfg_stream_copy(void* d_dst, const void* d_src, long long bytes, int reps, d
  * access mixes. */
 int tfg_stream_bench(void* d_dst, const void* d_src, long long bytes, int mode, int reps,
                      double* ms_avg);
+/* No counterpart in the reference (measurement): the cost of the two ways of
+ * separating dependent phases on this device, `grid` workgroups of 256
+ * threads: *us_barrier = one grid-wide barrier inside a cooperative launch
+ * (reps of them in one kernel), *us_boundary = one kernel boundary (reps
+ * empty kernels back to back on one stream).  TFG_EINVAL when the grid
+ * cannot be resident at once. */
+int tfg_sync_bench(int grid, int reps, double* us_barrier, double* us_boundary);
 
 /* SpMV engines: 0 LDS-staged rows (2040 nonzeros per block), 1 the same
  * with 4088, 2/3/4 sub-wavefront rows with 4/8/16 lanes per row, 5 the

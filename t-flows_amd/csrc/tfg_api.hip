@@ -2691,6 +2691,39 @@ int tfg_stream_bench(void* d_dst, const void* d_src, long long bytes, int mode, 
   return TFG_OK;
 }
 
+int tfg_sync_bench(int grid, int reps, double* us_barrier, double* us_boundary) {
+  if (grid < 1 || reps < 1 || !us_barrier || !us_boundary) return fail(TFG_EINVAL, "bad argument");
+  int dev = 0, cus = 0, coop = 0, per_cu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  if (!coop) return fail(TFG_ENOTSUP, "no cooperative launches on this device");
+  if (!tfg::grid_barrier_fit(&per_cu) || grid > per_cu * cus)
+    return fail(TFG_EINVAL, "%d workgroups cannot all be resident (at most %d)", grid, per_cu * cus);
+  double* sink = nullptr;
+  HIP_TRY(hipMalloc(&sink, sizeof(double)));
+  hipEvent_t e[4];
+  for (hipEvent_t& x : e) HIP_TRY(hipEventCreate(&x));
+  hipError_t r = tfg::launch_grid_barriers(nullptr, grid, 1, sink);   // warm-up
+  if (r == hipSuccess) r = hipEventRecord(e[0], nullptr);
+  if (r == hipSuccess) r = tfg::launch_grid_barriers(nullptr, grid, reps, sink);
+  if (r == hipSuccess) r = hipEventRecord(e[1], nullptr);
+  if (r == hipSuccess) r = tfg::launch_boundary(nullptr, grid, sink, 0);
+  if (r == hipSuccess) r = hipEventRecord(e[2], nullptr);
+  for (int i = 0; i < reps && r == hipSuccess; ++i) r = tfg::launch_boundary(nullptr, grid, sink, i);
+  if (r == hipSuccess) r = hipEventRecord(e[3], nullptr);
+  if (r == hipSuccess) r = hipEventSynchronize(e[3]);
+  float ms_b = 0.f, ms_k = 0.f;
+  if (r == hipSuccess) r = hipEventElapsedTime(&ms_b, e[0], e[1]);
+  if (r == hipSuccess) r = hipEventElapsedTime(&ms_k, e[2], e[3]);
+  for (hipEvent_t& x : e) (void)hipEventDestroy(x);
+  (void)hipFree(sink);
+  HIP_TRY(r);
+  *us_barrier = 1e3 * ms_b / reps;
+  *us_boundary = 1e3 * ms_k / reps;
+  return TFG_OK;
+}
+
 int tfg_synchronize(tfg_matrix* h) {
   if (!h) HIP_TRY(hipDeviceSynchronize());
   else HIP_TRY(hipStreamSynchronize(h->stream));

@@ -20,6 +20,7 @@
 // to the reference's serial arithmetic; only the order of the global dot
 // products differs (deterministic trees here).
 #include "tfg_spmv.h"
+#include <hip/hip_cooperative_groups.h>
 #include <algorithm>
 #include <type_traits>
 
@@ -706,6 +707,37 @@ hipError_t launch_read16(hipStream_t stream, int grid, const void* src, double* 
 hipError_t launch_fill16(hipStream_t stream, int grid, void* dst, long long n16) {
   k_fill16<<<grid, kBlock, 0, stream>>>(reinterpret_cast<dbl2*>(dst), n16);
   return hipGetLastError();
+}
+
+// The two ways of separating the phases of an iteration, timed: `reps`
+// grid-wide barriers inside one cooperative launch, and `reps` empty
+// kernels back to back on one stream (each a kernel boundary).
+__global__ __launch_bounds__(kBlock) void k_grid_barriers(int reps, double* sink) {
+  cooperative_groups::grid_group g = cooperative_groups::this_grid();
+  for (int r = 0; r < reps; ++r) g.sync();
+  if (blockIdx.x == 0 && threadIdx.x == 0) sink[0] = (double)reps;
+}
+
+__global__ __launch_bounds__(kBlock) void k_boundary(double* sink, int r) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) sink[0] = (double)r;
+}
+
+hipError_t launch_grid_barriers(hipStream_t stream, int grid, int reps, double* sink) {
+  void* args[] = {&reps, &sink};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_grid_barriers), dim3(grid),
+                                    dim3(kBlock), args, 0, stream);
+}
+
+hipError_t launch_boundary(hipStream_t stream, int grid, double* sink, int r) {
+  k_boundary<<<grid, kBlock, 0, stream>>>(sink, r);
+  return hipGetLastError();
+}
+
+int grid_barrier_fit(int* per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_grid_barriers, kBlock, 0) ==
+                 hipSuccess
+             ? 1
+             : 0;
 }
 
 hipError_t launch_spmv_plain(const Launch& L, const SpmvShape& m, const double* val,

@@ -93,6 +93,7 @@ def lib():
         L.tfg_synchronize.argtypes = [P]
         L.tfg_stream_copy.argtypes = [P, P, LL, I, ctypes.POINTER(D)]
         L.tfg_stream_bench.argtypes = [P, P, LL, I, I, ctypes.POINTER(D)]
+        L.tfg_sync_bench.argtypes = [I, I, ctypes.POINTER(D), ctypes.POINTER(D)]
         L.tfg_matrix_set_comm.argtypes = [P, ALLREDUCE_FN, EXCHANGE_FN, P]
         L.tfg_faces_create.argtypes = [P, I, P, P, P, P, P, ctypes.POINTER(P)]
         L.tfg_faces_destroy.argtypes = [P]
@@ -186,6 +187,14 @@ def stream_bench(dst, src, nbytes, mode, reps=10):
     _check(lib().tfg_stream_bench(dst.ptr, src.ptr if src is not None else None, int(nbytes),
                                   int(mode), int(reps), ctypes.byref(ms)))
     return ms.value
+
+
+def sync_bench(grid, reps=1000):
+    """(us per grid-wide barrier in one cooperative launch, us per kernel
+    boundary of back-to-back empty kernels), `grid` workgroups of 256."""
+    b, k = ctypes.c_double(), ctypes.c_double()
+    _check(lib().tfg_sync_bench(int(grid), int(reps), ctypes.byref(b), ctypes.byref(k)))
+    return b.value, k.value
 
 
 class Native:
