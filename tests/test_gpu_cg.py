@@ -204,6 +204,33 @@ def test_fortran_padded_names(cases, gpu):
     assert a == b
 
 
+def test_parity_symmetric_engine_64(cases, oracle, gpu):
+    """The headline's product (zero-padded symmetric diagonals, pair mode)
+    on a 64^3 two-phase 1:1000 system against the oracle, at the parity
+    bar: the reference run is the serial oracle, the ensemble's other orders
+    run on the OpenMP build (their products are the same bits; their sums
+    are the orders' own)."""
+    s = cases.hex_pressure(64, rho_out=1000.0)
+    print(f"parity case twophase64 sym engine n={s.n} nnz={s.nnz}", flush=True)
+    it_o, res_o, x_o = _oracle_solve(oracle, s, "cg", "jacobi", 5000, 1e-8, criterion="relres")
+
+    def other(so):
+        A = oracle.Matrix(s.row, s.col, s.dia, s.val)
+        x = np.zeros(s.n)
+        it, _ = oracle.solve("cg", "jacobi", A, x, s.b.copy(), 5000, 1e-8, criterion="relres",
+                             sum_order=so, omp=True)
+        return it, x
+
+    env, spread = parity_bar.envelope(other, it_o, x_o)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    nat.spmv_select(8)
+    it_g, res_g, x_g, rep = _gpu_solve(gpu, s, "cg", "jacobi", 5000, 1e-8, criterion=1, nat=nat)
+    assert nat.info()["spmv_kind"] == 8
+    print(f"  oracle niter={it_o} res={res_o:.4e}; gpu niter={it_g} res={res_g:.4e}", flush=True)
+    parity_bar.check("twophase64 sym engine", it_g, x_g, it_o, x_o, env, spread)
+    assert res_g < 1e-8 and rep["true_relres"] < 1.01e-8
+
+
 # ---- full size (8M cells) through size-independent properties --------------
 
 def test_twophase_200_relres(cases, gpu):
