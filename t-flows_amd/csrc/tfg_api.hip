@@ -2397,7 +2397,9 @@ int tfg_solve_native_noalloc(tfg_matrix* h, const char* solver, int solver_len, 
 
   HIP_TRY(hipSetDevice(h->device));
   hipStream_t S = h->stream;
-  HIP_TRY(hipMemsetAsync(part, 0, partb, S));
+  // zeroed by kernels: a caller's stream may be under capture, where memset
+  // nodes are avoided
+  HIP_TRY(tfg::launch_zero16(S, part, (long long)(partb / 16)));
   tfg::Launch L = launch_of(h, d_val, kind);
   tfg::CgArgs a = args_of(h, kind);
   a.val = d_val;
@@ -2411,8 +2413,8 @@ int tfg_solve_native_noalloc(tfg_matrix* h, const char* solver, int solver_len, 
   a.st = st;
   if (symk) {
     // the diagonals' absent entries and slot gaps must read zero
-    HIP_TRY(hipMemsetAsync(sd, 0, sdb, S));
-    HIP_TRY(hipMemsetAsync(flag, 0, sizeof(int), S));
+    HIP_TRY(tfg::launch_zero16(S, sd, (long long)(sdb / 16)));
+    HIP_TRY(tfg::launch_zero16(S, flag, (long long)(flb / 16)));
     HIP_TRY(tfg::launch_sym_pack(S, grid_vec_of(h), shape_of(h, tfg::K_SYM_DIA), d_val, sd, flag));
     int asym = 1;
     HIP_TRY(hipMemcpyAsync(&asym, flag, sizeof(int), hipMemcpyDeviceToHost, S));

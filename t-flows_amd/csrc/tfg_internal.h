@@ -360,6 +360,7 @@ hipError_t launch_copy16(hipStream_t stream, int grid, const void* src, void* ds
 // out: one double per workgroup (grid entries)
 hipError_t launch_read16(hipStream_t stream, int grid, const void* src, double* out, long long n16);
 hipError_t launch_fill16(hipStream_t stream, int grid, void* dst, long long n16);
+hipError_t launch_zero16(hipStream_t stream, void* dst, long long n16);
 hipError_t launch_grid_barriers(hipStream_t stream, int grid, int reps, double* sink);
 hipError_t launch_boundary(hipStream_t stream, int grid, double* sink, int r);
 int grid_barrier_fit(int* per_cu);

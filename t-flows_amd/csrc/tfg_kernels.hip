@@ -694,6 +694,21 @@ __global__ __launch_bounds__(kBlock) void k_fill16(dbl2* __restrict__ dst, long 
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) dst[i] = z;
 }
 
+// Zero fill of 16-byte chunks (a kernel, so that no memset node is needed
+// where a caller's stream may be captured).
+__global__ __launch_bounds__(kBlock) void k_zero16(dbl2* __restrict__ dst, long long n16) {
+  const long long stride = (long long)gridDim.x * kBlock;
+  const dbl2 z = {0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) dst[i] = z;
+}
+
+hipError_t launch_zero16(hipStream_t stream, void* dst, long long n16) {
+  const long long g = (n16 + kBlock - 1) / kBlock;
+  const int grid = (int)std::max<long long>(1, std::min<long long>(g, 2048));
+  k_zero16<<<grid, kBlock, 0, stream>>>(reinterpret_cast<dbl2*>(dst), n16);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy16(hipStream_t stream, int grid, const void* src, void* dst, long long n16) {
   k_copy16<<<grid, kBlock, 0, stream>>>(reinterpret_cast<const dbl2*>(src), reinterpret_cast<dbl2*>(dst), n16);
   return hipGetLastError();
