@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -568,6 +569,8 @@ void icc_levels(const int* row0, const int* col0, int n, bool forward, std::vect
 }
 
 int ensure_icc(tfg_matrix* h) {
+  if (!h->icc_rows[0] || !h->icc_rows[1])
+    return fail(TFG_ENOTSUP, "no DIC levels on this handle");
   const size_t n = (size_t)h->nt;
   for (double** v : {&h->icc_d, &h->icc_dinv, &h->icc_y, &h->icc_z})
     if (!*v) HIP_TRY(hipMalloc(v, n * sizeof(double)));
@@ -1596,8 +1599,9 @@ int tfg_matrix_create(int ni, int nt, int nnz, const int* row, const int* col, c
     for (int b = 0; b < 2 && e == hipSuccess; ++b)
       e = hipMemcpy(h->blk[b], blk[b].data(), sizeof(int) * blk[b].size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) { rc = fail(TFG_EHIP, "structure upload: %s", hipGetErrorString(e)); break; }
-    // DIC dependency levels
-    {
+    // DIC dependency levels (not for the renumbered handle of an internal
+    // ordering: ICC runs on the caller's handle, in the caller's order)
+    if (!g_inner_create) {
       std::vector<int> r0(row, row + nt + 1), c0(col, col + nnz);
       for (auto& v : r0) v -= 1;
       for (auto& v : c0) v -= 1;
@@ -1820,45 +1824,71 @@ namespace {
 // (George and Liu: repeat the sweep from a least-degree row of the last
 // level while the depth grows), neighbours in increasing degree; the whole
 // order reversed.  Returns new -> old.
+//
+// The search sweeps only need each row's level, so they run level by level
+// on up to 16 host threads: a row joins the next level through a compare-
+// and-swap of its mark, and which thread wins changes neither the levels
+// nor the pick (least degree, then smallest index), so the order is the same
+// on every run and for any thread count.  The ordering sweep itself follows
+// the queue and stays serial.
 std::vector<int> rcm_order(int n, const std::vector<int>& row, const std::vector<int>& col) {
-  std::vector<int> deg(n), order, stamp(n, -1), level(n, 0);
+  std::vector<int> deg(n), order, mark(n, 0);  // mark: -1 ordered, else the last search
   for (int i = 0; i < n; ++i) deg[i] = row[i + 1] - row[i];
   std::vector<int> by_deg(n);
   for (int i = 0; i < n; ++i) by_deg[i] = i;
   std::stable_sort(by_deg.begin(), by_deg.end(), [&](int a, int b) { return deg[a] < deg[b]; });
-  std::vector<char> done(n, 0);
   order.reserve(n);
-  std::vector<int> queue, nb;
-  queue.reserve(n);
-  int sweep = 0;
-  // breadth-first levels from s over rows not yet ordered; returns depth and
-  // a least-degree row of the last level
+  const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  std::vector<std::vector<int>> next(T);
+  std::vector<int> front, nb;
+  int search = 0;
+  // levels from s over rows not yet ordered; returns the depth and the
+  // least-degree row of the last level (smallest index on ties)
   auto levels = [&](int s, int& last) -> int {
-    ++sweep;
-    queue.clear();
-    queue.push_back(s);
-    stamp[s] = sweep;
-    level[s] = 0;
+    const int id = ++search;
+    front.assign(1, s);
+    mark[s] = id;
     int depth = 0;
-    for (size_t h = 0; h < queue.size(); ++h) {
-      const int u = queue[h];
-      for (int k = row[u]; k < row[u + 1]; ++k) {
-        const int v = col[k];
-        if (v == u || done[v] || stamp[v] == sweep) continue;
-        stamp[v] = sweep;
-        level[v] = level[u] + 1;
-        depth = std::max(depth, level[v]);
-        queue.push_back(v);
+    for (;;) {
+      auto part = [&](int t) {
+        std::vector<int>& out = next[t];
+        out.clear();
+        const size_t lo = front.size() * t / T, hi = front.size() * (t + 1) / T;
+        for (size_t h = lo; h < hi; ++h) {
+          const int u = front[h];
+          for (int k = row[u]; k < row[u + 1]; ++k) {
+            const int v = col[k];
+            int m = __atomic_load_n(&mark[v], __ATOMIC_RELAXED);
+            if (m == id || m < 0) continue;
+            if (__atomic_compare_exchange_n(&mark[v], &m, id, false, __ATOMIC_RELAXED,
+                                            __ATOMIC_RELAXED))
+              out.push_back(v);
+          }
+        }
+      };
+      if (T == 1 || front.size() < 4096) {
+        for (int t = 0; t < T; ++t) part(t);
+      } else {
+        std::vector<std::thread> th;
+        for (int t = 1; t < T; ++t) th.emplace_back(part, t);
+        part(0);
+        for (std::thread& x : th) x.join();
       }
+      size_t total = 0;
+      for (int t = 0; t < T; ++t) total += next[t].size();
+      if (total == 0) break;
+      ++depth;
+      front.clear();
+      for (int t = 0; t < T; ++t) front.insert(front.end(), next[t].begin(), next[t].end());
     }
     last = -1;
-    for (int u : queue)
-      if (level[u] == depth && (last < 0 || deg[u] < deg[last])) last = u;
+    for (int u : front)
+      if (last < 0 || deg[u] < deg[last] || (deg[u] == deg[last] && u < last)) last = u;
     return depth;
   };
   size_t cand = 0;
   while ((int)order.size() < n) {
-    while (done[by_deg[cand]]) ++cand;
+    while (mark[by_deg[cand]] < 0) ++cand;
     int s = by_deg[cand], last = -1;
     int depth = levels(s, last);
     for (int it = 0; it < 4; ++it) {
@@ -1871,13 +1901,13 @@ std::vector<int> rcm_order(int n, const std::vector<int>& row, const std::vector
     }
     const size_t start = order.size();
     order.push_back(s);
-    done[s] = 1;
+    mark[s] = -1;
     for (size_t h = start; h < order.size(); ++h) {
       const int u = order[h];
       nb.clear();
       for (int k = row[u]; k < row[u + 1]; ++k)
-        if (!done[col[k]]) {
-          done[col[k]] = 1;
+        if (mark[col[k]] >= 0) {
+          mark[col[k]] = -1;
           nb.push_back(col[k]);
         }
       std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] < deg[b]; });
