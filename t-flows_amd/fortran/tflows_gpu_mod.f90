@@ -13,6 +13,21 @@
   implicit none
 !==============================================================================!
 
+  ! tfg_options / tfg_report (tflows_gpu.h) of the extended solve
+  integer(c_int), parameter :: TFG_FP64 = 0, TFG_FP32_IR = 1
+  integer(c_int), parameter :: TFG_CRIT_NRMS = 0, TFG_CRIT_RELRES = 1
+  type, bind(C) :: Tflows_Gpu_Options
+    integer(c_int) :: precision = 0, criterion = 0, path = 0, inner_max = 0
+    real(c_double) :: inner_tol = 0.0_c_double
+    integer(c_int) :: timing = 0, true_residual = 0, textbook_bicg = 0,     &
+                      deflation = 0
+  end type
+  type, bind(C) :: Tflows_Gpu_Report
+    integer(c_int) :: status, iterations, refinements
+    real(c_double) :: true_relres, solve_ms, spmv_ms
+    integer(c_int) :: spmv_samples, deflated
+  end type
+
   ! tfg_momentum (tflows_gpu.h): one velocity component's momentum assembly
   ! (Compute_Momentum.f90:119-366).  Device addresses; cell arrays at cell 1,
   ! e.g. c_loc(ui % n(1)) for the (-n_bnd:n) arrays phi_n, gx, gy, gz, uj_i,
@@ -74,6 +89,62 @@
       real(c_double),         value          :: tol
       real(c_double),         intent(out)    :: fin_res
       real(c_double), optional, intent(in)   :: norm
+    end function
+
+    ! The same call on device addresses (device_ptrs = 1; c_loc of host
+    ! arrays with device_ptrs = 0), with options and a report
+    integer(c_int) function tfg_solve_native_ex(handle, solver, solver_len,   &
+                                                prec, prec_len, val, x, b,    &
+                                                miter, niter, tol, fin_res,   &
+                                                norm, device_ptrs, opt, rep)  &
+                                           bind(C, name='tfg_solve_native_ex')
+      import :: c_int, c_ptr, c_char, c_double, Tflows_Gpu_Options,          &
+                Tflows_Gpu_Report
+      type(c_ptr),              value         :: handle, val, x, b
+      character(kind=c_char),   intent(in)    :: solver(*), prec(*)
+      integer(c_int),           value         :: solver_len, prec_len, miter
+      integer(c_int),           intent(out)   :: niter
+      real(c_double),           value         :: tol
+      real(c_double),           intent(out)   :: fin_res
+      real(c_double), optional, intent(in)    :: norm
+      integer(c_int),           value         :: device_ptrs
+      type(Tflows_Gpu_Options), intent(in)    :: opt
+      type(Tflows_Gpu_Report),  intent(out)   :: rep
+    end function
+
+    ! Device memory of the handle's device (bytes), copies to and from it
+    integer(c_int) function tfg_device_alloc(dptr, bytes)                    &
+                                             bind(C, name='tfg_device_alloc')
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr),          intent(out) :: dptr
+      integer(c_long_long), value       :: bytes
+    end function
+    integer(c_int) function tfg_device_free(dptr)                            &
+                                             bind(C, name='tfg_device_free')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dptr
+    end function
+    integer(c_int) function tfg_memcpy_h2d(dst, src, bytes)                  &
+                                             bind(C, name='tfg_memcpy_h2d')
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr),          value :: dst, src
+      integer(c_long_long), value :: bytes
+    end function
+    integer(c_int) function tfg_memcpy_d2h(dst, src, bytes)                  &
+                                             bind(C, name='tfg_memcpy_d2h')
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr),          value :: dst, src
+      integer(c_long_long), value :: bytes
+    end function
+    integer(c_int) function tfg_synchronize(handle)                          &
+                                             bind(C, name='tfg_synchronize')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: handle
+    end function
+    integer(c_int) function tfg_faces_destroy(faces)                         &
+                                             bind(C, name='tfg_faces_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: faces
     end function
 
     type(c_ptr) function tfg_last_error() bind(C, name='tfg_last_error')
@@ -234,6 +305,36 @@
                                          prec, len(prec), a_val, x, b,       &
                                          miter, n_out, tol, fin_res, norm),  &
                         'tfg_solve_native')
+  niter = n_out
+  end subroutine
+
+!==============================================================================!
+  subroutine Tflows_Gpu_Solve_Device(handle, solver, prec, d_val, d_x, d_b,  &
+                                     miter, niter, tol, fin_res, opt, rep,   &
+                                     norm)
+!------------------------------------------------------------------------------!
+!>  Solve_Native's call on device-resident arrays (device addresses of
+!>  A % val, x(1:nt) and b, e.g. from tfg_device_alloc or an offload
+!>  runtime), with the options of tfg_solve_native_ex (precision, stopping
+!>  criterion, ...) and its report.
+!------------------------------------------------------------------------------!
+  type(c_ptr),              intent(in)  :: handle, d_val, d_x, d_b
+  character(*),             intent(in)  :: solver, prec
+  integer,                  intent(in)  :: miter
+  integer,                  intent(out) :: niter
+  real(c_double),           intent(in)  :: tol
+  real(c_double),           intent(out) :: fin_res
+  type(Tflows_Gpu_Options), intent(in)  :: opt
+  type(Tflows_Gpu_Report),  intent(out) :: rep
+  real(c_double), optional, intent(in)  :: norm
+!-----------------------------------[Locals]-----------------------------------!
+  integer(c_int) :: n_out
+!==============================================================================!
+  call Tflows_Gpu_Check(tfg_solve_native_ex(handle, solver, len(solver),     &
+                                            prec, len(prec), d_val, d_x,     &
+                                            d_b, miter, n_out, tol, fin_res, &
+                                            norm, 1, opt, rep),              &
+                        'tfg_solve_native_ex')
   niter = n_out
   end subroutine
 
