@@ -91,7 +91,9 @@ int tfg_matrix_destroy(tfg_matrix* handle);
  * textbook "bicg" via tfg_options.textbook_bicg -- FP64, jacobi or none, with
  * the same normalisation, stopping test and niter/fin_res conventions;
  * prec: "jacobi" | "icc" | anything else = none (Prec_Form.f90:47-89);
- * "icc" (the reference's default) with cg / bicg in FP64.
+ * "icc" (the reference's default) with cg / bicg, in FP64 or in the FP32
+ * form (TFG_FP32_IR: z = M^-1 r from the FP64 DIC sweeps of the FP32
+ * residual, rounded to FP32).
  * val(nnz), x(nt) = x0 on entry / solution on exit, b(nt).  val and b are
  * left unchanged (the reference scales and de-scales them in place,
  * Cg.f90:77-84, 245-250).  norm: NULL when the optional argument is absent.

@@ -1062,8 +1062,18 @@ static double dot32(const float* a, const float* b, int n) {
   return s;
 }
 
-/* FP32 Jacobi / unpreconditioned CG in the row-excess form with FP64
- * reliable updates; with `deflate`, the constant vector w is deflated
+/* z = fl32(M^-1 fl64(r)) of the ICC (DIC) of the scaled matrix, the FP32
+ * form's ICC: the FP64 sweeps of Prec_Solve.f90:64-92 on the widened FP32
+ * residual, rounded back. */
+static void ir_icc_z(const or_matrix* A, int ni, const double* d, const double* dinv,
+                     const float* r, double* rw, double* zw, float* z) {
+  for (int i = 0; i < ni; i++) rw[i] = (double)r[i];
+  or_prec_solve(A, ni, d, dinv, zw, rw, OR_PREC_ICC);
+  for (int i = 0; i < ni; i++) z[i] = (float)zw[i];
+}
+
+/* FP32 Jacobi / ICC / unpreconditioned CG in the row-excess form with FP64
+ * reliable updates (ICC: z from ir_icc_z); with `deflate`, the constant vector w is deflated
  * (deflated PCG, Saad, Yeung, Erhel and Guyomarc'h 2000): E = w'Aw =
  * sum_i s_i, every direction p = z + beta p - w (s'z) / E, and every
  * restart begins with the coarse correction x += c w, c = w'(b - A x) / E
@@ -1095,15 +1105,20 @@ int or_ir_cg(or_matrix* A, double* x, double* b, int prec, int miter, int* niter
   float* p = (float*)calloc((size_t)nt, sizeof(float));
   float* q = (float*)calloc((size_t)nt, sizeof(float));
   float* e = (float*)calloc((size_t)nt, sizeof(float));
-  if (!r64 || !s64 || !v32 || !rd || !r || !z || !p || !q || !e) {
+  const int icc = prec == OR_PREC_ICC;
+  double* dic = (double*)calloc((size_t)(icc ? 4 * nt : 1), sizeof(double));
+  if (!r64 || !s64 || !v32 || !rd || !r || !z || !p || !q || !e || !dic) {
     free(r64); free(s64); free(v32); free(rd); free(r); free(z); free(p); free(q); free(e);
+    free(dic);
     return -2;
   }
+  double *dic_d = dic, *dic_dinv = dic + nt, *rw = dic + 2 * nt, *zw = dic + 3 * nt;
   double res = 0.0, rho = 0.0, rref = 0.0;
   int iter = 0, nref = 0, brk = 0;
   if (check < 1) check = 1;
 
   const double fn = normalise(A, b);
+  if (icc) or_prec_form(A, ni, dic_d, dic_dinv, OR_PREC_ICC);   /* Cg.f90:89 */
   or_lap32_form(A, ni, 1.0, v32);             /* values already scaled */
   double E = 0.0;
   or_csum bsum = {0.0, 0.0};
@@ -1144,6 +1159,7 @@ int or_ir_cg(or_matrix* A, double* x, double* b, int prec, int miter, int* niter
           r[i] = (float)(r64[i] - c * s64[i]);
           z[i] = prec == OR_PREC_JACOBI ? r[i] * rd[i] : r[i];
         }
+        if (icc) ir_icc_z(A, ni, dic_d, dic_dinv, r, rw, zw, z);
         float mu = 0.0f;                      /* p = z - w (s'z) / E */
         if (deflate) {
           double g = 0.0;
@@ -1170,6 +1186,7 @@ int or_ir_cg(or_matrix* A, double* x, double* b, int prec, int miter, int* niter
         r[i] = r[i] - a32 * q[i];
         z[i] = prec == OR_PREC_JACOBI ? r[i] * rd[i] : r[i];
       }
+      if (icc) ir_icc_z(A, ni, dic_d, dic_dinv, r, rw, zw, z);
       const double rz = dot32(z, r, ni), rr = dot32(r, r, ni);
       iter = k;
       if (k % check == 0 || k == miter) {
@@ -1217,5 +1234,6 @@ done:
   *niter = iter;
   if (refinements) *refinements = nref;
   free(r64); free(s64); free(v32); free(rd); free(r); free(z); free(p); free(q); free(e);
+  free(dic);
   return brk ? 1 : 0;
 }

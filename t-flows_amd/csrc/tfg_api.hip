@@ -193,6 +193,8 @@ struct tfg_matrix {
   void* comm_ctx = nullptr;
   double* loc = nullptr;   // device: this rank's totals of a reduction
   double* vmax = nullptr;  // device: tfg_vel_max's partials and result
+  double* icc_r64 = nullptr;  // FP32 form with ICC: the widened FP32 residual
+  float* z32 = nullptr;       // FP32 form with ICC: z = M^-1 r rounded to FP32
   // offset-coded columns (structured grids): col = row + tab[code]
   uint8_t* code = nullptr;
   int* tab = nullptr;
@@ -388,7 +390,7 @@ void free_all(tfg_matrix* h) {
                  h->ns32[3], h->ns32[4], h->valt32, h->rt, h->nv[0], h->nv[1], h->nv[2],
                  h->nv[3], h->valt, h->tpos, h->code, h->tab, h->pat, h->poff, h->plen,
                  h->pslot, h->sym, h->sym_flag, h->sorder,
-                 h->loc, h->vmax, h->icc_rows[0],
+                 h->loc, h->vmax, h->icc_r64, h->z32, h->icc_rows[0],
                  h->icc_rows[1], h->icc_d, h->icc_dinv, h->icc_y, h->icc_z,
                  (void*)h->bx.tiles, h->bx.dg, h->bx.lo[0], h->bx.lo[1],
                  h->bx.lo[2], h->bx.up[0], h->bx.up[1], h->bx.up[2]};
@@ -865,11 +867,25 @@ int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int mi
   // the deflated update reads s; when deflation is off for certain, it
   // need not
   const bool defl_kernels = o.deflation != TFG_DEFLATE_OFF && (sym || check_sym);
+  // ICC: z = M^-1 r from the FP64 DIC sweeps of the widened FP32 residual
+  // (the reference's factor, formed once per solve), rounded to FP32
+  const bool icc = prec == tfg::PREC_ICC;
+  tfg::IccArgs ic{};
+  tfg::IrArgs fz = f;
+  if (icc) {
+    const int ri = ensure_icc(h);
+    if (ri != TFG_OK) return ri;
+    ic = icc_args(h);
+    if (!h->icc_r64) HIP_TRY(hipMalloc(&h->icc_r64, sizeof(double) * std::max<size_t>(n, 4)));
+    if (!h->z32) HIP_TRY(hipMalloc(&h->z32, sizeof(float) * std::max<size_t>(n, 4)));
+    fz.r32 = h->z32;  // the p update reads z where it reads r otherwise
+  }
 
   HIP_TRY(hipEventRecord(h->ev_t0, h->stream));
   HIP_TRY(tfg::launch_diag_sum(L, a));
   HIP_TRY(tfg::launch_prologue(L, a, ipr));
   HIP_TRY(tfg::launch_start(L, a));
+  if (icc) HIP_TRY(tfg::launch_icc_form(h->stream, ic, val, h->st));
   HIP_TRY(tfg::launch_ir_convert(L, L32, a, f, h->nnz));
   if (check_sym)
     HIP_TRY(tfg::launch_ir_symcheck(h->stream, L.grid_vec, val, h->tpos, h->nnz, h->st));
@@ -881,7 +897,14 @@ int solve_ir(tfg_matrix* h, int prec, const double* val, const double* b, int mi
   while (k <= miter && !stop) {
     const int kend = std::min(miter, k + check - 1);
     for (; k <= kend; ++k) {
-      HIP_TRY(tfg::launch_ir_pupdate(L, a, f));
+      if (icc) {
+        HIP_TRY(tfg::launch_ir_widen(L, a, f.r32, h->icc_r64));
+        HIP_TRY(tfg::launch_icc_apply(h->stream, ic, val, h->st, h->icc_r64, h->icc_z));
+        HIP_TRY(tfg::launch_ir_icc_rho(L, a, f, h->icc_z, h->z32));
+        HIP_TRY(tfg::launch_ir_pupdate(L, a, fz));
+      } else {
+        HIP_TRY(tfg::launch_ir_pupdate(L, a, f));
+      }
       const bool take = timing && (k % 4 == 1) && (int)sample_k.size() < max_samples;
       if (take) HIP_TRY(hipEventRecord(h->ev_sample[2 * sample_k.size()], h->stream));
       HIP_TRY(tfg::launch_ir_spmv(L32, a, f));
@@ -2111,8 +2134,8 @@ int tfg_solve_native_ex(tfg_matrix* h, const char* solver, int solver_len, const
   // Prec_Form.f90:47-89: 'jacobi', 'icc', anything else = none.
   const std::string pv = fstring(prec, prec_len);
   const int pr = pv == "jacobi" ? tfg::PREC_JACOBI : (pv == "icc" ? tfg::PREC_ICC : tfg::PREC_NONE);
-  if (pr == tfg::PREC_ICC && (method != tfg::NS_NONE || o.precision != TFG_FP64))
-    return fail(TFG_ENOTSUP, "preconditioner 'icc' runs with FP64 cg / bicg only");
+  if (pr == tfg::PREC_ICC && method != tfg::NS_NONE)
+    return fail(TFG_ENOTSUP, "preconditioner 'icc' runs with cg / bicg only");
 
   if (o.criterion != TFG_CRIT_NRMS && o.criterion != TFG_CRIT_RELRES)
     return fail(TFG_EINVAL, "unknown criterion %d", o.criterion);

@@ -373,6 +373,9 @@ hipError_t launch_ir_convert(const Launch& L, const Launch& L32, const CgArgs& a
 // State::sym_vals = 0 when some val[j] differs from val[tpos[j]] bitwise.
 hipError_t launch_ir_symcheck(hipStream_t s, int grid, const double* val, const int* tpos,
                               long long nnz, State* st);
+hipError_t launch_ir_widen(const Launch& L, const CgArgs& a, const float* r32, double* r64);
+hipError_t launch_ir_icc_rho(const Launch& L, const CgArgs& a, const IrArgs& f, const double* z64,
+                             float* z32);
 hipError_t launch_ir_pupdate(const Launch& L, const CgArgs& a, const IrArgs& f);
 hipError_t launch_ir_spmv(const Launch& L32, const CgArgs& a, const IrArgs& f);
 hipError_t launch_ir_update(const Launch& L, const CgArgs& a, const IrArgs& f, bool defl,

@@ -345,6 +345,37 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(CgArgs a, IrArgs f) {
 }
 
 // ---------------------------------------------------------------------------
+// ICC in the FP32 sweeps: z = M^-1 r comes from the FP64 DIC sweeps (the
+// reference's factor of the scaled matrix, Prec_Form.f90:68-78 /
+// Prec_Solve.f90:64-92) on the widened FP32 residual, rounded back to FP32.
+// The product and the updates stay FP32; (z, r) and (s, z) are taken here,
+// after the sweeps, over rows 0 and 1 of the update's partials (rows 2 and
+// 3, (r, r) and the NRMS sum, stay the update's).
+__global__ __launch_bounds__(kBlock) void k_ir_widen(CgArgs a, const float* __restrict__ r32,
+                                                     double* __restrict__ r64) {
+  if (a.st->done) return;
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) r64[i] = (double)r32[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_ir_icc_rho(CgArgs a, IrArgs f,
+                                                       const double* __restrict__ z64,
+                                                       float* __restrict__ z32) {
+  __shared__ double s_tmp[2 * 4], s_out[2];
+  if (a.st->done) return;
+  double v[2] = {0.0, 0.0};
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) {
+    const float z = (float)z64[i];
+    z32[i] = z;
+    v[0] = v[0] + (double)z * (double)f.r32[i];
+    v[1] = v[1] + (double)f.s32[i] * (double)z;
+  }
+  const int ops[2] = {OP_SUM, OP_SUM};
+  publish<2>(v, ops, a.pt.upd, a.pt.pstride, s_tmp, s_out);
+}
+
+// ---------------------------------------------------------------------------
 // Chunk end, first kernel: is a refinement due (State::refine)?  If so,
 // x += e + alpha_k p_k in FP64 (the correction with the last iteration's
 // deferred step) and e = 0.
@@ -562,6 +593,18 @@ hipError_t launch_ir_convert(const Launch& L, const Launch& L32, const CgArgs& a
 hipError_t launch_ir_symcheck(hipStream_t s, int grid, const double* val, const int* tpos,
                               long long nnz, State* st) {
   k_ir_symcheck<<<grid, kBlock, 0, s>>>(val, tpos, nnz, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_ir_widen(const Launch& L, const CgArgs& a, const float* r32, double* r64) {
+  k_ir_widen<<<L.grid_vec, kBlock, 0, L.stream>>>(a, r32, r64);
+  return hipGetLastError();
+}
+
+hipError_t launch_ir_icc_rho(const Launch& L, const CgArgs& a, const IrArgs& f, const double* z64,
+                             float* z32) {
+  // the grid of the update: every partial of rows 0 and 1 is rewritten
+  k_ir_icc_rho<<<a.pt.g_vec, kBlock, 0, L.stream>>>(a, f, z64, z32);
   return hipGetLastError();
 }
 

@@ -84,7 +84,7 @@ def test_fortran_device_resident(cases, oracle, tmp_path, solver, prec, norm):
     tfg_device_alloc): the same bits as the host-array call; and the FP32
     sweeps inside FP64 refinement through the same entry point, converged,
     with an error against the oracle's tightly converged solution no worse
-    than ten times the FP64 run's (jacobi; the FP32 form takes no icc)."""
+    than ten times the FP64 run's."""
     s = cases.hex_pressure(20, 16, 12, rho_out=10.0)
     _write(str(tmp_path), s, np.zeros(s.n))
     d = str(tmp_path)
@@ -92,8 +92,6 @@ def test_fortran_device_resident(cases, oracle, tmp_path, solver, prec, norm):
     it_d, res_d, true_d, x_d = _run_driver(d, solver, prec, norm, "device")
     assert (it_d, res_d) == (it_h, res_h) and np.array_equal(x_d, x_h)
     assert true_d < 1e-6
-    if prec != "jacobi":
-        return
     A = oracle.Matrix(s.row, s.col, s.dia, s.val.copy())
     xo = np.zeros(s.n)
     oracle.solve(solver, prec, A, xo, s.b.copy(), 5000, 1e-14, criterion="relres")

@@ -6,9 +6,11 @@ the reference), pinned by what the mathematics fixes:
   vector constant over the cells gives s_i x exactly;
 * the row sums are compensated: a row whose entries cancel gets its exact
   sum, where a plain left-to-right sum keeps a rounding error;
-* the solve converges to the dense solution, stops on the FP64 true
-  residual, counts iterations as documented, and its coarse correction
-  leaves the residual orthogonal to the constant vector;
+* the solve converges to the dense solution (jacobi, icc, none), stops on
+  the FP64 true residual, counts iterations as documented, and its coarse
+  correction leaves the residual orthogonal to the constant vector;
+* its first step is the FP64 method's first iterate alpha M^-1 r0 (ICC:
+  the DIC sweeps of the widened FP32 residual);
 * planted mistakes (a wrong sign in the product's differences, a plain
   FP32 diagonal instead of the row sum) fail these checks.
 """
@@ -97,7 +99,7 @@ def test_ir_solve_converges_to_the_dense_solution(cases, oracle):
                        (cases.hex_pressure(7, 6, 5), 0),
                        (cases.knn(6, 6, 6), 1)):
         xd = np.linalg.solve(s.scipy().toarray(), s.b)
-        for prec in ("jacobi", "none"):
+        for prec in ("jacobi", "none", "icc"):
             x = np.zeros(s.n)
             it, res, nref = oracle.ir_solve(prec, _mat(oracle, s), x, s.b.copy(), 5000, 1e-10,
                                             criterion="relres", delta=1e-3, check=8,
@@ -109,6 +111,26 @@ def test_ir_solve_converges_to_the_dense_solution(cases, oracle):
             assert res < 1e-10 and true < 1.01e-10 and abs(res - true) <= 1e-2 * true
             assert it % 8 == 0 and nref >= 1
             assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd), (s.name, prec)
+
+
+@pytest.mark.parametrize("prec", ["icc", "jacobi"])
+def test_ir_first_step_is_the_fp64_methods(cases, oracle, prec):
+    """One FP32 iteration and its refinement give x1 = alpha z0 with
+    z0 = M^-1 r0: the FP64 method's first iterate (Cg.f90 with one
+    iteration) to FP32 accuracy, and far from the other preconditioner's
+    (a missing sweep or the Jacobi z in place of the DIC fails)."""
+    s = cases.hex_pressure(9, 7, 6, rho_out=30.0)
+    x = np.zeros(s.n)
+    oracle.ir_solve(prec, _mat(oracle, s), x, s.b.copy(), 1, 1e-30, criterion="relres", check=1)
+    firsts = {}
+    for p in ("icc", "jacobi"):
+        xf = np.zeros(s.n)
+        oracle.solve("cg", p, _mat(oracle, s), xf, s.b.copy(), 1, 1e-30, criterion="relres")
+        firsts[p] = xf
+    other = "jacobi" if prec == "icc" else "icc"
+    err = np.linalg.norm(x - firsts[prec]) / np.linalg.norm(firsts[prec])
+    far = np.linalg.norm(x - firsts[other]) / np.linalg.norm(firsts[other])
+    assert err < 1e-6 and far > 1e-2, (err, far)
 
 
 def test_ir_solve_conventions(cases, oracle):
