@@ -1892,8 +1892,17 @@ std::vector<int> rcm_order(int n, const std::vector<int>& row, const std::vector
       if (T == 1 || front.size() < 4096) {
         for (int t = 0; t < T; ++t) part(t);
       } else {
+        // parts whose thread cannot be started run on this one
         std::vector<std::thread> th;
-        for (int t = 1; t < T; ++t) th.emplace_back(part, t);
+        int started = 1;
+        try {
+          for (int t = 1; t < T; ++t) {
+            th.emplace_back(part, t);
+            started = t + 1;
+          }
+        } catch (...) {
+        }
+        for (int t = started; t < T; ++t) part(t);
         part(0);
         for (std::thread& x : th) x.join();
       }
