@@ -243,6 +243,22 @@ def spmv_roofline(kind, nnz, n, sym_diagonals, launch_ms, traffic=None):
     }
 
 
+def practical_floor(G, read_bytes, write_bytes, launch_ms):
+    """The launch against what this GPU's HBM reaches in practice: a 1 GiB
+    read sweep and a 1 GiB write sweep (tfg_stream_bench, 16-byte accesses,
+    mean of 5 after a warm-up) give the ceilings; the floor is the engine's
+    format bytes at them (reads at the read rate, writes at the write
+    rate)."""
+    nbytes = 1 << 30
+    a, b = G.DeviceArray(nbytes, np.uint8), G.DeviceArray(nbytes, np.uint8)
+    rd = nbytes / (G.stream_bench(b, a, nbytes, 1, reps=5) * 1e-3) / 1e9
+    wr = nbytes / (G.stream_bench(b, None, nbytes, 2, reps=5) * 1e-3) / 1e9
+    floor_us = (read_bytes / rd + write_bytes / wr) / 1e3
+    return {"read_GBps": rd, "write_GBps": wr, "format_read_bytes": read_bytes,
+            "format_write_bytes": write_bytes, "floor_us": floor_us,
+            "frac": floor_us / (launch_ms * 1e3)}
+
+
 # SpMV engine -> text of its template argument in the kernel's full name
 ENGINE_TAG = {0: "StreamEngine<2040, false, double, false>", 1: "StreamEngine<4088,",
               2: "WaveEngine<4,", 3: "WaveEngine<8,", 4: "WaveEngine<16,",
@@ -988,6 +1004,14 @@ def main():
         cpu, parity, x_ref = cpu_baseline(s, x_gpu, its[-1])
         fp32 = fp32_summary(fp32_runs, x_gpu, x_ref, total / args.steps, tol, s.scipy().tocsr())
 
+    # the product against the measured HBM ceilings: its format bytes (all
+    # but q, which it writes, are read) at the read and write sweep rates
+    fmt_bytes = format_bytes(kind if (kind != 8 or sym_ran) else 7, s.nnz, s.n,
+                             nat.info()["sym_diagonals"])
+    practical = practical_floor(G, fmt_bytes - 8 * s.n, 8 * s.n, spmv_ms)
+    log(f"bench: product at {practical['frac']:.2f} of its practical floor "
+        f"({practical['floor_us']:.1f} us at {practical['read_GBps']:.0f} GB/s read, "
+        f"{practical['write_GBps']:.0f} GB/s write)")
     out = {
         "metric": "time to 1e-8 rel. residual, 8M-cell pressure solve; SpMV HBM GB/s vs 8 TB/s",
         "value": total / args.steps,
@@ -1014,6 +1038,7 @@ def main():
         "roofline": dict(
             spmv_roofline(kind if (kind != 8 or sym_ran) else 7, s.nnz, s.n,
                           nat.info()["sym_diagonals"], spmv_ms, traffic),
+            practical=practical,
             traffic_source=os.path.relpath(traffic_src, ROOT) if traffic_src else None,
             kernel=spmv_kernel + " (FP64 SpMV with fused p.q; engine "
             + G.Native.SPMV_KINDS[kind] + ")",
