@@ -48,6 +48,14 @@ def test_structure_validation_on_host(gpu):
     assert rc == gpu.TFG_EINVAL and "sizes" in msg
     # null arguments
     assert gpu.lib().tfg_matrix_create(2, 2, 2, None, None, None, ctypes.byref(h)) == gpu.TFG_EINVAL
+    # an empty system, and an empty row (every row holds at least its diagonal)
+    rc, msg = _create(gpu, 0, 0, i32([1]), i32([1]), i32([1]))
+    assert rc == gpu.TFG_EINVAL and "sizes" in msg
+    rc, msg = _create(gpu, 2, 2, i32([1, 2, 2]), i32([1, 1]), i32([1, 2]))
+    assert rc == gpu.TFG_EINVAL and ("empty" in msg or "nnz" in msg)
+    # the face structure's null arguments are refused before any device call
+    assert gpu.lib().tfg_faces_create(None, 0, None, None, None, None, None,
+                                      ctypes.byref(h)) == gpu.TFG_EINVAL
 
 
 def test_batch_validation_on_host(gpu):
