@@ -88,8 +88,9 @@ int tfg_matrix_destroy(tfg_matrix* handle);
  * solver: "cg" | "bicg" (Fortran blank padding allowed; len < 0: strlen);
  * beyond the reference, the textbook methods of its PETSc path
  * (Petsc_Interface.cpp:416-436): "cgs", "bcgs" (alias "bicgstab"), and
- * textbook "bicg" via tfg_options.textbook_bicg -- FP64, jacobi or none, with
- * the same normalisation, stopping test and niter/fin_res conventions;
+ * textbook "bicg" via tfg_options.textbook_bicg -- jacobi or none, in FP64 or
+ * the FP32 form, with the same normalisation, stopping test and niter/fin_res
+ * conventions;
  * prec: "jacobi" | "icc" | anything else = none (Prec_Form.f90:47-89);
  * "icc" (the reference's default) with cg / bicg, in FP64 or in the FP32
  * form (TFG_FP32_IR: z = M^-1 r from the FP64 DIC sweeps of the FP32
