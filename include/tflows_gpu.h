@@ -279,6 +279,12 @@ int tfg_pressure_assemble_ex(tfg_faces* faces, const double* fc, const double* f
  * to host memory (waits for the handle's stream). */
 int tfg_vel_max(tfg_matrix* h, const double* u, const double* v, const double* w,
                 double* vel_max);
+/* The smallest and largest value of a device vector over cells 1..nt: the
+ * pressure range of Compute_Pressure.f90:104-109 (p_min, p_max before their
+ * Global % Min/Max_Real); the pressure solve's norm is then
+ * p_nor = max(p_max - p_min, p_nor_c, |p_drop_x|, |p_drop_y|, |p_drop_z|)
+ * (:115-117).  Results to host memory (waits for the handle's stream). */
+int tfg_range(tfg_matrix* h, const double* v, double* vmin, double* vmax);
 
 /* ---- momentum assembly on the device ------------------------------------
  * Matrix, right-hand side and M % v_m of one velocity component from face

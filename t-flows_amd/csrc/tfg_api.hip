@@ -193,6 +193,7 @@ struct tfg_matrix {
   void* comm_ctx = nullptr;
   double* loc = nullptr;   // device: this rank's totals of a reduction
   double* vmax = nullptr;  // device: tfg_vel_max's partials and result
+  double* vrange = nullptr;  // device: tfg_range's partials and result
   double* icc_r64 = nullptr;  // FP32 form with ICC: the widened FP32 residual
   float* z32 = nullptr;       // FP32 form with ICC: z = M^-1 r rounded to FP32
   // offset-coded columns (structured grids): col = row + tab[code]
@@ -390,7 +391,7 @@ void free_all(tfg_matrix* h) {
                  h->ns32[3], h->ns32[4], h->valt32, h->rt, h->nv[0], h->nv[1], h->nv[2],
                  h->nv[3], h->valt, h->tpos, h->code, h->tab, h->pat, h->poff, h->plen,
                  h->pslot, h->sym, h->sym_flag, h->sorder,
-                 h->loc, h->vmax, h->icc_r64, h->z32, h->icc_rows[0],
+                 h->loc, h->vmax, h->vrange, h->icc_r64, h->z32, h->icc_rows[0],
                  h->icc_rows[1], h->icc_d, h->icc_dinv, h->icc_y, h->icc_z,
                  (void*)h->bx.tiles, h->bx.dg, h->bx.lo[0], h->bx.lo[1],
                  h->bx.lo[2], h->bx.up[0], h->bx.up[1], h->bx.up[2]};
@@ -1554,6 +1555,21 @@ int tfg_vel_max(tfg_matrix* h, const double* u, const double* v, const double* w
   HIP_TRY(hipMemcpyAsync(vel_max, h->vmax + 1024, sizeof(double), hipMemcpyDeviceToHost,
                          h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  return TFG_OK;
+}
+
+int tfg_range(tfg_matrix* h, const double* v, double* vmin, double* vmax) {
+  if (!h || !v || !vmin || !vmax) return fail(TFG_EINVAL, "null argument");
+  HIP_TRY(hipSetDevice(h->device));
+  const int G = std::max(1, std::min((h->nt + tfg::kBlock - 1) / tfg::kBlock, 1024));
+  if (!h->vrange) HIP_TRY(hipMalloc(&h->vrange, sizeof(double) * (2 * 1024 + 2)));
+  HIP_TRY(tfg::launch_range(h->stream, h->nt, v, h->vrange, G, h->vrange + 2 * 1024));
+  double out[2];
+  HIP_TRY(hipMemcpyAsync(out, h->vrange + 2 * 1024, sizeof(out), hipMemcpyDeviceToHost,
+                         h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  *vmin = out[0];
+  *vmax = out[1];
   return TFG_OK;
 }
 

@@ -102,6 +102,37 @@ __global__ __launch_bounds__(kBlock) void k_vel_max_final(const double* __restri
   if (threadIdx.x == 0) out[0] = s_out[0];
 }
 
+// p_min, p_max over the cells (Compute_Pressure.f90:104-109; HUGE
+// starts, which a finite field always replaces)
+__global__ __launch_bounds__(kBlock) void k_range(int n, const double* __restrict__ v,
+                                                 double* __restrict__ part) {
+  __shared__ double s_tmp[2 * 4], s_out[2];
+  double m[2] = {kHuge, -kHuge};
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    m[0] = fmin(m[0], v[i]);
+    m[1] = fmax(m[1], v[i]);
+  }
+  const int ops[2] = {OP_MIN, OP_MAX};
+  publish<2>(m, ops, part, 1024, s_tmp, s_out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_range_final(const double* __restrict__ part, int G,
+                                                       double* out) {
+  __shared__ double s_tmp[2 * 4], s_out[2];
+  const int ops[2] = {OP_MIN, OP_MAX};
+  gather<2>(part, 1024, G, ops, s_tmp, s_out);
+  if (threadIdx.x < 2) out[threadIdx.x] = s_out[threadIdx.x];
+}
+
+hipError_t launch_range(hipStream_t stream, int n, const double* v, double* part, int G,
+                        double* out) {
+  if (G > 1024) return hipErrorInvalidValue;
+  k_range<<<G, kBlock, 0, stream>>>(n, v, part);
+  k_range_final<<<1, kBlock, 0, stream>>>(part, G, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_vel_max(hipStream_t stream, int n, const double* u, const double* v,
                           const double* w, double* part, int G, double* out) {
   k_vel_max<<<G, kBlock, 0, stream>>>(n, u, v, w, part);

@@ -334,6 +334,8 @@ hipError_t launch_jacobi_apply(hipStream_t s, int n, const int* row, const doubl
                                const double* r, double* z);
 
 hipError_t launch_assemble(hipStream_t stream, const AsmArgs& g);
+hipError_t launch_range(hipStream_t stream, int n, const double* v, double* part, int G,
+                        double* out);
 hipError_t launch_vel_max(hipStream_t stream, int n, const double* u, const double* v,
                           const double* w, double* part, int G, double* out);
 

@@ -193,6 +193,14 @@
       integer(c_int), value :: singular
     end function
 
+    ! p_min, p_max of Compute_Pressure from device p % n (before Min/Max_Real)
+    integer(c_int) function tfg_range(handle, v, vmin, vmax)                 &
+                                         bind(C, name='tfg_range')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr),    value       :: handle, v
+      real(c_double), intent(out) :: vmin, vmax
+    end function
+
     ! vel_max of Compute_Momentum from device u, v, w (before Max_Real)
     integer(c_int) function tfg_vel_max(handle, u, v, w, vel_max)            &
                                          bind(C, name='tfg_vel_max')

@@ -100,6 +100,8 @@ def lib():
         L.tfg_pressure_assemble.argtypes = [P, P, P, P, P, P, P, I]
         L.tfg_pressure_assemble_ex.argtypes = [P, P, P, P, P, P, P, P, I]
         L.tfg_vel_max.argtypes = [P, P, P, P, ctypes.POINTER(ctypes.c_double)]
+        L.tfg_range.argtypes = [P, P, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double)]
         L.tfg_momentum_assemble.argtypes = [P, P, P, P, P]
         L.tfg_matrix_reorder.argtypes = [P, I]
         L.tfg_rhie_chow_fluxes.argtypes = [P, P, P]
@@ -260,6 +262,12 @@ class Native:
                 raise TypeError(f"{name} must be a contiguous float64 array of length {n}")
         return self._solve(solver, prec, _ptr(val), _ptr(x), _ptr(b), miter, tol, norm, options,
                            report, device=False)
+
+    def value_range(self, dv):
+        """(min, max) of a device vector over the handle's nt cells (tfg_range)."""
+        lo, hi = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _check(lib().tfg_range(self.h, dv.ptr, ctypes.byref(lo), ctypes.byref(hi)))
+        return lo.value, hi.value
 
     def vel_max(self, du, dv, dw):
         """max(MICRO, max |u|) over the handle's nt cells (tfg_vel_max)."""

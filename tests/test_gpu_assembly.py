@@ -63,6 +63,19 @@ def test_vel_max(oracle, gpu, cases):
     assert nat.vel_max(z, z, z) == 1e-6
 
 
+def test_pressure_range(cases, gpu):
+    """Compute_Pressure.f90:104-109: p_min and p_max over the cells, the
+    pressure solve's norm range (a minimum and a maximum: numpy's exactly)."""
+    s = cases.hex_pressure(19, 17, 13)
+    nat = gpu.Native(s.row, s.col, s.dia)
+    p = np.random.default_rng(2).uniform(-3.0, 5.0, s.n)
+    p[123] = -7.5
+    p[s.n - 1] = 9.25
+    assert nat.value_range(gpu.DeviceArray(p)) == (-7.5, 9.25)
+    c = np.full(s.n, 0.125)
+    assert nat.value_range(gpu.DeviceArray(c)) == (0.125, 0.125)
+
+
 def test_solve_on_assembled_system(cases, oracle, gpu):
     f = cases.hex_faces(16, 14, 12)
     f.b_region[:] = 0
