@@ -2781,10 +2781,11 @@ int tfg_sync_bench(int grid, int reps, double* us_barrier, double* us_boundary) 
   if (!tfg::grid_barrier_fit(&per_cu) || grid > per_cu * cus)
     return fail(TFG_EINVAL, "%d workgroups cannot all be resident (at most %d)", grid, per_cu * cus);
   double* sink = nullptr;
-  HIP_TRY(hipMalloc(&sink, sizeof(double)));
-  hipEvent_t e[4];
-  for (hipEvent_t& x : e) HIP_TRY(hipEventCreate(&x));
-  hipError_t r = tfg::launch_grid_barriers(nullptr, grid, 1, sink);   // warm-up
+  hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipError_t r = hipMalloc(&sink, sizeof(double));
+  for (hipEvent_t& x : e)
+    if (r == hipSuccess) r = hipEventCreate(&x);
+  if (r == hipSuccess) r = tfg::launch_grid_barriers(nullptr, grid, 1, sink);   // warm-up
   if (r == hipSuccess) r = hipEventRecord(e[0], nullptr);
   if (r == hipSuccess) r = tfg::launch_grid_barriers(nullptr, grid, reps, sink);
   if (r == hipSuccess) r = hipEventRecord(e[1], nullptr);
@@ -2796,8 +2797,9 @@ int tfg_sync_bench(int grid, int reps, double* us_barrier, double* us_boundary) 
   float ms_b = 0.f, ms_k = 0.f;
   if (r == hipSuccess) r = hipEventElapsedTime(&ms_b, e[0], e[1]);
   if (r == hipSuccess) r = hipEventElapsedTime(&ms_k, e[2], e[3]);
-  for (hipEvent_t& x : e) (void)hipEventDestroy(x);
-  (void)hipFree(sink);
+  for (hipEvent_t& x : e)
+    if (x) (void)hipEventDestroy(x);
+  if (sink) (void)hipFree(sink);
   HIP_TRY(r);
   *us_barrier = 1e3 * ms_b / reps;
   *us_boundary = 1e3 * ms_k / reps;
