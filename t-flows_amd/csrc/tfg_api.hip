@@ -2750,24 +2750,25 @@ int tfg_stream_bench(void* d_dst, const void* d_src, long long bytes, int mode, 
   if (mode == 1 && bytes < (long long)grid * 8)
     return fail(TFG_EINVAL, "the read sweep writes %d sums to d_dst: it needs %lld bytes", grid,
                 (long long)grid * 8);
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
   auto once = [&]() -> hipError_t {
     if (mode == 0) return tfg::launch_copy16(nullptr, grid, d_src, d_dst, bytes / 16);
     if (mode == 1) return tfg::launch_read16(nullptr, grid, d_src, static_cast<double*>(d_dst), bytes / 16);
     return tfg::launch_fill16(nullptr, grid, d_dst, bytes / 16);
   };
-  HIP_TRY(once());
-  HIP_TRY(hipEventRecord(e0, nullptr));
-  for (int r = 0; r < reps; ++r) HIP_TRY(once());
-  HIP_TRY(hipEventRecord(e1, nullptr));
-  HIP_TRY(hipEventSynchronize(e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t r = hipEventCreate(&e0);
+  if (r == hipSuccess) r = hipEventCreate(&e1);
+  if (r == hipSuccess) r = once();   // warm-up
+  if (r == hipSuccess) r = hipEventRecord(e0, nullptr);
+  for (int k = 0; k < reps && r == hipSuccess; ++k) r = once();
+  if (r == hipSuccess) r = hipEventRecord(e1, nullptr);
+  if (r == hipSuccess) r = hipEventSynchronize(e1);
   float ms = 0.f;
-  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  if (r == hipSuccess) r = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  HIP_TRY(r);
   *ms_avg = ms / reps;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return TFG_OK;
 }
 
