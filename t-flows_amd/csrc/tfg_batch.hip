@@ -76,7 +76,7 @@ struct BatchArgs {
   const int* bsys;          // system of each block
   const int* sblk;          // nsys+1: first block of each system
   const int* srow;          // nsys+1: first row of each system
-  const int* act;           // active blocks
+  const int4v* act;         // active blocks: {block, first row, end row, system}
   const double* val;
   const double* b;
   double* x;
@@ -101,8 +101,19 @@ struct BatchCfg {
   int diag_mask;  // BatchArgs::diag_mask
 };
 
-__device__ __forceinline__ bool on_diagonals(const BatchArgs& a, int s, const SysDiag& g) {
-  return g.k > 0 && !a.bad[s] && ((a.diag_mask >> a.scls[s]) & 1);
+// What decides whether system s takes its products from its diagonals: its
+// SysDiag (k = 0 when no system is on diagonals), its asymmetry flag and
+// its size class, loaded together; on_diagonals() combines them.
+struct DiagInfo {
+  SysDiag g;
+  int bad, cls;
+};
+__device__ __forceinline__ DiagInfo diag_info(const BatchArgs& a, int s) {
+  if (!a.sdg) return DiagInfo{SysDiag{}, 0, 0};
+  return DiagInfo{a.sdg[s], a.bad[s], (int)a.scls[s]};
+}
+__device__ __forceinline__ bool on_diagonals(const BatchArgs& a, const DiagInfo& d) {
+  return d.g.k > 0 && !d.bad && ((a.diag_mask >> d.cls) & 1);
 }
 
 __device__ __forceinline__ double bmeasure(const State* st, double nrms, double xmin,
@@ -112,13 +123,32 @@ __device__ __forceinline__ double bmeasure(const State* st, double nrms, double 
   return sqrt(nrms) / range;
 }
 
-// Block of this workgroup in the active list, dealt to the XCDs in
-// contiguous ranges (workgroup g runs on XCD g % 8); -1 past the list.
-__device__ __forceinline__ int active_block(const BatchArgs& a) {
+// Entry of this workgroup in the active list, dealt to the XCDs in
+// contiguous ranges (workgroup g runs on XCD g % 8): {block, first row, end
+// row, system} in one 16-byte load, so a workgroup's row loads and its
+// system's state follow from one load; false past the list.
+struct ActEntry {
+  int bl, r0, r1, s;
+};
+// Takes the values here (an empty asm), so the loads that produced them are
+// issued before any branch that follows: a workgroup's row loads and its
+// system's state are then in flight together instead of one after another.
+__device__ __forceinline__ void hold1(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void hold1(double v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void hold1(int v) { asm volatile("" ::"v"(v)); }
+template <class... T>
+__device__ __forceinline__ void hold(T... v) {
+  (hold1(v), ...);
+}
+
+__device__ __forceinline__ bool active_entry(const BatchArgs& a, ActEntry& e) {
   const int g = blockIdx.x, x = g % kXcds, l = g / kXcds;
   const int first = (int)(((long long)a.nact * x) / kXcds);
   const int end = (int)(((long long)a.nact * (x + 1)) / kXcds);
-  return first + l < end ? a.act[first + l] : -1;
+  if (first + l >= end) return false;
+  const int4v v = a.act[first + l];
+  e = ActEntry{v.x, v.y, v.z, v.w};
+  return true;
 }
 
 // Sum (min, max) of one system's block partials, fixed order; s_out[k].
@@ -244,16 +274,23 @@ __global__ __launch_bounds__(kBlock) void k_bt_start(BatchArgs a) {
 }
 
 // p = z + beta p (p = z in a system's first iteration)      Cg.f90:145-171
+// (Row loads and the system's state are in flight together; a thread past
+// the block loads the block's last row and stores nothing.)
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_bt_pupdate(BatchArgs a) {
-  const int bl = active_block(a);
-  if (bl < 0) return;
-  const State* st = &a.st[a.bsys[bl]];
-  if (st->done) return;
-  const int i = a.m.blk[bl] + threadIdx.x;
-  if (i >= a.m.blk[bl + 1]) return;
-  const double z = PREC == PREC_JACOBI ? a.r[i] * (1.0 / a.ad[i]) : a.r[i];
-  a.p[i] = st->iter == 0 ? z : z + st->aux1 * a.p[i];
+  ActEntry e;
+  if (!active_entry(a, e)) return;
+  const int ti = e.r0 + (int)threadIdx.x;
+  const int i = min(ti, e.r1 - 1);
+  const double r = a.r[i], pv = a.p[i];
+  const double d = PREC == PREC_JACOBI ? a.ad[i] : 1.0;
+  const State* st = &a.st[e.s];
+  const int done = st->done, iter = st->iter;
+  const double beta = st->aux1;
+  hold(r, pv, d, done, iter, beta);
+  if (done || ti >= e.r1) return;
+  const double z = PREC == PREC_JACOBI ? r * (1.0 / d) : r;
+  a.p[i] = iter == 0 ? z : z + beta * pv;
 }
 
 // Row i of a system on diagonals (SysDiag): a_ii x_i, the lower entries from
@@ -289,18 +326,22 @@ template <class E>
 __global__ __launch_bounds__(kBlock) void k_bt_spmv(BatchArgs a) {
   __shared__ typename E::Smem sm;
   __shared__ double s_tmp[4], s_out[1];
-  const int bl = active_block(a);
-  if (bl < 0) return;
-  const int s = a.bsys[bl];
+  ActEntry e;
+  if (!active_entry(a, e)) return;
+  const int bl = e.bl, s = e.s;
   const State* st = &a.st[s];
-  if (st->done) return;
+  const int done = st->done;
   const double fn = st->fn;
+  const DiagInfo di = diag_info(a, s);
+  const SysDiag& g = di.g;
+  const int r0 = a.srow[s], r1 = a.srow[s + 1];
+  hold(done, fn, g.k, di.bad, di.cls, r0, r1);
+  const bool dia = on_diagonals(a, di);
+  if (done) return;
   double v[1] = {0.0};
-  const SysDiag g = a.sdg ? a.sdg[s] : SysDiag{};
-  if (on_diagonals(a, s, g)) {
-    const int i = a.m.blk[bl] + (int)threadIdx.x;
-    if (i < a.m.blk[bl + 1]) {
-      const int r0 = a.srow[s], r1 = a.srow[s + 1];
+  if (dia) {
+    const int i = e.r0 + (int)threadIdx.x;
+    if (i < e.r1) {
       double qi = 0.0;
       if (g.k == 1) qi = bt_dia_row<1>(a, g, i, r0, r1, fn);
       else if (g.k == 2) qi = bt_dia_row<2>(a, g, i, r0, r1, fn);
@@ -382,17 +423,20 @@ __global__ __launch_bounds__(kBlock) void k_bt_alpha(BatchArgs a) {
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_bt_update(BatchArgs a) {
   __shared__ double s_tmp[5 * 4], s_out[5];
-  const int bl = active_block(a);
-  if (bl < 0) return;
-  const State* st = &a.st[a.bsys[bl]];
-  if (st->done) return;
+  ActEntry e;
+  if (!active_entry(a, e)) return;
+  const int ti = e.r0 + (int)threadIdx.x;
+  const int i = min(ti, e.r1 - 1);
+  const double d = a.ad[i], x0 = a.x[i], p0 = a.p[i], r0 = a.r[i], q0 = a.q[i];
+  const State* st = &a.st[e.s];
+  const int done = st->done;
   const double alpha = st->alpha;
-  const int i = a.m.blk[bl] + threadIdx.x;
+  hold(d, x0, p0, r0, q0, done, alpha);
+  if (done) return;
   double v[5] = {0.0, kHuge, -kHuge, 0.0, 0.0};
-  if (i < a.m.blk[bl + 1]) {
-    const double d = a.ad[i];
-    const double xv = a.x[i] + alpha * a.p[i];
-    const double rv = a.r[i] - alpha * a.q[i];
+  if (ti < e.r1) {
+    const double xv = x0 + alpha * p0;
+    const double rv = r0 - alpha * q0;
     a.x[i] = xv;
     a.r[i] = rv;
     const double z = PREC == PREC_JACOBI ? rv * (1.0 / d) : rv;
@@ -403,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_update(BatchArgs a) {
     v[4] = rv * rv;
   }
   const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
-  block_publish<5>(a, BP_UPD, bl, v, ops, s_tmp, s_out);
+  block_publish<5>(a, BP_UPD, e.bl, v, ops, s_tmp, s_out);
 }
 
 // stopping test of iteration k (Cg.f90:214-222), next rho and beta
@@ -560,12 +604,12 @@ __global__ __launch_bounds__(kBlock) void k_bt32_sysinit(BatchArgs a) {
 // x += e + alpha p (the correction with the last iteration's step) when a
 // refinement is due
 __global__ __launch_bounds__(kBlock) void k_bt32_fold(BatchArgs a) {
-  const int bl = active_block(a);
-  if (bl < 0) return;
-  const State* st = &a.st[a.bsys[bl]];
+  ActEntry e;
+  if (!active_entry(a, e)) return;
+  const State* st = &a.st[e.s];
   if (st->done || !st->refine) return;
-  const int i = a.m.blk[bl] + threadIdx.x;
-  if (i >= a.m.blk[bl + 1]) return;
+  const int i = e.r0 + (int)threadIdx.x;
+  if (i >= e.r1) return;
   const float ap = (float)st->alpha;
   a.x[i] = a.x[i] + (double)(a.f.e32[i] + ap * a.f.p32[i]);
   a.f.e32[i] = 0.0f;
@@ -577,9 +621,10 @@ template <class E>
 __global__ __launch_bounds__(kBlock) void k_bt32_resid(BatchArgs a) {
   __shared__ typename E::Smem sm;
   __shared__ double s_tmp[5 * 4], s_out[5];
-  const int bl = active_block(a);
-  if (bl < 0) return;
-  const State* st = &a.st[a.bsys[bl]];
+  ActEntry e;
+  if (!active_entry(a, e)) return;
+  const int bl = e.bl;
+  const State* st = &a.st[e.s];
   if (st->done || !st->refine) return;
   const double fn = st->fn;
   double v[5] = {0.0, kHuge, -kHuge, 0.0, 0.0};
@@ -632,16 +677,17 @@ __global__ __launch_bounds__(kBlock) void k_bt32_test(BatchArgs a) {
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_bt32_restart(BatchArgs a) {
   __shared__ double s_tmp[4 * 4], s_out[4];
-  const int bl = active_block(a);
-  if (bl < 0) return;
-  const State* st = &a.st[a.bsys[bl]];
+  ActEntry e;
+  if (!active_entry(a, e)) return;
+  const int bl = e.bl;
+  const State* st = &a.st[e.s];
   if (st->done || !st->refine) return;
   const double c = st->corr;
   const bool defl = st->defl_e > 0.0;
   const bool nrm = st->criterion != TFG_CRIT_RELRES;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  const int i = a.m.blk[bl] + (int)threadIdx.x;
-  if (i < a.m.blk[bl + 1]) {
+  const int i = e.r0 + (int)threadIdx.x;
+  if (i < e.r1) {
     const float sv = a.f.s32[i], d = a.f.rd32[i];
     if (defl) a.x[i] = a.x[i] + c;
     const float r = (float)(a.r[i] - c * a.f.s64[i]);
@@ -689,22 +735,25 @@ __global__ __launch_bounds__(kBlock) void k_bt32_rstart(BatchArgs a) {
 // deferred e += alpha_{k-1} p_{k-1} on the p read anyway
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_bt32_pupdate(BatchArgs a) {
-  const int bl = active_block(a);
-  if (bl < 0) return;
-  const State* st = &a.st[a.bsys[bl]];
-  if (st->done) return;
-  const int i = a.m.blk[bl] + threadIdx.x;
-  if (i >= a.m.blk[bl + 1]) return;
-  const bool fresh = st->iter == st->kref;
-  const float mu = (float)st->aux2;
-  const float r = a.f.r32[i];
-  const float z = PREC == PREC_JACOBI ? r * a.f.rd32[i] : r;
+  ActEntry e;
+  if (!active_entry(a, e)) return;
+  const int ti = e.r0 + (int)threadIdx.x;
+  const int i = min(ti, e.r1 - 1);
+  const float r = a.f.r32[i], pv = a.f.p32[i], ev = a.f.e32[i];
+  const float d = PREC == PREC_JACOBI ? a.f.rd32[i] : 1.0f;
+  const State* st = &a.st[e.s];
+  const int done = st->done, iter = st->iter, kref = st->kref;
+  const double mu64 = st->aux2, beta64 = st->aux1, alpha64 = st->alpha;
+  hold(r, pv, ev, d, done, iter, kref, mu64, beta64, alpha64);
+  if (done || ti >= e.r1) return;
+  const bool fresh = iter == kref;
+  const float mu = (float)mu64;
+  const float z = PREC == PREC_JACOBI ? r * d : r;
   if (fresh) {
     a.f.p32[i] = z - mu;
   } else {
-    const float beta = (float)st->aux1, ap = (float)st->alpha;
-    const float pv = a.f.p32[i];
-    a.f.e32[i] = a.f.e32[i] + ap * pv;
+    const float beta = (float)beta64, ap = (float)alpha64;
+    a.f.e32[i] = ev + ap * pv;
     a.f.p32[i] = (z + beta * pv) - mu;
   }
 }
@@ -742,17 +791,21 @@ template <class E>
 __global__ __launch_bounds__(kBlock) void k_bt32_spmv(BatchArgs a) {
   __shared__ typename E::Smem sm;
   __shared__ double s_tmp[4], s_out[1];
-  const int bl = active_block(a);
-  if (bl < 0) return;
-  const int s = a.bsys[bl];
+  ActEntry e;
+  if (!active_entry(a, e)) return;
+  const int bl = e.bl, s = e.s;
   const State* st = &a.st[s];
-  if (st->done) return;
+  const int done = st->done;
+  const DiagInfo di = diag_info(a, s);
+  const SysDiag& g = di.g;
+  const int r0 = a.srow[s], r1 = a.srow[s + 1];
+  hold(done, g.k, di.bad, di.cls, r0, r1);
+  const bool dia = on_diagonals(a, di);
+  if (done) return;
   double v[1] = {0.0};
-  const SysDiag g = a.sdg ? a.sdg[s] : SysDiag{};
-  if (on_diagonals(a, s, g)) {
-    const int i = a.m.blk[bl] + (int)threadIdx.x;
-    if (i < a.m.blk[bl + 1]) {
-      const int r0 = a.srow[s], r1 = a.srow[s + 1];
+  if (dia) {
+    const int i = e.r0 + (int)threadIdx.x;
+    if (i < e.r1) {
       float qi;
       if (g.k == 1) qi = bt32_dia_row<1>(a, g, i, r0, r1);
       else if (g.k == 2) qi = bt32_dia_row<2>(a, g, i, r0, r1);
@@ -799,22 +852,26 @@ __global__ __launch_bounds__(kBlock) void k_bt32_alpha(BatchArgs a) {
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_bt32_update(BatchArgs a) {
   __shared__ double s_tmp[4 * 4], s_out[4];
-  const int bl = active_block(a);
-  if (bl < 0) return;
-  const State* st = &a.st[a.bsys[bl]];
-  if (st->done) return;
-  const float a32 = (float)st->alpha;
-  const bool defl = st->defl_e > 0.0;
-  const bool nrm = st->criterion != TFG_CRIT_RELRES;
+  ActEntry e;
+  if (!active_entry(a, e)) return;
+  const int ti = e.r0 + (int)threadIdx.x;
+  const int i = min(ti, e.r1 - 1);
+  const float d = a.f.rd32[i], r0 = a.f.r32[i], q0 = a.f.q32[i], sv = a.f.s32[i];
+  const State* st = &a.st[e.s];
+  const int done = st->done, crit = st->criterion;
+  const double alpha64 = st->alpha, defl_e = st->defl_e;
+  hold(d, r0, q0, sv, done, crit, alpha64, defl_e);
+  if (done) return;
+  const float a32 = (float)alpha64;
+  const bool defl = defl_e > 0.0;
+  const bool nrm = crit != TFG_CRIT_RELRES;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  const int i = a.m.blk[bl] + (int)threadIdx.x;
-  if (i < a.m.blk[bl + 1]) {
-    const float d = a.f.rd32[i];
-    const float r = a.f.r32[i] - a32 * a.f.q32[i];
+  if (ti < e.r1) {
+    const float r = r0 - a32 * q0;
     a.f.r32[i] = r;
     const float z = PREC == PREC_JACOBI ? r * d : r;
     v[0] = (double)z * (double)r;
-    v[1] = defl ? (double)a.f.s32[i] * (double)z : 0.0;
+    v[1] = defl ? (double)sv * (double)z : 0.0;
     v[2] = (double)r * (double)r;
     if (nrm) {
       const double t = (double)(r * d);
@@ -822,7 +879,7 @@ __global__ __launch_bounds__(kBlock) void k_bt32_update(BatchArgs a) {
     }
   }
   const int ops[4] = {OP_SUM, OP_SUM, OP_SUM, OP_SUM};
-  block_publish<4>(a, BP_UPD, bl, v, ops, s_tmp, s_out);
+  block_publish<4>(a, BP_UPD, e.bl, v, ops, s_tmp, s_out);
 }
 
 // Per system, end of iteration k: beta and mu of the next iteration and, at
@@ -884,8 +941,10 @@ struct tfg_batch {
   int device = 0, nsys = 0, nt = 0, nnz = 0, nblk = 0, cu_count = 0;
   hipStream_t stream = nullptr;
   std::vector<int> h_srow, h_sblk;  // host copies of the system offsets
+  std::vector<int> h_blk, h_bsys;   // host copies of the row blocks and their systems
   int *row = nullptr, *col = nullptr, *blk = nullptr, *bsys = nullptr, *sblk = nullptr,
-      *srow = nullptr, *act = nullptr;
+      *srow = nullptr;
+  tfg::int4v* act = nullptr;        // active list entries {block, first row, end row, system}
   double *val = nullptr, *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr,
          *ad = nullptr, *part = nullptr;
   tfg::State* st = nullptr;
@@ -1070,10 +1129,18 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
   HIP_TRY(hipGetLastError());
 
   // active list: every block; rebuilt from the polled state as systems stop
-  std::vector<int> act(h->nblk);
-  for (int t = 0; t < h->nblk; ++t) act[t] = t;
+  auto entry = [&](int t) {
+    int4v v;
+    v.x = t;
+    v.y = h->h_blk[t];
+    v.z = h->h_blk[t + 1];
+    v.w = h->h_bsys[t];
+    return v;
+  };
+  std::vector<int4v> act(h->nblk);
+  for (int t = 0; t < h->nblk; ++t) act[t] = entry(t);
   a.nact = h->nblk;
-  HIP_TRY(hipMemcpyAsync(h->act, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, S));
+  HIP_TRY(hipMemcpyAsync(h->act, act.data(), sizeof(int4v) * act.size(), hipMemcpyHostToDevice, S));
   HIP_TRY(hipStreamSynchronize(S));
 
 #define TFG_BT32_ENGINE(CALL)                                               \
@@ -1148,7 +1215,7 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
   // runs (stopped systems are masked on the device, so the lag only costs
   // the masked launches of one chunk).
   int k = 0, slot = 0, prev = -1;
-  std::vector<int> nact;
+  std::vector<int4v> nact;
   while (k < miter) {
     const int cnt = std::min(chunk, miter - k);
     if (use_graph && cnt == chunk) HIP_TRY(hipGraphLaunch(exec, S));
@@ -1165,14 +1232,14 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
       for (int s = 0; s < h->nsys; ++s) {
         if (!ps[s].done) {
           ++running;
-          for (int t = h->h_sblk[s]; t < h->h_sblk[s + 1]; ++t) nact.push_back(t);
+          for (int t = h->h_sblk[s]; t < h->h_sblk[s + 1]; ++t) nact.push_back(entry(t));
         }
       }
       if (!running) break;
       // compaction once an eighth of the active blocks have stopped (the
       // copy is stream-ordered after the chunk already queued)
       if ((int)nact.size() * 8 <= a.nact * 7) {
-        HIP_TRY(hipMemcpyAsync(h->act, nact.data(), sizeof(int) * nact.size(), hipMemcpyHostToDevice, S));
+        HIP_TRY(hipMemcpyAsync(h->act, nact.data(), sizeof(int4v) * nact.size(), hipMemcpyHostToDevice, S));
         HIP_TRY(hipStreamSynchronize(S));
         a.nact = (int)nact.size();
         if (use_graph) {
@@ -1319,6 +1386,8 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
   h->nblk = (int)bsys.size();
   h->h_srow = srow;
   h->h_sblk = sblk;
+  h->h_bsys = bsys;
+  h->h_blk = blk;
   h->done_ms.assign(nsys, 0.0);
   int rc = TFG_OK;
   do {
@@ -1333,7 +1402,7 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
     dmal((void**)&h->bsys, sizeof(int) * bsys.size());
     dmal((void**)&h->sblk, sizeof(int) * sblk.size());
     dmal((void**)&h->srow, sizeof(int) * srow.size());
-    dmal((void**)&h->act, sizeof(int) * bsys.size());
+    dmal((void**)&h->act, sizeof(tfg::int4v) * bsys.size());
     for (double** v : {&h->x, &h->r, &h->p, &h->q, &h->ad}) dmal((void**)v, sizeof(double) * N);
     dmal((void**)&h->part, sizeof(double) * (size_t)tfg::BP_ROWS * bsys.size());
     dmal((void**)&h->st, sizeof(tfg::State) * (size_t)nsys);
