@@ -33,22 +33,28 @@ __device__ __forceinline__ double red_identity(int op) {
   return op == OP_SUM ? 0.0 : (op == OP_MIN ? INFINITY : -INFINITY);
 }
 
-__device__ __forceinline__ double wave_reduce(double v, int op) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = red_apply(op, v, __shfl_xor(v, off, 64));
-  return v;
-}
-
 // Reduce NV values over the workgroup; on return s_out[k] holds the totals
 // (visible to all threads).  s_tmp needs NV * (kBlock/64) doubles.
 template <int NV>
 __device__ __forceinline__ void block_reduce(double (&v)[NV], const int (&ops)[NV],
                                              double* s_tmp, double* s_out) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  // xor butterfly inside the wavefront (lane distance 32, 16, .., 1), all NV
+  // values per step, so a step's NV lane exchanges are in flight together
+  double w[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double w = wave_reduce(v[k], ops[k]);
-    if (lane == 0) s_tmp[k * (kBlock / 64) + wid] = w;
+  for (int k = 0; k < NV; ++k) w[k] = v[k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) o[k] = __shfl_xor(w[k], off, 64);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) w[k] = red_apply(ops[k], w[k], o[k]);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s_tmp[k * (kBlock / 64) + wid] = w[k];
   }
   __syncthreads();
   if (threadIdx.x < NV) {
