@@ -33,6 +33,47 @@ __device__ __forceinline__ double red_identity(int op) {
   return op == OP_SUM ? 0.0 : (op == OP_MIN ? INFINITY : -INFINITY);
 }
 
+// Value of lane (lane ^ OFF) inside the wavefront.  Distances 32 and 16
+// cross rows of 16 lanes and go through the LDS crossbar (ds_bpermute);
+// 8, 4, 2, 1 stay inside a row and are DPP moves on the VALU: row rotate by
+// 8; half-row mirror then quad reverse (lane ^ 7 ^ 3); quad permutes.  Every
+// lane of the wavefront must be active.
+template <int OFF>
+__device__ __forceinline__ int xor_lane_b32(int v) {
+  if constexpr (OFF == 8) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);   // row_ror:8
+  } else if constexpr (OFF == 4) {
+    const int t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    return __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, false);         // quad_perm 3,2,1,0
+  } else if constexpr (OFF == 2) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+  } else {
+    static_assert(OFF == 1, "DPP distances are 8, 4, 2, 1");
+    return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+  }
+}
+template <int OFF>
+__device__ __forceinline__ double xor_lane(double v) {
+  if constexpr (OFF >= 16) {
+    return __shfl_xor(v, OFF, 64);
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = xor_lane_b32<OFF>((int)(unsigned)(b & 0xffffffffLL));
+    const int hi = xor_lane_b32<OFF>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  }
+}
+
+// One butterfly step on NV values: all NV exchanges, then all NV combines.
+template <int OFF, int NV>
+__device__ __forceinline__ void butterfly_step(double (&w)[NV], const int (&ops)[NV]) {
+  double o[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) o[k] = xor_lane<OFF>(w[k]);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) w[k] = red_apply(ops[k], w[k], o[k]);
+}
+
 // Reduce NV values over the workgroup; on return s_out[k] holds the totals
 // (visible to all threads).  s_tmp needs NV * (kBlock/64) doubles.
 template <int NV>
@@ -44,14 +85,12 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], const int (&ops)[N
   double w[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) w[k] = v[k];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    double o[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) o[k] = __shfl_xor(w[k], off, 64);
-#pragma unroll
-    for (int k = 0; k < NV; ++k) w[k] = red_apply(ops[k], w[k], o[k]);
-  }
+  butterfly_step<32>(w, ops);
+  butterfly_step<16>(w, ops);
+  butterfly_step<8>(w, ops);
+  butterfly_step<4>(w, ops);
+  butterfly_step<2>(w, ops);
+  butterfly_step<1>(w, ops);
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) s_tmp[k * (kBlock / 64) + wid] = w[k];
