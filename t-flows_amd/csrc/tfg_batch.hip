@@ -77,6 +77,9 @@ struct BatchArgs {
   const int* sblk;          // nsys+1: first block of each system
   const int* srow;          // nsys+1: first row of each system
   const int4v* act;         // active blocks: {block, first row, end row, system}
+  const int4v* act_dia;     // the active blocks whose system takes its products from
+  const int4v* act_csr;     // its diagonals (SysDiag), and the others
+  int nact_dia, nact_csr;
   const double* val;
   const double* b;
   double* x;
@@ -91,30 +94,14 @@ struct BatchArgs {
   double* sd;               // the diagonals of the systems with k > 0
   int* bad;                 // per system: 1 when this solve's val is not symmetric
   Bt32 f;                   // FP32 form
-  const uint8_t* scls;      // size class of each system (0: < 32768 rows, 1: < 262144, 2)
-  int diag_mask;            // bit c: systems of class c on diagonals use the diagonals
 };
 
 // Launch choices of a batched solve (chosen by the batch autotuner).
 struct BatchCfg {
   int chunk;      // iterations per replayed graph (FP64)
-  int diag_mask;  // BatchArgs::diag_mask
+  int diag_mask;  // bit c: systems of class c on diagonals use the diagonals
 };
 
-// What decides whether system s takes its products from its diagonals: its
-// SysDiag (k = 0 when no system is on diagonals), its asymmetry flag and
-// its size class, loaded together; on_diagonals() combines them.
-struct DiagInfo {
-  SysDiag g;
-  int bad, cls;
-};
-__device__ __forceinline__ DiagInfo diag_info(const BatchArgs& a, int s) {
-  if (!a.sdg) return DiagInfo{SysDiag{}, 0, 0};
-  return DiagInfo{a.sdg[s], a.bad[s], (int)a.scls[s]};
-}
-__device__ __forceinline__ bool on_diagonals(const BatchArgs& a, const DiagInfo& d) {
-  return d.g.k > 0 && !d.bad && ((a.diag_mask >> d.cls) & 1);
-}
 
 __device__ __forceinline__ double bmeasure(const State* st, double nrms, double xmin,
                                            double xmax, double rr) {
@@ -141,14 +128,17 @@ __device__ __forceinline__ void hold(T... v) {
   (hold1(v), ...);
 }
 
-__device__ __forceinline__ bool active_entry(const BatchArgs& a, ActEntry& e) {
+__device__ __forceinline__ bool list_entry(const int4v* list, int nact, ActEntry& e) {
   const int g = blockIdx.x, x = g % kXcds, l = g / kXcds;
-  const int first = (int)(((long long)a.nact * x) / kXcds);
-  const int end = (int)(((long long)a.nact * (x + 1)) / kXcds);
+  const int first = (int)(((long long)nact * x) / kXcds);
+  const int end = (int)(((long long)nact * (x + 1)) / kXcds);
   if (first + l >= end) return false;
-  const int4v v = a.act[first + l];
+  const int4v v = list[first + l];
   e = ActEntry{v.x, v.y, v.z, v.w};
   return true;
+}
+__device__ __forceinline__ bool active_entry(const BatchArgs& a, ActEntry& e) {
+  return list_entry(a.act, a.nact, e);
 }
 
 // Sum (min, max) of one system's block partials, fixed order; s_out[k].
@@ -322,45 +312,53 @@ __device__ __forceinline__ double bt_dia_row(const BatchArgs& a, const SysDiag& 
 }
 
 // q = (A fn) p and the block's (p, q)                       Cg.f90:176-195
+// Blocks of systems on diagonals (act_dia): no LDS staging, so the kernel
+// runs at full occupancy.
+__global__ __launch_bounds__(kBlock) void k_bt_spmv_dia(BatchArgs a) {
+  __shared__ double s_tmp[4], s_out[1];
+  ActEntry e;
+  if (!list_entry(a.act_dia, a.nact_dia, e)) return;
+  const int s = e.s;
+  const State* st = &a.st[s];
+  const int done = st->done;
+  const double fn = st->fn;
+  const SysDiag g = a.sdg[s];
+  const int r0 = a.srow[s], r1 = a.srow[s + 1];
+  hold(done, fn, g.k, r0, r1);
+  if (done) return;
+  double v[1] = {0.0};
+  const int i = e.r0 + (int)threadIdx.x;
+  if (i < e.r1) {
+    double qi = 0.0;
+    if (g.k == 1) qi = bt_dia_row<1>(a, g, i, r0, r1, fn);
+    else if (g.k == 2) qi = bt_dia_row<2>(a, g, i, r0, r1, fn);
+    else qi = bt_dia_row<3>(a, g, i, r0, r1, fn);
+    a.q[i] = qi;
+    v[0] = v[0] + a.p[i] * qi;
+  }
+  const int ops[1] = {OP_SUM};
+  block_publish<1>(a, BP_SPMV, e.bl, v, ops, s_tmp, s_out);
+}
+
+// The other blocks (act_csr): the CSR row engine E
 template <class E>
 __global__ __launch_bounds__(kBlock) void k_bt_spmv(BatchArgs a) {
   __shared__ typename E::Smem sm;
   __shared__ double s_tmp[4], s_out[1];
   ActEntry e;
-  if (!active_entry(a, e)) return;
-  const int bl = e.bl, s = e.s;
-  const State* st = &a.st[s];
-  const int done = st->done;
+  if (!list_entry(a.act_csr, a.nact_csr, e)) return;
+  const State* st = &a.st[e.s];
+  if (st->done) return;
   const double fn = st->fn;
-  const DiagInfo di = diag_info(a, s);
-  const SysDiag& g = di.g;
-  const int r0 = a.srow[s], r1 = a.srow[s + 1];
-  hold(done, fn, g.k, di.bad, di.cls, r0, r1);
-  const bool dia = on_diagonals(a, di);
-  if (done) return;
   double v[1] = {0.0};
-  if (dia) {
-    const int i = e.r0 + (int)threadIdx.x;
-    if (i < e.r1) {
-      double qi = 0.0;
-      if (g.k == 1) qi = bt_dia_row<1>(a, g, i, r0, r1, fn);
-      else if (g.k == 2) qi = bt_dia_row<2>(a, g, i, r0, r1, fn);
-      else qi = bt_dia_row<3>(a, g, i, r0, r1, fn);
-      a.q[i] = qi;
-      v[0] = v[0] + a.p[i] * qi;
-    }
-    const int ops[1] = {OP_SUM};
-    block_publish<1>(a, BP_SPMV, bl, v, ops, s_tmp, s_out);
-    return;
-  }
   E::template one<false>(
-      sm, a.m, a.val, fn, a.p, s_tmp, s_out, bl, [](int) { return 0.0; },
+      sm, a.m, a.val, fn, a.p, s_tmp, s_out, e.bl, [](int) { return 0.0; },
       [&](int i, double qi, double pi, double) {
         a.q[i] = qi;
         v[0] = v[0] + pi * qi;
       });
   const int ops[1] = {OP_SUM};
-  block_publish<1>(a, BP_SPMV, bl, v, ops, s_tmp, s_out);
+  block_publish<1>(a, BP_SPMV, e.bl, v, ops, s_tmp, s_out);
 }
 
 // The diagonals of every system with k > 0 from val, one thread per row:
@@ -786,45 +784,50 @@ __device__ __forceinline__ float bt32_dia_row(const BatchArgs& a, const SysDiag&
   return acc;
 }
 
-// q = A32 p and the block's (p, q)
+// q = A32 p and the block's (p, q): blocks of systems on diagonals
+__global__ __launch_bounds__(kBlock) void k_bt32_spmv_dia(BatchArgs a) {
+  __shared__ double s_tmp[4], s_out[1];
+  ActEntry e;
+  if (!list_entry(a.act_dia, a.nact_dia, e)) return;
+  const int s = e.s;
+  const State* st = &a.st[s];
+  const int done = st->done;
+  const SysDiag g = a.sdg[s];
+  const int r0 = a.srow[s], r1 = a.srow[s + 1];
+  hold(done, g.k, r0, r1);
+  if (done) return;
+  double v[1] = {0.0};
+  const int i = e.r0 + (int)threadIdx.x;
+  if (i < e.r1) {
+    float qi;
+    if (g.k == 1) qi = bt32_dia_row<1>(a, g, i, r0, r1);
+    else if (g.k == 2) qi = bt32_dia_row<2>(a, g, i, r0, r1);
+    else qi = bt32_dia_row<3>(a, g, i, r0, r1);
+    a.f.q32[i] = qi;
+    v[0] = (double)a.f.p32[i] * (double)qi;
+  }
+  const int ops[1] = {OP_SUM};
+  block_publish<1>(a, BP_SPMV, e.bl, v, ops, s_tmp, s_out);
+}
+
+// the other blocks: the CSR row engine E on the FP32 row-excess values
 template <class E>
 __global__ __launch_bounds__(kBlock) void k_bt32_spmv(BatchArgs a) {
   __shared__ typename E::Smem sm;
   __shared__ double s_tmp[4], s_out[1];
   ActEntry e;
-  if (!active_entry(a, e)) return;
-  const int bl = e.bl, s = e.s;
-  const State* st = &a.st[s];
-  const int done = st->done;
-  const DiagInfo di = diag_info(a, s);
-  const SysDiag& g = di.g;
-  const int r0 = a.srow[s], r1 = a.srow[s + 1];
-  hold(done, g.k, di.bad, di.cls, r0, r1);
-  const bool dia = on_diagonals(a, di);
-  if (done) return;
+  if (!list_entry(a.act_csr, a.nact_csr, e)) return;
+  const State* st = &a.st[e.s];
+  if (st->done) return;
   double v[1] = {0.0};
-  if (dia) {
-    const int i = e.r0 + (int)threadIdx.x;
-    if (i < e.r1) {
-      float qi;
-      if (g.k == 1) qi = bt32_dia_row<1>(a, g, i, r0, r1);
-      else if (g.k == 2) qi = bt32_dia_row<2>(a, g, i, r0, r1);
-      else qi = bt32_dia_row<3>(a, g, i, r0, r1);
-      a.f.q32[i] = qi;
-      v[0] = (double)a.f.p32[i] * (double)qi;
-    }
-    const int ops[1] = {OP_SUM};
-    block_publish<1>(a, BP_SPMV, bl, v, ops, s_tmp, s_out);
-    return;
-  }
   E::template one<false, true>(
-      sm, a.m, a.f.val32, 1.0f, a.f.p32, s_tmp, s_out, bl, [](int) { return 0.0f; },
+      sm, a.m, a.f.val32, 1.0f, a.f.p32, s_tmp, s_out, e.bl, [](int) { return 0.0f; },
       [&](int i, float qi, float pi, float) {
         a.f.q32[i] = qi;
         v[0] = v[0] + (double)pi * (double)qi;
       });
   const int ops[1] = {OP_SUM};
-  block_publish<1>(a, BP_SPMV, bl, v, ops, s_tmp, s_out);
+  block_publish<1>(a, BP_SPMV, e.bl, v, ops, s_tmp, s_out);
 }
 
 // alpha = rho_k / (p, q) per system (FP64)
@@ -959,7 +962,10 @@ struct tfg_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_poll[2] = {nullptr, nullptr};
   tfg::Bt32 f{};                      // FP32 form (allocated on first use)
   bool sd32_zeroed = false;
-  uint8_t* scls = nullptr;            // size class of each system (device)
+  std::vector<uint8_t> h_scls;        // size class of each system (0: < 32768 rows, 1: < 262144, 2)
+  std::vector<int> h_sdk;             // diagonals of each system (SysDiag::k; 0: CSR rows only)
+  std::vector<int> h_bad;             // bad flags of the last solve, read back from the device
+  tfg::int4v *act_dia = nullptr, *act_csr = nullptr;  // the active list split by product
   bool class_diag[3] = {false, false, false};  // a class holds a system on diagonals
   // batch autotuner, per precision (FP64, FP32): choice, probes, best probe
   bool tuned[2] = {false, false};
@@ -1018,7 +1024,7 @@ void batch_free(tfg_batch* h) {
   void* dev[] = {h->row, h->col, h->blk, h->bsys, h->sblk, h->srow, h->act, h->val, h->b,
                  h->x, h->r, h->p, h->q, h->ad, h->part, h->st, h->clk, h->sdg, h->sd, h->bad,
                  h->f.val32, h->f.sd32, h->f.s64, h->f.s32, h->f.rd32, h->f.r32, h->f.p32,
-                 h->f.q32, h->f.e32, h->f.asym, h->scls, h->x0};
+                 h->f.q32, h->f.e32, h->f.asym, h->act_dia, h->act_csr, h->x0};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   if (h->hst) (void)hipHostFree(h->hst);
@@ -1100,8 +1106,6 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
     a.bad = h->bad;
   }
   a.f = h->f;
-  a.scls = h->scls;
-  a.diag_mask = cfg.diag_mask;
   const bool vec = ((uintptr_t)dval % 16) == 0 && h->nnz >= 4;
   const bool vec32 = h->nnz >= 4;  // val32 is the library's, 16-byte aligned
   const hipStream_t S = h->stream;
@@ -1128,20 +1132,57 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
   }
   HIP_TRY(hipGetLastError());
 
-  // active list: every block; rebuilt from the polled state as systems stop
-  auto entry = [&](int t) {
-    int4v v;
-    v.x = t;
-    v.y = h->h_blk[t];
-    v.z = h->h_blk[t + 1];
-    v.w = h->h_bsys[t];
-    return v;
+  // Which systems take their products from their diagonals: on diagonals,
+  // values symmetric in this solve (the flags k_bt_sympack set), class
+  // enabled by cfg.diag_mask.
+  std::vector<char> dia(h->nsys, 0);
+  if (a.sdg) {
+    h->h_bad.resize(h->nsys);
+    HIP_TRY(hipMemcpyAsync(h->h_bad.data(), h->bad, sizeof(int) * (size_t)h->nsys,
+                           hipMemcpyDeviceToHost, S));
+    HIP_TRY(hipStreamSynchronize(S));
+    for (int s = 0; s < h->nsys; ++s)
+      dia[s] = h->h_sdk[s] > 0 && !h->h_bad[s] && ((cfg.diag_mask >> h->h_scls[s]) & 1);
+  }
+  // Active lists (every block of the running systems; all of them, those on
+  // diagonals, the others), rebuilt from the polled state as systems stop.
+  std::vector<int4v> act, act_d, act_c;
+  auto build_lists = [&](const State* ps) -> int {
+    act.clear();
+    act_d.clear();
+    act_c.clear();
+    for (int s = 0; s < h->nsys; ++s) {
+      if (ps && ps[s].done) continue;
+      for (int t = h->h_sblk[s]; t < h->h_sblk[s + 1]; ++t) {
+        int4v v;
+        v.x = t;
+        v.y = h->h_blk[t];
+        v.z = h->h_blk[t + 1];
+        v.w = s;
+        act.push_back(v);
+        (dia[s] ? act_d : act_c).push_back(v);
+      }
+    }
+    return (int)act.size();
   };
-  std::vector<int4v> act(h->nblk);
-  for (int t = 0; t < h->nblk; ++t) act[t] = entry(t);
-  a.nact = h->nblk;
-  HIP_TRY(hipMemcpyAsync(h->act, act.data(), sizeof(int4v) * act.size(), hipMemcpyHostToDevice, S));
-  HIP_TRY(hipStreamSynchronize(S));
+  auto upload_lists = [&]() -> int {
+    HIP_TRY(hipMemcpyAsync(h->act, act.data(), sizeof(int4v) * act.size(), hipMemcpyHostToDevice, S));
+    if (!act_d.empty())
+      HIP_TRY(hipMemcpyAsync(h->act_dia, act_d.data(), sizeof(int4v) * act_d.size(),
+                             hipMemcpyHostToDevice, S));
+    if (!act_c.empty())
+      HIP_TRY(hipMemcpyAsync(h->act_csr, act_c.data(), sizeof(int4v) * act_c.size(),
+                             hipMemcpyHostToDevice, S));
+    HIP_TRY(hipStreamSynchronize(S));
+    a.nact = (int)act.size();
+    a.nact_dia = (int)act_d.size();
+    a.nact_csr = (int)act_c.size();
+    return TFG_OK;
+  };
+  a.act_dia = h->act_dia;
+  a.act_csr = h->act_csr;
+  build_lists(nullptr);
+  { const int ru = upload_lists(); if (ru != TFG_OK) return ru; }
 
 #define TFG_BT32_ENGINE(CALL)                                               \
   do {                                                                      \
@@ -1163,11 +1204,14 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
   };
   auto enqueue_chunk = [&](int count) -> hipError_t {
     const int G = kXcds * ((a.nact + kXcds - 1) / kXcds);
+    const int Gd = kXcds * ((a.nact_dia + kXcds - 1) / kXcds);
+    const int Gc = kXcds * ((a.nact_csr + kXcds - 1) / kXcds);
     if (fp32) {
       for (int c = 0; c < count; ++c) {
         if (pr == PREC_JACOBI) k_bt32_pupdate<PREC_JACOBI><<<G, B, 0, S>>>(a);
         else k_bt32_pupdate<PREC_NONE><<<G, B, 0, S>>>(a);
-        TFG_BT32_ENGINE((k_bt32_spmv<E><<<G, B, 0, S>>>(a)));
+        if (Gd) k_bt32_spmv_dia<<<Gd, B, 0, S>>>(a);
+        if (Gc) TFG_BT32_ENGINE((k_bt32_spmv<E><<<Gc, B, 0, S>>>(a)));
         k_bt32_alpha<<<a.nsys, B, 0, S>>>(a);
         if (pr == PREC_JACOBI) k_bt32_update<PREC_JACOBI><<<G, B, 0, S>>>(a);
         else k_bt32_update<PREC_NONE><<<G, B, 0, S>>>(a);
@@ -1180,7 +1224,8 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
     for (int c = 0; c < count; ++c) {
       if (pr == PREC_JACOBI) k_bt_pupdate<PREC_JACOBI><<<G, B, 0, S>>>(a);
       else k_bt_pupdate<PREC_NONE><<<G, B, 0, S>>>(a);
-      TFG_BT_ENGINE((k_bt_spmv<E><<<G, B, 0, S>>>(a)));
+      if (Gd) k_bt_spmv_dia<<<Gd, B, 0, S>>>(a);
+      if (Gc) TFG_BT_ENGINE((k_bt_spmv<E><<<Gc, B, 0, S>>>(a)));
       k_bt_alpha<<<a.nsys, B, 0, S>>>(a);
       if (pr == PREC_JACOBI) k_bt_update<PREC_JACOBI><<<G, B, 0, S>>>(a);
       else k_bt_update<PREC_NONE><<<G, B, 0, S>>>(a);
@@ -1215,7 +1260,6 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
   // runs (stopped systems are masked on the device, so the lag only costs
   // the masked launches of one chunk).
   int k = 0, slot = 0, prev = -1;
-  std::vector<int4v> nact;
   while (k < miter) {
     const int cnt = std::min(chunk, miter - k);
     if (use_graph && cnt == chunk) HIP_TRY(hipGraphLaunch(exec, S));
@@ -1228,20 +1272,14 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
       HIP_TRY(hipEventSynchronize(h->ev_poll[prev]));
       const State* ps = h->hst + (size_t)prev * h->nsys;
       int running = 0;
-      nact.clear();
-      for (int s = 0; s < h->nsys; ++s) {
-        if (!ps[s].done) {
-          ++running;
-          for (int t = h->h_sblk[s]; t < h->h_sblk[s + 1]; ++t) nact.push_back(entry(t));
-        }
-      }
+      for (int s = 0; s < h->nsys; ++s) running += ps[s].done ? 0 : 1;
       if (!running) break;
       // compaction once an eighth of the active blocks have stopped (the
-      // copy is stream-ordered after the chunk already queued)
-      if ((int)nact.size() * 8 <= a.nact * 7) {
-        HIP_TRY(hipMemcpyAsync(h->act, nact.data(), sizeof(int4v) * nact.size(), hipMemcpyHostToDevice, S));
-        HIP_TRY(hipStreamSynchronize(S));
-        a.nact = (int)nact.size();
+      // copies are stream-ordered after the chunk already queued)
+      const int still = build_lists(ps);
+      if (still * 8 <= a.nact * 7) {
+        rc = upload_lists();
+        if (rc != TFG_OK) return rc;
         if (use_graph) {
           rc = build_graph();
           if (rc != TFG_OK) return rc;
@@ -1407,7 +1445,8 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
     dmal((void**)&h->part, sizeof(double) * (size_t)tfg::BP_ROWS * bsys.size());
     dmal((void**)&h->st, sizeof(tfg::State) * (size_t)nsys);
     dmal((void**)&h->clk, sizeof(unsigned long long) * (size_t)(nsys + 1));
-    dmal((void**)&h->scls, (size_t)nsys);
+    dmal((void**)&h->act_dia, sizeof(tfg::int4v) * bsys.size());
+    dmal((void**)&h->act_csr, sizeof(tfg::int4v) * bsys.size());
     if (nsd > 0) {
       dmal((void**)&h->sdg, sizeof(tfg::SysDiag) * (size_t)nsys);
       dmal((void**)&h->sd, sizeof(double) * (size_t)sd_len);
@@ -1432,12 +1471,13 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
     if (h->sdg) up(h->sdg, sdg.data(), sizeof(tfg::SysDiag) * sdg.size());
     {
       // size classes of the autotuner: < 32768 rows, < 262144, larger
-      std::vector<uint8_t> cls(nsys);
+      h->h_scls.resize(nsys);
+      h->h_sdk.resize(nsys);
       for (int q = 0; q < nsys; ++q) {
-        cls[q] = n[q] < (1 << 15) ? 0 : (n[q] < (1 << 18) ? 1 : 2);
-        if (sdg[q].k > 0) h->class_diag[cls[q]] = true;
+        h->h_scls[q] = n[q] < (1 << 15) ? 0 : (n[q] < (1 << 18) ? 1 : 2);
+        h->h_sdk[q] = sdg[q].k;
+        if (sdg[q].k > 0) h->class_diag[h->h_scls[q]] = true;
       }
-      up(h->scls, cls.data(), (size_t)nsys);
     }
     if (e != hipSuccess) { rc = fail(TFG_EHIP, "batch upload: %s", hipGetErrorString(e)); break; }
   } while (0);
