@@ -70,6 +70,17 @@ struct SysDiag {
   int off[kBatchSym];       // ascending
 };
 
+// Entry of the active list of systems on diagonals: the block and its rows,
+// the system, its rows and its SysDiag, in one 64-byte load, so the row
+// loads of a workgroup follow from that one load.
+struct alignas(16) DiaEntry {
+  int bl, r0, r1, s;
+  int sr0, sr1, k, pad;
+  int off[4];
+  long long base, stride;
+};
+static_assert(sizeof(DiaEntry) == 64, "one 64-byte load");
+
 struct BatchArgs {
   int nsys, nblk, nact;
   SpmvShape m;              // concatenated CSR; m.blk = all row blocks
@@ -77,7 +88,7 @@ struct BatchArgs {
   const int* sblk;          // nsys+1: first block of each system
   const int* srow;          // nsys+1: first row of each system
   const int4v* act;         // active blocks: {block, first row, end row, system}
-  const int4v* act_dia;     // the active blocks whose system takes its products from
+  const DiaEntry* act_dia;  // the active blocks whose system takes its products from
   const int4v* act_csr;     // its diagonals (SysDiag), and the others
   int nact_dia, nact_csr;
   const double* val;
@@ -287,57 +298,81 @@ __global__ __launch_bounds__(kBlock) void k_bt_pupdate(BatchArgs a) {
 // their mirrors for j = k-1 .. 0, then the upper entries for j = 0 .. k-1:
 // the CSR order, so the product is the CSR engine's bit for bit (a missing
 // entry adds a zero product; the x index of one is clamped into the system).
-template <int K>
-__device__ __forceinline__ double bt_dia_row(const BatchArgs& a, const SysDiag& g, int i, int r0,
-                                             int r1, double fn) {
-  const double* sd = a.sd + g.base;
-  const long long li = i - r0;
-  double lo[K], up[K], xl[K], xu[K];
-  const double d = sd[li];
-  const double x0 = a.p[i];
+// dia_load reads the row's values and the x entries they multiply (the
+// system's rows are e.sr0 .. e.sr1-1), dia_hold keeps those loads issued
+// ahead of the stop test, dia_sum adds them up.
+template <int K, class T>
+struct DiaRow {
+  T d, x0, lo[K], up[K], xl[K], xu[K];
+};
+template <int K, class T>
+__device__ __forceinline__ void dia_load(const T* sd, const T* x, const DiaEntry& e, int i,
+                                         DiaRow<K, T>& w) {
+  const long long li = i - e.sr0;
+  w.d = sd[li];
+  w.x0 = x[i];
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    const int o = g.off[j];
-    up[j] = sd[(j + 1) * g.stride + li];
-    lo[j] = sd[(j + 1) * g.stride + li - o];
-    xu[j] = a.p[min(i + o, r1 - 1)];
-    xl[j] = a.p[max(i - o, r0)];
+    const int o = e.off[j];
+    w.up[j] = sd[(j + 1) * e.stride + li];
+    w.lo[j] = sd[(j + 1) * e.stride + li - o];
+    w.xu[j] = x[min(i + o, e.sr1 - 1)];
+    w.xl[j] = x[max(i - o, e.sr0)];
   }
-  double acc = 0.0 + (d * fn) * x0;
+}
+template <int K, class T>
+__device__ __forceinline__ void dia_hold(const DiaRow<K, T>& w) {
+  hold(w.d, w.x0);
 #pragma unroll
-  for (int j = K - 1; j >= 0; --j) acc = acc + (lo[j] * fn) * xl[j];
+  for (int j = 0; j < K; ++j) hold(w.lo[j], w.up[j], w.xl[j], w.xu[j]);
+}
+template <int K>
+__device__ __forceinline__ double dia_sum(const DiaRow<K, double>& w, double fn) {
+  double acc = 0.0 + (w.d * fn) * w.x0;
 #pragma unroll
-  for (int j = 0; j < K; ++j) acc = acc + (up[j] * fn) * xu[j];
+  for (int j = K - 1; j >= 0; --j) acc = acc + (w.lo[j] * fn) * w.xl[j];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc = acc + (w.up[j] * fn) * w.xu[j];
   return acc;
 }
 
 // q = (A fn) p and the block's (p, q)                       Cg.f90:176-195
 // Blocks of systems on diagonals (act_dia): no LDS staging, so the kernel
 // runs at full occupancy.
-__global__ __launch_bounds__(kBlock) void k_bt_spmv_dia(BatchArgs a) {
-  __shared__ double s_tmp[4], s_out[1];
-  ActEntry e;
-  if (!list_entry(a.act_dia, a.nact_dia, e)) return;
-  const int s = e.s;
-  const State* st = &a.st[s];
+template <int K>
+__device__ __forceinline__ void bt_spmv_dia_block(const BatchArgs& a, const DiaEntry& e,
+                                                  double* s_tmp, double* s_out) {
+  // the row's loads and the state's are in flight together; rows past the
+  // block load the block's last row and are dropped
+  const int ti = e.r0 + (int)threadIdx.x;
+  const int i = min(ti, e.r1 - 1);
+  DiaRow<K, double> w;
+  dia_load<K, double>(a.sd + e.base, a.p, e, i, w);
+  const State* st = &a.st[e.s];
   const int done = st->done;
   const double fn = st->fn;
-  const SysDiag g = a.sdg[s];
-  const int r0 = a.srow[s], r1 = a.srow[s + 1];
-  hold(done, fn, g.k, r0, r1);
+  dia_hold<K, double>(w);
+  hold(done, fn);
   if (done) return;
   double v[1] = {0.0};
-  const int i = e.r0 + (int)threadIdx.x;
-  if (i < e.r1) {
-    double qi = 0.0;
-    if (g.k == 1) qi = bt_dia_row<1>(a, g, i, r0, r1, fn);
-    else if (g.k == 2) qi = bt_dia_row<2>(a, g, i, r0, r1, fn);
-    else qi = bt_dia_row<3>(a, g, i, r0, r1, fn);
+  if (ti < e.r1) {
+    const double qi = dia_sum<K>(w, fn);
     a.q[i] = qi;
-    v[0] = v[0] + a.p[i] * qi;
+    v[0] = v[0] + w.x0 * qi;
   }
   const int ops[1] = {OP_SUM};
   block_publish<1>(a, BP_SPMV, e.bl, v, ops, s_tmp, s_out);
+}
+__global__ __launch_bounds__(kBlock) void k_bt_spmv_dia(BatchArgs a) {
+  __shared__ double s_tmp[4], s_out[1];
+  const int gi = blockIdx.x, x = gi % kXcds, l = gi / kXcds;
+  const int first = (int)(((long long)a.nact_dia * x) / kXcds);
+  const int end = (int)(((long long)a.nact_dia * (x + 1)) / kXcds);
+  if (first + l >= end) return;
+  const DiaEntry e = a.act_dia[first + l];
+  if (e.k == 1) bt_spmv_dia_block<1>(a, e, s_tmp, s_out);
+  else if (e.k == 2) bt_spmv_dia_block<2>(a, e, s_tmp, s_out);
+  else bt_spmv_dia_block<3>(a, e, s_tmp, s_out);
 }
 
 // The other blocks (act_csr): the CSR row engine E
@@ -756,58 +791,52 @@ __global__ __launch_bounds__(kBlock) void k_bt32_pupdate(BatchArgs a) {
   }
 }
 
-// Row i of a system on diagonals in the row-excess form: s_i x_i, the lower
-// entries (x_{i-d} - x_i) for j = k-1 .. 0, then the upper ones: the CSR
-// order of the row-excess CSR product, so the same bits (a missing entry is
-// a zero times a finite difference)
+// Row i of a system on diagonals in the row-excess form (loads by dia_load
+// on the FP32 diagonals): s_i x_i, the lower entries (x_{i-d} - x_i) for
+// j = k-1 .. 0, then the upper ones: the CSR order of the row-excess CSR
+// product, so the same bits (a missing entry is a zero times a finite
+// difference)
 template <int K>
-__device__ __forceinline__ float bt32_dia_row(const BatchArgs& a, const SysDiag& g, int i, int r0,
-                                              int r1) {
-  const float* sd = a.f.sd32 + g.base;
-  const long long li = i - r0;
-  float lo[K], up[K], xl[K], xu[K];
-  const float sv = sd[li];
-  const float x0 = a.f.p32[i];
+__device__ __forceinline__ float dia_sum32(const DiaRow<K, float>& w) {
+  float acc = w.d * w.x0;
 #pragma unroll
-  for (int j = 0; j < K; ++j) {
-    const int o = g.off[j];
-    up[j] = sd[(j + 1) * g.stride + li];
-    lo[j] = sd[(j + 1) * g.stride + li - o];
-    xu[j] = a.f.p32[min(i + o, r1 - 1)];
-    xl[j] = a.f.p32[max(i - o, r0)];
-  }
-  float acc = sv * x0;
+  for (int j = K - 1; j >= 0; --j) acc = acc + w.lo[j] * (w.xl[j] - w.x0);
 #pragma unroll
-  for (int j = K - 1; j >= 0; --j) acc = acc + lo[j] * (xl[j] - x0);
-#pragma unroll
-  for (int j = 0; j < K; ++j) acc = acc + up[j] * (xu[j] - x0);
+  for (int j = 0; j < K; ++j) acc = acc + w.up[j] * (w.xu[j] - w.x0);
   return acc;
 }
 
 // q = A32 p and the block's (p, q): blocks of systems on diagonals
-__global__ __launch_bounds__(kBlock) void k_bt32_spmv_dia(BatchArgs a) {
-  __shared__ double s_tmp[4], s_out[1];
-  ActEntry e;
-  if (!list_entry(a.act_dia, a.nact_dia, e)) return;
-  const int s = e.s;
-  const State* st = &a.st[s];
-  const int done = st->done;
-  const SysDiag g = a.sdg[s];
-  const int r0 = a.srow[s], r1 = a.srow[s + 1];
-  hold(done, g.k, r0, r1);
+template <int K>
+__device__ __forceinline__ void bt32_spmv_dia_block(const BatchArgs& a, const DiaEntry& e,
+                                                    double* s_tmp, double* s_out) {
+  const int ti = e.r0 + (int)threadIdx.x;
+  const int i = min(ti, e.r1 - 1);  // as in bt_spmv_dia_block
+  DiaRow<K, float> w;
+  dia_load<K, float>(a.f.sd32 + e.base, a.f.p32, e, i, w);
+  const int done = a.st[e.s].done;
+  dia_hold<K, float>(w);
+  hold(done);
   if (done) return;
   double v[1] = {0.0};
-  const int i = e.r0 + (int)threadIdx.x;
-  if (i < e.r1) {
-    float qi;
-    if (g.k == 1) qi = bt32_dia_row<1>(a, g, i, r0, r1);
-    else if (g.k == 2) qi = bt32_dia_row<2>(a, g, i, r0, r1);
-    else qi = bt32_dia_row<3>(a, g, i, r0, r1);
+  if (ti < e.r1) {
+    const float qi = dia_sum32<K>(w);
     a.f.q32[i] = qi;
-    v[0] = (double)a.f.p32[i] * (double)qi;
+    v[0] = (double)w.x0 * (double)qi;
   }
   const int ops[1] = {OP_SUM};
   block_publish<1>(a, BP_SPMV, e.bl, v, ops, s_tmp, s_out);
+}
+__global__ __launch_bounds__(kBlock) void k_bt32_spmv_dia(BatchArgs a) {
+  __shared__ double s_tmp[4], s_out[1];
+  const int gi = blockIdx.x, x = gi % kXcds, l = gi / kXcds;
+  const int first = (int)(((long long)a.nact_dia * x) / kXcds);
+  const int end = (int)(((long long)a.nact_dia * (x + 1)) / kXcds);
+  if (first + l >= end) return;
+  const DiaEntry e = a.act_dia[first + l];
+  if (e.k == 1) bt32_spmv_dia_block<1>(a, e, s_tmp, s_out);
+  else if (e.k == 2) bt32_spmv_dia_block<2>(a, e, s_tmp, s_out);
+  else bt32_spmv_dia_block<3>(a, e, s_tmp, s_out);
 }
 
 // the other blocks: the CSR row engine E on the FP32 row-excess values
@@ -965,7 +994,9 @@ struct tfg_batch {
   std::vector<uint8_t> h_scls;        // size class of each system (0: < 32768 rows, 1: < 262144, 2)
   std::vector<int> h_sdk;             // diagonals of each system (SysDiag::k; 0: CSR rows only)
   std::vector<int> h_bad;             // bad flags of the last solve, read back from the device
-  tfg::int4v *act_dia = nullptr, *act_csr = nullptr;  // the active list split by product
+  tfg::DiaEntry* act_dia = nullptr;   // the active list split by product
+  tfg::int4v* act_csr = nullptr;
+  std::vector<tfg::SysDiag> h_sdg;    // host copy of sdg
   bool class_diag[3] = {false, false, false};  // a class holds a system on diagonals
   // batch autotuner, per precision (FP64, FP32): choice, probes, best probe
   bool tuned[2] = {false, false};
@@ -1146,7 +1177,8 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
   }
   // Active lists (every block of the running systems; all of them, those on
   // diagonals, the others), rebuilt from the polled state as systems stop.
-  std::vector<int4v> act, act_d, act_c;
+  std::vector<int4v> act, act_c;
+  std::vector<DiaEntry> act_d;
   auto build_lists = [&](const State* ps) -> int {
     act.clear();
     act_d.clear();
@@ -1160,7 +1192,23 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
         v.z = h->h_blk[t + 1];
         v.w = s;
         act.push_back(v);
-        (dia[s] ? act_d : act_c).push_back(v);
+        if (!dia[s]) {
+          act_c.push_back(v);
+          continue;
+        }
+        const SysDiag& g = h->h_sdg[s];
+        DiaEntry d{};
+        d.bl = t;
+        d.r0 = v.y;
+        d.r1 = v.z;
+        d.s = s;
+        d.sr0 = h->h_srow[s];
+        d.sr1 = h->h_srow[s + 1];
+        d.k = g.k;
+        for (int j = 0; j < kBatchSym; ++j) d.off[j] = g.off[j];
+        d.base = g.base;
+        d.stride = g.stride;
+        act_d.push_back(d);
       }
     }
     return (int)act.size();
@@ -1168,7 +1216,7 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
   auto upload_lists = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(h->act, act.data(), sizeof(int4v) * act.size(), hipMemcpyHostToDevice, S));
     if (!act_d.empty())
-      HIP_TRY(hipMemcpyAsync(h->act_dia, act_d.data(), sizeof(int4v) * act_d.size(),
+      HIP_TRY(hipMemcpyAsync(h->act_dia, act_d.data(), sizeof(DiaEntry) * act_d.size(),
                              hipMemcpyHostToDevice, S));
     if (!act_c.empty())
       HIP_TRY(hipMemcpyAsync(h->act_csr, act_c.data(), sizeof(int4v) * act_c.size(),
@@ -1445,7 +1493,7 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
     dmal((void**)&h->part, sizeof(double) * (size_t)tfg::BP_ROWS * bsys.size());
     dmal((void**)&h->st, sizeof(tfg::State) * (size_t)nsys);
     dmal((void**)&h->clk, sizeof(unsigned long long) * (size_t)(nsys + 1));
-    dmal((void**)&h->act_dia, sizeof(tfg::int4v) * bsys.size());
+    dmal((void**)&h->act_dia, sizeof(tfg::DiaEntry) * bsys.size());
     dmal((void**)&h->act_csr, sizeof(tfg::int4v) * bsys.size());
     if (nsd > 0) {
       dmal((void**)&h->sdg, sizeof(tfg::SysDiag) * (size_t)nsys);
@@ -1476,6 +1524,7 @@ int tfg_batch_create(int nsys, const int* n, const int* nnz, const int* row, con
       for (int q = 0; q < nsys; ++q) {
         h->h_scls[q] = n[q] < (1 << 15) ? 0 : (n[q] < (1 << 18) ? 1 : 2);
         h->h_sdk[q] = sdg[q].k;
+        h->h_sdg.push_back(sdg[q]);
         if (sdg[q].k > 0) h->class_diag[h->h_scls[q]] = true;
       }
     }
