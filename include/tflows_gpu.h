@@ -443,12 +443,15 @@ int tfg_batch_info(tfg_batch* h, int* nsys, int* n, int* nnz, int* blocks);
  * replayed graph holds (FP64: 8, 16 or 32; the FP32 form keeps 8, its
  * refinement cadence) and, per system-size class (bit 0: < 32768 rows,
  * bit 1: < 262144, bit 2: larger), whether systems on diagonals take their
- * products from the diagonals (bit set) or the CSR rows.  Every choice gives
- * the same bits.  Reports the choice for precision (TFG_FP64 | TFG_FP32_IR):
- * chunk and diag_classes -1 before the first solve; probes timed and the
- * best probe's device time.  Any pointer may be NULL. */
-int tfg_batch_tuning(tfg_batch* h, int precision, int* chunk, int* diag_classes, int* probes,
-                     double* probe_ms);
+ * products from the diagonals (bit set) or the CSR rows, and the rows each
+ * thread of the vector kernels (p update, x and r update) takes: 1, or 2
+ * (one from each of two row blocks per workgroup).  Every choice gives the
+ * same bits.  Reports the choice for precision (TFG_FP64 | TFG_FP32_IR):
+ * chunk, diag_classes and rows_per_thread -1 before the first solve;
+ * probes timed and the best probe's device time.  Any pointer may be
+ * NULL.  TFG_BATCH_ROWS=1|2 in the environment forces the rows per thread. */
+int tfg_batch_tuning(tfg_batch* h, int precision, int* chunk, int* diag_classes,
+                     int* rows_per_thread, int* probes, double* probe_ms);
 
 /* ---- allocation-free solve ----------------------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device

@@ -111,6 +111,7 @@ struct BatchArgs {
 struct BatchCfg {
   int chunk;      // iterations per replayed graph (FP64)
   int diag_mask;  // bit c: systems of class c on diagonals use the diagonals
+  int rows;       // rows per thread of the vector kernels (1 or 2 blocks per workgroup)
 };
 
 
@@ -150,6 +151,40 @@ __device__ __forceinline__ bool list_entry(const int4v* list, int nact, ActEntry
 }
 __device__ __forceinline__ bool active_entry(const BatchArgs& a, ActEntry& e) {
   return list_entry(a.act, a.nact, e);
+}
+
+// The vector kernels (p update, x/r update) take NE consecutive entries of
+// the XCD's range per workgroup, each thread one row of each (NE = 1 or 2,
+// the batch autotuner's rows per thread).  Returns how many exist (0:
+// none); a missing one repeats the last, so its loads stay in bounds, and
+// stores nothing.
+template <int NE>
+__device__ __forceinline__ int active_entries(const BatchArgs& a, ActEntry (&e)[NE]) {
+  const int g = blockIdx.x, x = g % kXcds, l = g / kXcds;
+  const int first = (int)(((long long)a.nact * x) / kXcds);
+  const int end = (int)(((long long)a.nact * (x + 1)) / kXcds);
+  const int j = first + l * NE;
+  if (j >= end) return 0;
+  const int n = min(NE, end - j);
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int4v v = a.act[j + min(k, n - 1)];
+    e[k] = ActEntry{v.x, v.y, v.z, v.w};
+  }
+  return n;
+}
+
+// Publishes the block totals of the live entries: slot k * NV + c of the
+// reduction holds value c of entry k.  Each slot has its own butterfly, so
+// an entry's totals are those of a one-entry workgroup, bit for bit.
+template <int NV, int NE>
+__device__ __forceinline__ void entries_publish(const BatchArgs& a, int row0,
+                                                const ActEntry (&e)[NE], const bool (&live)[NE],
+                                                double (&v)[NE * NV], const int (&ops)[NE * NV],
+                                                double* s_tmp, double* s_out) {
+  block_reduce<NE * NV>(v, ops, s_tmp, s_out);
+  const int t = (int)threadIdx.x, k = t / NV, c = t % NV;
+  if (t < NE * NV && live[k]) a.part[(size_t)(row0 + c) * a.nblk + e[k].bl] = s_out[t];
 }
 
 // Sum (min, max) of one system's block partials, fixed order; s_out[k].
@@ -277,21 +312,32 @@ __global__ __launch_bounds__(kBlock) void k_bt_start(BatchArgs a) {
 // p = z + beta p (p = z in a system's first iteration)      Cg.f90:145-171
 // (Row loads and the system's state are in flight together; a thread past
 // the block loads the block's last row and stores nothing.)
-template <int PREC>
+template <int PREC, int NE>
 __global__ __launch_bounds__(kBlock) void k_bt_pupdate(BatchArgs a) {
-  ActEntry e;
-  if (!active_entry(a, e)) return;
-  const int ti = e.r0 + (int)threadIdx.x;
-  const int i = min(ti, e.r1 - 1);
-  const double r = a.r[i], pv = a.p[i];
-  const double d = PREC == PREC_JACOBI ? a.ad[i] : 1.0;
-  const State* st = &a.st[e.s];
-  const int done = st->done, iter = st->iter;
-  const double beta = st->aux1;
-  hold(r, pv, d, done, iter, beta);
-  if (done || ti >= e.r1) return;
-  const double z = PREC == PREC_JACOBI ? r * (1.0 / d) : r;
-  a.p[i] = iter == 0 ? z : z + beta * pv;
+  ActEntry e[NE];
+  const int n = active_entries<NE>(a, e);
+  if (n == 0) return;
+  int i[NE], done[NE], iter[NE];
+  double r[NE], pv[NE], d[NE], beta[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    i[k] = min(e[k].r0 + (int)threadIdx.x, e[k].r1 - 1);
+    r[k] = a.r[i[k]];
+    pv[k] = a.p[i[k]];
+    d[k] = PREC == PREC_JACOBI ? a.ad[i[k]] : 1.0;
+    const State* st = &a.st[e[k].s];
+    done[k] = st->done;
+    iter[k] = st->iter;
+    beta[k] = st->aux1;
+  }
+#pragma unroll
+  for (int k = 0; k < NE; ++k) hold(r[k], pv[k], d[k], done[k], iter[k], beta[k]);
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    if (k >= n || done[k] || e[k].r0 + (int)threadIdx.x >= e[k].r1) continue;
+    const double z = PREC == PREC_JACOBI ? r[k] * (1.0 / d[k]) : r[k];
+    a.p[i[k]] = iter[k] == 0 ? z : z + beta[k] * pv[k];
+  }
 }
 
 // Row i of a system on diagonals (SysDiag): a_ii x_i, the lower entries from
@@ -453,34 +499,64 @@ __global__ __launch_bounds__(kBlock) void k_bt_alpha(BatchArgs a) {
 }
 
 // x += alpha p, r -= alpha q and the block's stopping sums  Cg.f90:204-218
-template <int PREC>
+template <int PREC, int NE>
 __global__ __launch_bounds__(kBlock) void k_bt_update(BatchArgs a) {
-  __shared__ double s_tmp[5 * 4], s_out[5];
-  ActEntry e;
-  if (!active_entry(a, e)) return;
-  const int ti = e.r0 + (int)threadIdx.x;
-  const int i = min(ti, e.r1 - 1);
-  const double d = a.ad[i], x0 = a.x[i], p0 = a.p[i], r0 = a.r[i], q0 = a.q[i];
-  const State* st = &a.st[e.s];
-  const int done = st->done;
-  const double alpha = st->alpha;
-  hold(d, x0, p0, r0, q0, done, alpha);
-  if (done) return;
-  double v[5] = {0.0, kHuge, -kHuge, 0.0, 0.0};
-  if (ti < e.r1) {
-    const double xv = x0 + alpha * p0;
-    const double rv = r0 - alpha * q0;
-    a.x[i] = xv;
-    a.r[i] = rv;
-    const double z = PREC == PREC_JACOBI ? rv * (1.0 / d) : rv;
-    v[0] = (rv * rv) / (d * d);
-    v[1] = xv;
-    v[2] = xv;
-    v[3] = z * rv;
-    v[4] = rv * rv;
+  constexpr int NV = 5;
+  __shared__ double s_tmp[NE * NV * 4], s_out[NE * NV];
+  ActEntry e[NE];
+  const int n = active_entries<NE>(a, e);
+  if (n == 0) return;
+  int i[NE], done[NE];
+  bool live[NE];
+  double d[NE], x0[NE], p0[NE], r0[NE], q0[NE], alpha[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    i[k] = min(e[k].r0 + (int)threadIdx.x, e[k].r1 - 1);
+    d[k] = a.ad[i[k]];
+    x0[k] = a.x[i[k]];
+    p0[k] = a.p[i[k]];
+    r0[k] = a.r[i[k]];
+    q0[k] = a.q[i[k]];
+    const State* st = &a.st[e[k].s];
+    done[k] = st->done;
+    alpha[k] = st->alpha;
   }
-  const int ops[5] = {OP_SUM, OP_MIN, OP_MAX, OP_SUM, OP_SUM};
-  block_publish<5>(a, BP_UPD, e.bl, v, ops, s_tmp, s_out);
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    hold(d[k], x0[k], p0[k], r0[k], q0[k], done[k], alpha[k]);
+    live[k] = k < n && !done[k];
+    any = any || live[k];
+  }
+  if (!any) return;
+  double v[NE * NV];
+  int ops[NE * NV];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    double* w = v + k * NV;
+    w[0] = 0.0;
+    w[1] = kHuge;
+    w[2] = -kHuge;
+    w[3] = 0.0;
+    w[4] = 0.0;
+    ops[k * NV + 0] = OP_SUM;
+    ops[k * NV + 1] = OP_MIN;
+    ops[k * NV + 2] = OP_MAX;
+    ops[k * NV + 3] = OP_SUM;
+    ops[k * NV + 4] = OP_SUM;
+    if (!live[k] || e[k].r0 + (int)threadIdx.x >= e[k].r1) continue;
+    const double xv = x0[k] + alpha[k] * p0[k];
+    const double rv = r0[k] - alpha[k] * q0[k];
+    a.x[i[k]] = xv;
+    a.r[i[k]] = rv;
+    const double z = PREC == PREC_JACOBI ? rv * (1.0 / d[k]) : rv;
+    w[0] = (rv * rv) / (d[k] * d[k]);
+    w[1] = xv;
+    w[2] = xv;
+    w[3] = z * rv;
+    w[4] = rv * rv;
+  }
+  entries_publish<NV, NE>(a, BP_UPD, e, live, v, ops, s_tmp, s_out);
 }
 
 // stopping test of iteration k (Cg.f90:214-222), next rho and beta
@@ -766,28 +842,44 @@ __global__ __launch_bounds__(kBlock) void k_bt32_rstart(BatchArgs a) {
 
 // p = z + beta p - mu (a fresh start after a restart: p = z - mu), with the
 // deferred e += alpha_{k-1} p_{k-1} on the p read anyway
-template <int PREC>
+template <int PREC, int NE>
 __global__ __launch_bounds__(kBlock) void k_bt32_pupdate(BatchArgs a) {
-  ActEntry e;
-  if (!active_entry(a, e)) return;
-  const int ti = e.r0 + (int)threadIdx.x;
-  const int i = min(ti, e.r1 - 1);
-  const float r = a.f.r32[i], pv = a.f.p32[i], ev = a.f.e32[i];
-  const float d = PREC == PREC_JACOBI ? a.f.rd32[i] : 1.0f;
-  const State* st = &a.st[e.s];
-  const int done = st->done, iter = st->iter, kref = st->kref;
-  const double mu64 = st->aux2, beta64 = st->aux1, alpha64 = st->alpha;
-  hold(r, pv, ev, d, done, iter, kref, mu64, beta64, alpha64);
-  if (done || ti >= e.r1) return;
-  const bool fresh = iter == kref;
-  const float mu = (float)mu64;
-  const float z = PREC == PREC_JACOBI ? r * d : r;
-  if (fresh) {
-    a.f.p32[i] = z - mu;
-  } else {
-    const float beta = (float)beta64, ap = (float)alpha64;
-    a.f.e32[i] = ev + ap * pv;
-    a.f.p32[i] = (z + beta * pv) - mu;
+  ActEntry e[NE];
+  const int n = active_entries<NE>(a, e);
+  if (n == 0) return;
+  int i[NE], done[NE], iter[NE], kref[NE];
+  float r[NE], pv[NE], ev[NE], d[NE];
+  double mu64[NE], beta64[NE], alpha64[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    i[k] = min(e[k].r0 + (int)threadIdx.x, e[k].r1 - 1);
+    r[k] = a.f.r32[i[k]];
+    pv[k] = a.f.p32[i[k]];
+    ev[k] = a.f.e32[i[k]];
+    d[k] = PREC == PREC_JACOBI ? a.f.rd32[i[k]] : 1.0f;
+    const State* st = &a.st[e[k].s];
+    done[k] = st->done;
+    iter[k] = st->iter;
+    kref[k] = st->kref;
+    mu64[k] = st->aux2;
+    beta64[k] = st->aux1;
+    alpha64[k] = st->alpha;
+  }
+#pragma unroll
+  for (int k = 0; k < NE; ++k)
+    hold(r[k], pv[k], ev[k], d[k], done[k], iter[k], kref[k], mu64[k], beta64[k], alpha64[k]);
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    if (k >= n || done[k] || e[k].r0 + (int)threadIdx.x >= e[k].r1) continue;
+    const float mu = (float)mu64[k];
+    const float z = PREC == PREC_JACOBI ? r[k] * d[k] : r[k];
+    if (iter[k] == kref[k]) {  // a fresh start after a restart
+      a.f.p32[i[k]] = z - mu;
+    } else {
+      const float beta = (float)beta64[k], ap = (float)alpha64[k];
+      a.f.e32[i[k]] = ev[k] + ap * pv[k];
+      a.f.p32[i[k]] = (z + beta * pv[k]) - mu;
+    }
   }
 }
 
@@ -881,37 +973,62 @@ __global__ __launch_bounds__(kBlock) void k_bt32_alpha(BatchArgs a) {
 }
 
 // r -= alpha q and the sums (z, r), (s, z), (r, r), NRMS of the FP32 residual
-template <int PREC>
+template <int PREC, int NE>
 __global__ __launch_bounds__(kBlock) void k_bt32_update(BatchArgs a) {
-  __shared__ double s_tmp[4 * 4], s_out[4];
-  ActEntry e;
-  if (!active_entry(a, e)) return;
-  const int ti = e.r0 + (int)threadIdx.x;
-  const int i = min(ti, e.r1 - 1);
-  const float d = a.f.rd32[i], r0 = a.f.r32[i], q0 = a.f.q32[i], sv = a.f.s32[i];
-  const State* st = &a.st[e.s];
-  const int done = st->done, crit = st->criterion;
-  const double alpha64 = st->alpha, defl_e = st->defl_e;
-  hold(d, r0, q0, sv, done, crit, alpha64, defl_e);
-  if (done) return;
-  const float a32 = (float)alpha64;
-  const bool defl = defl_e > 0.0;
-  const bool nrm = crit != TFG_CRIT_RELRES;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  if (ti < e.r1) {
-    const float r = r0 - a32 * q0;
-    a.f.r32[i] = r;
-    const float z = PREC == PREC_JACOBI ? r * d : r;
-    v[0] = (double)z * (double)r;
-    v[1] = defl ? (double)sv * (double)z : 0.0;
-    v[2] = (double)r * (double)r;
-    if (nrm) {
-      const double t = (double)(r * d);
-      v[3] = t * t;
+  constexpr int NV = 4;
+  __shared__ double s_tmp[NE * NV * 4], s_out[NE * NV];
+  ActEntry e[NE];
+  const int n = active_entries<NE>(a, e);
+  if (n == 0) return;
+  int i[NE], crit[NE], done[NE];
+  bool live[NE];
+  float d[NE], r0[NE], q0[NE], sv[NE];
+  double alpha64[NE], defl_e[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    i[k] = min(e[k].r0 + (int)threadIdx.x, e[k].r1 - 1);
+    d[k] = a.f.rd32[i[k]];
+    r0[k] = a.f.r32[i[k]];
+    q0[k] = a.f.q32[i[k]];
+    sv[k] = a.f.s32[i[k]];
+    const State* st = &a.st[e[k].s];
+    done[k] = st->done;
+    crit[k] = st->criterion;
+    alpha64[k] = st->alpha;
+    defl_e[k] = st->defl_e;
+  }
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    hold(d[k], r0[k], q0[k], sv[k], done[k], crit[k], alpha64[k], defl_e[k]);
+    live[k] = k < n && !done[k];
+    any = any || live[k];
+  }
+  if (!any) return;
+  double v[NE * NV];
+  int ops[NE * NV];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    double* w = v + k * NV;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      w[c] = 0.0;
+      ops[k * NV + c] = OP_SUM;
+    }
+    if (!live[k] || e[k].r0 + (int)threadIdx.x >= e[k].r1) continue;
+    const float a32 = (float)alpha64[k];
+    const float r = r0[k] - a32 * q0[k];
+    a.f.r32[i[k]] = r;
+    const float z = PREC == PREC_JACOBI ? r * d[k] : r;
+    w[0] = (double)z * (double)r;
+    w[1] = defl_e[k] > 0.0 ? (double)sv[k] * (double)z : 0.0;
+    w[2] = (double)r * (double)r;
+    if (crit[k] != TFG_CRIT_RELRES) {
+      const double t = (double)(r * d[k]);
+      w[3] = t * t;
     }
   }
-  const int ops[4] = {OP_SUM, OP_SUM, OP_SUM, OP_SUM};
-  block_publish<4>(a, BP_UPD, e.bl, v, ops, s_tmp, s_out);
+  entries_publish<NV, NE>(a, BP_UPD, e, live, v, ops, s_tmp, s_out);
 }
 
 // Per system, end of iteration k: beta and mu of the next iteration and, at
@@ -1000,7 +1117,7 @@ struct tfg_batch {
   bool class_diag[3] = {false, false, false};  // a class holds a system on diagonals
   // batch autotuner, per precision (FP64, FP32): choice, probes, best probe
   bool tuned[2] = {false, false};
-  tfg::BatchCfg cfg[2] = {{16, 7}, {16, 7}};
+  tfg::BatchCfg cfg[2] = {{16, 7, 1}, {16, 7, 1}};
   int probes[2] = {0, 0};
   double probe_ms[2] = {0.0, 0.0};
   double* x0 = nullptr;               // the caller's x0 while probing
@@ -1250,19 +1367,29 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
     else k_bt32_restart<PREC_NONE><<<G, B, 0, S>>>(a);
     k_bt32_rstart<<<a.nsys, B, 0, S>>>(a);
   };
+  // the vector kernels with cfg.rows entries per workgroup, grid Gv
+#define TFG_BT_VEC(KERNEL)                                                            \
+  do {                                                                                \
+    if (two && pr == PREC_JACOBI) KERNEL<PREC_JACOBI, 2><<<Gv, B, 0, S>>>(a);          \
+    else if (two) KERNEL<PREC_NONE, 2><<<Gv, B, 0, S>>>(a);                            \
+    else if (pr == PREC_JACOBI) KERNEL<PREC_JACOBI, 1><<<Gv, B, 0, S>>>(a);            \
+    else KERNEL<PREC_NONE, 1><<<Gv, B, 0, S>>>(a);                                     \
+  } while (0)
+  const char* force_rows = getenv("TFG_BATCH_ROWS");
+  const bool two = force_rows && (*force_rows == '1' || *force_rows == '2') ? *force_rows == '2'
+                                                                            : cfg.rows == 2;
   auto enqueue_chunk = [&](int count) -> hipError_t {
-    const int G = kXcds * ((a.nact + kXcds - 1) / kXcds);
+    const int per = (a.nact + kXcds - 1) / kXcds;
+    const int G = kXcds * per, Gv = kXcds * (two ? (per + 1) / 2 : per);
     const int Gd = kXcds * ((a.nact_dia + kXcds - 1) / kXcds);
     const int Gc = kXcds * ((a.nact_csr + kXcds - 1) / kXcds);
     if (fp32) {
       for (int c = 0; c < count; ++c) {
-        if (pr == PREC_JACOBI) k_bt32_pupdate<PREC_JACOBI><<<G, B, 0, S>>>(a);
-        else k_bt32_pupdate<PREC_NONE><<<G, B, 0, S>>>(a);
+        TFG_BT_VEC(k_bt32_pupdate);
         if (Gd) k_bt32_spmv_dia<<<Gd, B, 0, S>>>(a);
         if (Gc) TFG_BT32_ENGINE((k_bt32_spmv<E><<<Gc, B, 0, S>>>(a)));
         k_bt32_alpha<<<a.nsys, B, 0, S>>>(a);
-        if (pr == PREC_JACOBI) k_bt32_update<PREC_JACOBI><<<G, B, 0, S>>>(a);
-        else k_bt32_update<PREC_NONE><<<G, B, 0, S>>>(a);
+        TFG_BT_VEC(k_bt32_update);
         k_bt32_check<<<a.nsys, B, 0, S>>>(a);
       }
       k_bt32_fold<<<G, B, 0, S>>>(a);
@@ -1270,13 +1397,11 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
       return hipGetLastError();
     }
     for (int c = 0; c < count; ++c) {
-      if (pr == PREC_JACOBI) k_bt_pupdate<PREC_JACOBI><<<G, B, 0, S>>>(a);
-      else k_bt_pupdate<PREC_NONE><<<G, B, 0, S>>>(a);
+      TFG_BT_VEC(k_bt_pupdate);
       if (Gd) k_bt_spmv_dia<<<Gd, B, 0, S>>>(a);
       if (Gc) TFG_BT_ENGINE((k_bt_spmv<E><<<Gc, B, 0, S>>>(a)));
       k_bt_alpha<<<a.nsys, B, 0, S>>>(a);
-      if (pr == PREC_JACOBI) k_bt_update<PREC_JACOBI><<<G, B, 0, S>>>(a);
-      else k_bt_update<PREC_NONE><<<G, B, 0, S>>>(a);
+      TFG_BT_VEC(k_bt_update);
       k_bt_check<<<a.nsys, B, 0, S>>>(a);
     }
     return hipGetLastError();
@@ -1352,6 +1477,7 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
   return TFG_OK;
 #undef TFG_BT_ENGINE
 #undef TFG_BT32_ENGINE
+#undef TFG_BT_VEC
 }
 
 // The batch autotuner: short timed probes (the first kTuneIters iterations
@@ -1368,7 +1494,7 @@ int batch_tune(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doub
                const double* db, int miter, double tol, const double* norm) {
   const int p = fp32 ? 1 : 0;
   if (h->tuned[p]) return TFG_OK;
-  h->cfg[p] = BatchCfg{16, 7};
+  h->cfg[p] = BatchCfg{16, 7, 1};
   h->probes[p] = 0;
   h->probe_ms[p] = 0.0;
   const char* env = getenv("TFG_BATCH_TUNE");
@@ -1394,16 +1520,21 @@ int batch_tune(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doub
     return TFG_OK;
   };
   for (int ch : fp32 ? std::vector<int>{16} : std::vector<int>{8, 16, 32}) {
-    const int rc = probe(BatchCfg{ch, 7});
+    const int rc = probe(BatchCfg{ch, 7, 1});
     if (rc != TFG_OK) return rc;
   }
   if (h->sdg) {
     const BatchCfg base = pick;
     for (int c = 0; c < 3; ++c) {
       if (!h->class_diag[c]) continue;  // no system on diagonals in this class
-      const int rc = probe(BatchCfg{base.chunk, base.diag_mask & ~(1 << c)});
+      const int rc = probe(BatchCfg{base.chunk, base.diag_mask & ~(1 << c), 1});
       if (rc != TFG_OK) return rc;
     }
+  }
+  {  // two rows per thread in the vector kernels
+    const BatchCfg base = pick;
+    const int rc = probe(BatchCfg{base.chunk, base.diag_mask, 2});
+    if (rc != TFG_OK) return rc;
   }
   h->cfg[p] = pick;
   h->probe_ms[p] = best;
@@ -1625,14 +1756,15 @@ int tfg_batch_solve(tfg_batch* h, const char* solver, int solver_len, const char
   return TFG_OK;
 }
 
-int tfg_batch_tuning(tfg_batch* h, int precision, int* chunk, int* diag_classes, int* probes,
-                     double* probe_ms) {
+int tfg_batch_tuning(tfg_batch* h, int precision, int* chunk, int* diag_classes,
+                     int* rows_per_thread, int* probes, double* probe_ms) {
   if (!h) return fail(TFG_EINVAL, "null handle");
   if (precision != TFG_FP64 && precision != TFG_FP32_IR)
     return fail(TFG_EINVAL, "unknown precision %d", precision);
   const int p = precision == TFG_FP32_IR ? 1 : 0;
   if (chunk) *chunk = !h->tuned[p] ? -1 : (p ? tfg::kBt32Check : h->cfg[p].chunk);
   if (diag_classes) *diag_classes = h->tuned[p] ? h->cfg[p].diag_mask : -1;
+  if (rows_per_thread) *rows_per_thread = h->tuned[p] ? h->cfg[p].rows : -1;
   if (probes) *probes = h->probes[p];
   if (probe_ms) *probe_ms = h->probe_ms[p];
   return TFG_OK;

@@ -111,7 +111,7 @@ def lib():
                                       D, P, P, I, P, P]
         L.tfg_batch_times.argtypes = [P, P]
         L.tfg_batch_info.argtypes = [P, P, P, P, P]
-        L.tfg_batch_tuning.argtypes = [P, I, P, P, P, P]
+        L.tfg_batch_tuning.argtypes = [P, I, P, P, P, P, P]
         L.tfg_solve_native_noalloc.argtypes = [P, ctypes.c_char_p, I, ctypes.c_char_p, I, P, P,
                                                P, I, P, D, P, P, P, P]
         L.tfg_last_error.restype = ctypes.c_char_p
@@ -440,13 +440,14 @@ class Batch:
     def tuning(self, precision=0):
         """The batch autotuner's choice for a precision: chunk (iterations per
         replayed graph), diag_classes (bit c: systems of size class c on
-        diagonals use the diagonal products), probes timed, best probe ms."""
-        c, d, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        diagonals use the diagonal products), rows_per_thread of the vector
+        kernels, probes timed, best probe ms."""
+        c, d, r, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ms = ctypes.c_double()
         _check(lib().tfg_batch_tuning(self.h, int(precision), ctypes.byref(c), ctypes.byref(d),
-                                      ctypes.byref(p), ctypes.byref(ms)))
-        return {"chunk": c.value, "diag_classes": d.value, "probes": p.value,
-                "best_probe_ms": ms.value}
+                                      ctypes.byref(r), ctypes.byref(p), ctypes.byref(ms)))
+        return {"chunk": c.value, "diag_classes": d.value, "rows_per_thread": r.value,
+                "probes": p.value, "best_probe_ms": ms.value}
 
     def times(self):
         ms = np.zeros(self.nsys)

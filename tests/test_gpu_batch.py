@@ -329,11 +329,38 @@ def test_batch_fp32_256_systems(cases, gpu):
     print(f"worst true relres {worst:.3e}", flush=True)
 
 
+def test_batch_rows_per_thread_keeps_bits(cases, gpu, monkeypatch):
+    """Two rows per thread in the vector kernels (two row blocks per
+    workgroup, each with its own reduction slots) gives the bits of one row
+    per thread, with systems stopping at different iterations, in FP64 and
+    the FP32 form."""
+    systems = _systems(cases, [("hex", 12, 100.0), ("knn", 7, 0), ("hex", 20, 1000.0),
+                               ("hex", 5, 1.0), ("hex", 17, 10.0)])
+    val, b = _concat(systems, "val"), _concat(systems, "b")
+    monkeypatch.setenv("TFG_BATCH_TUNE", "0")
+    for precision in (gpu.FP64, gpu.FP32_IR):
+        out = []
+        for rows in ("1", "2"):
+            monkeypatch.setenv("TFG_BATCH_ROWS", rows)
+            bt = _batch(gpu, systems)
+            print(f"precision {precision} rows {rows}: {bt.info()}", flush=True)
+            opt = gpu.Options()
+            opt.precision = precision
+            opt.criterion = 1
+            x = np.zeros(int(bt.offsets[-1]))
+            niter, fin, _ = bt.solve("cg", "jacobi", val, x, b, 3000, 1e-9, options=opt)
+            out.append((niter, fin, x))
+            bt.close()
+        assert len(set(out[0][0].tolist())) > 1        # the systems stop at different iterations
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+        assert np.array_equal(out[0][2], out[1][2])
+
+
 def test_batch_autotuner_keeps_bits(cases, gpu, monkeypatch):
     """The autotuner's probes (graph length, diagonal or CSR products per
-    size class) leave the caller's x0 alone and every choice gives the same
-    bits: a tuned solve equals one with the defaults (TFG_BATCH_TUNE=0), in
-    FP64 and in the FP32 form, from a nonzero x0."""
+    size class, rows per thread) leave the caller's x0 alone and every
+    choice gives the same bits: a tuned solve equals one with the defaults
+    (TFG_BATCH_TUNE=0), in FP64 and in the FP32 form, from a nonzero x0."""
     systems = _systems(cases, [("hex", 12, 100.0), ("knn", 7, 0), ("hex", 20, 1000.0),
                                ("hex", 33, 10.0), ("hex", 9, 1.0)])
     val, b = _concat(systems, "val"), _concat(systems, "b")
@@ -353,10 +380,11 @@ def test_batch_autotuner_keeps_bits(cases, gpu, monkeypatch):
             print(f"precision {precision} TFG_BATCH_TUNE={tune}: {t}, max niter {niter.max()}",
                   flush=True)
             if tune == "1":
-                assert t["probes"] >= 2 and t["chunk"] in (8, 16, 32)
+                assert t["probes"] >= 3 and t["chunk"] in (8, 16, 32)
                 assert 0 <= t["diag_classes"] <= 7 and t["best_probe_ms"] > 0.0
+                assert t["rows_per_thread"] in (1, 2)
             else:
-                assert t["probes"] == 0 and t["diag_classes"] == 7
+                assert t["probes"] == 0 and t["diag_classes"] == 7 and t["rows_per_thread"] == 1
             out.append((niter, fin, x))
             bt.close()
         assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
