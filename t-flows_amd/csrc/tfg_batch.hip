@@ -1296,7 +1296,7 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
   // diagonals, the others), rebuilt from the polled state as systems stop.
   std::vector<int4v> act, act_c;
   std::vector<DiaEntry> act_d;
-  auto build_lists = [&](const State* ps) -> int {
+  auto build_lists = [&](const State* ps) {
     act.clear();
     act_d.clear();
     act_c.clear();
@@ -1328,7 +1328,6 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
         act_d.push_back(d);
       }
     }
-    return (int)act.size();
   };
   auto upload_lists = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(h->act, act.data(), sizeof(int4v) * act.size(), hipMemcpyHostToDevice, S));
@@ -1444,13 +1443,18 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
     if (prev >= 0) {
       HIP_TRY(hipEventSynchronize(h->ev_poll[prev]));
       const State* ps = h->hst + (size_t)prev * h->nsys;
-      int running = 0;
-      for (int s = 0; s < h->nsys; ++s) running += ps[s].done ? 0 : 1;
+      int running = 0, still = 0;
+      for (int s = 0; s < h->nsys; ++s) {
+        if (ps[s].done) continue;
+        ++running;
+        still += h->h_sblk[s + 1] - h->h_sblk[s];
+      }
       if (!running) break;
       // compaction once an eighth of the active blocks have stopped (the
-      // copies are stream-ordered after the chunk already queued)
-      const int still = build_lists(ps);
+      // lists are rebuilt only then; the copies are stream-ordered after
+      // the chunk already queued)
       if (still * 8 <= a.nact * 7) {
+        build_lists(ps);
         rc = upload_lists();
         if (rc != TFG_OK) return rc;
         if (use_graph) {
