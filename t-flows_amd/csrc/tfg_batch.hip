@@ -7,14 +7,17 @@
 // Layout: the systems' CSR arrays are concatenated into one block-diagonal
 // matrix (global 0-based rows and columns).  Row blocks (<= 256 rows,
 // <= kCap nonzeros) never straddle two systems.  Every block-level kernel
-// runs one workgroup per block of the ACTIVE list (blocks of systems not yet
-// stopped) and publishes one partial per block; a one-workgroup-per-system
-// kernel then sweeps that system's partials in a fixed order and writes its
-// scalars.  A system that stops is masked at once (its workgroups return)
-// and removed from the active list at the next poll, so late iterations
-// only launch the systems still running.  Per iteration:
-//   pupdate (blocks) | spmv + (p,q) (blocks) | alpha (systems)
-//   | update + sums (blocks) | check (systems: stop test, beta)
+// runs one workgroup per block (the vector kernels: one or two blocks) of
+// the ACTIVE list (blocks of systems not yet stopped) and publishes one
+// partial per block; a one-workgroup-per-system kernel then sweeps that
+// system's partials in a fixed order and writes its scalars.  A system that
+// stops is masked at once (its workgroups return) and removed from the
+// active list at the next compaction, so late iterations only launch the
+// systems still running.  The products run as two kernels over two lists:
+// the systems on diagonals (SysDiag) and the others (CSR rows).  Per
+// iteration:
+//   pupdate (blocks) | spmv + (p,q) (blocks: diagonals, CSR) | alpha
+//   (systems) | update + sums (blocks) | check (systems: stop test, beta)
 // The kernels read their iteration number from the system's state, so a
 // chunk of iterations is captured once as a hipGraph and replayed.
 #include <hip/hip_runtime.h>
