@@ -125,10 +125,8 @@ __device__ __forceinline__ double bmeasure(const State* st, double nrms, double 
   return sqrt(nrms) / range;
 }
 
-// Entry of this workgroup in the active list, dealt to the XCDs in
-// contiguous ranges (workgroup g runs on XCD g % 8): {block, first row, end
-// row, system} in one 16-byte load, so a workgroup's row loads and its
-// system's state follow from one load; false past the list.
+// An active-list entry: {block, first row, end row, system}, one 16-byte
+// load, so a workgroup's row loads and its system's state follow from it.
 struct ActEntry {
   int bl, r0, r1, s;
 };
@@ -143,6 +141,8 @@ __device__ __forceinline__ void hold(T... v) {
   (hold1(v), ...);
 }
 
+// Entry of this workgroup in a list of nact entries, dealt to the XCDs in
+// contiguous ranges (workgroup g runs on XCD g % 8); false past the list.
 __device__ __forceinline__ bool list_entry(const int4v* list, int nact, ActEntry& e) {
   const int g = blockIdx.x, x = g % kXcds, l = g / kXcds;
   const int first = (int)(((long long)nact * x) / kXcds);
