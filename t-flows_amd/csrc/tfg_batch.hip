@@ -73,13 +73,15 @@ struct SysDiag {
   int off[kBatchSym];       // ascending
 };
 
-// Entry of the active list of systems on diagonals: the block and its rows,
-// the system, its rows and its SysDiag, in one 64-byte load, so the row
-// loads of a workgroup follow from that one load.
+// Entry of the active list of systems on diagonals: one row block, or two
+// consecutive blocks of one system (bl1 >= 0: rows r0..rm-1 are block bl0,
+// rm..r1-1 block bl1; otherwise rm = r1), the system, its rows and its
+// SysDiag, in one 64-byte load, so the row loads of a workgroup follow from
+// that one load.
 struct alignas(16) DiaEntry {
-  int bl, r0, r1, s;
-  int sr0, sr1, k, pad;
-  int off[4];
+  int bl0, bl1, r0, rm;
+  int r1, s, sr0, sr1;
+  int k, off[kBatchSym];
   long long base, stride;
 };
 static_assert(sizeof(DiaEntry) == 64, "one 64-byte load");
@@ -388,40 +390,55 @@ __device__ __forceinline__ double dia_sum(const DiaRow<K, double>& w, double fn)
 // q = (A fn) p and the block's (p, q)                       Cg.f90:176-195
 // Blocks of systems on diagonals (act_dia): no LDS staging, so the kernel
 // runs at full occupancy.
-template <int K>
+template <int K, int NE>
 __device__ __forceinline__ void bt_spmv_dia_block(const BatchArgs& a, const DiaEntry& e,
                                                   double* s_tmp, double* s_out) {
-  // the row's loads and the state's are in flight together; rows past the
-  // block load the block's last row and are dropped
-  const int ti = e.r0 + (int)threadIdx.x;
-  const int i = min(ti, e.r1 - 1);
-  DiaRow<K, double> w;
-  dia_load<K, double>(a.sd + e.base, a.p, e, i, w);
+  // the rows' loads and the state's are in flight together; rows past a
+  // block load its last row and are dropped.  Thread t takes row t of the
+  // entry's first block and (NE = 2) row t of its second.
+  const int lo[2] = {e.r0, e.rm}, hi[2] = {e.rm, e.r1}, bl[2] = {e.bl0, e.bl1};
+  int ti[NE], i[NE];
+  DiaRow<K, double> w[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    ti[k] = lo[k] + (int)threadIdx.x;
+    i[k] = min(ti[k], max(hi[k], e.r0 + 1) - 1);
+    dia_load<K, double>(a.sd + e.base, a.p, e, i[k], w[k]);
+  }
   const State* st = &a.st[e.s];
   const int done = st->done;
   const double fn = st->fn;
-  dia_hold<K, double>(w);
+#pragma unroll
+  for (int k = 0; k < NE; ++k) dia_hold<K, double>(w[k]);
   hold(done, fn);
   if (done) return;
-  double v[1] = {0.0};
-  if (ti < e.r1) {
-    const double qi = dia_sum<K>(w, fn);
-    a.q[i] = qi;
-    v[0] = v[0] + w.x0 * qi;
+  double v[NE];
+  int ops[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    v[k] = 0.0;
+    ops[k] = OP_SUM;
+    if (ti[k] < hi[k]) {
+      const double qi = dia_sum<K>(w[k], fn);
+      a.q[i[k]] = qi;
+      v[k] = v[k] + w[k].x0 * qi;
+    }
   }
-  const int ops[1] = {OP_SUM};
-  block_publish<1>(a, BP_SPMV, e.bl, v, ops, s_tmp, s_out);
+  block_reduce<NE>(v, ops, s_tmp, s_out);
+  const int t = (int)threadIdx.x;
+  if (t < NE && bl[t] >= 0) a.part[(size_t)BP_SPMV * a.nblk + bl[t]] = s_out[t];
 }
+template <int NE>
 __global__ __launch_bounds__(kBlock) void k_bt_spmv_dia(BatchArgs a) {
-  __shared__ double s_tmp[4], s_out[1];
+  __shared__ double s_tmp[NE * 4], s_out[NE];
   const int gi = blockIdx.x, x = gi % kXcds, l = gi / kXcds;
   const int first = (int)(((long long)a.nact_dia * x) / kXcds);
   const int end = (int)(((long long)a.nact_dia * (x + 1)) / kXcds);
   if (first + l >= end) return;
   const DiaEntry e = a.act_dia[first + l];
-  if (e.k == 1) bt_spmv_dia_block<1>(a, e, s_tmp, s_out);
-  else if (e.k == 2) bt_spmv_dia_block<2>(a, e, s_tmp, s_out);
-  else bt_spmv_dia_block<3>(a, e, s_tmp, s_out);
+  if (e.k == 1) bt_spmv_dia_block<1, NE>(a, e, s_tmp, s_out);
+  else if (e.k == 2) bt_spmv_dia_block<2, NE>(a, e, s_tmp, s_out);
+  else bt_spmv_dia_block<3, NE>(a, e, s_tmp, s_out);
 }
 
 // The other blocks (act_csr): the CSR row engine E
@@ -902,36 +919,50 @@ __device__ __forceinline__ float dia_sum32(const DiaRow<K, float>& w) {
 }
 
 // q = A32 p and the block's (p, q): blocks of systems on diagonals
-template <int K>
+template <int K, int NE>
 __device__ __forceinline__ void bt32_spmv_dia_block(const BatchArgs& a, const DiaEntry& e,
                                                     double* s_tmp, double* s_out) {
-  const int ti = e.r0 + (int)threadIdx.x;
-  const int i = min(ti, e.r1 - 1);  // as in bt_spmv_dia_block
-  DiaRow<K, float> w;
-  dia_load<K, float>(a.f.sd32 + e.base, a.f.p32, e, i, w);
+  const int lo[2] = {e.r0, e.rm}, hi[2] = {e.rm, e.r1}, bl[2] = {e.bl0, e.bl1};
+  int ti[NE], i[NE];
+  DiaRow<K, float> w[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {  // as in bt_spmv_dia_block
+    ti[k] = lo[k] + (int)threadIdx.x;
+    i[k] = min(ti[k], max(hi[k], e.r0 + 1) - 1);
+    dia_load<K, float>(a.f.sd32 + e.base, a.f.p32, e, i[k], w[k]);
+  }
   const int done = a.st[e.s].done;
-  dia_hold<K, float>(w);
+#pragma unroll
+  for (int k = 0; k < NE; ++k) dia_hold<K, float>(w[k]);
   hold(done);
   if (done) return;
-  double v[1] = {0.0};
-  if (ti < e.r1) {
-    const float qi = dia_sum32<K>(w);
-    a.f.q32[i] = qi;
-    v[0] = (double)w.x0 * (double)qi;
+  double v[NE];
+  int ops[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    v[k] = 0.0;
+    ops[k] = OP_SUM;
+    if (ti[k] < hi[k]) {
+      const float qi = dia_sum32<K>(w[k]);
+      a.f.q32[i[k]] = qi;
+      v[k] = (double)w[k].x0 * (double)qi;
+    }
   }
-  const int ops[1] = {OP_SUM};
-  block_publish<1>(a, BP_SPMV, e.bl, v, ops, s_tmp, s_out);
+  block_reduce<NE>(v, ops, s_tmp, s_out);
+  const int t = (int)threadIdx.x;
+  if (t < NE && bl[t] >= 0) a.part[(size_t)BP_SPMV * a.nblk + bl[t]] = s_out[t];
 }
+template <int NE>
 __global__ __launch_bounds__(kBlock) void k_bt32_spmv_dia(BatchArgs a) {
-  __shared__ double s_tmp[4], s_out[1];
+  __shared__ double s_tmp[NE * 4], s_out[NE];
   const int gi = blockIdx.x, x = gi % kXcds, l = gi / kXcds;
   const int first = (int)(((long long)a.nact_dia * x) / kXcds);
   const int end = (int)(((long long)a.nact_dia * (x + 1)) / kXcds);
   if (first + l >= end) return;
   const DiaEntry e = a.act_dia[first + l];
-  if (e.k == 1) bt32_spmv_dia_block<1>(a, e, s_tmp, s_out);
-  else if (e.k == 2) bt32_spmv_dia_block<2>(a, e, s_tmp, s_out);
-  else bt32_spmv_dia_block<3>(a, e, s_tmp, s_out);
+  if (e.k == 1) bt32_spmv_dia_block<1, NE>(a, e, s_tmp, s_out);
+  else if (e.k == 2) bt32_spmv_dia_block<2, NE>(a, e, s_tmp, s_out);
+  else bt32_spmv_dia_block<3, NE>(a, e, s_tmp, s_out);
 }
 
 // the other blocks: the CSR row engine E on the FP32 row-excess values
@@ -1299,28 +1330,38 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
   // diagonals, the others), rebuilt from the polled state as systems stop.
   std::vector<int4v> act, act_c;
   std::vector<DiaEntry> act_d;
+  // Rows per thread of the block kernels (the autotuner's choice, or
+  // TFG_BATCH_ROWS): two takes two blocks per workgroup in the vector
+  // kernels and two consecutive blocks of a system per diagonal entry.
+  const char* force_rows = getenv("TFG_BATCH_ROWS");
+  const bool two = force_rows && (*force_rows == '1' || *force_rows == '2') ? *force_rows == '2'
+                                                                            : cfg.rows == 2;
   auto build_lists = [&](const State* ps) {
     act.clear();
     act_d.clear();
     act_c.clear();
     for (int s = 0; s < h->nsys; ++s) {
       if (ps && ps[s].done) continue;
-      for (int t = h->h_sblk[s]; t < h->h_sblk[s + 1]; ++t) {
+      const int b0 = h->h_sblk[s], b1 = h->h_sblk[s + 1];
+      for (int t = b0; t < b1; ++t) {
         int4v v;
         v.x = t;
         v.y = h->h_blk[t];
         v.z = h->h_blk[t + 1];
         v.w = s;
         act.push_back(v);
-        if (!dia[s]) {
-          act_c.push_back(v);
-          continue;
-        }
-        const SysDiag& g = h->h_sdg[s];
+        if (!dia[s]) act_c.push_back(v);
+      }
+      if (!dia[s]) continue;
+      const SysDiag& g = h->h_sdg[s];
+      for (int t = b0; t < b1; t += two ? 2 : 1) {
+        const bool pair = two && t + 1 < b1;
         DiaEntry d{};
-        d.bl = t;
-        d.r0 = v.y;
-        d.r1 = v.z;
+        d.bl0 = t;
+        d.bl1 = pair ? t + 1 : -1;
+        d.r0 = h->h_blk[t];
+        d.rm = h->h_blk[t + 1];
+        d.r1 = pair ? h->h_blk[t + 2] : d.rm;
         d.s = s;
         d.sr0 = h->h_srow[s];
         d.sr1 = h->h_srow[s + 1];
@@ -1377,9 +1418,6 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
     else if (pr == PREC_JACOBI) KERNEL<PREC_JACOBI, 1><<<Gv, B, 0, S>>>(a);            \
     else KERNEL<PREC_NONE, 1><<<Gv, B, 0, S>>>(a);                                     \
   } while (0)
-  const char* force_rows = getenv("TFG_BATCH_ROWS");
-  const bool two = force_rows && (*force_rows == '1' || *force_rows == '2') ? *force_rows == '2'
-                                                                            : cfg.rows == 2;
   auto enqueue_chunk = [&](int count) -> hipError_t {
     const int per = (a.nact + kXcds - 1) / kXcds;
     const int G = kXcds * per, Gv = kXcds * (two ? (per + 1) / 2 : per);
@@ -1388,7 +1426,8 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
     if (fp32) {
       for (int c = 0; c < count; ++c) {
         TFG_BT_VEC(k_bt32_pupdate);
-        if (Gd) k_bt32_spmv_dia<<<Gd, B, 0, S>>>(a);
+        if (Gd && two) k_bt32_spmv_dia<2><<<Gd, B, 0, S>>>(a);
+        else if (Gd) k_bt32_spmv_dia<1><<<Gd, B, 0, S>>>(a);
         if (Gc) TFG_BT32_ENGINE((k_bt32_spmv<E><<<Gc, B, 0, S>>>(a)));
         k_bt32_alpha<<<a.nsys, B, 0, S>>>(a);
         TFG_BT_VEC(k_bt32_update);
@@ -1400,7 +1439,8 @@ int batch_run(tfg_batch* h, int pr, bool fp32, const tfg_options& o, const doubl
     }
     for (int c = 0; c < count; ++c) {
       TFG_BT_VEC(k_bt_pupdate);
-      if (Gd) k_bt_spmv_dia<<<Gd, B, 0, S>>>(a);
+      if (Gd && two) k_bt_spmv_dia<2><<<Gd, B, 0, S>>>(a);
+      else if (Gd) k_bt_spmv_dia<1><<<Gd, B, 0, S>>>(a);
       if (Gc) TFG_BT_ENGINE((k_bt_spmv<E><<<Gc, B, 0, S>>>(a)));
       k_bt_alpha<<<a.nsys, B, 0, S>>>(a);
       TFG_BT_VEC(k_bt_update);
