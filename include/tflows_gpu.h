@@ -444,8 +444,9 @@ int tfg_batch_info(tfg_batch* h, int* nsys, int* n, int* nnz, int* blocks);
  * refinement cadence) and, per system-size class (bit 0: < 32768 rows,
  * bit 1: < 262144, bit 2: larger), whether systems on diagonals take their
  * products from the diagonals (bit set) or the CSR rows, and the rows each
- * thread of the vector kernels (p update, x and r update) takes: 1, or 2
- * (one from each of two row blocks per workgroup).  Every choice gives the
+ * thread of the block kernels (p update, x and r update, diagonal
+ * products) takes: 1, or 2 (one from each of two row blocks per
+ * workgroup).  Every choice gives the
  * same bits.  Reports the choice for precision (TFG_FP64 | TFG_FP32_IR):
  * chunk, diag_classes and rows_per_thread -1 before the first solve;
  * probes timed and the best probe's device time.  Any pointer may be
